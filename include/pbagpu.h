@@ -1,0 +1,202 @@
+/*
+ * pbagpu.h — C ABI of the MI355X-native per-blastwave radiation hot path.
+ *
+ * The path has three stages, each one entry point over a whole batch:
+ *
+ *   1. pbagpu_dyn_evolve   blastwave dynamics of every run (one ODE system per
+ *                          run, shared adaptive DOP853 step), plus the per-step
+ *                          derived quantities and injection-spectrum limits.
+ *      Replaces the step loop PyBlastAfterglow::run (src/main.cpp:157-195) ->
+ *      EvolveODEsystem::advanceTimeSubStep (src/model_evolve.h:439-566) with the
+ *      RHS callback RHS_comb (src/model_evolve.h:1587-1689, bound through
+ *      IntegratorBase::SetRHS, src/utilitites/ode_solvers.h:2061-2086), and the
+ *      per-step addOtherVars / computeInjectionElectronDistributionLimits{FS,RS}
+ *      (src/blastwave/blastwave.h:525-756, src/blastwave/blastwave_radiation.h:976-1107).
+ *
+ *   2. pbagpu_ele_evolve   electron kinetics (Chang-Cooper) and comoving
+ *                          synchrotron + SSC emissivity of every population
+ *                          (forward / reverse shock of every blastwave).
+ *      Replaces the per-step call EvolveODEsystem::computeMicrophysics
+ *      (src/model_evolve.h:934-953) -> evolveElectronDistirubitonComputeComoving
+ *      RadiationSpectra{FS,RS} (src/blastwave/blastwave_radiation.h:1109-1182) ->
+ *      ElectronAndRadiation::evaluateElectronDistributionNumericMixed and
+ *      computeRadiationNumericMixed (src/microphysics/microphysics.h:359-599).
+ *
+ *   3. pbagpu_eats_lightcurve  equal-time-arrival-surface quadrature of every
+ *                          layer at every (t_obs, nu_obs) pair.
+ *      Replaces EjectaBase::evalEjectaLightCurves (src/ejecta/ejecta_base.h:1164-1192)
+ *      -> EATS::evalLightCurve / evalFluxDensA (src/eats.h:808-856) with the
+ *      integrand callback fluxFuncA = fluxDensAdaptiveWithComov
+ *      (src/blastwave/blastwave_radiation.h:75-141, bound at :867-874).
+ *
+ * All compute entry points take DEVICE pointers and a HIP stream (NULL: the
+ * null stream).  Structures are plain data; the library never frees caller
+ * memory.  Every entry point returns 0 on success or a PBAGPU_E* code; the
+ * per-item status arrays carry a PBAGPU_S* bitmask so the host can reproduce
+ * the reference's log-and-exit behaviour.
+ */
+#ifndef PBAGPU_H
+#define PBAGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- codes */
+enum {
+    PBAGPU_OK = 0,
+    PBAGPU_E_ARG = -1,        /* bad argument / unsupported size */
+    PBAGPU_E_HIP = -2,        /* HIP runtime error */
+    PBAGPU_E_UNSUPPORTED = -3 /* option combination outside the implemented set */
+};
+/* per-item status bits (OR-ed) */
+enum {
+    PBAGPU_S_NONFINITE = 1,       /* NaN/inf met where the reference exits */
+    PBAGPU_S_TERMINATED = 2,      /* blastwave termination (allow_termination=no -> exit in the reference) */
+    PBAGPU_S_NMAX = 4,            /* ODE / quadrature budget exceeded */
+    PBAGPU_S_BAD_MPHYS = 8,       /* B<=0, gamma_min>=gamma_max, ... (microphysics.h:371-376) */
+    PBAGPU_S_NU_OUT_OF_GRID = 16, /* comoving frequency outside [freq1, freq2] (microphysics.h:1191-1202) */
+    PBAGPU_S_HEAP_FULL = 32       /* adaptive-quadrature interval store exhausted */
+};
+
+/* ------------------------------------------------------- configuration */
+/* Microphysics of one shock; parfile keys *_fs / *_rs
+ * (ElectronAndRaidaionBase::setBasePars, src/microphysics/microphysics_base.h:59-560;
+ *  ElectronAndRadiation::setPars / allocateForNumericSpectra, microphysics.h:72-345). */
+typedef struct {
+    double eps_e, eps_b, eps_t, p;
+    double gam2;          /* top of the Chang-Cooper grid (gam1 is not used by that grid) */
+    double freq1, freq2;  /* comoving frequency grid */
+    double beta_min;      /* no microphysics below this shock velocity */
+    double epsilon_e_rad; /* < 0: radiative losses computed inside the RHS */
+    int ngam, nfreq;
+    int max_substeps;
+    int ssc;              /* method_ssc = numeric */
+    int adi_loss_ele;     /* num_ele_use_adi_loss */
+    int pad_;
+} pba_shock_params;
+
+/* One afterglow model (all its blastwaves share these); parfile main + GRB sections. */
+typedef struct {
+    double n_ism, d_l, z, theta_obs;
+    double rtol;                  /* ODE rtol = atol */
+    double tb0;                   /* ODE start time as given in the parfile (the grid's first
+                                     point, 10^log10(tb0), can differ from it by an ulp) */
+    double tprompt, rs_shutoff_rho, min_gamma0_for_rs, rs_gamma0_frac_no_exceed;
+    double mom_when_start_spread, theta_max;
+    double rtol_theta, rtol_phi;  /* EATS tolerances */
+    int nmax_theta, nmax_phi;
+    int fsrs;                     /* bw_type = fsrs */
+    int do_rs, do_rs_radiation;
+    int adiab_loss, adiab_loss_rs;
+    int exponential_rho4, init_deltaR4;
+    int counter_jet;
+    int nmax;                     /* ODE step budget per output interval (0: unlimited) */
+    int pad_;
+    pba_shock_params fs, rs;
+} pba_run_params;
+
+/* Initial data of one blastwave (one layer of the adaptive discretisation;
+ * EjectaID2 m_data, src/ejecta/ejecta_id.h:13-341). */
+typedef struct {
+    double E0, M0, mom0;
+    double theta_c_l, theta_c_h, theta_c; /* layer angles */
+    double ctheta0;
+    double theta_wing, theta_core;
+    int ilayer, nlayers;
+} pba_layer_init;
+
+/* ------------------------------------------------ per-step table layout */
+/* The dynamics stage writes, for every blastwave, a [PBA_NV][nt] table of
+ * doubles (variable-major, so that one variable's time series is contiguous).
+ * Names follow BW::VARNAMES (src/blastwave/blastwave_pars.h:211-250). */
+enum pba_var {
+    PV_tburst = 0, PV_R, PV_Rsh, PV_tt, PV_tcomov, PV_Gamma, PV_beta, PV_mom, PV_theta, PV_ctheta,
+    PV_M2, PV_Eint2, PV_Erad2, PV_Esh2, PV_Ead2, PV_rho, PV_drhodr, PV_adi, PV_rho2, PV_GammaFsh,
+    PV_thickness, PV_U_p, PV_gm, PV_gc, PV_gM, PV_B, PV_Theta, PV_z_cool, PV_nprime, PV_accel_frac,
+    PV_Eint3, PV_Erad3, PV_Ead3, PV_Esh3, PV_M3, PV_deltaR4, PV_Gamma43, PV_adi3, PV_rho4, PV_rho3,
+    PV_GammaRsh, PV_Rrsh, PV_thickness_rs, PV_U_p3, PV_gm_rs, PV_gc_rs, PV_gM_rs, PV_B3, PV_Theta_rs,
+    PV_z_cool_rs, PV_nprime_rs, PV_accel_frac_rs, PV_GammaRel,
+    PV_mflags,  /* bit0: forward-shock electrons computed at this step; bit1: reverse shock
+                   (isBlastWaveValidForElectronCalc / considerReverseShock,
+                   blastwave_radiation.h:881-974) */
+    PBA_NV
+};
+
+/* Per-blastwave scalar results of the dynamics stage. */
+typedef struct {
+    int status;      /* PBAGPU_S_* */
+    int i_end_r;     /* last usable step + 1 for the observer integral */
+    int fsrs;        /* 1 if evolved with the reverse shock */
+    int n_steps;     /* accepted ODE steps (all intervals) */
+    int n_rhs;       /* RHS evaluations */
+    int pad_;
+} pba_bw_result;
+
+/* ----------------------------------------------------------- lifecycle */
+int pbagpu_device_count(int *count);
+int pbagpu_set_device(int device);
+int pbagpu_version(void);
+
+/* ------------------------------------------------------------- stage 1 */
+/* Evolve n_runs independent models of bw_per_run blastwaves each.
+ *   params   [n_runs]            host array
+ *   init     [n_runs*bw_per_run] host array
+ *   d_tgrid  [nt] device          burster-frame output grid (MakeLogspaceVec(tb0,tb1,ntb))
+ *   d_table  [n_runs*bw_per_run][PBA_NV][nt] device, written
+ *   res      [n_runs*bw_per_run] host array, written
+ */
+int pbagpu_dyn_evolve(const pba_run_params *params, int n_runs, int bw_per_run,
+                      const pba_layer_init *init, int nt, const double *d_tgrid,
+                      double *d_table, pba_bw_result *res, void *stream);
+
+/* ------------------------------------------------------------- stage 2 */
+/* One electron population: shock (0 forward, 1 reverse) of blastwave `bw`
+ * (index into the dynamics table), with the microphysics of run `run`. */
+typedef struct {
+    int bw;
+    int run;
+    int shock;
+    int pad_;
+} pba_population;
+
+/* Electron kinetics + emissivity of n_pop populations over all nt steps.
+ *   d_table  dynamics table from stage 1
+ *   d_jall   [n_pop][nt][nfreq] device, written: comoving emissivity per electron
+ *            (total_rad.j_all, base_state.h:257-310)
+ *   d_fall   [n_pop][nt][ngam]  device or NULL: electron spectrum per step (ele.f_all)
+ *   d_status [n_pop] device int, written
+ * The grid sizes ngam/nfreq come from the shock parameters and must be equal
+ * within one call.  precision: 0 = FP64 everywhere; 1 = SSC contraction in
+ * BF16 hi/lo split on MFMA with FP32 accumulation. */
+int pbagpu_ele_evolve(const pba_run_params *params, int n_runs,
+                      const pba_population *pops, int n_pop,
+                      const double *d_table, int bw_total, int nt,
+                      double *d_jall, double *d_fall, int *d_status,
+                      int precision, void *stream);
+
+/* ------------------------------------------------------------- stage 3 */
+/* Observer light curve of every layer at n_pairs (t_obs, nu_obs) pairs.
+ *   layers     [n_layers] host: blastwave index in the dynamics table and the
+ *              population indices of its forward / reverse shock emissivity (-1: none)
+ *   d_out      [n_layers][n_pairs] device, written, mJy
+ */
+typedef struct {
+    double theta_c_l, theta_c_h; /* angular extent of the layer (EjectaID2 adaptive grid) */
+    int bw;
+    int run;
+    int pop_fs;
+    int pop_rs;      /* -1 if the layer has no reverse-shock emission */
+    int i_end_r;     /* from pba_bw_result */
+} pba_eats_layer;
+
+int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs,
+                           const pba_eats_layer *layers, int n_layers,
+                           const double *d_table, int nt, const double *d_jall, int nfreq,
+                           int n_pairs, const double *d_tobs, const double *d_nuobs,
+                           double *d_out, int *d_status, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

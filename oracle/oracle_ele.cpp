@@ -1,0 +1,307 @@
+/*
+ * Oracle, stage 2: electron kinetics and comoving emissivity (TEST INFRASTRUCTURE ONLY).
+ *
+ * Per population (forward or reverse shock of one blastwave), over all steps:
+ *   - analytic injection spectrum   powerLawElectronDistributionAnalytic (microphysics.h:228-280)
+ *   - first step                    initializeElectronDistribution       (microphysics.h:347-357)
+ *   - later steps                   evaluateElectronDistributionNumericMixed (microphysics.h:359-489)
+ *       Chang-Cooper substeps       ChangCooper::setHeatCoolTerms / computeDeltaJ / setupVectors /
+ *                                   solve / setSourceFunction (numeric_model.h:449-804),
+ *                                   tridiagonal_solver (numeric_model.h:19-43)
+ *   - emissivity                    computeRadiationNumericMixed (microphysics.h:492-599):
+ *       synchrotron kernel          SynKernel::cycSynch (CSYN, kernels.h:414-437) -- Aharonian et al. (2010)
+ *                                   eq. D7 fit, restated from the published formula
+ *       synchrotron spectrum        computeSynSpectrum (numeric_model.h:148-184)
+ *       SSC kernel and spectrum     SSCKernel::sscNava / evalSSCkernel (kernels.h:33-117),
+ *                                   computeSSCSpectrum (numeric_model.h:247-285)
+ * Reference quirks kept: the last frequency bin of the synchrotron spectrum stays 0;
+ * the last heating bin stays 0; the number of substeps is floor(dt/dt_cool) so the
+ * substeps cover slightly less than dt; the photon density of a step uses the
+ * volume and comoving thickness of the previous step.  The SSC sum over seed
+ * frequencies in computeSSCSpectrum runs to the last bin and reads one element past
+ * the frequency grid there; the photon density in that bin is identically 0
+ * (the synchrotron spectrum leaves it 0, and the SSC spectrum cannot reach it), so the
+ * term is dropped here.
+ */
+#include "oracle_common.h"
+#include <cstring>
+#include <cstdio>
+
+namespace orc {
+
+// SynKernel::cycSynch (kernels.h:414-437)
+double cyc_synch(double B, double nu, double gam) {
+    double em;
+    if (gam > 2.) {
+        double nu_crit = (3. * qe * B * gam * gam) / (4. * pi * me * c);
+        double x = nu / nu_crit;
+        double t1n = 1.808 * std::pow(x, 1. / 3.);
+        double t1d = std::sqrt(1 + 3.4 * std::pow(x, 2. / 3.));
+        double t2n = 1. + 2.21 * std::pow(x, 2. / 3.) + 0.347 * std::pow(x, 4. / 3.);
+        double t2d = 1. + 1.353 * std::pow(x, 2. / 3.) + 0.217 * std::pow(x, 4. / 3.);
+        em = t1n / t1d * t2n / t2d * std::exp(-x);
+        em *= std::sqrt(3.) * std::pow(qe, 3.) * B / mec2;
+    } else {
+        double nu_larmor = (3. * qe * B) / (2. * pi * me * c);
+        double x = nu / nu_larmor;
+        double psq = gam * gam - 1.;
+        em = (2. * psq) / (1. + 3. * psq) * std::exp((2. * (1. - x)) / (1. + 3. * psq));
+        em *= std::sqrt(3.) * std::pow(qe, 3.) * B / mec2;
+    }
+    return em;
+}
+
+// SSCKernel::sscNava (kernels.h:95-117): Blumenthal & Gould / Nava et al. eq. 33
+double ssc_nava(double nu, double gam, double nu_tilde) {
+    double en = nu * h_by_mec2;
+    double en_tilde = nu_tilde * h_by_mec2;
+    double K = 0.;
+    if (en_tilde / (4. * gam * gam) < en && en <= en_tilde)
+        K = (en / en_tilde) - (1. / (4. * gam * gam));
+    else if (en_tilde < en && en < (4. * gam * gam * en_tilde / (1. + 4. * gam * en_tilde))) {
+        double q = en / (4. * gam * en_tilde * (gam - en));
+        K = 2. * q * std::log(q) + (1. + 2. * q) * (1. - q) +
+            .5 * (1. - q) * std::pow(4. * gam * en_tilde * q, 2.) / (1. + 4. * gam * en_tilde * q);
+    } else
+        K = 0;
+    return K;
+}
+
+struct SSCTab {
+    int nf = 0, ng = 0;
+    std::vector<double> K;   // [nf][ng][nf]
+    std::vector<double> KI;  // [ng][nf]
+    void build(const std::vector<double> &nu, const std::vector<double> &e) {
+        nf = (int)nu.size(); ng = (int)e.size();
+        K.assign((size_t)nf * ng * nf, 0.);
+        for (int i = 0; i < nf; i++)
+            for (int j = 0; j < ng; j++)
+                for (int k = 0; k < nf; k++) K[((size_t)i * ng + j) * nf + k] = ssc_nava(nu[i], e[j], nu[k]);
+        KI.assign((size_t)ng * nf, 0.);
+        for (int j = 0; j < ng; j++)
+            for (int k = 0; k < nf; k++) {
+                double res = 0;
+                for (int i = 0; i < nf - 1; i++) res += K[((size_t)i * ng + j) * nf + k] * nu[i + 1] * (nu[i + 1] - nu[i]);
+                KI[(size_t)j * nf + k] = res;
+            }
+    }
+};
+
+struct PopOut { double *jall; double *fall; int *status; };
+
+static void power_law(const EleGrid &g, double p, double gm, double gc, double gM, double n_inj, double *N) {
+    if (gm > gc) {
+        double K_f = n_inj / (std::pow(gm, 1. - p) * (1. / gc - 1. / gm) + 1. / p * (std::pow(gm, -p) - std::pow(gM, -p)));
+        for (int i = 0; i < g.n; i++) {
+            double gamma = g.e[i];
+            if ((gc <= gamma) && (gamma < gm)) N[i] = K_f * std::pow(gm, 1. - p) * std::pow(gamma, -2.);
+            else if ((gm < gamma) && (gamma <= gM)) N[i] = K_f * std::pow(gamma, -p - 1.);
+        }
+    } else {
+        double K_s = n_inj / ((1. / (1. - p)) * (std::pow(gc, 1. - p) - std::pow(gm, 1. - p)) -
+                              gc / p * (std::pow(gM, -p) - std::pow(gc, -p)));
+        for (int i = 0; i < g.n; i++) {
+            double gamma = g.e[i];
+            if ((gm <= gamma) && (gamma < gc)) N[i] = K_s * std::pow(gamma, -p);
+            else if ((gc <= gamma) && (gamma <= gM)) N[i] = K_s * gc * std::pow(gamma, -p - 1.);
+        }
+    }
+}
+
+static void evolve_population(const pba_shock_params &sp, const SSCTab *ssc, int shock, const double *T, int nt,
+                              double *jall, double *fall, int *status) {
+    const int ng = sp.ngam, nf = sp.nfreq;
+    EleGrid g;
+    g.build(sp.gam2, ng);
+    std::vector<double> nu = freq_grid(sp);
+    std::vector<double> f(ng, 0.), tot_f(nf, 0.), heat(ng, 0.), src(ng, 0.), bb(ng, 0.), cc(ng, 0.);
+    std::vector<double> synj(nf, 0.), sscj(nf, 0.), K((size_t)nf * ng, 0.), tmp(ng, 0.), scat(ng, 0.);
+    bool init = false;
+    double dr_comov = 0., vol = 0.;
+    auto V = [&](int v, int it) { return T[(size_t)v * nt + it]; };
+    const int bit = shock == 0 ? 1 : 2;
+    const int vB = shock ? PV_B3 : PV_B, vgm = shock ? PV_gm_rs : PV_gm, vgM = shock ? PV_gM_rs : PV_gM;
+    const int vgc = shock ? PV_gc_rs : PV_gc, vacc = shock ? PV_accel_frac_rs : PV_accel_frac;
+    const int vnp = shock ? PV_nprime_rs : PV_nprime, vM = shock ? PV_M3 : PV_M2, vrho = shock ? PV_rho3 : PV_rho2;
+    const int vth = shock ? PV_thickness_rs : PV_thickness, vGsh = shock ? PV_GammaRsh : PV_GammaFsh;
+    const int vUp = shock ? PV_U_p3 : PV_U_p;
+    const double p = sp.p;
+    for (int it = 0; it < nt; it++) {
+        int flags = (int)V(PV_mflags, it);
+        if (!(flags & bit)) continue;
+        double B = V(vB, it), gm = V(vgm, it), gM = V(vgM, it), gc = V(vgc, it), acc = V(vacc, it);
+        double n_prime = V(vnp, it);
+        if (!init) {
+            // initializeElectronDistribution(tcomov, m, rho, u_p, dr_comov)
+            dr_comov = V(vth, it);
+            double N_tot = V(vM, it) / mp;
+            vol = V(vM, it) / V(vrho, it);
+            power_law(g, p, gm, gc, gM, N_tot, f.data());
+            init = true;
+            for (int i = 0; i < ng - 1; i++) f[i] *= acc;
+        } else {
+            double tc = V(PV_tcomov, it - 1), tc1 = V(PV_tcomov, it);
+            double m = V(vM, it - 1), m1 = V(vM, it), rp = V(vrho, it - 1), rp1 = V(vrho, it);
+            // the reverse shock passes GammaRsh[it-1] for both thicknesses (blastwave_radiation.h:1135-1138)
+            double drc = V(vth, it - 1) * V(vGsh, it - 1);
+            (void)vUp;
+            if ((B <= 0) || (gm < 1.) || (gM <= 1.) || (gm >= gM) || (acc > 1.)) { *status |= PBAGPU_S_BAD_MPHYS; }
+            dr_comov = drc;
+            double N_tot = m1 / mp;
+            std::fill(tmp.begin(), tmp.end(), 0.);
+            power_law(g, p, gm, gc, gM, N_tot, tmp.data());
+            double dt = tc1 - tc;
+            double volume = m / rp;
+            double volume_p1 = m1 / rp1;
+            double dlnVdt = sp.adi_loss_ele ? 1. / dt * (1. - volume / volume_p1) : 0.;
+            if ((!std::isfinite(dlnVdt)) || (dlnVdt < 0)) { *status |= PBAGPU_S_NONFINITE; }
+            double dlog_gam = (g.e[1] - g.e[0]) / g.e[0];
+            double delta_t_syn = sigmaT * gM * B * B / (6. * pi * me * c);
+            double delta_t_adi = (gM * gM - 1.) / (3. * gM * gM) * dlnVdt;
+            double delta_t = dlog_gam / (delta_t_syn + delta_t_adi);
+            long n_sub;
+            if (delta_t >= dt / 1.) { delta_t = dt / 1.; n_sub = 1; }
+            else if (delta_t < dt / (double)sp.max_substeps) { delta_t = dt / (double)sp.max_substeps; n_sub = sp.max_substeps; }
+            else n_sub = (long)(size_t)(dt / delta_t);
+            vol = volume;
+            if (gm <= 2) {
+                for (int i = 0; i < ng; i++) f[i] = tmp[i];
+            } else {
+                double Nsrc = (m1 - m) / mp / dt;
+                double index = -p;
+                double norm = (1 + index) / (std::pow(gM, index + 1) - std::pow(gm, index + 1));
+                for (int i = 0; i < ng; i++)
+                    src[i] = ((g.e[i] > gm) && (g.e[i] <= gM)) ? Nsrc * norm * std::pow(g.e[i], index) : 0.;
+                // heating / cooling terms (setHeatCoolTerms); identical in every substep of the step
+                const double cf = sigmaT / (6. * pi * me * c);
+                const double csc = 3. / 4. * h * sigmaT / me / c;
+                std::fill(heat.begin(), heat.end(), 0.);
+                for (int i = 0; i < ng - 1; i++) {
+                    double ge = g.half_e[i];
+                    double gsyn = cf * B * B * ge * ge;
+                    double gadi = dlnVdt * (ge * ge - 1.) / (3. * ge);
+                    double gssc = 0.;
+                    if (ssc) {
+                        double res = 0;
+                        for (int k = 0; k < nf - 1; k++) res += tot_f[k] / nu[k] * (nu[k + 1] - nu[k]) * ssc->KI[(size_t)i * nf + k];
+                        gssc = csc * res / ge / ge;
+                    }
+                    heat[i] = gsyn + gadi + gssc;
+                    if (!std::isfinite(heat[i])) *status |= PBAGPU_S_NONFINITE;
+                }
+                // setupVectors (zero dispersion => delta_j = 0, a = 0)
+                for (int k = ng - 2; k > 0; k--) {
+                    double odgb = 1.0 / g.dgbar[k];
+                    double braw = -odgb * ((0. + 0.) + 1. * heat[k - 1] - 0. * heat[k]);
+                    double craw = odgb * (1. * heat[k] + (1.0 / g.dg[k + 1]) * 0.);
+                    bb[k] = (1. - braw * delta_t) + 0. * delta_t;
+                    cc[k] = craw * -delta_t;
+                }
+                {
+                    double odgb = 1.0 / g.dgbar[ng - 1];
+                    double braw = -odgb * ((0. * 0. + (1.0 / g.dg[ng]) * 0.) + 1. * heat[ng - 1] - 0. * 0.);
+                    bb[ng - 1] = (1. - braw * delta_t) + 0. * delta_t;
+                    cc[ng - 1] = 0. * -delta_t;
+                    double odgb0 = 1.0 / g.dgbar[0];
+                    double braw0 = -odgb0 * (((1.0 / g.dg[0]) * 0. + 0. * 0.) + 0. * 0. - 0. * heat[0]);
+                    double craw0 = odgb0 * (1. * heat[0] + (1.0 / g.dg[0]) * 0.);
+                    bb[0] = (1. - braw0 * delta_t) + 0. * delta_t;
+                    cc[0] = craw0 * -delta_t;
+                }
+                for (long s = 0; s < n_sub; s++) {
+                    // solve: d = f + S dt; back substitution (forward sweep skipped as sum(a) == 0)
+                    double fn = (f[ng - 1] + src[ng - 1] * delta_t) / bb[ng - 1];
+                    f[ng - 1] = fn;
+                    for (int j = ng - 2; j > -1; j--) {
+                        double d = f[j] + src[j] * delta_t;
+                        double cp = cc[j] / bb[j], dp = d / bb[j];
+                        f[j] = dp - cp * f[j + 1];
+                    }
+                }
+            }
+        }
+        // computeRadiationNumericMixed
+        for (int i = 0; i < nf; i++)
+            for (int j = 0; j < ng; j++) K[(size_t)i * ng + j] = cyc_synch(B, nu[i], g.e[j]);
+        std::fill(synj.begin(), synj.end(), 0.);
+        for (int i = 0; i < nf - 1; i++) {
+            double res = 0.;
+            for (int j = 0; j < ng - 1; j++) res += K[(size_t)i * ng + j] * f[j] * (g.e[j + 1] - g.e[j]);
+            synj[i] = res;
+            if (!std::isfinite(res) || res < 0.) *status |= PBAGPU_S_NONFINITE;
+        }
+        std::fill(sscj.begin(), sscj.end(), 0.);
+        if (ssc) {
+            for (int i = 0; i < nf; i++) {
+                for (int j = 0; j < ng - 1; j++) {
+                    double s = 0.;
+                    const double *Kij = &ssc->K[((size_t)i * ng + j) * nf];
+                    for (int k = 0; k < nf - 1; k++)
+                        s += 3. / 4. * sigmaT * c / (g.e[j] * g.e[j]) * tot_f[k] / nu[k] * (nu[k + 1] - nu[k]) * Kij[k];
+                    scat[j] = s;
+                }
+                double acc_i = 0.;
+                for (int j = 0; j < ng - 1; j++) acc_i += f[j] * h * nu[i] * scat[j] * (g.e[j + 1] - g.e[j]);
+                sscj[i] = acc_i;
+            }
+        }
+        double Tesc = dr_comov / c;
+        double fac = Tesc / vol;
+        for (int i = 0; i < nf; i++) {
+            tot_f[i] = synj[i] * fac / (h * nu[i]);
+            if (ssc) tot_f[i] += sscj[i] * fac / (h * nu[i]);
+        }
+        double n_ele = 0.;
+        for (int i = 0; i < ng - 1; i++) n_ele += f[i] * (g.e[i + 1] - g.e[i]);
+        if (n_ele <= 0) *status |= PBAGPU_S_NONFINITE;
+        for (int i = 0; i < nf; i++) {
+            double jt = synj[i];
+            if (ssc) jt += sscj[i];
+            jt = jt * acc / n_ele;
+            if (!std::isfinite(jt) || jt < 0.) *status |= PBAGPU_S_NONFINITE;
+            jall[(size_t)it * nf + i] = jt;
+        }
+        if (fall)
+            for (int i = 0; i < ng; i++) fall[(size_t)it * ng + i] = f[i];
+    }
+}
+
+} // namespace orc
+
+extern "C" int orc_ele_evolve(const pba_run_params *params, int n_runs, const pba_population *pops, int n_pop,
+                              const double *table, int bw_total, int nt, double *jall, double *fall, int *status) {
+    if (!params || !pops || !table || !jall || !status || n_pop < 1) return PBAGPU_E_ARG;
+    const pba_shock_params &s0 = pops[0].shock ? params[pops[0].run].rs : params[pops[0].run].fs;
+    const int ng = s0.ngam, nf = s0.nfreq;
+    // one SSC table per (run, shock) that needs it
+    std::vector<orc::SSCTab> tabs((size_t)n_runs * 2);
+    std::vector<char> built((size_t)n_runs * 2, 0);
+    for (int q = 0; q < n_pop; q++) {
+        const pba_population &P = pops[q];
+        if (P.run < 0 || P.run >= n_runs || P.bw < 0 || P.bw >= bw_total) return PBAGPU_E_ARG;
+        const pba_shock_params &sp = P.shock ? params[P.run].rs : params[P.run].fs;
+        if (sp.ngam != ng || sp.nfreq != nf) return PBAGPU_E_ARG;
+        size_t key = (size_t)P.run * 2 + (P.shock ? 1 : 0);
+        if (sp.ssc && !built[key]) {
+            orc::EleGrid g;
+            g.build(sp.gam2, sp.ngam);
+            tabs[key].build(orc::freq_grid(sp), g.e);
+            built[key] = 1;
+        }
+    }
+#pragma omp parallel for schedule(dynamic)
+    for (int q = 0; q < n_pop; q++) {
+        const pba_population &P = pops[q];
+        const pba_shock_params &sp = P.shock ? params[P.run].rs : params[P.run].fs;
+        size_t key = (size_t)P.run * 2 + (P.shock ? 1 : 0);
+        status[q] = 0;
+        std::fill(jall + (size_t)q * nt * nf, jall + (size_t)(q + 1) * nt * nf, 0.);
+        if (fall) std::fill(fall + (size_t)q * nt * ng, fall + (size_t)(q + 1) * nt * ng, 0.);
+        orc::evolve_population(sp, sp.ssc ? &tabs[key] : nullptr, P.shock, table + (size_t)P.bw * PBA_NV * nt, nt,
+                               jall + (size_t)q * nt * nf, fall ? fall + (size_t)q * nt * ng : nullptr, &status[q]);
+    }
+    return PBAGPU_OK;
+}
+
+extern "C" double orc_cyc_synch(double B, double nu, double gam) { return orc::cyc_synch(B, nu, gam); }
+extern "C" double orc_ssc_nava(double nu, double gam, double nu_tilde) { return orc::ssc_nava(nu, gam, nu_tilde); }

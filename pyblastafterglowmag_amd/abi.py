@@ -1,0 +1,187 @@
+"""ctypes mirror of include/pbagpu.h and helpers that turn parfile sections into its structures.
+
+The mapping from parfile keys to structure fields follows the readers of the
+reference: BlastWaveBase::setBaseParams / setParamsRS
+(src/blastwave/blastwave_base.h:95-456), ElectronAndRaidaionBase::setBasePars
+(src/microphysics/microphysics_base.h:59-560), ElectronAndRadiation::setPars
+(src/microphysics/microphysics.h:72-130), EATS::setEatsPars (src/eats.h:718-770)
+and PyBlastAfterglow::setModelPars (src/main.cpp:93-154).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import inputs
+
+PBA_NV_NAMES = [
+    "tburst", "R", "Rsh", "tt", "tcomov", "Gamma", "beta", "mom", "theta", "ctheta",
+    "M2", "Eint2", "Erad2", "Esh2", "Ead2", "rho", "drhodr", "adi", "rho2", "GammaFsh",
+    "thickness", "U_p", "gamma_min", "gamma_c", "gamma_max", "B", "ThetaTherm", "z_cool", "nprime", "accel_frac",
+    "Eint3", "Erad3", "iEad3", "Esh3", "M3", "deltaR4", "Gamma43", "adi3", "rho4", "rho3",
+    "GammaRsh", "Rsh_rs", "thichness_rs", "U_p3", "gamma_min_rs", "gamma_c_rs", "gamma_max_rs", "B_rs",
+    "ThetaTherm_rs", "z_cool_rs", "nprime_rs", "accel_frac_rs", "GammaRel", "mflags",
+]
+PBA_NV = len(PBA_NV_NAMES)
+PV = {n: i for i, n in enumerate(PBA_NV_NAMES)}
+
+
+class ShockParams(C.Structure):
+    _fields_ = [("eps_e", C.c_double), ("eps_b", C.c_double), ("eps_t", C.c_double), ("p", C.c_double),
+                ("gam2", C.c_double), ("freq1", C.c_double), ("freq2", C.c_double), ("beta_min", C.c_double),
+                ("epsilon_e_rad", C.c_double), ("ngam", C.c_int), ("nfreq", C.c_int),
+                ("max_substeps", C.c_int), ("ssc", C.c_int), ("adi_loss_ele", C.c_int), ("pad_", C.c_int)]
+
+
+class RunParams(C.Structure):
+    _fields_ = [("n_ism", C.c_double), ("d_l", C.c_double), ("z", C.c_double), ("theta_obs", C.c_double),
+                ("rtol", C.c_double), ("tb0", C.c_double),
+                ("tprompt", C.c_double), ("rs_shutoff_rho", C.c_double), ("min_gamma0_for_rs", C.c_double),
+                ("rs_gamma0_frac_no_exceed", C.c_double), ("mom_when_start_spread", C.c_double),
+                ("theta_max", C.c_double), ("rtol_theta", C.c_double), ("rtol_phi", C.c_double),
+                ("nmax_theta", C.c_int), ("nmax_phi", C.c_int), ("fsrs", C.c_int), ("do_rs", C.c_int),
+                ("do_rs_radiation", C.c_int), ("adiab_loss", C.c_int), ("adiab_loss_rs", C.c_int),
+                ("exponential_rho4", C.c_int), ("init_deltaR4", C.c_int), ("counter_jet", C.c_int),
+                ("nmax", C.c_int), ("pad_", C.c_int), ("fs", ShockParams), ("rs", ShockParams)]
+
+
+class LayerInit(C.Structure):
+    _fields_ = [("E0", C.c_double), ("M0", C.c_double), ("mom0", C.c_double), ("theta_c_l", C.c_double),
+                ("theta_c_h", C.c_double), ("theta_c", C.c_double), ("ctheta0", C.c_double),
+                ("theta_wing", C.c_double), ("theta_core", C.c_double), ("ilayer", C.c_int), ("nlayers", C.c_int)]
+
+
+class BwResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("i_end_r", C.c_int), ("fsrs", C.c_int), ("n_steps", C.c_int),
+                ("n_rhs", C.c_int), ("pad_", C.c_int)]
+
+
+class Population(C.Structure):
+    _fields_ = [("bw", C.c_int), ("run", C.c_int), ("shock", C.c_int), ("pad_", C.c_int)]
+
+
+class EatsLayer(C.Structure):
+    _fields_ = [("theta_c_l", C.c_double), ("theta_c_h", C.c_double), ("bw", C.c_int), ("run", C.c_int),
+                ("pop_fs", C.c_int), ("pop_rs", C.c_int), ("i_end_r", C.c_int)]
+
+
+SUPPORTED_OPTS = dict(
+    method_eats="adaptive", method_quad="CADRE", method_comp_mode="comovSpec", method_spread="our",
+    method_limit_spread="MomValAndFrac", method_dgdr="our", method_eos="Nava13", method_dmdr="usingdthdr",
+    integrator="DOP853E", do_average_solution="no", lc_use_freq_to_time="no", ebl_tbl_fpath="none",
+)
+
+
+def _yes(v):
+    if v not in ("yes", "no"):
+        raise ValueError(f"boolean option expected yes/no, got {v!r}")
+    return 1 if v == "yes" else 0
+
+
+def _shock(pars, opts, sh):
+    s = ShockParams()
+    s.eps_e = pars[f"eps_e_{sh}"]
+    s.eps_b = pars[f"eps_b_{sh}"]
+    s.eps_t = pars.get(f"eps_t_{sh}", 0.0)
+    s.p = pars[f"p_{sh}"]
+    s.gam2 = pars[f"gam2_{sh}"]
+    s.freq1 = pars[f"freq1_{sh}"]
+    s.freq2 = pars[f"freq2_{sh}"]
+    s.beta_min = pars.get(f"beta_min_{sh}", 1e-5)
+    s.epsilon_e_rad = pars.get(f"epsilon_e_rad_{sh}", 0.0)
+    s.ngam = int(pars[f"ngam_{sh}"])
+    s.nfreq = int(pars[f"nfreq_{sh}"])
+    s.max_substeps = int(pars.get(f"max_substeps_{sh}", 1000))
+    s.ssc = 1 if opts.get(f"method_ssc_{sh}", "none") == "numeric" else 0
+    s.adi_loss_ele = _yes(opts.get(f"num_ele_use_adi_loss_{sh}", "yes"))
+    for key, want in ((f"method_ele_{sh}", "numeric"), (f"method_synchrotron_{sh}", "CSYN"),
+                      (f"use_ssa_{sh}", "no"), (f"method_pp_{sh}", "none"), (f"method_ne_{sh}", "useNe"),
+                      (f"method_gamma_min_{sh}", "useNumeric"), (f"method_gamma_c_{sh}", "useTcomov"),
+                      (f"method_gamma_max_{sh}", "useB"), (f"method_B_{sh}", "useU_b"),
+                      (f"method_nonrel_dist_{sh}", "use_Sironi"), (f"method_tau_{sh}", "smooth"),
+                      (f"method_Up_{sh}", "useEint2"), (f"method_thickness_{sh}", "useJoh06"),
+                      (f"method_vel_{sh}", "sameAsBW")):
+        if key in opts and opts[key] != want:
+            raise NotImplementedError(f"{key}={opts[key]} (only {want} is implemented)")
+    return s
+
+
+def run_params_from_parfile(path):
+    """(RunParams, main_pars, main_opts, grb_pars, grb_opts) from a parfile with a GRB section."""
+    mp_, mo = inputs.read_parfile(path, inputs.SEC_MAIN)
+    gp, go = inputs.read_parfile(path, inputs.SEC_GRB)
+    return run_params(mp_, mo, gp, go), mp_, mo, gp, go
+
+
+def run_params(mp_, mo, gp, go):
+    for key, want in SUPPORTED_OPTS.items():
+        have = mo.get(key, go.get(key, want))
+        if have != want:
+            raise NotImplementedError(f"{key}={have} (only {want} is implemented)")
+    if mp_.get("n_ism", -1) <= 0:
+        raise NotImplementedError("only a uniform ISM (n_ism > 0) is implemented")
+    if int(mp_.get("iout", 1)) != 1:
+        raise NotImplementedError("iout != 1")
+    P = RunParams()
+    P.n_ism = mp_["n_ism"]
+    P.d_l = mp_["d_l"]
+    P.z = mp_["z"]
+    P.theta_obs = mp_["theta_obs"]
+    P.rtol = mp_["rtol"]
+    P.tb0 = mp_["tb0"]
+    P.nmax = int(mp_.get("nmax", 100000))
+    P.tprompt = gp.get("tprompt", 1000.)
+    P.rs_shutoff_rho = gp.get("rs_shutOff_criterion_rho", 1e-50)
+    P.min_gamma0_for_rs = gp.get("min_Gamma0_for_rs", 0.)
+    P.rs_gamma0_frac_no_exceed = gp.get("rs_Gamma0_frac_no_exceed", .98)
+    P.mom_when_start_spread = gp.get("mom_when_start_spread", 2.)
+    P.theta_max = gp.get("theta_max", np.pi / 2.)
+    P.rtol_theta = gp["rtol_theta"]
+    P.rtol_phi = gp["rtol_phi"]
+    P.nmax_theta = int(gp.get("nmax_theta", 1000))
+    P.nmax_phi = int(gp.get("nmax_phi", 1000))
+    bw_type = go.get("bw_type", "fs")
+    if bw_type not in ("fs", "fsrs"):
+        raise NotImplementedError(f"bw_type={bw_type}")
+    P.fsrs = 1 if bw_type == "fsrs" else 0
+    P.do_rs = _yes(go.get("do_rs", "no")) if P.fsrs else 0
+    P.do_rs_radiation = _yes(go.get("do_rs_radiation", "no")) if P.fsrs else 0
+    P.adiab_loss = _yes(go.get("use_adiabLoss", "yes"))
+    P.adiab_loss_rs = _yes(go.get("use_adiabLoss_rs", "yes"))
+    P.exponential_rho4 = _yes(go.get("exponential_rho4", "yes"))
+    P.init_deltaR4 = _yes(go.get("init_deltaR4", "no"))
+    P.counter_jet = _yes(go.get("counter_jet", "yes"))
+    P.fs = _shock(gp, go, "fs")
+    P.rs = _shock(gp, go, "rs") if P.fsrs else _shock(gp, go, "fs")
+    return P
+
+
+def layer_inits(idd):
+    """LayerInit array of the adaptive layers of a jet_structure() dict (shell 0)."""
+    n = len(idd["ek"])
+    arr = (LayerInit * n)()
+    dth = idd["theta_wing"] / n
+    for i in range(n):
+        L = arr[i]
+        L.E0 = float(idd["ek"][i])
+        L.M0 = float(idd["mass"][i])
+        L.mom0 = float(idd["mom"][i])
+        L.theta_c_l = i * dth
+        L.theta_c_h = (i + 1) * dth
+        L.theta_c = i * dth + dth / 2.
+        L.ctheta0 = float(idd["ctheta"][i])
+        L.theta_wing = float(idd["theta_wing"])
+        L.theta_core = float(idd["theta_core"])
+        L.ilayer = i
+        L.nlayers = n
+    return arr
+
+
+def tgrid(mp_):
+    return inputs.logspace_vec(np.log10(mp_["tb0"]), np.log10(mp_["tb1"]), int(mp_["ntb"]))
+
+
+def obs_pairs(mo):
+    """(t_obs, nu_obs) pairs, time-major (cast_times_freqs, src/utilitites/utils.h:1419-1448)."""
+    ts = inputs.vec_from_string(mo["lc_times"])
+    fs = inputs.vec_from_string(mo["lc_freqs"])
+    return np.repeat(ts, len(fs)), np.tile(fs, len(ts))

@@ -1,0 +1,65 @@
+"""Named afterglow workloads (parfile sections + jet structure) used by tests and the benchmark.
+
+* ``tophat_fs_syn``      : single top-hat blastwave, forward shock, synchrotron only,
+                           200 gamma bins, 96 comoving freqs, 1000 steps, 96 nu x 128 t_obs.
+* ``gauss21_fsrs_ssc``   : 21-layer Gaussian jet, forward + reverse shock, synchrotron + SSC,
+                           theta_obs = 0.363 rad, same grids.
+Small variants (fewer steps / bins / pairs) are derived with ``scaled``.
+"""
+import copy
+
+import numpy as np
+
+
+def tophat_fs_syn():
+    main = dict(tb0=3e2, tb1=1e14, ntb=1000, n_ism=1e-2, d_l=1.27e26, z=0.0099, theta_obs=0.0,
+                lc_freqs="array logspace 1e8 1e29 96", lc_times="array logspace 3e3 1e10 128")
+    grb = dict(eps_e_fs=0.1, eps_b_fs=1e-3, p_fs=2.2, ngam_fs=200, gam1_fs=1., gam2_fs=1e8,
+               freq1_fs=1e5, freq2_fs=1e32, nfreq_fs=96, method_ssc_fs="none", method_ele_fs="numeric",
+               bw_type="fs", do_rs="no", method_eats="adaptive")
+    struct = dict(struct="tophat", eiso_c=1e52, gamma0c=300., theta_c=0.1)
+    return main, grb, struct
+
+
+def gauss21_fsrs_ssc():
+    main = dict(tb0=3e2, tb1=1e14, ntb=1000, n_ism=10 ** (-1.6), d_l=1.27e26, z=0.0099, theta_obs=0.363,
+                lc_freqs="array logspace 1e8 1e29 96", lc_times="array logspace 3e3 1e10 128")
+    grb = dict(eps_e_fs=10 ** (-3.42), eps_b_fs=10 ** (-4.02), p_fs=2.1,
+               eps_e_rs=10 ** (-3.42), eps_b_rs=10 ** (-4.02), p_rs=2.1,
+               ngam_fs=200, gam1_fs=1., gam2_fs=1e8, freq1_fs=1e5, freq2_fs=1e32, nfreq_fs=96,
+               ngam_rs=200, gam1_rs=1., gam2_rs=1e8, freq1_rs=1e5, freq2_rs=1e32, nfreq_rs=96,
+               method_ssc_fs="numeric", method_ssc_rs="numeric", method_ele_fs="numeric", method_ele_rs="numeric",
+               bw_type="fsrs", do_rs="yes", do_rs_radiation="yes", method_eats="adaptive")
+    struct = dict(struct="gaussian", eiso_c=10 ** 54.1, gamma0c=300., theta_c=0.061, theta_w=0.436,
+                  n_layers_a=21)
+    return main, grb, struct
+
+
+WORKLOADS = {"tophat_fs_syn": tophat_fs_syn, "gauss21_fsrs_ssc": gauss21_fsrs_ssc}
+
+
+def scaled(name, ntb=None, ngam=None, nfreq=None, n_nu_obs=None, n_t_obs=None, n_layers=None, **over):
+    """A smaller variant of a named workload (grids shrunk, physics unchanged)."""
+    main, grb, struct = WORKLOADS[name]()
+    main, grb, struct = copy.deepcopy(main), copy.deepcopy(grb), copy.deepcopy(struct)
+    if ntb:
+        main["ntb"] = ntb
+    for sh in ("fs", "rs"):
+        if ngam and f"ngam_{sh}" in grb:
+            grb[f"ngam_{sh}"] = ngam
+        if nfreq and f"nfreq_{sh}" in grb:
+            grb[f"nfreq_{sh}"] = nfreq
+    if n_nu_obs:
+        main["lc_freqs"] = f"array logspace 1e8 1e29 {n_nu_obs}"
+    if n_t_obs:
+        main["lc_times"] = f"array logspace 3e3 1e10 {n_t_obs}"
+    if n_layers and struct["struct"] == "gaussian":
+        struct["n_layers_a"] = n_layers
+    for k, v in over.items():
+        if k in struct:
+            struct[k] = v
+        elif k in ("tb0", "tb1", "ntb", "n_ism", "d_l", "z", "theta_obs", "lc_freqs", "lc_times", "rtol"):
+            main[k] = v
+        else:
+            grb[k] = v
+    return main, grb, struct
