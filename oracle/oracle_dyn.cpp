@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <climits>
+#include <cstdlib>
 
 namespace orc {
 
@@ -172,6 +173,7 @@ static double radiation_loss(const Shock &sh, MState &st, double dmdR, double dE
         if (gc <= 1.) gc = 1.001;
         if (gc > gM) gc = gM;
         p_tot = wspn99_ptot(s.p, gm, gc, gM, B, dmdR / mp, 1e6);
+        if (getenv("ORC_RL_TRACE")) fprintf(stderr, "    RL gm=%.6g gc=%.6g gM=%.6g B=%.6g acc=%.6g tcom=%.6g ptot=%.6g betaSh=%.6g\n", gm, gc, gM, B, st.acc, st.tcomov, p_tot, betaSh);
     }
     p_tot *= st.acc;
     p_tot = std::max(p_tot, 0.);
@@ -351,8 +353,9 @@ static void rhs_fsrs(Run &run, BW &b, const double *Y, double *out) {
         double G0 = b.Gamma0;
         long double dg43 = G0 - (G0 * G0 * Gamma - Gamma) / std::sqrt(Gamma * Gamma * G0 * G0 - G0 * G0 - Gamma * Gamma + 1.0);
         long double GammaEff3 = (adi3 * Gamma * Gamma - adi3 + 1.0) / Gamma;
-        double dGammaEff3dGamma = (double)(adi3 * (1.0 + 1.0 / (Gamma * Gamma)) -
-                                           dg43 / 3.0 / Gamma43 * Gamma43 * (Gamma - 1.0 / Gamma) - 1.0 / (Gamma * Gamma));
+        // returned as double by get_dGammaEff3dGamma, then held in a long double (blastwave_components.h:416)
+        long double dGammaEff3dGamma = (double)(adi3 * (1.0 + 1.0 / (Gamma * Gamma)) -
+                                                dg43 / 3.0 / Gamma43 * Gamma43 * (Gamma - 1.0 / Gamma) - 1.0 / (Gamma * Gamma));
         double dGammaEffdGamma = (adi * Gamma * Gamma + adi - 1.0) / (Gamma * Gamma);
         double GammaEff = (adi * Gamma * Gamma - adi + 1.0) / Gamma;
         double f_2 = GammaEff * (adi - 1.0) * Eint2 / Gamma;
@@ -578,6 +581,7 @@ static int dop853(Run &run, Dop &d, double &time, double *value, double timeN, d
         if ((time + 1.01 * h - timeN) > 0.0) { h = timeN - time; last = true; }
         double err, stf;
         integrate8(run, d, time, value, h, atol, rtol, err, stf);
+        if (getenv("ORC_DYN_TRACE")) fprintf(stderr, "  t=%.17g h=%.17g err=%.17g\n", time, h, err);
         double fac11 = std::pow(err, expo1);
         double fac = fac11 / (1.0 * safe);
         fac = fac < facc2 ? facc2 : (fac > facc1 ? facc1 : fac);
@@ -764,6 +768,7 @@ static void evolve_run(const pba_run_params *P, const pba_layer_init *init, int 
     double x = P->tb0;
     for (int it = 1; it < nt; it++) {
         double dt = tgrid[it] - tgrid[it - 1];
+        if (getenv("ORC_DYN_TRACE")) fprintf(stderr, "interval %d\n", it);
         int rc = dop853(run, d, x, Y.data(), x + dt, P->rtol, P->rtol, nsteps);
         for (int i = 0; i < nbw; i++) {
             BW &b = run.bws[i];

@@ -173,6 +173,8 @@ template <class F> static int cadre_split(F &f, Interval9 *i0, Interval9 *i1, In
 }
 
 struct CadreStats { long evals = 0; size_t max_heap = 0; };
+static size_t g_max_heap_phi = 0, g_max_heap_theta = 0;
+static long g_evals = 0;
 
 // m9_adapt with the CADRE interval rules (quadratures.h:1969-2115)
 template <class F> static double cadre_adapt(F &f, double xa, double xb, int Nmax, double atol, double rtol,
@@ -354,11 +356,18 @@ static double integrate_theta_phi(LayerCtx &L, CadreStats *st) {
             L.nevals++;
             return integrand(L, 1 - aomct, a_phi);
         };
-        double result = cadre_adapt(ct, omc0, omc1, P.nmax_theta, atol / (2. * Fcoeff * M_PI), P.rtol_theta, st);
+        CadreStats sti;
+        double result = cadre_adapt(ct, omc0, omc1, P.nmax_theta, atol / (2. * Fcoeff * M_PI), P.rtol_theta, &sti);
+        if (st) { st->evals += sti.evals; }
+        if (sti.max_heap > g_max_heap_theta) g_max_heap_theta = sti.max_heap;
+        g_evals += sti.evals;
         if (result != result || result < 0.0) return 0.;
         return result;
     };
-    return 2 * Fcoeff * cadre_adapt(phi_integrand, 0.0, M_PI, P.nmax_phi, atol / (2. * Fcoeff * M_PI), P.rtol_phi, st);
+    CadreStats sto;
+    double r = 2 * Fcoeff * cadre_adapt(phi_integrand, 0.0, M_PI, P.nmax_phi, atol / (2. * Fcoeff * M_PI), P.rtol_phi, &sto);
+    if (sto.max_heap > g_max_heap_phi) g_max_heap_phi = sto.max_heap;
+    return r;
 }
 
 static void pars_mu(LayerCtx &L, double t_obs) {
@@ -410,6 +419,14 @@ extern "C" int orc_eats_lightcurve(const pba_run_params *params, int n_runs, con
         status[l] = L.status;
     }
     return PBAGPU_OK;
+}
+
+/* Largest interval stores and integrand count seen by the last calls (single-threaded use). */
+extern "C" void orc_eats_stats(long *max_heap_phi, long *max_heap_theta, long *evals, int reset) {
+    if (max_heap_phi) *max_heap_phi = (long)orc::g_max_heap_phi;
+    if (max_heap_theta) *max_heap_theta = (long)orc::g_max_heap_theta;
+    if (evals) *evals = orc::g_evals;
+    if (reset) { orc::g_max_heap_phi = 0; orc::g_max_heap_theta = 0; orc::g_evals = 0; }
 }
 
 /* Stand-alone CADRE on built-in test integrands (closed-form checks in tests). */

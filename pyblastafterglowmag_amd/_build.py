@@ -51,6 +51,22 @@ def build_h5io():
     return out
 
 
+GPU_SOURCES = ["capi.hip", "dyn.hip", "ele.hip", "eats.hip"]
+
+
+def build_gpu():
+    """libpbagpu.so: every HIP kernel of the hot path plus the C ABI, for gfx950."""
+    os.makedirs(LIB, exist_ok=True)
+    objs = []
+    for src in GPU_SOURCES:
+        obj = os.path.join(LIB, src.replace(".hip", ".o"))
+        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    out = os.path.join(LIB, "libpbagpu.so")
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", out])
+    return out
+
+
 ORACLE = os.path.join(ROOT, "oracle")
 
 
@@ -72,10 +88,13 @@ def build_reference():
         _run(["bash", script])
 
 
-def build_all():
+def build_all(reference=True):
     build_h5io()
+    build_gpu()
     build_oracle()
+    if reference:
+        build_reference()
 
 
 if __name__ == "__main__":
-    build_all()
+    build_all(reference="--no-reference" not in sys.argv)

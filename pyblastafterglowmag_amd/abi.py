@@ -8,6 +8,7 @@ reference: BlastWaveBase::setBaseParams / setParamsRS
 and PyBlastAfterglow::setModelPars (src/main.cpp:93-154).
 """
 import ctypes as C
+import math
 
 import numpy as np
 
@@ -156,20 +157,28 @@ def run_params(mp_, mo, gp, go):
 
 
 def layer_inits(idd):
-    """LayerInit array of the adaptive layers of a jet_structure() dict (shell 0)."""
+    """LayerInit array of the layers of a jet_structure() dict (shell 0).
+
+    The reference's loader builds the adaptive grid and then the piece-wise grid, and the
+    latter overwrites the layer edges and centres (EjectaID2::_init_a_grid / _init_pw_grid,
+    src/ejecta/ejecta_id.h:142-182, called in that order at :298-300): the edges used by the
+    dynamics and the observer integral are 2 asin(i/n sin(theta_w/2)).
+    """
     n = len(idd["ek"])
     arr = (LayerInit * n)()
-    dth = idd["theta_wing"] / n
+    tw = float(idd["theta_wing"])
+    dth = tw / n
+    th_pw = [2.0 * math.asin((i / n) * math.sin(tw / 2.0)) for i in range(n + 1)]  # libm, as the C++ loader
     for i in range(n):
         L = arr[i]
         L.E0 = float(idd["ek"][i])
         L.M0 = float(idd["mass"][i])
         L.mom0 = float(idd["mom"][i])
-        L.theta_c_l = i * dth
-        L.theta_c_h = (i + 1) * dth
+        L.theta_c_l = th_pw[i]
+        L.theta_c_h = th_pw[i + 1]
         L.theta_c = i * dth + dth / 2.
-        L.ctheta0 = float(idd["ctheta"][i])
-        L.theta_wing = float(idd["theta_wing"])
+        L.ctheta0 = 0.5 * (th_pw[i + 1] + th_pw[i])
+        L.theta_wing = tw
         L.theta_core = float(idd["theta_core"])
         L.ilayer = i
         L.nlayers = n

@@ -1,0 +1,699 @@
+// Stage 1 — blastwave dynamics on the GPU.
+//
+// One workgroup (one 64-lane wavefront) integrates one run: lane i owns
+// blastwave i of the run.  All blastwaves of a run form one ODE system with one
+// shared adaptive step, as in EvolveODEsystem (src/model_evolve.h:72, 294): the
+// DOP853 error norm is a sum over every component of every blastwave, so the
+// per-lane partial norms are combined in lane order through LDS and every lane
+// then takes the same step decision.  After every output interval each lane
+// writes its blastwave's derived quantities and injection-spectrum limits
+// (one column of the [PBA_NV][nt] table).
+//
+// Follows: BlastWave::setInitConditions / rhs_fs / rhs_fsrs / insertSolution /
+// applyUnits / addOtherVars (src/blastwave/blastwave.h:73-756, 1515-2116),
+// ElectronAndRadiation::computeRadiationLoss (src/microphysics/microphysics.h:749-795),
+// computeEleSpectrumLimits (src/microphysics/microphysics_base.h:575-966),
+// dop853_integrator::integrate (src/utilitites/ode_solvers.h:1289-1980),
+// EvolveODEsystem::advanceTimeSubStep (src/model_evolve.h:439-566).
+// The reference evaluates Gamma43 and dM3/dR in x87 long double; FP64 is used here.
+#include "pba_device.h"
+
+namespace pbagpu {
+namespace dyn {
+
+enum QS { iR, iRsh, itt, itcomov, iGamma, iEint2, iEint3, itheta, iRrsh, iErad2, iErad3, iEsh2, iEsh3,
+          iEad2, iEad3, iM2, iM3, ideltaR4, NA };
+constexpr int NEQ_REF = 23;  // components per blastwave in the reference state vector
+
+struct MState { double acc, tcomov; };
+
+struct Shock { double e0, etop; };
+
+struct BW {
+    double E0, M0, R0, Gamma0, beta0, mom0, theta_b0, theta_core, theta_max, rho_def;
+    double Rw0, Wtt0;  // passive components of the reference state vector (enter the step-size estimate)
+    int ilayer, nlayers;
+    int fsrs, do_rs, do_rs_rad, shutOff, end_evolution, end_spreading;
+    int prev_idx_x, i_end_r, status;
+    MState fs_st, rs_st;
+};
+
+__device__ double x86_int_abs(double v) {
+    // int abs(int) applied to a double on x86 (the initial-step estimator of the
+    // reference calls the int overload): out-of-range values give INT_MIN.
+    if (!(v > -2147483649.0 && v < 2147483648.0)) return -2147483648.0;
+    int iv = (int)v;
+    return (double)(iv < 0 ? -iv : iv);
+}
+
+__device__ double dthetadr_our(const BW &b, double Gamma, double Gsh, double R, double adi, double theta) {
+    double Q0 = 2.0, Q = sqrt(2.0) * 3.0;
+    double mom = mom_from_gamma(Gsh);
+    double vperp = sqrt(adi * (adi - 1.0) * (Gamma - 1.0) / (1.0 + adi * (Gamma - 1.0))) * kC;
+    if ((theta < 0.5 * kPi) && (Q0 * mom * b.theta_core < 1)) {
+        double fac = 1.;
+        if (mom * b.theta_core * Q >= 1.0) fac = Q * (1. - Q0 * mom * b.theta_core) / (Q - Q0);
+        if (b.theta_b0 < b.theta_core) fac *= tan(0.5 * b.theta_b0) / tan(0.5 * b.theta_core);
+        double beta = beta_from_gamma(Gsh);
+        return vperp / R / Gsh / beta / kC * fac;
+    }
+    return 0.;
+}
+
+__device__ double wspn99_ptot(double p, double gm, double gc, double gM, double B, double nprime, double nu0) {
+    double f = (3.0 / (4.0 * kPi)) * (kQe * B) / (kMe * kC);
+    double nu_m, nu_c, p_max;
+    if (gm < gc) {
+        double XpS = 0.06 + 0.28 * p;
+        nu_m = XpS * gm * gm * f;
+        nu_c = XpS * gc * gc * f;
+        p_max = 11.17 * (p - 1.0) / (3.0 * p - 1.0) * (0.54 + 0.08 * p) * kQe * kQe * kQe * nprime * B / (kMe * kC * kC);
+    } else {
+        double XpF = 0.455 + 0.08 * p;
+        double phipF = 1.89 - 0.935 * p + 0.17 * (p * p);
+        nu_m = XpF * gm * gm * f;
+        nu_c = XpF * gc * gc * f;
+        p_max = 2.234 * phipF * kQe * kQe * kQe * nprime * B / (kMe * kC * kC);
+    }
+    double nu_M = 3.0 / (4.0 * kPi) * gM * gM * kQe * B / (kMe * kC);
+    double t0, t1, t2;
+    if (nu_m <= nu_c) {
+        t0 = (3. / 4.) * pow(1. / nu_m, 1. / 3.) * (pow(nu_m, 4. / 3.) - pow(nu0, 4. / 3.));
+        t1 = -1. / (3. - p) * (2 * pow(1. / nu_m, 1. / 2. - p / 2.)) * (pow(nu_m, 3. / 2. - p / 2.) - pow(nu_c, 3. / 2. - p / 2.));
+        t2 = pow(nu_c / nu_m, -(p - 1.) / 2.) * 2. / (p - 2.) * (nu_c * pow(nu_M, p / 2.) - nu_M * pow(nu_c, p / 2.)) *
+             pow(nu_c * nu_M / nu_c, -p / 2.);
+    } else {
+        t0 = (3. / 4.) * pow(1. / nu_c, 1. / 3.) * (pow(nu_c, 4. / 3.) - pow(nu0, 4. / 3.));
+        t1 = -2. / sqrt(1. / nu_c) * (sqrt(nu_c) - sqrt(nu_m));
+        t2 = 1. / sqrt(nu_m / nu_c) * 2. / (p - 2.) * (nu_m * pow(nu_M, p / 2.) - nu_M * pow(nu_m, p / 2.)) *
+             pow(nu_m * nu_M / nu_m, -p / 2.);
+    }
+    return (t0 + t1 + t2) * p_max;
+}
+
+__device__ double radiation_loss(const pba_shock_params &s, const Shock &sh, MState &st, double dmdR, double dEshdR,
+                                 double R, double Gamma, double Gsh, double adi, double m, double Eint, double rho,
+                                 double theta, double ncells) {
+    if (dmdR == 0 || dEshdR == 0 || m == 0 || rho == 0) return 0.;
+    double rho2 = rho2t(Gamma, adi, rho);
+    double U_p = (Eint == 0 || m == 0) ? 0. : Eint / (m / rho2);
+    double betaSh = beta_from_gamma(Gsh), n_prime = rho2 / kMp;
+    double omc = (1.0 - cos(theta) / ncells);
+    double thickness = m / (2.0 * kPi * omc * rho2 * Gamma * R * R);
+    double p_tot = 0.;
+    if (!(betaSh < s.beta_min)) {
+        double B = sqrt(8. * kPi * (U_p * s.eps_b));
+        double gM = sqrt(6.0 * kPi * kQe / kSigmaT / B);
+        if (gM > 0.99 * sh.etop) gM = 0.99 * sh.etop;
+        double gm = (s.p - 2.) / (s.p - 1.) * (U_p * s.eps_e) / (n_prime * kMe * kC * kC);
+        if (gm < 2.) {
+            double sir = (s.p - 2.0) / (s.p - 1.0) * s.eps_e * U_p / (n_prime * kMe * kC * kC);
+            st.acc = sir > 1. ? 1. : sir;
+            gm = gm < 1. ? 1. : gm;
+        }
+        if (gm < sh.e0) gm = sh.e0;
+        double gc = (6. * kPi * kMe * kC) / (kSigmaT * B * B * st.tcomov);
+        if (gc <= 1.) gc = 1.001;
+        if (gc > gM) gc = gM;
+        p_tot = wspn99_ptot(s.p, gm, gc, gM, B, dmdR / kMp, 1e6);
+    }
+    p_tot *= st.acc;
+    p_tot = fmax(p_tot, 0.);
+    double dErad2dR = p_tot * (thickness * Gsh / kC);
+    double eps_rad = dErad2dR / (dEshdR * s.eps_e);
+    if (eps_rad < 0. || !isfinite(eps_rad)) eps_rad = 0.;
+    return eps_rad;
+}
+
+__device__ void rhs_fs(const pba_run_params &P, const Shock &shfs, BW &b, const double *Y, double *o) {
+    double R = Y[iR], Gamma = Y[iGamma], Eint2 = Y[iEint2], theta = Y[itheta], M2 = Y[iM2];
+    if (Gamma < 1 || M2 < 0 || theta < 0) b.status |= PBAGPU_S_NONFINITE;
+    double rho = b.rho_def / b.M0;
+    double drhodr = 0.0 / b.M0;
+    double beta = beta_from_gamma(Gamma);
+    double adi = adi_nava(Gamma);
+    double Gsh = gamma_shock(Gamma, adi);
+    double dRdt = beta * kC;
+    double dthetadr = 0.0;
+    if ((theta < b.theta_max) && !b.end_spreading && Gamma * beta < fmax(P.mom_when_start_spread, 0. * b.mom0))
+        dthetadr = dthetadr_our(b, Gamma, Gsh, R, adi, theta);
+    double dM2dR = 2. * kPi * rho * ((1. - cos(theta)) + (1. / 3.) * sin(theta) * R * dthetadr) * R * R;
+    double GE = (adi * Gamma * Gamma - adi + 1.0) / Gamma;
+    double dGE = (adi * Gamma * Gamma + adi - 1.0) / (Gamma * Gamma);
+    double num = (Gamma - 1.) * (GE + 1.) * dM2dR - GE * (adi - 1.) * Eint2 * (dM2dR / M2 - drhodr / rho);
+    double den = (1. + M2) + Eint2 * dGE + GE * (adi - 1.) * Eint2 / Gamma;
+    double dGdR = -num / den;
+    double dEsh2dR = (Gamma - 1.0) * dM2dR;
+    double dlnV2dR = dM2dR / M2 - drhodr / rho - dGdR / Gamma;
+    double dEad2dR = P.adiab_loss ? -(adi - 1.0) * Eint2 * dlnV2dR : 0.0;
+    double eps_rad = P.fs.epsilon_e_rad;
+    if (P.fs.epsilon_e_rad < 0)
+        eps_rad = radiation_loss(P.fs, shfs, b.fs_st, dM2dR * b.M0, dEsh2dR * b.M0 * kC * kC, R, Gamma, Gsh, adi,
+                                 M2 * b.M0, Eint2 * b.M0 * kC * kC, rho * b.M0, theta, 1.);
+    double dErad2dR = eps_rad * P.fs.eps_e * dEsh2dR;
+    double dEint2dR = dEsh2dR + dEad2dR - dErad2dR;
+    double mom = mom_from_gamma(Gamma);
+    double bm = beta_from_mom(mom);
+    double dttdr = 1. / (kC * bm) * sqrt(1. + R * R * dthetadr * dthetadr) - (1. / kC);
+    if (!isfinite(dGdR) || !isfinite(dlnV2dR) || !isfinite(dEint2dR) || fabs(dRdt * dGdR) > b.Gamma0)
+        b.status |= PBAGPU_S_NONFINITE;
+    o[iR] = dRdt;
+    o[iRsh] = beta_from_gamma(Gsh) * kC;
+    o[itt] = dRdt * dttdr;
+    o[itcomov] = dRdt * (1.0 / mom / kC);
+    o[iGamma] = dRdt * dGdR;
+    o[iEint2] = dRdt * dEint2dR;
+    o[itheta] = dRdt * dthetadr;
+    o[iErad2] = dRdt * dErad2dR;
+    o[iEsh2] = dRdt * dEsh2dR;
+    o[iEad2] = dRdt * dEad2dR;
+    o[iM2] = dRdt * dM2dR;
+    o[iEint3] = 0.; o[iRrsh] = 0.; o[iErad3] = 0.; o[iEsh3] = 0.; o[iEad3] = 0.; o[iM3] = 0.; o[ideltaR4] = 0.;
+}
+
+__device__ void rhs_fsrs(const pba_run_params &P, const Shock &shfs, BW &b, const double *Y, double *o) {
+    double R = Y[iR], Gamma = Y[iGamma], Eint2 = Y[iEint2], Eint3 = Y[iEint3], theta = Y[itheta];
+    double M2 = Y[iM2], M3 = Y[iM3], deltaR4 = Y[ideltaR4];
+    if (Gamma < 1.) Gamma = (b.prev_idx_x == 0) ? b.Gamma0 : 1. + 1.e-5;
+    if (Eint3 < 0) Eint3 = 0.0;
+    if (M3 < 0 && b.prev_idx_x == 0) { M3 = 0; Eint3 = 0; deltaR4 = 0; }
+    double beta = beta_from_gamma(Gamma);
+    double G0 = b.Gamma0;
+    double G43 = Gamma * G0 * (1.0 / (G0 * G0) + 1.0 / (Gamma * Gamma) - 1.0 / (Gamma * Gamma) / (G0 * G0)) /
+                 (1.0 + beta * b.beta0);
+    if (G43 < 1.) G43 = 1.;
+    double adi3 = adi_nava(G43);
+    double GRsh = gamma_shock(G43, adi3);
+    double rho = b.rho_def / b.M0;
+    double drhodr = 0.0 / b.M0;
+    double dRdt = beta * kC;
+    double adi = adi_nava(Gamma);
+    double Gsh = gamma_shock(Gamma, adi);
+    double dthetadr = 0.0;
+    if ((theta < b.theta_max) && !b.end_spreading && Gamma * beta < fmax(P.mom_when_start_spread, 0. * b.mom0))
+        dthetadr = dthetadr_our(b, Gamma, Gsh, R, adi, theta);
+    double dM2dR = 2. * kPi * rho * ((1. - cos(theta)) + (1. / 3.) * sin(theta) * R * dthetadr) * R * R;
+    double ddR4 = 0., dM3dR = 0., dlnrho4dR = 0.;
+    if (!b.shutOff && Gamma < 0.9999 * G0) {
+        double alpha_of = P.tprompt * b.beta0 * kC;
+        double b0 = b.beta0;
+        ddR4 = (1.0 / (beta * beta * beta * beta) - 1.0 / (b0 * b0 * b0 * b0)) / (1.0 / beta + 1.0 / b0) /
+               (1.0 / (beta * beta) + 1.0 / (b0 * b0)) * b0;
+        if (P.exponential_rho4) {
+            dM3dR = (1.0 / alpha_of) * ddR4 * exp(-deltaR4 / alpha_of);
+            dlnrho4dR = -2.0 / R - ddR4 / alpha_of;
+        } else {
+            dM3dR = (1.0 / alpha_of) * ddR4;
+            dlnrho4dR = -2.0 / R;
+        }
+    }
+    double dlnrhodr = drhodr / rho;
+    double dGdR;
+    {
+        double dg43 = G0 - (G0 * G0 * Gamma - Gamma) / sqrt(Gamma * Gamma * G0 * G0 - G0 * G0 - Gamma * Gamma + 1.0);
+        double GE3 = (adi3 * Gamma * Gamma - adi3 + 1.0) / Gamma;
+        double dGE3 = adi3 * (1.0 + 1.0 / (Gamma * Gamma)) - dg43 / 3.0 / G43 * G43 * (Gamma - 1.0 / Gamma) -
+                      1.0 / (Gamma * Gamma);
+        double dGE = (adi * Gamma * Gamma + adi - 1.0) / (Gamma * Gamma);
+        double GE = (adi * Gamma * Gamma - adi + 1.0) / Gamma;
+        double f_2 = GE * (adi - 1.0) * Eint2 / Gamma;
+        double h_2 = GE * (adi - 1.0) * Eint2 * (dM2dR / M2 - dlnrhodr);
+        double fh3 = GE3 * (adi3 - 1.0) * Eint3;
+        double f_3 = fh3 / G43 * dg43;
+        double h_3 = (Eint3 != 0) ? fh3 * (dM3dR / M3 - dlnrho4dR) : 0.0;
+        dGdR = -((Gamma - 1.0) * (GE + 1.0) * dM2dR + (Gamma - G0 + GE3 * (G43 - 1.0)) * dM3dR - h_2 - h_3) /
+               ((M2 + M3) + Eint2 * dGE + Eint3 * dGE3 + f_2 + f_3);
+    }
+    if (dGdR > 0 && Gamma > P.rs_gamma0_frac_no_exceed * G0) dGdR = 0.0;
+    double dEsh2dR = (Gamma - 1.0) * dM2dR;
+    double dEsh3dR = (G43 - 1.0) * dM3dR;
+    double dlnV2dR = dM2dR / M2 - drhodr / rho - dGdR / Gamma;
+    double dEad2dR = P.adiab_loss ? -(adi - 1.0) * Eint2 * dlnV2dR : 0.0;
+    double dEad3dR = 0.;
+    if (!b.shutOff && M3 > 0.0) {
+        double dlnV3dR = dM3dR / M3 - dlnrho4dR - dGdR / Gamma;
+        if (P.adiab_loss_rs) dEad3dR = -(adi3 - 1) * Eint3 * dlnV3dR;
+    }
+    double eps_rad = P.fs.epsilon_e_rad;
+    if (P.fs.epsilon_e_rad < 0)
+        eps_rad = radiation_loss(P.fs, shfs, b.fs_st, dM2dR * b.M0, dEsh2dR * b.M0 * kC * kC, R, Gamma, Gsh, adi,
+                                 M2 * b.M0, Eint2 * b.M0 * kC * kC, rho * b.M0, theta, 1.);
+    double dErad2dR = eps_rad * P.fs.eps_e * dEsh2dR;
+    // the reverse-shock loss is evaluated with rho4 = 0 and returns 0 (blastwave.h:1991-1997)
+    double eps_rad_rs = P.rs.epsilon_e_rad < 0 ? 0. : P.rs.epsilon_e_rad;
+    double dErad3dR = eps_rad_rs * P.rs.eps_e * dEsh3dR;
+    double dEint2dR = dEsh2dR + dEad2dR - dErad2dR;
+    double dEint3dR = dEsh3dR + dEad3dR - dErad3dR;
+    double mom = Gamma * beta;
+    double bm = beta_from_mom(mom);
+    double dttdr = 1. / (kC * bm) * sqrt(1. + R * R * dthetadr * dthetadr) - (1. / kC);
+    if (!isfinite(dGdR) || dM2dR < 0. || !isfinite(dlnV2dR)) b.status |= PBAGPU_S_NONFINITE;
+    o[iR] = dRdt;
+    o[iRsh] = beta_from_gamma(Gsh) * kC;
+    o[itt] = dRdt * dttdr;
+    o[itcomov] = dRdt * (1.0 / beta / Gamma / kC);
+    o[iGamma] = dRdt * dGdR;
+    o[iRrsh] = beta_from_gamma(GRsh) * kC;
+    o[iEint2] = dRdt * dEint2dR;
+    o[iEint3] = dRdt * dEint3dR;
+    o[itheta] = dRdt * dthetadr;
+    o[iErad2] = dRdt * dErad2dR;
+    o[iErad3] = dRdt * dErad3dR;
+    o[iEsh2] = dRdt * dEsh2dR;
+    o[iEsh3] = dRdt * dEsh3dR;
+    o[iEad2] = dRdt * dEad2dR;
+    o[iEad3] = dRdt * dEad3dR;
+    o[iM2] = dRdt * dM2dR;
+    o[iM3] = dRdt * dM3dR;
+    o[ideltaR4] = dRdt * ddR4;
+}
+
+__device__ __forceinline__ void rhs(const pba_run_params &P, const Shock &shfs, BW &b, bool active, const double *Y,
+                                    double *o) {
+    if (!active || b.end_evolution) return;  // derivative entries of inactive blastwaves keep their (zero) values
+    if (b.fsrs) rhs_fsrs(P, shfs, b, Y, o);
+    else rhs_fs(P, shfs, b, Y, o);
+}
+
+struct Lim { double B, gm, gM, gc, Theta, z_cool, nprime, acc; bool ok; };
+
+__device__ Lim ele_limits(const pba_shock_params &s, const Shock &sh, MState &st, double U_p, double Gsh, double t_e,
+                          double tcomov, double n_prime) {
+    Lim L;
+    st.tcomov = tcomov;
+    double betaSh = beta_from_gamma(Gsh);
+    L.ok = !(betaSh < s.beta_min);
+    if (!L.ok) return L;
+    double B = sqrt(8. * kPi * (U_p * s.eps_b));
+    double gM = sqrt(6.0 * kPi * kQe / kSigmaT / B);
+    if (gM > 0.99 * sh.etop) gM = 0.99 * sh.etop;
+    double gm1 = (s.p - 2.) / (s.p - 1.) * (U_p * s.eps_e) / (n_prime * kMe * kC * kC);
+    double Theta0 = s.eps_t * (9.0 * 0.62 * kMp / (32.0 * 1.18 * kMe)) * betaSh * betaSh;
+    double Theta = (5.0 * Theta0 - 6.0 + sqrt(25.0 * Theta0 * Theta0 + 180.0 * Theta0 + 36.0)) / 30.0;
+    double z_cool = (6.0 * kPi * kMe * kC / (kSigmaT * B * B * t_e)) / Theta;
+    // bisection for gamma_min (rootfinders.h:17-58 with microphysics_base.h:792-806)
+    double gamma_ave = s.eps_e * U_p / (n_prime * kMe * kC * kC);
+    double p = s.p;
+    double gMa = pow(gM, -p + 2), gMb = pow(gM, -p + 1);
+    auto f = [&](double x) { return (p - 1) / (p - 2) * (gMa - pow(x, -p + 2)) / (gMb - pow(x, -p + 1)) - gamma_ave; };
+    int status = 0;
+    double gm;
+    {
+        double xa = 1., xb = 1.e8, fa = f(xa), fb = f(xb);
+        if (fa * fb > 0) { status = -1; gm = 0.; }
+        else if (fa == 0) { status = -1; gm = xa; }
+        else if (fb == 0) { status = -1; gm = xb; }
+        else {
+            double dm = xb - xa;
+            int i;
+            gm = 0.;
+            for (i = 0; i < 100; i++) {
+                dm *= 0.5;
+                double xm = xa + dm;
+                double fm = f(xm);
+                if (fm * fa >= 0) xa = xm;
+                if (fm == 0 || fabs(dm) < .001 * fabs(xm)) { gm = xm; break; }
+            }
+            if (i == 100) { status = -1; gm = xa; }
+        }
+    }
+    if (status < 0 && gm1 > 1.5) gm = gm1;
+    if (gm < 2. || status < 0) {
+        double sir = (p - 2.0) / (p - 1.0) * s.eps_e * U_p / (n_prime * kMe * kC * kC);
+        st.acc = sir > 1. ? 1. : sir;
+        gm = gm < 1. ? 1. : gm;
+    }
+    if (gm < sh.e0) gm = sh.e0;
+    double gc = (6. * kPi * kMe * kC) / (kSigmaT * B * B * tcomov);
+    if (gc <= 1.) gc = 1.001;
+    if (gc > gM) gc = gM;
+    L.B = B; L.gm = gm; L.gM = gM; L.gc = gc; L.Theta = Theta; L.z_cool = z_cool; L.nprime = n_prime; L.acc = st.acc;
+    return L;
+}
+
+__device__ double ctheta_of(double theta, int ilayer, int nlayers) {
+    if (ilayer <= 0) return 0.;
+    double fac0 = (double)ilayer / (double)nlayers, fac1 = (double)(ilayer + 1) / (double)nlayers;
+    double xx2 = 2. * asin(fac1 * sin(theta / 2.)), xx1 = 2. * asin(fac0 * sin(theta / 2.));
+    return 0.5 * (xx1 + xx2);
+}
+
+__device__ void post_step(const pba_run_params &P, const Shock &shfs, const Shock &shrs, BW &b, const double *Yc,
+                          int it, double tb, double *T, int nt) {
+    if (b.end_evolution) return;
+    auto W = [&](int v, double x) { T[(size_t)v * nt + it] = x; };
+    auto Rd = [&](int v, int t) { return T[(size_t)v * nt + t]; };
+    const double u = b.M0 * kC * kC;
+    double M2 = Yc[iM2] * b.M0, Eint2 = Yc[iEint2] * u;
+    double G = Yc[iGamma];
+    W(PV_tburst, tb); W(PV_R, Yc[iR]); W(PV_Rsh, Yc[iRsh]); W(PV_tt, Yc[itt]);
+    W(PV_mom, mom_from_gamma(G)); W(PV_Gamma, G); double beta = beta_from_gamma(G); W(PV_beta, beta);
+    W(PV_theta, Yc[itheta]); W(PV_M2, M2); W(PV_tcomov, Yc[itcomov]); W(PV_Ead2, Yc[iEad2] * u);
+    W(PV_Eint2, Eint2); W(PV_Esh2, Yc[iEsh2] * u); W(PV_Erad2, Yc[iErad2] * u);
+    double M3 = Yc[iM3] * b.M0, Eint3 = Yc[iEint3] * u;
+    if (b.fsrs) {
+        W(PV_Rrsh, Yc[iRrsh]); W(PV_Eint3, Eint3); W(PV_Ead3, Yc[iEad3] * u); W(PV_Erad3, Yc[iErad3] * u);
+        W(PV_Esh3, Yc[iEsh3] * u); W(PV_M3, M3); W(PV_deltaR4, Yc[ideltaR4]);
+    }
+    double R = Yc[iR], theta = Yc[itheta];
+    if (R < 1. || G < 1.) b.status |= PBAGPU_S_NONFINITE;
+    W(PV_ctheta, ctheta_of(theta, b.ilayer, b.nlayers));
+    double rho = b.rho_def;
+    W(PV_rho, rho); W(PV_drhodr, 0.);
+    double adi = adi_nava(G);
+    double rho2 = rho2t(G, adi, rho);
+    double Gsh = gamma_shock(G, adi);
+    W(PV_adi, adi); W(PV_rho2, rho2); W(PV_GammaFsh, Gsh);
+    double omc = (1. - cos(theta) / 1.);
+    W(PV_thickness, M2 / (2. * kPi * kMp * R * R * omc * G * (rho2 / kMp)));
+    double U_p = (Eint2 == 0 || M2 == 0) ? 0. : Eint2 / (M2 / rho2);
+    W(PV_U_p, U_p);
+    double GRsh = 0., U_p3 = 0., rho3 = 0.;
+    if (it > 0 && b.do_rs && !b.shutOff && b.fsrs) {
+        double lbeta = beta_from_mom(mom_from_gamma(G)), G0 = b.Gamma0;
+        double G43 = G * G0 * (1.0 / (G0 * G0) + 1.0 / (G * G) - 1.0 / (G * G) / (G0 * G0)) / (1.0 + lbeta * b.beta0);
+        double alpha_of = P.tprompt * b.beta0 * kC;
+        double rho4f = b.M0 / (2.0 * alpha_of * kPi * (1.0 - cos(b.theta_b0))) / (R * R);
+        double rho4 = P.exponential_rho4 ? rho4f * exp(-Yc[ideltaR4] / alpha_of) : rho4f;
+        double adi3 = adi_nava(G43);
+        rho3 = rho2t(G, adi3, rho4);
+        GRsh = gamma_shock(G43, adi3);
+        W(PV_Gamma43, G43); W(PV_rho4, rho4); W(PV_adi3, adi3); W(PV_rho3, rho3); W(PV_GammaRsh, GRsh);
+        W(PV_thickness_rs, M3 / (2. * kPi * kMp * R * R * omc * G43 * (rho3 / kMp)));
+        U_p3 = (Eint3 == 0 || M3 == 0) ? 0. : Eint3 / (M3 / rho3);
+        W(PV_U_p3, U_p3);
+    }
+    if (adi < 1. || rho2 < 0. || U_p < 0 || Gsh < 1.) b.status |= PBAGPU_S_NONFINITE;
+    int flags = 0;
+    bool valid = b.R0 >= 0;
+    if (valid && beta < P.fs.beta_min) {
+        if (b.i_end_r > it - 1) b.i_end_r = it - 1;
+        valid = false;
+    }
+    if (valid && (U_p <= 0 || rho2 <= 0)) valid = false;
+    if (valid && R < 1) valid = false;
+    if (valid) {
+        bool rs = b.fsrs && b.do_rs_rad && it > 0 && Rd(PV_thickness_rs, it - 1) > 0 && GRsh > 0 && U_p3 > 0;
+        if (rs) {
+            Lim L = ele_limits(P.rs, shrs, b.rs_st, U_p3, GRsh, Yc[itt], Yc[itcomov], rho3 / kMp);
+            if (L.ok) {
+                W(PV_B3, L.B); W(PV_gm_rs, L.gm); W(PV_gM_rs, L.gM); W(PV_gc_rs, L.gc); W(PV_Theta_rs, L.Theta);
+                W(PV_z_cool_rs, L.z_cool); W(PV_nprime_rs, L.nprime); W(PV_accel_frac_rs, L.acc);
+            }
+            flags |= 2;
+        }
+        Lim L = ele_limits(P.fs, shfs, b.fs_st, U_p, Gsh, Yc[itt], Yc[itcomov], rho2 / kMp);
+        if (L.ok) {
+            W(PV_B, L.B); W(PV_gm, L.gm); W(PV_gM, L.gM); W(PV_gc, L.gc); W(PV_Theta, L.Theta);
+            W(PV_z_cool, L.z_cool); W(PV_nprime, L.nprime); W(PV_accel_frac, L.acc);
+        }
+        flags |= 1;
+    }
+    W(PV_mflags, (double)flags);
+}
+
+// DOP853 tableau (Hairer & Wanner's published dop853 coefficients)
+namespace tab {
+__constant__ double c_[12] = {0., 0.526001519587677318785587544488E-01, 0.789002279381515978178381316732E-01,
+                              0.118350341907227396726757197510E+00, 0.281649658092772603273242802490E+00,
+                              0.333333333333333333333333333333E+00, 0.25E+00, 0.307692307692307692307692307692E+00,
+                              0.651282051282051282051282051282E+00, 0.6E+00, 0.857142857142857142857142857142E+00, 1.0};
+}
+constexpr double a21 = 5.26001519587677318785587544488E-2, a31 = 1.97250569845378994544595329183E-2,
+                 a32 = 5.91751709536136983633785987549E-2, a41 = 2.95875854768068491816892993775E-2,
+                 a43 = 8.87627564304205475450678981324E-2, a51 = 2.41365134159266685502369798665E-1,
+                 a53 = -8.84549479328286085344864962717E-1, a54 = 9.24834003261792003115737966543E-1,
+                 a61 = 3.7037037037037037037037037037E-2, a64 = 1.70828608729473871279604482173E-1,
+                 a65 = 1.25467687566822425016691814123E-1, a71 = 3.7109375E-2,
+                 a74 = 1.70252211019544039314978060272E-1, a75 = 6.02165389804559606850219397283E-2,
+                 a76 = -1.7578125E-2, a81 = 3.70920001185047927108779319836E-2,
+                 a84 = 1.70383925712239993810214054705E-1, a85 = 1.07262030446373284651809199168E-1,
+                 a86 = -1.53194377486244017527936158236E-2, a87 = 8.27378916381402288758473766002E-3,
+                 a91 = 6.24110958716075717114429577812E-1, a94 = -3.36089262944694129406857109825,
+                 a95 = -8.68219346841726006818189891453E-1, a96 = 2.75920996994467083049415600797E+1,
+                 a97 = 2.01540675504778934086186788979E+1, a98 = -4.34898841810699588477366255144E+1,
+                 aA1 = 4.77662536438264365890433908527E-1, aA4 = -2.48811461997166764192642586468,
+                 aA5 = -5.90290826836842996371446475743E-1, aA6 = 2.12300514481811942347288949897E+1,
+                 aA7 = 1.52792336328824235832596922938E+1, aA8 = -3.32882109689848629194453265587E+1,
+                 aA9 = -2.03312017085086261358222928593E-2, aB1 = -9.3714243008598732571704021658E-1,
+                 aB4 = 5.18637242884406370830023853209, aB5 = 1.09143734899672957818500254654,
+                 aB6 = -8.14978701074692612513997267357, aB7 = -1.85200656599969598641566180701E+1,
+                 aB8 = 2.27394870993505042818970056734E+1, aB9 = 2.49360555267965238987089396762,
+                 aBA = -3.0467644718982195003823669022, aC1 = 2.27331014751653820792359768449,
+                 aC4 = -1.05344954667372501984066689879E+1, aC5 = -2.00087205822486249909675718444,
+                 aC6 = -1.79589318631187989172765950534E+1, aC7 = 2.79488845294199600508499808837E+1,
+                 aC8 = -2.85899827713502369474065508674, aC9 = -8.87285693353062954433549289258,
+                 aCA = 1.23605671757943030647266201528E+1, aCB = 6.43392746015763530355970484046E-1;
+constexpr double b1 = 5.42937341165687622380535766363E-2, b6 = 4.45031289275240888144113950566,
+                 b7 = 1.89151789931450038304281599044, b8 = -5.8012039600105847814672114227,
+                 b9 = 3.1116436695781989440891606237E-1, bA = -1.52160949662516078556178806805E-1,
+                 bB = 2.01365400804030348374776537501E-1, bC = 4.47106157277725905176885569043E-2;
+constexpr double er1 = 0.1312004499419488073250102996E-01, er6 = -0.1225156446376204440720569753E+01,
+                 er7 = -0.4957589496572501915214079952E+00, er8 = 0.1664377182454986536961530415E+01,
+                 er9 = -0.3503288487499736816886487290E+00, erA = 0.3341791187130174790297318841E+00,
+                 erB = 0.8192320648511571246570742613E-01, erC = -0.2235530786388629525884427845E-01;
+constexpr double bhh1 = 0.244094488188976377952755905512E+00, bhh9 = 0.733846688281611857341361741547E+00,
+                 bhhC = 0.220588235294117647058823529412E-01;
+
+// Deterministic block-wide sum of one value per lane, added in lane order (all lanes get the same result).
+__device__ __forceinline__ double ordered_sum(double v, double *lds, int n) {
+    int lane = threadIdx.x;
+    __syncthreads();
+    lds[lane] = v;
+    __syncthreads();
+    double s = 0.;
+    for (int i = 0; i < n; i++) s += lds[i];
+    return s;
+}
+
+__global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restrict__ params,
+                                                 const pba_layer_init *__restrict__ init, int bw_per_run, int nt,
+                                                 const double *__restrict__ tgrid, double *__restrict__ table,
+                                                 pba_bw_result *__restrict__ res) {
+    __shared__ double red[64];
+    const int run = blockIdx.x;
+    const int lane = threadIdx.x;
+    const bool active = lane < bw_per_run;
+    const pba_run_params P = params[run];
+    Shock shfs, shrs;
+    {
+        double step = exp(log(P.fs.gam2) / (double)P.fs.ngam);
+        shfs.e0 = pow(step, 0.); shfs.etop = pow(step, (double)(P.fs.ngam - 1));
+        double stepr = exp(log(P.rs.gam2) / (double)P.rs.ngam);
+        shrs.e0 = pow(stepr, 0.); shrs.etop = pow(stepr, (double)(P.rs.ngam - 1));
+    }
+    BW b{};
+    double Y[NA], x[NA], k1[NA], k2[NA], k4[NA], k6[NA], k7[NA], k8[NA], k9[NA], kA[NA], kB[NA];
+    for (int k = 0; k < NA; k++) { Y[k] = 0.; x[k] = 0.; k1[k] = 0.; k2[k] = 0.; k4[k] = 0.; k6[k] = 0.; k7[k] = 0.;
+                                   k8[k] = 0.; k9[k] = 0.; kA[k] = 0.; kB[k] = 0.; }
+    const size_t bw_global = (size_t)run * bw_per_run + lane;
+    {   // clear this run's tables, all lanes striding over contiguous memory
+        double *Tr = table + (size_t)run * bw_per_run * PBA_NV * nt;
+        const size_t n = (size_t)bw_per_run * PBA_NV * nt;
+        for (size_t i = lane; i < n; i += 64) Tr[i] = 0.;
+        __syncthreads();
+    }
+    double *T = active ? table + bw_global * PBA_NV * nt : nullptr;
+    if (active) {
+        const pba_layer_init id = init[bw_global];
+        b.E0 = id.E0; b.M0 = id.M0; b.mom0 = id.mom0;
+        b.Gamma0 = gamma_from_mom(id.mom0);
+        b.R0 = beta_from_mom(id.mom0) * kC * tgrid[0];
+        b.theta_b0 = id.theta_c_h; b.theta_core = id.theta_core; b.theta_max = P.theta_max;
+        b.ilayer = id.ilayer; b.nlayers = id.nlayers;
+        b.fsrs = P.fsrs; b.do_rs = P.fsrs && P.do_rs; b.do_rs_rad = P.fsrs && P.do_rs_radiation;
+        b.rho_def = P.n_ism * kMppme;
+        b.fs_st.acc = 1.; b.fs_st.tcomov = -1.; b.rs_st.acc = 1.; b.rs_st.tcomov = -1.;
+        b.i_end_r = nt;
+        if (b.fsrs) {
+            double min_g0 = b.do_rs ? P.min_gamma0_for_rs : 0.;
+            if (b.Gamma0 < min_g0) { b.do_rs = 0; b.do_rs_rad = 0; b.shutOff = 1; b.fsrs = 0; }
+        }
+        if (b.M0 == 0. && b.E0 == 0.) b.end_evolution = 1;
+        else {
+            b.beta0 = beta_from_gamma(b.Gamma0);
+            double M20 = (2.0 / 3.0) * kPi * (1.0 - cos(b.theta_b0)) * b.rho_def * pow(b.R0, 3) / 1.;
+            if (b.M0 <= 0.) b.M0 = b.E0 / (b.Gamma0 * kC * kC);
+            Y[iR] = b.R0;
+            double b0m = beta_from_mom(b.mom0);
+            Y[itt] = b.R0 / (kC * b0m) * sqrt(1. + b.R0 * b.R0 * 0. * 0.) - b.R0 / kC;
+            Y[itcomov] = b.R0 / (b.beta0 * b.Gamma0 * kC);
+            Y[iEint2] = (b.Gamma0 - 1.) * M20 / b.M0 + 0. / b.M0 / kC / kC;
+            Y[iGamma] = b.Gamma0;
+            Y[itheta] = b.theta_b0;
+            Y[iM2] = M20 / b.M0;
+            if (b.fsrs && P.init_deltaR4) Y[ideltaR4] = kC * P.tprompt * b.beta0;
+            b.Rw0 = .5 * Y[iR];
+            b.Wtt0 = Y[itt];
+        }
+        post_step(P, shfs, shrs, b, Y, 0, tgrid[0], T, nt);
+    }
+    const double rtol = P.rtol, atol = P.rtol;
+    const double nsz = (double)(NEQ_REF * bw_per_run);
+    double time = P.tb0;
+    int nsteps = 0, nrhs = 0;
+    for (int it = 1; it < nt; it++) {
+        const double timeN = time + (tgrid[it] - tgrid[it - 1]);
+        const double hmax = fabs(timeN - time);
+        rhs(P, shfs, b, active, Y, k1); nrhs++;
+        // initial step (ode_solvers.h:1596-1628)
+        double h;
+        {
+            double dnf = 0., dny = 0.;
+            if (active) {
+                for (int k = 0; k < NA; k++) {
+                    double sk = atol + rtol * fabs(Y[k]);
+                    dnf += (k1[k] / sk) * (k1[k] / sk);
+                    dny += (Y[k] / sk) * (Y[k] / sk);
+                }
+                double skw = atol + rtol * fabs(b.Rw0);
+                dny += (b.Rw0 / skw) * (b.Rw0 / skw);
+                double skt = atol + rtol * fabs(b.Wtt0);
+                dny += (b.Wtt0 / skt) * (b.Wtt0 / skt);
+            }
+            dnf = ordered_sum(dnf, red, bw_per_run);
+            dny = ordered_sum(dny, red, bw_per_run);
+            double h1 = (dnf <= 1e-10 || dny <= 1e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
+            h1 = fmin(h1, hmax);
+            for (int k = 0; k < NA; k++) x[k] = Y[k] + h1 * k1[k];
+            rhs(P, shfs, b, active, x, k2); nrhs++;
+            double n2 = 0.;
+            if (active)
+                for (int k = 0; k < NA; k++) {
+                    double sk = atol + rtol * x86_int_abs(Y[k]);
+                    double z = (k2[k] - k1[k]) / sk;
+                    n2 += z * z;
+                }
+            n2 = ordered_sum(n2, red, bw_per_run);
+            n2 = sqrt(n2) / h1;
+            double der12 = fmax(n2, sqrt(dnf));
+            double h2 = (der12 <= 1e-15) ? fmax(1e-6, fabs(h1) * 1e-3) : pow(0.01 / der12, 1.0 / 8);
+            h = fmin(fmin(100 * h1, h2), hmax);
+        }
+        bool reject = false, last = false;
+        for (int guard = 0; guard < 100000; guard++) {   // step budget per output interval
+            if ((time + 1.01 * h - timeN) > 0.0) { h = timeN - time; last = true; }
+            // stages (ode_solvers.h:1434-1479); k3 and k5 share k2's storage, kC shares k4's
+            double *k3 = k2, *k5 = k2, *kC = k4;
+            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a21 * k1[k]);
+            rhs(P, shfs, b, active, x, k2);
+            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a31 * k1[k] + a32 * k2[k]);
+            rhs(P, shfs, b, active, x, k3);
+            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a41 * k1[k] + a43 * k3[k]);
+            rhs(P, shfs, b, active, x, k4);
+            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a51 * k1[k] + a53 * k3[k] + a54 * k4[k]);
+            rhs(P, shfs, b, active, x, k5);
+            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a61 * k1[k] + a64 * k4[k] + a65 * k5[k]);
+            rhs(P, shfs, b, active, x, k6);
+            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a71 * k1[k] + a74 * k4[k] + a75 * k5[k] + a76 * k6[k]);
+            rhs(P, shfs, b, active, x, k7);
+            for (int k = 0; k < NA; k++)
+                x[k] = Y[k] + h * (a81 * k1[k] + a84 * k4[k] + a85 * k5[k] + a86 * k6[k] + a87 * k7[k]);
+            rhs(P, shfs, b, active, x, k8);
+            for (int k = 0; k < NA; k++)
+                x[k] = Y[k] + h * (a91 * k1[k] + a94 * k4[k] + a95 * k5[k] + a96 * k6[k] + a97 * k7[k] + a98 * k8[k]);
+            rhs(P, shfs, b, active, x, k9);
+            for (int k = 0; k < NA; k++)
+                x[k] = Y[k] + h * (aA1 * k1[k] + aA4 * k4[k] + aA5 * k5[k] + aA6 * k6[k] + aA7 * k7[k] + aA8 * k8[k] +
+                                   aA9 * k9[k]);
+            rhs(P, shfs, b, active, x, kA);
+            for (int k = 0; k < NA; k++)
+                x[k] = Y[k] + h * (aB1 * k1[k] + aB4 * k4[k] + aB5 * k5[k] + aB6 * k6[k] + aB7 * k7[k] + aB8 * k8[k] +
+                                   aB9 * k9[k] + aBA * kA[k]);
+            rhs(P, shfs, b, active, x, kB);
+            for (int k = 0; k < NA; k++)
+                x[k] = Y[k] + h * (aC1 * k1[k] + aC4 * k4[k] + aC5 * k5[k] + aC6 * k6[k] + aC7 * k7[k] + aC8 * k8[k] +
+                                   aC9 * k9[k] + aCA * kA[k] + aCB * kB[k]);
+            rhs(P, shfs, b, active, x, kC);
+            nrhs += 11;
+            double e1s = 0., e2s = 0.;
+            for (int k = 0; k < NA; k++) {
+                double slope = b1 * k1[k] + b6 * k6[k] + b7 * k7[k] + b8 * k8[k] + b9 * k9[k] + bA * kA[k] + bB * kB[k] +
+                               bC * kC[k];
+                double yn = Y[k] + h * slope;
+                double sk = atol + rtol * fmax(fabs(Y[k]), fabs(yn));
+                double e1 = er1 * k1[k] + er6 * k6[k] + er7 * k7[k] + er8 * k8[k] + er9 * k9[k] + erA * kA[k] +
+                            erB * kB[k] + erC * kC[k];
+                double e2 = slope - bhh1 * k1[k] - bhh9 * k9[k] - bhhC * kC[k];
+                if (active) { e1s += (e1 / sk) * (e1 / sk); e2s += (e2 / sk) * (e2 / sk); }
+                x[k] = yn;
+            }
+            e1s = ordered_sum(e1s, red, bw_per_run);
+            e2s = ordered_sum(e2s, red, bw_per_run);
+            double deno = e1s + 0.01 * e2s;
+            if (deno <= 0.0) deno = 1.0;
+            double err = fabs(h) * e1s / sqrt(nsz * deno);
+            double fac11 = pow(err, 1.0 / 8.0);
+            double fac = fac11 / (1.0 * 0.9);
+            fac = fac < 1.0 / 6.0 ? 1.0 / 6.0 : (fac > 1.0 / 0.333 ? 1.0 / 0.333 : fac);
+            double hnew = h / fac;
+            if (err > 1.0) {
+                h /= fmin(1.0 / 0.333, fac11 / 0.9);
+                reject = true;
+                last = false;
+            } else {
+                rhs(P, shfs, b, active, x, k2); nrhs++;   // kD shares k2's storage
+                for (int k = 0; k < NA; k++) { k1[k] = k2[k]; Y[k] = x[k]; }
+                time += h;
+                nsteps++;
+                if (last) break;
+                if (fabs(hnew) > hmax) hnew = hmax;
+                if (reject) hnew = fmin(fabs(hnew), fabs(h));
+                reject = false;
+                h = hnew;
+            }
+            if (guard == 100000 - 1) b.status |= PBAGPU_S_NMAX;
+        }
+        if (active && !b.end_evolution) {
+            double g = Y[iGamma], be = beta_from_gamma(g), E2 = Y[iEint2];
+            if ((E2 <= 0.) || !isfinite(E2) || (g < 1.) || (be < 0) || (fabs(be) < 1.e-6) || !isfinite(be))
+                b.status |= PBAGPU_S_TERMINATED;
+            if (!b.end_spreading && Y[itheta] > 0.99 * b.theta_max) b.end_spreading = 1;
+            if (!(b.shutOff || !b.do_rs)) {
+                double alpha_of = P.tprompt * b.beta0 * kC;
+                double rho4f = b.M0 / (2.0 * alpha_of * kPi * (1.0 - cos(b.theta_b0))) / (Y[iR] * Y[iR]);
+                double rho4 = P.exponential_rho4 ? rho4f * exp(-Y[ideltaR4] / alpha_of) : rho4f;
+                if ((Y[iGamma] < b.Gamma0) && (rho4 < P.rs_shutoff_rho || Y[iM3] >= 1.)) b.shutOff = 1;
+            }
+            if (g < 1. || Y[itheta] < b.theta_b0 * 0.99 || Y[itt] <= 0.) b.status |= PBAGPU_S_NONFINITE;
+        }
+        b.prev_idx_x = it;
+        if (active) post_step(P, shfs, shrs, b, Y, it, tgrid[it], T, nt);
+    }
+    if (active) {
+        pba_bw_result r;
+        r.status = b.status; r.i_end_r = b.i_end_r; r.fsrs = b.fsrs; r.n_steps = nsteps; r.n_rhs = nrhs; r.pad_ = 0;
+        res[bw_global] = r;
+    }
+}
+
+}  // namespace dyn
+}  // namespace pbagpu
+
+extern "C" int pbagpu_dyn_evolve(const pba_run_params *params, int n_runs, int bw_per_run, const pba_layer_init *init,
+                                 int nt, const double *d_tgrid, double *d_table, pba_bw_result *res, void *stream) {
+    if (!params || !init || !d_tgrid || !d_table || !res || n_runs < 1 || bw_per_run < 1 || bw_per_run > 64 || nt < 2)
+        return PBAGPU_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nbw = (size_t)n_runs * bw_per_run;
+    pba_run_params *dP = nullptr;
+    pba_layer_init *dI = nullptr;
+    pba_bw_result *dR = nullptr;
+    if (hipMalloc(&dP, sizeof(pba_run_params) * n_runs) != hipSuccess) return PBAGPU_E_HIP;
+    if (hipMalloc(&dI, sizeof(pba_layer_init) * nbw) != hipSuccess) { hipFree(dP); return PBAGPU_E_HIP; }
+    if (hipMalloc(&dR, sizeof(pba_bw_result) * nbw) != hipSuccess) { hipFree(dP); hipFree(dI); return PBAGPU_E_HIP; }
+    int rc = PBAGPU_OK;
+    if (hipMemcpyAsync(dP, params, sizeof(pba_run_params) * n_runs, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dI, init, sizeof(pba_layer_init) * nbw, hipMemcpyHostToDevice, s) != hipSuccess)
+        rc = PBAGPU_E_HIP;
+    if (rc == PBAGPU_OK) {
+        hipLaunchKernelGGL(pbagpu::dyn::dyn_kernel, dim3(n_runs), dim3(64), 0, s, dP, dI, bw_per_run, nt, d_tgrid,
+                           d_table, dR);
+        if (hipGetLastError() != hipSuccess) rc = PBAGPU_E_HIP;
+    }
+    if (rc == PBAGPU_OK && hipMemcpyAsync(res, dR, sizeof(pba_bw_result) * nbw, hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = PBAGPU_E_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) rc = PBAGPU_E_HIP;
+    hipFree(dP); hipFree(dI); hipFree(dR);
+    return rc;
+}

@@ -1,0 +1,438 @@
+// Stage 3 — equal-time-arrival-surface (EATS) light curves on the GPU.
+//
+// One lane per (layer, t_obs, nu_obs) pair; lanes of a wavefront take consecutive
+// pairs of one layer (same t_obs, neighbouring nu_obs), so they share the surface
+// geometry and read the same rows of the layer's tables.  Each lane runs the
+// reference's nested adaptive quadrature: CADRE over phi in [0, pi], and for every
+// phi sample CADRE over 1 - cos(theta) between the layer edges found by
+// find_jet_edge, for the primary jet and the counter-jet (src/eats.h:103-856).
+//
+// The CADRE interval store keeps the reference's binary max-heap order
+// (src/utilitites/quadratures.h:655-770, 1613-2115) so that the sequence of
+// integrand samples and the final summation order are those of the reference.
+// Heap entries are (error, slot) pairs; the interval records live in a per-lane
+// pool in private memory, so a heap swap moves 12 bytes instead of 112.
+//
+// Integrand: EATS::integrand + fluxDensAdaptiveWithComov + ElectronAndRadiation::
+// fluxDens / computeFluxDensity (src/eats.h:175-458, src/blastwave/blastwave_radiation.h:75-141,
+// src/microphysics/microphysics.h:799-867, 1185-1217); the observer-angle
+// table mu[i] = (t_b[i] - t_obs/(1+z)) c / R[i] is evaluated on the fly during the
+// index search instead of being stored.
+#include "pba_device.h"
+
+namespace pbagpu {
+namespace eats {
+
+constexpr int HI = 256;  // inner (theta) interval store (reference cases need <= ~160)
+constexpr int HO = 256;  // outer (phi) interval store (off-axis cases need <= ~190)
+
+struct Iv {
+    double a, b, I, err, fa, fll, fl, flr, fm, frl, fr, frr, fb;
+    int refinement;
+};
+
+template <int H>
+struct Store {
+    Iv pool[H];
+    double herr[H];
+    int hslot[H];
+    int N, used, overflow;
+    __device__ void reset() { N = 0; used = 0; overflow = 0; }
+    __device__ int alloc() {
+        if (used >= H) { overflow = 1; return H - 1; }
+        return used++;
+    }
+    __device__ void push(int slot) {
+        if (N >= H) { overflow = 1; return; }
+        double e = pool[slot].err;
+        int c = N++;
+        herr[c] = e; hslot[c] = slot;
+        while (c != 0) {
+            int p = (c - 1) / 2;
+            if (!(herr[p] < herr[c])) break;
+            double te = herr[p]; herr[p] = herr[c]; herr[c] = te;
+            int ts = hslot[p]; hslot[p] = hslot[c]; hslot[c] = ts;
+            c = p;
+        }
+    }
+    __device__ int pop() {
+        int top = hslot[0];
+        herr[0] = herr[N - 1]; hslot[0] = hslot[N - 1];
+        N--;
+        int p = 0, c1 = 1, c2 = 2;
+        while (c1 < N) {
+            int c = c1;
+            double e = herr[c1];
+            if (c2 < N && herr[c2] > e) { c = c2; e = herr[c2]; }
+            if (e <= herr[p]) break;
+            double te = herr[p]; herr[p] = herr[c]; herr[c] = te;
+            int ts = hslot[p]; hslot[p] = hslot[c]; hslot[c] = ts;
+            p = c; c1 = 2 * p + 1; c2 = c1 + 1;
+        }
+        return top;
+    }
+    __device__ double total_I() const { double s = 0.; for (int i = 0; i < N; i++) s += pool[hslot[i]].I; return s; }
+    __device__ double total_err() const { double s = 0.; for (int i = 0; i < N; i++) s += pool[hslot[i]].err; return s; }
+};
+
+template <class F>
+__device__ int cadre_process(F &f, Iv *i) {
+    const double scaleTol = 0.2;
+    int count = 0;
+    if (i->refinement < 4) {
+        double fa = i->fa, fm = i->fm, fb = i->fb, fl, fr;
+        if (i->refinement == 0) {
+            fl = f(0.75 * i->a + 0.25 * i->b); count++;
+            fr = f(0.25 * i->a + 0.75 * i->b); count++;
+            i->fl = fl; i->fr = fr;
+        } else {
+            fl = i->fl; fr = i->fr;
+            i->refinement = 0;
+        }
+        double hh = 0.25 * (i->b - i->a);
+        double T0 = 2 * hh * (fa + fb);
+        double T1 = 0.5 * T0 + 2 * hh * fm;
+        double T2 = 0.5 * T1 + hh * (fl + fr);
+        double R0 = (T1 - T0) / (T2 - T1);
+        if (R0 > 4 - scaleTol && R0 < 4 + scaleTol) {
+            double fll = f(0.875 * i->a + 0.125 * i->b);
+            double flr = f(0.625 * i->a + 0.375 * i->b);
+            double frl = f(0.375 * i->a + 0.625 * i->b);
+            double frr = f(0.125 * i->a + 0.875 * i->b);
+            count += 4;
+            i->fll = fll; i->flr = flr; i->frl = frl; i->frr = frr;
+            i->refinement = 1;
+            double T3 = 0.5 * T2 + 0.5 * hh * (fll + flr + frl + frr);
+            double R1 = (T2 - T1) / (T3 - T2);
+            if (R1 > 4 - scaleTol && R1 < 4 + scaleTol) {
+                double R11 = (4 * T1 - T0) / 3.0;
+                double R21 = (4 * T2 - T1) / 3.0;
+                double R22 = (16 * R21 - R11) / 15.0;
+                i->fa = T3;
+                i->fll = (4 * T3 - T2) / 3.0;
+                i->fl = (16 * i->fll - R21) / 15.0;
+                i->flr = (64 * i->fl - R22) / 63.0;
+                i->I = i->flr;
+                i->err = fabs((i->fl - R22) / 63.0);
+                i->refinement = 4;
+            } else {
+                double errt = (T3 - T2) / 3.0;
+                i->err = fabs(errt);
+                i->I = T3 + errt;
+            }
+        } else if (R0 >= 4 + scaleTol || R0 < 2 || R0 != R0) {
+            double e10 = fabs(T1 - T0), e21 = fabs(T2 - T1);
+            i->err = e10 > e21 ? e10 : e21;
+            i->I = T2;
+        } else {
+            double err = -(T2 - T1) * (T2 - T1) / (T2 - 2 * T1 + T0);
+            i->err = fabs(err);
+            i->I = (fabs(T2 + err) < 1.0e-14 * fabs(T2)) ? 1.0e-14 * T2 : T2 + err;
+        }
+        return count;
+    }
+    int nint = 1 << i->refinement;
+    double hh = (i->b - i->a) / nint;
+    double sum = 0;
+    for (int n = 1; n < nint; n += 2) { sum += f(i->a + hh * n); count++; }
+    double T = 0.5 * i->fa + hh * sum;
+    double Rnm = i->fa, Rnmp = i->fll;
+    i->fa = T;
+    i->fll = (4.0 * i->fa - Rnm) / 3.0;
+    if (i->refinement == 1) { i->I = i->fll; i->err = fabs((i->fa - Rnm) / 3.0); i->refinement++; return count; }
+    Rnm = Rnmp; Rnmp = i->fl;
+    i->fl = (16 * i->fll - Rnm) / 15.0;
+    if (i->refinement == 2) { i->I = i->fl; i->err = fabs((i->fll - Rnm) / 15.0); i->refinement++; return count; }
+    Rnm = Rnmp; Rnmp = i->flr;
+    i->flr = (64 * i->fl - Rnm) / 63.0;
+    if (i->refinement == 3) { i->I = i->flr; i->err = fabs((i->fl - Rnm) / 63.0); i->refinement++; return count; }
+    Rnm = Rnmp; Rnmp = i->fm;
+    i->fm = (256 * i->flr - Rnm) / 255.0;
+    if (i->refinement == 4) { i->I = i->fm; i->err = fabs((i->flr - Rnm) / 255.0); i->refinement++; return count; }
+    Rnm = Rnmp; Rnmp = i->frl;
+    i->frl = (1024 * i->fm - Rnm) / 1023.0;
+    if (i->refinement == 5) { i->I = i->frl; i->err = fabs((i->fm - Rnm) / 1023.0); i->refinement++; return count; }
+    Rnm = Rnmp; Rnmp = i->fr;
+    i->fr = (4096 * i->frl - Rnm) / 4095.0;
+    if (i->refinement == 6) { i->I = i->fr; i->err = fabs((i->frl - Rnm) / 4095.0); i->refinement++; return count; }
+    Rnm = Rnmp; Rnmp = i->frr;
+    i->frr = (16384 * i->fr - Rnm) / 16383.0;
+    if (i->refinement == 7) { i->I = i->frr; i->err = fabs((i->fr - Rnm) / 16383.0); i->refinement++; return count; }
+    Rnm = Rnmp; Rnmp = i->fb;
+    i->fb = (65536 * i->frr - Rnm) / 65535.0;
+    if (i->refinement == 8) { i->I = i->fb; i->err = fabs((i->frr - Rnm) / 65535.0); i->refinement++; return count; }
+    Rnm = Rnmp;
+    i->I = (262144 * i->fb - Rnm) / 262143.0;
+    i->err = fabs((i->fb - Rnm) / 262143.0);
+    i->refinement++;
+    return count;
+}
+
+template <int H, class F>
+__device__ double cadre_adapt(Store<H> &S, F &f, double xa, double xb, int Nmax, double atol, double rtol) {
+    S.reset();
+    int s0 = S.alloc();
+    Iv &i0 = S.pool[s0];
+    i0.a = xa; i0.b = xb; i0.I = 0; i0.err = 0;
+    i0.fll = 0; i0.flr = 0; i0.frl = 0; i0.frr = 0; i0.fl = 0; i0.fr = 0; i0.refinement = 0;
+    i0.fa = f(xa);
+    i0.fb = f(xb);
+    i0.fm = f(0.5 * (xa + xb));
+    int n = 3;
+    n += cadre_process(f, &i0);
+    S.push(s0);
+    double I = i0.I, err = i0.err;
+    int num_intervals = 1, num_iterations = 0, last_check = 0;
+    while (n < Nmax && err > atol + rtol * fabs(I) && !S.overflow) {
+        int sl = S.pop();
+        Iv &iv = S.pool[sl];
+        if (iv.refinement >= 4) {
+            double oldI = iv.I, olderr = iv.err;
+            n += cadre_process(f, &iv);
+            S.push(sl);
+            err += iv.err - olderr;
+            I += iv.I - oldI;
+        } else {
+            Iv p = iv;
+            int s2 = S.alloc();
+            Iv &i1 = S.pool[sl];
+            Iv &i2 = S.pool[s2];
+            double xm = 0.5 * (p.a + p.b);
+            i1.a = p.a; i1.b = xm; i1.fa = p.fa; i1.fl = p.fll; i1.fm = p.fl; i1.fr = p.flr; i1.fb = p.fm;
+            i1.refinement = p.refinement;
+            i2.a = xm; i2.b = p.b; i2.fa = p.fm; i2.fl = p.frl; i2.fm = p.fr; i2.fr = p.frr; i2.fb = p.fb;
+            i2.refinement = p.refinement;
+            n += cadre_process(f, &i1);
+            n += cadre_process(f, &i2);
+            S.push(sl);
+            S.push(s2);
+            num_intervals++;
+            err += i1.err + i2.err - p.err;
+            I += i1.I + i2.I - p.I;
+        }
+        if (num_iterations == 2 * last_check) {
+            err = S.total_err();
+            I = S.total_I();
+            last_check = num_intervals;
+        }
+        num_iterations++;
+    }
+    return S.total_I();
+}
+
+struct LayerArgs {
+    const double *T;    // table [PBA_NV][nt]
+    const double *jfs;  // [nt][nf]
+    const double *jrs;  // or null
+    const double *nufs, *nurs;
+    double theta_c_l, theta_c_h;
+    double z, theta_obs, d_l, rtol_theta, rtol_phi;
+    int nmax_theta, nmax_phi, counter_jet, i_end_r;
+    int nt, nf;
+};
+
+struct Ctx {
+    const LayerArgs *L;
+    double tz;          // t_obs / (1 + z)
+    double nu_obs;
+    double sa, ca;      // sin, cos of the observer angle
+    bool cj;
+    int status;
+    __device__ double V(int v, int i) const { return L->T[(size_t)v * L->nt + i]; }
+    __device__ double mu_at(int i) const { return (V(PV_tburst, i) - tz) / V(PV_R, i) * kC; }
+    __device__ double obs_angle(double theta, double phi) const {
+        if (!cj) return sa * (sin(theta) * sin(phi)) + ca * cos(theta);
+        return sa * (sin(kPi - theta) * sin(phi)) + ca * cos(kPi - theta);
+    }
+    __device__ int mu_index(double mu) const {
+        auto arr = [&](int i) { return mu_at(i); };
+        return find_index(mu, arr, L->i_end_r);
+    }
+};
+
+// find_jet_edge (src/eats.h:103-173)
+__device__ double find_jet_edge(const Ctx &C, double phi, double th_con_hi) {
+    double mu = C.obs_angle(th_con_hi, phi);
+    int ia = C.mu_index(mu);
+    double a0 = C.V(PV_theta, ia), a1 = C.V(PV_theta, ia + 1);
+    if (a0 <= th_con_hi && th_con_hi <= a1) return th_con_hi;
+    double ta, tb;
+    if (th_con_hi < a0) { ta = th_con_hi; tb = 0.5 * M_PI; }
+    else { ta = 0.0; tb = th_con_hi; }
+    int i = 0;
+    while (tb - ta > 1.0e-5 && i < 100) {
+        double tc = 0.5 * (ta + tb);
+        mu = C.obs_angle(tc, phi);
+        ia = C.mu_index(mu);
+        if (tc < C.V(PV_theta, ia)) ta = tc; else tb = tc;
+        i++;
+    }
+    return ta;
+}
+
+__device__ double shock_flux(Ctx &C, const double *jall, const double *nu, double freq, int ia, int ib, double Gamma,
+                             double mu, double r, double ne, double n_prime, double theta) {
+    const int nf = C.L->nf;
+    if (freq >= nu[nf - 1] || freq <= nu[0]) { C.status |= PBAGPU_S_NU_OUT_OF_GRID; return 0.; }
+    auto arr = [&](int i) { return nu[i]; };
+    int ia_nu = find_index(freq, arr, nf), ib_nu = ia_nu + 1;
+    double x1 = nu[ia_nu], x2 = nu[ib_nu], y1 = C.V(PV_R, ia), y2 = C.V(PV_R, ib);
+    double q11 = jall[(size_t)ia * nf + ia_nu], q12 = jall[(size_t)ib * nf + ia_nu];
+    double q21 = jall[(size_t)ia * nf + ib_nu], q22 = jall[(size_t)ib * nf + ib_nu];
+    double x2x1 = x2 - x1, y2y1 = y2 - y1, x2x = x2 - freq, y2y = y2 - r, yy1 = r - y1, xx1 = freq - x1;
+    double em_prime = 1.0 / (x2x1 * y2y1) * (q11 * x2x * y2y + q21 * xx1 * y2y + q12 * x2x * yy1 + q22 * xx1 * yy1);
+    double beta = beta_from_gamma(Gamma);
+    double delta_D = Gamma * (1.0 - beta * mu);
+    double em_lab = em_prime / (delta_D * delta_D);
+    // no synchrotron self-absorption: the optical depth is 0 unless mu equals the shock velocity
+    double intensity = (mu == beta) ? em_lab * (-expm1(-1.0e100) / 1.0e100) : em_lab;
+    double omc = (1. - cos(theta) / 1.);
+    double lum = intensity * ne / (2. * kPi * omc) / Gamma / (1.0 - mu * beta);
+    if (!(lum >= 0) || !isfinite(lum)) C.status |= PBAGPU_S_NONFINITE;
+    return lum;
+}
+
+__device__ double integrand(Ctx &C, double cos_theta, double phi) {
+    const int N = C.L->i_end_r;
+    if (C.V(PV_R, 0) == 0.0 && C.V(PV_R, N - 1) == 0.0) return 0.0;
+    double a_theta = 2.0 * asin(sqrt(0.5 * (1.0 - cos_theta)));
+    double mu = C.obs_angle(a_theta, phi);
+    double mu0 = C.mu_at(0), muN = C.mu_at(N - 1);
+    if (mu < mu0 || mu > muN) return 0.;
+    if (mu0 >= mu) return 0.;
+    int ia = C.mu_index(mu), ib = ia + 1;
+    double mua = C.mu_at(ia), mub = C.mu_at(ib);
+    double tba = C.V(PV_tburst, ia), tbb = C.V(PV_tburst, ib);
+    double t_e = interp_lin(mu, mua, mub, tba, tbb);
+    if (t_e < 0.0 || !isfinite(t_e)) return 0.;
+    double Ra = C.V(PV_R, ia), Rb = C.V(PV_R, ib);
+    double r = interp_log(t_e, tba, tbb, Ra, Rb);
+    if (r <= 0.0 || !isfinite(r)) return 0.;
+    // shared log-interpolation weight in R for all state variables
+    double wr = log(r / Ra) / log(Rb / Ra);
+    auto li = [&](int v) { double ya = C.V(v, ia), yb = C.V(v, ib); return ya * pow(yb / ya, wr); };
+    double Gamma = li(PV_Gamma);
+    double beta = li(PV_beta);
+    double ne = li(PV_M2) / kMp;
+    double n_prime = li(PV_rho2) / kMp;
+    double nuprime = (1.0 + C.L->z) * C.nu_obs * (Gamma * (1.0 - beta * mu));
+    double theta = li(PV_theta);
+    double flux = shock_flux(C, C.L->jfs, C.L->nufs, nuprime, ia, ib, Gamma, mu, r, ne, n_prime, theta);
+    if (C.L->jrs && C.V(PV_thickness_rs, ia) > 0 && C.V(PV_thickness_rs, ib) > 0) {
+        double ne_rs = li(PV_M3) / kMp;
+        double np_rs = li(PV_rho3) / kMp;
+        flux += shock_flux(C, C.L->jrs, C.L->nurs, nuprime, ia, ib, Gamma, mu, r, ne_rs, np_rs, theta);
+    }
+    return flux;
+}
+
+__device__ double integrate_theta_phi(Ctx &C, Store<HO> &So, Store<HI> &Si) {
+    const LayerArgs &L = *C.L;
+    const double Fcoeff = kCgs2mJy / (4 * M_PI * L.d_l * L.d_l);
+    const double th1c = L.theta_c_h, th0c = L.theta_c_l;
+    auto phi_integrand = [&](double a_phi) -> double {
+        double th_1 = find_jet_edge(C, a_phi, th1c);
+        double th_0 = (th0c / th1c) * th_1;
+        double t0 = th_0 > 0.5 * M_PI ? 0.5 * M_PI : th_0;
+        double t1 = th_1 > 0.5 * M_PI ? 0.5 * M_PI : th_1;
+        double s0 = sin(0.5 * t0), s1 = sin(0.5 * t1);
+        auto ct = [&](double aomct) -> double { return integrand(C, 1 - aomct, a_phi); };
+        double res = cadre_adapt(Si, ct, 2.0 * s0 * s0, 2.0 * s1 * s1, L.nmax_theta, 0., L.rtol_theta);
+        if (Si.overflow) C.status |= PBAGPU_S_HEAP_FULL;
+        if (res != res || res < 0.0) return 0.;
+        return res;
+    };
+    double r = 2 * Fcoeff * cadre_adapt(So, phi_integrand, 0.0, M_PI, L.nmax_phi, 0., L.rtol_phi);
+    if (So.overflow) C.status |= PBAGPU_S_HEAP_FULL;
+    return r;
+}
+
+__global__ void __launch_bounds__(64) eats_kernel(const LayerArgs *__restrict__ layers, int n_layers, int n_pairs,
+                                                  const double *__restrict__ tobs, const double *__restrict__ nuobs,
+                                                  double *__restrict__ out, int *__restrict__ status) {
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (size_t)n_layers * n_pairs) return;
+    const int l = (int)(gid / n_pairs), k = (int)(gid % n_pairs);
+    const LayerArgs &L = layers[l];
+    Store<HO> So;
+    Store<HI> Si;
+    Ctx C;
+    C.L = &L;
+    C.status = 0;
+    C.sa = sin(L.theta_obs);
+    C.ca = cos(L.theta_obs);
+    const double t_obs = tobs[k];
+    C.tz = t_obs / (1.0 + L.z);
+    C.nu_obs = nuobs[k];
+    double fl = 0.;
+    if (L.i_end_r == 1) C.status |= PBAGPU_S_NONFINITE;  // a single usable step: no surface to integrate
+    if (L.i_end_r > 1) {
+        if (t_obs < C.V(PV_tburst, 0) || t_obs > C.V(PV_tburst, L.i_end_r - 1)) C.status |= PBAGPU_S_NONFINITE;
+        C.cj = false;
+        fl += integrate_theta_phi(C, So, Si);
+        if (L.counter_jet) {
+            C.cj = true;
+            fl += integrate_theta_phi(C, So, Si);
+        }
+    }
+    out[gid] = 0. + fl;
+    if (C.status) atomicOr(&status[l], C.status);
+}
+
+}  // namespace eats
+}  // namespace pbagpu
+
+extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, const pba_eats_layer *layers,
+                                      int n_layers, const double *d_table, int nt, const double *d_jall, int nfreq,
+                                      int n_pairs, const double *d_tobs, const double *d_nuobs, double *d_out,
+                                      int *d_status, void *stream) {
+    using namespace pbagpu::eats;
+    if (!params || !layers || !d_table || !d_jall || !d_out || !d_status || n_layers < 1 || n_pairs < 1 || nt < 2)
+        return PBAGPU_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    // frequency grids of every (run, shock), built on the host like MakeLogspaceVec (utils.h:91-105)
+    std::vector<double> hnu((size_t)n_runs * 2 * nfreq);
+    for (int r = 0; r < n_runs; r++)
+        for (int sh = 0; sh < 2; sh++) {
+            const pba_shock_params &sp = sh ? params[r].rs : params[r].fs;
+            if (sp.nfreq != nfreq) return PBAGPU_E_ARG;
+            double cur = std::pow(10.0, std::log10(sp.freq1));
+            double ratio = std::pow(10.0, (std::log10(sp.freq2) - std::log10(sp.freq1)) / sp.nfreq);
+            for (int i = 0; i < nfreq; i++) { hnu[((size_t)r * 2 + sh) * nfreq + i] = cur; cur *= ratio; }
+        }
+    double *dnu = nullptr;
+    LayerArgs *dL = nullptr;
+    if (hipMalloc(&dnu, sizeof(double) * hnu.size()) != hipSuccess) return PBAGPU_E_HIP;
+    if (hipMalloc(&dL, sizeof(LayerArgs) * n_layers) != hipSuccess) { (void)hipFree(dnu); return PBAGPU_E_HIP; }
+    std::vector<LayerArgs> hl(n_layers);
+    for (int l = 0; l < n_layers; l++) {
+        const pba_eats_layer &Y = layers[l];
+        if (Y.run < 0 || Y.run >= n_runs || Y.i_end_r > nt) { (void)hipFree(dnu); (void)hipFree(dL); return PBAGPU_E_ARG; }
+        const pba_run_params &P = params[Y.run];
+        LayerArgs &A = hl[l];
+        A.T = d_table + (size_t)Y.bw * PBA_NV * nt;
+        A.jfs = d_jall + (size_t)Y.pop_fs * nt * nfreq;
+        A.jrs = Y.pop_rs >= 0 ? d_jall + (size_t)Y.pop_rs * nt * nfreq : nullptr;
+        A.nufs = dnu + ((size_t)Y.run * 2) * nfreq;
+        A.nurs = dnu + ((size_t)Y.run * 2 + 1) * nfreq;
+        A.theta_c_l = Y.theta_c_l; A.theta_c_h = Y.theta_c_h;
+        A.z = P.z; A.theta_obs = P.theta_obs; A.d_l = P.d_l; A.rtol_theta = P.rtol_theta; A.rtol_phi = P.rtol_phi;
+        A.nmax_theta = P.nmax_theta; A.nmax_phi = P.nmax_phi; A.counter_jet = P.counter_jet; A.i_end_r = Y.i_end_r;
+        A.nt = nt; A.nf = nfreq;
+    }
+    int rc = PBAGPU_OK;
+    if (hipMemcpyAsync(dnu, hnu.data(), sizeof(double) * hnu.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dL, hl.data(), sizeof(LayerArgs) * n_layers, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemsetAsync(d_status, 0, sizeof(int) * n_layers, s) != hipSuccess)
+        rc = PBAGPU_E_HIP;
+    if (rc == PBAGPU_OK) {
+        size_t nth = (size_t)n_layers * n_pairs;
+        hipLaunchKernelGGL(eats_kernel, dim3((unsigned)((nth + 63) / 64)), dim3(64), 0, s, dL, n_layers, n_pairs, d_tobs,
+                           d_nuobs, d_out, d_status);
+        if (hipGetLastError() != hipSuccess) rc = PBAGPU_E_HIP;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) rc = PBAGPU_E_HIP;
+    (void)hipFree(dnu);
+    (void)hipFree(dL);
+    return rc;
+}

@@ -1,0 +1,436 @@
+// Stage 2 — electron kinetics (Chang-Cooper) and comoving synchrotron + SSC emissivity.
+//
+// One 64-lane wavefront per electron population (forward or reverse shock of one
+// blastwave), stepping through all output steps.  Lane L owns the contiguous
+// Lorentz-factor bins [L*m, L*m+m), m = ceil(ngam/64) <= 8.
+//
+// Chang-Cooper substep (ChangCooper::setupVectors / solve / tridiagonal_solver,
+// src/microphysics/numeric_model.h:19-43, 539-708): with zero dispersion the
+// sub-diagonal vanishes, the forward sweep is skipped by the reference, and the
+// update is the first-order linear recurrence f_j = d_j/b_j - (c_j/b_j) f_{j+1}
+// taken from the top bin down.  Each affine map x -> d_j/b_j - (c_j/b_j) x is
+// composed per lane, the lane maps are combined by a 6-round suffix scan over the
+// wavefront, and each lane then applies its maps: O(m + log 64) dependent steps
+// instead of O(ngam).  Heating/cooling terms and the tridiagonal coefficients are
+// constant over the substeps of one step and are built once per step.
+//
+// Emission (computeRadiationNumericMixed, src/microphysics/microphysics.h:492-599):
+// lanes own frequencies; the CSYN synchrotron kernel (Aharonian et al. 2010, eq. D7,
+// as SynKernel::cycSynch, src/microphysics/kernels.h:414-437) is evaluated on the fly
+// and contracted against f(gamma) held in LDS; the SSC contraction
+// (computeSSCSpectrum, numeric_model.h:247-285) reads the precomputed
+// Blumenthal-Gould / Nava kernel table (SSCKernel::evalSSCkernel, kernels.h:49-117).
+#include "pba_device.h"
+#include <vector>
+
+namespace pbagpu {
+namespace ele {
+
+constexpr int MAXG = 512;      // max Lorentz-factor bins
+constexpr int MAXF = 256;      // max frequency bins
+constexpr int MPL = MAXG / 64; // bins per lane
+
+// SSCKernel::sscNava (kernels.h:95-117)
+__device__ double ssc_nava(double nu, double gam, double nu_t, double h_by_mec2) {
+    double en = nu * h_by_mec2, et = nu_t * h_by_mec2;
+    if (et / (4. * gam * gam) < en && en <= et) return (en / et) - (1. / (4. * gam * gam));
+    if (et < en && en < (4. * gam * gam * et / (1. + 4. * gam * et))) {
+        double q = en / (4. * gam * et * (gam - en));
+        double t = 4. * gam * et * q;
+        return 2. * q * log(q) + (1. + 2. * q) * (1. - q) + .5 * (1. - q) * (t * t) / (1. + t);
+    }
+    return 0.;
+}
+
+// CSYN kernel (kernels.h:414-437)
+__device__ __forceinline__ double cyc_synch(double B, double nu, double gam, double pref) {
+    if (gam > 2.) {
+        double x = nu / ((3. * kQe * B * gam * gam) / (4. * kPi * kMe * kC));
+        double x13 = cbrt(x);
+        double x23 = x13 * x13;
+        double x43 = x23 * x23;
+        double t1n = 1.808 * x13, t1d = sqrt(1 + 3.4 * x23);
+        double t2n = 1. + 2.21 * x23 + 0.347 * x43;
+        double t2d = 1. + 1.353 * x23 + 0.217 * x43;
+        return t1n / t1d * t2n / t2d * exp(-x) * pref;
+    }
+    double x = nu / ((3. * kQe * B) / (2. * kPi * kMe * kC));
+    double psq = gam * gam - 1.;
+    return (2. * psq) / (1. + 3. * psq) * exp((2. * (1. - x)) / (1. + 3. * psq)) * pref;
+}
+
+// Kernel table K[i][j][k] and its frequency integral KI[j][k] (kernels.h:49-94).
+__global__ void ssc_table_kernel(const double *__restrict__ nu, const double *__restrict__ e, int nf, int ng,
+                                 double h_by_mec2, double *__restrict__ K) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)nf * ng * nf;
+    if (idx >= n) return;
+    int k = (int)(idx % nf);
+    int j = (int)((idx / nf) % ng);
+    int i = (int)(idx / ((size_t)nf * ng));
+    K[idx] = ssc_nava(nu[i], e[j], nu[k], h_by_mec2);
+}
+
+__global__ void ssc_integral_kernel(const double *__restrict__ nu, const double *__restrict__ K, int nf, int ng,
+                                    double *__restrict__ KI) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ng * nf) return;
+    int j = idx / nf, k = idx % nf;
+    double res = 0;
+    for (int i = 0; i < nf - 1; i++) res += K[((size_t)i * ng + j) * nf + k] * nu[i + 1] * (nu[i + 1] - nu[i]);
+    KI[idx] = res;
+}
+
+struct PopDesc {
+    const double *T;      // dynamics table of the blastwave
+    const double *K;      // SSC kernel table or null
+    const double *KI;
+    const double *e;      // Lorentz-factor grid [ng]
+    const double *half_e; // [ng-1]
+    const double *dg;     // delta_grid [ng+1]
+    const double *dgbar;  // delta_grid_bar [ng]
+    const double *nu;     // frequency grid [nf]
+    double *jall;         // [nt][nf]
+    double *fall;         // [nt][ng] or null
+    double p, eps_e;
+    int shock, ng, nf, max_substeps, adi_loss, ssc;
+};
+
+__device__ __forceinline__ void power_law(const double *e, int j, double p, double gm, double gc, double gM,
+                                          double n_inj, double &N) {
+    double gamma = e[j];
+    if (gm > gc) {
+        double K_f = n_inj / (pow(gm, 1. - p) * (1. / gc - 1. / gm) + 1. / p * (pow(gm, -p) - pow(gM, -p)));
+        if ((gc <= gamma) && (gamma < gm)) N = K_f * pow(gm, 1. - p) * pow(gamma, -2.);
+        else if ((gm < gamma) && (gamma <= gM)) N = K_f * pow(gamma, -p - 1.);
+    } else {
+        double K_s = n_inj / ((1. / (1. - p)) * (pow(gc, 1. - p) - pow(gm, 1. - p)) - gc / p * (pow(gM, -p) - pow(gc, -p)));
+        if ((gm <= gamma) && (gamma < gc)) N = K_s * pow(gamma, -p);
+        else if ((gc <= gamma) && (gamma <= gM)) N = K_s * gc * pow(gamma, -p - 1.);
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(64) ele_kernel(const PopDesc *__restrict__ pops, int nt, int *__restrict__ status) {
+    __shared__ double s_f[MAXG];
+    __shared__ double s_tf[MAXF];   // photon density of the previous step (total_rad.f)
+    __shared__ double s_w[MAXF];    // tot_f[k] / nu[k] * (nu[k+1] - nu[k])
+    __shared__ double s_dge[MAXG];  // e[j+1] - e[j]
+    const PopDesc D = pops[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int ng = D.ng, nf = D.nf;
+    const int m = (ng + 63) / 64;
+    const int j0 = lane * m;
+    const double p = D.p;
+    const double pref = sqrt(3.) * pow(kQe, 3.) * (1.0 / kMec2);
+    const int bit = D.shock == 0 ? 1 : 2;
+    const int vB = D.shock ? PV_B3 : PV_B, vgm = D.shock ? PV_gm_rs : PV_gm, vgM = D.shock ? PV_gM_rs : PV_gM;
+    const int vgc = D.shock ? PV_gc_rs : PV_gc, vacc = D.shock ? PV_accel_frac_rs : PV_accel_frac;
+    const int vM = D.shock ? PV_M3 : PV_M2, vrho = D.shock ? PV_rho3 : PV_rho2;
+    const int vth = D.shock ? PV_thickness_rs : PV_thickness, vGsh = D.shock ? PV_GammaRsh : PV_GammaFsh;
+    auto V = [&](int v, int t) { return D.T[(size_t)v * nt + t]; };
+
+    double f[MPL], src[MPL], bb[MPL], cp[MPL];
+    for (int r = 0; r < MPL; r++) { f[r] = 0.; src[r] = 0.; bb[r] = 1.; cp[r] = 0.; }
+    for (int j = lane; j < ng; j += 64) s_dge[j] = (j < ng - 1) ? D.e[j + 1] - D.e[j] : 0.;
+    for (int k = lane; k < nf; k += 64) s_tf[k] = 0.;
+    __syncthreads();
+    bool init = false;
+    double dr_comov = 0., vol = 0.;
+    int st = 0;
+    for (int it = 0; it < nt; it++) {
+        int flags = (int)V(PV_mflags, it);
+        if (!(flags & bit)) continue;
+        const double B = V(vB, it), gm = V(vgm, it), gM = V(vgM, it), gc = V(vgc, it), acc = V(vacc, it);
+        if (!init) {
+            dr_comov = V(vth, it);
+            const double N_tot = V(vM, it) / kMp;
+            vol = V(vM, it) / V(vrho, it);
+            for (int r = 0; r < m; r++) {
+                int j = j0 + r;
+                if (j < ng) {
+                    power_law(D.e, j, p, gm, gc, gM, N_tot, f[r]);
+                    if (j < ng - 1) f[r] *= acc;
+                }
+            }
+            init = true;
+        } else {
+            const double tc = V(PV_tcomov, it - 1), tc1 = V(PV_tcomov, it);
+            const double mm = V(vM, it - 1), m1 = V(vM, it), rp = V(vrho, it - 1), rp1 = V(vrho, it);
+            if ((B <= 0) || (gm < 1.) || (gM <= 1.) || (gm >= gM) || (acc > 1.)) st |= PBAGPU_S_BAD_MPHYS;
+            dr_comov = V(vth, it - 1) * V(vGsh, it - 1);
+            const double N_tot = m1 / kMp;
+            const double dt = tc1 - tc;
+            const double volume = mm / rp, volume_p1 = m1 / rp1;
+            const double dlnVdt = D.adi_loss ? 1. / dt * (1. - volume / volume_p1) : 0.;
+            if (!isfinite(dlnVdt) || dlnVdt < 0) st |= PBAGPU_S_NONFINITE;
+            const double dlog_gam = (D.e[1] - D.e[0]) / D.e[0];
+            const double dts = kSigmaT * gM * B * B / (6. * kPi * kMe * kC);
+            const double dta = (gM * gM - 1.) / (3. * gM * gM) * dlnVdt;
+            double delta_t = dlog_gam / (dts + dta);
+            long n_sub;
+            if (delta_t >= dt / 1.) { delta_t = dt / 1.; n_sub = 1; }
+            else if (delta_t < dt / (double)D.max_substeps) { delta_t = dt / (double)D.max_substeps; n_sub = D.max_substeps; }
+            else n_sub = (long)(dt / delta_t);
+            vol = volume;
+            if (gm <= 2) {
+                for (int r = 0; r < m; r++) {
+                    int j = j0 + r;
+                    double t = 0.;
+                    if (j < ng) power_law(D.e, j, p, gm, gc, gM, N_tot, t);
+                    f[r] = t;
+                }
+            } else {
+                const double Nsrc = (m1 - mm) / kMp / dt;
+                const double index = -p;
+                const double norm = (1 + index) / (pow(gM, index + 1) - pow(gm, index + 1));
+                // photon weights for the SSC cooling integral
+                if (D.ssc) {
+                    for (int k = lane; k < nf; k += 64)
+                        s_w[k] = (k < nf - 1) ? s_tf[k] / D.nu[k] * (D.nu[k + 1] - D.nu[k]) : 0.;
+                    __syncthreads();
+                }
+                const double cf = kSigmaT / (6. * kPi * kMe * kC);
+                const double csc = 3. / 4. * kH * kSigmaT / kMe / kC;
+                // heating terms of the lane's bins and of the bin below the lane's first bin
+                double heat[MPL + 1];
+                for (int r = -1; r < m; r++) {
+                    int j = j0 + r;
+                    double hv = 0.;
+                    if (j >= 0 && j < ng - 1) {
+                        double ge = D.half_e[j];
+                        double gssc = 0.;
+                        if (D.ssc) {
+                            double res = 0.;
+                            const double *KIr = D.KI + (size_t)j * nf;
+                            for (int k = 0; k < nf - 1; k++) res += s_w[k] * KIr[k];
+                            gssc = csc * res / ge / ge;
+                        }
+                        hv = cf * B * B * ge * ge + dlnVdt * (ge * ge - 1.) / (3. * ge) + gssc;
+                    }
+                    heat[r + 1] = hv;
+                }
+                for (int r = 0; r < m; r++) {
+                    int j = j0 + r;
+                    if (j >= ng) { bb[r] = 1.; cp[r] = 0.; src[r] = 0.; continue; }
+                    double ej = D.e[j];
+                    src[r] = ((ej > gm) && (ej <= gM)) ? Nsrc * norm * pow(ej, index) : 0.;
+                    double odgb = 1.0 / D.dgbar[j];
+                    double braw, craw;
+                    if (j == 0) { braw = -0.; craw = odgb * heat[r + 1]; }
+                    else if (j == ng - 1) { braw = -0.; craw = 0.; }
+                    else { braw = -odgb * heat[r]; craw = odgb * heat[r + 1]; }
+                    double b_ = (1. - braw * delta_t) + 0. * delta_t;
+                    double c_ = craw * -delta_t;
+                    bb[r] = b_;
+                    cp[r] = c_ / b_;
+                }
+                for (long s = 0; s < n_sub; s++) {
+                    // lane map: f_first = A + Bm * f_next
+                    double dp[MPL];
+                    double A = 0., Bm = 1.;
+                    for (int r = MPL - 1; r >= 0; r--) {
+                        if (r >= m) continue;
+                        int j = j0 + r;
+                        if (j >= ng) continue;
+                        dp[r] = (f[r] + src[r] * delta_t) / bb[r];
+                        A = dp[r] - cp[r] * A;
+                        Bm = -cp[r] * Bm;
+                    }
+                    // inclusive suffix scan of affine maps over the wavefront
+                    for (int d = 1; d < 64; d <<= 1) {
+                        double A2 = __shfl_down(A, d, 64), B2 = __shfl_down(Bm, d, 64);
+                        if (lane + d < 64) { A = A + Bm * A2; Bm = Bm * B2; }
+                    }
+                    double Fnext = __shfl_down(A, 1, 64);
+                    if (lane == 63) Fnext = 0.;
+                    for (int r = MPL - 1; r >= 0; r--) {
+                        if (r >= m) continue;
+                        int j = j0 + r;
+                        if (j >= ng) continue;
+                        Fnext = dp[r] - cp[r] * Fnext;
+                        f[r] = Fnext;
+                    }
+                }
+            }
+        }
+        // --- emission ---
+        double ne_part = 0.;
+        for (int r = 0; r < m; r++) {
+            int j = j0 + r;
+            if (j < ng) {
+                s_f[j] = f[r];
+                if (j < ng - 1) ne_part += f[r] * s_dge[j];
+            }
+        }
+        if (D.ssc)
+            for (int k = lane; k < nf; k += 64)
+                s_w[k] = (k < nf - 1) ? s_tf[k] / D.nu[k] * (D.nu[k + 1] - D.nu[k]) : 0.;
+        __syncthreads();
+        const double n_ele = wave_sum(ne_part);
+        if (n_ele <= 0) st |= PBAGPU_S_NONFINITE;
+        const double fac = (dr_comov / kC) / vol;
+        double tfn[MAXF / 64];
+        for (int q = 0; q < (nf + 63) / 64; q++) {
+            int i = lane + 64 * q;
+            tfn[q] = 0.;
+            if (i >= nf) continue;
+            const double nui = D.nu[i];
+            double synj = 0.;
+            if (i < nf - 1) {
+                for (int j = 0; j < ng - 1; j++) synj += cyc_synch(B, nui, D.e[j], pref) * s_f[j] * s_dge[j];
+                if (!isfinite(synj) || synj < 0.) st |= PBAGPU_S_NONFINITE;
+            }
+            double sscj = 0.;
+            if (D.ssc) {
+                const double c0 = 3. / 4. * kSigmaT * kC;
+                for (int j = 0; j < ng - 1; j++) {
+                    const double *Kr = D.K + ((size_t)i * ng + j) * nf;
+                    double s = 0.;
+                    for (int k = 0; k < nf - 1; k++) s += s_w[k] * Kr[k];
+                    double ej = D.e[j];
+                    sscj += s_f[j] * kH * nui * (c0 / (ej * ej) * s) * s_dge[j];
+                }
+            }
+            tfn[q] = synj * fac / (kH * nui) + (D.ssc ? sscj * fac / (kH * nui) : 0.);
+            double jt = (synj + sscj) * acc / n_ele;
+            if (!isfinite(jt) || jt < 0.) st |= PBAGPU_S_NONFINITE;
+            D.jall[(size_t)it * nf + i] = jt;
+        }
+        __syncthreads();
+        for (int q = 0; q < (nf + 63) / 64; q++) {
+            int i = lane + 64 * q;
+            if (i < nf) s_tf[i] = tfn[q];
+        }
+        if (D.fall)
+            for (int r = 0; r < m; r++) {
+                int j = j0 + r;
+                if (j < ng) D.fall[(size_t)it * ng + j] = f[r];
+            }
+        __syncthreads();
+    }
+    for (int d = 32; d >= 1; d >>= 1) st |= __shfl_xor(st, d, 64);
+    if (lane == 0) status[blockIdx.x] = st;
+}
+
+struct GridSet {
+    std::vector<double> e, half_e, dg, dgbar, nu;
+};
+
+static void build_grids(const pba_shock_params &s, GridSet &G) {
+    int n = s.ngam;
+    G.e.assign(n, 0.); G.half_e.assign(n - 1, 0.); G.dg.assign(n + 1, 0.); G.dgbar.assign(n, 0.);
+    double step = std::exp(std::log(s.gam2) / (double)n);
+    for (int i = 0; i < n; i++) {
+        G.e[i] = std::pow(step, (double)i);
+        if (i < n - 1) G.half_e[i] = .5 * G.e[i] * (step + 1.0);
+    }
+    for (int i = 0; i < n - 1; i++) G.dg[i + 1] = G.e[i + 1] - G.e[i];
+    G.dg[0] = G.e[0] * (1 - 1.0 / step);
+    G.dg[n] = G.e[n - 1] * (step - 1);
+    for (int i = 0; i < n; i++) G.dgbar[i] = (G.dg[i] + G.dg[i + 1]) / 2.;
+    int nf = s.nfreq;
+    G.nu.assign(nf, 0.);
+    double cur = std::pow(10.0, std::log10(s.freq1));
+    double ratio = std::pow(10.0, (std::log10(s.freq2) - std::log10(s.freq1)) / nf);
+    for (int i = 0; i < nf; i++) { G.nu[i] = cur; cur *= ratio; }
+}
+
+}  // namespace ele
+}  // namespace pbagpu
+
+extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const pba_population *pops, int n_pop,
+                                 const double *d_table, int bw_total, int nt, double *d_jall, double *d_fall,
+                                 int *d_status, int precision, void *stream) {
+    using namespace pbagpu::ele;
+    if (!params || !pops || !d_table || !d_jall || !d_status || n_pop < 1 || nt < 1) return PBAGPU_E_ARG;
+    if (precision != 0) return PBAGPU_E_UNSUPPORTED;
+    hipStream_t s = (hipStream_t)stream;
+    const pba_shock_params &s0 = pops[0].shock ? params[pops[0].run].rs : params[pops[0].run].fs;
+    const int ng = s0.ngam, nf = s0.nfreq;
+    if (ng < 2 || ng > MAXG || nf < 2 || nf > MAXF) return PBAGPU_E_ARG;
+    const double h_by_mec2 = pbagpu::kH / pbagpu::kMe / pbagpu::kC / pbagpu::kC;
+    // one grid set (and SSC table if needed) per (run, shock)
+    std::vector<int> key_of(n_pop);
+    std::vector<int> keys;
+    for (int q = 0; q < n_pop; q++) {
+        const pba_population &P = pops[q];
+        if (P.run < 0 || P.run >= n_runs || P.bw < 0 || P.bw >= bw_total || (P.shock != 0 && P.shock != 1))
+            return PBAGPU_E_ARG;
+        const pba_shock_params &sp = P.shock ? params[P.run].rs : params[P.run].fs;
+        if (sp.ngam != ng || sp.nfreq != nf) return PBAGPU_E_ARG;
+        int key = P.run * 2 + P.shock;
+        size_t pos = 0;
+        while (pos < keys.size() && keys[pos] != key) pos++;
+        if (pos == keys.size()) keys.push_back(key);
+        key_of[q] = (int)pos;
+    }
+    const size_t ktab = (size_t)nf * ng * nf;
+    const size_t per_key = (size_t)ng + (ng - 1) + (ng + 1) + ng + nf;
+    std::vector<double> hgrids(per_key * keys.size());
+    std::vector<char> need_ssc(keys.size(), 0);
+    for (size_t kk = 0; kk < keys.size(); kk++) {
+        const pba_shock_params &sp = (keys[kk] & 1) ? params[keys[kk] / 2].rs : params[keys[kk] / 2].fs;
+        GridSet G;
+        build_grids(sp, G);
+        double *o = &hgrids[per_key * kk];
+        std::copy(G.e.begin(), G.e.end(), o); o += ng;
+        std::copy(G.half_e.begin(), G.half_e.end(), o); o += ng - 1;
+        std::copy(G.dg.begin(), G.dg.end(), o); o += ng + 1;
+        std::copy(G.dgbar.begin(), G.dgbar.end(), o); o += ng;
+        std::copy(G.nu.begin(), G.nu.end(), o);
+        need_ssc[kk] = sp.ssc ? 1 : 0;
+    }
+    double *dgrids = nullptr, *dK = nullptr;
+    PopDesc *dpops = nullptr;
+    size_t n_ssc = 0;
+    std::vector<size_t> ssc_slot(keys.size(), 0);
+    for (size_t kk = 0; kk < keys.size(); kk++) if (need_ssc[kk]) ssc_slot[kk] = n_ssc++;
+    int rc = PBAGPU_OK;
+    if (hipMalloc(&dgrids, sizeof(double) * hgrids.size()) != hipSuccess) return PBAGPU_E_HIP;
+    if (n_ssc && hipMalloc(&dK, sizeof(double) * (ktab + (size_t)ng * nf) * n_ssc) != hipSuccess) {
+        hipFree(dgrids);
+        return PBAGPU_E_HIP;
+    }
+    if (hipMalloc(&dpops, sizeof(PopDesc) * n_pop) != hipSuccess) { hipFree(dgrids); hipFree(dK); return PBAGPU_E_HIP; }
+    hipMemcpyAsync(dgrids, hgrids.data(), sizeof(double) * hgrids.size(), hipMemcpyHostToDevice, s);
+    for (size_t kk = 0; kk < keys.size(); kk++) {
+        if (!need_ssc[kk]) continue;
+        double *base = dgrids + per_key * kk;
+        double *K = dK + (ktab + (size_t)ng * nf) * ssc_slot[kk];
+        double *KI = K + ktab;
+        const double *e = base, *nu = base + ng + (ng - 1) + (ng + 1) + ng;
+        hipLaunchKernelGGL(ssc_table_kernel, dim3((unsigned)((ktab + 255) / 256)), dim3(256), 0, s, nu, e, nf, ng,
+                           h_by_mec2, K);
+        hipLaunchKernelGGL(ssc_integral_kernel, dim3((unsigned)((ng * nf + 255) / 256)), dim3(256), 0, s, nu, K, nf, ng,
+                           KI);
+    }
+    std::vector<PopDesc> hp(n_pop);
+    for (int q = 0; q < n_pop; q++) {
+        const pba_population &P = pops[q];
+        const pba_shock_params &sp = P.shock ? params[P.run].rs : params[P.run].fs;
+        int kk = key_of[q];
+        double *base = dgrids + per_key * kk;
+        PopDesc &D = hp[q];
+        D.T = d_table + (size_t)P.bw * PBA_NV * nt;
+        D.e = base; D.half_e = base + ng; D.dg = base + ng + (ng - 1); D.dgbar = D.dg + (ng + 1); D.nu = D.dgbar + ng;
+        D.K = need_ssc[kk] ? dK + (ktab + (size_t)ng * nf) * ssc_slot[kk] : nullptr;
+        D.KI = D.K ? D.K + ktab : nullptr;
+        D.jall = d_jall + (size_t)q * nt * nf;
+        D.fall = d_fall ? d_fall + (size_t)q * nt * ng : nullptr;
+        D.p = sp.p; D.eps_e = sp.eps_e; D.shock = P.shock; D.ng = ng; D.nf = nf;
+        D.max_substeps = sp.max_substeps; D.adi_loss = sp.adi_loss_ele; D.ssc = need_ssc[kk];
+    }
+    hipMemcpyAsync(dpops, hp.data(), sizeof(PopDesc) * n_pop, hipMemcpyHostToDevice, s);
+    hipMemsetAsync(d_jall, 0, sizeof(double) * (size_t)n_pop * nt * nf, s);
+    if (d_fall) hipMemsetAsync(d_fall, 0, sizeof(double) * (size_t)n_pop * nt * ng, s);
+    hipLaunchKernelGGL(ele_kernel, dim3(n_pop), dim3(64), 0, s, dpops, nt, d_status);
+    if (hipGetLastError() != hipSuccess) rc = PBAGPU_E_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) rc = PBAGPU_E_HIP;
+    hipFree(dgrids); hipFree(dK); hipFree(dpops);
+    return rc;
+}

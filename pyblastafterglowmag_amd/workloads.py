@@ -63,3 +63,16 @@ def scaled(name, ntb=None, ngam=None, nfreq=None, n_nu_obs=None, n_t_obs=None, n
         else:
             grb[k] = v
     return main, grb, struct
+
+
+SMALL = {
+    # reduced grids of the two workloads above, small enough for the CPU oracle in tests
+    "tophat_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8),
+    "gauss5_small": dict(base="gauss21_fsrs_ssc", ntb=1000, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8, n_layers=5),
+}
+
+
+def small(name):
+    cfg = dict(SMALL[name])
+    base = cfg.pop("base")
+    return scaled(base, **cfg)

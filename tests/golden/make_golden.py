@@ -1,0 +1,100 @@
+"""Write the golden fixtures of the parity tests from the reference CPU program.
+
+Runs oracle/_ref/pba.out (built from /root/reference/src by oracle/build_ref.sh) on the
+parfile + initial-data files written by pyblastafterglowmag_amd.inputs, then stores the
+reference's outputs (lc.h5 light curves, dyn.h5 dynamics, spec.h5 comoving spectra) as
+compressed .npz next to this script.  Only data is stored (inputs and outputs).
+
+    python tests/golden/make_golden.py [name ...]
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from pyblastafterglowmag_amd import abi, h5io, inputs, workloads  # noqa: E402
+
+PBA = os.path.join(ROOT, "oracle", "_ref", "pba.out")
+DYN_VARS = ["R", "Gamma", "beta", "theta", "M2", "Eint2", "tt", "tcomov", "rho2", "GammaFsh", "thickness", "U_p",
+            "gamma_min", "gamma_max", "gamma_c", "B", "accel_frac", "Rsh"]
+DYN_VARS_RS = ["M3", "Eint3", "deltaR4", "Gamma43", "rho3", "GammaRsh", "thichness_rs", "U_p3", "gamma_min_rs",
+               "gamma_max_rs", "gamma_c_rs", "B_rs", "accel_frac_rs"]
+
+
+def build_case(name):
+    if name in workloads.SMALL:
+        return workloads.small(name)
+    return workloads.WORKLOADS[name]()
+
+
+def run_reference(name, wd, spec_every=1):
+    main, grb, struct = build_case(name)
+    grb = dict(grb, save_dynamics="yes", save_spec="yes")
+    inputs.write_run(wd, main, grb, struct)
+    subprocess.check_call([PBA, wd + "/", "parfile.par", "err"], stdout=subprocess.DEVNULL)
+    P, mp_, mo, gp, go = abi.run_params_from_parfile(os.path.join(wd, "parfile.par"))
+    idf = os.path.join(wd, go["fname_ejecta_id"])
+    out = {}
+    with open(os.path.join(wd, "parfile.par")) as f:
+        out["parfile"] = np.frombuffer(f.read().encode(), dtype=np.uint8)
+    for k in ("mom", "ctheta", "ek", "mass"):
+        out["id_" + k] = h5io.read(idf, k)
+    out["id_theta_wing"] = np.array(h5io.attr(idf, "theta_wing"))
+    out["id_theta_core"] = np.array(h5io.attr(idf, "theta_core"))
+    nl = len(out["id_mom"])
+    lc = os.path.join(wd, "lc.h5")
+    out["times"] = h5io.read(lc, "times")
+    out["freqs"] = h5io.read(lc, "freqs")
+    out["lc"] = np.stack([h5io.read(lc, f"shell=0 layer={il}/fluxdens") for il in range(nl)])
+    dyn = os.path.join(wd, "dyn.h5")
+    names = DYN_VARS + (DYN_VARS_RS if P.fsrs else [])
+    nt = int(mp_["ntb"])
+    D = np.zeros((nl, len(names), nt))
+    for il in range(nl):
+        for iv, v in enumerate(names):
+            ds = f"shell=0 layer={il}/{v}"
+            if h5io.has(dyn, ds):
+                D[il, iv] = h5io.read(dyn, ds)
+    out["dyn"] = D
+    out["dyn_names"] = np.array(names)
+    spec = os.path.join(wd, "spec.h5")
+    for tag in ("fs", "rs"):
+        J, F = [], []
+        for il in range(nl):
+            ds = f"shell=0 layer={il}/total_j_{tag}"
+            if h5io.has(spec, ds):
+                J.append(h5io.read(spec, ds).reshape(nt, -1)[::spec_every])
+                F.append(h5io.read(spec, f"shell=0 layer={il}/n_ele_{tag}").reshape(nt, -1)[::spec_every])
+            else:
+                J.append(None)
+        if any(j is not None for j in J):
+            ok = [i for i, j in enumerate(J) if j is not None]
+            out[f"spec_layers_{tag}"] = np.array(ok)
+            out[f"jall_{tag}"] = np.stack([J[i] for i in ok])
+            out[f"fall_{tag}"] = np.stack(F)
+    out["spec_every"] = np.array(spec_every)
+    return out
+
+
+def main(names):
+    if not os.path.exists(PBA):
+        raise SystemExit(f"reference program missing: {PBA} (run oracle/build_ref.sh)")
+    for name in names:
+        wd = tempfile.mkdtemp(prefix=f"golden_{name}_")
+        try:
+            every = {"tophat_small": 1, "gauss5_small": 5}.get(name, 10)
+            out = run_reference(name, wd, spec_every=every)
+            np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+            print(name, {k: v.shape for k, v in out.items()})
+        finally:
+            shutil.rmtree(wd, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or ["tophat_small", "gauss5_small", "tophat_fs_syn"])

@@ -1,0 +1,71 @@
+"""CPU oracle against the reference program's own outputs (golden fixtures written by
+tests/golden/make_golden.py from oracle/_ref/pba.out).
+
+On-axis forward-shock runs are reproduced bit for bit (dynamics, electron spectra,
+emissivities, light curves).  With the reverse shock, the oracle is bit-exact until the
+shared adaptive step first differs by an ulp, and stays within the tolerances below after.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden_run, load_golden, rel_err
+import pyoracle
+from pyblastafterglowmag_amd import abi
+
+
+def _populations(run, res):
+    pops, lays = [], []
+    for il in range(run.n_layers):
+        pfs = len(pops)
+        pops.append(abi.Population(il, 0, 0, 0))
+        prs = -1
+        if res[il].fsrs and run.params.do_rs_radiation:
+            prs = len(pops)
+            pops.append(abi.Population(il, 0, 1, 0))
+        li = run.inits[il]
+        lays.append(abi.EatsLayer(li.theta_c_l, li.theta_c_h, il, 0, pfs, prs, res[il].i_end_r))
+    return (abi.Population * len(pops))(*pops), (abi.EatsLayer * len(lays))(*lays)
+
+
+def _oracle_all(g):
+    run = golden_run(g)
+    table, res = pyoracle.dyn_evolve(run.params, run.inits, run.tgrid)
+    pops, lays = _populations(run, res)
+    jall, fall, st = pyoracle.ele_evolve(run.params, pops, table, want_f=True)
+    lc, st2 = pyoracle.eats_lightcurve(run.params, lays, table, jall, run.tobs, run.nuobs)
+    assert all(r.status == 0 for r in res) and not st.any() and not st2.any()
+    return run, table, res, pops, jall, fall, lc
+
+
+@pytest.mark.parametrize("name,tol_dyn,tol_j,tol_lc", [
+    ("tophat_small", 0.0, 0.0, 0.0),
+    ("gauss5_small", 1e-4, 1e-5, 1e-8),
+])
+def test_oracle_matches_reference(name, tol_dyn, tol_j, tol_lc):
+    g = load_golden(name)
+    run, table, res, pops, jall, fall, lc = _oracle_all(g)
+    # observer pairs are the reference's own
+    np.testing.assert_array_equal(run.tobs, g["times"])
+    np.testing.assert_array_equal(run.nuobs, g["freqs"])
+    names = [str(n) for n in g["dyn_names"]]
+    for il in range(run.n_layers):
+        for iv, n in enumerate(names):
+            ref = g["dyn"][il, iv]
+            mine = table[il, abi.PV[n]]
+            assert rel_err(mine, ref, 1e-300) <= tol_dyn, (name, il, n)
+    every = int(g["spec_every"])
+    for q, P in enumerate(pops):
+        tag = "fs" if P.shock == 0 else "rs"
+        layers = list(g[f"spec_layers_{tag}"])
+        k = layers.index(P.bw)
+        assert rel_err(jall[q][::every], g[f"jall_{tag}"][k], 1e-12) <= tol_j, (name, q)
+        assert rel_err(fall[q][::every], g[f"fall_{tag}"][k], 1e-12) <= tol_j, (name, q)
+    for il in range(run.n_layers):
+        assert rel_err(lc[il], g["lc"][il], 1e-30) <= tol_lc, (name, il)
+
+
+def test_oracle_full_tophat_bit_exact():
+    """Full-size single top-hat run (1000 steps, 200 gamma bins, 96 x 128 pairs)."""
+    g = load_golden("tophat_fs_syn")
+    run, table, res, pops, jall, fall, lc = _oracle_all(g)
+    np.testing.assert_array_equal(lc[0], g["lc"][0])
