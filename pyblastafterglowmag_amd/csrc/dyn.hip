@@ -24,6 +24,10 @@ namespace dyn {
 enum QS { iR, iRsh, itt, itcomov, iGamma, iEint2, iEint3, itheta, iRrsh, iErad2, iErad3, iEsh2, iEsh3,
           iEad2, iEad3, iM2, iM3, ideltaR4, NA };
 constexpr int NEQ_REF = 23;  // components per blastwave in the reference state vector
+// Accepted+rejected DOP853 steps allowed inside one output interval before the run is flagged
+// PBAGPU_S_NMAX.  The stiff start of a reverse-shock run (h ~ 1e-10 s) takes ~2.4e5 steps in the
+// first interval of a 5-layer Gaussian jet, so the budget sits well above that.
+constexpr int kMaxStepsPerInterval = 10000000;
 
 struct MState { double acc, tcomov; };
 
@@ -571,7 +575,7 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
             h = fmin(fmin(100 * h1, h2), hmax);
         }
         bool reject = false, last = false;
-        for (int guard = 0; guard < 100000; guard++) {   // step budget per output interval
+        for (int guard = 0; guard < kMaxStepsPerInterval; guard++) {
             if ((time + 1.01 * h - timeN) > 0.0) { h = timeN - time; last = true; }
             // stages (ode_solvers.h:1434-1479); k3 and k5 share k2's storage, kC shares k4's
             double *k3 = k2, *k5 = k2, *kC = k4;
@@ -642,7 +646,7 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
                 reject = false;
                 h = hnew;
             }
-            if (guard == 100000 - 1) b.status |= PBAGPU_S_NMAX;
+            if (guard == kMaxStepsPerInterval - 1) b.status |= PBAGPU_S_NMAX;
         }
         if (active && !b.end_evolution) {
             double g = Y[iGamma], be = beta_from_gamma(g), E2 = Y[iEint2];

@@ -42,7 +42,7 @@ __device__ double ssc_nava(double nu, double gam, double nu_t, double h_by_mec2)
     return 0.;
 }
 
-// CSYN kernel (kernels.h:414-437)
+// CSYN kernel (kernels.h:414-437); pref = sqrt(3) qe^3 / (me c^2)
 __device__ __forceinline__ double cyc_synch(double B, double nu, double gam, double pref) {
     if (gam > 2.) {
         double x = nu / ((3. * kQe * B * gam * gam) / (4. * kPi * kMe * kC));
@@ -52,11 +52,11 @@ __device__ __forceinline__ double cyc_synch(double B, double nu, double gam, dou
         double t1n = 1.808 * x13, t1d = sqrt(1 + 3.4 * x23);
         double t2n = 1. + 2.21 * x23 + 0.347 * x43;
         double t2d = 1. + 1.353 * x23 + 0.217 * x43;
-        return t1n / t1d * t2n / t2d * exp(-x) * pref;
+        return t1n / t1d * t2n / t2d * exp(-x) * (pref * B);
     }
     double x = nu / ((3. * kQe * B) / (2. * kPi * kMe * kC));
     double psq = gam * gam - 1.;
-    return (2. * psq) / (1. + 3. * psq) * exp((2. * (1. - x)) / (1. + 3. * psq)) * pref;
+    return (2. * psq) / (1. + 3. * psq) * exp((2. * (1. - x)) / (1. + 3. * psq)) * (pref * B);
 }
 
 // Kernel table K[i][j][k] and its frequency integral KI[j][k] (kernels.h:49-94).
