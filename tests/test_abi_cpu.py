@@ -1,0 +1,139 @@
+"""Boundary checks that need no GPU: the C-ABI library loads and exports every function that
+include/pbagpu.h declares, the ctypes mirror has the header's exact struct layout, and the
+parfile reader fills the structures the way the reference's readers do."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from pyblastafterglowmag_amd import abi, inputs, workloads
+
+HEADER = os.path.join(ROOT, "include", "pbagpu.h")
+LIB = os.path.join(ROOT, "pyblastafterglowmag_amd", "_lib", "libpbagpu.so")
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(pbagpu_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_entry_points():
+    names = _declared_functions()
+    assert "pbagpu_dyn_evolve" in names and "pbagpu_ele_evolve" in names and "pbagpu_eats_lightcurve" in names
+    assert len(names) >= 6
+
+
+def test_library_exports_every_declared_function():
+    if not os.path.exists(LIB):
+        pytest.skip("libpbagpu.so not built")
+    lib = C.CDLL(LIB)
+    missing = [n for n in _declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\sT\s(pbagpu_\w+)", out))
+    assert set(_declared_functions()) <= exported
+
+
+def test_version_needs_no_device():
+    if not os.path.exists(LIB):
+        pytest.skip("libpbagpu.so not built")
+    lib = C.CDLL(LIB)
+    assert lib.pbagpu_version() > 0
+
+
+_STRUCTS = {
+    "pba_shock_params": abi.ShockParams,
+    "pba_run_params": abi.RunParams,
+    "pba_layer_init": abi.LayerInit,
+    "pba_bw_result": abi.BwResult,
+    "pba_population": abi.Population,
+    "pba_eats_layer": abi.EatsLayer,
+}
+
+
+def test_struct_layout_matches_header(tmp_path):
+    """sizeof/offsetof of every boundary struct, compiled from the header by the host C compiler."""
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
+    for cname, py in _STRUCTS.items():
+        lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append(f'printf("PBA_NV %d\\n", (int)PBA_NV);')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln.strip():
+            parts = ln.split()
+            got[tuple(parts[:-1])] = int(parts[-1])
+    assert got[("PBA_NV",)] == abi.PBA_NV
+    for cname, py in _STRUCTS.items():
+        assert got[(cname, "sizeof")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_table_variable_order_matches_header():
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"enum pba_var \{(.*?)\};", text, flags=re.S).group(1)
+    names = [t.strip().split("=")[0].strip() for t in body.split(",") if t.strip()]
+    assert names[-1] == "PBA_NV"
+    assert len(names) - 1 == abi.PBA_NV
+
+
+def test_parfile_round_trip(tmp_path):
+    """Workload -> parfile text -> reader -> RunParams; the reference reads the same keys
+    (src/main.cpp:93-154, src/blastwave/blastwave_base.h:95-456)."""
+    main, grb, struct = workloads.gauss21_fsrs_ssc()
+    par = inputs.write_parfile(str(tmp_path), main, grb)
+    P, mp_, mo, gp, go = abi.run_params_from_parfile(par)
+    assert P.fsrs == 1 and P.do_rs == 1 and P.do_rs_radiation == 1
+    assert P.n_ism == pytest.approx(10 ** (-1.6), rel=1e-15)
+    assert P.theta_obs == 0.363
+    assert P.fs.ssc == 1 and P.rs.ssc == 1
+    assert P.fs.ngam == 200 and P.fs.nfreq == 96
+    assert P.fs.p == 2.1 and P.rs.p == 2.1
+    t = abi.tgrid(mp_)
+    # MakeLogspaceVec multiplies by (tb1/tb0)^(1/ntb): the grid stops one ratio short of tb1
+    ratio = (1e14 / 3e2) ** (1.0 / 1000)
+    assert len(t) == 1000 and t[0] == pytest.approx(3e2) and t[-1] == pytest.approx(1e14 / ratio, rel=1e-11)
+    assert np.allclose(t[1:] / t[:-1], ratio, rtol=1e-12)
+    tobs, nuobs = abi.obs_pairs(mo)
+    assert len(tobs) == 96 * 128
+    # time-major pairs: the first 96 pairs share the first observer time
+    assert np.all(tobs[:96] == tobs[0]) and len(np.unique(nuobs[:96])) == 96
+
+
+def test_unsupported_options_raise(tmp_path):
+    main, grb, struct = workloads.tophat_fs_syn()
+    grb = dict(grb, method_eats="piece-wise")
+    par = inputs.write_parfile(str(tmp_path), main, grb)
+    with pytest.raises(NotImplementedError):
+        abi.run_params_from_parfile(par)
+
+
+def test_layer_edges_piecewise():
+    """Layer edges are 2 asin(i/n sin(theta_w/2)) (src/ejecta/ejecta_id.h:142-182)."""
+    main, grb, struct = workloads.gauss21_fsrs_ssc()
+    idd = inputs.jet_structure(kind="adaptive", **struct)
+    L = abi.layer_inits(idd)
+    n = len(L)
+    tw = float(idd["theta_wing"])
+    for i in range(n):
+        assert L[i].theta_c_l == pytest.approx(2 * np.arcsin(i / n * np.sin(tw / 2)), rel=1e-15, abs=0)
+        assert L[i].theta_c_h > L[i].theta_c_l
+    # the layers tile the cap: sum of (cos l - cos h) = 1 - cos(theta_w)
+    s = sum(np.cos(L[i].theta_c_l) - np.cos(L[i].theta_c_h) for i in range(n))
+    assert s == pytest.approx(1 - np.cos(tw), rel=1e-12)
+    # cos(2 asin a) = 1 - 2 a^2: layer i spans (2i+1) equal solid-angle cells of the piece-wise grid
+    d = np.array([np.cos(L[i].theta_c_l) - np.cos(L[i].theta_c_h) for i in range(n)])
+    assert np.allclose(d / (2 * np.arange(n) + 1), d[0], rtol=1e-9)

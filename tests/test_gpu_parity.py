@@ -2,8 +2,11 @@
 
 Tolerances: the GPU evaluates the same FP64 algorithm, but with fused multiply-adds, the
 device math library and FP64 in place of x87 long double; ulp-level differences move the
-shared adaptive ODE step, which bounds agreement of the dynamics by the ODE tolerance
-(rtol = 1e-8) and of the light curves by 1e-6 relative (the FP64-path target).
+shared adaptive ODE step.  Forward-shock-only dynamics agree to 2e-6; with the reverse
+shock the step sequences part after the first ulp difference and the tables agree to 1e-4,
+the same bound the oracle keeps against the reference program.  Electron and observer
+stages are compared on identical inputs (the GPU's own tables): 1e-9 and 1e-10.  Light
+curves end to end against the reference program: 1e-6 relative (the FP64-path target).
 """
 import numpy as np
 import pytest
@@ -37,9 +40,10 @@ def test_stages_match_oracle(name):
     for il in range(run.n_layers):
         assert ex["results"][il][1] == res_o[il].i_end_r
         assert ex["results"][il][2] == res_o[il].fsrs
+        tol = 1e-4 if res_o[il].fsrs else 2e-6
         for n in DYN_CHECK:
             e = rel_err(ex["table"][il, abi.PV[n]], table_o[il, abi.PV[n]], 1e-300)
-            assert e <= 2e-6, (name, il, n, e)
+            assert e <= tol, (name, il, n, e)
     # electron stage on the GPU's own dynamics table vs oracle on the same table
     pops = (abi.Population * len(ex["pops"]))(*ex["pops"])
     jall_o, _, st = pyoracle.ele_evolve(run.params, pops, ex["table"])

@@ -1,0 +1,177 @@
+"""The CPU oracle against what the mathematics fixes, independent of the reference's outputs:
+closed-form integrals for the CADRE quadrature, the exact angle-averaged synchrotron
+function for the CSYN fit, the Thomson limit of the Compton kernel, and the coasting and
+Blandford-McKee limits of the blastwave equations."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, special
+
+from conftest import golden_run, load_golden
+import pyoracle
+from pyblastafterglowmag_amd import abi
+
+QE, ME, CL, MEC2 = 4.803204e-10, 9.1094e-28, 2.99792458e10, 8.187105649650028e-07
+H = 6.6260755e-27
+
+
+def _lib():
+    L = pyoracle.lib()
+    L.orc_cadre_test.restype = C.c_double
+    L.orc_cadre_test.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                                 C.POINTER(C.c_long), C.POINTER(C.c_long)]
+    L.orc_cyc_synch.restype = C.c_double
+    L.orc_cyc_synch.argtypes = [C.c_double] * 3
+    L.orc_ssc_nava.restype = C.c_double
+    L.orc_ssc_nava.argtypes = [C.c_double] * 3
+    return L
+
+
+@pytest.mark.parametrize("which,a,b,exact", [
+    (0, 0.0, math.pi, 2.0),
+    (1, -3.0, 3.0, math.sqrt(math.pi) * math.erf(3.0)),
+    (2, -1.0, 1.0, 2.0 / math.sqrt(1e-3) * math.atan(1.0 / math.sqrt(1e-3))),
+    (3, -1.0, 1.0, 4.0 / 3.0),
+    (4, 0.0, 2.0, 4.0),
+])
+def test_cadre_closed_forms(which, a, b, exact):
+    """CADRE (Cubature, src/eats.h:38-128 via the mesh9 intervals) meets its tolerance on
+    integrals with closed forms, including a sharp peak and a kink."""
+    ne, mh = C.c_long(0), C.c_long(0)
+    I = _lib().orc_cadre_test(which, a, b, 2000, 0.0, 1e-9, C.byref(ne), C.byref(mh))
+    assert abs(I - exact) <= 1e-8 * abs(exact), (I, exact)
+    assert ne.value > 0 and mh.value >= 1
+
+
+def _synch_R(x):
+    """Angle-averaged synchrotron function: R(x) = 1/2 int_0^pi F(x / sin a) sin^2 a da with
+    F(y) = y int_y^inf K_5/3, the isotropic-pitch-angle average that the CSYN fit approximates."""
+    def F(y):
+        if y > 700:
+            return 0.0
+        return y * integrate.quad(lambda t: special.kv(5. / 3., t), y, np.inf, limit=200)[0]
+    return 0.5 * integrate.quad(lambda a: F(x / math.sin(a)) * math.sin(a) ** 2, 1e-9, math.pi - 1e-9,
+                                limit=200, epsabs=0)[0]
+
+
+@pytest.mark.parametrize("x", [1e-3, 1e-2, 0.1, 0.5, 1.0, 3.0, 10.0])
+def test_csyn_fit_matches_synchrotron_function(x):
+    """SynKernel::cycSynch (src/microphysics/kernels.h:414-437) for gamma > 2 is
+    sqrt(3) e^3 B / (m c^2) R(nu/nu_c), nu_c = 3 e B gamma^2 / (4 pi m c), with R from the
+    Aharonian, Kelner & Prosekin (2010) fit: within 1 % of the exact R over the
+    spectrum's body (the fit is quoted to ~0.2 %)."""
+    B, gam = 1.3, 1e3
+    nu_c = 3 * QE * B * gam * gam / (4 * math.pi * ME * CL)
+    em = _lib().orc_cyc_synch(B, x * nu_c, gam)
+    exact = math.sqrt(3.) * QE ** 3 * B / MEC2 * _synch_R(x)
+    assert em == pytest.approx(exact, rel=1e-2)
+
+
+def test_csyn_scales_linearly_in_B_at_fixed_x():
+    L = _lib()
+    gam, x = 50.0, 0.7
+    e = []
+    for B in (0.1, 1.0, 10.0):
+        nu_c = 3 * QE * B * gam * gam / (4 * math.pi * ME * CL)
+        e.append(L.orc_cyc_synch(B, x * nu_c, gam))
+    assert e[1] / e[0] == pytest.approx(10.0, rel=1e-12) and e[2] / e[1] == pytest.approx(10.0, rel=1e-12)
+
+
+def test_ssc_kernel_thomson_limit_integral():
+    """SSCKernel::sscNava (src/microphysics/kernels.h:95-117) is the Blumenthal & Gould (1970)
+    isotropic kernel.  In the Thomson limit its upscattering branch integrates, over the
+    scattered photon energy, to (4/3) gamma^2 e_seed (so that with the 2 pi r0^2 c / gamma^2
+    prefactor the scattering rate is sigma_T c), and the downscattering branch to
+    e_seed / 2 (1 - 1/(16 gamma^4)) - e_seed/(4 gamma^2) (1 - 1/(4 gamma^2))."""
+    L = _lib()
+    gam = 30.0
+    nu_t = 1e9                        # h nu / m c^2 ~ 8e-12: deep Thomson
+    et = nu_t * H / MEC2
+    up_hi = 4 * gam * gam * et / (1 + 4 * gam * et)
+    nus = np.logspace(np.log10(nu_t / (4 * gam * gam)), np.log10(up_hi * MEC2 / H), 200001)
+    K = np.array([L.orc_ssc_nava(n, gam, nu_t) for n in nus])
+    e = nus * H / MEC2
+    down = e <= et
+    I_down = np.trapezoid(K[down], e[down])
+    I_up = np.trapezoid(K[~down], e[~down])
+    exp_down = et / 2 * (1 - 1 / (16 * gam ** 4)) - et / (4 * gam * gam) * (1 - 1 / (4 * gam * gam))
+    exp_up = 4. / 3. * gam * gam * et
+    assert I_down == pytest.approx(exp_down, rel=1e-5)
+    assert I_up == pytest.approx(exp_up, rel=2e-3)   # Klein-Nishina correction is O(gamma e_seed)
+    # kernel vanishes at the kinematic limits
+    assert L.orc_ssc_nava(up_hi * MEC2 / H * 1.0001, gam, nu_t) == 0.0
+    assert L.orc_ssc_nava(nu_t / (4 * gam * gam) * 0.999, gam, nu_t) == 0.0
+
+
+def test_ssc_kernel_klein_nishina_suppression():
+    """Energy-weighted integral of the upscattering branch: 16 int_0^1 q F(q) dq = 16/9 of
+    (gamma^2 e_seed)^2 in the Thomson limit; far below it once gamma e_seed >> 1, where the
+    Klein-Nishina term takes over."""
+    L = _lib()
+    gam = 1e4
+
+    def mean_gain(nu_t):
+        et = nu_t * H / MEC2
+        hi = 4 * gam * gam * et / (1 + 4 * gam * et)
+        nus = np.logspace(np.log10(nu_t * 1.000001), np.log10(hi * MEC2 / H * 0.999999), 20001)
+        e = nus * H / MEC2
+        K = np.array([L.orc_ssc_nava(n, gam, nu_t) for n in nus])
+        return np.trapezoid(K * e, e) / (gam * gam * et) ** 2
+    thomson = mean_gain(1e8)
+    kn = mean_gain(1e20)     # gamma * h nu / m c^2 ~ 8e2
+    assert thomson == pytest.approx(16. / 9., rel=1e-3)
+    assert kn < 1e-2 * thomson
+
+
+@pytest.fixture(scope="module")
+def tophat_dyn():
+    g = load_golden("tophat_fs_syn")
+    run = golden_run(g)
+    T, res = pyoracle.dyn_evolve(run.params, run.inits, run.tgrid)
+    return run, T[0]
+
+
+def test_dynamics_coasting_phase(tophat_dyn):
+    """Before the swept-up mass matters (M2 << M0 / Gamma0) the shell coasts: Gamma = Gamma0
+    and dR/dt = beta0 c in the burster frame."""
+    run, t = tophat_dyn
+    PV = abi.PV
+    tb = t[PV["tburst"]]
+    R = t[PV["R"]]
+    G = t[PV["Gamma"]]
+    g0 = math.sqrt(run.inits[0].mom0 ** 2 + 1)
+    b0 = run.inits[0].mom0 / g0
+    k = np.where(t[PV["M2"]] < 1e-6 * run.inits[0].M0 / g0)[0]
+    assert len(k) > 50
+    assert np.allclose(G[k], g0, rtol=1e-6)
+    dRdt = np.diff(R[k]) / np.diff(tb[k])
+    assert np.allclose(dRdt, b0 * CL, rtol=1e-6)
+
+
+def test_dynamics_blandford_mckee_slope(tophat_dyn):
+    """Adiabatic relativistic deceleration in a uniform medium: Gamma ~ R^(-3/2)
+    (Blandford & McKee 1976), checked where 10 < Gamma < Gamma0/5 and before spreading."""
+    run, t = tophat_dyn
+    PV = abi.PV
+    G, R, th = t[PV["Gamma"]], t[PV["R"]], t[PV["theta"]]
+    m = (G > 10) & (G < 60) & (th <= th[0] * 1.0001)
+    assert m.sum() > 10
+    slope = np.polyfit(np.log(R[m]), np.log(G[m]), 1)[0]
+    assert slope == pytest.approx(-1.5, abs=0.08)
+
+
+def test_dynamics_energy_budget_relativistic(tophat_dyn):
+    """Through the relativistic phase (Gamma > 10) kinetic + internal + radiated energy stays at
+    E0 to within the reference equations' own non-conservation (~1 %), with adiabatic
+    losses returned to the bulk flow."""
+    run, t = tophat_dyn
+    PV = abi.PV
+    G, M2, E2, adi, Er = (t[PV[n]] for n in ("Gamma", "M2", "Eint2", "adi", "Erad2"))
+    M0, E0 = run.inits[0].M0, run.inits[0].E0
+    geff = (adi * G * G - adi + 1) / G
+    Et = ((G - 1) * (M0 + M2) * CL * CL + geff * E2 + Er) / E0
+    m = G > 10
+    assert np.all(np.abs(Et[m] - 1) < 2e-2)
+    assert abs(Et[0] - 1) < 1e-12
