@@ -138,6 +138,21 @@ int pbagpu_device_count(int *count);
 int pbagpu_set_device(int device);
 int pbagpu_version(void);
 
+/* ---------------------------------------------------- kernel timing */
+/* No counterpart in the reference: instrumentation for benchmarks.  When
+ * enabled, every stage records HIP events on the caller's stream immediately
+ * before and after its main kernel and accumulates the elapsed device time.
+ * Not thread safe; meant for one benchmarking thread. */
+enum {
+    PBAGPU_STAGE_DYN = 0,      /* dyn_kernel */
+    PBAGPU_STAGE_ELE = 1,      /* ele_kernel */
+    PBAGPU_STAGE_EATS = 2,     /* eats_kernel */
+    PBAGPU_STAGE_SSC_TABLE = 3,/* Compton kernel table build */
+    PBAGPU_N_STAGES = 4
+};
+int pbagpu_timing_enable(int on);
+int pbagpu_timing_read(int stage, double *ms_total, long long *launches, int reset);
+
 /* ------------------------------------------------------------- stage 1 */
 /* Evolve n_runs independent models of bw_per_run blastwaves each.
  *   params   [n_runs]            host array

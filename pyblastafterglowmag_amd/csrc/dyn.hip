@@ -17,6 +17,7 @@
 // EvolveODEsystem::advanceTimeSubStep (src/model_evolve.h:439-566).
 // The reference evaluates Gamma43 and dM3/dR in x87 long double; FP64 is used here.
 #include "pba_device.h"
+#include "timing.h"
 
 namespace pbagpu {
 namespace dyn {
@@ -690,14 +691,18 @@ extern "C" int pbagpu_dyn_evolve(const pba_run_params *params, int n_runs, int b
     if (hipMemcpyAsync(dP, params, sizeof(pba_run_params) * n_runs, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(dI, init, sizeof(pba_layer_init) * nbw, hipMemcpyHostToDevice, s) != hipSuccess)
         rc = PBAGPU_E_HIP;
+    pbagpu::KernelTimer timer(PBAGPU_STAGE_DYN);
     if (rc == PBAGPU_OK) {
+        timer.start(s);
         hipLaunchKernelGGL(pbagpu::dyn::dyn_kernel, dim3(n_runs), dim3(64), 0, s, dP, dI, bw_per_run, nt, d_tgrid,
                            d_table, dR);
+        timer.stop(s);
         if (hipGetLastError() != hipSuccess) rc = PBAGPU_E_HIP;
     }
     if (rc == PBAGPU_OK && hipMemcpyAsync(res, dR, sizeof(pba_bw_result) * nbw, hipMemcpyDeviceToHost, s) != hipSuccess)
         rc = PBAGPU_E_HIP;
     if (hipStreamSynchronize(s) != hipSuccess) rc = PBAGPU_E_HIP;
+    timer.collect();
     hipFree(dP); hipFree(dI); hipFree(dR);
     return rc;
 }

@@ -19,6 +19,7 @@
 // table mu[i] = (t_b[i] - t_obs/(1+z)) c / R[i] is evaluated on the fly during the
 // index search instead of being stored.
 #include "pba_device.h"
+#include "timing.h"
 
 namespace pbagpu {
 namespace eats {
@@ -425,13 +426,17 @@ extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, 
         hipMemcpyAsync(dL, hl.data(), sizeof(LayerArgs) * n_layers, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemsetAsync(d_status, 0, sizeof(int) * n_layers, s) != hipSuccess)
         rc = PBAGPU_E_HIP;
+    pbagpu::KernelTimer timer(PBAGPU_STAGE_EATS);
     if (rc == PBAGPU_OK) {
         size_t nth = (size_t)n_layers * n_pairs;
+        timer.start(s);
         hipLaunchKernelGGL(eats_kernel, dim3((unsigned)((nth + 63) / 64)), dim3(64), 0, s, dL, n_layers, n_pairs, d_tobs,
                            d_nuobs, d_out, d_status);
+        timer.stop(s);
         if (hipGetLastError() != hipSuccess) rc = PBAGPU_E_HIP;
     }
     if (hipStreamSynchronize(s) != hipSuccess) rc = PBAGPU_E_HIP;
+    timer.collect();
     (void)hipFree(dnu);
     (void)hipFree(dL);
     return rc;

@@ -21,6 +21,7 @@
 // (computeSSCSpectrum, numeric_model.h:247-285) reads the precomputed
 // Blumenthal-Gould / Nava kernel table (SSCKernel::evalSSCkernel, kernels.h:49-117).
 #include "pba_device.h"
+#include "timing.h"
 #include <vector>
 
 namespace pbagpu {
@@ -398,6 +399,8 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
     }
     if (hipMalloc(&dpops, sizeof(PopDesc) * n_pop) != hipSuccess) { hipFree(dgrids); hipFree(dK); return PBAGPU_E_HIP; }
     hipMemcpyAsync(dgrids, hgrids.data(), sizeof(double) * hgrids.size(), hipMemcpyHostToDevice, s);
+    pbagpu::KernelTimer ttab(PBAGPU_STAGE_SSC_TABLE), tele(PBAGPU_STAGE_ELE);
+    if (n_ssc) ttab.start(s);
     for (size_t kk = 0; kk < keys.size(); kk++) {
         if (!need_ssc[kk]) continue;
         double *base = dgrids + per_key * kk;
@@ -409,6 +412,7 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
         hipLaunchKernelGGL(ssc_integral_kernel, dim3((unsigned)((ng * nf + 255) / 256)), dim3(256), 0, s, nu, K, nf, ng,
                            KI);
     }
+    if (n_ssc) ttab.stop(s);
     std::vector<PopDesc> hp(n_pop);
     for (int q = 0; q < n_pop; q++) {
         const pba_population &P = pops[q];
@@ -428,9 +432,13 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
     hipMemcpyAsync(dpops, hp.data(), sizeof(PopDesc) * n_pop, hipMemcpyHostToDevice, s);
     hipMemsetAsync(d_jall, 0, sizeof(double) * (size_t)n_pop * nt * nf, s);
     if (d_fall) hipMemsetAsync(d_fall, 0, sizeof(double) * (size_t)n_pop * nt * ng, s);
+    tele.start(s);
     hipLaunchKernelGGL(ele_kernel, dim3(n_pop), dim3(64), 0, s, dpops, nt, d_status);
+    tele.stop(s);
     if (hipGetLastError() != hipSuccess) rc = PBAGPU_E_HIP;
     if (hipStreamSynchronize(s) != hipSuccess) rc = PBAGPU_E_HIP;
+    ttab.collect();
+    tele.collect();
     hipFree(dgrids); hipFree(dK); hipFree(dpops);
     return rc;
 }

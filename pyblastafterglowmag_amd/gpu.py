@@ -14,8 +14,9 @@ from . import abi, h5io, inputs
 _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libpbagpu.so")
 _lib = None
 
-EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu_dyn_evolve",
-           "pbagpu_ele_evolve", "pbagpu_eats_lightcurve")
+EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu_timing_enable",
+           "pbagpu_timing_read", "pbagpu_dyn_evolve", "pbagpu_ele_evolve", "pbagpu_eats_lightcurve")
+STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE = 0, 1, 2, 3
 
 
 def lib():
@@ -31,10 +32,23 @@ def lib():
                                         vp, vp, C.POINTER(abi.BwResult), vp]
         L.pbagpu_ele_evolve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.Population), C.c_int, vp,
                                         C.c_int, C.c_int, vp, vp, vp, C.c_int, vp]
+        L.pbagpu_timing_enable.argtypes = [C.c_int]
+        L.pbagpu_timing_read.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]
         L.pbagpu_eats_lightcurve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsLayer), C.c_int, vp,
                                              C.c_int, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
+
+
+def timing_enable(on=True):
+    _check(lib().pbagpu_timing_enable(1 if on else 0), "timing_enable")
+
+
+def timing_read(stage, reset=False):
+    """(total device ms, launches) of a stage's main kernel since the last reset."""
+    ms, n = C.c_double(0.0), C.c_longlong(0)
+    _check(lib().pbagpu_timing_read(stage, C.byref(ms), C.byref(n), 1 if reset else 0), "timing_read")
+    return ms.value, n.value
 
 
 def _torch():
@@ -120,47 +134,106 @@ def populations(params_list, results, n_runs, bw_per_run):
     return pops, lays
 
 
+class Pipeline:
+    """Device buffers and the three stage calls for a fixed batch of Runs with equal grid sizes.
+
+    The dynamics table [n_bw][PBA_NV][nt], the emissivity table [n_pop][nt][nfreq] and the
+    light curves [n_layers][n_pairs] stay in device memory between stages; only the small
+    per-blastwave results of stage 1 come back to the host (to build the population list,
+    as the reference decides per blastwave whether the reverse shock radiates).
+    """
+
+    def __init__(self, runs, stream=None, precision=0):
+        torch = _torch()
+        self.torch = torch
+        self.L = lib()
+        self.dev = torch.device("cuda", torch.cuda.current_device())
+        self.runs = runs
+        self.n_runs = len(runs)
+        r0 = runs[0]
+        self.bpr = r0.n_layers
+        self.nt = len(r0.tgrid)
+        for r in runs:
+            if r.n_layers != self.bpr or len(r.tgrid) != self.nt or len(r.tobs) != len(r0.tobs):
+                raise ValueError("runs of one batch need equal layer count, time grid and pair count")
+            if not np.array_equal(r.tgrid, r0.tgrid) or not np.array_equal(r.tobs, r0.tobs) \
+                    or not np.array_equal(r.nuobs, r0.nuobs):
+                raise ValueError("runs of one batch share the time grid and the observer pairs")
+        self.nf = r0.params.fs.nfreq
+        self.npairs = len(r0.tobs)
+        self.precision = precision
+        self.stream = stream
+        self.sp = _stream_ptr(stream)
+        self.params = (abi.RunParams * self.n_runs)(*[r.params for r in runs])
+        self.inits = (abi.LayerInit * (self.n_runs * self.bpr))(*[li for r in runs for li in r.inits])
+        nbw = self.n_runs * self.bpr
+        self.d_tg = torch.from_numpy(r0.tgrid).to(self.dev)
+        self.d_table = torch.empty((nbw, abi.PBA_NV, self.nt), dtype=torch.float64, device=self.dev)
+        self.res = (abi.BwResult * nbw)()
+        # at most two populations per blastwave
+        self.d_jall = torch.empty((2 * nbw, self.nt, self.nf), dtype=torch.float64, device=self.dev)
+        self.d_est = torch.zeros(2 * nbw, dtype=torch.int32, device=self.dev)
+        self.d_tobs = torch.from_numpy(r0.tobs).to(self.dev)
+        self.d_nuobs = torch.from_numpy(r0.nuobs).to(self.dev)
+        self.d_out = torch.empty((nbw, self.npairs), dtype=torch.float64, device=self.dev)
+        self.d_lst = torch.zeros(nbw, dtype=torch.int32, device=self.dev)
+        self.pops = self.layers = None
+        torch.cuda.synchronize()
+
+    def dynamics(self):
+        _check(self.L.pbagpu_dyn_evolve(self.params, self.n_runs, self.bpr, self.inits, self.nt, _ptr(self.d_tg),
+                                        _ptr(self.d_table), self.res, self.sp), "dyn_evolve")
+        pops, lays = populations(self.params, self.res, self.n_runs, self.bpr)
+        self.pops = pops
+        self.parr = (abi.Population * len(pops))(*pops)
+        self.larr = (abi.EatsLayer * len(lays))()
+        for i, (bw, r, pfs, prs, iend) in enumerate(lays):
+            li = self.runs[r].inits[bw - r * self.bpr]
+            self.larr[i] = abi.EatsLayer(li.theta_c_l, li.theta_c_h, bw, r, pfs, prs, iend)
+        self.layers = lays
+
+    def electrons(self, want_f=False):
+        npop = len(self.pops)
+        self.d_fall = self.torch.empty((npop, self.nt, self.runs[0].params.fs.ngam), dtype=self.torch.float64,
+                                       device=self.dev) if want_f else None
+        _check(self.L.pbagpu_ele_evolve(self.params, self.n_runs, self.parr, npop, _ptr(self.d_table),
+                                        self.n_runs * self.bpr, self.nt, _ptr(self.d_jall), _ptr(self.d_fall),
+                                        _ptr(self.d_est), self.precision, self.sp), "ele_evolve")
+
+    def observer(self):
+        _check(self.L.pbagpu_eats_lightcurve(self.params, self.n_runs, self.larr, len(self.layers), _ptr(self.d_table),
+                                             self.nt, _ptr(self.d_jall), self.nf, self.npairs, _ptr(self.d_tobs),
+                                             _ptr(self.d_nuobs), _ptr(self.d_out), _ptr(self.d_lst), self.sp),
+               "eats_lightcurve")
+
+    def step(self):
+        self.dynamics()
+        self.electrons()
+        self.observer()
+
+    def status(self):
+        npop = len(self.pops)
+        return (np.array([r.status for r in self.res]), self.d_est[:npop].cpu().numpy(),
+                self.d_lst[:len(self.layers)].cpu().numpy())
+
+    def flux(self):
+        return self.d_out.cpu().numpy().reshape(self.n_runs, self.bpr, self.npairs)
+
+
 def lightcurves(runs, stream=None, want_intermediates=False, precision=0):
     """Run the three GPU stages for a list of Runs with equal grid sizes.
 
     Returns (flux [n_runs, n_layers, n_pairs] numpy array, extras dict)."""
-    torch = _torch()
-    L = lib()
-    dev = torch.device("cuda", torch.cuda.current_device())
-    n_runs = len(runs)
-    r0 = runs[0]
-    bpr = r0.n_layers
-    nt = len(r0.tgrid)
-    params = (abi.RunParams * n_runs)(*[r.params for r in runs])
-    inits = (abi.LayerInit * (n_runs * bpr))(*[li for r in runs for li in r.inits])
-    d_tg = torch.from_numpy(r0.tgrid).to(dev)
-    d_table = torch.empty((n_runs * bpr, abi.PBA_NV, nt), dtype=torch.float64, device=dev)
-    res = (abi.BwResult * (n_runs * bpr))()
-    sp = _stream_ptr(stream)
-    _check(L.pbagpu_dyn_evolve(params, n_runs, bpr, inits, nt, _ptr(d_tg), _ptr(d_table), res, sp), "dyn_evolve")
-    pops, lays = populations(params, res, n_runs, bpr)
-    nf = r0.params.fs.nfreq
-    npop = len(pops)
-    parr = (abi.Population * npop)(*pops)
-    d_jall = torch.empty((npop, nt, nf), dtype=torch.float64, device=dev)
-    d_est = torch.zeros(npop, dtype=torch.int32, device=dev)
-    _check(L.pbagpu_ele_evolve(params, n_runs, parr, npop, _ptr(d_table), n_runs * bpr, nt, _ptr(d_jall), None,
-                               _ptr(d_est), precision, sp), "ele_evolve")
-    larr = (abi.EatsLayer * len(lays))()
-    for i, (bw, r, pfs, prs, iend) in enumerate(lays):
-        li = runs[r].inits[bw - r * bpr]
-        larr[i] = abi.EatsLayer(li.theta_c_l, li.theta_c_h, bw, r, pfs, prs, iend)
-    npairs = len(r0.tobs)
-    d_tobs = torch.from_numpy(r0.tobs).to(dev)
-    d_nuobs = torch.from_numpy(r0.nuobs).to(dev)
-    d_out = torch.empty((len(lays), npairs), dtype=torch.float64, device=dev)
-    d_lst = torch.zeros(len(lays), dtype=torch.int32, device=dev)
-    _check(L.pbagpu_eats_lightcurve(params, n_runs, larr, len(lays), _ptr(d_table), nt, _ptr(d_jall), nf, npairs,
-                                    _ptr(d_tobs), _ptr(d_nuobs), _ptr(d_out), _ptr(d_lst), sp), "eats_lightcurve")
-    flux = d_out.cpu().numpy().reshape(n_runs, bpr, npairs)
-    extras = dict(dyn_status=np.array([r.status for r in res]), ele_status=d_est.cpu().numpy(),
-                  eats_status=d_lst.cpu().numpy())
+    pl = Pipeline(runs, stream=stream, precision=precision)
+    pl.dynamics()
+    pl.electrons(want_f=want_intermediates)
+    pl.observer()
+    dst, est, lst = pl.status()
+    flux = pl.flux()
+    extras = dict(dyn_status=dst, ele_status=est, eats_status=lst)
     if want_intermediates:
-        extras.update(table=d_table.cpu().numpy(), jall=d_jall.cpu().numpy(), pops=pops, layers=lays,
-                      results=[(r.status, r.i_end_r, r.fsrs, r.n_steps, r.n_rhs) for r in res])
+        npop = len(pl.pops)
+        extras.update(table=pl.d_table.cpu().numpy(), jall=pl.d_jall[:npop].cpu().numpy(),
+                      fall=pl.d_fall.cpu().numpy(), pops=pl.pops, layers=pl.layers,
+                      results=[(r.status, r.i_end_r, r.fsrs, r.n_steps, r.n_rhs) for r in pl.res])
     return flux, extras

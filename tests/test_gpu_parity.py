@@ -2,8 +2,8 @@
 
 Tolerances: the GPU evaluates the same FP64 algorithm, but with fused multiply-adds, the
 device math library and FP64 in place of x87 long double; ulp-level differences move the
-shared adaptive ODE step.  Forward-shock-only dynamics agree to 2e-6; with the reverse
-shock the step sequences part after the first ulp difference and the tables agree to 1e-4,
+shared adaptive ODE step.  Forward-shock-only runs agree to 2e-6; with the reverse shock
+in any layer of a run the step sequences part after the first ulp difference and the tables agree to 1e-4,
 the same bound the oracle keeps against the reference program.  Electron and observer
 stages are compared on identical inputs (the GPU's own tables): 1e-9 and 1e-10.  Light
 curves end to end against the reference program: 1e-6 relative (the FP64-path target).
@@ -37,10 +37,11 @@ def test_stages_match_oracle(name):
     print(f"[gpu] {name}: layers={run.n_layers} nt={len(run.tgrid)} pairs={len(run.tobs)}", flush=True)
     flux, ex = _gpu(run)
     table_o, res_o = pyoracle.dyn_evolve(run.params, run.inits, run.tgrid)
+    # one shared step for all layers of a run: a reverse shock in any layer moves every layer
+    tol = 1e-4 if any(r.fsrs for r in res_o) else 2e-6
     for il in range(run.n_layers):
         assert ex["results"][il][1] == res_o[il].i_end_r
         assert ex["results"][il][2] == res_o[il].fsrs
-        tol = 1e-4 if res_o[il].fsrs else 2e-6
         for n in DYN_CHECK:
             e = rel_err(ex["table"][il, abi.PV[n]], table_o[il, abi.PV[n]], 1e-300)
             assert e <= tol, (name, il, n, e)
