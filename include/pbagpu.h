@@ -137,6 +137,10 @@ typedef struct {
 int pbagpu_device_count(int *count);
 int pbagpu_set_device(int device);
 int pbagpu_version(void);
+/* Text of the last failure in this thread (failing call and HIP error), "" if none.
+ * The reference logs the cause before exit(1) (e.g. numeric_model.h:688-691); this
+ * string lets the host reproduce that message. */
+const char *pbagpu_last_error(void);
 
 /* ---------------------------------------------------- kernel timing */
 /* No counterpart in the reference: instrumentation for benchmarks.  When

@@ -39,3 +39,46 @@ extern "C" int pbagpu_timing_read(int stage, double *ms_total, long long *launch
     if (reset) { pbagpu::g_ms[stage] = 0.0; pbagpu::g_launches[stage] = 0; }
     return PBAGPU_OK;
 }
+
+// ------------------------------------------------------------- errors and workspace
+#include "runtime.h"
+#include <cstdio>
+#include <cstring>
+
+namespace pbagpu {
+static thread_local char g_err[512] = "";
+
+void set_error(const char *what, hipError_t e) {
+    std::snprintf(g_err, sizeof(g_err), "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+}
+
+void set_error_msg(const char *msg) { std::snprintf(g_err, sizeof(g_err), "%s", msg); }
+
+namespace {
+constexpr int kMaxDevices = 16;
+struct Buf { void *p = nullptr; size_t cap = 0; };
+Buf g_ws[kMaxDevices][WS_N];
+}  // namespace
+
+void *workspace(int slot, size_t bytes) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { set_error("hipGetDevice", e); return nullptr; }
+    if (dev < 0 || dev >= kMaxDevices || slot < 0 || slot >= WS_N) { set_error_msg("workspace: bad device/slot"); return nullptr; }
+    Buf &b = g_ws[dev][slot];
+    if (bytes == 0) bytes = 256;
+    if (b.cap >= bytes) return b.p;
+    if (b.p) {
+        e = hipFree(b.p);
+        b.p = nullptr; b.cap = 0;
+        if (e != hipSuccess) { set_error("hipFree(workspace)", e); return nullptr; }
+    }
+    size_t want = bytes + bytes / 2;
+    e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) { b.p = nullptr; set_error("hipMalloc(workspace)", e); return nullptr; }
+    b.cap = want;
+    return b.p;
+}
+}  // namespace pbagpu
+
+extern "C" const char *pbagpu_last_error(void) { return pbagpu::g_err; }

@@ -20,6 +20,9 @@
 // index search instead of being stored.
 #include "pba_device.h"
 #include "timing.h"
+#include "runtime.h"
+#include <algorithm>
+#include <vector>
 
 namespace pbagpu {
 namespace eats {
@@ -32,12 +35,21 @@ struct Iv {
     int refinement;
 };
 
+// Interval store of one lane: pool, heap keys and heap slots live in a lane-private,
+// contiguous block of global memory (one Iv is 112 contiguous bytes, so reading or
+// writing an interval touches one or two cache lines of that lane only).
 template <int H>
 struct Store {
-    Iv pool[H];
-    double herr[H];
-    int hslot[H];
+    Iv *pool;
+    double *herr;
+    int *hslot;
     int N, used, overflow;
+    static constexpr size_t bytes() { return (size_t)H * (sizeof(Iv) + sizeof(double) + sizeof(int)); }
+    __device__ void bind(char *base) {
+        pool = (Iv *)base;
+        herr = (double *)(base + (size_t)H * sizeof(Iv));
+        hslot = (int *)(base + (size_t)H * (sizeof(Iv) + sizeof(double)));
+    }
     __device__ void reset() { N = 0; used = 0; overflow = 0; }
     __device__ int alloc() {
         if (used >= H) { overflow = 1; return H - 1; }
@@ -232,15 +244,29 @@ struct LayerArgs {
     int nt, nf;
 };
 
+// Table columns the integrand reads, staged in LDS once per workgroup (one layer per
+// workgroup): every index search and interpolation then reads LDS instead of L2.
+enum Slot { S_tb, S_R, S_theta, S_Gamma, S_beta, S_M2, S_rho2, S_thrs, S_M3, S_rho3, NSLOT };
+__device__ __forceinline__ constexpr int slot_var(int s) {
+    return s == S_tb ? PV_tburst : s == S_R ? PV_R : s == S_theta ? PV_theta : s == S_Gamma ? PV_Gamma
+         : s == S_beta ? PV_beta : s == S_M2 ? PV_M2 : s == S_rho2 ? PV_rho2 : s == S_thrs ? PV_thickness_rs
+         : s == S_M3 ? PV_M3 : PV_rho3;
+}
+
+extern __shared__ double lds_tab[];
+
 struct Ctx {
     const LayerArgs *L;
+    int nt;
     double tz;          // t_obs / (1 + z)
     double nu_obs;
     double sa, ca;      // sin, cos of the observer angle
     bool cj;
     int status;
-    __device__ double V(int v, int i) const { return L->T[(size_t)v * L->nt + i]; }
-    __device__ double mu_at(int i) const { return (V(PV_tburst, i) - tz) / V(PV_R, i) * kC; }
+    __device__ double V(int s, int i) const { return lds_tab[s * nt + i]; }
+    __device__ double nu_fs(int i) const { return lds_tab[NSLOT * nt + i]; }
+    __device__ double nu_rs(int i) const { return lds_tab[NSLOT * nt + L->nf + i]; }
+    __device__ double mu_at(int i) const { return (V(S_tb, i) - tz) / V(S_R, i) * kC; }
     __device__ double obs_angle(double theta, double phi) const {
         if (!cj) return sa * (sin(theta) * sin(phi)) + ca * cos(theta);
         return sa * (sin(kPi - theta) * sin(phi)) + ca * cos(kPi - theta);
@@ -255,7 +281,7 @@ struct Ctx {
 __device__ double find_jet_edge(const Ctx &C, double phi, double th_con_hi) {
     double mu = C.obs_angle(th_con_hi, phi);
     int ia = C.mu_index(mu);
-    double a0 = C.V(PV_theta, ia), a1 = C.V(PV_theta, ia + 1);
+    double a0 = C.V(S_theta, ia), a1 = C.V(S_theta, ia + 1);
     if (a0 <= th_con_hi && th_con_hi <= a1) return th_con_hi;
     double ta, tb;
     if (th_con_hi < a0) { ta = th_con_hi; tb = 0.5 * M_PI; }
@@ -265,19 +291,20 @@ __device__ double find_jet_edge(const Ctx &C, double phi, double th_con_hi) {
         double tc = 0.5 * (ta + tb);
         mu = C.obs_angle(tc, phi);
         ia = C.mu_index(mu);
-        if (tc < C.V(PV_theta, ia)) ta = tc; else tb = tc;
+        if (tc < C.V(S_theta, ia)) ta = tc; else tb = tc;
         i++;
     }
     return ta;
 }
 
-__device__ double shock_flux(Ctx &C, const double *jall, const double *nu, double freq, int ia, int ib, double Gamma,
+// nu_off: offset of the shock's frequency grid in the staged LDS block
+__device__ double shock_flux(Ctx &C, const double *jall, int nu_off, double freq, int ia, int ib, double Gamma,
                              double mu, double r, double ne, double n_prime, double theta) {
     const int nf = C.L->nf;
-    if (freq >= nu[nf - 1] || freq <= nu[0]) { C.status |= PBAGPU_S_NU_OUT_OF_GRID; return 0.; }
-    auto arr = [&](int i) { return nu[i]; };
-    int ia_nu = find_index(freq, arr, nf), ib_nu = ia_nu + 1;
-    double x1 = nu[ia_nu], x2 = nu[ib_nu], y1 = C.V(PV_R, ia), y2 = C.V(PV_R, ib);
+    auto nu = [&](int i) { return lds_tab[nu_off + i]; };
+    if (freq >= nu(nf - 1) || freq <= nu(0)) { C.status |= PBAGPU_S_NU_OUT_OF_GRID; return 0.; }
+    int ia_nu = find_index(freq, nu, nf), ib_nu = ia_nu + 1;
+    double x1 = nu(ia_nu), x2 = nu(ib_nu), y1 = C.V(S_R, ia), y2 = C.V(S_R, ib);
     double q11 = jall[(size_t)ia * nf + ia_nu], q12 = jall[(size_t)ib * nf + ia_nu];
     double q21 = jall[(size_t)ia * nf + ib_nu], q22 = jall[(size_t)ib * nf + ib_nu];
     double x2x1 = x2 - x1, y2y1 = y2 - y1, x2x = x2 - freq, y2y = y2 - r, yy1 = r - y1, xx1 = freq - x1;
@@ -293,9 +320,11 @@ __device__ double shock_flux(Ctx &C, const double *jall, const double *nu, doubl
     return lum;
 }
 
-__device__ double integrand(Ctx &C, double cos_theta, double phi) {
+// Out of line on purpose: CADRE samples the integrand from eleven places, and one shared
+// copy of its code keeps the kernel inside the instruction cache.
+__device__ __noinline__ double integrand(Ctx &C, double cos_theta, double phi) {
     const int N = C.L->i_end_r;
-    if (C.V(PV_R, 0) == 0.0 && C.V(PV_R, N - 1) == 0.0) return 0.0;
+    if (C.V(S_R, 0) == 0.0 && C.V(S_R, N - 1) == 0.0) return 0.0;
     double a_theta = 2.0 * asin(sqrt(0.5 * (1.0 - cos_theta)));
     double mu = C.obs_angle(a_theta, phi);
     double mu0 = C.mu_at(0), muN = C.mu_at(N - 1);
@@ -303,81 +332,132 @@ __device__ double integrand(Ctx &C, double cos_theta, double phi) {
     if (mu0 >= mu) return 0.;
     int ia = C.mu_index(mu), ib = ia + 1;
     double mua = C.mu_at(ia), mub = C.mu_at(ib);
-    double tba = C.V(PV_tburst, ia), tbb = C.V(PV_tburst, ib);
+    double tba = C.V(S_tb, ia), tbb = C.V(S_tb, ib);
     double t_e = interp_lin(mu, mua, mub, tba, tbb);
     if (t_e < 0.0 || !isfinite(t_e)) return 0.;
-    double Ra = C.V(PV_R, ia), Rb = C.V(PV_R, ib);
+    double Ra = C.V(S_R, ia), Rb = C.V(S_R, ib);
     double r = interp_log(t_e, tba, tbb, Ra, Rb);
     if (r <= 0.0 || !isfinite(r)) return 0.;
     // shared log-interpolation weight in R for all state variables
     double wr = log(r / Ra) / log(Rb / Ra);
     auto li = [&](int v) { double ya = C.V(v, ia), yb = C.V(v, ib); return ya * pow(yb / ya, wr); };
-    double Gamma = li(PV_Gamma);
-    double beta = li(PV_beta);
-    double ne = li(PV_M2) / kMp;
-    double n_prime = li(PV_rho2) / kMp;
+    double Gamma = li(S_Gamma);
+    double beta = li(S_beta);
+    double ne = li(S_M2) / kMp;
+    double n_prime = li(S_rho2) / kMp;
     double nuprime = (1.0 + C.L->z) * C.nu_obs * (Gamma * (1.0 - beta * mu));
-    double theta = li(PV_theta);
-    double flux = shock_flux(C, C.L->jfs, C.L->nufs, nuprime, ia, ib, Gamma, mu, r, ne, n_prime, theta);
-    if (C.L->jrs && C.V(PV_thickness_rs, ia) > 0 && C.V(PV_thickness_rs, ib) > 0) {
-        double ne_rs = li(PV_M3) / kMp;
-        double np_rs = li(PV_rho3) / kMp;
-        flux += shock_flux(C, C.L->jrs, C.L->nurs, nuprime, ia, ib, Gamma, mu, r, ne_rs, np_rs, theta);
+    double theta = li(S_theta);
+    double flux = shock_flux(C, C.L->jfs, NSLOT * C.nt, nuprime, ia, ib, Gamma, mu, r, ne, n_prime, theta);
+    if (C.L->jrs && C.V(S_thrs, ia) > 0 && C.V(S_thrs, ib) > 0) {
+        double ne_rs = li(S_M3) / kMp;
+        double np_rs = li(S_rho3) / kMp;
+        flux += shock_flux(C, C.L->jrs, NSLOT * C.nt + C.L->nf, nuprime, ia, ib, Gamma, mu, r, ne_rs, np_rs, theta);
     }
     return flux;
+}
+
+// Inner (1 - cos theta) quadrature at one phi sample (phi_integrand, src/eats.h:460-585);
+// out of line for the same reason as the integrand.
+__device__ __noinline__ double phi_integrand(Ctx &C, Store<HI> &Si, double a_phi) {
+    const LayerArgs &L = *C.L;
+    const double th1c = L.theta_c_h, th0c = L.theta_c_l;
+    double th_1 = find_jet_edge(C, a_phi, th1c);
+    double th_0 = (th0c / th1c) * th_1;
+    double t0 = th_0 > 0.5 * M_PI ? 0.5 * M_PI : th_0;
+    double t1 = th_1 > 0.5 * M_PI ? 0.5 * M_PI : th_1;
+    double s0 = sin(0.5 * t0), s1 = sin(0.5 * t1);
+    auto ct = [&](double aomct) -> double { return integrand(C, 1 - aomct, a_phi); };
+    double res = cadre_adapt(Si, ct, 2.0 * s0 * s0, 2.0 * s1 * s1, L.nmax_theta, 0., L.rtol_theta);
+    if (Si.overflow) C.status |= PBAGPU_S_HEAP_FULL;
+    if (res != res || res < 0.0) return 0.;
+    return res;
 }
 
 __device__ double integrate_theta_phi(Ctx &C, Store<HO> &So, Store<HI> &Si) {
     const LayerArgs &L = *C.L;
     const double Fcoeff = kCgs2mJy / (4 * M_PI * L.d_l * L.d_l);
-    const double th1c = L.theta_c_h, th0c = L.theta_c_l;
-    auto phi_integrand = [&](double a_phi) -> double {
-        double th_1 = find_jet_edge(C, a_phi, th1c);
-        double th_0 = (th0c / th1c) * th_1;
-        double t0 = th_0 > 0.5 * M_PI ? 0.5 * M_PI : th_0;
-        double t1 = th_1 > 0.5 * M_PI ? 0.5 * M_PI : th_1;
-        double s0 = sin(0.5 * t0), s1 = sin(0.5 * t1);
-        auto ct = [&](double aomct) -> double { return integrand(C, 1 - aomct, a_phi); };
-        double res = cadre_adapt(Si, ct, 2.0 * s0 * s0, 2.0 * s1 * s1, L.nmax_theta, 0., L.rtol_theta);
-        if (Si.overflow) C.status |= PBAGPU_S_HEAP_FULL;
-        if (res != res || res < 0.0) return 0.;
-        return res;
-    };
-    double r = 2 * Fcoeff * cadre_adapt(So, phi_integrand, 0.0, M_PI, L.nmax_phi, 0., L.rtol_phi);
+    auto pf = [&](double a_phi) -> double { return phi_integrand(C, Si, a_phi); };
+    double r = 2 * Fcoeff * cadre_adapt(So, pf, 0.0, M_PI, L.nmax_phi, 0., L.rtol_phi);
     if (So.overflow) C.status |= PBAGPU_S_HEAP_FULL;
     return r;
 }
 
-__global__ void __launch_bounds__(64) eats_kernel(const LayerArgs *__restrict__ layers, int n_layers, int n_pairs,
-                                                  const double *__restrict__ tobs, const double *__restrict__ nuobs,
-                                                  double *__restrict__ out, int *__restrict__ status) {
-    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (size_t)n_layers * n_pairs) return;
-    const int l = (int)(gid / n_pairs), k = (int)(gid % n_pairs);
-    const LayerArgs &L = layers[l];
+constexpr int EATS_WG = 256;
+constexpr size_t LANE_WS = ((Store<HO>::bytes() + Store<HI>::bytes()) + 255) / 256 * 256;
+
+extern __shared__ double lds_tab[];   // [NSLOT][nt] table columns, then the two frequency grids
+
+// Persistent kernel: each workgroup repeatedly takes a chunk (one layer, a range of
+// consecutive (t_obs, nu_obs) pairs) from a global counter, stages that layer's table
+// columns in LDS, and its lanes take pairs of the chunk one at a time from an LDS counter
+// until the chunk is exhausted.  Every lane owns a fixed interval-store block of `ws`.
+__global__ void __launch_bounds__(EATS_WG) eats_kernel(const LayerArgs *__restrict__ layers, int n_layers, int n_pairs,
+                                                       int chunk, int *__restrict__ next_chunk,
+                                                       const double *__restrict__ tobs,
+                                                       const double *__restrict__ nuobs, double *__restrict__ out,
+                                                       int *__restrict__ status, char *__restrict__ ws) {
+    __shared__ int s_chunk, s_next;
+    const int chunks_per_layer = (n_pairs + chunk - 1) / chunk;
+    const int n_chunks = chunks_per_layer * n_layers;
+    char *lane_ws = ws + ((size_t)blockIdx.x * EATS_WG + threadIdx.x) * LANE_WS;
     Store<HO> So;
     Store<HI> Si;
-    Ctx C;
-    C.L = &L;
-    C.status = 0;
-    C.sa = sin(L.theta_obs);
-    C.ca = cos(L.theta_obs);
-    const double t_obs = tobs[k];
-    C.tz = t_obs / (1.0 + L.z);
-    C.nu_obs = nuobs[k];
-    double fl = 0.;
-    if (L.i_end_r == 1) C.status |= PBAGPU_S_NONFINITE;  // a single usable step: no surface to integrate
-    if (L.i_end_r > 1) {
-        if (t_obs < C.V(PV_tburst, 0) || t_obs > C.V(PV_tburst, L.i_end_r - 1)) C.status |= PBAGPU_S_NONFINITE;
-        C.cj = false;
-        fl += integrate_theta_phi(C, So, Si);
-        if (L.counter_jet) {
-            C.cj = true;
-            fl += integrate_theta_phi(C, So, Si);
+    So.bind(lane_ws);
+    Si.bind(lane_ws + Store<HO>::bytes());
+    int cur_layer = -1;
+    for (;;) {
+        __syncthreads();  // every lane is done with the previous chunk and its staged table
+        if (threadIdx.x == 0) s_chunk = atomicAdd(next_chunk, 1);
+        __syncthreads();
+        const int c = s_chunk;
+        if (c >= n_chunks) break;
+        const int l = c / chunks_per_layer;
+        const int p0 = (c % chunks_per_layer) * chunk;
+        const int p1 = min(p0 + chunk, n_pairs);
+        const LayerArgs &L = layers[l];
+        const int nt = L.nt, nf = L.nf;
+        if (l != cur_layer) {
+            for (int sl = 0; sl < NSLOT; sl++) {
+                const double *col = L.T + (size_t)slot_var(sl) * nt;
+                for (int i = threadIdx.x; i < nt; i += blockDim.x) lds_tab[sl * nt + i] = col[i];
+            }
+            for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+                lds_tab[NSLOT * nt + i] = L.nufs[i];
+                lds_tab[NSLOT * nt + nf + i] = L.nurs[i];
+            }
+            cur_layer = l;
         }
+        if (threadIdx.x == 0) s_next = p0;
+        __syncthreads();
+        Ctx C;
+        C.L = &L;
+        C.nt = nt;
+        C.sa = sin(L.theta_obs);
+        C.ca = cos(L.theta_obs);
+        int st_acc = 0;
+        for (;;) {
+            const int k = atomicAdd(&s_next, 1);
+            if (k >= p1) break;
+            C.status = 0;
+            const double t_obs = tobs[k];
+            C.tz = t_obs / (1.0 + L.z);
+            C.nu_obs = nuobs[k];
+            double fl = 0.;
+            if (L.i_end_r == 1) C.status |= PBAGPU_S_NONFINITE;  // a single usable step: no surface to integrate
+            if (L.i_end_r > 1) {
+                if (t_obs < C.V(S_tb, 0) || t_obs > C.V(S_tb, L.i_end_r - 1)) C.status |= PBAGPU_S_NONFINITE;
+                C.cj = false;
+                fl += integrate_theta_phi(C, So, Si);
+                if (L.counter_jet) {
+                    C.cj = true;
+                    fl += integrate_theta_phi(C, So, Si);
+                }
+            }
+            out[(size_t)l * n_pairs + k] = 0. + fl;
+            st_acc |= C.status;
+        }
+        if (st_acc) atomicOr(&status[l], st_acc);
     }
-    out[gid] = 0. + fl;
-    if (C.status) atomicOr(&status[l], C.status);
 }
 
 }  // namespace eats
@@ -390,6 +470,10 @@ extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, 
     using namespace pbagpu::eats;
     if (!params || !layers || !d_table || !d_jall || !d_out || !d_status || n_layers < 1 || n_pairs < 1 || nt < 2)
         return PBAGPU_E_ARG;
+    if (sizeof(double) * ((size_t)NSLOT * nt + 2 * (size_t)nfreq) > 160 * 1024) {
+        pbagpu::set_error_msg("eats: time grid too long for the LDS-staged table (nt <= 2000)");
+        return PBAGPU_E_ARG;
+    }
     hipStream_t s = (hipStream_t)stream;
     // frequency grids of every (run, shock), built on the host like MakeLogspaceVec (utils.h:91-105)
     std::vector<double> hnu((size_t)n_runs * 2 * nfreq);
@@ -401,14 +485,16 @@ extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, 
             double ratio = std::pow(10.0, (std::log10(sp.freq2) - std::log10(sp.freq1)) / sp.nfreq);
             for (int i = 0; i < nfreq; i++) { hnu[((size_t)r * 2 + sh) * nfreq + i] = cur; cur *= ratio; }
         }
-    double *dnu = nullptr;
-    LayerArgs *dL = nullptr;
-    if (hipMalloc(&dnu, sizeof(double) * hnu.size()) != hipSuccess) return PBAGPU_E_HIP;
-    if (hipMalloc(&dL, sizeof(LayerArgs) * n_layers) != hipSuccess) { (void)hipFree(dnu); return PBAGPU_E_HIP; }
+    for (int l = 0; l < n_layers; l++) {
+        const pba_eats_layer &Y = layers[l];
+        if (Y.run < 0 || Y.run >= n_runs || Y.i_end_r > nt || Y.i_end_r < 0 || Y.pop_fs < 0) return PBAGPU_E_ARG;
+    }
+    auto *dnu = (double *)pbagpu::workspace(pbagpu::WS_EATS_NU, sizeof(double) * hnu.size());
+    auto *dL = (LayerArgs *)pbagpu::workspace(pbagpu::WS_EATS_LAYERS, sizeof(LayerArgs) * n_layers);
+    if (!dnu || !dL) return PBAGPU_E_HIP;
     std::vector<LayerArgs> hl(n_layers);
     for (int l = 0; l < n_layers; l++) {
         const pba_eats_layer &Y = layers[l];
-        if (Y.run < 0 || Y.run >= n_runs || Y.i_end_r > nt) { (void)hipFree(dnu); (void)hipFree(dL); return PBAGPU_E_ARG; }
         const pba_run_params &P = params[Y.run];
         LayerArgs &A = hl[l];
         A.T = d_table + (size_t)Y.bw * PBA_NV * nt;
@@ -422,22 +508,39 @@ extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, 
         A.nt = nt; A.nf = nfreq;
     }
     int rc = PBAGPU_OK;
-    if (hipMemcpyAsync(dnu, hnu.data(), sizeof(double) * hnu.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(dL, hl.data(), sizeof(LayerArgs) * n_layers, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemsetAsync(d_status, 0, sizeof(int) * n_layers, s) != hipSuccess)
-        rc = PBAGPU_E_HIP;
+    PBA_TRY(rc, hipMemcpyAsync(dnu, hnu.data(), sizeof(double) * hnu.size(), hipMemcpyHostToDevice, s));
+    PBA_TRY(rc, hipMemcpyAsync(dL, hl.data(), sizeof(LayerArgs) * n_layers, hipMemcpyHostToDevice, s));
+    PBA_TRY(rc, hipMemsetAsync(d_status, 0, sizeof(int) * n_layers, s));
     pbagpu::KernelTimer timer(PBAGPU_STAGE_EATS);
     if (rc == PBAGPU_OK) {
-        size_t nth = (size_t)n_layers * n_pairs;
-        timer.start(s);
-        hipLaunchKernelGGL(eats_kernel, dim3((unsigned)((nth + 63) / 64)), dim3(64), 0, s, dL, n_layers, n_pairs, d_tobs,
-                           d_nuobs, d_out, d_status);
-        timer.stop(s);
-        if (hipGetLastError() != hipSuccess) rc = PBAGPU_E_HIP;
+        const size_t lds = sizeof(double) * ((size_t)NSLOT * nt + 2 * (size_t)nfreq);
+        int dev = 0, n_cu = 0;
+        PBA_TRY(rc, hipGetDevice(&dev));
+        PBA_TRY(rc, hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+        // chunks of consecutive pairs of one layer; enough chunks to keep every workgroup busy
+        const long total = (long)n_pairs * n_layers;
+        const long per_wg = total / (4L * std::max(n_cu, 1));   // about four chunks per workgroup
+        const int chunk = (int)std::max<long>(EATS_WG, std::min<long>(std::min<long>(n_pairs, 4 * EATS_WG),
+                                                                       per_wg / EATS_WG * EATS_WG));
+        const long n_chunks = (long)((n_pairs + chunk - 1) / chunk) * n_layers;
+        (void)hipFuncSetAttribute((const void *)eats_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int per_cu = 1;
+        PBA_TRY(rc, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, eats_kernel, EATS_WG, lds));
+        const int n_wg = (int)std::min<long>(n_chunks, (long)std::max(n_cu, 1) * std::max(per_cu, 1));
+        char *ws = (char *)pbagpu::workspace(pbagpu::WS_EATS_STORE, (size_t)n_wg * EATS_WG * LANE_WS);
+        int *counter = (int *)pbagpu::workspace(pbagpu::WS_EATS_COUNTER, sizeof(int));
+        if (!ws || !counter) rc = PBAGPU_E_HIP;
+        if (rc == PBAGPU_OK) PBA_TRY(rc, hipMemsetAsync(counter, 0, sizeof(int), s));
+        if (rc == PBAGPU_OK) {
+            (void)hipFuncSetAttribute((const void *)eats_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            timer.start(s);
+            hipLaunchKernelGGL(eats_kernel, dim3((unsigned)n_wg), dim3(EATS_WG), lds, s, dL, n_layers, n_pairs, chunk,
+                               counter, d_tobs, d_nuobs, d_out, d_status, ws);
+            PBA_TRY(rc, hipGetLastError());
+            timer.stop(s);
+        }
     }
-    if (hipStreamSynchronize(s) != hipSuccess) rc = PBAGPU_E_HIP;
+    PBA_TRY(rc, hipStreamSynchronize(s));
     timer.collect();
-    (void)hipFree(dnu);
-    (void)hipFree(dL);
     return rc;
 }

@@ -22,6 +22,7 @@
 // Blumenthal-Gould / Nava kernel table (SSCKernel::evalSSCkernel, kernels.h:49-117).
 #include "pba_device.h"
 #include "timing.h"
+#include "runtime.h"
 #include <vector>
 
 namespace pbagpu {
@@ -348,34 +349,39 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
                                  const double *d_table, int bw_total, int nt, double *d_jall, double *d_fall,
                                  int *d_status, int precision, void *stream) {
     using namespace pbagpu::ele;
-    if (!params || !pops || !d_table || !d_jall || !d_status || n_pop < 1 || nt < 1) return PBAGPU_E_ARG;
+    if (!params || !pops || !d_table || !d_jall || !d_status || n_pop < 1 || nt < 1 || n_runs < 1) return PBAGPU_E_ARG;
     if (precision != 0) return PBAGPU_E_UNSUPPORTED;
     hipStream_t s = (hipStream_t)stream;
     const pba_shock_params &s0 = pops[0].shock ? params[pops[0].run].rs : params[pops[0].run].fs;
     const int ng = s0.ngam, nf = s0.nfreq;
     if (ng < 2 || ng > MAXG || nf < 2 || nf > MAXF) return PBAGPU_E_ARG;
     const double h_by_mec2 = pbagpu::kH / pbagpu::kMe / pbagpu::kC / pbagpu::kC;
-    // one grid set (and SSC table if needed) per (run, shock)
+    // one grid set (and Compton table if any population of that grid needs it) per distinct
+    // grid (gam2, freq1, freq2); runs of a batch usually share one
+    struct Key { double gam2, freq1, freq2; };
+    std::vector<Key> keys;
     std::vector<int> key_of(n_pop);
-    std::vector<int> keys;
+    std::vector<char> need_ssc;
     for (int q = 0; q < n_pop; q++) {
         const pba_population &P = pops[q];
         if (P.run < 0 || P.run >= n_runs || P.bw < 0 || P.bw >= bw_total || (P.shock != 0 && P.shock != 1))
             return PBAGPU_E_ARG;
         const pba_shock_params &sp = P.shock ? params[P.run].rs : params[P.run].fs;
         if (sp.ngam != ng || sp.nfreq != nf) return PBAGPU_E_ARG;
-        int key = P.run * 2 + P.shock;
         size_t pos = 0;
-        while (pos < keys.size() && keys[pos] != key) pos++;
-        if (pos == keys.size()) keys.push_back(key);
+        while (pos < keys.size() &&
+               !(keys[pos].gam2 == sp.gam2 && keys[pos].freq1 == sp.freq1 && keys[pos].freq2 == sp.freq2))
+            pos++;
+        if (pos == keys.size()) { keys.push_back({sp.gam2, sp.freq1, sp.freq2}); need_ssc.push_back(0); }
+        if (sp.ssc) need_ssc[pos] = 1;
         key_of[q] = (int)pos;
     }
     const size_t ktab = (size_t)nf * ng * nf;
     const size_t per_key = (size_t)ng + (ng - 1) + (ng + 1) + ng + nf;
     std::vector<double> hgrids(per_key * keys.size());
-    std::vector<char> need_ssc(keys.size(), 0);
     for (size_t kk = 0; kk < keys.size(); kk++) {
-        const pba_shock_params &sp = (keys[kk] & 1) ? params[keys[kk] / 2].rs : params[keys[kk] / 2].fs;
+        pba_shock_params sp = s0;
+        sp.gam2 = keys[kk].gam2; sp.freq1 = keys[kk].freq1; sp.freq2 = keys[kk].freq2;
         GridSet G;
         build_grids(sp, G);
         double *o = &hgrids[per_key * kk];
@@ -384,24 +390,20 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
         std::copy(G.dg.begin(), G.dg.end(), o); o += ng + 1;
         std::copy(G.dgbar.begin(), G.dgbar.end(), o); o += ng;
         std::copy(G.nu.begin(), G.nu.end(), o);
-        need_ssc[kk] = sp.ssc ? 1 : 0;
     }
-    double *dgrids = nullptr, *dK = nullptr;
-    PopDesc *dpops = nullptr;
     size_t n_ssc = 0;
     std::vector<size_t> ssc_slot(keys.size(), 0);
     for (size_t kk = 0; kk < keys.size(); kk++) if (need_ssc[kk]) ssc_slot[kk] = n_ssc++;
+    auto *dgrids = (double *)pbagpu::workspace(pbagpu::WS_ELE_GRIDS, sizeof(double) * hgrids.size());
+    auto *dK = n_ssc ? (double *)pbagpu::workspace(pbagpu::WS_ELE_KTAB, sizeof(double) * (ktab + (size_t)ng * nf) * n_ssc)
+                     : nullptr;
+    auto *dpops = (PopDesc *)pbagpu::workspace(pbagpu::WS_ELE_POPS, sizeof(PopDesc) * n_pop);
+    if (!dgrids || (n_ssc && !dK) || !dpops) return PBAGPU_E_HIP;
     int rc = PBAGPU_OK;
-    if (hipMalloc(&dgrids, sizeof(double) * hgrids.size()) != hipSuccess) return PBAGPU_E_HIP;
-    if (n_ssc && hipMalloc(&dK, sizeof(double) * (ktab + (size_t)ng * nf) * n_ssc) != hipSuccess) {
-        hipFree(dgrids);
-        return PBAGPU_E_HIP;
-    }
-    if (hipMalloc(&dpops, sizeof(PopDesc) * n_pop) != hipSuccess) { hipFree(dgrids); hipFree(dK); return PBAGPU_E_HIP; }
-    hipMemcpyAsync(dgrids, hgrids.data(), sizeof(double) * hgrids.size(), hipMemcpyHostToDevice, s);
+    PBA_TRY(rc, hipMemcpyAsync(dgrids, hgrids.data(), sizeof(double) * hgrids.size(), hipMemcpyHostToDevice, s));
     pbagpu::KernelTimer ttab(PBAGPU_STAGE_SSC_TABLE), tele(PBAGPU_STAGE_ELE);
     if (n_ssc) ttab.start(s);
-    for (size_t kk = 0; kk < keys.size(); kk++) {
+    for (size_t kk = 0; kk < keys.size() && rc == PBAGPU_OK; kk++) {
         if (!need_ssc[kk]) continue;
         double *base = dgrids + per_key * kk;
         double *K = dK + (ktab + (size_t)ng * nf) * ssc_slot[kk];
@@ -409,8 +411,10 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
         const double *e = base, *nu = base + ng + (ng - 1) + (ng + 1) + ng;
         hipLaunchKernelGGL(ssc_table_kernel, dim3((unsigned)((ktab + 255) / 256)), dim3(256), 0, s, nu, e, nf, ng,
                            h_by_mec2, K);
+        PBA_TRY(rc, hipGetLastError());
         hipLaunchKernelGGL(ssc_integral_kernel, dim3((unsigned)((ng * nf + 255) / 256)), dim3(256), 0, s, nu, K, nf, ng,
                            KI);
+        PBA_TRY(rc, hipGetLastError());
     }
     if (n_ssc) ttab.stop(s);
     std::vector<PopDesc> hp(n_pop);
@@ -422,23 +426,24 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
         PopDesc &D = hp[q];
         D.T = d_table + (size_t)P.bw * PBA_NV * nt;
         D.e = base; D.half_e = base + ng; D.dg = base + ng + (ng - 1); D.dgbar = D.dg + (ng + 1); D.nu = D.dgbar + ng;
-        D.K = need_ssc[kk] ? dK + (ktab + (size_t)ng * nf) * ssc_slot[kk] : nullptr;
+        D.K = (sp.ssc && need_ssc[kk]) ? dK + (ktab + (size_t)ng * nf) * ssc_slot[kk] : nullptr;
         D.KI = D.K ? D.K + ktab : nullptr;
         D.jall = d_jall + (size_t)q * nt * nf;
         D.fall = d_fall ? d_fall + (size_t)q * nt * ng : nullptr;
         D.p = sp.p; D.eps_e = sp.eps_e; D.shock = P.shock; D.ng = ng; D.nf = nf;
-        D.max_substeps = sp.max_substeps; D.adi_loss = sp.adi_loss_ele; D.ssc = need_ssc[kk];
+        D.max_substeps = sp.max_substeps; D.adi_loss = sp.adi_loss_ele; D.ssc = D.K != nullptr;
     }
-    hipMemcpyAsync(dpops, hp.data(), sizeof(PopDesc) * n_pop, hipMemcpyHostToDevice, s);
-    hipMemsetAsync(d_jall, 0, sizeof(double) * (size_t)n_pop * nt * nf, s);
-    if (d_fall) hipMemsetAsync(d_fall, 0, sizeof(double) * (size_t)n_pop * nt * ng, s);
-    tele.start(s);
-    hipLaunchKernelGGL(ele_kernel, dim3(n_pop), dim3(64), 0, s, dpops, nt, d_status);
-    tele.stop(s);
-    if (hipGetLastError() != hipSuccess) rc = PBAGPU_E_HIP;
-    if (hipStreamSynchronize(s) != hipSuccess) rc = PBAGPU_E_HIP;
+    PBA_TRY(rc, hipMemcpyAsync(dpops, hp.data(), sizeof(PopDesc) * n_pop, hipMemcpyHostToDevice, s));
+    PBA_TRY(rc, hipMemsetAsync(d_jall, 0, sizeof(double) * (size_t)n_pop * nt * nf, s));
+    if (d_fall) PBA_TRY(rc, hipMemsetAsync(d_fall, 0, sizeof(double) * (size_t)n_pop * nt * ng, s));
+    if (rc == PBAGPU_OK) {
+        tele.start(s);
+        hipLaunchKernelGGL(ele_kernel, dim3(n_pop), dim3(64), 0, s, dpops, nt, d_status);
+        PBA_TRY(rc, hipGetLastError());
+        tele.stop(s);
+    }
+    PBA_TRY(rc, hipStreamSynchronize(s));
     ttab.collect();
     tele.collect();
-    hipFree(dgrids); hipFree(dK); hipFree(dpops);
     return rc;
 }

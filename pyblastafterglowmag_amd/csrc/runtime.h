@@ -1,0 +1,34 @@
+// Host-side runtime shared by the stage entry points: error reporting and the library's
+// device workspace (growable buffers per device and slot, so that repeated calls do not
+// allocate).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include "../../include/pbagpu.h"
+
+namespace pbagpu {
+
+// Remember the failing call and its HIP error for pbagpu_last_error().
+void set_error(const char *what, hipError_t e);
+void set_error_msg(const char *msg);
+
+enum WsSlot {
+    WS_DYN_PARAMS = 0, WS_DYN_INIT, WS_DYN_RES,
+    WS_ELE_GRIDS, WS_ELE_KTAB, WS_ELE_POPS, WS_ELE_WORK,
+    WS_EATS_NU, WS_EATS_LAYERS, WS_EATS_STORE, WS_EATS_COUNTER,
+    WS_N
+};
+
+// Device buffer of at least `bytes` for `slot` on the current device.  Growing a slot
+// frees its old buffer (hipFree waits for the device), so a call must not hold a
+// pointer from an earlier call.  Returns nullptr on failure (error recorded).
+void *workspace(int slot, size_t bytes);
+
+}  // namespace pbagpu
+
+// Evaluate a HIP call; on failure record it and set rc to PBAGPU_E_HIP.
+#define PBA_TRY(rc, call)                                                  \
+    do {                                                                   \
+        hipError_t e_ = (call);                                            \
+        if (e_ != hipSuccess) { pbagpu::set_error(#call, e_); (rc) = PBAGPU_E_HIP; } \
+    } while (0)

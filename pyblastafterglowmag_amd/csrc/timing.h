@@ -15,8 +15,10 @@ class KernelTimer {
     explicit KernelTimer(int stage) : stage_(stage), on_(timing_enabled()) {
         if (on_ && (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess)) on_ = false;
     }
-    void start(hipStream_t s) { if (on_) (void)hipEventRecord(a_, s); }
-    void stop(hipStream_t s) { if (on_) { (void)hipEventRecord(b_, s); armed_ = true; } }
+    void start(hipStream_t s) { if (on_ && hipEventRecord(a_, s) != hipSuccess) on_ = false; }
+    void stop(hipStream_t s) {
+        if (on_ && hipEventRecord(b_, s) == hipSuccess) armed_ = true;
+    }
     // call after the stream has been synchronised
     void collect() {
         float ms = 0.f;

@@ -14,7 +14,7 @@ from . import abi, h5io, inputs
 _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libpbagpu.so")
 _lib = None
 
-EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu_timing_enable",
+EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu_last_error", "pbagpu_timing_enable",
            "pbagpu_timing_read", "pbagpu_dyn_evolve", "pbagpu_ele_evolve", "pbagpu_eats_lightcurve")
 STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE = 0, 1, 2, 3
 
@@ -22,6 +22,10 @@ STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE = 0, 1, 2, 3
 def lib():
     global _lib
     if _lib is None:
+        # torch (ROCm build) bundles a HIP runtime with the same SONAME as the system one;
+        # importing torch first makes libpbagpu.so bind to that copy, so that device memory
+        # and streams from torch and the library's launches share one runtime.
+        import torch  # noqa: F401
         if not os.path.exists(_LIB_PATH):
             raise RuntimeError(f"GPU library missing: {_LIB_PATH} (run __graft_entry__.build())")
         L = C.CDLL(_LIB_PATH)
@@ -32,6 +36,8 @@ def lib():
                                         vp, vp, C.POINTER(abi.BwResult), vp]
         L.pbagpu_ele_evolve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.Population), C.c_int, vp,
                                         C.c_int, C.c_int, vp, vp, vp, C.c_int, vp]
+        L.pbagpu_last_error.restype = C.c_char_p
+        L.pbagpu_last_error.argtypes = []
         L.pbagpu_timing_enable.argtypes = [C.c_int]
         L.pbagpu_timing_read.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]
         L.pbagpu_eats_lightcurve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsLayer), C.c_int, vp,
@@ -70,7 +76,8 @@ def _stream_ptr(stream):
 
 def _check(rc, what):
     if rc != 0:
-        raise RuntimeError(f"{what} failed rc={rc}")
+        msg = lib().pbagpu_last_error()
+        raise RuntimeError(f"{what} failed rc={rc}: {msg.decode() if msg else ''}")
 
 
 class Run:
