@@ -33,13 +33,6 @@ constexpr int kMaxStepsPerInterval = 10000000;
 
 struct MState { double acc, tcomov; };
 
-// One blastwave's state or derivative vector inside the LDS stage slots ([slot][component][lane]).
-struct SV {
-    double *p;
-    int s;
-    __device__ double &operator[](int k) const { return p[k * s]; }
-};
-
 struct Shock { double e0, etop; };
 
 struct BW {
@@ -138,9 +131,7 @@ __device__ double radiation_loss(const pba_shock_params &s, const Shock &sh, MSt
     return eps_rad;
 }
 
-// The right-hand sides read the stage input from and write the derivative to LDS; the
-// integration loop calls them from a single site.
-__device__ __forceinline__ void rhs_fs(const pba_run_params &P, const Shock &shfs, BW &b, SV Y, SV o) {
+__device__ void rhs_fs(const pba_run_params &P, const Shock &shfs, BW &b, const double *Y, double *o) {
     double R = Y[iR], Gamma = Y[iGamma], Eint2 = Y[iEint2], theta = Y[itheta], M2 = Y[iM2];
     if (Gamma < 1 || M2 < 0 || theta < 0) b.status |= PBAGPU_S_NONFINITE;
     double rho = b.rho_def / b.M0;
@@ -186,7 +177,7 @@ __device__ __forceinline__ void rhs_fs(const pba_run_params &P, const Shock &shf
     o[iEint3] = 0.; o[iRrsh] = 0.; o[iErad3] = 0.; o[iEsh3] = 0.; o[iEad3] = 0.; o[iM3] = 0.; o[ideltaR4] = 0.;
 }
 
-__device__ __forceinline__ void rhs_fsrs(const pba_run_params &P, const Shock &shfs, BW &b, SV Y, SV o) {
+__device__ void rhs_fsrs(const pba_run_params &P, const Shock &shfs, BW &b, const double *Y, double *o) {
     double R = Y[iR], Gamma = Y[iGamma], Eint2 = Y[iEint2], Eint3 = Y[iEint3], theta = Y[itheta];
     double M2 = Y[iM2], M3 = Y[iM3], deltaR4 = Y[ideltaR4];
     if (Gamma < 1.) Gamma = (b.prev_idx_x == 0) ? b.Gamma0 : 1. + 1.e-5;
@@ -283,6 +274,13 @@ __device__ __forceinline__ void rhs_fsrs(const pba_run_params &P, const Shock &s
     o[ideltaR4] = dRdt * ddR4;
 }
 
+__device__ __forceinline__ void rhs(const pba_run_params &P, const Shock &shfs, BW &b, bool active, const double *Y,
+                                    double *o) {
+    if (!active || b.end_evolution) return;  // derivative entries of inactive blastwaves keep their (zero) values
+    if (b.fsrs) rhs_fsrs(P, shfs, b, Y, o);
+    else rhs_fs(P, shfs, b, Y, o);
+}
+
 struct Lim { double B, gm, gM, gc, Theta, z_cool, nprime, acc; bool ok; };
 
 __device__ Lim ele_limits(const pba_shock_params &s, const Shock &sh, MState &st, double U_p, double Gsh, double t_e,
@@ -346,9 +344,7 @@ __device__ double ctheta_of(double theta, int ilayer, int nlayers) {
     return 0.5 * (xx1 + xx2);
 }
 
-// Out of line: called once per output step from two places.
-__device__ __noinline__ void post_step(const pba_run_params &P, const Shock &shfs, const Shock &shrs, BW &b,
-                                       const double *Yc,
+__device__ void post_step(const pba_run_params &P, const Shock &shfs, const Shock &shrs, BW &b, const double *Yc,
                           int it, double tb, double *T, int nt) {
     if (b.end_evolution) return;
     auto W = [&](int v, double x) { T[(size_t)v * nt + it] = x; };
@@ -429,33 +425,31 @@ __constant__ double c_[12] = {0., 0.526001519587677318785587544488E-01, 0.789002
                               0.333333333333333333333333333333E+00, 0.25E+00, 0.307692307692307692307692307692E+00,
                               0.651282051282051282051282051282E+00, 0.6E+00, 0.857142857142857142857142857142E+00, 1.0};
 }
-// a[s][j]: coefficient of k_{j+1} in the input of stage s+1 (s = 1..11 gives k2..k12)
-__constant__ double A_[12][11] = {
-    {0},
-    {5.26001519587677318785587544488E-2},
-    {1.97250569845378994544595329183E-2, 5.91751709536136983633785987549E-2},
-    {2.95875854768068491816892993775E-2, 0., 8.87627564304205475450678981324E-2},
-    {2.41365134159266685502369798665E-1, 0., -8.84549479328286085344864962717E-1, 9.24834003261792003115737966543E-1},
-    {3.7037037037037037037037037037E-2, 0., 0., 1.70828608729473871279604482173E-1, 1.25467687566822425016691814123E-1},
-    {3.7109375E-2, 0., 0., 1.70252211019544039314978060272E-1, 6.02165389804559606850219397283E-2, -1.7578125E-2},
-    {3.70920001185047927108779319836E-2, 0., 0., 1.70383925712239993810214054705E-1,
-     1.07262030446373284651809199168E-1, -1.53194377486244017527936158236E-2, 8.27378916381402288758473766002E-3},
-    {6.24110958716075717114429577812E-1, 0., 0., -3.36089262944694129406857109825, -8.68219346841726006818189891453E-1,
-     2.75920996994467083049415600797E+1, 2.01540675504778934086186788979E+1, -4.34898841810699588477366255144E+1},
-    {4.77662536438264365890433908527E-1, 0., 0., -2.48811461997166764192642586468, -5.90290826836842996371446475743E-1,
-     2.12300514481811942347288949897E+1, 1.52792336328824235832596922938E+1, -3.32882109689848629194453265587E+1,
-     -2.03312017085086261358222928593E-2},
-    {-9.3714243008598732571704021658E-1, 0., 0., 5.18637242884406370830023853209, 1.09143734899672957818500254654,
-     -8.14978701074692612513997267357, -1.85200656599969598641566180701E+1, 2.27394870993505042818970056734E+1,
-     2.49360555267965238987089396762, -3.0467644718982195003823669022},
-    {2.27331014751653820792359768449, 0., 0., -1.05344954667372501984066689879E+1, -2.00087205822486249909675718444,
-     -1.79589318631187989172765950534E+1, 2.79488845294199600508499808837E+1, -2.85899827713502369474065508674,
-     -8.87285693353062954433549289258, 1.23605671757943030647266201528E+1, 6.43392746015763530355970484046E-1},
-};
-// LDS slot holding k_{j+1}: k2, k3 and k5 share one slot, k4 and k12 another (their lifetimes
-// do not overlap), so nine slots hold the twelve stage derivatives.
-__constant__ int SLOT_[12] = {0, 1, 1, 2, 1, 3, 4, 5, 6, 7, 8, 2};
-constexpr int NSLOT = 9;
+constexpr double a21 = 5.26001519587677318785587544488E-2, a31 = 1.97250569845378994544595329183E-2,
+                 a32 = 5.91751709536136983633785987549E-2, a41 = 2.95875854768068491816892993775E-2,
+                 a43 = 8.87627564304205475450678981324E-2, a51 = 2.41365134159266685502369798665E-1,
+                 a53 = -8.84549479328286085344864962717E-1, a54 = 9.24834003261792003115737966543E-1,
+                 a61 = 3.7037037037037037037037037037E-2, a64 = 1.70828608729473871279604482173E-1,
+                 a65 = 1.25467687566822425016691814123E-1, a71 = 3.7109375E-2,
+                 a74 = 1.70252211019544039314978060272E-1, a75 = 6.02165389804559606850219397283E-2,
+                 a76 = -1.7578125E-2, a81 = 3.70920001185047927108779319836E-2,
+                 a84 = 1.70383925712239993810214054705E-1, a85 = 1.07262030446373284651809199168E-1,
+                 a86 = -1.53194377486244017527936158236E-2, a87 = 8.27378916381402288758473766002E-3,
+                 a91 = 6.24110958716075717114429577812E-1, a94 = -3.36089262944694129406857109825,
+                 a95 = -8.68219346841726006818189891453E-1, a96 = 2.75920996994467083049415600797E+1,
+                 a97 = 2.01540675504778934086186788979E+1, a98 = -4.34898841810699588477366255144E+1,
+                 aA1 = 4.77662536438264365890433908527E-1, aA4 = -2.48811461997166764192642586468,
+                 aA5 = -5.90290826836842996371446475743E-1, aA6 = 2.12300514481811942347288949897E+1,
+                 aA7 = 1.52792336328824235832596922938E+1, aA8 = -3.32882109689848629194453265587E+1,
+                 aA9 = -2.03312017085086261358222928593E-2, aB1 = -9.3714243008598732571704021658E-1,
+                 aB4 = 5.18637242884406370830023853209, aB5 = 1.09143734899672957818500254654,
+                 aB6 = -8.14978701074692612513997267357, aB7 = -1.85200656599969598641566180701E+1,
+                 aB8 = 2.27394870993505042818970056734E+1, aB9 = 2.49360555267965238987089396762,
+                 aBA = -3.0467644718982195003823669022, aC1 = 2.27331014751653820792359768449,
+                 aC4 = -1.05344954667372501984066689879E+1, aC5 = -2.00087205822486249909675718444,
+                 aC6 = -1.79589318631187989172765950534E+1, aC7 = 2.79488845294199600508499808837E+1,
+                 aC8 = -2.85899827713502369474065508674, aC9 = -8.87285693353062954433549289258,
+                 aCA = 1.23605671757943030647266201528E+1, aCB = 6.43392746015763530355970484046E-1;
 constexpr double b1 = 5.42937341165687622380535766363E-2, b6 = 4.45031289275240888144113950566,
                  b7 = 1.89151789931450038304281599044, b8 = -5.8012039600105847814672114227,
                  b9 = 3.1116436695781989440891606237E-1, bA = -1.52160949662516078556178806805E-1,
@@ -483,7 +477,6 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
                                                  const double *__restrict__ tgrid, double *__restrict__ table,
                                                  pba_bw_result *__restrict__ res) {
     __shared__ double red[64];
-    extern __shared__ double sk[];   // stage derivatives [NSLOT][NA][bw_per_run]
     const int run = blockIdx.x;
     const int lane = threadIdx.x;
     const bool active = lane < bw_per_run;
@@ -496,16 +489,9 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
         shrs.e0 = pow(stepr, 0.); shrs.etop = pow(stepr, (double)(P.rs.ngam - 1));
     }
     BW b{};
-    double Y[NA];
-    for (int k = 0; k < NA; k++) Y[k] = 0.;
-    const int bpr = bw_per_run;
-    // stage slot q, component k of this lane
-    auto K = [&](int q, int k) -> double & { return sk[(q * NA + k) * bpr + lane]; };
-    if (active)
-        for (int q = 0; q <= NSLOT; q++)
-            for (int k = 0; k < NA; k++) K(q, k) = 0.;
-    // stage input x lives in the extra slot NSLOT
-    auto X = [&](int k) -> double & { return K(NSLOT, k); };
+    double Y[NA], x[NA], k1[NA], k2[NA], k4[NA], k6[NA], k7[NA], k8[NA], k9[NA], kA[NA], kB[NA];
+    for (int k = 0; k < NA; k++) { Y[k] = 0.; x[k] = 0.; k1[k] = 0.; k2[k] = 0.; k4[k] = 0.; k6[k] = 0.; k7[k] = 0.;
+                                   k8[k] = 0.; k9[k] = 0.; kA[k] = 0.; kB[k] = 0.; }
     const size_t bw_global = (size_t)run * bw_per_run + lane;
     {   // clear this run's tables, all lanes striding over contiguous memory
         double *Tr = table + (size_t)run * bw_per_run * PBA_NV * nt;
@@ -546,68 +532,25 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
             b.Rw0 = .5 * Y[iR];
             b.Wtt0 = Y[itt];
         }
-        double Yc[NA];
-        for (int k = 0; k < NA; k++) Yc[k] = Y[k];
-        post_step(P, shfs, shrs, b, Yc, 0, tgrid[0], T, nt);
+        post_step(P, shfs, shrs, b, Y, 0, tgrid[0], T, nt);
     }
     const double rtol = P.rtol, atol = P.rtol;
     const double nsz = (double)(NEQ_REF * bw_per_run);
     double time = P.tb0;
     int nsteps = 0, nrhs = 0;
-    // The integration is written as a state machine so that the right-hand side has a
-    // single call site (one inlined copy of rhs_fs / rhs_fsrs keeps the kernel compact):
-    //   START  k1 at the start of an output interval     (advanceTimeSubStep, model_evolve.h:439-566)
-    //   PROBE  derivative at Y + h0 k1 for the initial step (ode_solvers.h:1596-1628)
-    //   STAGE  stages k2..k12 of one step attempt         (ode_solvers.h:1434-1479)
-    //   FSAL   derivative at the accepted point, next k1
-    enum { START, PROBE, STAGE, FSAL };
-    int phase = START, st = 1, it = 1, guard = 0;
-    double timeN = 0., hmax = 0., h = 0., h0 = 0., dnf = 0., hnew = 0.;
-    bool reject = false, last = false;
-    while (it < nt) {
-        int q = 0;
-        if (phase == START) {
-            timeN = time + (tgrid[it] - tgrid[it - 1]);
-            hmax = fabs(timeN - time);
-            for (int k = 0; k < NA; k++) X(k) = Y[k];
-            q = 0;
-        } else if (phase == PROBE) {
-            q = 1;
-        } else if (phase == STAGE) {
-            // x = Y + h * sum_j a[st][j] k_j
-            for (int k = 0; k < NA; k++) {
-                double acc = 0.;
-                bool first = true;
-                for (int j = 0; j < st; j++) {
-                    const double a = A_[st][j];
-                    if (a == 0.) continue;
-                    const double t = a * K(SLOT_[j], k);
-                    acc = first ? t : acc + t;
-                    first = false;
-                }
-                X(k) = Y[k] + h * acc;
-            }
-            q = SLOT_[st];
-        } else {
-            q = 0;
-        }
-        // the one right-hand-side evaluation of the loop (inactive lanes keep their zeros)
-        if (active && !b.end_evolution) {
-            SV xv{&K(NSLOT, 0), bpr}, ov{&K(q, 0), bpr};
-            if (b.fsrs) rhs_fsrs(P, shfs, b, xv, ov);
-            else rhs_fs(P, shfs, b, xv, ov);
-        }
-        nrhs++;
-        bool end_interval = false;
-        if (phase == START) {
-            double dny = 0.;
-            dnf = 0.;
+    for (int it = 1; it < nt; it++) {
+        const double timeN = time + (tgrid[it] - tgrid[it - 1]);
+        const double hmax = fabs(timeN - time);
+        rhs(P, shfs, b, active, Y, k1); nrhs++;
+        // initial step (ode_solvers.h:1596-1628)
+        double h;
+        {
+            double dnf = 0., dny = 0.;
             if (active) {
                 for (int k = 0; k < NA; k++) {
-                    double sk_ = atol + rtol * fabs(Y[k]);
-                    double k1k = K(0, k);
-                    dnf += (k1k / sk_) * (k1k / sk_);
-                    dny += (Y[k] / sk_) * (Y[k] / sk_);
+                    double sk = atol + rtol * fabs(Y[k]);
+                    dnf += (k1[k] / sk) * (k1[k] / sk);
+                    dny += (Y[k] / sk) * (Y[k] / sk);
                 }
                 double skw = atol + rtol * fabs(b.Rw0);
                 dny += (b.Rw0 / skw) * (b.Rw0 / skw);
@@ -616,106 +559,112 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
             }
             dnf = ordered_sum(dnf, red, bw_per_run);
             dny = ordered_sum(dny, red, bw_per_run);
-            h0 = (dnf <= 1e-10 || dny <= 1e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
-            h0 = fmin(h0, hmax);
-            for (int k = 0; k < NA; k++) X(k) = Y[k] + h0 * K(0, k);
-            phase = PROBE;
-        } else if (phase == PROBE) {
+            double h1 = (dnf <= 1e-10 || dny <= 1e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
+            h1 = fmin(h1, hmax);
+            for (int k = 0; k < NA; k++) x[k] = Y[k] + h1 * k1[k];
+            rhs(P, shfs, b, active, x, k2); nrhs++;
             double n2 = 0.;
             if (active)
                 for (int k = 0; k < NA; k++) {
-                    double sk_ = atol + rtol * x86_int_abs(Y[k]);
-                    double z = (K(1, k) - K(0, k)) / sk_;
+                    double sk = atol + rtol * x86_int_abs(Y[k]);
+                    double z = (k2[k] - k1[k]) / sk;
                     n2 += z * z;
                 }
             n2 = ordered_sum(n2, red, bw_per_run);
-            n2 = sqrt(n2) / h0;
+            n2 = sqrt(n2) / h1;
             double der12 = fmax(n2, sqrt(dnf));
-            double h1 = (der12 <= 1e-15) ? fmax(1e-6, fabs(h0) * 1e-3) : pow(0.01 / der12, 1.0 / 8);
-            h = fmin(fmin(100 * h0, h1), hmax);
-            reject = false; last = false; guard = 0;
+            double h2 = (der12 <= 1e-15) ? fmax(1e-6, fabs(h1) * 1e-3) : pow(0.01 / der12, 1.0 / 8);
+            h = fmin(fmin(100 * h1, h2), hmax);
+        }
+        bool reject = false, last = false;
+        for (int guard = 0; guard < kMaxStepsPerInterval; guard++) {
             if ((time + 1.01 * h - timeN) > 0.0) { h = timeN - time; last = true; }
-            st = 1;
-            phase = STAGE;
-        } else if (phase == STAGE) {
-            if (st < 11) {
-                st++;
-            } else {
-                double e1s = 0., e2s = 0.;
-                for (int k = 0; k < NA; k++) {
-                    const double k1 = K(0, k), k6 = K(3, k), k7 = K(4, k), k8 = K(5, k), k9 = K(6, k), kA = K(7, k),
-                                 kB = K(8, k), kC = K(2, k);
-                    double slope = b1 * k1 + b6 * k6 + b7 * k7 + b8 * k8 + b9 * k9 + bA * kA + bB * kB + bC * kC;
-                    double yn = Y[k] + h * slope;
-                    double sk_ = atol + rtol * fmax(fabs(Y[k]), fabs(yn));
-                    double e1 = er1 * k1 + er6 * k6 + er7 * k7 + er8 * k8 + er9 * k9 + erA * kA + erB * kB + erC * kC;
-                    double e2 = slope - bhh1 * k1 - bhh9 * k9 - bhhC * kC;
-                    if (active) { e1s += (e1 / sk_) * (e1 / sk_); e2s += (e2 / sk_) * (e2 / sk_); }
-                    X(k) = yn;
-                }
-                e1s = ordered_sum(e1s, red, bw_per_run);
-                e2s = ordered_sum(e2s, red, bw_per_run);
-                double deno = e1s + 0.01 * e2s;
-                if (deno <= 0.0) deno = 1.0;
-                double err = fabs(h) * e1s / sqrt(nsz * deno);
-                double fac11 = pow(err, 1.0 / 8.0);
-                double fac = fac11 / (1.0 * 0.9);
-                fac = fac < 1.0 / 6.0 ? 1.0 / 6.0 : (fac > 1.0 / 0.333 ? 1.0 / 0.333 : fac);
-                hnew = h / fac;
-                if (err > 1.0) {
-                    h /= fmin(1.0 / 0.333, fac11 / 0.9);
-                    reject = true;
-                    last = false;
-                    if (++guard >= kMaxStepsPerInterval) { b.status |= PBAGPU_S_NMAX; end_interval = true; }
-                    else {
-                        if ((time + 1.01 * h - timeN) > 0.0) { h = timeN - time; last = true; }
-                        st = 1;
-                    }
-                } else {
-                    phase = FSAL;
-                }
+            // stages (ode_solvers.h:1434-1479); k3 and k5 share k2's storage, kC shares k4's
+            double *k3 = k2, *k5 = k2, *kC = k4;
+            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a21 * k1[k]);
+            rhs(P, shfs, b, active, x, k2);
+            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a31 * k1[k] + a32 * k2[k]);
+            rhs(P, shfs, b, active, x, k3);
+            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a41 * k1[k] + a43 * k3[k]);
+            rhs(P, shfs, b, active, x, k4);
+            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a51 * k1[k] + a53 * k3[k] + a54 * k4[k]);
+            rhs(P, shfs, b, active, x, k5);
+            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a61 * k1[k] + a64 * k4[k] + a65 * k5[k]);
+            rhs(P, shfs, b, active, x, k6);
+            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a71 * k1[k] + a74 * k4[k] + a75 * k5[k] + a76 * k6[k]);
+            rhs(P, shfs, b, active, x, k7);
+            for (int k = 0; k < NA; k++)
+                x[k] = Y[k] + h * (a81 * k1[k] + a84 * k4[k] + a85 * k5[k] + a86 * k6[k] + a87 * k7[k]);
+            rhs(P, shfs, b, active, x, k8);
+            for (int k = 0; k < NA; k++)
+                x[k] = Y[k] + h * (a91 * k1[k] + a94 * k4[k] + a95 * k5[k] + a96 * k6[k] + a97 * k7[k] + a98 * k8[k]);
+            rhs(P, shfs, b, active, x, k9);
+            for (int k = 0; k < NA; k++)
+                x[k] = Y[k] + h * (aA1 * k1[k] + aA4 * k4[k] + aA5 * k5[k] + aA6 * k6[k] + aA7 * k7[k] + aA8 * k8[k] +
+                                   aA9 * k9[k]);
+            rhs(P, shfs, b, active, x, kA);
+            for (int k = 0; k < NA; k++)
+                x[k] = Y[k] + h * (aB1 * k1[k] + aB4 * k4[k] + aB5 * k5[k] + aB6 * k6[k] + aB7 * k7[k] + aB8 * k8[k] +
+                                   aB9 * k9[k] + aBA * kA[k]);
+            rhs(P, shfs, b, active, x, kB);
+            for (int k = 0; k < NA; k++)
+                x[k] = Y[k] + h * (aC1 * k1[k] + aC4 * k4[k] + aC5 * k5[k] + aC6 * k6[k] + aC7 * k7[k] + aC8 * k8[k] +
+                                   aC9 * k9[k] + aCA * kA[k] + aCB * kB[k]);
+            rhs(P, shfs, b, active, x, kC);
+            nrhs += 11;
+            double e1s = 0., e2s = 0.;
+            for (int k = 0; k < NA; k++) {
+                double slope = b1 * k1[k] + b6 * k6[k] + b7 * k7[k] + b8 * k8[k] + b9 * k9[k] + bA * kA[k] + bB * kB[k] +
+                               bC * kC[k];
+                double yn = Y[k] + h * slope;
+                double sk = atol + rtol * fmax(fabs(Y[k]), fabs(yn));
+                double e1 = er1 * k1[k] + er6 * k6[k] + er7 * k7[k] + er8 * k8[k] + er9 * k9[k] + erA * kA[k] +
+                            erB * kB[k] + erC * kC[k];
+                double e2 = slope - bhh1 * k1[k] - bhh9 * k9[k] - bhhC * kC[k];
+                if (active) { e1s += (e1 / sk) * (e1 / sk); e2s += (e2 / sk) * (e2 / sk); }
+                x[k] = yn;
             }
-        } else {  // FSAL: the new k1 is in slot 0
-            for (int k = 0; k < NA; k++) Y[k] = X(k);
-            time += h;
-            nsteps++;
-            if (last) end_interval = true;
-            else {
+            e1s = ordered_sum(e1s, red, bw_per_run);
+            e2s = ordered_sum(e2s, red, bw_per_run);
+            double deno = e1s + 0.01 * e2s;
+            if (deno <= 0.0) deno = 1.0;
+            double err = fabs(h) * e1s / sqrt(nsz * deno);
+            double fac11 = pow(err, 1.0 / 8.0);
+            double fac = fac11 / (1.0 * 0.9);
+            fac = fac < 1.0 / 6.0 ? 1.0 / 6.0 : (fac > 1.0 / 0.333 ? 1.0 / 0.333 : fac);
+            double hnew = h / fac;
+            if (err > 1.0) {
+                h /= fmin(1.0 / 0.333, fac11 / 0.9);
+                reject = true;
+                last = false;
+            } else {
+                rhs(P, shfs, b, active, x, k2); nrhs++;   // kD shares k2's storage
+                for (int k = 0; k < NA; k++) { k1[k] = k2[k]; Y[k] = x[k]; }
+                time += h;
+                nsteps++;
+                if (last) break;
                 if (fabs(hnew) > hmax) hnew = hmax;
                 if (reject) hnew = fmin(fabs(hnew), fabs(h));
                 reject = false;
                 h = hnew;
-                if (++guard >= kMaxStepsPerInterval) { b.status |= PBAGPU_S_NMAX; end_interval = true; }
-                else {
-                    if ((time + 1.01 * h - timeN) > 0.0) { h = timeN - time; last = true; }
-                    st = 1;
-                    phase = STAGE;
-                }
             }
+            if (guard == kMaxStepsPerInterval - 1) b.status |= PBAGPU_S_NMAX;
         }
-        if (end_interval) {
-            if (active && !b.end_evolution) {
-                double g = Y[iGamma], be = beta_from_gamma(g), E2 = Y[iEint2];
-                if ((E2 <= 0.) || !isfinite(E2) || (g < 1.) || (be < 0) || (fabs(be) < 1.e-6) || !isfinite(be))
-                    b.status |= PBAGPU_S_TERMINATED;
-                if (!b.end_spreading && Y[itheta] > 0.99 * b.theta_max) b.end_spreading = 1;
-                if (!(b.shutOff || !b.do_rs)) {
-                    double alpha_of = P.tprompt * b.beta0 * kC;
-                    double rho4f = b.M0 / (2.0 * alpha_of * kPi * (1.0 - cos(b.theta_b0))) / (Y[iR] * Y[iR]);
-                    double rho4 = P.exponential_rho4 ? rho4f * exp(-Y[ideltaR4] / alpha_of) : rho4f;
-                    if ((Y[iGamma] < b.Gamma0) && (rho4 < P.rs_shutoff_rho || Y[iM3] >= 1.)) b.shutOff = 1;
-                }
-                if (g < 1. || Y[itheta] < b.theta_b0 * 0.99 || Y[itt] <= 0.) b.status |= PBAGPU_S_NONFINITE;
+        if (active && !b.end_evolution) {
+            double g = Y[iGamma], be = beta_from_gamma(g), E2 = Y[iEint2];
+            if ((E2 <= 0.) || !isfinite(E2) || (g < 1.) || (be < 0) || (fabs(be) < 1.e-6) || !isfinite(be))
+                b.status |= PBAGPU_S_TERMINATED;
+            if (!b.end_spreading && Y[itheta] > 0.99 * b.theta_max) b.end_spreading = 1;
+            if (!(b.shutOff || !b.do_rs)) {
+                double alpha_of = P.tprompt * b.beta0 * kC;
+                double rho4f = b.M0 / (2.0 * alpha_of * kPi * (1.0 - cos(b.theta_b0))) / (Y[iR] * Y[iR]);
+                double rho4 = P.exponential_rho4 ? rho4f * exp(-Y[ideltaR4] / alpha_of) : rho4f;
+                if ((Y[iGamma] < b.Gamma0) && (rho4 < P.rs_shutoff_rho || Y[iM3] >= 1.)) b.shutOff = 1;
             }
-            b.prev_idx_x = it;
-            if (active) {
-                double Yc[NA];
-                for (int k = 0; k < NA; k++) Yc[k] = Y[k];
-                post_step(P, shfs, shrs, b, Yc, it, tgrid[it], T, nt);
-            }
-            it++;
-            phase = START;
+            if (g < 1. || Y[itheta] < b.theta_b0 * 0.99 || Y[itt] <= 0.) b.status |= PBAGPU_S_NONFINITE;
         }
+        b.prev_idx_x = it;
+        if (active) post_step(P, shfs, shrs, b, Y, it, tgrid[it], T, nt);
     }
     if (active) {
         pba_bw_result r;
@@ -743,8 +692,7 @@ extern "C" int pbagpu_dyn_evolve(const pba_run_params *params, int n_runs, int b
     pbagpu::KernelTimer timer(PBAGPU_STAGE_DYN);
     if (rc == PBAGPU_OK) {
         timer.start(s);
-        const size_t lds = sizeof(double) * (pbagpu::dyn::NSLOT + 1) * pbagpu::dyn::NA * bw_per_run;
-        hipLaunchKernelGGL(pbagpu::dyn::dyn_kernel, dim3(n_runs), dim3(64), lds, s, dP, dI, bw_per_run, nt, d_tgrid,
+        hipLaunchKernelGGL(pbagpu::dyn::dyn_kernel, dim3(n_runs), dim3(64), 0, s, dP, dI, bw_per_run, nt, d_tgrid,
                            d_table, dR);
         PBA_TRY(rc, hipGetLastError());
         timer.stop(s);

@@ -88,150 +88,262 @@ struct Store {
     __device__ double total_err() const { double s = 0.; for (int i = 0; i < N; i++) s += pool[hslot[i]].err; return s; }
 };
 
-template <class F>
-__device__ int cadre_process(F &f, Iv *i) {
-    const double scaleTol = 0.2;
-    int count = 0;
-    if (i->refinement < 4) {
-        double fa = i->fa, fm = i->fm, fb = i->fb, fl, fr;
-        if (i->refinement == 0) {
-            fl = f(0.75 * i->a + 0.25 * i->b); count++;
-            fr = f(0.25 * i->a + 0.75 * i->b); count++;
-            i->fl = fl; i->fr = fr;
-        } else {
-            fl = i->fl; fr = i->fr;
-            i->refinement = 0;
-        }
-        double hh = 0.25 * (i->b - i->a);
-        double T0 = 2 * hh * (fa + fb);
-        double T1 = 0.5 * T0 + 2 * hh * fm;
-        double T2 = 0.5 * T1 + hh * (fl + fr);
-        double R0 = (T1 - T0) / (T2 - T1);
-        if (R0 > 4 - scaleTol && R0 < 4 + scaleTol) {
-            double fll = f(0.875 * i->a + 0.125 * i->b);
-            double flr = f(0.625 * i->a + 0.375 * i->b);
-            double frl = f(0.375 * i->a + 0.625 * i->b);
-            double frr = f(0.125 * i->a + 0.875 * i->b);
-            count += 4;
-            i->fll = fll; i->flr = flr; i->frl = frl; i->frr = frr;
-            i->refinement = 1;
-            double T3 = 0.5 * T2 + 0.5 * hh * (fll + flr + frl + frr);
-            double R1 = (T2 - T1) / (T3 - T2);
-            if (R1 > 4 - scaleTol && R1 < 4 + scaleTol) {
-                double R11 = (4 * T1 - T0) / 3.0;
-                double R21 = (4 * T2 - T1) / 3.0;
-                double R22 = (16 * R21 - R11) / 15.0;
-                i->fa = T3;
-                i->fll = (4 * T3 - T2) / 3.0;
-                i->fl = (16 * i->fll - R21) / 15.0;
-                i->flr = (64 * i->fl - R22) / 63.0;
-                i->I = i->flr;
-                i->err = fabs((i->fl - R22) / 63.0);
-                i->refinement = 4;
-            } else {
-                double errt = (T3 - T2) / 3.0;
-                i->err = fabs(errt);
-                i->I = T3 + errt;
-            }
-        } else if (R0 >= 4 + scaleTol || R0 < 2 || R0 != R0) {
-            double e10 = fabs(T1 - T0), e21 = fabs(T2 - T1);
-            i->err = e10 > e21 ? e10 : e21;
-            i->I = T2;
-        } else {
-            double err = -(T2 - T1) * (T2 - T1) / (T2 - 2 * T1 + T0);
-            i->err = fabs(err);
-            i->I = (fabs(T2 + err) < 1.0e-14 * fabs(T2)) ? 1.0e-14 * T2 : T2 + err;
-        }
-        return count;
-    }
-    int nint = 1 << i->refinement;
-    double hh = (i->b - i->a) / nint;
-    double sum = 0;
-    for (int n = 1; n < nint; n += 2) { sum += f(i->a + hh * n); count++; }
-    double T = 0.5 * i->fa + hh * sum;
-    double Rnm = i->fa, Rnmp = i->fll;
-    i->fa = T;
-    i->fll = (4.0 * i->fa - Rnm) / 3.0;
-    if (i->refinement == 1) { i->I = i->fll; i->err = fabs((i->fa - Rnm) / 3.0); i->refinement++; return count; }
-    Rnm = Rnmp; Rnmp = i->fl;
-    i->fl = (16 * i->fll - Rnm) / 15.0;
-    if (i->refinement == 2) { i->I = i->fl; i->err = fabs((i->fll - Rnm) / 15.0); i->refinement++; return count; }
-    Rnm = Rnmp; Rnmp = i->flr;
-    i->flr = (64 * i->fl - Rnm) / 63.0;
-    if (i->refinement == 3) { i->I = i->flr; i->err = fabs((i->fl - Rnm) / 63.0); i->refinement++; return count; }
-    Rnm = Rnmp; Rnmp = i->fm;
-    i->fm = (256 * i->flr - Rnm) / 255.0;
-    if (i->refinement == 4) { i->I = i->fm; i->err = fabs((i->flr - Rnm) / 255.0); i->refinement++; return count; }
-    Rnm = Rnmp; Rnmp = i->frl;
-    i->frl = (1024 * i->fm - Rnm) / 1023.0;
-    if (i->refinement == 5) { i->I = i->frl; i->err = fabs((i->fm - Rnm) / 1023.0); i->refinement++; return count; }
-    Rnm = Rnmp; Rnmp = i->fr;
-    i->fr = (4096 * i->frl - Rnm) / 4095.0;
-    if (i->refinement == 6) { i->I = i->fr; i->err = fabs((i->frl - Rnm) / 4095.0); i->refinement++; return count; }
-    Rnm = Rnmp; Rnmp = i->frr;
-    i->frr = (16384 * i->fr - Rnm) / 16383.0;
-    if (i->refinement == 7) { i->I = i->frr; i->err = fabs((i->fr - Rnm) / 16383.0); i->refinement++; return count; }
-    Rnm = Rnmp; Rnmp = i->fb;
-    i->fb = (65536 * i->frr - Rnm) / 65535.0;
-    if (i->refinement == 8) { i->I = i->fb; i->err = fabs((i->frr - Rnm) / 65535.0); i->refinement++; return count; }
-    Rnm = Rnmp;
-    i->I = (262144 * i->fb - Rnm) / 262143.0;
-    i->err = fabs((i->fb - Rnm) / 262143.0);
-    i->refinement++;
-    return count;
-}
+// Resumable CADRE: the reference's cadre_adapt / cadreProcessInterval / cadreSplitInterval
+// (src/utilitites/quadratures.h:1613-2115, as used by src/eats.h:460-684) rewritten as a
+// state machine that hands every integrand sample to its caller.  The caller evaluates the
+// integrand for all lanes of a wavefront at one place in the code, so lanes whose adaptive
+// paths differ still sample the integrand together; the bookkeeping between samples
+// (heap, Romberg updates) is short.  Samples are requested in the reference's order and
+// the arithmetic of every update is the reference's, so the result is the same number.
+template <int H>
+struct Cadre {
+    Store<H> S;
+    double atol, rtol;
+    int Nmax;
+    int n, num_intervals, num_iterations, last_check;
+    double I, err;
+    int pc;        // main program
+    int ppc;       // interval-processing program
+    int ret;       // main state to resume after processing
+    int cur;       // slot being processed
+    int sl, s2;    // slots of the interval popped / created by a split
+    int cnt;       // samples taken while processing `cur`
+    double oldI, olderr;
+    double fl, fr, fll, flr, frl, frr, sum, hh;
+    int nint, nidx;
+    double x;      // requested abscissa
+    double result;
 
-template <int H, class F>
-__device__ double cadre_adapt(Store<H> &S, F &f, double xa, double xb, int Nmax, double atol, double rtol) {
-    S.reset();
-    int s0 = S.alloc();
-    Iv &i0 = S.pool[s0];
-    i0.a = xa; i0.b = xb; i0.I = 0; i0.err = 0;
-    i0.fll = 0; i0.flr = 0; i0.frl = 0; i0.frr = 0; i0.fl = 0; i0.fr = 0; i0.refinement = 0;
-    i0.fa = f(xa);
-    i0.fb = f(xb);
-    i0.fm = f(0.5 * (xa + xb));
-    int n = 3;
-    n += cadre_process(f, &i0);
-    S.push(s0);
-    double I = i0.I, err = i0.err;
-    int num_intervals = 1, num_iterations = 0, last_check = 0;
-    while (n < Nmax && err > atol + rtol * fabs(I) && !S.overflow) {
-        int sl = S.pop();
-        Iv &iv = S.pool[sl];
-        if (iv.refinement >= 4) {
-            double oldI = iv.I, olderr = iv.err;
-            n += cadre_process(f, &iv);
-            S.push(sl);
-            err += iv.err - olderr;
-            I += iv.I - oldI;
-        } else {
-            Iv p = iv;
-            int s2 = S.alloc();
-            Iv &i1 = S.pool[sl];
-            Iv &i2 = S.pool[s2];
-            double xm = 0.5 * (p.a + p.b);
-            i1.a = p.a; i1.b = xm; i1.fa = p.fa; i1.fl = p.fll; i1.fm = p.fl; i1.fr = p.flr; i1.fb = p.fm;
-            i1.refinement = p.refinement;
-            i2.a = xm; i2.b = p.b; i2.fa = p.fm; i2.fl = p.frl; i2.fm = p.fr; i2.fr = p.frr; i2.fb = p.fb;
-            i2.refinement = p.refinement;
-            n += cadre_process(f, &i1);
-            n += cadre_process(f, &i2);
-            S.push(sl);
-            S.push(s2);
-            num_intervals++;
-            err += i1.err + i2.err - p.err;
-            I += i1.I + i2.I - p.I;
-        }
-        if (num_iterations == 2 * last_check) {
-            err = S.total_err();
-            I = S.total_I();
-            last_check = num_intervals;
-        }
-        num_iterations++;
+    enum { M_FA, M_FB, M_FM, M_AFTER_INIT, M_LOOP, M_AFTER_REFINE, M_AFTER_SPLIT1, M_AFTER_SPLIT2, M_DONE };
+    enum { P_START, P_FL, P_FR, P_R0, P_FLL, P_FLR, P_FRL, P_FRR, P_SUM, P_DONE };
+
+    // Begin the integral over [xa, xb]; always requests a first sample (x).
+    __device__ void start(double xa, double xb, int nmax, double at, double rt) {
+        S.reset();
+        Nmax = nmax; atol = at; rtol = rt;
+        int s0 = S.alloc();
+        Iv &i0 = S.pool[s0];
+        i0.a = xa; i0.b = xb; i0.I = 0; i0.err = 0;
+        i0.fll = 0; i0.flr = 0; i0.frl = 0; i0.frr = 0; i0.fl = 0; i0.fr = 0; i0.refinement = 0;
+        cur = s0;
+        pc = M_FA;
+        x = xa;
     }
-    return S.total_I();
-}
+
+    // Interval processing (cadreProcessInterval).  Returns true when it needs a sample (x set);
+    // false when `cur` is processed (cnt holds the number of samples it took).
+    __device__ bool process(double v, bool have_v) {
+        const double scaleTol = 0.2;
+        Iv &i = S.pool[cur];
+        for (;;) {
+            switch (ppc) {
+            case P_START:
+                cnt = 0;
+                if (i.refinement < 4) {
+                    if (i.refinement == 0) { ppc = P_FL; x = 0.75 * i.a + 0.25 * i.b; return true; }
+                    fl = i.fl; fr = i.fr;
+                    i.refinement = 0;
+                    ppc = P_R0;
+                } else {
+                    nint = 1 << i.refinement;
+                    hh = (i.b - i.a) / nint;
+                    sum = 0;
+                    nidx = 1;
+                    if (nidx < nint) { ppc = P_SUM; x = i.a + hh * nidx; return true; }
+                    ppc = P_DONE;   // not reached for refinement >= 4
+                }
+                break;
+            case P_FL:
+                fl = v; cnt++;
+                ppc = P_FR; x = 0.25 * i.a + 0.75 * i.b; return true;
+            case P_FR:
+                fr = v; cnt++;
+                i.fl = fl; i.fr = fr;
+                ppc = P_R0;
+                break;
+            case P_R0: {
+                double fa = i.fa, fm = i.fm, fb = i.fb;
+                double h4 = 0.25 * (i.b - i.a);
+                double T0 = 2 * h4 * (fa + fb);
+                double T1 = 0.5 * T0 + 2 * h4 * fm;
+                double T2 = 0.5 * T1 + h4 * (fl + fr);
+                double R0 = (T1 - T0) / (T2 - T1);
+                if (R0 > 4 - scaleTol && R0 < 4 + scaleTol) {
+                    ppc = P_FLL; x = 0.875 * i.a + 0.125 * i.b; return true;
+                } else if (R0 >= 4 + scaleTol || R0 < 2 || R0 != R0) {
+                    double e10 = fabs(T1 - T0), e21 = fabs(T2 - T1);
+                    i.err = e10 > e21 ? e10 : e21;
+                    i.I = T2;
+                } else {
+                    double e = -(T2 - T1) * (T2 - T1) / (T2 - 2 * T1 + T0);
+                    i.err = fabs(e);
+                    i.I = (fabs(T2 + e) < 1.0e-14 * fabs(T2)) ? 1.0e-14 * T2 : T2 + e;
+                }
+                ppc = P_DONE;
+                break;
+            }
+            case P_FLL: fll = v; cnt++; ppc = P_FLR; x = 0.625 * i.a + 0.375 * i.b; return true;
+            case P_FLR: flr = v; cnt++; ppc = P_FRL; x = 0.375 * i.a + 0.625 * i.b; return true;
+            case P_FRL: frl = v; cnt++; ppc = P_FRR; x = 0.125 * i.a + 0.875 * i.b; return true;
+            case P_FRR: {
+                frr = v; cnt++;
+                double fa = i.fa, fm = i.fm, fb = i.fb;
+                double h4 = 0.25 * (i.b - i.a);
+                double T0 = 2 * h4 * (fa + fb);
+                double T1 = 0.5 * T0 + 2 * h4 * fm;
+                double T2 = 0.5 * T1 + h4 * (fl + fr);
+                i.fll = fll; i.flr = flr; i.frl = frl; i.frr = frr;
+                i.refinement = 1;
+                double T3 = 0.5 * T2 + 0.5 * h4 * (fll + flr + frl + frr);
+                double R1 = (T2 - T1) / (T3 - T2);
+                if (R1 > 4 - scaleTol && R1 < 4 + scaleTol) {
+                    double R11 = (4 * T1 - T0) / 3.0;
+                    double R21 = (4 * T2 - T1) / 3.0;
+                    double R22 = (16 * R21 - R11) / 15.0;
+                    i.fa = T3;
+                    i.fll = (4 * T3 - T2) / 3.0;
+                    i.fl = (16 * i.fll - R21) / 15.0;
+                    i.flr = (64 * i.fl - R22) / 63.0;
+                    i.I = i.flr;
+                    i.err = fabs((i.fl - R22) / 63.0);
+                    i.refinement = 4;
+                } else {
+                    double errt = (T3 - T2) / 3.0;
+                    i.err = fabs(errt);
+                    i.I = T3 + errt;
+                }
+                ppc = P_DONE;
+                break;
+            }
+            case P_SUM:
+                sum += v; cnt++;
+                nidx += 2;
+                if (nidx < nint) { x = i.a + hh * nidx; return true; }
+                {
+                    double T = 0.5 * i.fa + hh * sum;
+                    double Rnm = i.fa, Rnmp = i.fll;
+                    i.fa = T;
+                    i.fll = (4.0 * i.fa - Rnm) / 3.0;
+                    if (i.refinement == 1) { i.I = i.fll; i.err = fabs((i.fa - Rnm) / 3.0); i.refinement++; ppc = P_DONE; break; }
+                    Rnm = Rnmp; Rnmp = i.fl;
+                    i.fl = (16 * i.fll - Rnm) / 15.0;
+                    if (i.refinement == 2) { i.I = i.fl; i.err = fabs((i.fll - Rnm) / 15.0); i.refinement++; ppc = P_DONE; break; }
+                    Rnm = Rnmp; Rnmp = i.flr;
+                    i.flr = (64 * i.fl - Rnm) / 63.0;
+                    if (i.refinement == 3) { i.I = i.flr; i.err = fabs((i.fl - Rnm) / 63.0); i.refinement++; ppc = P_DONE; break; }
+                    Rnm = Rnmp; Rnmp = i.fm;
+                    i.fm = (256 * i.flr - Rnm) / 255.0;
+                    if (i.refinement == 4) { i.I = i.fm; i.err = fabs((i.flr - Rnm) / 255.0); i.refinement++; ppc = P_DONE; break; }
+                    Rnm = Rnmp; Rnmp = i.frl;
+                    i.frl = (1024 * i.fm - Rnm) / 1023.0;
+                    if (i.refinement == 5) { i.I = i.frl; i.err = fabs((i.fm - Rnm) / 1023.0); i.refinement++; ppc = P_DONE; break; }
+                    Rnm = Rnmp; Rnmp = i.fr;
+                    i.fr = (4096 * i.frl - Rnm) / 4095.0;
+                    if (i.refinement == 6) { i.I = i.fr; i.err = fabs((i.frl - Rnm) / 4095.0); i.refinement++; ppc = P_DONE; break; }
+                    Rnm = Rnmp; Rnmp = i.frr;
+                    i.frr = (16384 * i.fr - Rnm) / 16383.0;
+                    if (i.refinement == 7) { i.I = i.frr; i.err = fabs((i.fr - Rnm) / 16383.0); i.refinement++; ppc = P_DONE; break; }
+                    Rnm = Rnmp; Rnmp = i.fb;
+                    i.fb = (65536 * i.frr - Rnm) / 65535.0;
+                    if (i.refinement == 8) { i.I = i.fb; i.err = fabs((i.frr - Rnm) / 65535.0); i.refinement++; ppc = P_DONE; break; }
+                    Rnm = Rnmp;
+                    i.I = (262144 * i.fb - Rnm) / 262143.0;
+                    i.err = fabs((i.fb - Rnm) / 262143.0);
+                    i.refinement++;
+                    ppc = P_DONE;
+                }
+                break;
+            default:  // P_DONE
+                return false;
+            }
+        }
+    }
+
+    // Feed the sample requested last; returns true with the next request in x, or false
+    // when the integral is complete (result).
+    __device__ bool feed(double v) {
+        bool have = true;
+        for (;;) {
+            switch (pc) {
+            case M_FA: S.pool[cur].fa = v; pc = M_FB; x = S.pool[cur].b; return true;
+            case M_FB: S.pool[cur].fb = v; pc = M_FM; x = 0.5 * (S.pool[cur].a + S.pool[cur].b); return true;
+            case M_FM:
+                S.pool[cur].fm = v;
+                n = 3;
+                ppc = P_START; ret = M_AFTER_INIT; pc = -1; have = false;
+                break;
+            case -1:  // processing `cur`
+                if (process(v, have)) return true;
+                n += cnt;
+                pc = ret;
+                break;
+            case M_AFTER_INIT: {
+                S.push(cur);
+                I = S.pool[cur].I; err = S.pool[cur].err;
+                num_intervals = 1; num_iterations = 0; last_check = 0;
+                pc = M_LOOP;
+                break;
+            }
+            case M_LOOP: {
+                if (!(n < Nmax && err > atol + rtol * fabs(I) && !S.overflow)) {
+                    result = S.total_I();
+                    pc = M_DONE;
+                    return false;
+                }
+                sl = S.pop();
+                Iv &iv = S.pool[sl];
+                if (iv.refinement >= 4) {
+                    oldI = iv.I; olderr = iv.err;
+                    cur = sl; ppc = P_START; ret = M_AFTER_REFINE; pc = -1; have = false;
+                } else {
+                    Iv p = iv;
+                    s2 = S.alloc();
+                    Iv &i1 = S.pool[sl];
+                    Iv &i2 = S.pool[s2];
+                    double xm = 0.5 * (p.a + p.b);
+                    i1.a = p.a; i1.b = xm; i1.fa = p.fa; i1.fl = p.fll; i1.fm = p.fl; i1.fr = p.flr; i1.fb = p.fm;
+                    i1.refinement = p.refinement;
+                    i2.a = xm; i2.b = p.b; i2.fa = p.fm; i2.fl = p.frl; i2.fm = p.fr; i2.fr = p.frr; i2.fb = p.fb;
+                    i2.refinement = p.refinement;
+                    oldI = p.I; olderr = p.err;
+                    cur = sl; ppc = P_START; ret = M_AFTER_SPLIT1; pc = -1; have = false;
+                }
+                break;
+            }
+            case M_AFTER_REFINE: {
+                S.push(sl);
+                err += S.pool[sl].err - olderr;
+                I += S.pool[sl].I - oldI;
+                pc = M_LOOP + 100;   // iteration end
+                break;
+            }
+            case M_AFTER_SPLIT1:
+                cur = s2; ppc = P_START; ret = M_AFTER_SPLIT2; pc = -1; have = false;
+                break;
+            case M_AFTER_SPLIT2: {
+                S.push(sl);
+                S.push(s2);
+                num_intervals++;
+                err += S.pool[sl].err + S.pool[s2].err - olderr;
+                I += S.pool[sl].I + S.pool[s2].I - oldI;
+                pc = M_LOOP + 100;
+                break;
+            }
+            case M_LOOP + 100:  // end of one iteration of the adaptive loop
+                if (num_iterations == 2 * last_check) {
+                    err = S.total_err();
+                    I = S.total_I();
+                    last_check = num_intervals;
+                }
+                num_iterations++;
+                pc = M_LOOP;
+                break;
+            default:
+                return false;
+            }
+        }
+    }
+};
 
 struct LayerArgs {
     const double *T;    // table [PBA_NV][nt]
@@ -255,52 +367,63 @@ __device__ __forceinline__ constexpr int slot_var(int s) {
 
 extern __shared__ double lds_tab[];
 
+// Per-lane context of one (layer, t_obs, nu_obs) pair; the layer's scalars are copied in so
+// that the integrand reads registers, not the layer descriptor.
 struct Ctx {
-    const LayerArgs *L;
-    int nt;
+    const double *jfs, *jrs;
+    double z;
+    int nt, nf, i_end_r;
     double tz;          // t_obs / (1 + z)
     double nu_obs;
     double sa, ca;      // sin, cos of the observer angle
+    double mu0, muN;    // mu at the first and last usable step for this t_obs
+    double sphi;        // sin(phi) of the current phi sample
     bool cj;
     int status;
     __device__ double V(int s, int i) const { return lds_tab[s * nt + i]; }
-    __device__ double nu_fs(int i) const { return lds_tab[NSLOT * nt + i]; }
-    __device__ double nu_rs(int i) const { return lds_tab[NSLOT * nt + L->nf + i]; }
     __device__ double mu_at(int i) const { return (V(S_tb, i) - tz) / V(S_R, i) * kC; }
-    __device__ double obs_angle(double theta, double phi) const {
-        if (!cj) return sa * (sin(theta) * sin(phi)) + ca * cos(theta);
-        return sa * (sin(kPi - theta) * sin(phi)) + ca * cos(kPi - theta);
+    // ImageExtend obsAngle / obsAngleCJ (src/image.h:361-378) with sin(phi) of the current sample
+    __device__ double obs_angle(double theta) const {
+        if (!cj) return sa * (sin(theta) * sphi) + ca * cos(theta);
+        return sa * (sin(kPi - theta) * sphi) + ca * cos(kPi - theta);
     }
     __device__ int mu_index(double mu) const {
         auto arr = [&](int i) { return mu_at(i); };
-        return find_index(mu, arr, L->i_end_r);
+        return find_index(mu, arr, i_end_r);
     }
 };
 
-// find_jet_edge (src/eats.h:103-173)
-__device__ double find_jet_edge(const Ctx &C, double phi, double th_con_hi) {
-    double mu = C.obs_angle(th_con_hi, phi);
-    int ia = C.mu_index(mu);
-    double a0 = C.V(S_theta, ia), a1 = C.V(S_theta, ia + 1);
-    if (a0 <= th_con_hi && th_con_hi <= a1) return th_con_hi;
+// find_jet_edge (src/eats.h:103-173) as a resumable bisection: one probe per call of step(),
+// so that lanes searching an edge do not hold up lanes sampling the integrand.
+struct EdgeSearch {
     double ta, tb;
-    if (th_con_hi < a0) { ta = th_con_hi; tb = 0.5 * M_PI; }
-    else { ta = 0.0; tb = th_con_hi; }
-    int i = 0;
-    while (tb - ta > 1.0e-5 && i < 100) {
+    int i;
+    // first probe at the layer's upper edge; returns true if that already is the answer
+    __device__ bool start(const Ctx &C, double th_con_hi) {
+        double mu = C.obs_angle(th_con_hi);
+        int ia = C.mu_index(mu);
+        double a0 = C.V(S_theta, ia), a1 = C.V(S_theta, ia + 1);
+        if (a0 <= th_con_hi && th_con_hi <= a1) { ta = th_con_hi; return true; }
+        if (th_con_hi < a0) { ta = th_con_hi; tb = 0.5 * M_PI; }
+        else { ta = 0.0; tb = th_con_hi; }
+        i = 0;
+        return !(tb - ta > 1.0e-5 && i < 100);
+    }
+    // one bisection probe; returns true when done (edge in ta)
+    __device__ bool step(const Ctx &C) {
         double tc = 0.5 * (ta + tb);
-        mu = C.obs_angle(tc, phi);
-        ia = C.mu_index(mu);
+        double mu = C.obs_angle(tc);
+        int ia = C.mu_index(mu);
         if (tc < C.V(S_theta, ia)) ta = tc; else tb = tc;
         i++;
+        return !(tb - ta > 1.0e-5 && i < 100);
     }
-    return ta;
-}
+};
 
 // nu_off: offset of the shock's frequency grid in the staged LDS block
 __device__ double shock_flux(Ctx &C, const double *jall, int nu_off, double freq, int ia, int ib, double Gamma,
                              double mu, double r, double ne, double n_prime, double theta) {
-    const int nf = C.L->nf;
+    const int nf = C.nf;
     auto nu = [&](int i) { return lds_tab[nu_off + i]; };
     if (freq >= nu(nf - 1) || freq <= nu(0)) { C.status |= PBAGPU_S_NU_OUT_OF_GRID; return 0.; }
     int ia_nu = find_index(freq, nu, nf), ib_nu = ia_nu + 1;
@@ -322,12 +445,12 @@ __device__ double shock_flux(Ctx &C, const double *jall, int nu_off, double freq
 
 // Out of line on purpose: CADRE samples the integrand from eleven places, and one shared
 // copy of its code keeps the kernel inside the instruction cache.
-__device__ __noinline__ double integrand(Ctx &C, double cos_theta, double phi) {
-    const int N = C.L->i_end_r;
+__device__ __noinline__ double integrand(Ctx &C, double cos_theta) {
+    const int N = C.i_end_r;
     if (C.V(S_R, 0) == 0.0 && C.V(S_R, N - 1) == 0.0) return 0.0;
     double a_theta = 2.0 * asin(sqrt(0.5 * (1.0 - cos_theta)));
-    double mu = C.obs_angle(a_theta, phi);
-    double mu0 = C.mu_at(0), muN = C.mu_at(N - 1);
+    double mu = C.obs_angle(a_theta);
+    const double mu0 = C.mu0, muN = C.muN;
     if (mu < mu0 || mu > muN) return 0.;
     if (mu0 >= mu) return 0.;
     int ia = C.mu_index(mu), ib = ia + 1;
@@ -345,41 +468,15 @@ __device__ __noinline__ double integrand(Ctx &C, double cos_theta, double phi) {
     double beta = li(S_beta);
     double ne = li(S_M2) / kMp;
     double n_prime = li(S_rho2) / kMp;
-    double nuprime = (1.0 + C.L->z) * C.nu_obs * (Gamma * (1.0 - beta * mu));
+    double nuprime = (1.0 + C.z) * C.nu_obs * (Gamma * (1.0 - beta * mu));
     double theta = li(S_theta);
-    double flux = shock_flux(C, C.L->jfs, NSLOT * C.nt, nuprime, ia, ib, Gamma, mu, r, ne, n_prime, theta);
-    if (C.L->jrs && C.V(S_thrs, ia) > 0 && C.V(S_thrs, ib) > 0) {
+    double flux = shock_flux(C, C.jfs, NSLOT * C.nt, nuprime, ia, ib, Gamma, mu, r, ne, n_prime, theta);
+    if (C.jrs && C.V(S_thrs, ia) > 0 && C.V(S_thrs, ib) > 0) {
         double ne_rs = li(S_M3) / kMp;
         double np_rs = li(S_rho3) / kMp;
-        flux += shock_flux(C, C.L->jrs, NSLOT * C.nt + C.L->nf, nuprime, ia, ib, Gamma, mu, r, ne_rs, np_rs, theta);
+        flux += shock_flux(C, C.jrs, NSLOT * C.nt + C.nf, nuprime, ia, ib, Gamma, mu, r, ne_rs, np_rs, theta);
     }
     return flux;
-}
-
-// Inner (1 - cos theta) quadrature at one phi sample (phi_integrand, src/eats.h:460-585);
-// out of line for the same reason as the integrand.
-__device__ __noinline__ double phi_integrand(Ctx &C, Store<HI> &Si, double a_phi) {
-    const LayerArgs &L = *C.L;
-    const double th1c = L.theta_c_h, th0c = L.theta_c_l;
-    double th_1 = find_jet_edge(C, a_phi, th1c);
-    double th_0 = (th0c / th1c) * th_1;
-    double t0 = th_0 > 0.5 * M_PI ? 0.5 * M_PI : th_0;
-    double t1 = th_1 > 0.5 * M_PI ? 0.5 * M_PI : th_1;
-    double s0 = sin(0.5 * t0), s1 = sin(0.5 * t1);
-    auto ct = [&](double aomct) -> double { return integrand(C, 1 - aomct, a_phi); };
-    double res = cadre_adapt(Si, ct, 2.0 * s0 * s0, 2.0 * s1 * s1, L.nmax_theta, 0., L.rtol_theta);
-    if (Si.overflow) C.status |= PBAGPU_S_HEAP_FULL;
-    if (res != res || res < 0.0) return 0.;
-    return res;
-}
-
-__device__ double integrate_theta_phi(Ctx &C, Store<HO> &So, Store<HI> &Si) {
-    const LayerArgs &L = *C.L;
-    const double Fcoeff = kCgs2mJy / (4 * M_PI * L.d_l * L.d_l);
-    auto pf = [&](double a_phi) -> double { return phi_integrand(C, Si, a_phi); };
-    double r = 2 * Fcoeff * cadre_adapt(So, pf, 0.0, M_PI, L.nmax_phi, 0., L.rtol_phi);
-    if (So.overflow) C.status |= PBAGPU_S_HEAP_FULL;
-    return r;
 }
 
 constexpr int EATS_WG = 256;
@@ -388,10 +485,14 @@ constexpr size_t LANE_WS = ((Store<HO>::bytes() + Store<HI>::bytes()) + 255) / 2
 extern __shared__ double lds_tab[];   // [NSLOT][nt] table columns, then the two frequency grids
 
 // Persistent kernel: each workgroup repeatedly takes a chunk (one layer, a range of
-// consecutive (t_obs, nu_obs) pairs) from a global counter, stages that layer's table
-// columns in LDS, and its lanes take pairs of the chunk one at a time from an LDS counter
-// until the chunk is exhausted.  Every lane owns a fixed interval-store block of `ws`.
-__global__ void __launch_bounds__(EATS_WG) eats_kernel(const LayerArgs *__restrict__ layers, int n_layers, int n_pairs,
+// consecutive (t_obs, nu_obs) pairs) from a global counter and stages that layer's table
+// columns in LDS; its lanes take pairs of the chunk one at a time from an LDS counter.
+// Per lane, the nested adaptive quadrature of one pair (evalFluxDensA -> integrate_theta_phi,
+// src/eats.h:587-684, 824-856: CADRE over phi in [0, pi], CADRE over 1 - cos(theta) between
+// the layer edges of find_jet_edge for every phi sample, primary jet then counter-jet) runs as
+// two resumable CADRE machines; every pass of the lane loop takes exactly one integrand
+// sample, at the single call site below.
+__global__ void __launch_bounds__(EATS_WG, 2) eats_kernel(const LayerArgs *__restrict__ layers, int n_layers, int n_pairs,
                                                        int chunk, int *__restrict__ next_chunk,
                                                        const double *__restrict__ tobs,
                                                        const double *__restrict__ nuobs, double *__restrict__ out,
@@ -400,10 +501,10 @@ __global__ void __launch_bounds__(EATS_WG) eats_kernel(const LayerArgs *__restri
     const int chunks_per_layer = (n_pairs + chunk - 1) / chunk;
     const int n_chunks = chunks_per_layer * n_layers;
     char *lane_ws = ws + ((size_t)blockIdx.x * EATS_WG + threadIdx.x) * LANE_WS;
-    Store<HO> So;
-    Store<HI> Si;
-    So.bind(lane_ws);
-    Si.bind(lane_ws + Store<HO>::bytes());
+    Cadre<HO> Mo;
+    Cadre<HI> Mi;
+    Mo.S.bind(lane_ws);
+    Mi.S.bind(lane_ws + Store<HO>::bytes());
     int cur_layer = -1;
     for (;;) {
         __syncthreads();  // every lane is done with the previous chunk and its staged table
@@ -429,32 +530,83 @@ __global__ void __launch_bounds__(EATS_WG) eats_kernel(const LayerArgs *__restri
         }
         if (threadIdx.x == 0) s_next = p0;
         __syncthreads();
+        const double Fcoeff = kCgs2mJy / (4 * M_PI * L.d_l * L.d_l);
+        const double th1c = L.theta_c_h, th0c = L.theta_c_l;
         Ctx C;
-        C.L = &L;
-        C.nt = nt;
+        C.jfs = L.jfs; C.jrs = L.jrs; C.z = L.z;
+        C.nt = nt; C.nf = nf; C.i_end_r = L.i_end_r;
         C.sa = sin(L.theta_obs);
         C.ca = cos(L.theta_obs);
-        int st_acc = 0;
+        C.status = 0;
+        C.sphi = 0.;
+        EdgeSearch E;
+        enum { NEED_PAIR, NEED_INNER, EDGE, EVAL, DONE };
+        int mode = NEED_PAIR, k = 0, st_acc = 0;
+        double flux = 0.;
         for (;;) {
-            const int k = atomicAdd(&s_next, 1);
-            if (k >= p1) break;
-            C.status = 0;
-            const double t_obs = tobs[k];
-            C.tz = t_obs / (1.0 + L.z);
-            C.nu_obs = nuobs[k];
-            double fl = 0.;
-            if (L.i_end_r == 1) C.status |= PBAGPU_S_NONFINITE;  // a single usable step: no surface to integrate
-            if (L.i_end_r > 1) {
-                if (t_obs < C.V(S_tb, 0) || t_obs > C.V(S_tb, L.i_end_r - 1)) C.status |= PBAGPU_S_NONFINITE;
-                C.cj = false;
-                fl += integrate_theta_phi(C, So, Si);
-                if (L.counter_jet) {
-                    C.cj = true;
-                    fl += integrate_theta_phi(C, So, Si);
+            // bookkeeping until this lane needs an edge probe or an integrand sample
+            while (mode == NEED_PAIR || mode == NEED_INNER) {
+                if (mode == NEED_PAIR) {
+                    k = atomicAdd(&s_next, 1);
+                    if (k >= p1) { mode = DONE; break; }
+                    C.status = 0;
+                    const double t_obs = tobs[k];
+                    C.tz = t_obs / (1.0 + L.z);
+                    C.nu_obs = nuobs[k];
+                    C.cj = false;
+                    flux = 0.;
+                    if (L.i_end_r == 1) C.status |= PBAGPU_S_NONFINITE;  // one usable step: no surface
+                    if (L.i_end_r > 1) {
+                        if (t_obs < C.V(S_tb, 0) || t_obs > C.V(S_tb, L.i_end_r - 1)) C.status |= PBAGPU_S_NONFINITE;
+                        C.mu0 = C.mu_at(0);
+                        C.muN = C.mu_at(L.i_end_r - 1);
+                        Mo.start(0.0, M_PI, L.nmax_phi, 0., L.rtol_phi);
+                        mode = NEED_INNER;
+                    } else {
+                        out[(size_t)l * n_pairs + k] = 0. + flux;
+                        st_acc |= C.status;
+                    }
+                } else {
+                    // a new phi sample (phi_integrand, src/eats.h:460-585): find the layer edge first
+                    C.sphi = sin(Mo.x);
+                    mode = E.start(C, th1c) ? EVAL + 100 : EDGE;
+                    break;
                 }
             }
-            out[(size_t)l * n_pairs + k] = 0. + fl;
-            st_acc |= C.status;
+            if (mode == DONE) break;
+            if (mode == EDGE) {
+                if (E.step(C)) mode = EVAL + 100;
+            }
+            if (mode == EVAL + 100) {
+                // edge known: start the inner quadrature over 1 - cos(theta)
+                double th_1 = E.ta;
+                double th_0 = (th0c / th1c) * th_1;
+                double t0 = th_0 > 0.5 * M_PI ? 0.5 * M_PI : th_0;
+                double t1 = th_1 > 0.5 * M_PI ? 0.5 * M_PI : th_1;
+                double s0 = sin(0.5 * t0), s1 = sin(0.5 * t1);
+                Mi.start(2.0 * s0 * s0, 2.0 * s1 * s1, L.nmax_theta, 0., L.rtol_theta);
+                mode = EVAL;
+                continue;
+            }
+            if (mode != EVAL) continue;
+            // the one integrand call of the loop
+            const double v = integrand(C, 1 - Mi.x);
+            if (Mi.feed(v)) continue;
+            double res = Mi.result;
+            if (Mi.S.overflow) C.status |= PBAGPU_S_HEAP_FULL;
+            if (res != res || res < 0.0) res = 0.;
+            if (Mo.feed(res)) { mode = NEED_INNER; continue; }
+            if (Mo.S.overflow) C.status |= PBAGPU_S_HEAP_FULL;
+            flux += 2 * Fcoeff * Mo.result;
+            if (L.counter_jet && !C.cj) {
+                C.cj = true;
+                Mo.start(0.0, M_PI, L.nmax_phi, 0., L.rtol_phi);
+                mode = NEED_INNER;
+            } else {
+                out[(size_t)l * n_pairs + k] = 0. + flux;
+                st_acc |= C.status;
+                mode = NEED_PAIR;
+            }
         }
         if (st_acc) atomicOr(&status[l], st_acc);
     }

@@ -117,7 +117,22 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-__global__ void __launch_bounds__(64) ele_kernel(const PopDesc *__restrict__ pops, int nt, int *__restrict__ status) {
+// Per-population state between the per-step launches (device workspace).
+struct PopState {
+    double *f;     // [n_pop][ng]   electron spectrum
+    double *ft;    // [ng][n_pop]   the same, transposed for the Compton contraction
+    double *tf;    // [n_pop][nf]   photon density of the previous step (total_rad.f)
+    double *wt;    // [nf][n_pop]   Compton photon weights of this step
+    double *jsyn;  // [n_pop][nf]
+    double *jssc;  // [n_pop][nf]
+    double *scal;  // [n_pop][4]    dr_comov, vol, n_ele, fac
+    int *flag;     // [n_pop][3]    initialised, active at this step, status bits
+};
+
+// Step `it` of every population, part 1: injection / Chang-Cooper update and the
+// synchrotron emissivity.  One 64-lane wavefront per population.
+__global__ void __launch_bounds__(64) ele_step_kernel(const PopDesc *__restrict__ pops, int n_pop, int it, int nt,
+                                                      PopState S) {
     __shared__ double s_f[MAXG];
     __shared__ double s_tf[MAXF];   // photon density of the previous step (total_rad.f)
     __shared__ double s_w[MAXF];    // tot_f[k] / nu[k] * (nu[k+1] - nu[k])
@@ -136,17 +151,26 @@ __global__ void __launch_bounds__(64) ele_kernel(const PopDesc *__restrict__ pop
     const int vth = D.shock ? PV_thickness_rs : PV_thickness, vGsh = D.shock ? PV_GammaRsh : PV_GammaFsh;
     auto V = [&](int v, int t) { return D.T[(size_t)v * nt + t]; };
 
-    double f[MPL], src[MPL], bb[MPL], cp[MPL];
-    for (int r = 0; r < MPL; r++) { f[r] = 0.; src[r] = 0.; bb[r] = 1.; cp[r] = 0.; }
-    for (int j = lane; j < ng; j += 64) s_dge[j] = (j < ng - 1) ? D.e[j + 1] - D.e[j] : 0.;
-    for (int k = lane; k < nf; k += 64) s_tf[k] = 0.;
-    __syncthreads();
-    bool init = false;
-    double dr_comov = 0., vol = 0.;
-    int st = 0;
-    for (int it = 0; it < nt; it++) {
+    const int q0 = blockIdx.x;
+    int *FL = S.flag + 3 * q0;
+    double *SC = S.scal + 4 * q0;
+    {
         int flags = (int)V(PV_mflags, it);
-        if (!(flags & bit)) continue;
+        if (!(flags & bit)) { if (lane == 0) FL[1] = 0; return; }
+    }
+    double f[MPL], src[MPL], bb[MPL], cp[MPL];
+    for (int r = 0; r < MPL; r++) {
+        int j = j0 + r;
+        f[r] = (r < m && j < ng) ? S.f[(size_t)q0 * ng + j] : 0.;
+        src[r] = 0.; bb[r] = 1.; cp[r] = 0.;
+    }
+    for (int j = lane; j < ng; j += 64) s_dge[j] = (j < ng - 1) ? D.e[j + 1] - D.e[j] : 0.;
+    for (int k = lane; k < nf; k += 64) s_tf[k] = S.tf[(size_t)q0 * nf + k];
+    __syncthreads();
+    bool init = FL[0] != 0;
+    double dr_comov = SC[0], vol = SC[1];
+    int st = 0;
+    {
         const double B = V(vB, it), gm = V(vgm, it), gM = V(vgM, it), gc = V(vgc, it), acc = V(vacc, it);
         if (!init) {
             dr_comov = V(vth, it);
@@ -260,63 +284,107 @@ __global__ void __launch_bounds__(64) ele_kernel(const PopDesc *__restrict__ pop
                 }
             }
         }
-        // --- emission ---
+        // --- emission: synchrotron part and what the Compton contraction needs ---
         double ne_part = 0.;
         for (int r = 0; r < m; r++) {
             int j = j0 + r;
             if (j < ng) {
                 s_f[j] = f[r];
                 if (j < ng - 1) ne_part += f[r] * s_dge[j];
+                S.f[(size_t)q0 * ng + j] = f[r];
+                S.ft[(size_t)j * n_pop + q0] = f[r];
             }
         }
         if (D.ssc)
             for (int k = lane; k < nf; k += 64)
-                s_w[k] = (k < nf - 1) ? s_tf[k] / D.nu[k] * (D.nu[k + 1] - D.nu[k]) : 0.;
+                S.wt[(size_t)k * n_pop + q0] = (k < nf - 1) ? s_tf[k] / D.nu[k] * (D.nu[k + 1] - D.nu[k]) : 0.;
         __syncthreads();
         const double n_ele = wave_sum(ne_part);
         if (n_ele <= 0) st |= PBAGPU_S_NONFINITE;
         const double fac = (dr_comov / kC) / vol;
-        double tfn[MAXF / 64];
-        for (int q = 0; q < (nf + 63) / 64; q++) {
-            int i = lane + 64 * q;
-            tfn[q] = 0.;
-            if (i >= nf) continue;
+        for (int i = lane; i < nf; i += 64) {
             const double nui = D.nu[i];
             double synj = 0.;
             if (i < nf - 1) {
                 for (int j = 0; j < ng - 1; j++) synj += cyc_synch(B, nui, D.e[j], pref) * s_f[j] * s_dge[j];
                 if (!isfinite(synj) || synj < 0.) st |= PBAGPU_S_NONFINITE;
             }
-            double sscj = 0.;
-            if (D.ssc) {
-                const double c0 = 3. / 4. * kSigmaT * kC;
-                for (int j = 0; j < ng - 1; j++) {
-                    const double *Kr = D.K + ((size_t)i * ng + j) * nf;
-                    double s = 0.;
-                    for (int k = 0; k < nf - 1; k++) s += s_w[k] * Kr[k];
-                    double ej = D.e[j];
-                    sscj += s_f[j] * kH * nui * (c0 / (ej * ej) * s) * s_dge[j];
-                }
-            }
-            tfn[q] = synj * fac / (kH * nui) + (D.ssc ? sscj * fac / (kH * nui) : 0.);
-            double jt = (synj + sscj) * acc / n_ele;
-            if (!isfinite(jt) || jt < 0.) st |= PBAGPU_S_NONFINITE;
-            D.jall[(size_t)it * nf + i] = jt;
-        }
-        __syncthreads();
-        for (int q = 0; q < (nf + 63) / 64; q++) {
-            int i = lane + 64 * q;
-            if (i < nf) s_tf[i] = tfn[q];
+            S.jsyn[(size_t)q0 * nf + i] = synj;
         }
         if (D.fall)
             for (int r = 0; r < m; r++) {
                 int j = j0 + r;
                 if (j < ng) D.fall[(size_t)it * ng + j] = f[r];
             }
-        __syncthreads();
+        for (int d = 32; d >= 1; d >>= 1) st |= __shfl_xor(st, d, 64);
+        if (lane == 0) {
+            FL[0] = 1; FL[1] = 1; FL[2] |= st;
+            SC[0] = dr_comov; SC[1] = vol; SC[2] = n_ele; SC[3] = fac;
+        }
+    }
+}
+
+// Step `it`, part 2: SSC emissivity of the populations of one Compton table
+// (computeSSCSpectrum, src/microphysics/numeric_model.h:247-285):
+//   j_ssc[i] = sum_j f_j h nu_i (3/4 sigma_T c / gamma_j^2) (sum_k w_k K[i][j][k]) (gamma_{j+1} - gamma_j)
+// Lane = population, so the table entry K[i][j][k] is the same for all lanes (a scalar
+// load used by 64 populations); grid.y splits the frequencies i into blocks of SSC_IB.
+constexpr int SSC_IB = 8;
+__global__ void __launch_bounds__(64) ssc_step_kernel(const int *__restrict__ plist, int npk, int n_pop,
+                                                      const double *__restrict__ K, const double *__restrict__ e,
+                                                      const double *__restrict__ nu, int ng, int nf, PopState S) {
+    extern __shared__ double s_wl[];   // [nf][64] photon weights of the 64 populations
+    const int lane = threadIdx.x;
+    const int idx = blockIdx.x * 64 + lane;
+    const int q0 = idx < npk ? plist[idx] : -1;
+    const bool act = q0 >= 0 && S.flag[3 * q0 + 1] != 0;
+    for (int k = 0; k < nf; k++) s_wl[k * 64 + lane] = act ? S.wt[(size_t)k * n_pop + q0] : 0.;
+    __syncthreads();
+    const double c0 = 3. / 4. * kSigmaT * kC;
+    const int i0 = blockIdx.y * SSC_IB, i1 = min(i0 + SSC_IB, nf);
+    for (int i = i0; i < i1; i++) {
+        const double nui = nu[i];
+        double sscj = 0.;
+        for (int j = 0; j < ng - 1; j++) {
+            const double *Kr = K + ((size_t)i * ng + j) * nf;
+            double sv = 0.;
+            for (int k = 0; k < nf - 1; k++) sv += s_wl[k * 64 + lane] * Kr[k];
+            const double ej = e[j];
+            const double fj = act ? S.ft[(size_t)j * n_pop + q0] : 0.;
+            sscj += fj * kH * nui * (c0 / (ej * ej) * sv) * (e[j + 1] - e[j]);
+        }
+        if (act) S.jssc[(size_t)q0 * nf + i] = sscj;
+    }
+}
+
+__global__ void ele_status_kernel(const int *__restrict__ flag, int n_pop, int *__restrict__ status) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n_pop) status[q] = flag[3 * q + 2];
+}
+
+// Step `it`, part 3: total emissivity per electron, photon density for the next step
+// (computeRadiationNumericMixed, src/microphysics/microphysics.h:492-599).
+__global__ void __launch_bounds__(64) ele_emit_kernel(const PopDesc *__restrict__ pops, int it, int nt, PopState S) {
+    const PopDesc D = pops[blockIdx.x];
+    const int q0 = blockIdx.x, lane = threadIdx.x, nf = D.nf;
+    const int *FL = S.flag + 3 * q0;
+    if (!FL[1]) return;
+    const double *SC = S.scal + 4 * q0;
+    const double n_ele = SC[2], fac = SC[3];
+    const int vacc = D.shock ? PV_accel_frac_rs : PV_accel_frac;
+    const double acc = D.T[(size_t)vacc * nt + it];
+    int st = 0;
+    for (int i = lane; i < nf; i += 64) {
+        const double nui = D.nu[i];
+        const double synj = S.jsyn[(size_t)q0 * nf + i];
+        const double sscj = D.ssc ? S.jssc[(size_t)q0 * nf + i] : 0.;
+        S.tf[(size_t)q0 * nf + i] = synj * fac / (kH * nui) + (D.ssc ? sscj * fac / (kH * nui) : 0.);
+        double jt = (synj + sscj) * acc / n_ele;
+        if (!isfinite(jt) || jt < 0.) st |= PBAGPU_S_NONFINITE;
+        D.jall[(size_t)it * nf + i] = jt;
     }
     for (int d = 32; d >= 1; d >>= 1) st |= __shfl_xor(st, d, 64);
-    if (lane == 0) status[blockIdx.x] = st;
+    if (lane == 0 && st) S.flag[3 * q0 + 2] |= st;
 }
 
 struct GridSet {
@@ -436,11 +504,56 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
     PBA_TRY(rc, hipMemcpyAsync(dpops, hp.data(), sizeof(PopDesc) * n_pop, hipMemcpyHostToDevice, s));
     PBA_TRY(rc, hipMemsetAsync(d_jall, 0, sizeof(double) * (size_t)n_pop * nt * nf, s));
     if (d_fall) PBA_TRY(rc, hipMemsetAsync(d_fall, 0, sizeof(double) * (size_t)n_pop * nt * ng, s));
+    // per-population state between the per-step launches
+    const size_t np = (size_t)n_pop;
+    const size_t n_dbl = np * ng * 2 + np * nf * 4 + np * 4;
+    double *wsd = (double *)pbagpu::workspace(pbagpu::WS_ELE_WORK, sizeof(double) * n_dbl + sizeof(int) * np * 3);
+    // populations of each Compton table, for the batched contraction
+    std::vector<int> hlist;
+    std::vector<size_t> list_off(n_ssc + 1, 0);
+    for (size_t t = 0; t < n_ssc; t++) {
+        list_off[t] = hlist.size();
+        for (int q = 0; q < n_pop; q++)
+            if (hp[q].ssc && ssc_slot[key_of[q]] == t && need_ssc[key_of[q]]) hlist.push_back(q);
+    }
+    list_off[n_ssc] = hlist.size();
+    int *dlist = hlist.empty() ? nullptr
+                               : (int *)pbagpu::workspace(pbagpu::WS_ELE_LIST, sizeof(int) * hlist.size());
+    if (!wsd || (!hlist.empty() && !dlist)) return PBAGPU_E_HIP;
+    PopState S;
+    S.f = wsd; S.ft = S.f + np * ng; S.tf = S.ft + np * ng; S.wt = S.tf + np * nf;
+    S.jsyn = S.wt + np * nf; S.jssc = S.jsyn + np * nf; S.scal = S.jssc + np * nf;
+    S.flag = (int *)(S.scal + np * 4);
+    PBA_TRY(rc, hipMemsetAsync(wsd, 0, sizeof(double) * n_dbl + sizeof(int) * np * 3, s));
+    if (dlist) PBA_TRY(rc, hipMemcpyAsync(dlist, hlist.data(), sizeof(int) * hlist.size(), hipMemcpyHostToDevice, s));
+    const size_t ssc_lds = sizeof(double) * (size_t)nf * 64;
+    if (ssc_lds > 64 * 1024)
+        (void)hipFuncSetAttribute((const void *)ssc_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)ssc_lds);
     if (rc == PBAGPU_OK) {
         tele.start(s);
-        hipLaunchKernelGGL(ele_kernel, dim3(n_pop), dim3(64), 0, s, dpops, nt, d_status);
-        PBA_TRY(rc, hipGetLastError());
+        for (int it = 0; it < nt && rc == PBAGPU_OK; it++) {
+            hipLaunchKernelGGL(ele_step_kernel, dim3(n_pop), dim3(64), 0, s, dpops, n_pop, it, nt, S);
+            for (size_t t = 0; t < n_ssc; t++) {
+                const int npk = (int)(list_off[t + 1] - list_off[t]);
+                if (npk == 0) continue;
+                size_t kk = 0;
+                while (!(need_ssc[kk] && ssc_slot[kk] == t)) kk++;
+                const double *base = dgrids + per_key * kk;
+                const double *e = base, *nu = base + ng + (ng - 1) + (ng + 1) + ng;
+                const double *K = dK + (ktab + (size_t)ng * nf) * t;
+                hipLaunchKernelGGL(ssc_step_kernel, dim3((unsigned)((npk + 63) / 64), (unsigned)((nf + SSC_IB - 1) / SSC_IB)),
+                                   dim3(64), ssc_lds, s, dlist + list_off[t], npk, n_pop, K, e, nu, ng, nf, S);
+            }
+            hipLaunchKernelGGL(ele_emit_kernel, dim3(n_pop), dim3(64), 0, s, dpops, it, nt, S);
+            PBA_TRY(rc, hipGetLastError());
+        }
         tele.stop(s);
+        // status of every population
+        if (rc == PBAGPU_OK)
+            hipLaunchKernelGGL(ele_status_kernel, dim3((unsigned)((n_pop + 63) / 64)), dim3(64), 0, s, S.flag, n_pop,
+                               d_status);
+        PBA_TRY(rc, hipGetLastError());
     }
     PBA_TRY(rc, hipStreamSynchronize(s));
     ttab.collect();
