@@ -124,7 +124,7 @@ struct PopState {
     double *tf;    // [n_pop][nf]   photon density of the previous step (total_rad.f)
     double *wt;    // [nf][n_pop]   Compton photon weights of this step
     double *jsyn;  // [n_pop][nf]
-    double *jssc;  // [n_pop][nf]
+    double *jssc;  // [n_pop][nf][SSC_JB] partial sums over blocks of Lorentz-factor bins
     double *scal;  // [n_pop][4]    dr_comov, vol, n_ele, fac
     int *flag;     // [n_pop][3]    initialised, active at this step, status bits
 };
@@ -327,9 +327,11 @@ __global__ void __launch_bounds__(64) ele_step_kernel(const PopDesc *__restrict_
 // Step `it`, part 2: SSC emissivity of the populations of one Compton table
 // (computeSSCSpectrum, src/microphysics/numeric_model.h:247-285):
 //   j_ssc[i] = sum_j f_j h nu_i (3/4 sigma_T c / gamma_j^2) (sum_k w_k K[i][j][k]) (gamma_{j+1} - gamma_j)
-// Lane = population, so the table entry K[i][j][k] is the same for all lanes (a scalar
-// load used by 64 populations); grid.y splits the frequencies i into blocks of SSC_IB.
-constexpr int SSC_IB = 8;
+// Lane = population, so the table entry K[i][j][k] is the same for all 64 lanes (one scalar
+// load feeds 64 populations).  grid = (population blocks of 64, frequency i, j-block); each
+// workgroup sums its block of Lorentz-factor bins with four independent k-chains and writes a
+// partial sum, which ele_emit_kernel adds up in j-block order.
+constexpr int SSC_JB = 8;
 __global__ void __launch_bounds__(64) ssc_step_kernel(const int *__restrict__ plist, int npk, int n_pop,
                                                       const double *__restrict__ K, const double *__restrict__ e,
                                                       const double *__restrict__ nu, int ng, int nf, PopState S) {
@@ -341,20 +343,38 @@ __global__ void __launch_bounds__(64) ssc_step_kernel(const int *__restrict__ pl
     for (int k = 0; k < nf; k++) s_wl[k * 64 + lane] = act ? S.wt[(size_t)k * n_pop + q0] : 0.;
     __syncthreads();
     const double c0 = 3. / 4. * kSigmaT * kC;
-    const int i0 = blockIdx.y * SSC_IB, i1 = min(i0 + SSC_IB, nf);
-    for (int i = i0; i < i1; i++) {
-        const double nui = nu[i];
-        double sscj = 0.;
-        for (int j = 0; j < ng - 1; j++) {
-            const double *Kr = K + ((size_t)i * ng + j) * nf;
-            double sv = 0.;
-            for (int k = 0; k < nf - 1; k++) sv += s_wl[k * 64 + lane] * Kr[k];
-            const double ej = e[j];
-            const double fj = act ? S.ft[(size_t)j * n_pop + q0] : 0.;
-            sscj += fj * kH * nui * (c0 / (ej * ej) * sv) * (e[j + 1] - e[j]);
+    const int i = blockIdx.y;
+    const int jn = ng - 1;
+    const int js = (jn + SSC_JB - 1) / SSC_JB;
+    const int j0 = blockIdx.z * js, j1 = min(j0 + js, jn);
+    const double nui = nu[i];
+    double part = 0.;
+    int j = j0;
+    for (; j + 4 <= j1; j += 4) {
+        const double *K0 = K + ((size_t)i * ng + j) * nf;
+        const double *K1 = K0 + nf, *K2 = K1 + nf, *K3 = K2 + nf;
+        double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
+        for (int k = 0; k < nf - 1; k++) {
+            const double w = s_wl[k * 64 + lane];
+            s0 += w * K0[k]; s1 += w * K1[k]; s2 += w * K2[k]; s3 += w * K3[k];
         }
-        if (act) S.jssc[(size_t)q0 * nf + i] = sscj;
+        const double sv[4] = {s0, s1, s2, s3};
+        for (int u = 0; u < 4; u++) {
+            const int jj = j + u;
+            const double ej = e[jj];
+            const double fj = act ? S.ft[(size_t)jj * n_pop + q0] : 0.;
+            part += fj * kH * nui * (c0 / (ej * ej) * sv[u]) * (e[jj + 1] - e[jj]);
+        }
     }
+    for (; j < j1; j++) {
+        const double *Kr = K + ((size_t)i * ng + j) * nf;
+        double sv = 0.;
+        for (int k = 0; k < nf - 1; k++) sv += s_wl[k * 64 + lane] * Kr[k];
+        const double ej = e[j];
+        const double fj = act ? S.ft[(size_t)j * n_pop + q0] : 0.;
+        part += fj * kH * nui * (c0 / (ej * ej) * sv) * (e[j + 1] - e[j]);
+    }
+    if (act) S.jssc[((size_t)q0 * nf + i) * SSC_JB + blockIdx.z] = part;
 }
 
 __global__ void ele_status_kernel(const int *__restrict__ flag, int n_pop, int *__restrict__ status) {
@@ -377,7 +397,9 @@ __global__ void __launch_bounds__(64) ele_emit_kernel(const PopDesc *__restrict_
     for (int i = lane; i < nf; i += 64) {
         const double nui = D.nu[i];
         const double synj = S.jsyn[(size_t)q0 * nf + i];
-        const double sscj = D.ssc ? S.jssc[(size_t)q0 * nf + i] : 0.;
+        double sscj = 0.;
+        if (D.ssc)
+            for (int jb = 0; jb < SSC_JB; jb++) sscj += S.jssc[((size_t)q0 * nf + i) * SSC_JB + jb];
         S.tf[(size_t)q0 * nf + i] = synj * fac / (kH * nui) + (D.ssc ? sscj * fac / (kH * nui) : 0.);
         double jt = (synj + sscj) * acc / n_ele;
         if (!isfinite(jt) || jt < 0.) st |= PBAGPU_S_NONFINITE;
@@ -506,7 +528,7 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
     if (d_fall) PBA_TRY(rc, hipMemsetAsync(d_fall, 0, sizeof(double) * (size_t)n_pop * nt * ng, s));
     // per-population state between the per-step launches
     const size_t np = (size_t)n_pop;
-    const size_t n_dbl = np * ng * 2 + np * nf * 4 + np * 4;
+    const size_t n_dbl = np * ng * 2 + np * nf * (3 + SSC_JB) + np * 4;
     double *wsd = (double *)pbagpu::workspace(pbagpu::WS_ELE_WORK, sizeof(double) * n_dbl + sizeof(int) * np * 3);
     // populations of each Compton table, for the batched contraction
     std::vector<int> hlist;
@@ -522,7 +544,7 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
     if (!wsd || (!hlist.empty() && !dlist)) return PBAGPU_E_HIP;
     PopState S;
     S.f = wsd; S.ft = S.f + np * ng; S.tf = S.ft + np * ng; S.wt = S.tf + np * nf;
-    S.jsyn = S.wt + np * nf; S.jssc = S.jsyn + np * nf; S.scal = S.jssc + np * nf;
+    S.jsyn = S.wt + np * nf; S.jssc = S.jsyn + np * nf; S.scal = S.jssc + np * nf * SSC_JB;
     S.flag = (int *)(S.scal + np * 4);
     PBA_TRY(rc, hipMemsetAsync(wsd, 0, sizeof(double) * n_dbl + sizeof(int) * np * 3, s));
     if (dlist) PBA_TRY(rc, hipMemcpyAsync(dlist, hlist.data(), sizeof(int) * hlist.size(), hipMemcpyHostToDevice, s));
@@ -542,7 +564,7 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
                 const double *base = dgrids + per_key * kk;
                 const double *e = base, *nu = base + ng + (ng - 1) + (ng + 1) + ng;
                 const double *K = dK + (ktab + (size_t)ng * nf) * t;
-                hipLaunchKernelGGL(ssc_step_kernel, dim3((unsigned)((npk + 63) / 64), (unsigned)((nf + SSC_IB - 1) / SSC_IB)),
+                hipLaunchKernelGGL(ssc_step_kernel, dim3((unsigned)((npk + 63) / 64), (unsigned)nf, (unsigned)SSC_JB),
                                    dim3(64), ssc_lds, s, dlist + list_off[t], npk, n_pop, K, e, nu, ng, nf, S);
             }
             hipLaunchKernelGGL(ele_emit_kernel, dim3(n_pop), dim3(64), 0, s, dpops, it, nt, S);
