@@ -6,19 +6,23 @@ shocks, 200 electron Lorentz-factor bins x 96 comoving frequencies, 1000 burster
 light curve at 96 nu_obs x 128 t_obs (theta_obs = 0.363 rad, n_ism = 10^-1.6 cm^-3,
 E_iso,c = 10^54.1 erg, Gamma0 = 300, theta_c = 0.061, theta_w = 0.436).
 
-One step = the three GPU stages (dynamics, electron kinetics + emissivity, observer
-quadrature) over one batch of independent runs of that workload, with every input and
-intermediate resident in device memory.  value = batch x steps / timed seconds.
+One step = one batch of independent runs of that workload through the three GPU stages, with
+every input and intermediate resident in device memory.  Batches are double-buffered: while
+the electron and observer stages of batch i run on one stream, the dynamics of batch i+1 run on
+a second stream (the dynamics kernel occupies one wavefront per run, the observer kernel the
+whole chip).  One timed step therefore completes the light curves of one batch;
+value = batch x steps / timed seconds.
 
-The JSON line carries a roofline entry for the kernel that takes the most device time
-(events recorded on the stream it runs on, by the library itself) and a CPU baseline: the
-CPU restatement of the same algorithm (oracle/) timed on this host on a bounded sample of the
-same workload and scaled to light curves per second.
+The JSON line carries a roofline entry for the kernel with the most device time in the timed
+steps (HIP events recorded by the library on the stream each kernel runs on) and a CPU baseline:
+the CPU restatement of the same algorithm (oracle/) timed on this host on a bounded sample of one
+run of the same workload, scaled to light curves per second.
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -26,53 +30,35 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+# algorithmic work per unit (counted from the source; see DESIGN.md for the derivations)
+FLOPS_PER_RHS = {0: 250.0, 1: 330.0}   # one right-hand side of one blastwave (forward shock / with reverse shock)
+BYTES_PER_SAMPLE = 144.0               # one EATS integrand sample: 7 state columns at 2 steps + 4 emissivities
 
 
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=64, help="independent runs per step")
+    ap.add_argument("--batch", type=int, default=32, help="independent runs per step")
     ap.add_argument("--precision", type=int, default=0, help="0: FP64 everywhere")
     ap.add_argument("--workload", default="gauss21_fsrs_ssc", choices=["gauss21_fsrs_ssc", "tophat_fs_syn"])
-    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="budget of the electron and observer parts of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stages", action="store_true", help="also print per-stage device times to stderr")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device-to-device (reference kernel for the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
-
-
-def algorithmic_work(pl):
-    """Algorithmic FP64 flops per stage of one pipeline step (counted per unit below).
-
-    electron stage, per population step (a step with electrons, table mflags):
-        SSC contraction   2 nf ng nf   (photon-density x kernel table x spectrum; only with SSC)
-        synchrotron       2 ng nf      (spectrum x synchrotron kernel)
-    observer stage: not counted (adaptive; reported as null)
-    dynamics stage: not counted (adaptive; reported as null)
-    """
-    import numpy as np
-    from pyblastafterglowmag_amd import abi
-    mfl = pl.d_table[:, abi.PV["mflags"], :].cpu().numpy().astype(np.int64)
-    nf = pl.nf
-    ng = pl.runs[0].params.fs.ngam
-    fl = 0.0
-    for q, P in enumerate(pl.pops):
-        bit = 1 if P.shock == 0 else 2
-        nsteps = int(np.count_nonzero(mfl[P.bw] & bit))
-        pr = pl.params[P.run]
-        sh = pr.fs if P.shock == 0 else pr.rs
-        per = 2.0 * ng * nf + (2.0 * nf * ng * nf if sh.ssc else 0.0)
-        fl += nsteps * per
-    return {"ele": fl}
 
 
 def cpu_baseline(workload, budget_s):
     """Time the CPU restatement on a bounded sample of one run of the workload and scale it.
 
-    Sample: the full dynamics of one run (all layers), the electron stage of a subset of its
-    populations, and the observer quadrature of every layer at a subset of (t_obs, nu_obs)
-    pairs; light curves/s = 1 / (t_dyn + t_ele/k_pop * n_pop + t_eats/k_pair * n_pair).
+    Sample: the full dynamics of one run (all layers; it is one ODE system), the electron stage
+    of a subset of its populations, and the observer quadrature of every layer at a subset of
+    (t_obs, nu_obs) pairs spread over the grid;
+    light curves/s = 1 / (t_dyn + t_ele / k_pop * n_pop + t_eats / k_pair * n_pair).
     """
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -83,12 +69,12 @@ def cpu_baseline(workload, budget_s):
     t0 = time.perf_counter()
     table, res = pyoracle.dyn_evolve(run.params, run.inits, run.tgrid)
     t_dyn = time.perf_counter() - t0
+    print(f"[cpu] dynamics {t_dyn:.1f}s", file=sys.stderr, flush=True)
     pops, lays = gpu.populations((abi.RunParams * 1)(run.params), res, 1, run.n_layers)
     n_pop = len(pops)
-    # electron stage: populations in chunks of `threads` until a third of the budget is used
     k_pop, t_ele = 0, 0.0
     jall = np.zeros((n_pop, len(run.tgrid), run.params.fs.nfreq))
-    while k_pop < n_pop and t_ele < budget_s / 3:
+    while k_pop < n_pop and t_ele < budget_s / 2:
         chunk = list(range(k_pop, min(n_pop, k_pop + threads)))
         parr = (abi.Population * len(chunk))(*[pops[q] for q in chunk])
         t0 = time.perf_counter()
@@ -96,29 +82,78 @@ def cpu_baseline(workload, budget_s):
         t_ele += time.perf_counter() - t0
         jall[chunk] = j
         k_pop += len(chunk)
-    for q in range(k_pop, n_pop):       # unmeasured populations reuse a measured spectrum table
+        print(f"[cpu] electrons {k_pop}/{n_pop} populations {t_ele:.1f}s", file=sys.stderr, flush=True)
+    for q in range(k_pop, n_pop):       # unmeasured populations reuse a measured emissivity table
         jall[q] = jall[q % max(k_pop, 1)]
     larr = (abi.EatsLayer * len(lays))()
     for i, (bw, r, pfs, prs, iend) in enumerate(lays):
         li = run.inits[bw]
         larr[i] = abi.EatsLayer(li.theta_c_l, li.theta_c_h, bw, r, pfs, prs, iend)
     n_pair = len(run.tobs)
-    # observer stage: pairs spread over the grid, in growing sets until the budget is used
     k_pair, t_eats = 0, 0.0
-    stride = 64
-    while t_eats < budget_s / 2 and k_pair < n_pair and stride >= 1:
-        sel = np.arange(0, n_pair, stride)
+    stride = 256
+    while t_eats < budget_s / 4 and k_pair < n_pair and stride >= 1:
+        sel = np.arange(stride // 2, n_pair, stride)
         t0 = time.perf_counter()
         pyoracle.eats_lightcurve(run.params, larr, table, jall, run.tobs[sel], run.nuobs[sel])
         t_eats = time.perf_counter() - t0
         k_pair = len(sel)
         stride //= 2
+        print(f"[cpu] observer {k_pair} pairs {t_eats:.1f}s", file=sys.stderr, flush=True)
     per_lc = t_dyn + t_ele / max(k_pop, 1) * n_pop + t_eats / max(k_pair, 1) * n_pair
     return {"value": 1.0 / per_lc, "unit": "light curves/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle (CPU restatement, OpenMP over populations and layers) on one {workload} run: "
-                       f"full dynamics {t_dyn:.2f}s, electrons of {k_pop}/{n_pop} populations {t_ele:.2f}s, "
-                       f"observer quadrature of all layers at {k_pair}/{n_pair} pairs {t_eats:.2f}s; "
-                       f"scaled to {per_lc:.1f}s per light curve")}
+            "sample": (f"oracle (CPU restatement, OpenMP over populations and layers, {threads} threads) on one "
+                       f"{workload} run: full dynamics {t_dyn:.1f}s, electrons of {k_pop}/{n_pop} populations "
+                       f"{t_ele:.1f}s, observer quadrature of all layers at {k_pair}/{n_pair} pairs {t_eats:.1f}s; "
+                       f"scaled to {per_lc:.0f}s per light curve")}
+
+
+def roofline(stage_ms, pls, gpu):
+    """Roofline entry of the stage with the most device time in the timed steps."""
+    import numpy as np
+    from pyblastafterglowmag_amd import abi
+    names = {gpu.STAGE_DYN: "dyn_kernel", gpu.STAGE_ELE: "ele (per-step kernels)", gpu.STAGE_EATS: "eats_kernel",
+             gpu.STAGE_SSC_TABLE: "ssc_table"}
+    dom = max(stage_ms, key=lambda s: stage_ms[s][0])
+    ms, launches = stage_ms[dom]
+    avg_s = ms / max(launches, 1) / 1e3
+    roof = {"kernel": names[dom], "avg_launch_ms": ms / max(launches, 1), "bound": None, "achieved": None,
+            "peak": None, "unit": None, "frac": None, "traffic": None}
+    if dom == gpu.STAGE_EATS:
+        samples = gpu.timing_units(gpu.STAGE_EATS) / max(launches, 1)
+        ach = samples * BYTES_PER_SAMPLE / avg_s / 1e9
+        roof.update(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
+                    samples_per_launch=samples, bytes_per_sample=BYTES_PER_SAMPLE)
+    elif dom == gpu.STAGE_DYN:
+        pl = pls[0]
+        flops = 0.0
+        for r in range(pl.n_runs):
+            for i in range(pl.bpr):
+                res = pl.res[r * pl.bpr + i]
+                flops += res.n_rhs * FLOPS_PER_RHS[1 if res.fsrs else 0]
+        ach = flops / avg_s / 1e12
+        roof.update(bound="mfma", achieved=ach, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP64_TFLOPS,
+                    dtype_peak="fp64", flops_per_rhs=FLOPS_PER_RHS)
+    elif dom == gpu.STAGE_ELE:
+        pl = pls[0]
+        mfl = pl.d_table[:, abi.PV["mflags"], :].cpu().numpy().astype(np.int64)
+        ng, nf = pl.runs[0].params.fs.ngam, pl.nf
+        fl = 0.0
+        for P in pl.pops:
+            bit = 1 if P.shock == 0 else 2
+            n = int(np.count_nonzero(mfl[P.bw] & bit))
+            sh = pl.params[P.run].fs if P.shock == 0 else pl.params[P.run].rs
+            fl += n * (2.0 * ng * nf + (2.0 * nf * ng * nf if sh.ssc else 0.0))
+        ach = fl / avg_s / 1e12
+        roof.update(bound="mfma", achieved=ach, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP64_TFLOPS,
+                    dtype_peak="fp64")
+    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tr = json.load(f)
+        key = f"{roof['kernel']}|{pls[0].runs[0].n_layers}x{pls[0].n_runs}"
+        roof["traffic"] = tr.get(key)
+    return roof
 
 
 def main():
@@ -133,47 +168,56 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no ROCm device visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     run = gpu.Run.from_workload(*getattr(workloads, args.workload)())
     runs = [run] * args.batch
-    stream = torch.cuda.Stream()
-    with torch.cuda.stream(stream):
-        pl = gpu.Pipeline(runs, stream=stream, precision=args.precision)
-        gpu.timing_enable(True)
-        for _ in range(args.warmup):
-            pl.step()
-        for st in range(4):
-            gpu.timing_read(st, reset=True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            pl.step()
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-    dst, est, lst = pl.status()
-    if dst.any() or est.any() or lst.any():
-        print(f"bench.py: non-zero status dyn={sorted(set(dst.tolist()))} ele={sorted(set(est.tolist()))} "
-              f"eats={sorted(set(lst.tolist()))}", file=sys.stderr)
-        sys.exit(3)
-    names = {gpu.STAGE_DYN: "dyn_kernel", gpu.STAGE_ELE: "ele_kernel", gpu.STAGE_EATS: "eats_kernel",
-             gpu.STAGE_SSC_TABLE: "ssc_table"}
-    stage_ms = {names[s]: gpu.timing_read(s) for s in range(4)}
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    pls = [gpu.Pipeline(runs, stream=streams[0], precision=args.precision),
+           gpu.Pipeline(runs, stream=streams[1], precision=args.precision)]
+    gpu.timing_enable(True)
+    t_fill = time.perf_counter()
+    pls[0].dynamics()          # fill the pipeline: dynamics of the first batch
+    print(f"[bench] pipeline filled: dynamics of batch 0 in {time.perf_counter() - t_fill:.1f}s", file=sys.stderr,
+          flush=True)
+
+    def iteration(i):
+        cur, nxt = pls[i % 2], pls[(i + 1) % 2]
+        th = threading.Thread(target=nxt.dynamics)
+        th.start()
+        cur.electrons()
+        cur.observer()
+        th.join()
+        print(f"[bench] step {i} done (batch {i} light curves, batch {i + 1} dynamics)", file=sys.stderr, flush=True)
+        dst, est, lst = cur.status()
+        if dst.any() or est.any() or lst.any():
+            print(f"bench.py: non-zero status dyn={sorted(set(dst.tolist()))} ele={sorted(set(est.tolist()))} "
+                  f"eats={sorted(set(lst.tolist()))}", file=sys.stderr)
+            sys.exit(3)
+
+    for i in range(args.warmup):
+        iteration(i)
+    torch.cuda.synchronize()
+    for st in range(4):
+        gpu.timing_read(st, reset=True)
+        gpu.timing_units(st, reset=True)
+    t0 = time.perf_counter()
+    for i in range(args.warmup, args.warmup + args.steps):
+        iteration(i)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stage_ms = {s: gpu.timing_read(s) for s in range(4)}
     if args.stages:
-        for k, (ms, n) in stage_ms.items():
-            print(f"[stage] {k}: {ms:.3f} ms over {n} launches", file=sys.stderr)
-        print(f"[stage] wall per step {dt / args.steps * 1e3:.1f} ms", file=sys.stderr)
-    dom = max(stage_ms, key=lambda k: stage_ms[k][0])
-    ms_dom, n_dom = stage_ms[dom]
-    avg_s = ms_dom / max(n_dom, 1) / 1e3
-    work = algorithmic_work(pl)
-    roof = {"kernel": dom, "avg_launch_ms": ms_dom / max(n_dom, 1), "bound": "mfma", "achieved": None,
-            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": None, "traffic": None,
-            "dtype_peak": "fp64"}
-    if dom == "ele_kernel":
-        ach = work["ele"] / avg_s / 1e12
-        roof.update(achieved=ach, frac=ach / PEAK_FP64_TFLOPS)
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.workload, args.cpu_seconds)
+        for s, (ms, n) in stage_ms.items():
+            print(f"[stage] {s}: {ms:.1f} ms over {n} launches", file=sys.stderr)
+        print(f"[stage] wall per step {dt / args.steps * 1e3:.1f} ms; eats samples "
+              f"{gpu.timing_units(gpu.STAGE_EATS):.3e}", file=sys.stderr)
+    roof = roofline(stage_ms, pls, gpu)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload, args.cpu_seconds)
     lcs = args.batch * args.steps / dt
     line = {
         "metric": "light curves/sec (96nu x128t), plain FP64 vs mixed+surrogate GPU @<=1% err",
@@ -182,7 +226,7 @@ def main():
         "dtype": "f64" if args.precision == 0 else "f64+bf16", "data": "synthetic",
         "config": {"workload": args.workload, "runs_per_step": args.batch, "layers": run.n_layers,
                    "ngam": run.params.fs.ngam, "nfreq": run.params.fs.nfreq, "nt": len(run.tgrid),
-                   "pairs": len(run.tobs), "precision": args.precision},
+                   "pairs": len(run.tobs), "precision": args.precision, "pipeline": "double-buffered"},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

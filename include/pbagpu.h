@@ -145,8 +145,9 @@ const char *pbagpu_last_error(void);
 /* ---------------------------------------------------- kernel timing */
 /* No counterpart in the reference: instrumentation for benchmarks.  When
  * enabled, every stage records HIP events on the caller's stream immediately
- * before and after its main kernel and accumulates the elapsed device time.
- * Not thread safe; meant for one benchmarking thread. */
+ * before and after its main kernel and accumulates the elapsed device time
+ * (guarded by a mutex: stages may be driven from several host threads, one
+ * stream each). */
 enum {
     PBAGPU_STAGE_DYN = 0,      /* dyn_kernel */
     PBAGPU_STAGE_ELE = 1,      /* ele_kernel */
@@ -156,6 +157,10 @@ enum {
 };
 int pbagpu_timing_enable(int on);
 int pbagpu_timing_read(int stage, double *ms_total, long long *launches, int reset);
+/* Work units counted while timing is enabled: integrand samples of the observer
+ * stage (PBAGPU_STAGE_EATS); other stages report 0 (their units follow from the
+ * per-blastwave results and the step flags on the host). */
+int pbagpu_timing_units(int stage, double *units, int reset);
 
 /* ------------------------------------------------------------- stage 1 */
 /* Evolve n_runs independent models of bw_per_run blastwaves each.

@@ -13,19 +13,37 @@ extern "C" int pbagpu_set_device(int device) {
 extern "C" int pbagpu_version(void) { return 1; }
 
 // ------------------------------------------------------------------ kernel timing
+#include <mutex>
+
 namespace pbagpu {
 static bool g_timing = false;
+static std::mutex g_timing_mu;   // stages may run on several host threads (one stream each)
 static double g_ms[PBAGPU_N_STAGES];
 static long long g_launches[PBAGPU_N_STAGES];
+static double g_units[PBAGPU_N_STAGES];
 
 bool timing_enabled() { return g_timing; }
 
 void timing_add(int stage, double ms) {
     if (stage < 0 || stage >= PBAGPU_N_STAGES) return;
+    std::lock_guard<std::mutex> lk(g_timing_mu);
     g_ms[stage] += ms;
     g_launches[stage] += 1;
 }
+void timing_add_units(int stage, double units) {
+    if (stage < 0 || stage >= PBAGPU_N_STAGES) return;
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    g_units[stage] += units;
+}
 }  // namespace pbagpu
+
+extern "C" int pbagpu_timing_units(int stage, double *units, int reset) {
+    if (stage < 0 || stage >= PBAGPU_N_STAGES) return PBAGPU_E_ARG;
+    std::lock_guard<std::mutex> lk(pbagpu::g_timing_mu);
+    if (units) *units = pbagpu::g_units[stage];
+    if (reset) pbagpu::g_units[stage] = 0.0;
+    return PBAGPU_OK;
+}
 
 extern "C" int pbagpu_timing_enable(int on) {
     pbagpu::g_timing = on != 0;
@@ -34,6 +52,7 @@ extern "C" int pbagpu_timing_enable(int on) {
 
 extern "C" int pbagpu_timing_read(int stage, double *ms_total, long long *launches, int reset) {
     if (stage < 0 || stage >= PBAGPU_N_STAGES) return PBAGPU_E_ARG;
+    std::lock_guard<std::mutex> lk(pbagpu::g_timing_mu);
     if (ms_total) *ms_total = pbagpu::g_ms[stage];
     if (launches) *launches = pbagpu::g_launches[stage];
     if (reset) { pbagpu::g_ms[stage] = 0.0; pbagpu::g_launches[stage] = 0; }

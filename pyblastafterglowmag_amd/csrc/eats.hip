@@ -496,7 +496,8 @@ __global__ void __launch_bounds__(EATS_WG, 2) eats_kernel(const LayerArgs *__res
                                                        int chunk, int *__restrict__ next_chunk,
                                                        const double *__restrict__ tobs,
                                                        const double *__restrict__ nuobs, double *__restrict__ out,
-                                                       int *__restrict__ status, char *__restrict__ ws) {
+                                                       int *__restrict__ status, char *__restrict__ ws,
+                                                       unsigned long long *__restrict__ n_samples) {
     __shared__ int s_chunk, s_next;
     const int chunks_per_layer = (n_pairs + chunk - 1) / chunk;
     const int n_chunks = chunks_per_layer * n_layers;
@@ -542,6 +543,7 @@ __global__ void __launch_bounds__(EATS_WG, 2) eats_kernel(const LayerArgs *__res
         EdgeSearch E;
         enum { NEED_PAIR, NEED_INNER, EDGE, EVAL, DONE };
         int mode = NEED_PAIR, k = 0, st_acc = 0;
+        unsigned long long samples = 0;
         double flux = 0.;
         for (;;) {
             // bookkeeping until this lane needs an edge probe or an integrand sample
@@ -591,6 +593,7 @@ __global__ void __launch_bounds__(EATS_WG, 2) eats_kernel(const LayerArgs *__res
             if (mode != EVAL) continue;
             // the one integrand call of the loop
             const double v = integrand(C, 1 - Mi.x);
+            samples++;
             if (Mi.feed(v)) continue;
             double res = Mi.result;
             if (Mi.S.overflow) C.status |= PBAGPU_S_HEAP_FULL;
@@ -609,6 +612,7 @@ __global__ void __launch_bounds__(EATS_WG, 2) eats_kernel(const LayerArgs *__res
             }
         }
         if (st_acc) atomicOr(&status[l], st_acc);
+        if (n_samples) atomicAdd(n_samples, samples);
     }
 }
 
@@ -685,11 +689,22 @@ extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, 
         if (rc == PBAGPU_OK) PBA_TRY(rc, hipMemsetAsync(counter, 0, sizeof(int), s));
         if (rc == PBAGPU_OK) {
             (void)hipFuncSetAttribute((const void *)eats_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            unsigned long long *d_samples = nullptr;
+            if (pbagpu::timing_enabled()) {
+                d_samples = (unsigned long long *)pbagpu::workspace(pbagpu::WS_EATS_SAMPLES, sizeof(unsigned long long));
+                if (d_samples) PBA_TRY(rc, hipMemsetAsync(d_samples, 0, sizeof(unsigned long long), s));
+            }
             timer.start(s);
             hipLaunchKernelGGL(eats_kernel, dim3((unsigned)n_wg), dim3(EATS_WG), lds, s, dL, n_layers, n_pairs, chunk,
-                               counter, d_tobs, d_nuobs, d_out, d_status, ws);
+                               counter, d_tobs, d_nuobs, d_out, d_status, ws, d_samples);
             PBA_TRY(rc, hipGetLastError());
             timer.stop(s);
+            if (d_samples && rc == PBAGPU_OK) {
+                unsigned long long hs = 0;
+                PBA_TRY(rc, hipMemcpyAsync(&hs, d_samples, sizeof(hs), hipMemcpyDeviceToHost, s));
+                PBA_TRY(rc, hipStreamSynchronize(s));
+                pbagpu::timing_add_units(PBAGPU_STAGE_EATS, (double)hs);
+            }
         }
     }
     PBA_TRY(rc, hipStreamSynchronize(s));

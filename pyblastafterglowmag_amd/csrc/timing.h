@@ -9,6 +9,7 @@ namespace pbagpu {
 
 bool timing_enabled();
 void timing_add(int stage, double ms);
+void timing_add_units(int stage, double units);
 
 class KernelTimer {
   public:

@@ -15,7 +15,7 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "li
 _lib = None
 
 EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu_last_error", "pbagpu_timing_enable",
-           "pbagpu_timing_read", "pbagpu_dyn_evolve", "pbagpu_ele_evolve", "pbagpu_eats_lightcurve")
+           "pbagpu_timing_read", "pbagpu_timing_units", "pbagpu_dyn_evolve", "pbagpu_ele_evolve", "pbagpu_eats_lightcurve")
 STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE = 0, 1, 2, 3
 
 
@@ -40,6 +40,7 @@ def lib():
         L.pbagpu_last_error.argtypes = []
         L.pbagpu_timing_enable.argtypes = [C.c_int]
         L.pbagpu_timing_read.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]
+        L.pbagpu_timing_units.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int]
         L.pbagpu_eats_lightcurve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsLayer), C.c_int, vp,
                                              C.c_int, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         _lib = L
@@ -55,6 +56,12 @@ def timing_read(stage, reset=False):
     ms, n = C.c_double(0.0), C.c_longlong(0)
     _check(lib().pbagpu_timing_read(stage, C.byref(ms), C.byref(n), 1 if reset else 0), "timing_read")
     return ms.value, n.value
+
+
+def timing_units(stage, reset=False):
+    u = C.c_double(0.0)
+    _check(lib().pbagpu_timing_units(stage, C.byref(u), 1 if reset else 0), "timing_units")
+    return u.value
 
 
 def _torch():
