@@ -377,6 +377,128 @@ __global__ void __launch_bounds__(64) ssc_step_kernel(const int *__restrict__ pl
     if (act) S.jssc[((size_t)q0 * nf + i) * SSC_JB + blockIdx.z] = part;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Mixed-precision SSC contraction (precision = 1): the same sum as ssc_step_kernel, on MFMA in
+// BF16 with FP32 accumulation.  Written as a GEMM over the flattened (j, k) index:
+//   C[i][p] = sum_{j in block} sum_k K[i][j][k] * (an_j^p * wn_k^p)
+// with an_j = a_j / max_{j in block} a_j, a_j = f_j (gamma_{j+1}-gamma_j) (3/4 sigma_T c)/gamma_j^2,
+// and wn_k = w_k / max_k w_k, so every operand lies in [0, 1] whatever the decades the spectra
+// span; the block and photon scales multiply the FP32 result back in FP64.  Both operands are
+// split into a BF16 pair x = hi + lo and the product is hi*hi + hi*lo + lo*hi (the lo*lo term is
+// below FP32 resolution), which keeps ~16 significant bits per operand.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short bf16_rn(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even (operands are finite)
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
+
+// K (FP64 [nf][ng][nf]) -> BF16 pair [nf][ng][kp], k padded with zeros to kp (multiple of 16)
+__global__ void ssc_table_bf16_kernel(const double *__restrict__ K, int nf, int ng, int kp,
+                                      unsigned short *__restrict__ Khi, unsigned short *__restrict__ Klo) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)nf * ng * kp;
+    if (idx >= n) return;
+    int k = (int)(idx % kp);
+    size_t ij = idx / kp;
+    float v = k < nf - 1 ? (float)K[ij * nf + k] : 0.f;   // the last frequency carries no photons
+    unsigned short h = bf16_rn(v);
+    Khi[idx] = h;
+    Klo[idx] = bf16_rn(v - bf16_to_f(h));
+}
+
+// Per population of the list: normalised operands and their scales for this step.
+//   an [npk][ng]  (FP32), amax [npk][SSC_JB] (FP64), wn [npk][kp] (FP32), wmax [npk] (FP64)
+__global__ void __launch_bounds__(64) ssc_prep_kernel(const int *__restrict__ plist, int npk, int n_pop,
+                                                      const double *__restrict__ e, int ng, int nf, int kp,
+                                                      PopState S, float *__restrict__ an, double *__restrict__ amax,
+                                                      float *__restrict__ wn, double *__restrict__ wmax) {
+    const int q = blockIdx.x, lane = threadIdx.x;
+    const int q0 = plist[q];
+    const bool act = S.flag[3 * q0 + 1] != 0;
+    const double c0 = 3. / 4. * kSigmaT * kC;
+    const int jn = ng - 1, js = (jn + SSC_JB - 1) / SSC_JB;
+    for (int jb = 0; jb < SSC_JB; jb++) {
+        const int j0 = jb * js, j1 = min(j0 + js, jn);
+        double m = 0.;
+        for (int j = j0 + lane; j < j1; j += 64) {
+            double ej = e[j];
+            double aj = act ? S.ft[(size_t)j * n_pop + q0] * (e[j + 1] - e[j]) * (c0 / (ej * ej)) : 0.;
+            m = fmax(m, fabs(aj));
+        }
+        for (int d = 32; d >= 1; d >>= 1) m = fmax(m, __shfl_xor(m, d, 64));
+        for (int j = j0 + lane; j < j1; j += 64) {
+            double ej = e[j];
+            double aj = act ? S.ft[(size_t)j * n_pop + q0] * (e[j + 1] - e[j]) * (c0 / (ej * ej)) : 0.;
+            an[(size_t)q * ng + j] = m > 0. ? (float)(aj / m) : 0.f;
+        }
+        if (lane == 0) amax[(size_t)q * SSC_JB + jb] = m;
+    }
+    double w = 0.;
+    for (int k = lane; k < nf - 1; k += 64) w = fmax(w, fabs(act ? S.wt[(size_t)k * n_pop + q0] : 0.));
+    for (int d = 32; d >= 1; d >>= 1) w = fmax(w, __shfl_xor(w, d, 64));
+    for (int k = lane; k < kp; k += 64) {
+        double wk = (act && k < nf - 1) ? S.wt[(size_t)k * n_pop + q0] : 0.;
+        wn[(size_t)q * kp + k] = w > 0. ? (float)(wk / w) : 0.f;
+    }
+    if (lane == 0) wmax[q] = w;
+}
+
+// One wavefront = one 16 (frequencies i) x 16 (populations) tile of one j-block.
+// grid = (ceil(nf/16), ceil(npk/16), SSC_JB)
+__global__ void __launch_bounds__(64) ssc_mfma_kernel(const int *__restrict__ plist, int npk, const double *__restrict__ nu,
+                                                      int ng, int nf, int kp, const unsigned short *__restrict__ Khi,
+                                                      const unsigned short *__restrict__ Klo,
+                                                      const float *__restrict__ an, const double *__restrict__ amax,
+                                                      const float *__restrict__ wn, const double *__restrict__ wmax,
+                                                      PopState S) {
+    const int lane = threadIdx.x;
+    const int i0 = blockIdx.x * 16, p0 = blockIdx.y * 16, jb = blockIdx.z;
+    const int jn = ng - 1, js = (jn + SSC_JB - 1) / SSC_JB;
+    const int j0 = jb * js, j1 = min(j0 + js, jn);
+    // operand coordinates of this lane (bf16 16x16x16: A row = lane%16, k = 4*(lane/16)+r;
+    // B column = lane%16, k = 4*(lane/16)+r; D row = 4*(lane/16)+r, column = lane%16)
+    const int ia = i0 + (lane & 15);
+    const int pb = p0 + (lane & 15);
+    const int kq = 4 * (lane >> 4);
+    const bool arow = ia < nf, bcol = pb < npk;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int j = j0; j < j1; j++) {
+        const float anj = bcol ? an[(size_t)pb * ng + j] : 0.f;
+        for (int kc = 0; kc < kp; kc += 16) {
+            bf16x4 ahi, alo, bhi, blo;
+            if (arow) {
+                const size_t o = ((size_t)ia * ng + j) * kp + kc + kq;
+                const unsigned short *ph = Khi + o, *pl = Klo + o;
+                for (int r = 0; r < 4; r++) { ahi[r] = (short)ph[r]; alo[r] = (short)pl[r]; }
+            } else {
+                for (int r = 0; r < 4; r++) { ahi[r] = 0; alo[r] = 0; }
+            }
+            for (int r = 0; r < 4; r++) {
+                float v = bcol ? anj * wn[(size_t)pb * kp + kc + kq + r] : 0.f;
+                unsigned short h = bf16_rn(v);
+                bhi[r] = (short)h;
+                blo[r] = (short)bf16_rn(v - bf16_to_f(h));
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, bhi, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, blo, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bhi, acc, 0, 0, 0);
+        }
+    }
+    const int pc = p0 + (lane & 15);
+    if (pc >= npk) return;
+    const int q0 = plist[pc];
+    if (S.flag[3 * q0 + 1] == 0) return;
+    const double scale = amax[(size_t)pc * SSC_JB + jb] * wmax[pc];
+    for (int r = 0; r < 4; r++) {
+        const int i = i0 + 4 * (lane >> 4) + r;
+        if (i < nf) S.jssc[((size_t)q0 * nf + i) * SSC_JB + jb] = kH * nu[i] * (scale * (double)acc[r]);
+    }
+}
+
 __global__ void ele_status_kernel(const int *__restrict__ flag, int n_pop, int *__restrict__ status) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n_pop) status[q] = flag[3 * q + 2];
@@ -440,7 +562,7 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
                                  int *d_status, int precision, void *stream) {
     using namespace pbagpu::ele;
     if (!params || !pops || !d_table || !d_jall || !d_status || n_pop < 1 || nt < 1 || n_runs < 1) return PBAGPU_E_ARG;
-    if (precision != 0) return PBAGPU_E_UNSUPPORTED;
+    if (precision != 0 && precision != 1) return PBAGPU_E_UNSUPPORTED;
     hipStream_t s = (hipStream_t)stream;
     const pba_shock_params &s0 = pops[0].shock ? params[pops[0].run].rs : params[pops[0].run].fs;
     const int ng = s0.ngam, nf = s0.nfreq;
@@ -506,6 +628,20 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
                            KI);
         PBA_TRY(rc, hipGetLastError());
     }
+    // BF16 pair of every Compton table for the mixed-precision contraction
+    const int kp = (nf + 15) / 16 * 16;
+    const size_t kbf = (size_t)nf * ng * kp;
+    unsigned short *dKb = nullptr;
+    if (precision == 1 && n_ssc) {
+        dKb = (unsigned short *)pbagpu::workspace(pbagpu::WS_ELE_KBF, sizeof(unsigned short) * 2 * kbf * n_ssc);
+        if (!dKb) return PBAGPU_E_HIP;
+        for (size_t t = 0; t < n_ssc && rc == PBAGPU_OK; t++) {
+            const double *K = dK + (ktab + (size_t)ng * nf) * t;
+            hipLaunchKernelGGL(ssc_table_bf16_kernel, dim3((unsigned)((kbf + 255) / 256)), dim3(256), 0, s, K, nf, ng, kp,
+                               dKb + 2 * kbf * t, dKb + 2 * kbf * t + kbf);
+            PBA_TRY(rc, hipGetLastError());
+        }
+    }
     if (n_ssc) ttab.stop(s);
     std::vector<PopDesc> hp(n_pop);
     for (int q = 0; q < n_pop; q++) {
@@ -542,6 +678,19 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
     int *dlist = hlist.empty() ? nullptr
                                : (int *)pbagpu::workspace(pbagpu::WS_ELE_LIST, sizeof(int) * hlist.size());
     if (!wsd || (!hlist.empty() && !dlist)) return PBAGPU_E_HIP;
+    // operands of the mixed-precision contraction (per population of the SSC lists)
+    float *d_an = nullptr, *d_wn = nullptr;
+    double *d_amax = nullptr, *d_wmax = nullptr;
+    if (precision == 1 && !hlist.empty()) {
+        const size_t nl = hlist.size();
+        char *mw = (char *)pbagpu::workspace(pbagpu::WS_ELE_MIXED,
+                                             nl * (sizeof(float) * (ng + kp) + sizeof(double) * (SSC_JB + 1)));
+        if (!mw) return PBAGPU_E_HIP;
+        d_amax = (double *)mw;
+        d_wmax = d_amax + nl * SSC_JB;
+        d_an = (float *)(d_wmax + nl);
+        d_wn = d_an + nl * ng;
+    }
     PopState S;
     S.f = wsd; S.ft = S.f + np * ng; S.tf = S.ft + np * ng; S.wt = S.tf + np * nf;
     S.jsyn = S.wt + np * nf; S.jssc = S.jsyn + np * nf; S.scal = S.jssc + np * nf * SSC_JB;
@@ -564,8 +713,19 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
                 const double *base = dgrids + per_key * kk;
                 const double *e = base, *nu = base + ng + (ng - 1) + (ng + 1) + ng;
                 const double *K = dK + (ktab + (size_t)ng * nf) * t;
-                hipLaunchKernelGGL(ssc_step_kernel, dim3((unsigned)((npk + 63) / 64), (unsigned)nf, (unsigned)SSC_JB),
-                                   dim3(64), ssc_lds, s, dlist + list_off[t], npk, n_pop, K, e, nu, ng, nf, S);
+                if (precision == 1) {
+                    const size_t lo = list_off[t];
+                    hipLaunchKernelGGL(ssc_prep_kernel, dim3((unsigned)npk), dim3(64), 0, s, dlist + lo, npk, n_pop, e, ng,
+                                       nf, kp, S, d_an + lo * ng, d_amax + lo * SSC_JB, d_wn + lo * kp, d_wmax + lo);
+                    hipLaunchKernelGGL(ssc_mfma_kernel, dim3((unsigned)((nf + 15) / 16), (unsigned)((npk + 15) / 16),
+                                                             (unsigned)SSC_JB),
+                                       dim3(64), 0, s, dlist + lo, npk, nu, ng, nf, kp, dKb + 2 * kbf * t,
+                                       dKb + 2 * kbf * t + kbf, d_an + lo * ng, d_amax + lo * SSC_JB, d_wn + lo * kp,
+                                       d_wmax + lo, S);
+                } else {
+                    hipLaunchKernelGGL(ssc_step_kernel, dim3((unsigned)((npk + 63) / 64), (unsigned)nf, (unsigned)SSC_JB),
+                                       dim3(64), ssc_lds, s, dlist + list_off[t], npk, n_pop, K, e, nu, ng, nf, S);
+                }
             }
             hipLaunchKernelGGL(ele_emit_kernel, dim3(n_pop), dim3(64), 0, s, dpops, it, nt, S);
             PBA_TRY(rc, hipGetLastError());

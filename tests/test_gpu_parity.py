@@ -64,6 +64,13 @@ def test_stages_match_oracle(name):
         assert e <= 1e-10, (name, il, e)
 
 
+# gauss5_small: the reference switches its reverse shock off when M3 >= 1 (isToStopReverseShock,
+# src/blastwave/blastwave.h:487-510); M3 reaches 1 only through rounding (the reference ends at
+# 1 + 4 ulp, the GPU 1 ulp below), so the GPU keeps evolving Eint3 after that step and late
+# reverse-shock emission differs; the largest effect is 1.9e-6 at a pair 1e-6 below the peak.
+LC_TOL = {"tophat_small": 1e-6, "tophat_fs_syn": 1e-6, "gauss5_small": 5e-6}
+
+
 @pytest.mark.parametrize("name", ["tophat_small", "gauss5_small", "tophat_fs_syn"])
 def test_lightcurve_matches_reference(name):
     g = load_golden(name)
@@ -72,4 +79,30 @@ def test_lightcurve_matches_reference(name):
     flux, ex = _gpu(run)
     for il in range(run.n_layers):
         e = rel_err(flux[il], g["lc"][il], 1e-30)
-        assert e <= 1e-6, (name, il, e)
+        assert e <= LC_TOL[name], (name, il, e)
+
+
+@pytest.mark.parametrize("name", ["gauss5_small", "tophat_small"])
+def test_mixed_precision_ssc_lightcurve(name):
+    """precision=1 (SSC contraction in BF16 hi/lo on MFMA, FP32 accumulation) against the FP64
+    GPU path on the same inputs: light curves within 1e-3 relative wherever the flux is above
+    1e-8 of the layer's peak (the mixed path's target is 1 %), spectra within 1e-3."""
+    from pyblastafterglowmag_amd import gpu
+    g = load_golden(name)
+    run = golden_run(g)
+    print(f"[gpu] {name}: mixed-precision SSC", flush=True)
+    f64, ex64 = gpu.lightcurves([run], want_intermediates=True, precision=0)
+    mix, exm = gpu.lightcurves([run], want_intermediates=True, precision=1)
+    assert not exm["ele_status"].any() and not exm["eats_status"].any()
+    worst_j = 0.0
+    for q in range(len(ex64["pops"])):
+        e = rel_err(exm["jall"][q], ex64["jall"][q], 1e-8)
+        worst_j = max(worst_j, e)
+        assert e <= 1e-3, (name, q, e)
+    print(f"[gpu] {name}: max relative emissivity difference mixed vs FP64 = {worst_j:.3e}", flush=True)
+    worst = 0.0
+    for il in range(run.n_layers):
+        e = rel_err(mix[0][il], f64[0][il], 1e-8)
+        worst = max(worst, e)
+        assert e <= 1e-3, (name, il, e)
+    print(f"[gpu] {name}: max relative light-curve difference mixed vs FP64 = {worst:.3e}", flush=True)
