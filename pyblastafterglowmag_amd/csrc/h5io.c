@@ -148,3 +148,90 @@ int h5io_write_lc(const char *path, int nshells, int nlayers, long npairs,
     H5Fclose(f);
     return rc ? -3 : 0;
 }
+
+/* ---- incremental writer: open a file, add 1-D datasets and scalar attributes to groups
+ * (created on first use; "" or "/" is the root group), close.  Used for the light-curve,
+ * dynamics and spectrum files in the reference's layout (src/ejecta/ejecta.h:165-222, 354-595). */
+static hid_t group_of(hid_t f, const char *group) {
+    if (!group || !group[0] || (group[0] == '/' && !group[1])) return H5Gopen2(f, "/", H5P_DEFAULT);
+    if (H5Lexists(f, group, H5P_DEFAULT) > 0) return H5Gopen2(f, group, H5P_DEFAULT);
+    return H5Gcreate2(f, group, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+}
+
+long h5io_create(const char *path) {
+    quiet();
+    hid_t f = H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    return (long)f;
+}
+
+int h5io_add_dataset(long file, const char *group, const char *name, long n, const double *data) {
+    hid_t g = group_of((hid_t)file, group);
+    if (g < 0) return -1;
+    int rc = write_1d(g, name, n, data);
+    H5Gclose(g);
+    return rc;
+}
+
+int h5io_add_attr(long file, const char *group, const char *name, double v) {
+    hid_t g = group_of((hid_t)file, group);
+    if (g < 0) return -1;
+    int rc = 0;
+    if (H5Aexists(g, name) <= 0) rc = write_attr(g, name, v);
+    H5Gclose(g);
+    return rc;
+}
+
+int h5io_close(long file) { return H5Fclose((hid_t)file) < 0 ? -1 : 0; }
+
+/* number of attributes on an object (for layout checks) and the name of attribute i */
+int h5io_attr_count(const char *path, const char *obj) {
+    quiet();
+    hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    H5O_info_t info;
+    int n = -2;
+    if (H5Oget_info_by_name(f, obj, &info, H5P_DEFAULT) >= 0) n = (int)info.num_attrs;
+    H5Fclose(f);
+    return n;
+}
+
+int h5io_attr_name(const char *path, const char *obj, int i, char *buf, int len) {
+    quiet();
+    hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    hid_t o = H5Oopen(f, obj, H5P_DEFAULT);
+    int rc = -2;
+    if (o >= 0) {
+        hid_t a = H5Aopen_by_idx(o, ".", H5_INDEX_NAME, H5_ITER_INC, (hsize_t)i, H5P_DEFAULT, H5P_DEFAULT);
+        if (a >= 0) { rc = H5Aget_name(a, (size_t)len, buf) >= 0 ? 0 : -3; H5Aclose(a); }
+        H5Oclose(o);
+    }
+    H5Fclose(f);
+    return rc;
+}
+
+/* number of links (datasets, groups) in a group and the name of link i (by name order) */
+int h5io_link_count(const char *path, const char *group) {
+    quiet();
+    hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    hid_t g = H5Gopen2(f, group, H5P_DEFAULT);
+    int n = -2;
+    if (g >= 0) {
+        H5G_info_t info;
+        if (H5Gget_info(g, &info) >= 0) n = (int)info.nlinks;
+        H5Gclose(g);
+    }
+    H5Fclose(f);
+    return n;
+}
+
+int h5io_link_name(const char *path, const char *group, int i, char *buf, int len) {
+    quiet();
+    hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    int rc = H5Lget_name_by_idx(f, group, H5_INDEX_NAME, H5_ITER_INC, (hsize_t)i, buf, (size_t)len, H5P_DEFAULT) >= 0
+                 ? 0 : -2;
+    H5Fclose(f);
+    return rc;
+}

@@ -22,7 +22,73 @@ def _load():
         _lib.h5io_read_attr.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, dp]
         _lib.h5io_write_lc.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_long,
                                        dp, dp, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), dp]
+        _lib.h5io_create.argtypes = [ctypes.c_char_p]
+        _lib.h5io_create.restype = ctypes.c_long
+        _lib.h5io_add_dataset.argtypes = [ctypes.c_long, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_long, dp]
+        _lib.h5io_add_attr.argtypes = [ctypes.c_long, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double]
+        _lib.h5io_close.argtypes = [ctypes.c_long]
+        _lib.h5io_attr_count.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        _lib.h5io_attr_name.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        _lib.h5io_link_count.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        _lib.h5io_link_name.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
     return _lib
+
+
+class Writer:
+    """Incremental HDF5 writer: 1-D float64 datasets and scalar float64 attributes in groups."""
+
+    def __init__(self, path):
+        self.f = _load().h5io_create(path.encode())
+        if self.f < 0:
+            raise IOError(f"cannot create {path}")
+        self.path = path
+
+    def dataset(self, group, name, data):
+        a = np.ascontiguousarray(np.asarray(data, dtype=np.float64).ravel())
+        if _load().h5io_add_dataset(self.f, group.encode(), name.encode(), a.size, _dp(a)) != 0:
+            raise IOError(f"{self.path}: dataset {group}/{name}")
+
+    def attr(self, group, name, value):
+        if _load().h5io_add_attr(self.f, group.encode(), name.encode(), float(value)) != 0:
+            raise IOError(f"{self.path}: attribute {group}@{name}")
+
+    def close(self):
+        if self.f is not None:
+            _load().h5io_close(self.f)
+            self.f = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def attr_names(path, obj="/"):
+    lib = _load()
+    n = lib.h5io_attr_count(path.encode(), obj.encode())
+    if n < 0:
+        raise KeyError(f"{obj} in {path}: rc={n}")
+    out = []
+    buf = ctypes.create_string_buffer(256)
+    for i in range(n):
+        if lib.h5io_attr_name(path.encode(), obj.encode(), i, buf, 256) == 0:
+            out.append(buf.value.decode())
+    return out
+
+
+def members(path, group="/"):
+    """Names of the datasets and groups in a group (name order)."""
+    lib = _load()
+    n = lib.h5io_link_count(path.encode(), group.encode())
+    if n < 0:
+        raise KeyError(f"{group} in {path}: rc={n}")
+    out = []
+    buf = ctypes.create_string_buffer(256)
+    for i in range(n):
+        if lib.h5io_link_name(path.encode(), group.encode(), i, buf, 256) == 0:
+            out.append(buf.value.decode())
+    return out
 
 
 def _dp(a):

@@ -1,0 +1,119 @@
+"""HDF5 outputs in the reference program's layout, written from the GPU stages' buffers.
+
+* light curves  ``Ejecta::saveDenseLightCurves``     (src/ejecta/ejecta.h:511-573)
+* dynamics      ``Ejecta::saveEjectaBWsDynamics``    (src/ejecta/ejecta.h:165-222)
+* spectra       ``Ejecta::computeSaveEjectaSpectrum`` (src/ejecta/ejecta.h:354-509): the electron
+  spectrum (``n_ele_*``) and the total comoving emissivity (``total_j_*``) per step, with the
+  ``times_freqs/freqs/times_gams/gams`` grids; the GPU path does not keep the separate
+  synchrotron/SSC parts, intensities or cooling rates, so those datasets are not written.
+
+Group names are ``"shell=0 layer=<i>"``; file attributes are nshells, nlayers (and ntimes,
+ncells for dynamics/spectra) plus every numeric parameter of the main and GRB parfile sections.
+"""
+import math
+
+import numpy as np
+
+from . import abi, h5io
+
+C_LIGHT = 2.99792458e10
+M_PROTON = 1.6726e-24
+
+# BW::VARS for the forward-shock and forward+reverse-shock blastwave types, in the reference's
+# order (src/blastwave/blastwave_pars.h:255-274), with their BW::VARNAMES (:230-250).
+DYN_FS = ["R", "Rsh", "rho", "drhodr", "GammaRel", "Gamma", "beta", "mom", "Eint2", "U_p", "theta", "ctheta",
+          "Erad2", "Esh2", "Ead2", "M2", "tcomov", "tburst", "tt", "thickness", "adi", "rho2", "GammaFsh",
+          "gamma_min", "gamma_c", "gamma_max", "B", "ThetaTherm", "z_cool", "x", "nprime", "accel_frac"]
+DYN_FSRS = (DYN_FS[:23] + ["GammaRsh", "Rsh_rs"] + DYN_FS[23:] +
+            ["Eint3", "Erad3", "iEad3", "Esh3", "M3", "deltaR4", "Gamma43", "adi3", "rho4", "rho3",
+             "thichness_rs", "U_p3", "gamma_min_rs", "gamma_c_rs", "gamma_max_rs", "B_rs", "ThetaTherm_rs",
+             "z_cool_rs", "x_rs", "nprime_rs", "accel_frac_rs"])
+
+
+def file_attrs(mp_, gp, extra):
+    out = dict(extra)
+    for d in (mp_, gp):
+        for k, v in d.items():
+            out.setdefault(k, float(v))
+    return out
+
+
+def layer_attrs(run, il, dyn=False):
+    """Per-layer attributes (BlastWave Pars: E0, M0, Gamma0, ..., blastwave.h:73-241)."""
+    li = run.inits[il]
+    P = run.params
+    g0 = math.sqrt(li.mom0 * li.mom0 + 1.0)
+    beta0 = li.mom0 / g0
+    tb0 = float(run.tgrid[0])
+    M0 = li.M0 if li.M0 > 0 else li.E0 / (g0 * C_LIGHT * C_LIGHT)
+    rho = P.n_ism * (M_PROTON + 9.1094e-28)
+    rd = (3. / (4. * math.pi) * 1. / (C_LIGHT * C_LIGHT * M_PROTON) * li.E0 / (rho / M_PROTON * g0 * g0)) ** (1. / 3.)
+    out = {"E0": li.E0, "M0": M0, "Gamma0": g0, "beta0": beta0, "mom0": li.mom0, "R0": beta0 * C_LIGHT * tb0,
+           "Eint0": 0.0, "Rd": rd, "t0": tb0, "theta_b0": li.theta_c_h, "theta_a": 0.0}
+    if dyn:
+        out.update({"theta_c": li.theta_c, "theta_c_l": li.theta_c_l, "theta_c_h": li.theta_c_h, "cil": 1.0})
+    return out
+
+
+def write_lightcurves(path, run, flux, mp_, gp):
+    """flux: [n_layers, n_pairs] of one run, mJy."""
+    nl = run.n_layers
+    with h5io.Writer(path) as w:
+        for il in range(nl):
+            g = f"shell=0 layer={il}"
+            w.dataset(g, "fluxdens", flux[il])
+            for k, v in layer_attrs(run, il).items():
+                w.attr(g, k, v)
+        w.dataset("/", "times", run.tobs)
+        w.dataset("/", "freqs", run.nuobs)
+        for k, v in file_attrs(mp_, gp, {"nshells": 1.0, "nlayers": float(nl)}).items():
+            w.attr("/", k, v)
+
+
+def write_dynamics(path, run, table, fsrs, mp_, gp):
+    """table: [n_layers, PBA_NV, nt]; fsrs: per-layer flags from the dynamics results."""
+    nl, _, nt = table.shape
+    with h5io.Writer(path) as w:
+        for il in range(nl):
+            g = f"shell=0 layer={il}"
+            for name in (DYN_FSRS if fsrs[il] else DYN_FS):
+                w.dataset(g, name, table[il, abi.PV[name]] if name in abi.PV else np.zeros(nt))
+            for k, v in layer_attrs(run, il, dyn=True).items():
+                w.attr(g, k, v)
+        extra = {"nshells": 1.0, "nlayers": float(nl), "ntimes": float(nt), "ncells": float(nl)}
+        for k, v in file_attrs(mp_, gp, extra).items():
+            w.attr("/", k, v)
+
+
+def write_spectra(path, run, pops, jall, fall, ele_grid, nu_grid, mp_, gp):
+    """jall: [n_pop, nt, nf] total comoving emissivity per electron; fall: [n_pop, nt, ng]."""
+    nl = run.n_layers
+    nt = len(run.tgrid)
+    with h5io.Writer(path) as w:
+        for q, P in enumerate(pops):
+            g = f"shell=0 layer={P.bw}"
+            tag = "fs" if P.shock == 0 else "rs"
+            if fall is not None:
+                w.dataset(g, f"n_ele_{tag}", fall[q])
+            w.dataset(g, f"total_j_{tag}", jall[q])
+        w.dataset("/", "times_freqs", np.repeat(run.tgrid, len(nu_grid)))
+        w.dataset("/", "freqs", np.tile(nu_grid, nt))
+        w.dataset("/", "times_gams", np.repeat(run.tgrid, len(ele_grid)))
+        w.dataset("/", "gams", np.tile(ele_grid, nt))
+        extra = {"nshells": 1.0, "nlayers": float(nl), "ntimes": float(nt), "ncells": float(nl)}
+        for k, v in file_attrs(mp_, gp, extra).items():
+            w.attr("/", k, v)
+
+
+def ele_grid(shock_params):
+    """Chang-Cooper Lorentz-factor grid e_i = step^i, step = exp(ln(gam2)/n)
+    (Electrons::build_grid_chang_cooper, base_state.h:217-253)."""
+    n = int(shock_params.ngam)
+    step = math.exp(math.log(shock_params.gam2) / n)
+    return np.array([step ** i for i in range(n)])
+
+
+def freq_grid(shock_params):
+    """Comoving frequency grid by repeated multiplication (MakeLogspaceVec, utils.h:91-105)."""
+    from .inputs import logspace_vec
+    return logspace_vec(math.log10(shock_params.freq1), math.log10(shock_params.freq2), int(shock_params.nfreq))

@@ -1,0 +1,77 @@
+"""Drop-in driver: ``python -m pyblastafterglowmag_amd.pba <workdir> <parfile> [loglevel] [--precision N]``.
+
+Mirrors the reference CLI ``pba.out <workdir> <parfile> <loglevel>`` (src/main.cpp:208-384) for the
+GRB afterglow part of a parfile: it reads the same parfile and initial-data file, runs the three
+stages on the GPU (dynamics, electrons + emissivity, EATS light curves) and writes the files the
+reference writes for ``do_lc`` / ``save_dynamics`` / ``save_spec`` (``fname_light_curve``,
+``fname_dyn``, ``fname_spectrum``) in the same layout (see ``output``).  Errors end the process
+with status 1 after a message, as the reference does.
+"""
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="pba", description=__doc__.split("\n")[0])
+    ap.add_argument("workdir")
+    ap.add_argument("parfile")
+    ap.add_argument("loglevel", nargs="?", default="info")
+    ap.add_argument("--precision", type=int, default=0,
+                    help="0: FP64 everywhere; 1: SSC contraction in BF16 hi/lo on MFMA (FP32 accumulation)")
+    a = ap.parse_args(argv)
+    verbose = a.loglevel == "info"
+
+    from . import abi, gpu, output
+    workdir = a.workdir if a.workdir.endswith("/") else a.workdir + "/"
+    try:
+        P, mp_, mo, gp, go = abi.run_params_from_parfile(os.path.join(workdir, a.parfile))
+    except (NotImplementedError, KeyError, ValueError) as e:
+        print(f"[pba] unsupported or invalid parfile: {e}", file=sys.stderr)
+        return 1
+    if go.get("run_bws", "yes") != "yes":
+        print("[pba] run_bws=no: nothing to evolve", file=sys.stderr)
+        return 0
+    t0 = time.time()
+    run = gpu.Run.from_workdir(workdir, a.parfile)
+    save_spec = go.get("save_spec", "no") == "yes"
+    pl = gpu.Pipeline([run], precision=a.precision)
+    pl.dynamics()
+    dst = np.array([r.status for r in pl.res])
+    if dst.any():
+        print(f"[pba] blastwave evolution failed: status {dst.tolist()}", file=sys.stderr)
+        return 1
+    pl.electrons(want_f=save_spec)
+    pl.observer()
+    _, est, lst = pl.status()
+    if est.any() or lst.any():
+        print(f"[pba] microphysics/EATS failed: electrons {est.tolist()} light curve {lst.tolist()}", file=sys.stderr)
+        return 1
+    if verbose:
+        print(f"[pba] {run.n_layers} layers evolved and integrated in {time.time() - t0:.2f}s", flush=True)
+    if go.get("do_lc", "yes") == "yes":
+        path = workdir + go.get("fname_light_curve", "lc.h5")
+        output.write_lightcurves(path, run, pl.flux()[0], mp_, gp)
+        if verbose:
+            print(f"[pba] light curves -> {path}", flush=True)
+    if go.get("save_dynamics", "no") == "yes":
+        path = workdir + go.get("fname_dyn", "dyn.h5")
+        output.write_dynamics(path, run, pl.d_table.cpu().numpy(), [r.fsrs for r in pl.res], mp_, gp)
+        if verbose:
+            print(f"[pba] dynamics -> {path}", flush=True)
+    if save_spec:
+        path = workdir + go.get("fname_spectrum", "spec.h5")
+        npop = len(pl.pops)
+        sh = run.params.fs
+        output.write_spectra(path, run, pl.pops, pl.d_jall[:npop].cpu().numpy(), pl.d_fall.cpu().numpy(),
+                             output.ele_grid(sh), output.freq_grid(sh), mp_, gp)
+        if verbose:
+            print(f"[pba] spectra -> {path}", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -52,6 +52,12 @@ def run_reference(name, wd, spec_every=1):
     out["times"] = h5io.read(lc, "times")
     out["freqs"] = h5io.read(lc, "freqs")
     out["lc"] = np.stack([h5io.read(lc, f"shell=0 layer={il}/fluxdens") for il in range(nl)])
+    # file layout of the reference's outputs (names only) and the per-layer light-curve attributes
+    out["lc_root_members"] = np.array(h5io.members(lc, "/"))
+    out["lc_root_attrs"] = np.array(h5io.attr_names(lc, "/"))
+    names_la = h5io.attr_names(lc, "shell=0 layer=0")
+    out["lc_layer_attr_names"] = np.array(names_la)
+    out["lc_layer_attrs"] = np.array([[h5io.attr(lc, k, f"shell=0 layer={il}") for k in names_la] for il in range(nl)])
     dyn = os.path.join(wd, "dyn.h5")
     names = DYN_VARS + (DYN_VARS_RS if P.fsrs else [])
     nt = int(mp_["ntb"])
@@ -63,6 +69,9 @@ def run_reference(name, wd, spec_every=1):
                 D[il, iv] = h5io.read(dyn, ds)
     out["dyn"] = D
     out["dyn_names"] = np.array(names)
+    out["dyn_layer_members"] = np.array(h5io.members(dyn, "shell=0 layer=0"))
+    out["dyn_layer_attr_names"] = np.array(h5io.attr_names(dyn, "shell=0 layer=0"))
+    out["dyn_root_attrs"] = np.array(h5io.attr_names(dyn, "/"))
     spec = os.path.join(wd, "spec.h5")
     for tag in ("fs", "rs"):
         J, F = [], []
@@ -79,6 +88,8 @@ def run_reference(name, wd, spec_every=1):
             out[f"jall_{tag}"] = np.stack([J[i] for i in ok])
             out[f"fall_{tag}"] = np.stack(F)
     out["spec_every"] = np.array(spec_every)
+    out["spec_root_members"] = np.array(h5io.members(spec, "/"))
+    out["spec_layer_members"] = np.array(h5io.members(spec, "shell=0 layer=0"))
     return out
 
 

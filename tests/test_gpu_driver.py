@@ -1,0 +1,45 @@
+"""The drop-in driver (python -m pyblastafterglowmag_amd.pba <workdir> <parfile>) on the GPU:
+it reads the parfile and initial data the reference reads and writes lc.h5 / dyn.h5 / spec.h5 in
+the reference's layout with the reference's values (within the GPU-path tolerances)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, rel_err
+from pyblastafterglowmag_amd import abi, h5io, inputs, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    return workloads.small(name) if name in workloads.SMALL else workloads.WORKLOADS[name]()
+
+
+@pytest.mark.parametrize("name", ["tophat_small", "tophat_fs_syn"])
+def test_driver_outputs_match_reference(name, tmp_path):
+    from pyblastafterglowmag_amd import pba
+    g = load_golden(name)
+    main, grb, struct = _case(name)
+    grb = dict(grb, save_dynamics="yes", save_spec="yes")
+    wd = str(tmp_path) + "/"
+    inputs.write_run(wd, main, grb, struct)
+    print(f"[gpu] driver on {name}", flush=True)
+    assert pba.main([wd, "parfile.par", "err"]) == 0
+    lc = wd + "lc.h5"
+    assert sorted(h5io.members(lc, "/")) == sorted(str(x) for x in g["lc_root_members"])
+    np.testing.assert_array_equal(h5io.read(lc, "times"), g["times"])
+    for il in range(len(g["lc"])):
+        e = rel_err(h5io.read(lc, f"shell=0 layer={il}/fluxdens"), g["lc"][il], 1e-30)
+        assert e <= 1e-6, (name, il, e)
+    dyn = wd + "dyn.h5"
+    assert sorted(h5io.members(dyn, "shell=0 layer=0")) == sorted(str(x) for x in g["dyn_layer_members"])
+    for iv, v in enumerate(str(x) for x in g["dyn_names"]):
+        e = rel_err(h5io.read(dyn, f"shell=0 layer=0/{v}"), g["dyn"][0, iv], 1e-300)
+        assert e <= 2e-6, (name, v, e)
+    spec = wd + "spec.h5"
+    every = int(g["spec_every"])
+    P = abi.run_params_from_parfile(wd + "parfile.par")[0]
+    nt, nf = int(main.get("ntb", 1000)), P.fs.nfreq
+    j = h5io.read(spec, "shell=0 layer=0/total_j_fs").reshape(nt, nf)[::every]
+    assert rel_err(j, g["jall_fs"][0], 1e-12) <= 1e-6

@@ -1,0 +1,73 @@
+"""Output files in the reference program's layout (pyblastafterglowmag_amd.output): group and
+dataset names, attribute names and the per-layer attribute values, checked against what the
+reference program itself wrote for the same parfiles (recorded in the golden fixtures)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import golden_run, load_golden
+from pyblastafterglowmag_amd import abi, h5io, output
+
+CASES = ["tophat_small", "gauss5_small", "tophat_fs_syn"]
+
+
+def _parse(g, tmp_path):
+    p = tmp_path / "parfile.par"
+    p.write_bytes(bytes(g["parfile"]))
+    return abi.run_params_from_parfile(str(p))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_lightcurve_file_layout(name, tmp_path):
+    g = load_golden(name)
+    run = golden_run(g)
+    P, mp_, mo, gp, go = _parse(g, tmp_path)
+    path = str(tmp_path / "lc.h5")
+    output.write_lightcurves(path, run, g["lc"], mp_, gp)
+    assert sorted(h5io.members(path, "/")) == sorted(str(x) for x in g["lc_root_members"])
+    assert sorted(h5io.attr_names(path, "/")) == sorted(str(x) for x in g["lc_root_attrs"])
+    names = [str(x) for x in g["lc_layer_attr_names"]]
+    assert sorted(h5io.attr_names(path, "shell=0 layer=0")) == sorted(names)
+    for il in range(run.n_layers):
+        for k, ref in zip(names, g["lc_layer_attrs"][il]):
+            mine = h5io.attr(path, k, f"shell=0 layer={il}")
+            assert mine == pytest.approx(ref, rel=1e-12, abs=0), (name, il, k, mine, ref)
+        np.testing.assert_array_equal(h5io.read(path, f"shell=0 layer={il}/fluxdens"), g["lc"][il])
+    np.testing.assert_array_equal(h5io.read(path, "times"), g["times"])
+    np.testing.assert_array_equal(h5io.read(path, "freqs"), g["freqs"])
+    assert h5io.attr(path, "n_ism") == P.n_ism
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_dynamics_file_layout(name, tmp_path):
+    g = load_golden(name)
+    run = golden_run(g)
+    P, mp_, mo, gp, go = _parse(g, tmp_path)
+    nt = len(run.tgrid)
+    table = np.zeros((run.n_layers, abi.PBA_NV, nt))
+    table[:, abi.PV["Gamma"], :] = 2.0
+    fsrs = [P.fsrs] * run.n_layers
+    path = str(tmp_path / "dyn.h5")
+    output.write_dynamics(path, run, table, fsrs, mp_, gp)
+    assert sorted(h5io.members(path, "shell=0 layer=0")) == sorted(str(x) for x in g["dyn_layer_members"])
+    assert sorted(h5io.attr_names(path, "shell=0 layer=0")) == sorted(str(x) for x in g["dyn_layer_attr_names"])
+    assert sorted(h5io.attr_names(path, "/")) == sorted(str(x) for x in g["dyn_root_attrs"])
+    np.testing.assert_array_equal(h5io.read(path, "shell=0 layer=0/Gamma"), np.full(nt, 2.0))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_spectrum_file_grids(name, tmp_path):
+    """Grid datasets of the spectrum file equal the reference's (times_freqs/freqs/times_gams/gams),
+    and the datasets written per layer are among the reference's names."""
+    g = load_golden(name)
+    run = golden_run(g)
+    P, mp_, mo, gp, go = _parse(g, tmp_path)
+    nt, nf, ng = len(run.tgrid), P.fs.nfreq, P.fs.ngam
+    pops = [abi.Population(i, 0, 0, 0) for i in range(run.n_layers)]
+    path = str(tmp_path / "spec.h5")
+    output.write_spectra(path, run, pops, np.zeros((len(pops), nt, nf)), np.zeros((len(pops), nt, ng)),
+                         output.ele_grid(P.fs), output.freq_grid(P.fs), mp_, gp)
+    assert set(h5io.members(path, "/")) <= set(str(x) for x in g["spec_root_members"])
+    assert set(h5io.members(path, "shell=0 layer=0")) <= set(str(x) for x in g["spec_layer_members"])
+    assert {"times_freqs", "freqs", "times_gams", "gams"} <= set(h5io.members(path, "/"))
