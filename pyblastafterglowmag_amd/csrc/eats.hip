@@ -387,6 +387,11 @@ struct Ctx {
         if (!cj) return sa * (sin(theta) * sphi) + ca * cos(theta);
         return sa * (sin(kPi - theta) * sphi) + ca * cos(kPi - theta);
     }
+    // the same from sin and cos of theta (cj: sin(pi - theta) = sin theta, cos(pi - theta) = -cos theta)
+    __device__ double obs_angle_sc(double st, double ct) const {
+        if (!cj) return sa * (st * sphi) + ca * ct;
+        return sa * (st * sphi) - ca * ct;
+    }
     __device__ int mu_index(double mu) const {
         auto arr = [&](int i) { return mu_at(i); };
         return find_index(mu, arr, i_end_r);
@@ -448,8 +453,11 @@ __device__ double shock_flux(Ctx &C, const double *jall, int nu_off, double freq
 __device__ __noinline__ double integrand(Ctx &C, double cos_theta) {
     const int N = C.i_end_r;
     if (C.V(S_R, 0) == 0.0 && C.V(S_R, N - 1) == 0.0) return 0.0;
-    double a_theta = 2.0 * asin(sqrt(0.5 * (1.0 - cos_theta)));
-    double mu = C.obs_angle(a_theta);
+    // theta = 2 asin(s), s = sqrt((1 - cos_theta)/2): sin(theta) = 2 s sqrt(1 - s^2), cos(theta) = 1 - 2 s^2
+    // (the same angle the reference builds with asin and then takes sin/cos of)
+    const double sh = sqrt(0.5 * (1.0 - cos_theta));
+    const double st = 2.0 * sh * sqrt(1.0 - sh * sh), ct = 1.0 - 2.0 * sh * sh;
+    double mu = C.obs_angle_sc(st, ct);
     const double mu0 = C.mu0, muN = C.muN;
     if (mu < mu0 || mu > muN) return 0.;
     if (mu0 >= mu) return 0.;
@@ -459,10 +467,13 @@ __device__ __noinline__ double integrand(Ctx &C, double cos_theta) {
     double t_e = interp_lin(mu, mua, mub, tba, tbb);
     if (t_e < 0.0 || !isfinite(t_e)) return 0.;
     double Ra = C.V(S_R, ia), Rb = C.V(S_R, ib);
-    double r = interp_log(t_e, tba, tbb, Ra, Rb);
+    // interpSegLog: r = Ra (Rb/Ra)^w, w = ln(t_e/t_a) / ln(t_b/t_a); every state variable below is
+    // interpolated in R with the exponent ln(r/Ra)/ln(Rb/Ra), which is w itself
+    const double w = log(t_e / tba) / log(tbb / tba);
+    double r = Ra * pow(Rb / Ra, w);
     if (r <= 0.0 || !isfinite(r)) return 0.;
     // shared log-interpolation weight in R for all state variables
-    double wr = log(r / Ra) / log(Rb / Ra);
+    const double wr = w;
     auto li = [&](int v) { double ya = C.V(v, ia), yb = C.V(v, ib); return ya * pow(yb / ya, wr); };
     double Gamma = li(S_Gamma);
     double beta = li(S_beta);
