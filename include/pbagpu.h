@@ -96,13 +96,16 @@ typedef struct {
     pba_shock_params fs, rs;
 } pba_run_params;
 
-/* Initial data of one blastwave (one layer of the adaptive discretisation;
- * EjectaID2 m_data, src/ejecta/ejecta_id.h:13-341). */
+/* Initial data of one blastwave (one layer; EjectaID2 m_data,
+ * src/ejecta/ejecta_id.h:13-341, as BlastWaveBase::setAllParametersForOneLayer
+ * reads it, src/blastwave/blastwave_base.h:295-312). */
 typedef struct {
-    double E0, M0, mom0;
+    double E0, M0, mom0;                  /* piece-wise: already divided by the layer's 2i+1 cells */
     double theta_c_l, theta_c_h, theta_c; /* layer angles */
     double ctheta0;
     double theta_wing, theta_core;
+    double theta_b0;  /* initial half-opening angle: theta_c_h (adaptive) or theta_wing (piece-wise) */
+    double ncells;    /* 1 (adaptive) or the total cell count n^2 of the jet (piece-wise); >= 1 */
     int ilayer, nlayers;
 } pba_layer_init;
 
@@ -153,12 +156,14 @@ enum {
     PBAGPU_STAGE_ELE = 1,      /* ele_kernel */
     PBAGPU_STAGE_EATS = 2,     /* eats_kernel */
     PBAGPU_STAGE_SSC_TABLE = 3,/* Compton kernel table build */
-    PBAGPU_N_STAGES = 4
+    PBAGPU_STAGE_EATS_PW = 4,  /* eats_pw_kernel (piece-wise observer stage) */
+    PBAGPU_N_STAGES = 5
 };
 int pbagpu_timing_enable(int on);
 int pbagpu_timing_read(int stage, double *ms_total, long long *launches, int reset);
 /* Work units counted while timing is enabled: integrand samples of the observer
- * stage (PBAGPU_STAGE_EATS); other stages report 0 (their units follow from the
+ * stage (PBAGPU_STAGE_EATS), cell fluxes of the piece-wise observer stage
+ * (PBAGPU_STAGE_EATS_PW); other stages report 0 (their units follow from the
  * per-blastwave results and the step flags on the host). */
 int pbagpu_timing_units(int stage, double *units, int reset);
 
@@ -219,6 +224,35 @@ int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs,
                            const double *d_table, int nt, const double *d_jall, int nfreq,
                            int n_pairs, const double *d_tobs, const double *d_nuobs,
                            double *d_out, int *d_status, void *stream);
+
+/* Piece-wise EATS: the layer's light curve as the sum over its 2i+1 phi cells
+ * at the layer centre angle, primary jet and (counter_jet) counter-jet.
+ * Replaces EATS::evalLightCurve for method_eats = piece-wise ->
+ * EATS::evalSkyMapPW (src/eats.h:808-816, 859-883, 1064-1213) with
+ * fluxDensPieceWiseWithComov (src/blastwave/blastwave_radiation.h:14-72).
+ * As there, the reverse-shock term (rs = 1) takes the forward-shock
+ * emissivity (pop_fs) with the reverse-shock downstream state.
+ *   layers  [n_layers] host
+ *   d_out   [n_layers][n_pairs] device, written, mJy
+ *   d_status[n_layers] device, written: PBAGPU_S_* bits (a cell before the
+ *           first arrival time, a comoving frequency off the grid, a
+ *           non-finite or negative cell flux: where the reference exits) */
+typedef struct {
+    double ctheta0;  /* cell polar angle (EjectaID2 ictheta) */
+    double ncells;   /* total cell count n^2 (BlastWave Pars ncells) */
+    int cil;         /* phi cells of this layer, 2*ilayer+1 (EjectaID2::CellsInLayer) */
+    int bw;
+    int run;
+    int pop_fs;
+    int rs;          /* 1: add the reverse-shock term (fsrs blastwave with RS radiation) */
+    int i_end_r;
+} pba_eats_pw_layer;
+
+int pbagpu_eats_lightcurve_pw(const pba_run_params *params, int n_runs,
+                              const pba_eats_pw_layer *layers, int n_layers,
+                              const double *d_table, int nt, const double *d_jall, int nfreq,
+                              int n_pairs, const double *d_tobs, const double *d_nuobs,
+                              double *d_out, int *d_status, void *stream);
 
 #ifdef __cplusplus
 }

@@ -613,7 +613,7 @@ static void bw_init(Run &run, BW &b, const pba_layer_init &id, double t0_grid, d
     b.Gamma0 = GammaFromMom(id.mom0);
     b.beta0 = BetaFromMom(id.mom0);
     b.R0 = BetaFromMom(id.mom0) * c * t0_grid;      // ejecta_id.h:333-337
-    b.theta_a = 0.; b.theta_b0 = id.theta_c_h; b.ncells = 1.;
+    b.theta_a = 0.; b.theta_b0 = id.theta_b0; b.ncells = id.ncells;   // blastwave_base.h:304-312
     b.ctheta0 = id.ctheta0; b.theta_w = id.theta_wing; b.theta_core = id.theta_core;
     b.theta_max = P.theta_max;
     b.ilayer = id.ilayer; b.nlayers = id.nlayers;

@@ -18,6 +18,7 @@
 #include "oracle_common.h"
 #include <cstring>
 #include <cstdio>
+#include <limits>
 
 namespace orc {
 
@@ -272,7 +273,7 @@ static inline double intensity_smooth(double em, double dtau) {
 // ElectronAndRadiation::fluxDens + computeFluxDensity (useNe, sameAsBW, smooth, no SSA)
 static double flux_dens_shock(LayerCtx &L, const double *jall, const std::vector<double> &nu, double freq,
                               size_t ia, size_t ib, double Gamma, double mu, double r, double ne, double n_prime,
-                              double theta) {
+                              double theta, double ncells = 1.) {
     int nf = (int)nu.size();
     if (freq >= nu[nf - 1] || freq <= nu[0]) { L.status |= PBAGPU_S_NU_OUT_OF_GRID; return 0.; }
     size_t ia_nu = findIndex(freq, nu.data(), nf), ib_nu = ia_nu + 1;
@@ -290,7 +291,6 @@ static double flux_dens_shock(LayerCtx &L, const double *jall, const std::vector
     double abs_lab = abs_prime * delta_D;
     double beta_shock = BetaFromGamma(Gamma);
     double m2 = ne * mp;
-    double ncells = 1.;
     double dr = m2 / (2. * pi * mp * r * r * (1. - std::cos(theta) / ncells) * Gamma * n_prime);
     double dtau = (mu == beta_shock) ? 1.0e100 : abs_lab * dr * (1. - mu * beta_shock) / (mu - beta_shock);
     double intensity = intensity_smooth(em_lab, dtau);
@@ -415,6 +415,139 @@ extern "C" int orc_eats_lightcurve(const pba_run_params *params, int n_runs, con
                 }
             }
             out[(size_t)l * n_pairs + k] = 0. + fl;
+        }
+        status[l] = L.status;
+    }
+    return PBAGPU_OK;
+}
+
+namespace orc {
+
+// findClosestIndex (src/utilitites/interpolators.h:37-68): the closest of the initial guess and
+// the entries a bisection of the whole vector visits; an exact hit returns at once.
+static int find_closest_index(double target, const std::vector<double> &v, int guess) {
+    int size = (int)v.size(), low = 0, high = size - 1, closest = guess;
+    if (closest < 0 || closest >= size) return -1;
+    double best = std::abs(v[closest] - target);
+    while (low <= high) {
+        int mid = low + (high - low) / 2;
+        double d = std::abs(v[mid] - target);
+        if (d < best) { best = d; closest = mid; }
+        if (v[mid] == target) return mid;
+        else if (v[mid] < target) low = mid + 1;
+        else high = mid - 1;
+    }
+    return closest;
+}
+
+struct PwCtx {
+    const pba_run_params *P;
+    const double *T; int nt; int i_end_r;
+    const double *jfs;                 // [nt][nf] forward-shock emissivity
+    std::vector<double> nu_fs;
+    bool rs;
+    double ctheta0, ncells;
+    int cil;
+    std::vector<double> ttobs;         // [nt], entries past i_end_r stay at DBL_MAX
+    int status = 0;
+    const double *V(int v) const { return T + (size_t)v * nt; }
+};
+
+// fluxDens for the piece-wise method (microphysics.h:1185-1217): theta = the cell angle and the
+// jet's total cell count, forward-shock emissivity for both shocks (blastwave_radiation.h:43-66).
+static double flux_dens_shock_pw(PwCtx &L, double freq, size_t ia, size_t ib, double Gamma, double mu, double r,
+                                 double ne, double n_prime) {
+    LayerCtx S;
+    S.P = L.P; S.T = L.T; S.nt = L.nt; S.i_end_r = L.i_end_r;
+    double f = flux_dens_shock(S, L.jfs, L.nu_fs, freq, ia, ib, Gamma, mu, r, ne, n_prime, L.ctheta0, L.ncells);
+    L.status |= S.status;
+    return f;
+}
+
+// EATS::evalSkyMapPW, one jet (src/eats.h:1064-1213), with fluxDensPieceWiseWithComov
+// (src/blastwave/blastwave_radiation.h:14-72) as the cell flux.
+static double sky_map_pw(PwCtx &L, double t_obs, double nu_obs, bool cj) {
+    const pba_run_params &P = *L.P;
+    const double *R = L.V(PV_R), *tt = L.V(PV_tt), *G = L.V(PV_Gamma);
+    const int N = L.i_end_r;
+    if (R[0] == 0. && G[0] == 0.) return 0.;
+    if (tt[0] == 0 && N < L.nt && tt[N] == 0) L.status |= PBAGPU_S_NONFINITE;
+    const double dmax = std::numeric_limits<double>::max();
+    std::vector<double> mu(L.cil, dmax), ta(L.cil, 0.), tb(L.cil, 0.), r(L.cil, 0.);
+    std::vector<size_t> ia(L.cil, 0), ib(L.cil, 0);
+    for (int i = 0; i < L.cil; i++) {
+        double cphi = (double)i * 2.0 * M_PI / (double)L.cil;
+        mu[i] = obs_angle(L.ctheta0, cphi, P.theta_obs, cj);
+        for (int k = 0; k < N; k++) L.ttobs[k] = tt[k] + R[k] / c * (1.0 - mu[i]);
+        if (t_obs < L.ttobs[0]) { L.status |= PBAGPU_S_NONFINITE; mu[i] = dmax; continue; }
+        if (t_obs > L.ttobs[N - 1]) { mu[i] = dmax; continue; }
+        int guess = i > 0 ? (int)ia[i - 1] : (int)(L.ttobs.size() / 2);
+        ia[i] = (size_t)find_closest_index(t_obs, L.ttobs, guess);
+        if (ia[i] >= (size_t)(N - 1)) { mu[i] = dmax; continue; }
+        ib[i] = ia[i] + 1;
+        ta[i] = L.ttobs[ia[i]];
+        tb[i] = L.ttobs[ib[i]];
+        r[i] = interpSegLog(ia[i], ib[i], t_obs, L.ttobs.data(), R);
+        if (r[i] <= 0.0 || !std::isfinite(r[i])) { mu[i] = dmax; continue; }
+    }
+    double tot = 0.;
+    for (int i = 0; i < L.cil; i++) {
+        if (mu[i] == dmax) continue;
+        size_t a = ia[i], b = ib[i];
+        auto lg = [&](int v) {
+            const double *Y = L.V(v);
+            return Y[a] * std::pow(Y[b] / Y[a], std::log(t_obs / ta[i]) / std::log(tb[i] / ta[i]));
+        };
+        double Gamma = lg(PV_Gamma), beta = lg(PV_beta);
+        double nprime = lg(PV_rho2) / mp, nprotons = lg(PV_M2) / mp;
+        double nuprime = (1.0 + P.z) * nu_obs * (Gamma * (1.0 - beta * mu[i]));
+        double f = flux_dens_shock_pw(L, nuprime, a, b, Gamma, mu[i], r[i], nprotons, nprime);
+        if (L.rs) {
+            const double *th_rs = L.V(PV_thickness_rs);
+            if (th_rs[a] > 0 && th_rs[b] > 0) {
+                double nprime_rs = lg(PV_rho3) / mp, ne_rs = lg(PV_M3) / mp;
+                f += flux_dens_shock_pw(L, nuprime, a, b, Gamma, mu[i], r[i], ne_rs, nprime_rs);
+            }
+        }
+        f *= (1.0 + P.z) / (2.0 * P.d_l * P.d_l);
+        if (!std::isfinite(f) || f < 0) L.status |= PBAGPU_S_NONFINITE;
+        tot += f;
+    }
+    return tot * cgs2mJy;
+}
+
+} // namespace orc
+
+extern "C" int orc_eats_lightcurve_pw(const pba_run_params *params, int n_runs, const pba_eats_pw_layer *layers,
+                                      int n_layers, const double *table, int nt, const double *jall, int nfreq,
+                                      int n_pairs, const double *tobs, const double *nuobs, double *out, int *status) {
+    (void)n_runs;
+    if (!params || !layers || !table || !jall || !out || !status || n_layers < 1) return PBAGPU_E_ARG;
+#pragma omp parallel for schedule(dynamic)
+    for (int l = 0; l < n_layers; l++) {
+        const pba_eats_pw_layer &Ly = layers[l];
+        orc::PwCtx L;
+        L.P = &params[Ly.run];
+        L.T = table + (size_t)Ly.bw * PBA_NV * nt;
+        L.nt = nt;
+        L.i_end_r = Ly.i_end_r;
+        L.jfs = jall + (size_t)Ly.pop_fs * nt * nfreq;
+        L.nu_fs = orc::freq_grid(L.P->fs);
+        L.rs = Ly.rs != 0;
+        L.ctheta0 = Ly.ctheta0;
+        L.ncells = Ly.ncells;
+        L.cil = Ly.cil;
+        L.ttobs.assign(nt, std::numeric_limits<double>::max());
+        const double *R = L.V(PV_R), *tb = L.V(PV_tburst);
+        for (int k = 0; k < n_pairs; k++) {
+            double fl = 0.;
+            if (L.i_end_r > 0) {   // evalEjectaLightCurves skips blastwaves that never evolved
+                if (R[0] > 0 && (tobs[k] < tb[0] || tobs[k] > tb[L.i_end_r - 1])) L.status |= PBAGPU_S_NONFINITE;
+                double pj = orc::sky_map_pw(L, tobs[k], nuobs[k], false), cj = 0.;
+                if (L.P->counter_jet) cj = orc::sky_map_pw(L, tobs[k], nuobs[k], true);
+                fl = pj + cj;
+            }
+            out[(size_t)l * n_pairs + k] = fl;
         }
         status[l] = L.status;
     }

@@ -26,6 +26,9 @@ def lib():
         if hasattr(_lib, "orc_eats_lightcurve"):
             _lib.orc_eats_lightcurve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsLayer), C.c_int,
                                                  dp, C.c_int, dp, C.c_int, C.c_int, dp, dp, dp, ip]
+        if hasattr(_lib, "orc_eats_lightcurve_pw"):
+            _lib.orc_eats_lightcurve_pw.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsPwLayer),
+                                                    C.c_int, dp, C.c_int, dp, C.c_int, C.c_int, dp, dp, dp, ip]
     return _lib
 
 
@@ -86,4 +89,22 @@ def eats_lightcurve(params, layers, table, jall, tobs, nuobs, n_runs=1):
                                    _dp(np.ascontiguousarray(nuobs, dtype=np.float64)), _dp(out), _ip(st))
     if rc != 0:
         raise RuntimeError(f"orc_eats_lightcurve rc={rc}")
+    return out, st
+
+
+def eats_lightcurve_pw(params, layers, table, jall, tobs, nuobs, n_runs=1):
+    """Piece-wise EATS light curves; layers: EatsPwLayer array."""
+    if isinstance(params, abi.RunParams):
+        params = (abi.RunParams * 1)(params)
+    nbw, nv, nt = table.shape
+    nf = jall.shape[2]
+    n = len(tobs)
+    out = np.zeros((len(layers), n))
+    st = np.zeros(len(layers), dtype=np.int32)
+    rc = lib().orc_eats_lightcurve_pw(params, n_runs, layers, len(layers), _dp(np.ascontiguousarray(table)), nt,
+                                      _dp(np.ascontiguousarray(jall)), nf, n,
+                                      _dp(np.ascontiguousarray(tobs, dtype=np.float64)),
+                                      _dp(np.ascontiguousarray(nuobs, dtype=np.float64)), _dp(out), _ip(st))
+    if rc != 0:
+        raise RuntimeError(f"orc_eats_lightcurve_pw rc={rc}")
     return out, st

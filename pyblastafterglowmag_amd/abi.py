@@ -48,12 +48,18 @@ class RunParams(C.Structure):
 class LayerInit(C.Structure):
     _fields_ = [("E0", C.c_double), ("M0", C.c_double), ("mom0", C.c_double), ("theta_c_l", C.c_double),
                 ("theta_c_h", C.c_double), ("theta_c", C.c_double), ("ctheta0", C.c_double),
-                ("theta_wing", C.c_double), ("theta_core", C.c_double), ("ilayer", C.c_int), ("nlayers", C.c_int)]
+                ("theta_wing", C.c_double), ("theta_core", C.c_double), ("theta_b0", C.c_double),
+                ("ncells", C.c_double), ("ilayer", C.c_int), ("nlayers", C.c_int)]
 
 
 class BwResult(C.Structure):
     _fields_ = [("status", C.c_int), ("i_end_r", C.c_int), ("fsrs", C.c_int), ("n_steps", C.c_int),
                 ("n_rhs", C.c_int), ("pad_", C.c_int)]
+
+
+class EatsPwLayer(C.Structure):
+    _fields_ = [("ctheta0", C.c_double), ("ncells", C.c_double), ("cil", C.c_int), ("bw", C.c_int), ("run", C.c_int),
+                ("pop_fs", C.c_int), ("rs", C.c_int), ("i_end_r", C.c_int)]
 
 
 class Population(C.Structure):
@@ -65,8 +71,9 @@ class EatsLayer(C.Structure):
                 ("pop_fs", C.c_int), ("pop_rs", C.c_int), ("i_end_r", C.c_int)]
 
 
+EATS_METHODS = ("adaptive", "piece-wise")
 SUPPORTED_OPTS = dict(
-    method_eats="adaptive", method_quad="CADRE", method_comp_mode="comovSpec", method_spread="our",
+    method_eats=EATS_METHODS, method_quad="CADRE", method_comp_mode="comovSpec", method_spread="our",
     method_limit_spread="MomValAndFrac", method_dgdr="our", method_eos="Nava13", method_dmdr="usingdthdr",
     integrator="DOP853E", do_average_solution="no", lc_use_freq_to_time="no", ebl_tbl_fpath="none",
 )
@@ -115,9 +122,10 @@ def run_params_from_parfile(path):
 
 def run_params(mp_, mo, gp, go):
     for key, want in SUPPORTED_OPTS.items():
-        have = mo.get(key, go.get(key, want))
-        if have != want:
-            raise NotImplementedError(f"{key}={have} (only {want} is implemented)")
+        allowed = want if isinstance(want, tuple) else (want,)
+        have = mo.get(key, go.get(key, allowed[0]))
+        if have not in allowed:
+            raise NotImplementedError(f"{key}={have} (only {' / '.join(allowed)} is implemented)")
     if mp_.get("n_ism", -1) <= 0:
         raise NotImplementedError("only a uniform ISM (n_ism > 0) is implemented")
     if int(mp_.get("iout", 1)) != 1:
@@ -156,14 +164,21 @@ def run_params(mp_, mo, gp, go):
     return P
 
 
-def layer_inits(idd):
+def layer_inits(idd, method="adaptive"):
     """LayerInit array of the layers of a jet_structure() dict (shell 0).
 
     The reference's loader builds the adaptive grid and then the piece-wise grid, and the
     latter overwrites the layer edges and centres (EjectaID2::_init_a_grid / _init_pw_grid,
     src/ejecta/ejecta_id.h:142-182, called in that order at :298-300): the edges used by the
     dynamics and the observer integral are 2 asin(i/n sin(theta_w/2)).
+
+    method="piece-wise" (ejecta_id.h:315-322, blastwave_base.h:304-312): E0 and M0 of layer i
+    are divided by its 2i+1 cells, every blastwave starts at theta_b0 = theta_wing, and the
+    swept-up mass, shock thickness and radiative loss are shared by ncells = n^2 cells.
     """
+    if method not in EATS_METHODS:
+        raise NotImplementedError(f"method_eats={method}")
+    pw = method == "piece-wise"
     n = len(idd["ek"])
     arr = (LayerInit * n)()
     tw = float(idd["theta_wing"])
@@ -171,8 +186,9 @@ def layer_inits(idd):
     th_pw = [2.0 * math.asin((i / n) * math.sin(tw / 2.0)) for i in range(n + 1)]  # libm, as the C++ loader
     for i in range(n):
         L = arr[i]
-        L.E0 = float(idd["ek"][i])
-        L.M0 = float(idd["mass"][i])
+        cil = float(cells_in_layer(i)) if pw else 1.0
+        L.E0 = float(idd["ek"][i]) / cil
+        L.M0 = float(idd["mass"][i]) / cil
         L.mom0 = float(idd["mom"][i])
         L.theta_c_l = th_pw[i]
         L.theta_c_h = th_pw[i + 1]
@@ -180,9 +196,21 @@ def layer_inits(idd):
         L.ctheta0 = 0.5 * (th_pw[i + 1] + th_pw[i])
         L.theta_wing = tw
         L.theta_core = float(idd["theta_core"])
+        L.theta_b0 = tw if pw else th_pw[i + 1]
+        L.ncells = float(total_cells(n)) if pw else 1.0
         L.ilayer = i
         L.nlayers = n
     return arr
+
+
+def cells_in_layer(i):
+    """phi cells of layer i (EjectaID2::CellsInLayer, src/ejecta/ejecta_id.h:119-121)."""
+    return 2 * i + 1
+
+
+def total_cells(n_layers):
+    """Cells of an n-layer jet (EjectaID2::_evalTotalNcells, src/ejecta/ejecta_id.h:149-155)."""
+    return sum(cells_in_layer(i) for i in range(n_layers))
 
 
 def tgrid(mp_):

@@ -37,6 +37,7 @@ struct Shock { double e0, etop; };
 
 struct BW {
     double E0, M0, R0, Gamma0, beta0, mom0, theta_b0, theta_core, theta_max, rho_def;
+    double ncells;     // cells sharing this blastwave: 1 (adaptive EATS) or the total cell count (piece-wise)
     double Rw0, Wtt0;  // passive components of the reference state vector (enter the step-size estimate)
     int ilayer, nlayers;
     int fsrs, do_rs, do_rs_rad, shutOff, end_evolution, end_spreading;
@@ -143,7 +144,7 @@ __device__ void rhs_fs(const pba_run_params &P, const Shock &shfs, BW &b, const 
     double dthetadr = 0.0;
     if ((theta < b.theta_max) && !b.end_spreading && Gamma * beta < fmax(P.mom_when_start_spread, 0. * b.mom0))
         dthetadr = dthetadr_our(b, Gamma, Gsh, R, adi, theta);
-    double dM2dR = 2. * kPi * rho * ((1. - cos(theta)) + (1. / 3.) * sin(theta) * R * dthetadr) * R * R;
+    double dM2dR = 2. * kPi * rho * ((1. - cos(theta)) + (1. / 3.) * sin(theta) * R * dthetadr) * R * R / b.ncells;
     double GE = (adi * Gamma * Gamma - adi + 1.0) / Gamma;
     double dGE = (adi * Gamma * Gamma + adi - 1.0) / (Gamma * Gamma);
     double num = (Gamma - 1.) * (GE + 1.) * dM2dR - GE * (adi - 1.) * Eint2 * (dM2dR / M2 - drhodr / rho);
@@ -155,7 +156,7 @@ __device__ void rhs_fs(const pba_run_params &P, const Shock &shfs, BW &b, const 
     double eps_rad = P.fs.epsilon_e_rad;
     if (P.fs.epsilon_e_rad < 0)
         eps_rad = radiation_loss(P.fs, shfs, b.fs_st, dM2dR * b.M0, dEsh2dR * b.M0 * kC * kC, R, Gamma, Gsh, adi,
-                                 M2 * b.M0, Eint2 * b.M0 * kC * kC, rho * b.M0, theta, 1.);
+                                 M2 * b.M0, Eint2 * b.M0 * kC * kC, rho * b.M0, theta, b.ncells);
     double dErad2dR = eps_rad * P.fs.eps_e * dEsh2dR;
     double dEint2dR = dEsh2dR + dEad2dR - dErad2dR;
     double mom = mom_from_gamma(Gamma);
@@ -198,7 +199,7 @@ __device__ void rhs_fsrs(const pba_run_params &P, const Shock &shfs, BW &b, cons
     double dthetadr = 0.0;
     if ((theta < b.theta_max) && !b.end_spreading && Gamma * beta < fmax(P.mom_when_start_spread, 0. * b.mom0))
         dthetadr = dthetadr_our(b, Gamma, Gsh, R, adi, theta);
-    double dM2dR = 2. * kPi * rho * ((1. - cos(theta)) + (1. / 3.) * sin(theta) * R * dthetadr) * R * R;
+    double dM2dR = 2. * kPi * rho * ((1. - cos(theta)) + (1. / 3.) * sin(theta) * R * dthetadr) * R * R / b.ncells;
     double ddR4 = 0., dM3dR = 0., dlnrho4dR = 0.;
     if (!b.shutOff && Gamma < 0.9999 * G0) {
         double alpha_of = P.tprompt * b.beta0 * kC;
@@ -243,7 +244,7 @@ __device__ void rhs_fsrs(const pba_run_params &P, const Shock &shfs, BW &b, cons
     double eps_rad = P.fs.epsilon_e_rad;
     if (P.fs.epsilon_e_rad < 0)
         eps_rad = radiation_loss(P.fs, shfs, b.fs_st, dM2dR * b.M0, dEsh2dR * b.M0 * kC * kC, R, Gamma, Gsh, adi,
-                                 M2 * b.M0, Eint2 * b.M0 * kC * kC, rho * b.M0, theta, 1.);
+                                 M2 * b.M0, Eint2 * b.M0 * kC * kC, rho * b.M0, theta, b.ncells);
     double dErad2dR = eps_rad * P.fs.eps_e * dEsh2dR;
     // the reverse-shock loss is evaluated with rho4 = 0 and returns 0 (blastwave.h:1991-1997)
     double eps_rad_rs = P.rs.epsilon_e_rad < 0 ? 0. : P.rs.epsilon_e_rad;
@@ -370,7 +371,7 @@ __device__ void post_step(const pba_run_params &P, const Shock &shfs, const Shoc
     double rho2 = rho2t(G, adi, rho);
     double Gsh = gamma_shock(G, adi);
     W(PV_adi, adi); W(PV_rho2, rho2); W(PV_GammaFsh, Gsh);
-    double omc = (1. - cos(theta) / 1.);
+    double omc = (1. - cos(theta) / b.ncells);
     W(PV_thickness, M2 / (2. * kPi * kMp * R * R * omc * G * (rho2 / kMp)));
     double U_p = (Eint2 == 0 || M2 == 0) ? 0. : Eint2 / (M2 / rho2);
     W(PV_U_p, U_p);
@@ -505,7 +506,7 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
         b.E0 = id.E0; b.M0 = id.M0; b.mom0 = id.mom0;
         b.Gamma0 = gamma_from_mom(id.mom0);
         b.R0 = beta_from_mom(id.mom0) * kC * tgrid[0];
-        b.theta_b0 = id.theta_c_h; b.theta_core = id.theta_core; b.theta_max = P.theta_max;
+        b.theta_b0 = id.theta_b0; b.ncells = id.ncells; b.theta_core = id.theta_core; b.theta_max = P.theta_max;
         b.ilayer = id.ilayer; b.nlayers = id.nlayers;
         b.fsrs = P.fsrs; b.do_rs = P.fsrs && P.do_rs; b.do_rs_rad = P.fsrs && P.do_rs_radiation;
         b.rho_def = P.n_ism * kMppme;
@@ -518,7 +519,7 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
         if (b.M0 == 0. && b.E0 == 0.) b.end_evolution = 1;
         else {
             b.beta0 = beta_from_gamma(b.Gamma0);
-            double M20 = (2.0 / 3.0) * kPi * (1.0 - cos(b.theta_b0)) * b.rho_def * pow(b.R0, 3) / 1.;
+            double M20 = (2.0 / 3.0) * kPi * (1.0 - cos(b.theta_b0)) * b.rho_def * pow(b.R0, 3) / b.ncells;
             if (b.M0 <= 0.) b.M0 = b.E0 / (b.Gamma0 * kC * kC);
             Y[iR] = b.R0;
             double b0m = beta_from_mom(b.mom0);
@@ -682,6 +683,13 @@ extern "C" int pbagpu_dyn_evolve(const pba_run_params *params, int n_runs, int b
         return PBAGPU_E_ARG;
     hipStream_t s = (hipStream_t)stream;
     const size_t nbw = (size_t)n_runs * bw_per_run;
+    for (size_t i = 0; i < nbw; i++) {
+        // ncells < 1 is an error of the reference too (blastwave.h:175-181)
+        if (!(init[i].ncells >= 1.0) || !std::isfinite(init[i].ncells) || !(init[i].theta_b0 > 0.0)) {
+            pbagpu::set_error_msg("pba_layer_init: ncells must be >= 1 and theta_b0 > 0");
+            return PBAGPU_E_ARG;
+        }
+    }
     auto *dP = (pba_run_params *)pbagpu::workspace(pbagpu::WS_DYN_PARAMS, sizeof(pba_run_params) * n_runs);
     auto *dI = (pba_layer_init *)pbagpu::workspace(pbagpu::WS_DYN_INIT, sizeof(pba_layer_init) * nbw);
     auto *dR = (pba_bw_result *)pbagpu::workspace(pbagpu::WS_DYN_RES, sizeof(pba_bw_result) * nbw);

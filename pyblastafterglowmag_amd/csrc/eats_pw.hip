@@ -1,0 +1,264 @@
+// Stage 3, piece-wise method: the light curve of a layer as a sum over its phi cells.
+//
+// One lane per (layer, t_obs, nu_obs) pair; one workgroup per (layer, block of PW_WG
+// consecutive pairs).  The workgroup stages the layer's table columns and the comoving
+// frequency grid in LDS once; every lane then walks the layer's 2i+1 cells in the order of
+// EATS::evalSkyMapPW (src/eats.h:1064-1213), primary jet then counter-jet
+// (src/eats.h:859-883):
+//   * the arrival time of every step, ttobs[k] = tt[k] + R[k]/c (1 - mu), is not stored:
+//     it is evaluated only at the steps the bisection of findClosestIndex probes
+//     (src/utilitites/interpolators.h:37-68), so one cell costs O(log nt), not O(nt);
+//   * the previous cell's index is the initial guess of the next search, as there;
+//   * the cell flux follows fluxDensPieceWiseWithComov (src/blastwave/blastwave_radiation.h:14-72)
+//     and ElectronAndRadiation::fluxDens / computeFluxDensity
+//     (src/microphysics/microphysics.h:799-867, 1185-1217), with the cell angle and the
+//     jet's total cell count in the shell-thickness and luminosity terms and the
+//     forward-shock emissivity for both shocks.
+#include "pba_device.h"
+#include "timing.h"
+#include "runtime.h"
+#include <cfloat>
+#include <vector>
+
+namespace pbagpu {
+namespace eatspw {
+
+constexpr int PW_WG = 256;
+
+// Table columns the cell flux reads, staged in LDS per workgroup.
+enum Col { C_tt, C_R, C_Gamma, C_beta, C_M2, C_rho2, C_thrs, C_M3, C_rho3, NCOL };
+__device__ __forceinline__ constexpr int col_var(int c) {
+    return c == C_tt ? PV_tt : c == C_R ? PV_R : c == C_Gamma ? PV_Gamma : c == C_beta ? PV_beta
+         : c == C_M2 ? PV_M2 : c == C_rho2 ? PV_rho2 : c == C_thrs ? PV_thickness_rs : c == C_M3 ? PV_M3 : PV_rho3;
+}
+
+struct PwArgs {
+    const double *T;    // table [PBA_NV][nt]
+    const double *jfs;  // forward-shock emissivity [nt][nf]
+    const double *nu;   // forward-shock comoving frequency grid [nf]
+    double ctheta0, ncells;
+    double z, theta_obs, d_l;
+    int cil, rs, i_end_r, counter_jet;
+};
+
+extern __shared__ double lds_pw[];
+
+struct Lane {
+    int nt, nf, N;    // N = i_end_r
+    int status;
+    __device__ double V(int c, int k) const { return lds_pw[c * nt + k]; }
+    __device__ double nu(int i) const { return lds_pw[NCOL * nt + i]; }
+    // arrival time of step k for a cell with 1 - mu = omu; the steps past i_end_r hold
+    // DBL_MAX in the reference's vector.  Product and sum are rounded separately, as there.
+    __device__ double ttobs(int k, double omu) const {
+        if (k >= N) return DBL_MAX;
+        return __dadd_rn(V(C_tt, k), __dmul_rn(V(C_R, k) / kC, omu));
+    }
+    // findClosestIndex over the nt-long arrival-time vector
+    __device__ int closest(double t, double omu, int guess) const {
+        int low = 0, high = nt - 1, best_i = guess;
+        double best = fabs(ttobs(guess, omu) - t);
+        while (low <= high) {
+            int mid = low + (high - low) / 2;
+            double v = ttobs(mid, omu);
+            double d = fabs(v - t);
+            if (d < best) { best = d; best_i = mid; }
+            if (v == t) return mid;
+            if (v < t) low = mid + 1; else high = mid - 1;
+        }
+        return best_i;
+    }
+    // fluxDens + computeFluxDensity (useNe, sameAsBW, smooth optical depth, no SSA)
+    __device__ double shock_flux(const double *jall, double freq, int ia, int ib, double Gamma, double mu, double r,
+                                 double ne, double n_prime, double theta, double ncells) {
+        if (freq >= nu(nf - 1) || freq <= nu(0)) { status |= PBAGPU_S_NU_OUT_OF_GRID; return 0.; }
+        auto grid = [&](int i) { return nu(i); };
+        int ia_nu = find_index(freq, grid, nf), ib_nu = ia_nu + 1;
+        double x1 = nu(ia_nu), x2 = nu(ib_nu), y1 = V(C_R, ia), y2 = V(C_R, ib);
+        double q11 = jall[(size_t)ia * nf + ia_nu], q12 = jall[(size_t)ib * nf + ia_nu];
+        double q21 = jall[(size_t)ia * nf + ib_nu], q22 = jall[(size_t)ib * nf + ib_nu];
+        double x2x1 = x2 - x1, y2y1 = y2 - y1, x2x = x2 - freq, y2y = y2 - r, yy1 = r - y1, xx1 = freq - x1;
+        double em_prime = 1.0 / (x2x1 * y2y1) * (q11 * x2x * y2y + q21 * xx1 * y2y + q12 * x2x * yy1 + q22 * xx1 * yy1);
+        double beta = beta_from_gamma(Gamma);
+        double delta_D = Gamma * (1.0 - beta * mu);
+        double em_lab = em_prime / (delta_D * delta_D);
+        // absorption is identically 0: the optical depth vanishes unless mu equals the shock velocity
+        double intensity = (mu == beta) ? em_lab * (-expm1(-1.0e100) / 1.0e100) : em_lab;
+        double omc = (1. - cos(theta) / ncells);
+        double lum = intensity * ne / (2. * kPi * omc) / Gamma / (1.0 - mu * beta);
+        if (!(lum >= 0) || !isfinite(lum)) status |= PBAGPU_S_NONFINITE;
+        return lum;
+    }
+};
+
+// One jet of EATS::evalSkyMapPW: the sum of the cell fluxes, mJy.
+__device__ double sky_map_pw(Lane &L, const PwArgs &A, double t_obs, double nu_obs, double sa, double ca, double st,
+                             double ct, unsigned long long &cells) {
+    const int N = L.N;
+    if (L.V(C_R, 0) == 0. && L.V(C_Gamma, 0) == 0.) return 0.;
+    if (L.V(C_tt, 0) == 0. && N < L.nt && L.V(C_tt, N) == 0.) L.status |= PBAGPU_S_NONFINITE;
+    double tot = 0.;
+    int ia_prev = 0;
+    for (int i = 0; i < A.cil; i++) {
+        const double cphi = (double)i * 2.0 * M_PI / (double)A.cil;
+        // obsAngle / obsAngleCJ (src/image.h:361-378); st, ct are sin and cos of the cell angle,
+        // or of pi minus it for the counter-jet
+        const double mu = sa * (st * sin(cphi)) + ca * ct;
+        const double omu = 1.0 - mu;
+        int guess = i > 0 ? ia_prev : L.nt / 2;
+        ia_prev = 0;   // a cell that ends before t_obs leaves its index at 0
+        if (t_obs < L.ttobs(0, omu)) { L.status |= PBAGPU_S_NONFINITE; continue; }
+        if (t_obs > L.ttobs(N - 1, omu)) continue;
+        const int ia = L.closest(t_obs, omu, guess);
+        ia_prev = ia;
+        if (ia >= N - 1) continue;
+        const int ib = ia + 1;
+        const double ta = L.ttobs(ia, omu), tb = L.ttobs(ib, omu);
+        // one exponent for every log-log interpolation of this cell (interpSegLog, interpolators.h:114-133)
+        const double w = log(t_obs / ta) / log(tb / ta);
+        auto lg = [&](int c) {
+            double ya = L.V(c, ia);
+            return ya * pow(L.V(c, ib) / ya, w);
+        };
+        const double r = lg(C_R);
+        if (r <= 0.0 || !isfinite(r)) continue;
+        const double Gamma = lg(C_Gamma), beta = lg(C_beta);
+        const double nprime = lg(C_rho2) / kMp, nprotons = lg(C_M2) / kMp;
+        const double nuprime = (1.0 + A.z) * nu_obs * (Gamma * (1.0 - beta * mu));
+        double f = L.shock_flux(A.jfs, nuprime, ia, ib, Gamma, mu, r, nprotons, nprime, A.ctheta0, A.ncells);
+        if (A.rs && L.V(C_thrs, ia) > 0 && L.V(C_thrs, ib) > 0) {
+            const double nprime_rs = lg(C_rho3) / kMp, ne_rs = lg(C_M3) / kMp;
+            f += L.shock_flux(A.jfs, nuprime, ia, ib, Gamma, mu, r, ne_rs, nprime_rs, A.ctheta0, A.ncells);
+        }
+        f *= (1.0 + A.z) / (2.0 * A.d_l * A.d_l);
+        if (!isfinite(f) || f < 0) L.status |= PBAGPU_S_NONFINITE;
+        tot += f;
+        cells++;
+    }
+    return tot * kCgs2mJy;
+}
+
+// grid: (ceil(n_pairs / PW_WG), n_layers)
+__global__ void __launch_bounds__(PW_WG) eats_pw_kernel(const PwArgs *__restrict__ layers, int n_pairs, int nt, int nf,
+                                                        const double *__restrict__ tobs,
+                                                        const double *__restrict__ nuobs, double *__restrict__ out,
+                                                        int *__restrict__ status,
+                                                        unsigned long long *__restrict__ n_cells) {
+    const int l = blockIdx.y;
+    const PwArgs A = layers[l];
+    for (int idx = threadIdx.x; idx < NCOL * nt; idx += PW_WG) {
+        const int c = idx / nt, k = idx - c * nt;
+        lds_pw[idx] = A.T[(size_t)col_var(c) * nt + k];
+    }
+    for (int i = threadIdx.x; i < nf; i += PW_WG) lds_pw[NCOL * nt + i] = A.nu[i];
+    __syncthreads();
+
+    const int k = blockIdx.x * PW_WG + threadIdx.x;
+    if (k >= n_pairs) return;
+    Lane L;
+    L.nt = nt; L.nf = nf; L.N = A.i_end_r; L.status = 0;
+    double fl = 0.;
+    unsigned long long cells = 0;
+    if (A.i_end_r > 0) {   // evalEjectaLightCurves skips blastwaves that never evolved
+        const double t = tobs[k], nu = nuobs[k];
+        // evalSkyMapPW(t_obs, nu_obs, offset): t_obs inside the evolved burster times
+        if (L.V(C_R, 0) > 0) {
+            const double tb0 = A.T[(size_t)PV_tburst * nt], tb1 = A.T[(size_t)PV_tburst * nt + A.i_end_r - 1];
+            if (t < tb0 || t > tb1) L.status |= PBAGPU_S_NONFINITE;
+        }
+        const double th = A.ctheta0, th_cj = kPi - A.ctheta0;
+        const double sa = sin(A.theta_obs), ca = cos(A.theta_obs);
+        const double pj = sky_map_pw(L, A, t, nu, sa, ca, sin(th), cos(th), cells);
+        double cjf = 0.;
+        if (A.counter_jet) cjf = sky_map_pw(L, A, t, nu, sa, ca, sin(th_cj), cos(th_cj), cells);
+        fl = pj + cjf;
+    }
+    out[(size_t)l * n_pairs + k] = fl;
+    if (L.status) atomicOr(&status[l], L.status);
+    if (n_cells) {
+        // one atomic per wavefront
+        unsigned long long s = cells;
+        for (int d = 32; d > 0; d >>= 1) s += __shfl_down(s, d, 64);
+        if ((threadIdx.x & 63) == 0) atomicAdd(n_cells, s);
+    }
+}
+
+}  // namespace eatspw
+}  // namespace pbagpu
+
+extern "C" int pbagpu_eats_lightcurve_pw(const pba_run_params *params, int n_runs, const pba_eats_pw_layer *layers,
+                                         int n_layers, const double *d_table, int nt, const double *d_jall, int nfreq,
+                                         int n_pairs, const double *d_tobs, const double *d_nuobs, double *d_out,
+                                         int *d_status, void *stream) {
+    using namespace pbagpu::eatspw;
+    if (!params || !layers || !d_table || !d_jall || !d_tobs || !d_nuobs || !d_out || !d_status || n_runs < 1 ||
+        n_layers < 1 || n_pairs < 1 || nt < 2 || nfreq < 2 || n_layers > 65535)
+        return PBAGPU_E_ARG;
+    const size_t lds = sizeof(double) * ((size_t)NCOL * nt + (size_t)nfreq);
+    if (lds > 160 * 1024) {
+        pbagpu::set_error_msg("eats_pw: time grid too long for the LDS-staged table (nt <= 2200)");
+        return PBAGPU_E_ARG;
+    }
+    for (int l = 0; l < n_layers; l++) {
+        const pba_eats_pw_layer &Y = layers[l];
+        if (Y.run < 0 || Y.run >= n_runs || Y.i_end_r > nt || Y.i_end_r < 0 || Y.pop_fs < 0 || Y.bw < 0 ||
+            Y.cil < 1 || !(Y.ncells >= 1.0)) {
+            pbagpu::set_error_msg("eats_pw: bad layer descriptor");
+            return PBAGPU_E_ARG;
+        }
+    }
+    for (int r = 0; r < n_runs; r++)
+        if (params[r].fs.nfreq != nfreq) return PBAGPU_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    // forward-shock frequency grid of every run (MakeLogspaceVec, utils.h:91-105)
+    std::vector<double> hnu((size_t)n_runs * nfreq);
+    for (int r = 0; r < n_runs; r++) {
+        const pba_shock_params &sp = params[r].fs;
+        double cur = std::pow(10.0, std::log10(sp.freq1));
+        const double ratio = std::pow(10.0, (std::log10(sp.freq2) - std::log10(sp.freq1)) / sp.nfreq);
+        for (int i = 0; i < nfreq; i++) { hnu[(size_t)r * nfreq + i] = cur; cur *= ratio; }
+    }
+    auto *dnu = (double *)pbagpu::workspace(pbagpu::WS_PW_NU, sizeof(double) * hnu.size());
+    auto *dL = (PwArgs *)pbagpu::workspace(pbagpu::WS_PW_LAYERS, sizeof(PwArgs) * n_layers);
+    if (!dnu || !dL) return PBAGPU_E_HIP;
+    std::vector<PwArgs> hl(n_layers);
+    for (int l = 0; l < n_layers; l++) {
+        const pba_eats_pw_layer &Y = layers[l];
+        const pba_run_params &P = params[Y.run];
+        PwArgs &A = hl[l];
+        A.T = d_table + (size_t)Y.bw * PBA_NV * nt;
+        A.jfs = d_jall + (size_t)Y.pop_fs * nt * nfreq;
+        A.nu = dnu + (size_t)Y.run * nfreq;
+        A.ctheta0 = Y.ctheta0; A.ncells = Y.ncells;
+        A.z = P.z; A.theta_obs = P.theta_obs; A.d_l = P.d_l;
+        A.cil = Y.cil; A.rs = Y.rs ? 1 : 0; A.i_end_r = Y.i_end_r; A.counter_jet = P.counter_jet;
+    }
+    int rc = PBAGPU_OK;
+    PBA_TRY(rc, hipMemcpyAsync(dnu, hnu.data(), sizeof(double) * hnu.size(), hipMemcpyHostToDevice, s));
+    PBA_TRY(rc, hipMemcpyAsync(dL, hl.data(), sizeof(PwArgs) * n_layers, hipMemcpyHostToDevice, s));
+    PBA_TRY(rc, hipMemsetAsync(d_status, 0, sizeof(int) * n_layers, s));
+    pbagpu::KernelTimer timer(PBAGPU_STAGE_EATS_PW);
+    unsigned long long *d_cells = nullptr;
+    if (rc == PBAGPU_OK && pbagpu::timing_enabled()) {
+        d_cells = (unsigned long long *)pbagpu::workspace(pbagpu::WS_PW_SAMPLES, sizeof(unsigned long long));
+        if (d_cells) PBA_TRY(rc, hipMemsetAsync(d_cells, 0, sizeof(unsigned long long), s));
+    }
+    if (rc == PBAGPU_OK) {
+        (void)hipFuncSetAttribute((const void *)eats_pw_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        dim3 grid((unsigned)((n_pairs + PW_WG - 1) / PW_WG), (unsigned)n_layers);
+        timer.start(s);
+        hipLaunchKernelGGL(eats_pw_kernel, grid, dim3(PW_WG), lds, s, dL, n_pairs, nt, nfreq, d_tobs, d_nuobs, d_out,
+                           d_status, d_cells);
+        PBA_TRY(rc, hipGetLastError());
+        timer.stop(s);
+        if (d_cells && rc == PBAGPU_OK) {
+            unsigned long long hc = 0;
+            PBA_TRY(rc, hipMemcpyAsync(&hc, d_cells, sizeof(hc), hipMemcpyDeviceToHost, s));
+            PBA_TRY(rc, hipStreamSynchronize(s));
+            pbagpu::timing_add_units(PBAGPU_STAGE_EATS_PW, (double)hc);
+        }
+    }
+    PBA_TRY(rc, hipStreamSynchronize(s));
+    timer.collect();
+    return rc;
+}

@@ -15,8 +15,9 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "li
 _lib = None
 
 EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu_last_error", "pbagpu_timing_enable",
-           "pbagpu_timing_read", "pbagpu_timing_units", "pbagpu_dyn_evolve", "pbagpu_ele_evolve", "pbagpu_eats_lightcurve")
-STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE = 0, 1, 2, 3
+           "pbagpu_timing_read", "pbagpu_timing_units", "pbagpu_dyn_evolve", "pbagpu_ele_evolve", "pbagpu_eats_lightcurve",
+           "pbagpu_eats_lightcurve_pw")
+STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW = 0, 1, 2, 3, 4
 
 
 def lib():
@@ -43,6 +44,8 @@ def lib():
         L.pbagpu_timing_units.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int]
         L.pbagpu_eats_lightcurve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsLayer), C.c_int, vp,
                                              C.c_int, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+        L.pbagpu_eats_lightcurve_pw.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsPwLayer), C.c_int,
+                                                vp, C.c_int, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -94,7 +97,8 @@ class Run:
     parfile (src/main.cpp:93-154, src/ejecta/ejecta_base.h:141-345, src/ejecta/ejecta.h:35-80).
     """
 
-    def __init__(self, params, inits, tgrid, tobs, nuobs):
+    def __init__(self, params, inits, tgrid, tobs, nuobs, method="adaptive"):
+        self.method = method
         self.params = params
         self.inits = inits
         self.tgrid = np.ascontiguousarray(tgrid, dtype=np.float64)
@@ -111,7 +115,8 @@ class Run:
                    mass=h5io.read(idf, "mass"), theta_wing=h5io.attr(idf, "theta_wing"),
                    theta_core=h5io.attr(idf, "theta_core"))
         tobs, nuobs = abi.obs_pairs(mo)
-        return cls(P, abi.layer_inits(idd), abi.tgrid(mp_), tobs, nuobs)
+        method = go.get("method_eats", "adaptive")
+        return cls(P, abi.layer_inits(idd, method), abi.tgrid(mp_), tobs, nuobs, method)
 
     @classmethod
     def from_workload(cls, main, grb, struct):
@@ -121,9 +126,10 @@ class Run:
         go = dict(inputs.GRB_DEFAULTS["opts"], **{k: v for k, v in grb.items() if k in inputs.GRB_DEFAULTS["opts"]})
         go["use_1d_id"] = "yes"
         P = abi.run_params(mp_, mo, gp, go)
-        idd = inputs.jet_structure(kind="adaptive", **struct)
+        method = go.get("method_eats", "adaptive")
+        idd = inputs.jet_structure(kind=method, **struct)
         tobs, nuobs = abi.obs_pairs(mo)
-        return cls(P, abi.layer_inits(idd), abi.tgrid(mp_), tobs, nuobs)
+        return cls(P, abi.layer_inits(idd, method), abi.tgrid(mp_), tobs, nuobs, method)
 
     @property
     def n_layers(self):
@@ -170,9 +176,12 @@ class Pipeline:
         for r in runs:
             if r.n_layers != self.bpr or len(r.tgrid) != self.nt or len(r.tobs) != len(r0.tobs):
                 raise ValueError("runs of one batch need equal layer count, time grid and pair count")
+            if r.method != r0.method:
+                raise ValueError("runs of one batch share the EATS method")
             if not np.array_equal(r.tgrid, r0.tgrid) or not np.array_equal(r.tobs, r0.tobs) \
                     or not np.array_equal(r.nuobs, r0.nuobs):
                 raise ValueError("runs of one batch share the time grid and the observer pairs")
+        self.method = r0.method
         self.nf = r0.params.fs.nfreq
         self.npairs = len(r0.tobs)
         self.precision = precision
@@ -200,10 +209,18 @@ class Pipeline:
         pops, lays = populations(self.params, self.res, self.n_runs, self.bpr)
         self.pops = pops
         self.parr = (abi.Population * len(pops))(*pops)
-        self.larr = (abi.EatsLayer * len(lays))()
-        for i, (bw, r, pfs, prs, iend) in enumerate(lays):
-            li = self.runs[r].inits[bw - r * self.bpr]
-            self.larr[i] = abi.EatsLayer(li.theta_c_l, li.theta_c_h, bw, r, pfs, prs, iend)
+        if self.method == "piece-wise":
+            self.larr = (abi.EatsPwLayer * len(lays))()
+            for i, (bw, r, pfs, prs, iend) in enumerate(lays):
+                li = self.runs[r].inits[bw - r * self.bpr]
+                rs = 1 if (self.res[bw].fsrs and self.params[r].do_rs_radiation) else 0
+                self.larr[i] = abi.EatsPwLayer(li.ctheta0, li.ncells, abi.cells_in_layer(li.ilayer), bw, r, pfs, rs,
+                                               iend)
+        else:
+            self.larr = (abi.EatsLayer * len(lays))()
+            for i, (bw, r, pfs, prs, iend) in enumerate(lays):
+                li = self.runs[r].inits[bw - r * self.bpr]
+                self.larr[i] = abi.EatsLayer(li.theta_c_l, li.theta_c_h, bw, r, pfs, prs, iend)
         self.layers = lays
 
     def electrons(self, want_f=False):
@@ -215,6 +232,12 @@ class Pipeline:
                                         _ptr(self.d_est), self.precision, self.sp), "ele_evolve")
 
     def observer(self):
+        if self.method == "piece-wise":
+            _check(self.L.pbagpu_eats_lightcurve_pw(self.params, self.n_runs, self.larr, len(self.layers),
+                                                    _ptr(self.d_table), self.nt, _ptr(self.d_jall), self.nf,
+                                                    self.npairs, _ptr(self.d_tobs), _ptr(self.d_nuobs),
+                                                    _ptr(self.d_out), _ptr(self.d_lst), self.sp), "eats_lightcurve_pw")
+            return
         _check(self.L.pbagpu_eats_lightcurve(self.params, self.n_runs, self.larr, len(self.layers), _ptr(self.d_table),
                                              self.nt, _ptr(self.d_jall), self.nf, self.npairs, _ptr(self.d_tobs),
                                              _ptr(self.d_nuobs), _ptr(self.d_out), _ptr(self.d_lst), self.sp),

@@ -8,7 +8,9 @@
   synchrotron/SSC parts, intensities or cooling rates, so those datasets are not written.
 
 Group names are ``"shell=0 layer=<i>"``; file attributes are nshells, nlayers (and ntimes,
-ncells for dynamics/spectra) plus every numeric parameter of the main and GRB parfile sections.
+ncells for dynamics/spectra; ncells is the piece-wise cell count n^2 for either EATS method, as
+EjectaID2 sets it, ejecta_id.h:180-181) plus every numeric parameter of the main and GRB parfile
+sections.
 """
 import math
 
@@ -49,9 +51,10 @@ def layer_attrs(run, il, dyn=False):
     rho = P.n_ism * (M_PROTON + 9.1094e-28)
     rd = (3. / (4. * math.pi) * 1. / (C_LIGHT * C_LIGHT * M_PROTON) * li.E0 / (rho / M_PROTON * g0 * g0)) ** (1. / 3.)
     out = {"E0": li.E0, "M0": M0, "Gamma0": g0, "beta0": beta0, "mom0": li.mom0, "R0": beta0 * C_LIGHT * tb0,
-           "Eint0": 0.0, "Rd": rd, "t0": tb0, "theta_b0": li.theta_c_h, "theta_a": 0.0}
+           "Eint0": 0.0, "Rd": rd, "t0": tb0, "theta_b0": li.theta_b0, "theta_a": 0.0}
     if dyn:
-        out.update({"theta_c": li.theta_c, "theta_c_l": li.theta_c_l, "theta_c_h": li.theta_c_h, "cil": 1.0})
+        out.update({"theta_c": li.theta_c, "theta_c_l": li.theta_c_l, "theta_c_h": li.theta_c_h,
+                    "cil": float(abi.cells_in_layer(il))})
     return out
 
 
@@ -80,7 +83,7 @@ def write_dynamics(path, run, table, fsrs, mp_, gp):
                 w.dataset(g, name, table[il, abi.PV[name]] if name in abi.PV else np.zeros(nt))
             for k, v in layer_attrs(run, il, dyn=True).items():
                 w.attr(g, k, v)
-        extra = {"nshells": 1.0, "nlayers": float(nl), "ntimes": float(nt), "ncells": float(nl)}
+        extra = {"nshells": 1.0, "nlayers": float(nl), "ntimes": float(nt), "ncells": float(abi.total_cells(nl))}
         for k, v in file_attrs(mp_, gp, extra).items():
             w.attr("/", k, v)
 
@@ -100,7 +103,7 @@ def write_spectra(path, run, pops, jall, fall, ele_grid, nu_grid, mp_, gp):
         w.dataset("/", "freqs", np.tile(nu_grid, nt))
         w.dataset("/", "times_gams", np.repeat(run.tgrid, len(ele_grid)))
         w.dataset("/", "gams", np.tile(ele_grid, nt))
-        extra = {"nshells": 1.0, "nlayers": float(nl), "ntimes": float(nt), "ncells": float(nl)}
+        extra = {"nshells": 1.0, "nlayers": float(nl), "ntimes": float(nt), "ncells": float(abi.total_cells(nl))}
         for k, v in file_attrs(mp_, gp, extra).items():
             w.attr("/", k, v)
 

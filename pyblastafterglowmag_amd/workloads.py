@@ -56,7 +56,7 @@ def scaled(name, ntb=None, ngam=None, nfreq=None, n_nu_obs=None, n_t_obs=None, n
     if n_layers and struct["struct"] == "gaussian":
         struct["n_layers_a"] = n_layers
     for k, v in over.items():
-        if k in struct:
+        if k in struct or k == "n_layers_pw":
             struct[k] = v
         elif k in ("tb0", "tb1", "ntb", "n_ism", "d_l", "z", "theta_obs", "lc_freqs", "lc_times", "rtol"):
             main[k] = v
@@ -69,6 +69,16 @@ SMALL = {
     # reduced grids of the two workloads above, small enough for the CPU oracle in tests
     "tophat_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8),
     "gauss5_small": dict(base="gauss21_fsrs_ssc", ntb=1000, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8, n_layers=5),
+    # piece-wise grids (layer i holds 2i+1 phi cells); the Gaussian wing is cut at 0.2 rad,
+    # as the piece-wise initial data keep every layer and a slower wing stops short of t_obs.
+    # The reverse shock ends on the density criterion (rho4 < 1e-30): with the default 1e-50 it
+    # ends when the swept-up fraction M3 -> 1 first rounds to >= 1, a step set by the last ulp
+    # of M3 (blastwave.h:487-510), and the piece-wise reverse-shock term (forward-shock
+    # emissivity times the reverse-shock mass) is large enough for that step to show.
+    "gauss_pw_small": dict(base="gauss21_fsrs_ssc", ntb=1000, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
+                           method_eats="piece-wise", n_layers_pw=6, theta_w=0.2, rs_shutOff_criterion_rho=1e-30),
+    "tophat_pw_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
+                            method_eats="piece-wise", n_layers_pw=4),
 }
 
 

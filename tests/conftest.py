@@ -34,7 +34,8 @@ def golden_run(g):
     idd = dict(mom=g["id_mom"], ctheta=g["id_ctheta"], ek=g["id_ek"], mass=g["id_mass"],
                theta_wing=float(g["id_theta_wing"]), theta_core=float(g["id_theta_core"]))
     tobs, nuobs = abi.obs_pairs(mo)
-    return gpu.Run(P, abi.layer_inits(idd), abi.tgrid(mp_), tobs, nuobs)
+    method = go.get("method_eats", "adaptive")
+    return gpu.Run(P, abi.layer_inits(idd, method), abi.tgrid(mp_), tobs, nuobs, method)
 
 
 def rel_err(a, b, floor_frac=0.0):

@@ -70,8 +70,12 @@ def run_reference(name, wd, spec_every=1):
     out["dyn"] = D
     out["dyn_names"] = np.array(names)
     out["dyn_layer_members"] = np.array(h5io.members(dyn, "shell=0 layer=0"))
-    out["dyn_layer_attr_names"] = np.array(h5io.attr_names(dyn, "shell=0 layer=0"))
+    names_da = h5io.attr_names(dyn, "shell=0 layer=0")
+    out["dyn_layer_attr_names"] = np.array(names_da)
+    out["dyn_layer_attrs"] = np.array([[h5io.attr(dyn, k, f"shell=0 layer={il}") for k in names_da]
+                                       for il in range(nl)])
     out["dyn_root_attrs"] = np.array(h5io.attr_names(dyn, "/"))
+    out["dyn_root_counts"] = np.array([h5io.attr(dyn, k) for k in ("nshells", "nlayers", "ntimes", "ncells")])
     spec = os.path.join(wd, "spec.h5")
     for tag in ("fs", "rs"):
         J, F = [], []
@@ -108,4 +112,4 @@ def main(names):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["tophat_small", "gauss5_small", "tophat_fs_syn"])
+    main(sys.argv[1:] or ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gauss_pw_small"])

@@ -25,7 +25,10 @@ def _declared_functions():
 def test_header_declares_entry_points():
     names = _declared_functions()
     assert "pbagpu_dyn_evolve" in names and "pbagpu_ele_evolve" in names and "pbagpu_eats_lightcurve" in names
+    assert "pbagpu_eats_lightcurve_pw" in names
     assert len(names) >= 6
+    from pyblastafterglowmag_amd import gpu
+    assert set(names) == set(gpu.EXPORTS)
 
 
 def test_library_exports_every_declared_function():
@@ -53,6 +56,7 @@ _STRUCTS = {
     "pba_bw_result": abi.BwResult,
     "pba_population": abi.Population,
     "pba_eats_layer": abi.EatsLayer,
+    "pba_eats_pw_layer": abi.EatsPwLayer,
 }
 
 
@@ -115,10 +119,35 @@ def test_parfile_round_trip(tmp_path):
 
 def test_unsupported_options_raise(tmp_path):
     main, grb, struct = workloads.tophat_fs_syn()
-    grb = dict(grb, method_eats="piece-wise")
-    par = inputs.write_parfile(str(tmp_path), main, grb)
+    for key, val in (("method_quad", "trap_adapt"), ("method_spread", "None"), ("method_eats", "adaptive-x")):
+        par = inputs.write_parfile(str(tmp_path), main, dict(grb, **{key: val}))
+        with pytest.raises(NotImplementedError):
+            abi.run_params_from_parfile(par)
+    par = inputs.write_parfile(str(tmp_path), main, dict(grb, method_eats="piece-wise"))
+    abi.run_params_from_parfile(par)
+
+
+def test_layer_inits_piecewise():
+    """Piece-wise blastwaves: E0, M0 per cell (divided by 2i+1, ejecta_id.h:315-322), theta_b0 =
+    theta_wing and ncells = n^2 (blastwave_base.h:304-312); adaptive keeps ncells = 1, theta_b0 = theta_c_h."""
+    main, grb, struct = workloads.gauss21_fsrs_ssc()
+    idd = inputs.jet_structure(kind="piece-wise", **dict(struct, n_layers_pw=7))
+    n = len(idd["ek"])
+    assert n == 7
+    P = abi.layer_inits(idd, "piece-wise")
+    A = abi.layer_inits(idd, "adaptive")
+    tw = float(idd["theta_wing"])
+    assert abi.total_cells(n) == n * n
+    for i in range(n):
+        assert P[i].E0 * (2 * i + 1) == pytest.approx(idd["ek"][i], rel=1e-15)
+        assert P[i].M0 * (2 * i + 1) == pytest.approx(idd["mass"][i], rel=1e-15)
+        assert P[i].theta_b0 == tw and P[i].ncells == n * n
+        assert A[i].theta_b0 == A[i].theta_c_h and A[i].ncells == 1.0
+        assert P[i].ctheta0 == pytest.approx(0.5 * (P[i].theta_c_l + P[i].theta_c_h), rel=1e-15)
+    # the jet's energy is shared by the cells: sum over cells = sum over the initial-data layers
+    assert sum(P[i].E0 * abi.cells_in_layer(i) for i in range(n)) == pytest.approx(float(np.sum(idd["ek"])), rel=1e-14)
     with pytest.raises(NotImplementedError):
-        abi.run_params_from_parfile(par)
+        abi.layer_inits(idd, "something")
 
 
 def test_layer_edges_piecewise():

@@ -16,7 +16,7 @@ def _case(name):
     return workloads.small(name) if name in workloads.SMALL else workloads.WORKLOADS[name]()
 
 
-@pytest.mark.parametrize("name", ["tophat_small", "tophat_fs_syn"])
+@pytest.mark.parametrize("name", ["tophat_small", "tophat_fs_syn", "tophat_pw_small"])
 def test_driver_outputs_match_reference(name, tmp_path):
     from pyblastafterglowmag_amd import pba
     g = load_golden(name)

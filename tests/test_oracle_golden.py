@@ -14,17 +14,24 @@ from pyblastafterglowmag_amd import abi
 
 
 def _populations(run, res):
+    """Populations and observer-stage layer descriptors (EatsLayer, or EatsPwLayer for a
+    piece-wise run) of a single run."""
     pops, lays = [], []
     for il in range(run.n_layers):
         pfs = len(pops)
         pops.append(abi.Population(il, 0, 0, 0))
         prs = -1
-        if res[il].fsrs and run.params.do_rs_radiation:
+        rs = res[il].fsrs and run.params.do_rs_radiation
+        if rs:
             prs = len(pops)
             pops.append(abi.Population(il, 0, 1, 0))
         li = run.inits[il]
-        lays.append(abi.EatsLayer(li.theta_c_l, li.theta_c_h, il, 0, pfs, prs, res[il].i_end_r))
-    return (abi.Population * len(pops))(*pops), (abi.EatsLayer * len(lays))(*lays)
+        if run.method == "piece-wise":
+            lays.append(abi.EatsPwLayer(li.ctheta0, li.ncells, abi.cells_in_layer(li.ilayer), il, 0, pfs,
+                                        1 if rs else 0, res[il].i_end_r))
+        else:
+            lays.append(abi.EatsLayer(li.theta_c_l, li.theta_c_h, il, 0, pfs, prs, res[il].i_end_r))
+    return (abi.Population * len(pops))(*pops), (type(lays[0]) * len(lays))(*lays)
 
 
 def _oracle_all(g):
@@ -32,7 +39,8 @@ def _oracle_all(g):
     table, res = pyoracle.dyn_evolve(run.params, run.inits, run.tgrid)
     pops, lays = _populations(run, res)
     jall, fall, st = pyoracle.ele_evolve(run.params, pops, table, want_f=True)
-    lc, st2 = pyoracle.eats_lightcurve(run.params, lays, table, jall, run.tobs, run.nuobs)
+    eats = pyoracle.eats_lightcurve_pw if run.method == "piece-wise" else pyoracle.eats_lightcurve
+    lc, st2 = eats(run.params, lays, table, jall, run.tobs, run.nuobs)
     assert all(r.status == 0 for r in res) and not st.any() and not st2.any()
     return run, table, res, pops, jall, fall, lc
 
@@ -40,6 +48,8 @@ def _oracle_all(g):
 @pytest.mark.parametrize("name,tol_dyn,tol_j,tol_lc", [
     ("tophat_small", 0.0, 0.0, 0.0),
     ("gauss5_small", 1e-4, 1e-5, 1e-8),
+    ("tophat_pw_small", 0.0, 0.0, 0.0),
+    ("gauss_pw_small", 1e-4, 1e-5, 1e-8),
 ])
 def test_oracle_matches_reference(name, tol_dyn, tol_j, tol_lc):
     g = load_golden(name)

@@ -9,7 +9,7 @@ import pytest
 from conftest import golden_run, load_golden
 from pyblastafterglowmag_amd import abi, h5io, output
 
-CASES = ["tophat_small", "gauss5_small", "tophat_fs_syn"]
+CASES = ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gauss_pw_small"]
 
 
 def _parse(g, tmp_path):
@@ -54,6 +54,14 @@ def test_dynamics_file_layout(name, tmp_path):
     assert sorted(h5io.attr_names(path, "shell=0 layer=0")) == sorted(str(x) for x in g["dyn_layer_attr_names"])
     assert sorted(h5io.attr_names(path, "/")) == sorted(str(x) for x in g["dyn_root_attrs"])
     np.testing.assert_array_equal(h5io.read(path, "shell=0 layer=0/Gamma"), np.full(nt, 2.0))
+    # per-layer values (E0, M0, theta_b0, cil, ...) and the file's counts (ncells = n^2)
+    names = [str(x) for x in g["dyn_layer_attr_names"]]
+    for il in range(run.n_layers):
+        for k, ref in zip(names, g["dyn_layer_attrs"][il]):
+            mine = h5io.attr(path, k, f"shell=0 layer={il}")
+            assert mine == pytest.approx(ref, rel=1e-12, abs=0), (name, il, k, mine, ref)
+    counts = [h5io.attr(path, k) for k in ("nshells", "nlayers", "ntimes", "ncells")]
+    np.testing.assert_array_equal(counts, g["dyn_root_counts"])
 
 
 @pytest.mark.parametrize("name", CASES)
