@@ -254,6 +254,27 @@ int pbagpu_eats_lightcurve_pw(const pba_run_params *params, int n_runs,
                               int n_pairs, const double *d_tobs, const double *d_nuobs,
                               double *d_out, int *d_status, void *stream);
 
+/* Raw sky-map image arrays, in the order of IMG::m_names (src/image.h:14-26). */
+enum pba_img_var {
+    PBA_IMG_r = 0, PBA_IMG_ctheta, PBA_IMG_cphi, PBA_IMG_mu, PBA_IMG_xrs, PBA_IMG_yrs, PBA_IMG_intensity,
+    PBA_IMG_NV
+};
+
+/* Piece-wise sky maps of one run: for every (t_obs, nu_obs) image, every cell's
+ * raw entry as EATS::evalSkyMapPW writes it into its `out` arrays
+ * (src/eats.h:1163-1184), before Ejecta::computeEjectaSkyMapPW's normalisation
+ * (src/ejecta/ejecta_base.h:471-545).  Cells are stacked layer after layer in
+ * the order of `layers` (all of run 0, params[0]); ncells = sum of cil.
+ *   d_image  [n_images][PBA_IMG_NV][2 * ncells] device, written: primary-jet cells,
+ *            then counter-jet cells; cells that do not contribute are 0
+ *   d_flux   [n_layers][n_images] device, written: each layer's flux density, mJy
+ *            (the value evalSkyMapPW returns)
+ *   remove_mu  skymap_remove_mu: 0 divides the intensity by |mu|, as the default */
+int pbagpu_eats_skymap_pw(const pba_run_params *params, const pba_eats_pw_layer *layers, int n_layers,
+                          const double *d_table, int nt, const double *d_jall, int nfreq,
+                          int n_images, const double *d_tobs, const double *d_nuobs, int remove_mu,
+                          double *d_image, double *d_flux, int *d_status, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -466,7 +466,25 @@ static double flux_dens_shock_pw(PwCtx &L, double freq, size_t ia, size_t ib, do
 
 // EATS::evalSkyMapPW, one jet (src/eats.h:1064-1213), with fluxDensPieceWiseWithComov
 // (src/blastwave/blastwave_radiation.h:14-72) as the cell flux.
-static double sky_map_pw(PwCtx &L, double t_obs, double nu_obs, bool cj) {
+// Raw image arrays of one (t_obs, nu_obs): IMG::m_names order, 2 ncells entries each.
+struct PwImage {
+    double *out = nullptr;   // [PBA_IMG_NV][2 * ncells]
+    size_t ncells = 0, offset = 0;
+    bool remove_mu = false;
+};
+
+// imageXXs / imageYYs / imageXXsCJ / imageYYsCJ (src/image.h:379-392)
+static void image_xy(bool cj, double theta, double phi, double alpha, double &x, double &y) {
+    if (!cj) {
+        x = -1.0 * std::cos(alpha) * (std::sin(theta) * std::sin(phi)) + std::sin(alpha) * std::cos(theta);
+        y = +1. * std::sin(theta) * std::cos(phi);
+    } else {
+        x = -1.0 * std::cos(alpha) * std::sin(pi - theta) * std::sin(phi) + std::sin(alpha) * std::cos(pi - theta);
+        y = +1. * std::sin(pi - theta) * std::cos(phi);
+    }
+}
+
+static double sky_map_pw(PwCtx &L, double t_obs, double nu_obs, bool cj, const PwImage *im = nullptr) {
     const pba_run_params &P = *L.P;
     const double *R = L.V(PV_R), *tt = L.V(PV_tt), *G = L.V(PV_Gamma);
     const int N = L.i_end_r;
@@ -511,6 +529,27 @@ static double sky_map_pw(PwCtx &L, double t_obs, double nu_obs, bool cj) {
         }
         f *= (1.0 + P.z) / (2.0 * P.d_l * P.d_l);
         if (!std::isfinite(f) || f < 0) L.status |= PBAGPU_S_NONFINITE;
+        if (im && im->out) {   // eats.h:1163-1184
+            const double rad2mas = 206264806.247;
+            // fluxDensPieceWiseWithComov's ctheta (interpSegLin with the cell's ta, tb, interpolators.h:107-113)
+            const double *cth = L.V(PV_ctheta);
+            double ctheta = cth[a] + (cth[b] - cth[a]) * (t_obs - ta[i]) / (tb[i] - ta[i]);
+            double phi_cell = (double)i * 2.0 * M_PI / (double)L.cil, x, y;
+            image_xy(cj, ctheta, phi_cell, P.theta_obs, x, y);
+            const size_t vs = 2 * im->ncells, j = (cj ? im->ncells : 0) + im->offset + i;
+            double *o = im->out;
+            o[PBA_IMG_intensity * vs + j] = f / (r[i] * r[i]) * cgs2mJy;
+            o[PBA_IMG_xrs * vs + j] = r[i] * x;
+            o[PBA_IMG_yrs * vs + j] = r[i] * y;
+            o[PBA_IMG_r * vs + j] = r[i];
+            o[PBA_IMG_ctheta * vs + j] = L.ctheta0;
+            o[PBA_IMG_cphi * vs + j] = phi_cell;
+            o[PBA_IMG_mu * vs + j] = mu[i];
+            if (!im->remove_mu) o[PBA_IMG_intensity * vs + j] /= std::abs(mu[i]);
+            o[PBA_IMG_xrs * vs + j] *= (rad2mas / P.d_l);
+            o[PBA_IMG_yrs * vs + j] *= (rad2mas / P.d_l);
+            o[PBA_IMG_intensity * vs + j] *= (P.d_l * P.d_l / rad2mas / rad2mas);
+        }
         tot += f;
     }
     return tot * cgs2mJy;
@@ -550,6 +589,54 @@ extern "C" int orc_eats_lightcurve_pw(const pba_run_params *params, int n_runs, 
             out[(size_t)l * n_pairs + k] = fl;
         }
         status[l] = L.status;
+    }
+    return PBAGPU_OK;
+}
+
+/* Piece-wise raw sky maps of one run's layers (run 0): image [n_images][PBA_IMG_NV][2 ncells] and the
+ * per-layer flux [n_layers][n_images]; Ejecta::computeEjectaSkyMapPW's call sequence
+ * (src/ejecta/ejecta_base.h:471-495), the normalisation after it is host logic. */
+extern "C" int orc_eats_skymap_pw(const pba_run_params *params, const pba_eats_pw_layer *layers, int n_layers,
+                                  const double *table, int nt, const double *jall, int nfreq, int n_images,
+                                  const double *tobs, const double *nuobs, int remove_mu, double *image,
+                                  double *flux, int *status) {
+    if (!params || !layers || !table || !jall || !image || !flux || !status || n_layers < 1) return PBAGPU_E_ARG;
+    size_t ncells = 0;
+    for (int l = 0; l < n_layers; l++) ncells += (size_t)layers[l].cil;
+    std::fill(image, image + (size_t)n_images * PBA_IMG_NV * 2 * ncells, 0.);
+    size_t offset = 0;
+    for (int l = 0; l < n_layers; l++) {
+        const pba_eats_pw_layer &Ly = layers[l];
+        orc::PwCtx L;
+        L.P = &params[0];
+        L.T = table + (size_t)Ly.bw * PBA_NV * nt;
+        L.nt = nt;
+        L.i_end_r = Ly.i_end_r;
+        L.jfs = jall + (size_t)Ly.pop_fs * nt * nfreq;
+        L.nu_fs = orc::freq_grid(L.P->fs);
+        L.rs = Ly.rs != 0;
+        L.ctheta0 = Ly.ctheta0;
+        L.ncells = Ly.ncells;
+        L.cil = Ly.cil;
+        L.ttobs.assign(nt, std::numeric_limits<double>::max());
+        const double *R = L.V(PV_R), *tb = L.V(PV_tburst);
+        for (int k = 0; k < n_images; k++) {
+            double fl = 0.;
+            orc::PwImage im;
+            im.out = image + (size_t)k * PBA_IMG_NV * 2 * ncells;
+            im.ncells = ncells;
+            im.offset = offset;
+            im.remove_mu = remove_mu != 0;
+            if (L.i_end_r > 0) {
+                if (R[0] > 0 && (tobs[k] < tb[0] || tobs[k] > tb[L.i_end_r - 1])) L.status |= PBAGPU_S_NONFINITE;
+                double pj = orc::sky_map_pw(L, tobs[k], nuobs[k], false, &im), cj = 0.;
+                if (L.P->counter_jet) cj = orc::sky_map_pw(L, tobs[k], nuobs[k], true, &im);
+                fl = pj + cj;
+            }
+            flux[(size_t)l * n_images + k] = fl;
+        }
+        status[l] = L.status;
+        offset += (size_t)Ly.cil;
     }
     return PBAGPU_OK;
 }

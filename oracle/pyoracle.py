@@ -26,6 +26,9 @@ def lib():
         if hasattr(_lib, "orc_eats_lightcurve"):
             _lib.orc_eats_lightcurve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsLayer), C.c_int,
                                                  dp, C.c_int, dp, C.c_int, C.c_int, dp, dp, dp, ip]
+        if hasattr(_lib, "orc_eats_skymap_pw"):
+            _lib.orc_eats_skymap_pw.argtypes = [C.POINTER(abi.RunParams), C.POINTER(abi.EatsPwLayer), C.c_int, dp,
+                                                C.c_int, dp, C.c_int, C.c_int, dp, dp, C.c_int, dp, dp, ip]
         if hasattr(_lib, "orc_eats_lightcurve_pw"):
             _lib.orc_eats_lightcurve_pw.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsPwLayer),
                                                     C.c_int, dp, C.c_int, dp, C.c_int, C.c_int, dp, dp, dp, ip]
@@ -108,3 +111,25 @@ def eats_lightcurve_pw(params, layers, table, jall, tobs, nuobs, n_runs=1):
     if rc != 0:
         raise RuntimeError(f"orc_eats_lightcurve_pw rc={rc}")
     return out, st
+
+
+def eats_skymap_pw(params, layers, table, jall, tobs, nuobs, remove_mu=False):
+    """Raw piece-wise sky maps of one run: (image [n_images][PBA_IMG_NV][2 ncells], flux [n_layers][n_images],
+    status [n_layers])."""
+    if isinstance(params, abi.RunParams):
+        params = (abi.RunParams * 1)(params)
+    nbw, nv, nt = table.shape
+    nf = jall.shape[2]
+    n = len(tobs)
+    ncells = sum(L.cil for L in layers)
+    image = np.zeros((n, abi.PBA_IMG_NV, 2 * ncells))
+    flux = np.zeros((len(layers), n))
+    st = np.zeros(len(layers), dtype=np.int32)
+    rc = lib().orc_eats_skymap_pw(params, layers, len(layers), _dp(np.ascontiguousarray(table)), nt,
+                                  _dp(np.ascontiguousarray(jall)), nf, n,
+                                  _dp(np.ascontiguousarray(tobs, dtype=np.float64)),
+                                  _dp(np.ascontiguousarray(nuobs, dtype=np.float64)), 1 if remove_mu else 0,
+                                  _dp(image), _dp(flux), _ip(st))
+    if rc != 0:
+        raise RuntimeError(f"orc_eats_skymap_pw rc={rc}")
+    return image, flux, st

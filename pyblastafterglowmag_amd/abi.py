@@ -24,6 +24,10 @@ PBA_NV_NAMES = [
 ]
 PBA_NV = len(PBA_NV_NAMES)
 PV = {n: i for i, n in enumerate(PBA_NV_NAMES)}
+# raw sky-map arrays (enum pba_img_var; IMG::m_names, src/image.h:14-26)
+IMG_NAMES = ["r", "ctheta", "cphi", "mu", "xrs", "yrs", "intensity"]
+PBA_IMG_NV = len(IMG_NAMES)
+IMG = {n: i for i, n in enumerate(IMG_NAMES)}
 
 
 class ShockParams(C.Structure):

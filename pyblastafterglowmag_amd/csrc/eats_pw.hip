@@ -14,6 +14,9 @@
 //     (src/microphysics/microphysics.h:799-867, 1185-1217), with the cell angle and the
 //     jet's total cell count in the shell-thickness and luminosity terms and the
 //     forward-shock emissivity for both shocks.
+// With an image buffer the same kernel also writes every cell's raw sky-map entry (radius,
+// angles, mu, sky-plane position in mas and intensity), as evalSkyMapPW fills its `out`
+// arrays (src/eats.h:1163-1184); cells that do not contribute hold zeros, as there.
 #include "pba_device.h"
 #include "timing.h"
 #include "runtime.h"
@@ -39,6 +42,15 @@ struct PwArgs {
     double ctheta0, ncells;
     double z, theta_obs, d_l;
     int cil, rs, i_end_r, counter_jet;
+    int img_off;     // first cell of this layer in the image (cells of earlier layers before it)
+    int remove_mu;   // skymap_remove_mu
+};
+
+// Raw sky-map image of one (t_obs, nu_obs): PBA_IMG_NV arrays of 2 ncells entries, primary
+// jet cells first, then the counter-jet cells.
+struct Img {
+    double *base;    // null: light curve only
+    int ncells;
 };
 
 extern __shared__ double lds_pw[];
@@ -91,10 +103,26 @@ struct Lane {
     }
 };
 
+// imageXXs / imageYYs and their counter-jet forms (src/image.h:379-392)
+__device__ __forceinline__ void image_xy(bool cj, double theta, double phi, double sa, double ca, double &x,
+                                         double &y) {
+    if (!cj) {
+        x = -1.0 * ca * (sin(theta) * sin(phi)) + sa * cos(theta);
+        y = +1. * sin(theta) * cos(phi);
+    } else {
+        x = -1.0 * ca * sin(kPi - theta) * sin(phi) + sa * cos(kPi - theta);
+        y = +1. * sin(kPi - theta) * cos(phi);
+    }
+}
+
+constexpr double kRad2mas = 206264806.247;
+
 // One jet of EATS::evalSkyMapPW: the sum of the cell fluxes, mJy.
 __device__ double sky_map_pw(Lane &L, const PwArgs &A, double t_obs, double nu_obs, double sa, double ca, double st,
-                             double ct, unsigned long long &cells) {
+                             double ct, bool cj, const Img &im, unsigned long long &cells) {
     const int N = L.N;
+    double *img = im.base ? im.base + (cj ? im.ncells : 0) + A.img_off : nullptr;
+    const size_t vs = 2 * (size_t)im.ncells;   // stride between the image arrays
     if (L.V(C_R, 0) == 0. && L.V(C_Gamma, 0) == 0.) return 0.;
     if (L.V(C_tt, 0) == 0. && N < L.nt && L.V(C_tt, N) == 0.) L.status |= PBAGPU_S_NONFINITE;
     double tot = 0.;
@@ -132,6 +160,22 @@ __device__ double sky_map_pw(Lane &L, const PwArgs &A, double t_obs, double nu_o
         }
         f *= (1.0 + A.z) / (2.0 * A.d_l * A.d_l);
         if (!isfinite(f) || f < 0) L.status |= PBAGPU_S_NONFINITE;
+        if (img) {
+            // fluxDensPieceWiseWithComov's ctheta: linear in time between the bracketing steps
+            const double *cth = A.T + (size_t)PV_ctheta * L.nt;
+            const double ctheta = cth[ia] + (cth[ib] - cth[ia]) * (t_obs - ta) / (tb - ta);
+            double x, y;
+            image_xy(cj, ctheta, cphi, sa, ca, x, y);
+            double inten = f / (r * r) * kCgs2mJy;
+            if (!A.remove_mu) inten /= fabs(mu);
+            img[PBA_IMG_r * vs + i] = r;
+            img[PBA_IMG_ctheta * vs + i] = A.ctheta0;
+            img[PBA_IMG_cphi * vs + i] = cphi;
+            img[PBA_IMG_mu * vs + i] = mu;
+            img[PBA_IMG_xrs * vs + i] = r * x * (kRad2mas / A.d_l);
+            img[PBA_IMG_yrs * vs + i] = r * y * (kRad2mas / A.d_l);
+            img[PBA_IMG_intensity * vs + i] = inten * (A.d_l * A.d_l / kRad2mas / kRad2mas);
+        }
         tot += f;
         cells++;
     }
@@ -143,7 +187,8 @@ __global__ void __launch_bounds__(PW_WG) eats_pw_kernel(const PwArgs *__restrict
                                                         const double *__restrict__ tobs,
                                                         const double *__restrict__ nuobs, double *__restrict__ out,
                                                         int *__restrict__ status,
-                                                        unsigned long long *__restrict__ n_cells) {
+                                                        unsigned long long *__restrict__ n_cells, double *image,
+                                                        int ncells) {
     const int l = blockIdx.y;
     const PwArgs A = layers[l];
     for (int idx = threadIdx.x; idx < NCOL * nt; idx += PW_WG) {
@@ -159,6 +204,14 @@ __global__ void __launch_bounds__(PW_WG) eats_pw_kernel(const PwArgs *__restrict
     L.nt = nt; L.nf = nf; L.N = A.i_end_r; L.status = 0;
     double fl = 0.;
     unsigned long long cells = 0;
+    if (image) {   // this layer's cells of both jets start at zero, as the reference's image arrays
+        double *b = image + (size_t)k * PBA_IMG_NV * 2 * ncells + A.img_off;
+        for (int v = 0; v < PBA_IMG_NV; v++)
+            for (int i = 0; i < A.cil; i++) {
+                b[(size_t)v * 2 * ncells + i] = 0.;
+                b[(size_t)v * 2 * ncells + ncells + i] = 0.;
+            }
+    }
     if (A.i_end_r > 0) {   // evalEjectaLightCurves skips blastwaves that never evolved
         const double t = tobs[k], nu = nuobs[k];
         // evalSkyMapPW(t_obs, nu_obs, offset): t_obs inside the evolved burster times
@@ -168,9 +221,12 @@ __global__ void __launch_bounds__(PW_WG) eats_pw_kernel(const PwArgs *__restrict
         }
         const double th = A.ctheta0, th_cj = kPi - A.ctheta0;
         const double sa = sin(A.theta_obs), ca = cos(A.theta_obs);
-        const double pj = sky_map_pw(L, A, t, nu, sa, ca, sin(th), cos(th), cells);
+        Img im;
+        im.base = image ? image + (size_t)k * PBA_IMG_NV * 2 * ncells : nullptr;
+        im.ncells = ncells;
+        const double pj = sky_map_pw(L, A, t, nu, sa, ca, sin(th), cos(th), false, im, cells);
         double cjf = 0.;
-        if (A.counter_jet) cjf = sky_map_pw(L, A, t, nu, sa, ca, sin(th_cj), cos(th_cj), cells);
+        if (A.counter_jet) cjf = sky_map_pw(L, A, t, nu, sa, ca, sin(th_cj), cos(th_cj), true, im, cells);
         fl = pj + cjf;
     }
     out[(size_t)l * n_pairs + k] = fl;
@@ -186,30 +242,38 @@ __global__ void __launch_bounds__(PW_WG) eats_pw_kernel(const PwArgs *__restrict
 }  // namespace eatspw
 }  // namespace pbagpu
 
-extern "C" int pbagpu_eats_lightcurve_pw(const pba_run_params *params, int n_runs, const pba_eats_pw_layer *layers,
-                                         int n_layers, const double *d_table, int nt, const double *d_jall, int nfreq,
-                                         int n_pairs, const double *d_tobs, const double *d_nuobs, double *d_out,
-                                         int *d_status, void *stream) {
-    using namespace pbagpu::eatspw;
+namespace pbagpu {
+namespace eatspw {
+
+// Shared host part of the light-curve and sky-map entries; image == nullptr for light curves.
+static int launch(const pba_run_params *params, int n_runs, const pba_eats_pw_layer *layers, int n_layers,
+                  const double *d_table, int nt, const double *d_jall, int nfreq, int n_pairs, const double *d_tobs,
+                  const double *d_nuobs, double *d_out, int *d_status, double *d_image, int remove_mu,
+                  hipStream_t s) {
     if (!params || !layers || !d_table || !d_jall || !d_tobs || !d_nuobs || !d_out || !d_status || n_runs < 1 ||
         n_layers < 1 || n_pairs < 1 || nt < 2 || nfreq < 2 || n_layers > 65535)
         return PBAGPU_E_ARG;
     const size_t lds = sizeof(double) * ((size_t)NCOL * nt + (size_t)nfreq);
     if (lds > 160 * 1024) {
-        pbagpu::set_error_msg("eats_pw: time grid too long for the LDS-staged table (nt <= 2200)");
+        set_error_msg("eats_pw: time grid too long for the LDS-staged table (nt <= 2200)");
         return PBAGPU_E_ARG;
     }
+    long ncells = 0;
     for (int l = 0; l < n_layers; l++) {
         const pba_eats_pw_layer &Y = layers[l];
         if (Y.run < 0 || Y.run >= n_runs || Y.i_end_r > nt || Y.i_end_r < 0 || Y.pop_fs < 0 || Y.bw < 0 ||
             Y.cil < 1 || !(Y.ncells >= 1.0)) {
-            pbagpu::set_error_msg("eats_pw: bad layer descriptor");
+            set_error_msg("eats_pw: bad layer descriptor");
             return PBAGPU_E_ARG;
         }
+        ncells += Y.cil;
+    }
+    if (d_image && ((size_t)n_pairs * PBA_IMG_NV * 2 * ncells > ((size_t)1 << 40) || ncells > (1L << 30))) {
+        set_error_msg("eats_pw: sky-map image too large");
+        return PBAGPU_E_ARG;
     }
     for (int r = 0; r < n_runs; r++)
         if (params[r].fs.nfreq != nfreq) return PBAGPU_E_ARG;
-    hipStream_t s = (hipStream_t)stream;
     // forward-shock frequency grid of every run (MakeLogspaceVec, utils.h:91-105)
     std::vector<double> hnu((size_t)n_runs * nfreq);
     for (int r = 0; r < n_runs; r++) {
@@ -218,10 +282,11 @@ extern "C" int pbagpu_eats_lightcurve_pw(const pba_run_params *params, int n_run
         const double ratio = std::pow(10.0, (std::log10(sp.freq2) - std::log10(sp.freq1)) / sp.nfreq);
         for (int i = 0; i < nfreq; i++) { hnu[(size_t)r * nfreq + i] = cur; cur *= ratio; }
     }
-    auto *dnu = (double *)pbagpu::workspace(pbagpu::WS_PW_NU, sizeof(double) * hnu.size());
-    auto *dL = (PwArgs *)pbagpu::workspace(pbagpu::WS_PW_LAYERS, sizeof(PwArgs) * n_layers);
+    auto *dnu = (double *)workspace(WS_PW_NU, sizeof(double) * hnu.size());
+    auto *dL = (PwArgs *)workspace(WS_PW_LAYERS, sizeof(PwArgs) * n_layers);
     if (!dnu || !dL) return PBAGPU_E_HIP;
     std::vector<PwArgs> hl(n_layers);
+    int off = 0;
     for (int l = 0; l < n_layers; l++) {
         const pba_eats_pw_layer &Y = layers[l];
         const pba_run_params &P = params[Y.run];
@@ -232,15 +297,17 @@ extern "C" int pbagpu_eats_lightcurve_pw(const pba_run_params *params, int n_run
         A.ctheta0 = Y.ctheta0; A.ncells = Y.ncells;
         A.z = P.z; A.theta_obs = P.theta_obs; A.d_l = P.d_l;
         A.cil = Y.cil; A.rs = Y.rs ? 1 : 0; A.i_end_r = Y.i_end_r; A.counter_jet = P.counter_jet;
+        A.img_off = off; A.remove_mu = remove_mu ? 1 : 0;
+        off += Y.cil;
     }
     int rc = PBAGPU_OK;
     PBA_TRY(rc, hipMemcpyAsync(dnu, hnu.data(), sizeof(double) * hnu.size(), hipMemcpyHostToDevice, s));
     PBA_TRY(rc, hipMemcpyAsync(dL, hl.data(), sizeof(PwArgs) * n_layers, hipMemcpyHostToDevice, s));
     PBA_TRY(rc, hipMemsetAsync(d_status, 0, sizeof(int) * n_layers, s));
-    pbagpu::KernelTimer timer(PBAGPU_STAGE_EATS_PW);
+    KernelTimer timer(PBAGPU_STAGE_EATS_PW);
     unsigned long long *d_cells = nullptr;
-    if (rc == PBAGPU_OK && pbagpu::timing_enabled()) {
-        d_cells = (unsigned long long *)pbagpu::workspace(pbagpu::WS_PW_SAMPLES, sizeof(unsigned long long));
+    if (rc == PBAGPU_OK && timing_enabled()) {
+        d_cells = (unsigned long long *)workspace(WS_PW_SAMPLES, sizeof(unsigned long long));
         if (d_cells) PBA_TRY(rc, hipMemsetAsync(d_cells, 0, sizeof(unsigned long long), s));
     }
     if (rc == PBAGPU_OK) {
@@ -248,17 +315,46 @@ extern "C" int pbagpu_eats_lightcurve_pw(const pba_run_params *params, int n_run
         dim3 grid((unsigned)((n_pairs + PW_WG - 1) / PW_WG), (unsigned)n_layers);
         timer.start(s);
         hipLaunchKernelGGL(eats_pw_kernel, grid, dim3(PW_WG), lds, s, dL, n_pairs, nt, nfreq, d_tobs, d_nuobs, d_out,
-                           d_status, d_cells);
+                           d_status, d_cells, d_image, (int)ncells);
         PBA_TRY(rc, hipGetLastError());
         timer.stop(s);
         if (d_cells && rc == PBAGPU_OK) {
             unsigned long long hc = 0;
             PBA_TRY(rc, hipMemcpyAsync(&hc, d_cells, sizeof(hc), hipMemcpyDeviceToHost, s));
             PBA_TRY(rc, hipStreamSynchronize(s));
-            pbagpu::timing_add_units(PBAGPU_STAGE_EATS_PW, (double)hc);
+            timing_add_units(PBAGPU_STAGE_EATS_PW, (double)hc);
         }
     }
     PBA_TRY(rc, hipStreamSynchronize(s));
     timer.collect();
     return rc;
+}
+
+}  // namespace eatspw
+}  // namespace pbagpu
+
+extern "C" int pbagpu_eats_lightcurve_pw(const pba_run_params *params, int n_runs, const pba_eats_pw_layer *layers,
+                                         int n_layers, const double *d_table, int nt, const double *d_jall, int nfreq,
+                                         int n_pairs, const double *d_tobs, const double *d_nuobs, double *d_out,
+                                         int *d_status, void *stream) {
+    return pbagpu::eatspw::launch(params, n_runs, layers, n_layers, d_table, nt, d_jall, nfreq, n_pairs, d_tobs,
+                                  d_nuobs, d_out, d_status, nullptr, 0, (hipStream_t)stream);
+}
+
+extern "C" int pbagpu_eats_skymap_pw(const pba_run_params *params, const pba_eats_pw_layer *layers, int n_layers,
+                                     const double *d_table, int nt, const double *d_jall, int nfreq, int n_images,
+                                     const double *d_tobs, const double *d_nuobs, int remove_mu, double *d_image,
+                                     double *d_flux, int *d_status, void *stream) {
+    if (!d_image || !layers || n_layers < 1) return PBAGPU_E_ARG;
+    for (int l = 0; l < n_layers; l++)
+        if (layers[l].run != layers[0].run) {
+            pbagpu::set_error_msg("eats_skymap_pw: the layers of one image belong to one run");
+            return PBAGPU_E_ARG;
+        }
+    if (layers[0].run != 0) {
+        pbagpu::set_error_msg("eats_skymap_pw: layers refer to params[0] (run = 0)");
+        return PBAGPU_E_ARG;
+    }
+    return pbagpu::eatspw::launch(params, 1, layers, n_layers, d_table, nt, d_jall, nfreq, n_images, d_tobs, d_nuobs,
+                                  d_flux, d_status, d_image, remove_mu, (hipStream_t)stream);
 }

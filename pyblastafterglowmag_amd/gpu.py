@@ -16,7 +16,7 @@ _lib = None
 
 EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu_last_error", "pbagpu_timing_enable",
            "pbagpu_timing_read", "pbagpu_timing_units", "pbagpu_dyn_evolve", "pbagpu_ele_evolve", "pbagpu_eats_lightcurve",
-           "pbagpu_eats_lightcurve_pw")
+           "pbagpu_eats_lightcurve_pw", "pbagpu_eats_skymap_pw")
 STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW = 0, 1, 2, 3, 4
 
 
@@ -46,6 +46,8 @@ def lib():
                                              C.c_int, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.pbagpu_eats_lightcurve_pw.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsPwLayer), C.c_int,
                                                 vp, C.c_int, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+        L.pbagpu_eats_skymap_pw.argtypes = [C.POINTER(abi.RunParams), C.POINTER(abi.EatsPwLayer), C.c_int, vp, C.c_int,
+                                            vp, C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -242,6 +244,33 @@ class Pipeline:
                                              self.nt, _ptr(self.d_jall), self.nf, self.npairs, _ptr(self.d_tobs),
                                              _ptr(self.d_nuobs), _ptr(self.d_out), _ptr(self.d_lst), self.sp),
                "eats_lightcurve")
+
+    def skymaps(self, times, freqs, remove_mu=False):
+        """Raw piece-wise sky maps of a one-run batch at every (freq, time), frequency-major as
+        Ejecta::computeSaveEjectaSkyImages orders them (src/ejecta/ejecta.h:259-312).
+
+        Returns (image [n_images][PBA_IMG_NV][2 ncells], flux [n_layers][n_images], status [n_layers])."""
+        if self.method != "piece-wise":
+            raise NotImplementedError("sky maps are implemented for the piece-wise EATS method only")
+        if self.n_runs != 1:
+            raise ValueError("sky maps are computed for one run at a time")
+        torch = self.torch
+        times = np.asarray(times, dtype=np.float64)
+        freqs = np.asarray(freqs, dtype=np.float64)
+        t_img = np.tile(times, len(freqs))
+        nu_img = np.repeat(freqs, len(times))
+        n_img = len(t_img)
+        nl = len(self.layers)
+        ncells = sum(self.larr[i].cil for i in range(nl))
+        d_t = torch.from_numpy(t_img).to(self.dev)
+        d_nu = torch.from_numpy(nu_img).to(self.dev)
+        d_img = torch.empty((n_img, abi.PBA_IMG_NV, 2 * ncells), dtype=torch.float64, device=self.dev)
+        d_flux = torch.empty((nl, n_img), dtype=torch.float64, device=self.dev)
+        d_st = torch.zeros(nl, dtype=torch.int32, device=self.dev)
+        _check(self.L.pbagpu_eats_skymap_pw(self.params, self.larr, nl, _ptr(self.d_table), self.nt, _ptr(self.d_jall),
+                                            self.nf, n_img, _ptr(d_t), _ptr(d_nu), 1 if remove_mu else 0,
+                                            _ptr(d_img), _ptr(d_flux), _ptr(d_st), self.sp), "eats_skymap_pw")
+        return d_img.cpu().numpy(), d_flux.cpu().numpy(), d_st.cpu().numpy()
 
     def step(self):
         self.dynamics()

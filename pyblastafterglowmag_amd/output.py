@@ -13,6 +13,7 @@ EjectaID2 sets it, ejecta_id.h:180-181) plus every numeric parameter of the main
 sections.
 """
 import math
+import os
 
 import numpy as np
 
@@ -120,3 +121,92 @@ def freq_grid(shock_params):
     """Comoving frequency grid by repeated multiplication (MakeLogspaceVec, utils.h:91-105)."""
     from .inputs import logspace_vec
     return logspace_vec(math.log10(shock_params.freq1), math.log10(shock_params.freq2), int(shock_params.nfreq))
+
+
+DBL_MAX = np.finfo(np.float64).max
+DBL_MIN = np.finfo(np.float64).tiny   # std::numeric_limits<double>::min()
+
+
+def skymap_images(image, flux, nlayers):
+    """Normalised and cleaned raw sky maps, as the reference post-processes evalSkyMapPW's arrays.
+
+    image: [n_images][PBA_IMG_NV][2 ncells] raw arrays of one shell (pbagpu_eats_skymap_pw);
+    flux:  [n_layers][n_images] per-layer flux densities, mJy.
+    Returns one dict per image with the cleaned arrays (IMG::m_names), the shell flux and the
+    attributes saveRawImage writes.
+
+    * Ejecta::computeEjectaSkyMapPW (src/ejecta/ejecta_base.h:471-545): the shell flux is the sum
+      of the layers' fluxes; a positive flux scales the intensities by flux / (dx dy) / max(I),
+      with dx, dy the extent of the cells' sky positions (cells that did not contribute sit at 0
+      and count), and sets the image extent.
+    * ImageExtend::removeZeroEntries (src/image.h:120-190): unless one jet has every cell above
+      1e-10 of the peak intensity, only cells above it in either jet are kept (both jets' entries
+      of those cells).
+    """
+    out = []
+    n_img = image.shape[0]
+    for k in range(n_img):
+        data = np.array(image[k], dtype=np.float64)
+        fl_shell = 0.0
+        for il in range(flux.shape[0]):
+            fl_shell += float(flux[il, k])
+        total = 0.0
+        bounds = dict(xmin=DBL_MAX, ymin=DBL_MAX, xmax=DBL_MIN, ymax=DBL_MIN)
+        if fl_shell <= 0 or not math.isfinite(fl_shell):
+            total += fl_shell
+        else:
+            X, Y, I = data[abi.IMG["xrs"]], data[abi.IMG["yrs"]], data[abi.IMG["intensity"]]
+            xmin, xmax, ymin, ymax, imax = DBL_MAX, DBL_MIN, DBL_MAX, DBL_MIN, DBL_MIN
+            for i in range(len(I)):   # the reference's sequential scan (strict comparisons)
+                if X[i] < xmin: xmin = X[i]
+                if X[i] > xmax: xmax = X[i]
+                if Y[i] < ymin: ymin = Y[i]
+                if Y[i] > ymax: ymax = Y[i]
+                if I[i] > imax: imax = I[i]
+            if imax <= 0 or not math.isfinite(imax):
+                raise RuntimeError(f"sky map {k}: maximum intensity {imax}")
+            dfnu = fl_shell / ((xmax - xmin) * (ymax - ymin))
+            data[abi.IMG["intensity"]] = I * (dfnu / imax)
+            if xmin < bounds["xmin"]: bounds["xmin"] = xmin
+            if ymin < bounds["ymin"]: bounds["ymin"] = ymin
+            if xmax > bounds["xmax"]: bounds["xmax"] = xmax
+            if ymax > bounds["ymax"]: bounds["ymax"] = ymax
+            total += fl_shell
+        # removeZeroEntries
+        nc = data.shape[1] // 2
+        I = data[abi.IMG["intensity"]]
+        max_pj = max(DBL_MIN, float(np.max(I[:nc]))) if nc else DBL_MIN
+        max_cj = max(DBL_MIN, float(np.max(I[nc:]))) if nc else DBL_MIN
+        thr = max(max_pj, max_cj) * 1.e-10
+        nz_pj = int(np.count_nonzero(I[:nc] > thr))
+        nz_cj = int(np.count_nonzero(I[nc:] > thr))
+        if not (nz_pj == nc or nz_cj == nc):
+            keep = np.nonzero((I[:nc] > thr) | (I[nc:] > thr))[0]
+            data = np.concatenate([data[:, keep], data[:, nc + keep]], axis=1)
+        out.append(dict(arrays={n: data[i] for i, n in enumerate(abi.IMG_NAMES)}, fluxs=np.array([fl_shell]),
+                        attrs=dict(flux=total, nshells=1.0 if data.shape[1] else 0.0, nlayers=float(nlayers),
+                                   **bounds)))
+    return out
+
+
+def write_raw_skymaps(workdir, images, times, freqs, mp_, gp):
+    """raw_skymap_<i>.h5 files (saveRawImage, src/image.h:250-300): group "shell=0" with the
+    IMG::m_names arrays, dataset "fluxs", attributes flux, nshells, nlayers, time, freq and
+    the image extent plus every numeric parfile parameter.  Image i is frequency-major:
+    i = ifreq * len(times) + it (Ejecta::computeSaveEjectaSkyImages, src/ejecta/ejecta.h:259-312)."""
+    paths = []
+    for i, im in enumerate(images):
+        path = os.path.join(workdir, f"raw_skymap_{i}.h5")
+        with h5io.Writer(path) as w:
+            if im["attrs"]["nshells"] > 0:
+                for n in abi.IMG_NAMES:
+                    w.dataset("shell=0", n, im["arrays"][n])
+            w.dataset("/", "fluxs", im["fluxs"])
+            attrs = dict(im["attrs"], time=float(times[i % len(times)]), freq=float(freqs[i // len(times)]))
+            for d in (mp_, gp):   # parfile parameters override, as in the reference's map
+                for k, v in d.items():
+                    attrs[k] = float(v)
+            for k, v in attrs.items():
+                w.attr("/", k, v)
+        paths.append(path)
+    return paths

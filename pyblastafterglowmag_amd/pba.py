@@ -3,8 +3,8 @@
 Mirrors the reference CLI ``pba.out <workdir> <parfile> <loglevel>`` (src/main.cpp:208-384) for the
 GRB afterglow part of a parfile: it reads the same parfile and initial-data file, runs the three
 stages on the GPU (dynamics, electrons + emissivity, EATS light curves) and writes the files the
-reference writes for ``do_lc`` / ``save_dynamics`` / ``save_spec`` (``fname_light_curve``,
-``fname_dyn``, ``fname_spectrum``) in the same layout (see ``output``).  Errors end the process
+reference writes for ``do_lc`` / ``save_dynamics`` / ``save_spec`` / ``do_skymap`` (``fname_light_curve``,
+``fname_dyn``, ``fname_spectrum``, ``raw_skymap_<i>.h5``) in the same layout (see ``output``).  Errors end the process
 with status 1 after a message, as the reference does.
 """
 import argparse
@@ -62,6 +62,24 @@ def main(argv=None):
         output.write_dynamics(path, run, pl.d_table.cpu().numpy(), [r.fsrs for r in pl.res], mp_, gp)
         if verbose:
             print(f"[pba] dynamics -> {path}", flush=True)
+    if go.get("do_skymap", "no") == "yes":
+        if go.get("save_raw_skymap", "no") != "yes":
+            print("[pba] do_skymap=yes needs save_raw_skymap=yes (only raw images are saved)", file=sys.stderr)
+            return 1
+        if run.method != "piece-wise":
+            print("[pba] sky maps are implemented for method_eats=piece-wise only", file=sys.stderr)
+            return 1
+        from . import inputs
+        times = inputs.vec_from_string(mo["skymap_times"])
+        freqs = inputs.vec_from_string(mo["skymap_freqs"])
+        image, flux, sst = pl.skymaps(times, freqs, go.get("skymap_remove_mu", "no") == "yes")
+        if sst.any():
+            print(f"[pba] sky map failed: status {sst.tolist()}", file=sys.stderr)
+            return 1
+        paths = output.write_raw_skymaps(workdir, output.skymap_images(image, flux, run.n_layers), times, freqs,
+                                         mp_, gp)
+        if verbose:
+            print(f"[pba] {len(paths)} raw sky maps -> {workdir}raw_skymap_*.h5", flush=True)
     if save_spec:
         path = workdir + go.get("fname_spectrum", "spec.h5")
         npop = len(pl.pops)

@@ -33,9 +33,16 @@ def build_case(name):
     return workloads.WORKLOADS[name]()
 
 
+SKY_ATTRS = ["flux", "nshells", "nlayers", "time", "freq", "xmin", "ymin", "xmax", "ymax"]
+
+
 def run_reference(name, wd, spec_every=1):
     main, grb, struct = build_case(name)
     grb = dict(grb, save_dynamics="yes", save_spec="yes")
+    sky = grb.get("method_eats") == "piece-wise"
+    if sky:   # raw sky maps at 2 frequencies x 2 observer times
+        main = dict(main, skymap_times="array 1e5 1e7", skymap_freqs="array 1e9 1e18")
+        grb = dict(grb, do_skymap="yes", save_raw_skymap="yes")
     inputs.write_run(wd, main, grb, struct)
     subprocess.check_call([PBA, wd + "/", "parfile.par", "err"], stdout=subprocess.DEVNULL)
     P, mp_, mo, gp, go = abi.run_params_from_parfile(os.path.join(wd, "parfile.par"))
@@ -92,6 +99,18 @@ def run_reference(name, wd, spec_every=1):
             out[f"jall_{tag}"] = np.stack([J[i] for i in ok])
             out[f"fall_{tag}"] = np.stack(F)
     out["spec_every"] = np.array(spec_every)
+    if sky:
+        n = 0
+        while os.path.exists(os.path.join(wd, f"raw_skymap_{n}.h5")):
+            f = os.path.join(wd, f"raw_skymap_{n}.h5")
+            for v in h5io.members(f, "shell=0"):
+                out[f"sky{n}_{v}"] = h5io.read(f, f"shell=0/{v}")
+            out[f"sky{n}_fluxs"] = h5io.read(f, "fluxs")
+            out[f"sky{n}_attrs"] = np.array([h5io.attr(f, k) for k in SKY_ATTRS])
+            n += 1
+        out["sky_n"] = np.array(n)
+        out["sky_root_members"] = np.array(h5io.members(os.path.join(wd, "raw_skymap_0.h5"), "/"))
+        out["sky_root_attrs"] = np.array(h5io.attr_names(os.path.join(wd, "raw_skymap_0.h5"), "/"))
     out["spec_root_members"] = np.array(h5io.members(spec, "/"))
     out["spec_layer_members"] = np.array(h5io.members(spec, "shell=0 layer=0"))
     return out

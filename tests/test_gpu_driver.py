@@ -43,3 +43,28 @@ def test_driver_outputs_match_reference(name, tmp_path):
     nt, nf = int(main.get("ntb", 1000)), P.fs.nfreq
     j = h5io.read(spec, "shell=0 layer=0/total_j_fs").reshape(nt, nf)[::every]
     assert rel_err(j, g["jall_fs"][0], 1e-12) <= 1e-6
+
+
+def test_driver_raw_skymaps_match_reference(tmp_path):
+    """do_skymap = yes with save_raw_skymap = yes on the piece-wise top-hat jet: the driver writes
+    raw_skymap_<i>.h5 with the reference's arrays and attributes."""
+    from pyblastafterglowmag_amd import pba
+    name = "tophat_pw_small"
+    g = load_golden(name)
+    main, grb, struct = _case(name)
+    main = dict(main, skymap_times="array 1e5 1e7", skymap_freqs="array 1e9 1e18")
+    grb = dict(grb, do_skymap="yes", save_raw_skymap="yes")
+    wd = str(tmp_path) + "/"
+    inputs.write_run(wd, main, grb, struct)
+    print(f"[gpu] driver sky maps on {name}", flush=True)
+    assert pba.main([wd, "parfile.par", "err"]) == 0
+    for i in range(int(g["sky_n"])):
+        f = wd + f"raw_skymap_{i}.h5"
+        assert sorted(h5io.members(f, "/")) == sorted(str(x) for x in g["sky_root_members"])
+        assert sorted(h5io.attr_names(f, "/")) == sorted(str(x) for x in g["sky_root_attrs"])
+        for v in abi.IMG_NAMES:
+            e = rel_err(h5io.read(f, f"shell=0/{v}"), g[f"sky{i}_{v}"], 1e-300)
+            assert e <= 1e-6, (i, v, e)
+        for k, ref in zip(["flux", "nshells", "nlayers", "time", "freq", "xmin", "ymin", "xmax", "ymax"],
+                          g[f"sky{i}_attrs"]):
+            assert h5io.attr(f, k) == pytest.approx(ref, rel=1e-6, abs=0), (i, k)

@@ -115,3 +115,39 @@ def test_mixed_precision_ssc_lightcurve(name):
         worst = max(worst, e)
         assert e <= 1e-3, (name, il, e)
     print(f"[gpu] {name}: max relative light-curve difference mixed vs FP64 = {worst:.3e}", flush=True)
+
+
+@pytest.mark.parametrize("name", ["tophat_pw_small", "gauss_pw_small"])
+def test_skymaps_match_oracle_and_reference(name):
+    """Raw piece-wise sky maps (pbagpu_eats_skymap_pw): every cell's entry against the oracle on the
+    GPU's own tables (1e-10), and the normalised, cleaned images against the reference program's
+    raw_skymap_<i>.h5 (the light-curve tolerance of the fixture)."""
+    from pyblastafterglowmag_amd import gpu, output
+    g = load_golden(name)
+    run = golden_run(g)
+    print(f"[gpu] {name}: raw sky maps", flush=True)
+    pl = gpu.Pipeline([run])
+    pl.dynamics()
+    pl.electrons()
+    pl.observer()
+    times, freqs = np.array([1e5, 1e7]), np.array([1e9, 1e18])
+    image, flux, st = pl.skymaps(times, freqs)
+    assert not st.any(), st
+    table = pl.d_table.cpu().numpy()
+    jall = pl.d_jall[:len(pl.pops)].cpu().numpy()
+    lays = (abi.EatsPwLayer * len(pl.layers))(*[pl.larr[i] for i in range(len(pl.layers))])
+    image_o, flux_o, st_o = pyoracle.eats_skymap_pw(run.params, lays, table, jall, np.tile(times, 2),
+                                                    np.repeat(freqs, 2))
+    assert not st_o.any()
+    for k in range(image.shape[0]):
+        for v in range(abi.PBA_IMG_NV):
+            e = rel_err(image[k, v], image_o[k, v], 1e-300)
+            assert e <= 1e-10, (name, k, abi.IMG_NAMES[v], e)
+    assert rel_err(flux, flux_o, 1e-300) <= 1e-10
+    ims = output.skymap_images(image, flux, run.n_layers)
+    for i, im in enumerate(ims):
+        for v in abi.IMG_NAMES:
+            ref = g[f"sky{i}_{v}"]
+            assert im["arrays"][v].shape == ref.shape, (name, i, v)
+            e = rel_err(im["arrays"][v], ref, 1e-300)
+            assert e <= LC_TOL[name], (name, i, v, e)

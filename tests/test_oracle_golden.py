@@ -79,3 +79,35 @@ def test_oracle_full_tophat_bit_exact():
     g = load_golden("tophat_fs_syn")
     run, table, res, pops, jall, fall, lc = _oracle_all(g)
     np.testing.assert_array_equal(lc[0], g["lc"][0])
+
+
+SKY_ATTRS = ["flux", "nshells", "nlayers", "time", "freq", "xmin", "ymin", "xmax", "ymax"]
+
+
+@pytest.mark.parametrize("name,tol", [("tophat_pw_small", 0.0), ("gauss_pw_small", 1e-12)])
+def test_oracle_skymaps_match_reference(name, tol):
+    """Raw piece-wise sky maps: the oracle's per-cell arrays, normalised and cleaned by the host
+    code the GPU path uses (output.skymap_images), against the reference's raw_skymap_<i>.h5."""
+    from pyblastafterglowmag_amd import inputs, output
+    g = load_golden(name)
+    run, table, res, pops, jall, fall, lc = _oracle_all(g)
+    _, lays = _populations(run, res)
+    times, freqs = inputs.vec_from_string("array 1e5 1e7"), inputs.vec_from_string("array 1e9 1e18")
+    image, flux, st = pyoracle.eats_skymap_pw(run.params, lays, table, jall, np.tile(times, len(freqs)),
+                                              np.repeat(freqs, len(times)))
+    assert not st.any()
+    # each layer's flux is its light-curve value at that (t, nu)
+    lc_img, _ = pyoracle.eats_lightcurve_pw(run.params, lays, table, jall, np.tile(times, len(freqs)),
+                                            np.repeat(freqs, len(times)))
+    np.testing.assert_array_equal(flux, lc_img)
+    ims = output.skymap_images(image, flux, run.n_layers)
+    assert len(ims) == int(g["sky_n"]) == 4
+    for i, im in enumerate(ims):
+        for v in abi.IMG_NAMES:
+            ref = g[f"sky{i}_{v}"]
+            assert im["arrays"][v].shape == ref.shape, (name, i, v)
+            assert rel_err(im["arrays"][v], ref, 1e-300) <= tol, (name, i, v)
+        assert rel_err(im["fluxs"], g[f"sky{i}_fluxs"]) <= tol
+        attrs = dict(im["attrs"], time=times[i % 2], freq=freqs[i // 2])
+        for k, ref in zip(SKY_ATTRS, g[f"sky{i}_attrs"]):
+            assert attrs[k] == pytest.approx(ref, rel=tol, abs=0), (name, i, k)

@@ -79,3 +79,25 @@ def test_spectrum_file_grids(name, tmp_path):
     assert set(h5io.members(path, "/")) <= set(str(x) for x in g["spec_root_members"])
     assert set(h5io.members(path, "shell=0 layer=0")) <= set(str(x) for x in g["spec_layer_members"])
     assert {"times_freqs", "freqs", "times_gams", "gams"} <= set(h5io.members(path, "/"))
+
+
+@pytest.mark.parametrize("name", ["tophat_pw_small", "gauss_pw_small"])
+def test_raw_skymap_file_layout(name, tmp_path):
+    """raw_skymap_<i>.h5: root members, root attribute names and the shell group's datasets as the
+    reference program wrote them, from the reference's own arrays."""
+    g = load_golden(name)
+    run = golden_run(g)
+    P, mp_, mo, gp, go = _parse(g, tmp_path)
+    n = int(g["sky_n"])
+    images = [dict(arrays={v: g[f"sky{i}_{v}"] for v in abi.IMG_NAMES}, fluxs=g[f"sky{i}_fluxs"],
+                   attrs=dict(zip(["flux", "nshells", "nlayers", "xmin", "ymin", "xmax", "ymax"],
+                                  g[f"sky{i}_attrs"][[0, 1, 2, 5, 6, 7, 8]])))
+              for i in range(n)]
+    paths = output.write_raw_skymaps(str(tmp_path), images, [1e5, 1e7], [1e9, 1e18], mp_, gp)
+    assert [os.path.basename(p) for p in paths] == [f"raw_skymap_{i}.h5" for i in range(n)]
+    for i, path in enumerate(paths):
+        assert sorted(h5io.members(path, "/")) == sorted(str(x) for x in g["sky_root_members"])
+        assert sorted(h5io.attr_names(path, "/")) == sorted(str(x) for x in g["sky_root_attrs"])
+        assert sorted(h5io.members(path, "shell=0")) == sorted(abi.IMG_NAMES)
+        assert h5io.attr(path, "time") == [1e5, 1e7][i % 2] and h5io.attr(path, "freq") == [1e9, 1e18][i // 2]
+        np.testing.assert_array_equal(h5io.read(path, "shell=0/intensity"), g[f"sky{i}_intensity"])
