@@ -33,6 +33,7 @@ PEAK_HBM_GBS = 8000.0
 # algorithmic work per unit (counted from the source; see DESIGN.md for the derivations)
 FLOPS_PER_RHS = {0: 250.0, 1: 330.0}   # one right-hand side of one blastwave (forward shock / with reverse shock)
 BYTES_PER_SAMPLE = 144.0               # one EATS integrand sample: 7 state columns at 2 steps + 4 emissivities
+FLOPS_PER_CELL = 152.0                 # one piece-wise cell flux, forward-shock term (reverse-shock term not counted)
 
 
 def parse():
@@ -42,7 +43,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=48, help="independent runs per step")
     ap.add_argument("--precision", type=int, default=0, help="0: FP64 everywhere")
-    ap.add_argument("--workload", default="gauss21_fsrs_ssc", choices=["gauss21_fsrs_ssc", "tophat_fs_syn"])
+    ap.add_argument("--workload", default="gauss21_fsrs_ssc", choices=["gauss21_fsrs_ssc", "tophat_fs_syn", "tophat64_pw"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget of the electron and observer parts of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -85,17 +86,21 @@ def cpu_baseline(workload, budget_s):
         print(f"[cpu] electrons {k_pop}/{n_pop} populations {t_ele:.1f}s", file=sys.stderr, flush=True)
     for q in range(k_pop, n_pop):       # unmeasured populations reuse a measured emissivity table
         jall[q] = jall[q % max(k_pop, 1)]
-    larr = (abi.EatsLayer * len(lays))()
+    pw = run.method == "piece-wise"
+    larr = (abi.EatsPwLayer if pw else abi.EatsLayer) * len(lays)
+    larr = larr()
     for i, (bw, r, pfs, prs, iend) in enumerate(lays):
         li = run.inits[bw]
-        larr[i] = abi.EatsLayer(li.theta_c_l, li.theta_c_h, bw, r, pfs, prs, iend)
+        larr[i] = (abi.EatsPwLayer(li.ctheta0, li.ncells, abi.cells_in_layer(li.ilayer), bw, r, pfs, 1 if prs >= 0 else 0,
+                                   iend) if pw else abi.EatsLayer(li.theta_c_l, li.theta_c_h, bw, r, pfs, prs, iend))
+    eats = pyoracle.eats_lightcurve_pw if pw else pyoracle.eats_lightcurve
     n_pair = len(run.tobs)
     k_pair, t_eats = 0, 0.0
     stride = 256
     while t_eats < budget_s / 4 and k_pair < n_pair and stride >= 1:
         sel = np.arange(stride // 2, n_pair, stride)
         t0 = time.perf_counter()
-        pyoracle.eats_lightcurve(run.params, larr, table, jall, run.tobs[sel], run.nuobs[sel])
+        eats(run.params, larr, table, jall, run.tobs[sel], run.nuobs[sel])
         t_eats = time.perf_counter() - t0
         k_pair = len(sel)
         stride //= 2
@@ -113,7 +118,7 @@ def roofline(stage_ms, pls, gpu):
     import numpy as np
     from pyblastafterglowmag_amd import abi
     names = {gpu.STAGE_DYN: "dyn_kernel", gpu.STAGE_ELE: "ele (per-step kernels)", gpu.STAGE_EATS: "eats_kernel",
-             gpu.STAGE_SSC_TABLE: "ssc_table"}
+             gpu.STAGE_SSC_TABLE: "ssc_table", gpu.STAGE_EATS_PW: "eats_pw_kernel"}
     dom = max(stage_ms, key=lambda s: stage_ms[s][0])
     ms, launches = stage_ms[dom]
     avg_s = ms / max(launches, 1) / 1e3
@@ -124,6 +129,13 @@ def roofline(stage_ms, pls, gpu):
         ach = samples * BYTES_PER_SAMPLE / avg_s / 1e9
         roof.update(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
                     samples_per_launch=samples, bytes_per_sample=BYTES_PER_SAMPLE)
+    elif dom == gpu.STAGE_EATS_PW:
+        pl = pls[0]
+        cells = gpu.timing_units(gpu.STAGE_EATS_PW) / max(launches, 1)
+        fl = cells * FLOPS_PER_CELL
+        ach = fl / avg_s / 1e12
+        roof.update(bound="mfma", achieved=ach, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP64_TFLOPS,
+                    dtype_peak="fp64", cells_per_launch=cells, flops_per_cell=FLOPS_PER_CELL)
     elif dom == gpu.STAGE_DYN:
         pl = pls[0]
         flops = 0.0
@@ -202,7 +214,7 @@ def main():
     for i in range(args.warmup):
         iteration(i)
     torch.cuda.synchronize()
-    for st in range(4):
+    for st in range(gpu.N_STAGES):
         gpu.timing_read(st, reset=True)
         gpu.timing_units(st, reset=True)
     t0 = time.perf_counter()
@@ -210,12 +222,13 @@ def main():
         iteration(i)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    stage_ms = {s: gpu.timing_read(s) for s in range(4)}
+    stage_ms = {s: gpu.timing_read(s) for s in range(gpu.N_STAGES)}
     if args.stages:
         for s, (ms, n) in stage_ms.items():
             print(f"[stage] {s}: {ms:.1f} ms over {n} launches", file=sys.stderr)
         print(f"[stage] wall per step {dt / args.steps * 1e3:.1f} ms; eats samples "
-              f"{gpu.timing_units(gpu.STAGE_EATS):.3e}", file=sys.stderr)
+              f"{gpu.timing_units(gpu.STAGE_EATS):.3e}; piece-wise cells {gpu.timing_units(gpu.STAGE_EATS_PW):.3e}",
+              file=sys.stderr)
     roof = roofline(stage_ms, pls, gpu)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload, args.cpu_seconds)
     lcs = args.batch * args.steps / dt

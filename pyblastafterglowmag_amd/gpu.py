@@ -18,6 +18,7 @@ EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu
            "pbagpu_timing_read", "pbagpu_timing_units", "pbagpu_dyn_evolve", "pbagpu_ele_evolve", "pbagpu_eats_lightcurve",
            "pbagpu_eats_lightcurve_pw", "pbagpu_eats_skymap_pw")
 STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW = 0, 1, 2, 3, 4
+N_STAGES = 5
 
 
 def lib():

@@ -4,6 +4,7 @@
                            200 gamma bins, 96 comoving freqs, 1000 steps, 96 nu x 128 t_obs.
 * ``gauss21_fsrs_ssc``   : 21-layer Gaussian jet, forward + reverse shock, synchrotron + SSC,
                            theta_obs = 0.363 rad, same grids.
+* ``tophat64_pw``        : the top-hat jet on a 64-layer piece-wise grid (4096 cells).
 Small variants (fewer steps / bins / pairs) are derived with ``scaled``.
 """
 import copy
@@ -35,7 +36,20 @@ def gauss21_fsrs_ssc():
     return main, grb, struct
 
 
-WORKLOADS = {"tophat_fs_syn": tophat_fs_syn, "gauss21_fsrs_ssc": gauss21_fsrs_ssc}
+def tophat64_pw():
+    """The top-hat jet of tophat_fs_syn on a 64-layer piece-wise grid (2i+1 phi cells per layer,
+    4096 cells), same physics and grids: the piece-wise workload with the cell count of C3.
+
+    The 21-layer Gaussian physics on a 64-layer piece-wise grid is not a workload the reference
+    completes: its cell fluxes include negative intensities (bilinear extrapolation of the
+    emissivity between steps) and the reference exits (microphysics.h:835-841)."""
+    main, grb, struct = tophat_fs_syn()
+    grb = dict(grb, method_eats="piece-wise")
+    struct = dict(struct, n_layers_pw=64)
+    return main, grb, struct
+
+
+WORKLOADS = {"tophat_fs_syn": tophat_fs_syn, "gauss21_fsrs_ssc": gauss21_fsrs_ssc, "tophat64_pw": tophat64_pw}
 
 
 def scaled(name, ntb=None, ngam=None, nfreq=None, n_nu_obs=None, n_t_obs=None, n_layers=None, **over):

@@ -36,10 +36,15 @@ def build_case(name):
 SKY_ATTRS = ["flux", "nshells", "nlayers", "time", "freq", "xmin", "ymin", "xmax", "ymax"]
 
 
+# full-size cases: light curves of every `stride`-th layer, dynamics of layer 0, no spectra or sky maps
+LARGE = {"tophat64_pw": 7}
+
+
 def run_reference(name, wd, spec_every=1):
     main, grb, struct = build_case(name)
-    grb = dict(grb, save_dynamics="yes", save_spec="yes")
-    sky = grb.get("method_eats") == "piece-wise"
+    large = name in LARGE
+    grb = dict(grb, save_dynamics="yes", save_spec="no" if large else "yes")
+    sky = grb.get("method_eats") == "piece-wise" and not large
     if sky:   # raw sky maps at 2 frequencies x 2 observer times
         main = dict(main, skymap_times="array 1e5 1e7", skymap_freqs="array 1e9 1e18")
         grb = dict(grb, do_skymap="yes", save_raw_skymap="yes")
@@ -58,18 +63,21 @@ def run_reference(name, wd, spec_every=1):
     lc = os.path.join(wd, "lc.h5")
     out["times"] = h5io.read(lc, "times")
     out["freqs"] = h5io.read(lc, "freqs")
-    out["lc"] = np.stack([h5io.read(lc, f"shell=0 layer={il}/fluxdens") for il in range(nl)])
+    lc_layers = list(range(0, nl, LARGE[name])) if large else list(range(nl))
+    out["lc_layers"] = np.array(lc_layers)
+    out["lc"] = np.stack([h5io.read(lc, f"shell=0 layer={il}/fluxdens") for il in lc_layers])
     # file layout of the reference's outputs (names only) and the per-layer light-curve attributes
     out["lc_root_members"] = np.array(h5io.members(lc, "/"))
     out["lc_root_attrs"] = np.array(h5io.attr_names(lc, "/"))
     names_la = h5io.attr_names(lc, "shell=0 layer=0")
     out["lc_layer_attr_names"] = np.array(names_la)
-    out["lc_layer_attrs"] = np.array([[h5io.attr(lc, k, f"shell=0 layer={il}") for k in names_la] for il in range(nl)])
+    out["lc_layer_attrs"] = np.array([[h5io.attr(lc, k, f"shell=0 layer={il}") for k in names_la] for il in lc_layers])
     dyn = os.path.join(wd, "dyn.h5")
     names = DYN_VARS + (DYN_VARS_RS if P.fsrs else [])
     nt = int(mp_["ntb"])
-    D = np.zeros((nl, len(names), nt))
-    for il in range(nl):
+    n_dyn = 1 if large else nl
+    D = np.zeros((n_dyn, len(names), nt))
+    for il in range(n_dyn):
         for iv, v in enumerate(names):
             ds = f"shell=0 layer={il}/{v}"
             if h5io.has(dyn, ds):
@@ -80,9 +88,12 @@ def run_reference(name, wd, spec_every=1):
     names_da = h5io.attr_names(dyn, "shell=0 layer=0")
     out["dyn_layer_attr_names"] = np.array(names_da)
     out["dyn_layer_attrs"] = np.array([[h5io.attr(dyn, k, f"shell=0 layer={il}") for k in names_da]
-                                       for il in range(nl)])
+                                       for il in range(n_dyn)])
     out["dyn_root_attrs"] = np.array(h5io.attr_names(dyn, "/"))
     out["dyn_root_counts"] = np.array([h5io.attr(dyn, k) for k in ("nshells", "nlayers", "ntimes", "ncells")])
+    if large:
+        out["spec_every"] = np.array(spec_every)
+        return out
     spec = os.path.join(wd, "spec.h5")
     for tag in ("fs", "rs"):
         J, F = [], []

@@ -111,3 +111,25 @@ def test_oracle_skymaps_match_reference(name, tol):
         attrs = dict(im["attrs"], time=times[i % 2], freq=freqs[i // 2])
         for k, ref in zip(SKY_ATTRS, g[f"sky{i}_attrs"]):
             assert attrs[k] == pytest.approx(ref, rel=tol, abs=0), (name, i, k)
+
+
+def test_oracle_full_piecewise_tophat():
+    """Full-size piece-wise run (64 layers, 4096 cells, 1000 steps, 200 gamma bins, 96 x 128 pairs):
+    the oracle's light curves of the recorded layers equal the reference's bit for bit."""
+    g = load_golden("tophat64_pw")
+    run = golden_run(g)
+    table, res = pyoracle.dyn_evolve(run.params, run.inits, run.tgrid)
+    assert all(r.status == 0 for r in res)
+    names = [str(n) for n in g["dyn_names"]]
+    for iv, n in enumerate(names):
+        np.testing.assert_array_equal(table[0, abi.PV[n]], g["dyn"][0, iv])
+    layers = [int(i) for i in g["lc_layers"]]
+    pops = (abi.Population * len(layers))(*[abi.Population(il, 0, 0, 0) for il in layers])
+    jall, _, st = pyoracle.ele_evolve(run.params, pops, table)
+    assert not st.any()
+    lays = (abi.EatsPwLayer * len(layers))(*[
+        abi.EatsPwLayer(run.inits[il].ctheta0, run.inits[il].ncells, abi.cells_in_layer(il), il, 0, k, 0,
+                        res[il].i_end_r) for k, il in enumerate(layers)])
+    lc, st2 = pyoracle.eats_lightcurve_pw(run.params, lays, table, jall, run.tobs, run.nuobs)
+    assert not st2.any()
+    np.testing.assert_array_equal(lc, g["lc"])
