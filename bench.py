@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=48, help="independent runs per step")
+    ap.add_argument("--batch", type=int, default=60, help="independent runs per step")
     ap.add_argument("--precision", type=int, default=0, help="0: FP64 everywhere")
     ap.add_argument("--workload", default="gauss21_fsrs_ssc", choices=["gauss21_fsrs_ssc", "tophat_fs_syn", "tophat64_pw"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0,

@@ -27,6 +27,8 @@
 namespace pbagpu {
 namespace eats {
 
+constexpr int WAVE = 64;
+
 constexpr int HI = 256;  // inner (theta) interval store (reference cases need <= ~160)
 constexpr int HO = 256;  // outer (phi) interval store (off-axis cases need <= ~190)
 
@@ -448,8 +450,8 @@ __device__ double shock_flux(Ctx &C, const double *jall, int nu_off, double freq
     return lum;
 }
 
-// Out of line on purpose: CADRE samples the integrand from eleven places, and one shared
-// copy of its code keeps the kernel inside the instruction cache.
+// Out of line on purpose: inlined at its one call site, the kernel needs more registers than
+// the two waves per SIMD it runs at and spills almost twice as much.
 __device__ __noinline__ double integrand(Ctx &C, double cos_theta) {
     const int N = C.i_end_r;
     if (C.V(S_R, 0) == 0.0 && C.V(S_R, N - 1) == 0.0) return 0.0;
@@ -495,38 +497,53 @@ constexpr size_t LANE_WS = ((Store<HO>::bytes() + Store<HI>::bytes()) + 255) / 2
 
 extern __shared__ double lds_tab[];   // [NSLOT][nt] table columns, then the two frequency grids
 
-// Persistent kernel: each workgroup repeatedly takes a chunk (one layer, a range of
-// consecutive (t_obs, nu_obs) pairs) from a global counter and stages that layer's table
-// columns in LDS; its lanes take pairs of the chunk one at a time from an LDS counter.
+// Persistent kernel: each workgroup stages one layer's table columns in LDS and its lanes take
+// that layer's (t_obs, nu_obs) pairs one at a time from the layer's global counter; a
+// workgroup changes layer only when the layer has no pairs left (see the loop below).
 // Per lane, the nested adaptive quadrature of one pair (evalFluxDensA -> integrate_theta_phi,
 // src/eats.h:587-684, 824-856: CADRE over phi in [0, pi], CADRE over 1 - cos(theta) between
 // the layer edges of find_jet_edge for every phi sample, primary jet then counter-jet) runs as
 // two resumable CADRE machines; every pass of the lane loop takes exactly one integrand
 // sample, at the single call site below.
 __global__ void __launch_bounds__(EATS_WG, 2) eats_kernel(const LayerArgs *__restrict__ layers, int n_layers, int n_pairs,
-                                                       int chunk, int *__restrict__ next_chunk,
+                                                       int *__restrict__ pair_next,
                                                        const double *__restrict__ tobs,
                                                        const double *__restrict__ nuobs, double *__restrict__ out,
                                                        int *__restrict__ status, char *__restrict__ ws,
                                                        unsigned long long *__restrict__ n_samples) {
-    __shared__ int s_chunk, s_next;
-    const int chunks_per_layer = (n_pairs + chunk - 1) / chunk;
-    const int n_chunks = chunks_per_layer * n_layers;
+    __shared__ int s_layer;
     char *lane_ws = ws + ((size_t)blockIdx.x * EATS_WG + threadIdx.x) * LANE_WS;
     Cadre<HO> Mo;
     Cadre<HI> Mi;
     Mo.S.bind(lane_ws);
     Mi.S.bind(lane_ws + Store<HO>::bytes());
     int cur_layer = -1;
+    // Workgroups start spread over the layers; each keeps its layer until that layer's pairs are
+    // all taken, then joins the next layer (cyclically) that still has pairs.  Lanes take pairs
+    // one at a time from the layer's counter, so they wait for each other only when their
+    // workgroup changes layer (the staged table is shared).
+    int probe = (int)(((long)blockIdx.x * n_layers) / gridDim.x);
     for (;;) {
-        __syncthreads();  // every lane is done with the previous chunk and its staged table
-        if (threadIdx.x == 0) s_chunk = atomicAdd(next_chunk, 1);
+        __syncthreads();  // every lane is done with the previous layer and its staged table
+        if (threadIdx.x < WAVE) {
+            int found = -1;
+            for (int base = 0; base < n_layers && found < 0; base += WAVE) {
+                const int t = base + threadIdx.x;
+                int lt = probe + t;
+                if (lt >= n_layers) lt -= n_layers;
+                const bool open = t < n_layers && __atomic_load_n(&pair_next[lt], __ATOMIC_RELAXED) < n_pairs;
+                const unsigned long long m = __ballot(open);
+                if (m) {
+                    int f = probe + base + __ffsll((long long)m) - 1;
+                    found = f >= n_layers ? f - n_layers : f;
+                }
+            }
+            if (threadIdx.x == 0) s_layer = found;
+        }
         __syncthreads();
-        const int c = s_chunk;
-        if (c >= n_chunks) break;
-        const int l = c / chunks_per_layer;
-        const int p0 = (c % chunks_per_layer) * chunk;
-        const int p1 = min(p0 + chunk, n_pairs);
+        const int l = s_layer;
+        if (l < 0) break;
+        probe = l;
         const LayerArgs &L = layers[l];
         const int nt = L.nt, nf = L.nf;
         if (l != cur_layer) {
@@ -540,7 +557,6 @@ __global__ void __launch_bounds__(EATS_WG, 2) eats_kernel(const LayerArgs *__res
             }
             cur_layer = l;
         }
-        if (threadIdx.x == 0) s_next = p0;
         __syncthreads();
         const double Fcoeff = kCgs2mJy / (4 * M_PI * L.d_l * L.d_l);
         const double th1c = L.theta_c_h, th0c = L.theta_c_l;
@@ -560,8 +576,8 @@ __global__ void __launch_bounds__(EATS_WG, 2) eats_kernel(const LayerArgs *__res
             // bookkeeping until this lane needs an edge probe or an integrand sample
             while (mode == NEED_PAIR || mode == NEED_INNER) {
                 if (mode == NEED_PAIR) {
-                    k = atomicAdd(&s_next, 1);
-                    if (k >= p1) { mode = DONE; break; }
+                    k = atomicAdd(&pair_next[l], 1);
+                    if (k >= n_pairs) { mode = DONE; break; }
                     C.status = 0;
                     const double t_obs = tobs[k];
                     C.tz = t_obs / (1.0 + L.z);
@@ -684,20 +700,15 @@ extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, 
         int dev = 0, n_cu = 0;
         PBA_TRY(rc, hipGetDevice(&dev));
         PBA_TRY(rc, hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-        // chunks of consecutive pairs of one layer; enough chunks to keep every workgroup busy
-        const long total = (long)n_pairs * n_layers;
-        const long per_wg = total / (4L * std::max(n_cu, 1));   // about four chunks per workgroup
-        const int chunk = (int)std::max<long>(EATS_WG, std::min<long>(std::min<long>(n_pairs, 4 * EATS_WG),
-                                                                       per_wg / EATS_WG * EATS_WG));
-        const long n_chunks = (long)((n_pairs + chunk - 1) / chunk) * n_layers;
+        const long n_items = ((long)n_pairs + EATS_WG - 1) / EATS_WG * n_layers;   // workgroup-sized pieces
         (void)hipFuncSetAttribute((const void *)eats_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         int per_cu = 1;
         PBA_TRY(rc, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, eats_kernel, EATS_WG, lds));
-        const int n_wg = (int)std::min<long>(n_chunks, (long)std::max(n_cu, 1) * std::max(per_cu, 1));
+        const int n_wg = (int)std::min<long>(n_items, (long)std::max(n_cu, 1) * std::max(per_cu, 1));
         char *ws = (char *)pbagpu::workspace(pbagpu::WS_EATS_STORE, (size_t)n_wg * EATS_WG * LANE_WS);
-        int *counter = (int *)pbagpu::workspace(pbagpu::WS_EATS_COUNTER, sizeof(int));
+        int *counter = (int *)pbagpu::workspace(pbagpu::WS_EATS_COUNTER, sizeof(int) * n_layers);
         if (!ws || !counter) rc = PBAGPU_E_HIP;
-        if (rc == PBAGPU_OK) PBA_TRY(rc, hipMemsetAsync(counter, 0, sizeof(int), s));
+        if (rc == PBAGPU_OK) PBA_TRY(rc, hipMemsetAsync(counter, 0, sizeof(int) * n_layers, s));
         if (rc == PBAGPU_OK) {
             (void)hipFuncSetAttribute((const void *)eats_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             unsigned long long *d_samples = nullptr;
@@ -706,8 +717,8 @@ extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, 
                 if (d_samples) PBA_TRY(rc, hipMemsetAsync(d_samples, 0, sizeof(unsigned long long), s));
             }
             timer.start(s);
-            hipLaunchKernelGGL(eats_kernel, dim3((unsigned)n_wg), dim3(EATS_WG), lds, s, dL, n_layers, n_pairs, chunk,
-                               counter, d_tobs, d_nuobs, d_out, d_status, ws, d_samples);
+            hipLaunchKernelGGL(eats_kernel, dim3((unsigned)n_wg), dim3(EATS_WG), lds, s, dL, n_layers, n_pairs, counter,
+                               d_tobs, d_nuobs, d_out, d_status, ws, d_samples);
             PBA_TRY(rc, hipGetLastError());
             timer.stop(s);
             if (d_samples && rc == PBAGPU_OK) {
