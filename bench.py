@@ -43,7 +43,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=60, help="independent runs per step")
     ap.add_argument("--precision", type=int, default=0, help="0: FP64 everywhere")
-    ap.add_argument("--workload", default="gauss21_fsrs_ssc", choices=["gauss21_fsrs_ssc", "tophat_fs_syn", "tophat64_pw"])
+    ap.add_argument("--workload", default="gauss21_fsrs_ssc",
+                    choices=["gauss21_fsrs_ssc", "tophat_fs_syn", "tophat64_pw", "lhs_members"],
+                    help="lhs_members: every run a different member of the 1024-member Latin-hypercube batch")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget of the electron and observer parts of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -51,6 +53,18 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device (reference kernel for the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def _workload(workloads, name, member):
+    return workloads.lhs_member(member) if name == "lhs_members" else getattr(workloads, name)()
+
+
+def rejected_runs(pl, dst, est, lst):
+    """Runs of a batch with any non-zero status bit (dynamics, electrons or observer stage)."""
+    bad = {i // pl.bpr for i in range(len(dst)) if dst[i]}
+    bad |= {pl.pops[q].run for q in range(len(est)) if est[q]}
+    bad |= {pl.layers[k][1] for k in range(len(lst)) if lst[k]}
+    return bad
 
 
 def cpu_baseline(workload, budget_s):
@@ -65,7 +79,7 @@ def cpu_baseline(workload, budget_s):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from pyblastafterglowmag_amd import abi, gpu, workloads
-    run = gpu.Run.from_workload(*getattr(workloads, workload)())
+    run = gpu.Run.from_workload(*_workload(workloads, workload, 0))
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
     t0 = time.perf_counter()
     table, res = pyoracle.dyn_evolve(run.params, run.inits, run.tgrid)
@@ -186,16 +200,22 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    run = gpu.Run.from_workload(*getattr(workloads, args.workload)())
-    runs = [run] * args.batch
+    if args.workload == "lhs_members":   # batch b holds members b*batch ... (b+1)*batch - 1
+        batches = [[gpu.Run.from_workload(*_workload(workloads, args.workload, b * args.batch + i))
+                    for i in range(args.batch)] for b in range(2)]
+    else:
+        batches = [[gpu.Run.from_workload(*_workload(workloads, args.workload, 0))] * args.batch] * 2
+    run = batches[0][0]
     streams = [torch.cuda.Stream(), torch.cuda.Stream()]
-    pls = [gpu.Pipeline(runs, stream=streams[0], precision=args.precision),
-           gpu.Pipeline(runs, stream=streams[1], precision=args.precision)]
+    pls = [gpu.Pipeline(batches[0], stream=streams[0], precision=args.precision),
+           gpu.Pipeline(batches[1], stream=streams[1], precision=args.precision)]
     gpu.timing_enable(True)
     t_fill = time.perf_counter()
     pls[0].dynamics()          # fill the pipeline: dynamics of the first batch
     print(f"[bench] pipeline filled: dynamics of batch 0 in {time.perf_counter() - t_fill:.1f}s", file=sys.stderr,
           flush=True)
+
+    rejected = {}
 
     def iteration(i):
         cur, nxt = pls[i % 2], pls[(i + 1) % 2]
@@ -207,9 +227,15 @@ def main():
         print(f"[bench] step {i} done (batch {i} light curves, batch {i + 1} dynamics)", file=sys.stderr, flush=True)
         dst, est, lst = cur.status()
         if dst.any() or est.any() or lst.any():
-            print(f"bench.py: non-zero status dyn={sorted(set(dst.tolist()))} ele={sorted(set(est.tolist()))} "
-                  f"eats={sorted(set(lst.tolist()))}", file=sys.stderr)
-            sys.exit(3)
+            bad = rejected_runs(cur, dst, est, lst)
+            if args.workload != "lhs_members":
+                print(f"bench.py: non-zero status dyn={sorted(set(dst.tolist()))} ele={sorted(set(est.tolist()))} "
+                      f"eats={sorted(set(lst.tolist()))}", file=sys.stderr)
+                sys.exit(3)
+            # a member the reference would stop on (its status bits mark where) yields no light curve
+            rejected[i] = len(bad)
+            print(f"[bench] step {i}: {len(bad)} of {cur.n_runs} members flagged (no light curve): {sorted(bad)}",
+                  file=sys.stderr, flush=True)
 
     for i in range(args.warmup):
         iteration(i)
@@ -231,7 +257,8 @@ def main():
               file=sys.stderr)
     roof = roofline(stage_ms, pls, gpu)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload, args.cpu_seconds)
-    lcs = args.batch * args.steps / dt
+    n_rej = sum(rejected.get(i, 0) for i in range(args.warmup, args.warmup + args.steps))
+    lcs = (args.batch * args.steps - n_rej) / dt
     line = {
         "metric": "light curves/sec (96nu x128t), plain FP64 vs mixed+surrogate GPU @<=1% err",
         "value": lcs, "unit": "light curves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -239,7 +266,9 @@ def main():
         "dtype": "f64" if args.precision == 0 else "f64+bf16", "data": "synthetic",
         "config": {"workload": args.workload, "runs_per_step": args.batch, "layers": run.n_layers,
                    "ngam": run.params.fs.ngam, "nfreq": run.params.fs.nfreq, "nt": len(run.tgrid),
-                   "pairs": len(run.tobs), "precision": args.precision, "pipeline": "double-buffered"},
+                   "pairs": len(run.tobs), "precision": args.precision, "pipeline": "double-buffered",
+                   **({"members_flagged": n_rej, "members": "Latin hypercube, seed 20261203, runs b*batch .. "
+                       "(b+1)*batch-1 per step"} if args.workload == "lhs_members" else {})},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -826,6 +826,7 @@ static void evolve_run(const pba_run_params *P, const pba_layer_init *init, int 
 extern "C" int orc_dyn_evolve(const pba_run_params *params, int n_runs, int bw_per_run, const pba_layer_init *init,
                               int nt, const double *tgrid, double *table, pba_bw_result *res) {
     if (!params || !init || !tgrid || !table || !res || n_runs < 1 || bw_per_run < 1 || nt < 2) return PBAGPU_E_ARG;
+#pragma omp parallel for schedule(dynamic)   // runs are independent
     for (int r = 0; r < n_runs; r++)
         orc::evolve_run(&params[r], init + (size_t)r * bw_per_run, bw_per_run, nt, tgrid,
                         table + (size_t)r * bw_per_run * PBA_NV * nt, res + (size_t)r * bw_per_run);

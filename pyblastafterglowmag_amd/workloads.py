@@ -49,6 +49,41 @@ def tophat64_pw():
     return main, grb, struct
 
 
+# Parameter ranges of the batched-member workload: a Latin hypercube over
+# log10 E_iso,c, Gamma0,c, theta_c, log10 n_ism, log10 eps_e, log10 eps_b, p, theta_obs.
+LHS_RANGES = (("log10_eiso", 50.0, 54.0), ("gamma0c", 10.0, 1000.0), ("theta_c", 0.02, 0.2),
+              ("log10_n_ism", -4.0, 1.0), ("log10_eps_e", -3.0, -0.5), ("log10_eps_b", -5.0, -1.0),
+              ("p", 2.05, 2.8), ("theta_obs", 0.0, 0.8))
+LHS_SEED = 20261203
+
+
+def latin_hypercube(n, d, seed=LHS_SEED):
+    """n points in [0, 1)^d with exactly one point in each of the n equal strata of every
+    dimension (a random permutation of the strata per dimension, a uniform offset inside each)."""
+    rng = np.random.default_rng(seed)
+    strata = np.stack([rng.permutation(n) for _ in range(d)], axis=1)
+    return (strata + rng.random((n, d))) / n
+
+
+def lhs_member(i, n=1024, seed=LHS_SEED, theta_w_over_theta_c=4.0):
+    """Member i of the n-member Latin-hypercube batch: the 21-layer Gaussian jet of
+    gauss21_fsrs_ssc (forward + reverse shock, synchrotron + SSC, same grids) with the eight
+    parameters of LHS_RANGES drawn from the hypercube.  The jet's wing is theta_w =
+    theta_w_over_theta_c x theta_c (capped at the 0.436 rad of gauss21_fsrs_ssc), so that every
+    member keeps its 21 layers (the initial data drop layers with beta0 <= 1e-5)."""
+    u = latin_hypercube(n, len(LHS_RANGES), seed)[i]
+    v = {name: float(lo + (hi - lo) * x) for (name, lo, hi), x in zip(LHS_RANGES, u)}
+    main, grb, struct = gauss21_fsrs_ssc()
+    main = dict(main, n_ism=10 ** v["log10_n_ism"], theta_obs=v["theta_obs"])
+    micro = {}
+    for sh in ("fs", "rs"):
+        micro.update({f"eps_e_{sh}": 10 ** v["log10_eps_e"], f"eps_b_{sh}": 10 ** v["log10_eps_b"], f"p_{sh}": v["p"]})
+    grb = dict(grb, **micro)
+    struct = dict(struct, eiso_c=10 ** v["log10_eiso"], gamma0c=v["gamma0c"], theta_c=v["theta_c"],
+                  theta_w=min(0.436, theta_w_over_theta_c * v["theta_c"]))
+    return main, grb, struct
+
+
 WORKLOADS = {"tophat_fs_syn": tophat_fs_syn, "gauss21_fsrs_ssc": gauss21_fsrs_ssc, "tophat64_pw": tophat64_pw}
 
 

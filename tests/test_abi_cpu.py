@@ -166,3 +166,26 @@ def test_layer_edges_piecewise():
     # cos(2 asin a) = 1 - 2 a^2: layer i spans (2i+1) equal solid-angle cells of the piece-wise grid
     d = np.array([np.cos(L[i].theta_c_l) - np.cos(L[i].theta_c_h) for i in range(n)])
     assert np.allclose(d / (2 * np.arange(n) + 1), d[0], rtol=1e-9)
+
+
+def test_latin_hypercube_members():
+    """Batched-member workload: one point per stratum in every dimension, parameters inside their
+    ranges, a fixed seed gives the same members, and every member keeps the 21 layers and reads
+    back through the reference's parfile format."""
+    import tempfile
+    n = 64
+    u = workloads.latin_hypercube(n, len(workloads.LHS_RANGES))
+    assert u.shape == (n, 8) and u.min() >= 0 and u.max() < 1
+    for j in range(8):
+        assert sorted(np.floor(u[:, j] * n).astype(int)) == list(range(n))
+    np.testing.assert_array_equal(u, workloads.latin_hypercube(n, len(workloads.LHS_RANGES)))
+    assert not np.array_equal(u, workloads.latin_hypercube(n, len(workloads.LHS_RANGES), seed=1))
+    for i in (0, 17, 63):
+        main, grb, struct = workloads.lhs_member(i, n=n)
+        assert 1e50 <= struct["eiso_c"] < 1e54 and 10 <= struct["gamma0c"] < 1000 and 0.02 <= struct["theta_c"] < 0.2
+        assert 1e-4 <= main["n_ism"] < 10 and 0 <= main["theta_obs"] < 0.8 and 2.05 <= grb["p_fs"] < 2.8
+        assert 1e-3 <= grb["eps_e_fs"] < 10 ** -0.5 and 1e-5 <= grb["eps_b_rs"] < 0.1
+        assert len(inputs.jet_structure(kind="adaptive", **struct)["ek"]) == 21
+        with tempfile.TemporaryDirectory() as d:
+            P = abi.run_params_from_parfile(inputs.write_parfile(d, main, grb))[0]
+        assert P.n_ism == pytest.approx(main["n_ism"], rel=1e-12) and P.fs.p == pytest.approx(grb["p_fs"], rel=1e-12)

@@ -154,3 +154,50 @@ def test_skymaps_match_oracle_and_reference(name):
             assert im["arrays"][v].shape == ref.shape, (name, i, v)
             e = rel_err(im["arrays"][v], ref, 1e-300)
             assert e <= LC_TOL[name], (name, i, v, e)
+
+
+def _lhs_small(i):
+    from pyblastafterglowmag_amd import gpu, workloads
+    main, grb, struct = workloads.lhs_member(i)
+    main = dict(main, lc_freqs="array logspace 1e8 1e29 6", lc_times="array logspace 3e3 1e10 8")
+    for sh in ("fs", "rs"):
+        grb[f"ngam_{sh}"] = 64
+        grb[f"nfreq_{sh}"] = 48
+    return gpu.Run.from_workload(main, grb, struct)
+
+
+def test_heterogeneous_batch_matches_oracle():
+    """Two members of the Latin-hypercube workload (different E_iso, Gamma0, theta_c, n_ism, eps_e,
+    eps_b, p and theta_obs) in one batch, with the microphysics grids reduced: every stage
+    against the oracle run by run (dynamics 1e-4, emissivities 1e-9 and light curves 1e-10 on the
+    GPU's own tables), so that per-run parameters reach every kernel correctly."""
+    from pyblastafterglowmag_amd import gpu
+    runs = [_lhs_small(0), _lhs_small(1)]
+    nl = runs[0].n_layers
+    print(f"[gpu] heterogeneous batch: 2 runs x {nl} layers", flush=True)
+    flux, ex = gpu.lightcurves(runs, want_intermediates=True)
+    assert not ex["dyn_status"].any() and not ex["ele_status"].any() and not ex["eats_status"].any()
+    params = (abi.RunParams * 2)(*[r.params for r in runs])
+    inits = (abi.LayerInit * (2 * nl))(*[li for r in runs for li in r.inits])
+    table_o, res_o = pyoracle.dyn_evolve(params, inits, runs[0].tgrid, n_runs=2)
+    for bw in range(2 * nl):
+        assert ex["results"][bw][1] == res_o[bw].i_end_r and ex["results"][bw][2] == res_o[bw].fsrs, bw
+        for n in DYN_CHECK:
+            e = rel_err(ex["table"][bw, abi.PV[n]], table_o[bw, abi.PV[n]], 1e-300)
+            assert e <= 1e-4, (bw, n, e)
+    pops = (abi.Population * len(ex["pops"]))(*ex["pops"])
+    jall_o, _, st = pyoracle.ele_evolve(params, pops, ex["table"], n_runs=2)
+    assert not st.any()
+    for q in range(len(ex["pops"])):
+        e = rel_err(ex["jall"][q], jall_o[q], 1e-9)
+        assert e <= 1e-9, (q, e)
+    lays = (abi.EatsLayer * len(ex["layers"]))()
+    for i, (bw, r, pfs, prs, iend) in enumerate(ex["layers"]):
+        li = runs[r].inits[bw - r * nl]
+        lays[i] = abi.EatsLayer(li.theta_c_l, li.theta_c_h, bw, r, pfs, prs, iend)
+    lc_o, st2 = pyoracle.eats_lightcurve(params, lays, ex["table"], ex["jall"], runs[0].tobs, runs[0].nuobs, n_runs=2)
+    assert not st2.any()
+    got = flux.reshape(2 * nl, -1)
+    for i in range(2 * nl):
+        e = rel_err(got[i], lc_o[i], 1e-30)
+        assert e <= 1e-10, (i, e)
