@@ -73,7 +73,7 @@ typedef struct {
     int max_substeps;
     int ssc;              /* method_ssc = numeric */
     int adi_loss_ele;     /* num_ele_use_adi_loss */
-    int pad_;
+    int ssa;              /* use_ssa: synchrotron self-absorption (computeSSA, numeric_model.h:190-238) */
 } pba_shock_params;
 
 /* One afterglow model (all its blastwaves share these); parfile main + GRB sections. */
@@ -193,6 +193,9 @@ typedef struct {
  *   d_table  dynamics table from stage 1
  *   d_jall   [n_pop][nt][nfreq] device, written: comoving emissivity per electron
  *            (total_rad.j_all, base_state.h:257-310)
+ *   d_aall   [n_pop][nt][nfreq] device, written: comoving absorption coefficient
+ *            (total_rad.a_all; 0 where the shock has ssa = 0); may be NULL when no
+ *            population's shock has ssa = 1
  *   d_fall   [n_pop][nt][ngam]  device or NULL: electron spectrum per step (ele.f_all)
  *   d_status [n_pop] device int, written
  * The grid sizes ngam/nfreq come from the shock parameters and must be equal
@@ -201,7 +204,7 @@ typedef struct {
 int pbagpu_ele_evolve(const pba_run_params *params, int n_runs,
                       const pba_population *pops, int n_pop,
                       const double *d_table, int bw_total, int nt,
-                      double *d_jall, double *d_fall, int *d_status,
+                      double *d_jall, double *d_aall, double *d_fall, int *d_status,
                       int precision, void *stream);
 
 /* ------------------------------------------------------------- stage 3 */
@@ -209,6 +212,8 @@ int pbagpu_ele_evolve(const pba_run_params *params, int n_runs,
  *   layers     [n_layers] host: blastwave index in the dynamics table and the
  *              population indices of its forward / reverse shock emissivity (-1: none)
  *   d_out      [n_layers][n_pairs] device, written, mJy
+ *   d_aall     absorption table from stage 2, or NULL (no absorption: the optical
+ *              depth is 0, as with use_ssa = no)
  */
 typedef struct {
     double theta_c_l, theta_c_h; /* angular extent of the layer (EjectaID2 adaptive grid) */
@@ -221,7 +226,7 @@ typedef struct {
 
 int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs,
                            const pba_eats_layer *layers, int n_layers,
-                           const double *d_table, int nt, const double *d_jall, int nfreq,
+                           const double *d_table, int nt, const double *d_jall, const double *d_aall, int nfreq,
                            int n_pairs, const double *d_tobs, const double *d_nuobs,
                            double *d_out, int *d_status, void *stream);
 
@@ -250,7 +255,8 @@ typedef struct {
 
 int pbagpu_eats_lightcurve_pw(const pba_run_params *params, int n_runs,
                               const pba_eats_pw_layer *layers, int n_layers,
-                              const double *d_table, int nt, const double *d_jall, int nfreq,
+                              const double *d_table, int nt, const double *d_jall, const double *d_aall,
+                              int nfreq,
                               int n_pairs, const double *d_tobs, const double *d_nuobs,
                               double *d_out, int *d_status, void *stream);
 
@@ -271,7 +277,7 @@ enum pba_img_var {
  *            (the value evalSkyMapPW returns)
  *   remove_mu  skymap_remove_mu: 0 divides the intensity by |mu|, as the default */
 int pbagpu_eats_skymap_pw(const pba_run_params *params, const pba_eats_pw_layer *layers, int n_layers,
-                          const double *d_table, int nt, const double *d_jall, int nfreq,
+                          const double *d_table, int nt, const double *d_jall, const double *d_aall, int nfreq,
                           int n_images, const double *d_tobs, const double *d_nuobs, int remove_mu,
                           double *d_image, double *d_flux, int *d_status, void *stream);
 

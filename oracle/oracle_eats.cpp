@@ -229,6 +229,7 @@ struct LayerCtx {
     const pba_run_params *P;
     const double *T; int nt; int i_end_r;
     const double *jfs, *jrs;           // [nt][nf]
+    const double *afs = nullptr, *ars = nullptr;   // absorption (use_ssa) or null
     std::vector<double> nu_fs, nu_rs;
     bool rs;
     double theta_c_l, theta_c_h;
@@ -271,9 +272,9 @@ static inline double intensity_smooth(double em, double dtau) {
 }
 
 // ElectronAndRadiation::fluxDens + computeFluxDensity (useNe, sameAsBW, smooth, no SSA)
-static double flux_dens_shock(LayerCtx &L, const double *jall, const std::vector<double> &nu, double freq,
-                              size_t ia, size_t ib, double Gamma, double mu, double r, double ne, double n_prime,
-                              double theta, double ncells = 1.) {
+static double flux_dens_shock(LayerCtx &L, const double *jall, const double *aall, const std::vector<double> &nu,
+                              double freq, size_t ia, size_t ib, double Gamma, double mu, double r, double ne,
+                              double n_prime, double theta, double ncells = 1.) {
     int nf = (int)nu.size();
     if (freq >= nu[nf - 1] || freq <= nu[0]) { L.status |= PBAGPU_S_NU_OUT_OF_GRID; return 0.; }
     size_t ia_nu = findIndex(freq, nu.data(), nf), ib_nu = ia_nu + 1;
@@ -283,7 +284,12 @@ static double flux_dens_shock(LayerCtx &L, const double *jall, const std::vector
     double q21 = jall[ia * nf + ib_nu], q22 = jall[ib * nf + ib_nu];
     double x2x1 = x2 - x1, y2y1 = y2 - y1, x2x = x2 - freq, y2y = y2 - r, yy1 = r - y1, xx1 = freq - x1;
     double em_prime = 1.0 / (x2x1 * y2y1) * (q11 * x2x * y2y + q21 * xx1 * y2y + q12 * x2x * yy1 + q22 * xx1 * yy1);
-    double abs_prime = 0.;  // no synchrotron self-absorption: the absorption table is identically 0
+    double abs_prime = 0.;  // without synchrotron self-absorption the absorption table is identically 0
+    if (aall) {
+        double a11 = aall[ia * nf + ia_nu], a12 = aall[ib * nf + ia_nu];
+        double a21 = aall[ia * nf + ib_nu], a22 = aall[ib * nf + ib_nu];
+        abs_prime = 1.0 / (x2x1 * y2y1) * (a11 * x2x * y2y + a21 * xx1 * y2y + a12 * x2x * yy1 + a22 * xx1 * yy1);
+    }
     double beta = BetaFromGamma(Gamma);
     double a = 1.0 - beta * mu;
     double delta_D = Gamma * a;
@@ -326,13 +332,13 @@ static double integrand(LayerCtx &L, double cos_theta, double phi) {
     double delta_D = Gamma * a;
     double nuprime = (1.0 + L.P->z) * L.nu_obs * delta_D;
     double theta = interpSegLog(ia, ib, r, R, L.V(PV_theta));
-    double flux = flux_dens_shock(L, L.jfs, L.nu_fs, nuprime, ia, ib, Gamma, mu, r, ne, n_prime, theta);
+    double flux = flux_dens_shock(L, L.jfs, L.afs, L.nu_fs, nuprime, ia, ib, Gamma, mu, r, ne, n_prime, theta);
     if (L.rs) {
         const double *th_rs = L.V(PV_thickness_rs);
         if (th_rs[ia] > 0 && th_rs[ib] > 0) {
             double ne_rs = interpSegLog(ia, ib, r, R, L.V(PV_M3)) / mp;
             double n_prime_rs = interpSegLog(ia, ib, r, R, L.V(PV_rho3)) / mp;
-            flux += flux_dens_shock(L, L.jrs, L.nu_rs, nuprime, ia, ib, Gamma, mu, r, ne_rs, n_prime_rs, theta);
+            flux += flux_dens_shock(L, L.jrs, L.ars, L.nu_rs, nuprime, ia, ib, Gamma, mu, r, ne_rs, n_prime_rs, theta);
         }
     }
     return flux;
@@ -379,7 +385,8 @@ static void pars_mu(LayerCtx &L, double t_obs) {
 } // namespace orc
 
 extern "C" int orc_eats_lightcurve(const pba_run_params *params, int n_runs, const pba_eats_layer *layers, int n_layers,
-                                   const double *table, int nt, const double *jall, int nfreq, int n_pairs,
+                                   const double *table, int nt, const double *jall, const double *aall, int nfreq,
+                                   int n_pairs,
                                    const double *tobs, const double *nuobs, double *out, int *status) {
     if (!params || !layers || !table || !jall || !out || !status || n_layers < 1) return PBAGPU_E_ARG;
 #pragma omp parallel for schedule(dynamic)
@@ -393,6 +400,8 @@ extern "C" int orc_eats_lightcurve(const pba_run_params *params, int n_runs, con
         L.jfs = jall + (size_t)Ly.pop_fs * nt * nfreq;
         L.rs = Ly.pop_rs >= 0;
         L.jrs = L.rs ? jall + (size_t)Ly.pop_rs * nt * nfreq : nullptr;
+        L.afs = (aall && L.P->fs.ssa) ? aall + (size_t)Ly.pop_fs * nt * nfreq : nullptr;
+        L.ars = (aall && L.P->rs.ssa && L.rs) ? aall + (size_t)Ly.pop_rs * nt * nfreq : nullptr;
         L.nu_fs = orc::freq_grid(L.P->fs);
         L.nu_rs = orc::freq_grid(L.P->rs);
         L.theta_c_l = Ly.theta_c_l;
@@ -444,6 +453,7 @@ struct PwCtx {
     const pba_run_params *P;
     const double *T; int nt; int i_end_r;
     const double *jfs;                 // [nt][nf] forward-shock emissivity
+    const double *afs = nullptr;       // [nt][nf] forward-shock absorption (use_ssa) or null
     std::vector<double> nu_fs;
     bool rs;
     double ctheta0, ncells;
@@ -459,7 +469,8 @@ static double flux_dens_shock_pw(PwCtx &L, double freq, size_t ia, size_t ib, do
                                  double ne, double n_prime) {
     LayerCtx S;
     S.P = L.P; S.T = L.T; S.nt = L.nt; S.i_end_r = L.i_end_r;
-    double f = flux_dens_shock(S, L.jfs, L.nu_fs, freq, ia, ib, Gamma, mu, r, ne, n_prime, L.ctheta0, L.ncells);
+    double f = flux_dens_shock(S, L.jfs, L.afs, L.nu_fs, freq, ia, ib, Gamma, mu, r, ne, n_prime, L.ctheta0,
+                               L.ncells);
     L.status |= S.status;
     return f;
 }
@@ -558,7 +569,8 @@ static double sky_map_pw(PwCtx &L, double t_obs, double nu_obs, bool cj, const P
 } // namespace orc
 
 extern "C" int orc_eats_lightcurve_pw(const pba_run_params *params, int n_runs, const pba_eats_pw_layer *layers,
-                                      int n_layers, const double *table, int nt, const double *jall, int nfreq,
+                                      int n_layers, const double *table, int nt, const double *jall,
+                                      const double *aall, int nfreq,
                                       int n_pairs, const double *tobs, const double *nuobs, double *out, int *status) {
     (void)n_runs;
     if (!params || !layers || !table || !jall || !out || !status || n_layers < 1) return PBAGPU_E_ARG;
@@ -571,6 +583,7 @@ extern "C" int orc_eats_lightcurve_pw(const pba_run_params *params, int n_runs, 
         L.nt = nt;
         L.i_end_r = Ly.i_end_r;
         L.jfs = jall + (size_t)Ly.pop_fs * nt * nfreq;
+        L.afs = (aall && L.P->fs.ssa) ? aall + (size_t)Ly.pop_fs * nt * nfreq : nullptr;
         L.nu_fs = orc::freq_grid(L.P->fs);
         L.rs = Ly.rs != 0;
         L.ctheta0 = Ly.ctheta0;
@@ -597,7 +610,8 @@ extern "C" int orc_eats_lightcurve_pw(const pba_run_params *params, int n_runs, 
  * per-layer flux [n_layers][n_images]; Ejecta::computeEjectaSkyMapPW's call sequence
  * (src/ejecta/ejecta_base.h:471-495), the normalisation after it is host logic. */
 extern "C" int orc_eats_skymap_pw(const pba_run_params *params, const pba_eats_pw_layer *layers, int n_layers,
-                                  const double *table, int nt, const double *jall, int nfreq, int n_images,
+                                  const double *table, int nt, const double *jall, const double *aall, int nfreq,
+                                  int n_images,
                                   const double *tobs, const double *nuobs, int remove_mu, double *image,
                                   double *flux, int *status) {
     if (!params || !layers || !table || !jall || !image || !flux || !status || n_layers < 1) return PBAGPU_E_ARG;
@@ -613,6 +627,7 @@ extern "C" int orc_eats_skymap_pw(const pba_run_params *params, const pba_eats_p
         L.nt = nt;
         L.i_end_r = Ly.i_end_r;
         L.jfs = jall + (size_t)Ly.pop_fs * nt * nfreq;
+        L.afs = (aall && L.P->fs.ssa) ? aall + (size_t)Ly.pop_fs * nt * nfreq : nullptr;
         L.nu_fs = orc::freq_grid(L.P->fs);
         L.rs = Ly.rs != 0;
         L.ctheta0 = Ly.ctheta0;

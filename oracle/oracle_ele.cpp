@@ -109,13 +109,14 @@ static void power_law(const EleGrid &g, double p, double gm, double gc, double g
 }
 
 static void evolve_population(const pba_shock_params &sp, const SSCTab *ssc, int shock, const double *T, int nt,
-                              double *jall, double *fall, int *status) {
+                              double *jall, double *aall, double *fall, int *status) {
     const int ng = sp.ngam, nf = sp.nfreq;
     EleGrid g;
     g.build(sp.gam2, ng);
     std::vector<double> nu = freq_grid(sp);
     std::vector<double> f(ng, 0.), tot_f(nf, 0.), heat(ng, 0.), src(ng, 0.), bb(ng, 0.), cc(ng, 0.);
     std::vector<double> synj(nf, 0.), sscj(nf, 0.), K((size_t)nf * ng, 0.), tmp(ng, 0.), scat(ng, 0.);
+    std::vector<double> syna(nf, 0.), dfde(ng, 0.), fg2(ng, 0.);   // computeSSA state (ele.dfde persists)
     bool init = false;
     double dr_comov = 0., vol = 0.;
     auto V = [&](int v, int it) { return T[(size_t)v * nt + it]; };
@@ -230,6 +231,23 @@ static void evolve_population(const pba_shock_params &sp, const SSCTab *ssc, int
             synj[i] = res;
             if (!std::isfinite(res) || res < 0.) *status |= PBAGPU_S_NONFINITE;
         }
+        if (sp.ssa) {
+            // computeSSA + dfdx (numeric_model.h:190-238, base_state.h:482-536)
+            for (int j = 0; j < ng; j++) fg2[j] = f[j] / g.e[j] / g.e[j];
+            for (int j = 1; j < ng - 1; j++)
+                if (fg2[j] > 0) dfde[j] = 0.5 * (fg2[j + 1] - fg2[j - 1]) / (g.e[j] - g.e[j - 1]);
+            if (fg2[0] > 0) dfde[0] = -1.5 * fg2[0] + 2. * fg2[1] - 0.5 * fg2[2];
+            if (fg2[ng - 1] > 0) dfde[ng - 1] = 1.5 * fg2[ng - 1] - 2. * fg2[ng - 2] + 0.5 * fg2[ng - 3];
+            for (int i = 0; i < nf; i++) {
+                double a = 0.;
+                for (int j = 0; j < ng - 1; j++)
+                    a += g.e[j] * g.e[j] * (g.e[j + 1] - g.e[j]) * dfde[j] * K[(size_t)i * ng + j];
+                a *= -1. / (8. * pi * me * nu[i] * nu[i]);
+                a = std::abs(a);
+                if (!std::isfinite(a)) a = 1e99;
+                syna[i] = a;
+            }
+        }
         std::fill(sscj.begin(), sscj.end(), 0.);
         if (ssc) {
             for (int i = 0; i < nf; i++) {
@@ -260,6 +278,8 @@ static void evolve_population(const pba_shock_params &sp, const SSCTab *ssc, int
             jt = jt * acc / n_ele;
             if (!std::isfinite(jt) || jt < 0.) *status |= PBAGPU_S_NONFINITE;
             jall[(size_t)it * nf + i] = jt;
+            // total_rad.a = syn.a / n_ele * accel_frac * n_prime (microphysics.h:576-579)
+            if (aall && sp.ssa) aall[(size_t)it * nf + i] = syna[i] / n_ele * acc * (V(vrho, it) / mp);
         }
         if (fall)
             for (int i = 0; i < ng; i++) fall[(size_t)it * ng + i] = f[i];
@@ -269,7 +289,8 @@ static void evolve_population(const pba_shock_params &sp, const SSCTab *ssc, int
 } // namespace orc
 
 extern "C" int orc_ele_evolve(const pba_run_params *params, int n_runs, const pba_population *pops, int n_pop,
-                              const double *table, int bw_total, int nt, double *jall, double *fall, int *status) {
+                              const double *table, int bw_total, int nt, double *jall, double *aall, double *fall,
+                              int *status) {
     if (!params || !pops || !table || !jall || !status || n_pop < 1) return PBAGPU_E_ARG;
     const pba_shock_params &s0 = pops[0].shock ? params[pops[0].run].rs : params[pops[0].run].fs;
     const int ng = s0.ngam, nf = s0.nfreq;
@@ -297,8 +318,10 @@ extern "C" int orc_ele_evolve(const pba_run_params *params, int n_runs, const pb
         status[q] = 0;
         std::fill(jall + (size_t)q * nt * nf, jall + (size_t)(q + 1) * nt * nf, 0.);
         if (fall) std::fill(fall + (size_t)q * nt * ng, fall + (size_t)(q + 1) * nt * ng, 0.);
+        if (aall) std::fill(aall + (size_t)q * nt * nf, aall + (size_t)(q + 1) * nt * nf, 0.);
         orc::evolve_population(sp, sp.ssc ? &tabs[key] : nullptr, P.shock, table + (size_t)P.bw * PBA_NV * nt, nt,
-                               jall + (size_t)q * nt * nf, fall ? fall + (size_t)q * nt * ng : nullptr, &status[q]);
+                               jall + (size_t)q * nt * nf, aall ? aall + (size_t)q * nt * nf : nullptr,
+                               fall ? fall + (size_t)q * nt * ng : nullptr, &status[q]);
     }
     return PBAGPU_OK;
 }

@@ -22,16 +22,16 @@ def lib():
                                         C.c_int, dp, dp, C.POINTER(abi.BwResult)]
         if hasattr(_lib, "orc_ele_evolve"):
             _lib.orc_ele_evolve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.Population), C.c_int,
-                                            dp, C.c_int, C.c_int, dp, dp, ip]
+                                            dp, C.c_int, C.c_int, dp, dp, dp, ip]
         if hasattr(_lib, "orc_eats_lightcurve"):
             _lib.orc_eats_lightcurve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsLayer), C.c_int,
-                                                 dp, C.c_int, dp, C.c_int, C.c_int, dp, dp, dp, ip]
+                                                 dp, C.c_int, dp, dp, C.c_int, C.c_int, dp, dp, dp, ip]
         if hasattr(_lib, "orc_eats_skymap_pw"):
             _lib.orc_eats_skymap_pw.argtypes = [C.POINTER(abi.RunParams), C.POINTER(abi.EatsPwLayer), C.c_int, dp,
-                                                C.c_int, dp, C.c_int, C.c_int, dp, dp, C.c_int, dp, dp, ip]
+                                                C.c_int, dp, dp, C.c_int, C.c_int, dp, dp, C.c_int, dp, dp, ip]
         if hasattr(_lib, "orc_eats_lightcurve_pw"):
             _lib.orc_eats_lightcurve_pw.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsPwLayer),
-                                                    C.c_int, dp, C.c_int, dp, C.c_int, C.c_int, dp, dp, dp, ip]
+                                                    C.c_int, dp, C.c_int, dp, dp, C.c_int, C.c_int, dp, dp, dp, ip]
     return _lib
 
 
@@ -59,7 +59,12 @@ def dyn_evolve(params, inits, tgrid, n_runs=1):
     return table, res
 
 
-def ele_evolve(params, pops, table, n_runs=1, want_f=False):
+def _dpn(a):
+    return None if a is None else _dp(np.ascontiguousarray(a))
+
+
+def ele_evolve(params, pops, table, n_runs=1, want_f=False, want_a=False):
+    """(jall, fall, status), plus the absorption table aall when want_a (zero where use_ssa is off)."""
     if isinstance(params, abi.RunParams):
         params = (abi.RunParams * 1)(params)
     nbw, nv, nt = table.shape
@@ -71,14 +76,15 @@ def ele_evolve(params, pops, table, n_runs=1, want_f=False):
     jall = np.zeros((npop, nt, nf))
     fall = np.zeros((npop, nt, ng)) if want_f else None
     st = np.zeros(npop, dtype=np.int32)
-    rc = lib().orc_ele_evolve(params, n_runs, pops, npop, _dp(tab), nbw, nt, _dp(jall),
+    aall = np.zeros((npop, nt, nf))
+    rc = lib().orc_ele_evolve(params, n_runs, pops, npop, _dp(tab), nbw, nt, _dp(jall), _dp(aall),
                               _dp(fall) if want_f else None, _ip(st))
     if rc != 0:
         raise RuntimeError(f"orc_ele_evolve rc={rc}")
-    return jall, fall, st
+    return (jall, fall, st, aall) if want_a else (jall, fall, st)
 
 
-def eats_lightcurve(params, layers, table, jall, tobs, nuobs, n_runs=1):
+def eats_lightcurve(params, layers, table, jall, tobs, nuobs, n_runs=1, aall=None):
     if isinstance(params, abi.RunParams):
         params = (abi.RunParams * 1)(params)
     nbw, nv, nt = table.shape
@@ -87,7 +93,7 @@ def eats_lightcurve(params, layers, table, jall, tobs, nuobs, n_runs=1):
     out = np.zeros((len(layers), n))
     st = np.zeros(len(layers), dtype=np.int32)
     rc = lib().orc_eats_lightcurve(params, n_runs, layers, len(layers), _dp(np.ascontiguousarray(table)), nt,
-                                   _dp(np.ascontiguousarray(jall)), nf, n,
+                                   _dp(np.ascontiguousarray(jall)), _dpn(aall), nf, n,
                                    _dp(np.ascontiguousarray(tobs, dtype=np.float64)),
                                    _dp(np.ascontiguousarray(nuobs, dtype=np.float64)), _dp(out), _ip(st))
     if rc != 0:
@@ -95,7 +101,7 @@ def eats_lightcurve(params, layers, table, jall, tobs, nuobs, n_runs=1):
     return out, st
 
 
-def eats_lightcurve_pw(params, layers, table, jall, tobs, nuobs, n_runs=1):
+def eats_lightcurve_pw(params, layers, table, jall, tobs, nuobs, n_runs=1, aall=None):
     """Piece-wise EATS light curves; layers: EatsPwLayer array."""
     if isinstance(params, abi.RunParams):
         params = (abi.RunParams * 1)(params)
@@ -105,7 +111,7 @@ def eats_lightcurve_pw(params, layers, table, jall, tobs, nuobs, n_runs=1):
     out = np.zeros((len(layers), n))
     st = np.zeros(len(layers), dtype=np.int32)
     rc = lib().orc_eats_lightcurve_pw(params, n_runs, layers, len(layers), _dp(np.ascontiguousarray(table)), nt,
-                                      _dp(np.ascontiguousarray(jall)), nf, n,
+                                      _dp(np.ascontiguousarray(jall)), _dpn(aall), nf, n,
                                       _dp(np.ascontiguousarray(tobs, dtype=np.float64)),
                                       _dp(np.ascontiguousarray(nuobs, dtype=np.float64)), _dp(out), _ip(st))
     if rc != 0:
@@ -113,7 +119,7 @@ def eats_lightcurve_pw(params, layers, table, jall, tobs, nuobs, n_runs=1):
     return out, st
 
 
-def eats_skymap_pw(params, layers, table, jall, tobs, nuobs, remove_mu=False):
+def eats_skymap_pw(params, layers, table, jall, tobs, nuobs, remove_mu=False, aall=None):
     """Raw piece-wise sky maps of one run: (image [n_images][PBA_IMG_NV][2 ncells], flux [n_layers][n_images],
     status [n_layers])."""
     if isinstance(params, abi.RunParams):
@@ -126,7 +132,7 @@ def eats_skymap_pw(params, layers, table, jall, tobs, nuobs, remove_mu=False):
     flux = np.zeros((len(layers), n))
     st = np.zeros(len(layers), dtype=np.int32)
     rc = lib().orc_eats_skymap_pw(params, layers, len(layers), _dp(np.ascontiguousarray(table)), nt,
-                                  _dp(np.ascontiguousarray(jall)), nf, n,
+                                  _dp(np.ascontiguousarray(jall)), _dpn(aall), nf, n,
                                   _dp(np.ascontiguousarray(tobs, dtype=np.float64)),
                                   _dp(np.ascontiguousarray(nuobs, dtype=np.float64)), 1 if remove_mu else 0,
                                   _dp(image), _dp(flux), _ip(st))

@@ -351,6 +351,8 @@ struct LayerArgs {
     const double *T;    // table [PBA_NV][nt]
     const double *jfs;  // [nt][nf]
     const double *jrs;  // or null
+    const double *afs;  // absorption [nt][nf] (use_ssa) or null
+    const double *ars;  // or null
     const double *nufs, *nurs;
     double theta_c_l, theta_c_h;
     double z, theta_obs, d_l, rtol_theta, rtol_phi;
@@ -373,6 +375,7 @@ extern __shared__ double lds_tab[];
 // that the integrand reads registers, not the layer descriptor.
 struct Ctx {
     const double *jfs, *jrs;
+    const double *afs, *ars;   // absorption tables or null
     double z;
     int nt, nf, i_end_r;
     double tz;          // t_obs / (1 + z)
@@ -428,30 +431,35 @@ struct EdgeSearch {
 };
 
 // nu_off: offset of the shock's frequency grid in the staged LDS block
-__device__ double shock_flux(Ctx &C, const double *jall, int nu_off, double freq, int ia, int ib, double Gamma,
-                             double mu, double r, double ne, double n_prime, double theta) {
+// SSA: absorption tables may be present (a separate instantiation, so that runs without
+// synchrotron self-absorption do not carry the absorption code's registers).
+template <bool SSA>
+__device__ __forceinline__ double shock_flux(Ctx &C, const double *jall, const double *aall, int nu_off, double freq,
+                                             int ia, int ib, double Gamma, double mu, double r, double ne,
+                                             double n_prime, double theta) {
     const int nf = C.nf;
     auto nu = [&](int i) { return lds_tab[nu_off + i]; };
     if (freq >= nu(nf - 1) || freq <= nu(0)) { C.status |= PBAGPU_S_NU_OUT_OF_GRID; return 0.; }
-    int ia_nu = find_index(freq, nu, nf), ib_nu = ia_nu + 1;
-    double x1 = nu(ia_nu), x2 = nu(ib_nu), y1 = C.V(S_R, ia), y2 = C.V(S_R, ib);
-    double q11 = jall[(size_t)ia * nf + ia_nu], q12 = jall[(size_t)ib * nf + ia_nu];
-    double q21 = jall[(size_t)ia * nf + ib_nu], q22 = jall[(size_t)ib * nf + ib_nu];
-    double x2x1 = x2 - x1, y2y1 = y2 - y1, x2x = x2 - freq, y2y = y2 - r, yy1 = r - y1, xx1 = freq - x1;
-    double em_prime = 1.0 / (x2x1 * y2y1) * (q11 * x2x * y2y + q21 * xx1 * y2y + q12 * x2x * yy1 + q22 * xx1 * yy1);
-    double beta = beta_from_gamma(Gamma);
-    double delta_D = Gamma * (1.0 - beta * mu);
-    double em_lab = em_prime / (delta_D * delta_D);
-    // no synchrotron self-absorption: the optical depth is 0 unless mu equals the shock velocity
-    double intensity = (mu == beta) ? em_lab * (-expm1(-1.0e100) / 1.0e100) : em_lab;
-    double omc = (1. - cos(theta) / 1.);
-    double lum = intensity * ne / (2. * kPi * omc) / Gamma / (1.0 - mu * beta);
-    if (!(lum >= 0) || !isfinite(lum)) C.status |= PBAGPU_S_NONFINITE;
+    const int ia_nu = find_index(freq, nu, nf), ib_nu = ia_nu + 1;
+    const double x1 = nu(ia_nu), x2 = nu(ib_nu), y1 = C.V(S_R, ia), y2 = C.V(S_R, ib);
+    const double em_prime = table_bilinear(jall, nf, ia, ib, ia_nu, ib_nu, x1, x2, y1, y2, freq, r);
+    const double beta = beta_from_gamma(Gamma);
+    const double delta_D = Gamma * (1.0 - beta * mu);
+    const double omc = (1. - cos(theta) / 1.);
+    // no absorption table: the optical depth is 0 unless mu equals the shock velocity
+    const double dtau = (mu == beta) ? 1.0e100
+                      : (SSA && aall) ? absorption_depth(aall, nf, ia, ib, ia_nu, ib_nu, x1, x2, y1, y2, freq, r, Gamma,
+                                                         beta, delta_D, mu, ne, n_prime, omc)
+                                      : 0.;
+    bool bad = false;
+    const double lum = shock_luminosity(em_prime, dtau, Gamma, beta, delta_D, mu, ne, omc, bad);
+    if (bad) C.status |= PBAGPU_S_NONFINITE;
     return lum;
 }
 
 // Out of line on purpose: inlined at its one call site, the kernel needs more registers than
 // the two waves per SIMD it runs at and spills almost twice as much.
+template <bool SSA>
 __device__ __noinline__ double integrand(Ctx &C, double cos_theta) {
     const int N = C.i_end_r;
     if (C.V(S_R, 0) == 0.0 && C.V(S_R, N - 1) == 0.0) return 0.0;
@@ -483,11 +491,11 @@ __device__ __noinline__ double integrand(Ctx &C, double cos_theta) {
     double n_prime = li(S_rho2) / kMp;
     double nuprime = (1.0 + C.z) * C.nu_obs * (Gamma * (1.0 - beta * mu));
     double theta = li(S_theta);
-    double flux = shock_flux(C, C.jfs, NSLOT * C.nt, nuprime, ia, ib, Gamma, mu, r, ne, n_prime, theta);
+    double flux = shock_flux<SSA>(C, C.jfs, C.afs, NSLOT * C.nt, nuprime, ia, ib, Gamma, mu, r, ne, n_prime, theta);
     if (C.jrs && C.V(S_thrs, ia) > 0 && C.V(S_thrs, ib) > 0) {
         double ne_rs = li(S_M3) / kMp;
         double np_rs = li(S_rho3) / kMp;
-        flux += shock_flux(C, C.jrs, NSLOT * C.nt + C.nf, nuprime, ia, ib, Gamma, mu, r, ne_rs, np_rs, theta);
+        flux += shock_flux<SSA>(C, C.jrs, C.ars, NSLOT * C.nt + C.nf, nuprime, ia, ib, Gamma, mu, r, ne_rs, np_rs, theta);
     }
     return flux;
 }
@@ -505,6 +513,7 @@ extern __shared__ double lds_tab[];   // [NSLOT][nt] table columns, then the two
 // the layer edges of find_jet_edge for every phi sample, primary jet then counter-jet) runs as
 // two resumable CADRE machines; every pass of the lane loop takes exactly one integrand
 // sample, at the single call site below.
+template <bool SSA>
 __global__ void __launch_bounds__(EATS_WG, 2) eats_kernel(const LayerArgs *__restrict__ layers, int n_layers, int n_pairs,
                                                        int *__restrict__ pair_next,
                                                        const double *__restrict__ tobs,
@@ -561,7 +570,7 @@ __global__ void __launch_bounds__(EATS_WG, 2) eats_kernel(const LayerArgs *__res
         const double Fcoeff = kCgs2mJy / (4 * M_PI * L.d_l * L.d_l);
         const double th1c = L.theta_c_h, th0c = L.theta_c_l;
         Ctx C;
-        C.jfs = L.jfs; C.jrs = L.jrs; C.z = L.z;
+        C.jfs = L.jfs; C.jrs = L.jrs; C.afs = L.afs; C.ars = L.ars; C.z = L.z;
         C.nt = nt; C.nf = nf; C.i_end_r = L.i_end_r;
         C.sa = sin(L.theta_obs);
         C.ca = cos(L.theta_obs);
@@ -619,7 +628,7 @@ __global__ void __launch_bounds__(EATS_WG, 2) eats_kernel(const LayerArgs *__res
             }
             if (mode != EVAL) continue;
             // the one integrand call of the loop
-            const double v = integrand(C, 1 - Mi.x);
+            const double v = integrand<SSA>(C, 1 - Mi.x);
             samples++;
             if (Mi.feed(v)) continue;
             double res = Mi.result;
@@ -647,9 +656,9 @@ __global__ void __launch_bounds__(EATS_WG, 2) eats_kernel(const LayerArgs *__res
 }  // namespace pbagpu
 
 extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, const pba_eats_layer *layers,
-                                      int n_layers, const double *d_table, int nt, const double *d_jall, int nfreq,
-                                      int n_pairs, const double *d_tobs, const double *d_nuobs, double *d_out,
-                                      int *d_status, void *stream) {
+                                      int n_layers, const double *d_table, int nt, const double *d_jall,
+                                      const double *d_aall, int nfreq, int n_pairs, const double *d_tobs,
+                                      const double *d_nuobs, double *d_out, int *d_status, void *stream) {
     using namespace pbagpu::eats;
     if (!params || !layers || !d_table || !d_jall || !d_out || !d_status || n_layers < 1 || n_pairs < 1 || nt < 2)
         return PBAGPU_E_ARG;
@@ -683,6 +692,8 @@ extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, 
         A.T = d_table + (size_t)Y.bw * PBA_NV * nt;
         A.jfs = d_jall + (size_t)Y.pop_fs * nt * nfreq;
         A.jrs = Y.pop_rs >= 0 ? d_jall + (size_t)Y.pop_rs * nt * nfreq : nullptr;
+        A.afs = (d_aall && P.fs.ssa) ? d_aall + (size_t)Y.pop_fs * nt * nfreq : nullptr;
+        A.ars = (d_aall && P.rs.ssa && Y.pop_rs >= 0) ? d_aall + (size_t)Y.pop_rs * nt * nfreq : nullptr;
         A.nufs = dnu + ((size_t)Y.run * 2) * nfreq;
         A.nurs = dnu + ((size_t)Y.run * 2 + 1) * nfreq;
         A.theta_c_l = Y.theta_c_l; A.theta_c_h = Y.theta_c_h;
@@ -701,24 +712,31 @@ extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, 
         PBA_TRY(rc, hipGetDevice(&dev));
         PBA_TRY(rc, hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
         const long n_items = ((long)n_pairs + EATS_WG - 1) / EATS_WG * n_layers;   // workgroup-sized pieces
-        (void)hipFuncSetAttribute((const void *)eats_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        bool ssa = false;   // any absorption table: the instantiation with the absorption code
+        for (const LayerArgs &A : hl) ssa = ssa || A.afs || A.ars;
+        const void *kfn = ssa ? (const void *)eats_kernel<true> : (const void *)eats_kernel<false>;
+        (void)hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         int per_cu = 1;
-        PBA_TRY(rc, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, eats_kernel, EATS_WG, lds));
+        PBA_TRY(rc, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, EATS_WG, lds));
         const int n_wg = (int)std::min<long>(n_items, (long)std::max(n_cu, 1) * std::max(per_cu, 1));
         char *ws = (char *)pbagpu::workspace(pbagpu::WS_EATS_STORE, (size_t)n_wg * EATS_WG * LANE_WS);
         int *counter = (int *)pbagpu::workspace(pbagpu::WS_EATS_COUNTER, sizeof(int) * n_layers);
         if (!ws || !counter) rc = PBAGPU_E_HIP;
         if (rc == PBAGPU_OK) PBA_TRY(rc, hipMemsetAsync(counter, 0, sizeof(int) * n_layers, s));
         if (rc == PBAGPU_OK) {
-            (void)hipFuncSetAttribute((const void *)eats_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            (void)hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             unsigned long long *d_samples = nullptr;
             if (pbagpu::timing_enabled()) {
                 d_samples = (unsigned long long *)pbagpu::workspace(pbagpu::WS_EATS_SAMPLES, sizeof(unsigned long long));
                 if (d_samples) PBA_TRY(rc, hipMemsetAsync(d_samples, 0, sizeof(unsigned long long), s));
             }
             timer.start(s);
-            hipLaunchKernelGGL(eats_kernel, dim3((unsigned)n_wg), dim3(EATS_WG), lds, s, dL, n_layers, n_pairs, counter,
-                               d_tobs, d_nuobs, d_out, d_status, ws, d_samples);
+            if (ssa)
+                hipLaunchKernelGGL(eats_kernel<true>, dim3((unsigned)n_wg), dim3(EATS_WG), lds, s, dL, n_layers, n_pairs,
+                                   counter, d_tobs, d_nuobs, d_out, d_status, ws, d_samples);
+            else
+                hipLaunchKernelGGL(eats_kernel<false>, dim3((unsigned)n_wg), dim3(EATS_WG), lds, s, dL, n_layers, n_pairs,
+                                   counter, d_tobs, d_nuobs, d_out, d_status, ws, d_samples);
             PBA_TRY(rc, hipGetLastError());
             timer.stop(s);
             if (d_samples && rc == PBAGPU_OK) {

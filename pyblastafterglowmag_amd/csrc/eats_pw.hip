@@ -38,6 +38,7 @@ __device__ __forceinline__ constexpr int col_var(int c) {
 struct PwArgs {
     const double *T;    // table [PBA_NV][nt]
     const double *jfs;  // forward-shock emissivity [nt][nf]
+    const double *afs;  // forward-shock absorption [nt][nf] (use_ssa) or null
     const double *nu;   // forward-shock comoving frequency grid [nf]
     double ctheta0, ncells;
     double z, theta_obs, d_l;
@@ -80,25 +81,24 @@ struct Lane {
         }
         return best_i;
     }
-    // fluxDens + computeFluxDensity (useNe, sameAsBW, smooth optical depth, no SSA)
-    __device__ double shock_flux(const double *jall, double freq, int ia, int ib, double Gamma, double mu, double r,
-                                 double ne, double n_prime, double theta, double ncells) {
+    // fluxDens + computeFluxDensity (useNe, sameAsBW, smooth optical depth)
+    __device__ double shock_flux(const double *jall, const double *aall, double freq, int ia, int ib, double Gamma,
+                                 double mu, double r, double ne, double n_prime, double theta, double ncells) {
         if (freq >= nu(nf - 1) || freq <= nu(0)) { status |= PBAGPU_S_NU_OUT_OF_GRID; return 0.; }
         auto grid = [&](int i) { return nu(i); };
-        int ia_nu = find_index(freq, grid, nf), ib_nu = ia_nu + 1;
-        double x1 = nu(ia_nu), x2 = nu(ib_nu), y1 = V(C_R, ia), y2 = V(C_R, ib);
-        double q11 = jall[(size_t)ia * nf + ia_nu], q12 = jall[(size_t)ib * nf + ia_nu];
-        double q21 = jall[(size_t)ia * nf + ib_nu], q22 = jall[(size_t)ib * nf + ib_nu];
-        double x2x1 = x2 - x1, y2y1 = y2 - y1, x2x = x2 - freq, y2y = y2 - r, yy1 = r - y1, xx1 = freq - x1;
-        double em_prime = 1.0 / (x2x1 * y2y1) * (q11 * x2x * y2y + q21 * xx1 * y2y + q12 * x2x * yy1 + q22 * xx1 * yy1);
-        double beta = beta_from_gamma(Gamma);
-        double delta_D = Gamma * (1.0 - beta * mu);
-        double em_lab = em_prime / (delta_D * delta_D);
-        // absorption is identically 0: the optical depth vanishes unless mu equals the shock velocity
-        double intensity = (mu == beta) ? em_lab * (-expm1(-1.0e100) / 1.0e100) : em_lab;
-        double omc = (1. - cos(theta) / ncells);
-        double lum = intensity * ne / (2. * kPi * omc) / Gamma / (1.0 - mu * beta);
-        if (!(lum >= 0) || !isfinite(lum)) status |= PBAGPU_S_NONFINITE;
+        const int ia_nu = find_index(freq, grid, nf), ib_nu = ia_nu + 1;
+        const double x1 = nu(ia_nu), x2 = nu(ib_nu), y1 = V(C_R, ia), y2 = V(C_R, ib);
+        const double em_prime = table_bilinear(jall, nf, ia, ib, ia_nu, ib_nu, x1, x2, y1, y2, freq, r);
+        const double beta = beta_from_gamma(Gamma);
+        const double delta_D = Gamma * (1.0 - beta * mu);
+        const double omc = (1. - cos(theta) / ncells);
+        const double dtau = (mu == beta) ? 1.0e100
+                          : aall ? absorption_depth(aall, nf, ia, ib, ia_nu, ib_nu, x1, x2, y1, y2, freq, r, Gamma,
+                                                    beta, delta_D, mu, ne, n_prime, omc)
+                                 : 0.;
+        bool bad = false;
+        const double lum = shock_luminosity(em_prime, dtau, Gamma, beta, delta_D, mu, ne, omc, bad);
+        if (bad) status |= PBAGPU_S_NONFINITE;
         return lum;
     }
 };
@@ -153,10 +153,10 @@ __device__ double sky_map_pw(Lane &L, const PwArgs &A, double t_obs, double nu_o
         const double Gamma = lg(C_Gamma), beta = lg(C_beta);
         const double nprime = lg(C_rho2) / kMp, nprotons = lg(C_M2) / kMp;
         const double nuprime = (1.0 + A.z) * nu_obs * (Gamma * (1.0 - beta * mu));
-        double f = L.shock_flux(A.jfs, nuprime, ia, ib, Gamma, mu, r, nprotons, nprime, A.ctheta0, A.ncells);
+        double f = L.shock_flux(A.jfs, A.afs, nuprime, ia, ib, Gamma, mu, r, nprotons, nprime, A.ctheta0, A.ncells);
         if (A.rs && L.V(C_thrs, ia) > 0 && L.V(C_thrs, ib) > 0) {
             const double nprime_rs = lg(C_rho3) / kMp, ne_rs = lg(C_M3) / kMp;
-            f += L.shock_flux(A.jfs, nuprime, ia, ib, Gamma, mu, r, ne_rs, nprime_rs, A.ctheta0, A.ncells);
+            f += L.shock_flux(A.jfs, A.afs, nuprime, ia, ib, Gamma, mu, r, ne_rs, nprime_rs, A.ctheta0, A.ncells);
         }
         f *= (1.0 + A.z) / (2.0 * A.d_l * A.d_l);
         if (!isfinite(f) || f < 0) L.status |= PBAGPU_S_NONFINITE;
@@ -247,7 +247,7 @@ namespace eatspw {
 
 // Shared host part of the light-curve and sky-map entries; image == nullptr for light curves.
 static int launch(const pba_run_params *params, int n_runs, const pba_eats_pw_layer *layers, int n_layers,
-                  const double *d_table, int nt, const double *d_jall, int nfreq, int n_pairs, const double *d_tobs,
+                  const double *d_table, int nt, const double *d_jall, const double *d_aall, int nfreq, int n_pairs, const double *d_tobs,
                   const double *d_nuobs, double *d_out, int *d_status, double *d_image, int remove_mu,
                   hipStream_t s) {
     if (!params || !layers || !d_table || !d_jall || !d_tobs || !d_nuobs || !d_out || !d_status || n_runs < 1 ||
@@ -293,6 +293,7 @@ static int launch(const pba_run_params *params, int n_runs, const pba_eats_pw_la
         PwArgs &A = hl[l];
         A.T = d_table + (size_t)Y.bw * PBA_NV * nt;
         A.jfs = d_jall + (size_t)Y.pop_fs * nt * nfreq;
+        A.afs = (d_aall && P.fs.ssa) ? d_aall + (size_t)Y.pop_fs * nt * nfreq : nullptr;
         A.nu = dnu + (size_t)Y.run * nfreq;
         A.ctheta0 = Y.ctheta0; A.ncells = Y.ncells;
         A.z = P.z; A.theta_obs = P.theta_obs; A.d_l = P.d_l;
@@ -334,15 +335,16 @@ static int launch(const pba_run_params *params, int n_runs, const pba_eats_pw_la
 }  // namespace pbagpu
 
 extern "C" int pbagpu_eats_lightcurve_pw(const pba_run_params *params, int n_runs, const pba_eats_pw_layer *layers,
-                                         int n_layers, const double *d_table, int nt, const double *d_jall, int nfreq,
-                                         int n_pairs, const double *d_tobs, const double *d_nuobs, double *d_out,
-                                         int *d_status, void *stream) {
-    return pbagpu::eatspw::launch(params, n_runs, layers, n_layers, d_table, nt, d_jall, nfreq, n_pairs, d_tobs,
+                                         int n_layers, const double *d_table, int nt, const double *d_jall,
+                                         const double *d_aall, int nfreq, int n_pairs, const double *d_tobs,
+                                         const double *d_nuobs, double *d_out, int *d_status, void *stream) {
+    return pbagpu::eatspw::launch(params, n_runs, layers, n_layers, d_table, nt, d_jall, d_aall, nfreq, n_pairs, d_tobs,
                                   d_nuobs, d_out, d_status, nullptr, 0, (hipStream_t)stream);
 }
 
 extern "C" int pbagpu_eats_skymap_pw(const pba_run_params *params, const pba_eats_pw_layer *layers, int n_layers,
-                                     const double *d_table, int nt, const double *d_jall, int nfreq, int n_images,
+                                     const double *d_table, int nt, const double *d_jall, const double *d_aall,
+                                     int nfreq, int n_images,
                                      const double *d_tobs, const double *d_nuobs, int remove_mu, double *d_image,
                                      double *d_flux, int *d_status, void *stream) {
     if (!d_image || !layers || n_layers < 1) return PBAGPU_E_ARG;
@@ -355,6 +357,6 @@ extern "C" int pbagpu_eats_skymap_pw(const pba_run_params *params, const pba_eat
         pbagpu::set_error_msg("eats_skymap_pw: layers refer to params[0] (run = 0)");
         return PBAGPU_E_ARG;
     }
-    return pbagpu::eatspw::launch(params, 1, layers, n_layers, d_table, nt, d_jall, nfreq, n_images, d_tobs, d_nuobs,
-                                  d_flux, d_status, d_image, remove_mu, (hipStream_t)stream);
+    return pbagpu::eatspw::launch(params, 1, layers, n_layers, d_table, nt, d_jall, d_aall, nfreq, n_images, d_tobs,
+                                  d_nuobs, d_flux, d_status, d_image, remove_mu, (hipStream_t)stream);
 }

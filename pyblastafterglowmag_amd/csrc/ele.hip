@@ -93,9 +93,10 @@ struct PopDesc {
     const double *dgbar;  // delta_grid_bar [ng]
     const double *nu;     // frequency grid [nf]
     double *jall;         // [nt][nf]
+    double *aall;         // [nt][nf] absorption (use_ssa) or null
     double *fall;         // [nt][ng] or null
     double p, eps_e;
-    int shock, ng, nf, max_substeps, adi_loss, ssc;
+    int shock, ng, nf, max_substeps, adi_loss, ssc, ssa;
 };
 
 __device__ __forceinline__ void power_law(const double *e, int j, double p, double gm, double gc, double gM,
@@ -126,6 +127,8 @@ struct PopState {
     double *jsyn;  // [n_pop][nf]
     double *jssc;  // [n_pop][nf][SSC_JB] partial sums over blocks of Lorentz-factor bins
     double *scal;  // [n_pop][4]    dr_comov, vol, n_ele, fac
+    double *dfde;  // [n_pop][ng]   d(f/gamma^2)/dgamma of the last computed step (computeSSA's
+                   //               ele.dfde: bins where f = 0 keep their earlier value)
     int *flag;     // [n_pop][3]    initialised, active at this step, status bits
 };
 
@@ -137,6 +140,7 @@ __global__ void __launch_bounds__(64) ele_step_kernel(const PopDesc *__restrict_
     __shared__ double s_tf[MAXF];   // photon density of the previous step (total_rad.f)
     __shared__ double s_w[MAXF];    // tot_f[k] / nu[k] * (nu[k+1] - nu[k])
     __shared__ double s_dge[MAXG];  // e[j+1] - e[j]
+    __shared__ double s_wa[MAXG];   // e_j^2 (e_{j+1} - e_j) d(f/gamma^2)/dgamma (synchrotron self-absorption)
     const PopDesc D = pops[blockIdx.x];
     const int lane = threadIdx.x;
     const int ng = D.ng, nf = D.nf;
@@ -302,14 +306,45 @@ __global__ void __launch_bounds__(64) ele_step_kernel(const PopDesc *__restrict_
         const double n_ele = wave_sum(ne_part);
         if (n_ele <= 0) st |= PBAGPU_S_NONFINITE;
         const double fac = (dr_comov / kC) / vol;
+        if (D.ssa) {
+            // computeSSA + dfdx (numeric_model.h:190-238, base_state.h:482-536): derivative of
+            // f / gamma^2, central in the interior (divided by the lower spacing), one-sided
+            // second-order stencils at the ends (not divided by a spacing); bins with f = 0 keep
+            // the value of the last step that set them
+            auto tmp = [&](int j) { return s_f[j] / D.e[j] / D.e[j]; };
+            double *dfde = S.dfde + (size_t)q0 * ng;
+            for (int j = lane; j < ng; j += 64) {
+                const double tj = tmp(j);
+                if (tj > 0) {
+                    if (j == 0) dfde[j] = -1.5 * tj + 2. * tmp(1) - 0.5 * tmp(2);
+                    else if (j == ng - 1) dfde[j] = 1.5 * tj - 2. * tmp(j - 1) + 0.5 * tmp(j - 2);
+                    else dfde[j] = 0.5 * (tmp(j + 1) - tmp(j - 1)) / (D.e[j] - D.e[j - 1]);
+                }
+                if (j < ng - 1) s_wa[j] = D.e[j] * D.e[j] * (D.e[j + 1] - D.e[j]) * dfde[j];
+            }
+            __syncthreads();
+        }
+        const double acc_ssa = D.ssa ? V(vacc, it) : 0., nprime_ssa = D.ssa ? V(vrho, it) / kMp : 0.;
         for (int i = lane; i < nf; i += 64) {
             const double nui = D.nu[i];
-            double synj = 0.;
-            if (i < nf - 1) {
-                for (int j = 0; j < ng - 1; j++) synj += cyc_synch(B, nui, D.e[j], pref) * s_f[j] * s_dge[j];
-                if (!isfinite(synj) || synj < 0.) st |= PBAGPU_S_NONFINITE;
+            double synj = 0., asum = 0.;
+            const bool emit = i < nf - 1;   // computeSynSpectrum leaves the last frequency at 0
+            if (emit || D.ssa) {
+                for (int j = 0; j < ng - 1; j++) {
+                    const double k = cyc_synch(B, nui, D.e[j], pref);
+                    if (emit) synj += k * s_f[j] * s_dge[j];
+                    if (D.ssa) asum += s_wa[j] * k;
+                }
+                if (emit && (!isfinite(synj) || synj < 0.)) st |= PBAGPU_S_NONFINITE;
             }
             S.jsyn[(size_t)q0 * nf + i] = synj;
+            if (D.ssa) {
+                asum *= -1. / (8. * kPi * kMe * nui * nui);
+                asum = fabs(asum);
+                if (!isfinite(asum)) asum = 1e99;
+                // total_rad.a = syn.a / n_ele * accel_frac * n_prime (microphysics.h:576-579)
+                D.aall[(size_t)it * nf + i] = asum / n_ele * acc_ssa * nprime_ssa;
+            }
         }
         if (D.fall)
             for (int r = 0; r < m; r++) {
@@ -558,10 +593,11 @@ static void build_grids(const pba_shock_params &s, GridSet &G) {
 }  // namespace pbagpu
 
 extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const pba_population *pops, int n_pop,
-                                 const double *d_table, int bw_total, int nt, double *d_jall, double *d_fall,
-                                 int *d_status, int precision, void *stream) {
+                                 const double *d_table, int bw_total, int nt, double *d_jall, double *d_aall,
+                                 double *d_fall, int *d_status, int precision, void *stream) {
     using namespace pbagpu::ele;
     if (!params || !pops || !d_table || !d_jall || !d_status || n_pop < 1 || nt < 1 || n_runs < 1) return PBAGPU_E_ARG;
+    if (pops[0].run < 0 || pops[0].run >= n_runs) return PBAGPU_E_ARG;
     if (precision != 0 && precision != 1) return PBAGPU_E_UNSUPPORTED;
     hipStream_t s = (hipStream_t)stream;
     const pba_shock_params &s0 = pops[0].shock ? params[pops[0].run].rs : params[pops[0].run].fs;
@@ -580,6 +616,10 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
             return PBAGPU_E_ARG;
         const pba_shock_params &sp = P.shock ? params[P.run].rs : params[P.run].fs;
         if (sp.ngam != ng || sp.nfreq != nf) return PBAGPU_E_ARG;
+        if (sp.ssa && !d_aall) {
+            pbagpu::set_error_msg("ele_evolve: a shock with ssa = 1 needs the absorption table d_aall");
+            return PBAGPU_E_ARG;
+        }
         size_t pos = 0;
         while (pos < keys.size() &&
                !(keys[pos].gam2 == sp.gam2 && keys[pos].freq1 == sp.freq1 && keys[pos].freq2 == sp.freq2))
@@ -655,16 +695,19 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
         D.K = (sp.ssc && need_ssc[kk]) ? dK + (ktab + (size_t)ng * nf) * ssc_slot[kk] : nullptr;
         D.KI = D.K ? D.K + ktab : nullptr;
         D.jall = d_jall + (size_t)q * nt * nf;
+        D.aall = sp.ssa ? d_aall + (size_t)q * nt * nf : nullptr;
+        D.ssa = sp.ssa ? 1 : 0;
         D.fall = d_fall ? d_fall + (size_t)q * nt * ng : nullptr;
         D.p = sp.p; D.eps_e = sp.eps_e; D.shock = P.shock; D.ng = ng; D.nf = nf;
         D.max_substeps = sp.max_substeps; D.adi_loss = sp.adi_loss_ele; D.ssc = D.K != nullptr;
     }
     PBA_TRY(rc, hipMemcpyAsync(dpops, hp.data(), sizeof(PopDesc) * n_pop, hipMemcpyHostToDevice, s));
     PBA_TRY(rc, hipMemsetAsync(d_jall, 0, sizeof(double) * (size_t)n_pop * nt * nf, s));
+    if (d_aall) PBA_TRY(rc, hipMemsetAsync(d_aall, 0, sizeof(double) * (size_t)n_pop * nt * nf, s));
     if (d_fall) PBA_TRY(rc, hipMemsetAsync(d_fall, 0, sizeof(double) * (size_t)n_pop * nt * ng, s));
     // per-population state between the per-step launches
     const size_t np = (size_t)n_pop;
-    const size_t n_dbl = np * ng * 2 + np * nf * (3 + SSC_JB) + np * 4;
+    const size_t n_dbl = np * ng * 3 + np * nf * (3 + SSC_JB) + np * 4;
     double *wsd = (double *)pbagpu::workspace(pbagpu::WS_ELE_WORK, sizeof(double) * n_dbl + sizeof(int) * np * 3);
     // populations of each Compton table, for the batched contraction
     std::vector<int> hlist;
@@ -694,7 +737,8 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
     PopState S;
     S.f = wsd; S.ft = S.f + np * ng; S.tf = S.ft + np * ng; S.wt = S.tf + np * nf;
     S.jsyn = S.wt + np * nf; S.jssc = S.jsyn + np * nf; S.scal = S.jssc + np * nf * SSC_JB;
-    S.flag = (int *)(S.scal + np * 4);
+    S.dfde = S.scal + np * 4;
+    S.flag = (int *)(S.dfde + np * ng);
     PBA_TRY(rc, hipMemsetAsync(wsd, 0, sizeof(double) * n_dbl + sizeof(int) * np * 3, s));
     if (dlist) PBA_TRY(rc, hipMemcpyAsync(dlist, hlist.data(), sizeof(int) * hlist.size(), hipMemcpyHostToDevice, s));
     const size_t ssc_lds = sizeof(double) * (size_t)nf * 64;

@@ -55,6 +55,52 @@ __device__ __forceinline__ double interp_log(double x, double xa, double xb, dou
     return ya * pow(yb / ya, log(x / xa) / log(xb / xa));
 }
 
+// computeIntensity with the smooth optical-depth rule (microphysics.h:11-58)
+__device__ __forceinline__ double intensity_smooth(double em, double dtau) {
+    if (dtau == 0) return em;
+    const double abs_fac = dtau > 0.0 ? -expm1(-dtau) / dtau : expm1(dtau) / dtau;
+    return em * abs_fac;
+}
+
+// computeFluxDensity (microphysics.h:799-867) for method_ne = useNe, method_shock_vel = sameAsBW,
+// method_tau = smooth: the luminosity of one shock from its comoving emissivity and the optical
+// depth along the line of sight.  `bad` is set where the reference stops (negative or non-finite
+// intensity or luminosity).
+__device__ __forceinline__ double shock_luminosity(double em_prime, double dtau, double Gamma, double beta,
+                                                   double delta_D, double mu, double ne, double omc, bool &bad) {
+    const double em_lab = em_prime / (delta_D * delta_D);
+    const double intensity = intensity_smooth(em_lab, dtau);
+    if (intensity < 0 || !isfinite(intensity)) bad = true;
+    const double lum = intensity * ne / (2. * kPi * omc) / Gamma / (1.0 - mu * beta);
+    if (!(lum >= 0) || !isfinite(lum)) bad = true;
+    return lum;
+}
+
+// bilinear interpolation of a [step][frequency] table at (freq, r) inside the cell
+// (ia, ib) x (ia_nu, ib_nu) (Interp2d::InterpolateBilinear, interpolators.h:724-808)
+__device__ __forceinline__ double table_bilinear(const double *tab, int nf, int ia, int ib, int ia_nu, int ib_nu,
+                                                 double x1, double x2, double y1, double y2, double freq, double r) {
+    const double q11 = tab[(size_t)ia * nf + ia_nu], q12 = tab[(size_t)ib * nf + ia_nu];
+    const double q21 = tab[(size_t)ia * nf + ib_nu], q22 = tab[(size_t)ib * nf + ib_nu];
+    const double x2x1 = x2 - x1, y2y1 = y2 - y1, x2x = x2 - freq, y2y = y2 - r, yy1 = r - y1, xx1 = freq - x1;
+    return 1.0 / (x2x1 * y2y1) * (q11 * x2x * y2y + q21 * xx1 * y2y + q12 * x2x * yy1 + q22 * xx1 * yy1);
+}
+
+// Optical depth of the shell with synchrotron self-absorption (computeFluxDensity,
+// microphysics.h:806-832): absorption interpolated from the table, boosted to the lab frame,
+// over the shell thickness of Johannesson et al. (2006, eq. 33).
+__device__ __forceinline__ double absorption_depth(const double *aall, int nf, int ia, int ib, int ia_nu,
+                                                       int ib_nu, double x1, double x2, double y1, double y2,
+                                                       double freq, double r, double Gamma, double beta,
+                                                       double delta_D, double mu, double ne, double n_prime,
+                                                       double omc) {
+    const double abs_prime = table_bilinear(aall, nf, ia, ib, ia_nu, ib_nu, x1, x2, y1, y2, freq, r);
+    const double abs_lab = abs_prime * delta_D;
+    const double m2 = ne * kMp;
+    const double dr = m2 / (2. * kPi * kMp * r * r * omc * Gamma * n_prime);
+    return abs_lab * dr * (1. - mu * beta) / (mu - beta);
+}
+
 // log grid by repeated multiplication (utils.h:91-105)
 __device__ __forceinline__ double logspace_at(double start, double ratio, int i) {
     double v = start;

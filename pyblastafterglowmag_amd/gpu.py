@@ -37,18 +37,18 @@ def lib():
         L.pbagpu_dyn_evolve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.c_int, C.POINTER(abi.LayerInit), C.c_int,
                                         vp, vp, C.POINTER(abi.BwResult), vp]
         L.pbagpu_ele_evolve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.Population), C.c_int, vp,
-                                        C.c_int, C.c_int, vp, vp, vp, C.c_int, vp]
+                                        C.c_int, C.c_int, vp, vp, vp, vp, C.c_int, vp]
         L.pbagpu_last_error.restype = C.c_char_p
         L.pbagpu_last_error.argtypes = []
         L.pbagpu_timing_enable.argtypes = [C.c_int]
         L.pbagpu_timing_read.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]
         L.pbagpu_timing_units.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int]
         L.pbagpu_eats_lightcurve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsLayer), C.c_int, vp,
-                                             C.c_int, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+                                             C.c_int, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.pbagpu_eats_lightcurve_pw.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsPwLayer), C.c_int,
-                                                vp, C.c_int, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+                                                vp, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.pbagpu_eats_skymap_pw.argtypes = [C.POINTER(abi.RunParams), C.POINTER(abi.EatsPwLayer), C.c_int, vp, C.c_int,
-                                            vp, C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp]
+                                            vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -198,6 +198,9 @@ class Pipeline:
         self.res = (abi.BwResult * nbw)()
         # at most two populations per blastwave
         self.d_jall = torch.empty((2 * nbw, self.nt, self.nf), dtype=torch.float64, device=self.dev)
+        # absorption tables only when a shock of the batch has synchrotron self-absorption
+        self.ssa = any(r.params.fs.ssa or r.params.rs.ssa for r in runs)
+        self.d_aall = torch.empty_like(self.d_jall) if self.ssa else None
         self.d_est = torch.zeros(2 * nbw, dtype=torch.int32, device=self.dev)
         self.d_tobs = torch.from_numpy(r0.tobs).to(self.dev)
         self.d_nuobs = torch.from_numpy(r0.nuobs).to(self.dev)
@@ -231,18 +234,21 @@ class Pipeline:
         self.d_fall = self.torch.empty((npop, self.nt, self.runs[0].params.fs.ngam), dtype=self.torch.float64,
                                        device=self.dev) if want_f else None
         _check(self.L.pbagpu_ele_evolve(self.params, self.n_runs, self.parr, npop, _ptr(self.d_table),
-                                        self.n_runs * self.bpr, self.nt, _ptr(self.d_jall), _ptr(self.d_fall),
+                                        self.n_runs * self.bpr, self.nt, _ptr(self.d_jall), _ptr(self.d_aall),
+                                        _ptr(self.d_fall),
                                         _ptr(self.d_est), self.precision, self.sp), "ele_evolve")
 
     def observer(self):
         if self.method == "piece-wise":
             _check(self.L.pbagpu_eats_lightcurve_pw(self.params, self.n_runs, self.larr, len(self.layers),
-                                                    _ptr(self.d_table), self.nt, _ptr(self.d_jall), self.nf,
+                                                    _ptr(self.d_table), self.nt, _ptr(self.d_jall), _ptr(self.d_aall),
+                                                    self.nf,
                                                     self.npairs, _ptr(self.d_tobs), _ptr(self.d_nuobs),
                                                     _ptr(self.d_out), _ptr(self.d_lst), self.sp), "eats_lightcurve_pw")
             return
         _check(self.L.pbagpu_eats_lightcurve(self.params, self.n_runs, self.larr, len(self.layers), _ptr(self.d_table),
-                                             self.nt, _ptr(self.d_jall), self.nf, self.npairs, _ptr(self.d_tobs),
+                                             self.nt, _ptr(self.d_jall), _ptr(self.d_aall), self.nf, self.npairs,
+                                             _ptr(self.d_tobs),
                                              _ptr(self.d_nuobs), _ptr(self.d_out), _ptr(self.d_lst), self.sp),
                "eats_lightcurve")
 
@@ -269,6 +275,7 @@ class Pipeline:
         d_flux = torch.empty((nl, n_img), dtype=torch.float64, device=self.dev)
         d_st = torch.zeros(nl, dtype=torch.int32, device=self.dev)
         _check(self.L.pbagpu_eats_skymap_pw(self.params, self.larr, nl, _ptr(self.d_table), self.nt, _ptr(self.d_jall),
+                                            _ptr(self.d_aall),
                                             self.nf, n_img, _ptr(d_t), _ptr(d_nu), 1 if remove_mu else 0,
                                             _ptr(d_img), _ptr(d_flux), _ptr(d_st), self.sp), "eats_skymap_pw")
         return d_img.cpu().numpy(), d_flux.cpu().numpy(), d_st.cpu().numpy()
@@ -300,6 +307,7 @@ def lightcurves(runs, stream=None, want_intermediates=False, precision=0):
     extras = dict(dyn_status=dst, ele_status=est, eats_status=lst)
     if want_intermediates:
         npop = len(pl.pops)
+        extras.update(aall=pl.d_aall[:npop].cpu().numpy() if pl.ssa else None)
         extras.update(table=pl.d_table.cpu().numpy(), jall=pl.d_jall[:npop].cpu().numpy(),
                       fall=pl.d_fall.cpu().numpy(), pops=pl.pops, layers=pl.layers,
                       results=[(r.status, r.i_end_r, r.fsrs, r.n_steps, r.n_rhs) for r in pl.res])

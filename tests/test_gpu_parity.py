@@ -30,7 +30,8 @@ def _gpu(run):
     return flux[0], ex
 
 
-@pytest.mark.parametrize("name", ["tophat_small", "gauss5_small", "tophat_pw_small", "gauss_pw_small"])
+@pytest.mark.parametrize("name", ["tophat_small", "gauss5_small", "tophat_pw_small", "gauss_pw_small",
+                                  "tophat_ssa_small", "gauss5_ssa_small"])
 def test_stages_match_oracle(name):
     g = load_golden(name)
     run = golden_run(g)
@@ -47,11 +48,14 @@ def test_stages_match_oracle(name):
             assert e <= tol, (name, il, n, e)
     # electron stage on the GPU's own dynamics table vs oracle on the same table
     pops = (abi.Population * len(ex["pops"]))(*ex["pops"])
-    jall_o, _, st = pyoracle.ele_evolve(run.params, pops, ex["table"])
+    jall_o, _, st, aall_o = pyoracle.ele_evolve(run.params, pops, ex["table"], want_a=True)
     assert not st.any()
     for q in range(len(ex["pops"])):
         e = rel_err(ex["jall"][q], jall_o[q], 1e-9)
         assert e <= 1e-9, (name, q, e)
+        if ex["aall"] is not None:   # synchrotron self-absorption
+            e = rel_err(ex["aall"][q], aall_o[q], 1e-9)
+            assert e <= 1e-9, (name, q, "absorption", e)
     # observer stage on the GPU's own tables vs oracle on the same tables
     if run.method == "piece-wise":
         lays = (abi.EatsPwLayer * len(ex["layers"]))()
@@ -59,13 +63,15 @@ def test_stages_match_oracle(name):
             li = run.inits[bw]
             lays[i] = abi.EatsPwLayer(li.ctheta0, li.ncells, abi.cells_in_layer(li.ilayer), bw, r, pfs,
                                       1 if prs >= 0 else 0, iend)
-        lc_o, st2 = pyoracle.eats_lightcurve_pw(run.params, lays, ex["table"], ex["jall"], run.tobs, run.nuobs)
+        lc_o, st2 = pyoracle.eats_lightcurve_pw(run.params, lays, ex["table"], ex["jall"], run.tobs, run.nuobs,
+                                                aall=ex["aall"])
     else:
         lays = (abi.EatsLayer * len(ex["layers"]))()
         for i, (bw, r, pfs, prs, iend) in enumerate(ex["layers"]):
             li = run.inits[bw]
             lays[i] = abi.EatsLayer(li.theta_c_l, li.theta_c_h, bw, r, pfs, prs, iend)
-        lc_o, st2 = pyoracle.eats_lightcurve(run.params, lays, ex["table"], ex["jall"], run.tobs, run.nuobs)
+        lc_o, st2 = pyoracle.eats_lightcurve(run.params, lays, ex["table"], ex["jall"], run.tobs, run.nuobs,
+                                             aall=ex["aall"])
     assert not st2.any()
     for il in range(run.n_layers):
         e = rel_err(flux[il], lc_o[il], 1e-30)
@@ -77,11 +83,11 @@ def test_stages_match_oracle(name):
 # 1 + 4 ulp, the GPU 1 ulp below), so the GPU keeps evolving Eint3 after that step and late
 # reverse-shock emission differs; the largest effect is 1.9e-6 at a pair 1e-6 below the peak.
 LC_TOL = {"tophat_small": 1e-6, "tophat_fs_syn": 1e-6, "gauss5_small": 5e-6, "tophat_pw_small": 1e-6,
-          "gauss_pw_small": 5e-6, "tophat64_pw": 1e-6}
+          "gauss_pw_small": 5e-6, "tophat64_pw": 1e-6, "tophat_ssa_small": 1e-6, "gauss5_ssa_small": 5e-6}
 
 
 @pytest.mark.parametrize("name", ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gauss_pw_small",
-                                  "tophat64_pw"])
+                                  "tophat64_pw", "tophat_ssa_small", "gauss5_ssa_small"])
 def test_lightcurve_matches_reference(name):
     """Light curves end to end against the reference program; tophat64_pw is the full-size piece-wise
     case (64 layers, 4096 cells, 96 x 128 pairs; the fixture holds every 7th layer)."""

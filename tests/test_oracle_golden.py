@@ -38,9 +38,9 @@ def _oracle_all(g):
     run = golden_run(g)
     table, res = pyoracle.dyn_evolve(run.params, run.inits, run.tgrid)
     pops, lays = _populations(run, res)
-    jall, fall, st = pyoracle.ele_evolve(run.params, pops, table, want_f=True)
+    jall, fall, st, aall = pyoracle.ele_evolve(run.params, pops, table, want_f=True, want_a=True)
     eats = pyoracle.eats_lightcurve_pw if run.method == "piece-wise" else pyoracle.eats_lightcurve
-    lc, st2 = eats(run.params, lays, table, jall, run.tobs, run.nuobs)
+    lc, st2 = eats(run.params, lays, table, jall, run.tobs, run.nuobs, aall=aall)
     assert all(r.status == 0 for r in res) and not st.any() and not st2.any()
     return run, table, res, pops, jall, fall, lc
 
@@ -50,6 +50,8 @@ def _oracle_all(g):
     ("gauss5_small", 1e-4, 1e-5, 1e-8),
     ("tophat_pw_small", 0.0, 0.0, 0.0),
     ("gauss_pw_small", 1e-4, 1e-5, 1e-8),
+    ("tophat_ssa_small", 0.0, 0.0, 0.0),
+    ("gauss5_ssa_small", 1e-4, 1e-5, 1e-8),
 ])
 def test_oracle_matches_reference(name, tol_dyn, tol_j, tol_lc):
     g = load_golden(name)
