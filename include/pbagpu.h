@@ -74,6 +74,9 @@ typedef struct {
     int ssc;              /* method_ssc = numeric */
     int adi_loss_ele;     /* num_ele_use_adi_loss */
     int ssa;              /* use_ssa: synchrotron self-absorption (computeSSA, numeric_model.h:190-238) */
+    int syn;              /* method_synchrotron: 0 = CSYN (SynKernel::cycSynch, kernels.h:414-437),
+                             1 = Dermer09 (SynKernel::synchDermer, kernels.h:451-473) */
+    int pad_;
 } pba_shock_params;
 
 /* One afterglow model (all its blastwaves share these); parfile main + GRB sections. */

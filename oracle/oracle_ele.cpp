@@ -29,6 +29,19 @@
 
 namespace orc {
 
+// SynKernel::synchDermer (kernels.h:451-473)
+double synch_dermer(double B, double nu, double gam) {
+    double nu_crit = (3. * qe * B * gam * gam) / (4. * pi * me * c);
+    double x = nu / nu_crit;
+    double t1n = 1.808 * std::pow(x, 1. / 3.);
+    double t1d = std::sqrt(1 + 3.4 * std::pow(x, 2. / 3.));
+    double t2n = 1. + 2.21 * std::pow(x, 2. / 3.) + 0.347 * std::pow(x, 4. / 3.);
+    double t2d = 1. + 1.353 * std::pow(x, 2. / 3.) + 0.217 * std::pow(x, 4. / 3.);
+    double res = t1n / t1d * t2n / t2d * std::exp(-x);
+    res *= std::sqrt(3.) * std::pow(qe, 3.) * B / (me * c * c);
+    return res;
+}
+
 // SynKernel::cycSynch (kernels.h:414-437)
 double cyc_synch(double B, double nu, double gam) {
     double em;
@@ -223,7 +236,8 @@ static void evolve_population(const pba_shock_params &sp, const SSCTab *ssc, int
         }
         // computeRadiationNumericMixed
         for (int i = 0; i < nf; i++)
-            for (int j = 0; j < ng; j++) K[(size_t)i * ng + j] = cyc_synch(B, nu[i], g.e[j]);
+            for (int j = 0; j < ng; j++)
+                K[(size_t)i * ng + j] = sp.syn == 1 ? synch_dermer(B, nu[i], g.e[j]) : cyc_synch(B, nu[i], g.e[j]);
         std::fill(synj.begin(), synj.end(), 0.);
         for (int i = 0; i < nf - 1; i++) {
             double res = 0.;

@@ -34,7 +34,8 @@ class ShockParams(C.Structure):
     _fields_ = [("eps_e", C.c_double), ("eps_b", C.c_double), ("eps_t", C.c_double), ("p", C.c_double),
                 ("gam2", C.c_double), ("freq1", C.c_double), ("freq2", C.c_double), ("beta_min", C.c_double),
                 ("epsilon_e_rad", C.c_double), ("ngam", C.c_int), ("nfreq", C.c_int),
-                ("max_substeps", C.c_int), ("ssc", C.c_int), ("adi_loss_ele", C.c_int), ("ssa", C.c_int)]
+                ("max_substeps", C.c_int), ("ssc", C.c_int), ("adi_loss_ele", C.c_int), ("ssa", C.c_int),
+                ("syn", C.c_int), ("pad_", C.c_int)]
 
 
 class RunParams(C.Structure):
@@ -89,6 +90,10 @@ def _yes(v):
     return 1 if v == "yes" else 0
 
 
+# method_synchrotron values of the numeric electron path and their code in pba_shock_params.syn
+SYNCHROTRON_KERNELS = {"CSYN": 0, "Dermer09": 1}
+
+
 def _shock(pars, opts, sh):
     s = ShockParams()
     s.eps_e = pars[f"eps_e_{sh}"]
@@ -106,7 +111,11 @@ def _shock(pars, opts, sh):
     s.ssc = 1 if opts.get(f"method_ssc_{sh}", "none") == "numeric" else 0
     s.adi_loss_ele = _yes(opts.get(f"num_ele_use_adi_loss_{sh}", "yes"))
     s.ssa = _yes(opts.get(f"use_ssa_{sh}", "no"))
-    for key, want in ((f"method_ele_{sh}", "numeric"), (f"method_synchrotron_{sh}", "CSYN"),
+    syn = opts.get(f"method_synchrotron_{sh}", "CSYN")
+    if syn not in SYNCHROTRON_KERNELS:
+        raise NotImplementedError(f"method_synchrotron_{sh}={syn} (only {' / '.join(SYNCHROTRON_KERNELS)} is implemented)")
+    s.syn = SYNCHROTRON_KERNELS[syn]
+    for key, want in ((f"method_ele_{sh}", "numeric"),
                       (f"method_pp_{sh}", "none"), (f"method_ne_{sh}", "useNe"),
                       (f"method_gamma_min_{sh}", "useNumeric"), (f"method_gamma_c_{sh}", "useTcomov"),
                       (f"method_gamma_max_{sh}", "useB"), (f"method_B_{sh}", "useU_b"),

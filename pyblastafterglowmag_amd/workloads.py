@@ -133,6 +133,9 @@ SMALL = {
                              use_ssa_fs="yes"),
     "gauss5_ssa_small": dict(base="gauss21_fsrs_ssc", ntb=1000, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
                              n_layers=5, use_ssa_fs="yes", use_ssa_rs="yes"),
+    # the Dermer (2009) synchrotron kernel in place of CSYN
+    "tophat_dermer_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
+                                method_synchrotron_fs="Dermer09"),
 }
 
 
