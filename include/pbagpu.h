@@ -95,7 +95,9 @@ typedef struct {
     int exponential_rho4, init_deltaR4;
     int counter_jet;
     int nmax;                     /* ODE step budget per output interval (0: unlimited) */
-    int pad_;
+    int integrator;               /* 0: DOP853E, adaptive (IntegratorEmbedded<dop853_integrator>);
+                                     1: RK4, one classical step per output interval
+                                     (IntegratorStatic<rk4_integrator>, ode_solvers.h:319-366, 2099-2118) */
     pba_shock_params fs, rs;
 } pba_run_params;
 

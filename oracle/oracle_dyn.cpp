@@ -765,11 +765,30 @@ static void evolve_run(const pba_run_params *P, const pba_layer_init *init, int 
     for (int i = 0; i < nbw; i++)
         post_step(run, run.bws[i], &Y[(size_t)i * NEQ], 0, tgrid[0], table + (size_t)i * PBA_NV * nt, nt);
     Dop d(neq);
+    std::vector<double> kn(neq, 0.), xn(neq, 0.), dl(neq, 0.);
     double x = P->tb0;
     for (int it = 1; it < nt; it++) {
         double dt = tgrid[it] - tgrid[it - 1];
-        if (getenv("ORC_DYN_TRACE")) fprintf(stderr, "interval %d\n", it);
-        int rc = dop853(run, d, x, Y.data(), x + dt, P->rtol, P->rtol, nsteps);
+        int rc = 0;
+        if (P->integrator == 1) {
+            // IntegratorStatic<rk4_integrator>: one classical RK4 step over the interval (ode_solvers.h:319-366)
+            const double h = dt;
+            rhs_all(run, x, Y.data(), kn.data());
+            for (size_t i = 0; i < neq; i++) { dl[i] = (1.0 / 6.0) * h * kn[i]; xn[i] = Y[i] + 0.5 * h * kn[i]; }
+            rhs_all(run, x + 0.5 * h, xn.data(), kn.data());
+            for (size_t i = 0; i < neq; i++) { dl[i] += (2.0 / 6.0) * h * kn[i]; xn[i] = Y[i] + 0.5 * h * kn[i]; }
+            rhs_all(run, x + 0.5 * h, xn.data(), kn.data());
+            for (size_t i = 0; i < neq; i++) { dl[i] += (2.0 / 6.0) * h * kn[i]; xn[i] = Y[i] + h * kn[i]; }
+            rhs_all(run, x + h, xn.data(), kn.data());
+            for (size_t i = 0; i < neq; i++) {
+                const double a = dl[i] + (1.0 / 6.0) * h * kn[i];
+                Y[i] += a;
+            }
+            x += h;
+            nsteps++;
+        } else {
+            rc = dop853(run, d, x, Y.data(), x + dt, P->rtol, P->rtol, nsteps);
+        }
         for (int i = 0; i < nbw; i++) {
             BW &b = run.bws[i];
             if (rc != 0) b.status |= PBAGPU_S_NMAX;

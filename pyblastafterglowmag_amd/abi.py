@@ -47,7 +47,7 @@ class RunParams(C.Structure):
                 ("nmax_theta", C.c_int), ("nmax_phi", C.c_int), ("fsrs", C.c_int), ("do_rs", C.c_int),
                 ("do_rs_radiation", C.c_int), ("adiab_loss", C.c_int), ("adiab_loss_rs", C.c_int),
                 ("exponential_rho4", C.c_int), ("init_deltaR4", C.c_int), ("counter_jet", C.c_int),
-                ("nmax", C.c_int), ("pad_", C.c_int), ("fs", ShockParams), ("rs", ShockParams)]
+                ("nmax", C.c_int), ("integrator", C.c_int), ("fs", ShockParams), ("rs", ShockParams)]
 
 
 class LayerInit(C.Structure):
@@ -77,10 +77,11 @@ class EatsLayer(C.Structure):
 
 
 EATS_METHODS = ("adaptive", "piece-wise")
+INTEGRATORS = ("DOP853E", "RK4")   # pba_run_params.integrator 0 / 1
 SUPPORTED_OPTS = dict(
     method_eats=EATS_METHODS, method_quad="CADRE", method_comp_mode="comovSpec", method_spread="our",
     method_limit_spread="MomValAndFrac", method_dgdr="our", method_eos="Nava13", method_dmdr="usingdthdr",
-    integrator="DOP853E", do_average_solution="no", lc_use_freq_to_time="no", ebl_tbl_fpath="none",
+    integrator=INTEGRATORS, do_average_solution="no", lc_use_freq_to_time="no", ebl_tbl_fpath="none",
 )
 
 
@@ -152,6 +153,7 @@ def run_params(mp_, mo, gp, go):
     P.rtol = mp_["rtol"]
     P.tb0 = mp_["tb0"]
     P.nmax = int(mp_.get("nmax", 100000))
+    P.integrator = INTEGRATORS.index(mo.get("integrator", "DOP853E"))
     P.tprompt = gp.get("tprompt", 1000.)
     P.rs_shutoff_rho = gp.get("rs_shutOff_criterion_rho", 1e-50)
     P.min_gamma0_for_rs = gp.get("min_Gamma0_for_rs", 0.)

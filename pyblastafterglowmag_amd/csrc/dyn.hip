@@ -542,6 +542,26 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
     for (int it = 1; it < nt; it++) {
         const double timeN = time + (tgrid[it] - tgrid[it - 1]);
         const double hmax = fabs(timeN - time);
+        if (P.integrator == 1) {
+            // RK4: one classical step over the output interval (rk4_integrator, ode_solvers.h:319-366),
+            // every blastwave on its own (the step has no error control that couples them)
+            const double h = tgrid[it] - tgrid[it - 1];
+            double *kn = k1, *dl = k6;
+            rhs(P, shfs, b, active, Y, kn);
+            for (int k = 0; k < NA; k++) { dl[k] = (1.0 / 6.0) * h * kn[k]; x[k] = Y[k] + 0.5 * h * kn[k]; }
+            rhs(P, shfs, b, active, x, kn);
+            for (int k = 0; k < NA; k++) { dl[k] += (2.0 / 6.0) * h * kn[k]; x[k] = Y[k] + 0.5 * h * kn[k]; }
+            rhs(P, shfs, b, active, x, kn);
+            for (int k = 0; k < NA; k++) { dl[k] += (2.0 / 6.0) * h * kn[k]; x[k] = Y[k] + h * kn[k]; }
+            rhs(P, shfs, b, active, x, kn);
+            for (int k = 0; k < NA; k++) {
+                const double a = dl[k] + (1.0 / 6.0) * h * kn[k];
+                Y[k] += a;
+            }
+            time += h;
+            nrhs += 4;
+            nsteps++;
+        } else {
         rhs(P, shfs, b, active, Y, k1); nrhs++;
         // initial step (ode_solvers.h:1596-1628)
         double h;
@@ -651,6 +671,7 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
             }
             if (guard == kMaxStepsPerInterval - 1) b.status |= PBAGPU_S_NMAX;
         }
+        }   // DOP853E
         if (active && !b.end_evolution) {
             double g = Y[iGamma], be = beta_from_gamma(g), E2 = Y[iEint2];
             if ((E2 <= 0.) || !isfinite(E2) || (g < 1.) || (be < 0) || (fabs(be) < 1.e-6) || !isfinite(be))

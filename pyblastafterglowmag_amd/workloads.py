@@ -107,7 +107,7 @@ def scaled(name, ntb=None, ngam=None, nfreq=None, n_nu_obs=None, n_t_obs=None, n
     for k, v in over.items():
         if k in struct or k == "n_layers_pw":
             struct[k] = v
-        elif k in ("tb0", "tb1", "ntb", "n_ism", "d_l", "z", "theta_obs", "lc_freqs", "lc_times", "rtol"):
+        elif k in ("tb0", "tb1", "ntb", "n_ism", "d_l", "z", "theta_obs", "lc_freqs", "lc_times", "rtol", "integrator"):
             main[k] = v
         else:
             grb[k] = v
@@ -136,6 +136,9 @@ SMALL = {
     # the Dermer (2009) synchrotron kernel in place of CSYN
     "tophat_dermer_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
                                 method_synchrotron_fs="Dermer09"),
+    # the classical RK4 integrator, one step per output interval, in place of adaptive DOP853
+    "tophat_rk4_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
+                             integrator="RK4"),
 }
 
 
