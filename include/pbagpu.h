@@ -162,7 +162,8 @@ enum {
     PBAGPU_STAGE_EATS = 2,     /* eats_kernel */
     PBAGPU_STAGE_SSC_TABLE = 3,/* Compton kernel table build */
     PBAGPU_STAGE_EATS_PW = 4,  /* eats_pw_kernel (piece-wise observer stage) */
-    PBAGPU_N_STAGES = 5
+    PBAGPU_STAGE_SKY_HIST = 5, /* sky_bin_kernel (sky-map histogram) */
+    PBAGPU_N_STAGES = 6
 };
 int pbagpu_timing_enable(int on);
 int pbagpu_timing_read(int stage, double *ms_total, long long *launches, int reset);
@@ -285,6 +286,52 @@ int pbagpu_eats_skymap_pw(const pba_run_params *params, const pba_eats_pw_layer 
                           const double *d_table, int nt, const double *d_jall, const double *d_aall, int nfreq,
                           int n_images, const double *d_tobs, const double *d_nuobs, int remove_mu,
                           double *d_image, double *d_flux, int *d_status, void *stream);
+
+/* ------------------------------------------------- sky-map binning */
+/* Histogram image, flux centroid and data extent of raw sky maps: the binning part
+ * of ProcessRawSkymap._process_file (package/src/PyBlastAfterglowMag/skymap_process.py:465-485),
+ * which calls
+ *   _compute_position_of_the_flux_centroid (:98-105): np.average of x and y weighted by I,
+ *   _get_data_extend (:107-189): min / max of x and y over the non-empty entries,
+ *   np.linspace(min * extend_grid, max * extend_grid, num=nx) for the edges (:480-481),
+ *   _total_hist_skymaps (:371-383): np.histogramdd of (x, y) weighted by I.
+ * The scattered-data interpolation (scipy griddata), the smoothing filters and the
+ * FWHM search of that routine are not part of this entry.
+ *
+ * An image is a list of entries (ProcessRawSkymap._get_skymap's per-shell principal-
+ * and counter-jet arrays, :191-328); entry k holds points [seg_off[k], seg_off[k+1])
+ * of d_x / d_y / d_w, and image i holds entries [img_seg[i], img_seg[i+1]).
+ *   img_seg   [n_images+1] host, img_seg[0] = 0, non-decreasing
+ *   seg_off   [img_seg[n_images]+1] host, seg_off[0] = 0, non-decreasing
+ *   d_x, d_y, d_w [seg_off[last]] device: sky positions (mas) and intensities
+ *   edges_given  0: the edges are computed from the data extent and written to
+ *                d_edges_x / d_edges_y; 1: they are read from there (must increase)
+ *   d_edges_x [n_images][nx], d_edges_y [n_images][ny] device
+ *   d_hist    [n_images][nx-1][ny-1] device, written (x bin major, as np.histogramdd)
+ *   d_count   [n_images][nx-1][ny-1] device or NULL: points per bin
+ *   d_moments [n_images][PBA_SKY_NM] device, written
+ *   d_status  [n_images] device, written: PBA_SKY_* (an image with a non-zero status
+ *             keeps an all-zero histogram; the reference raises an exception there) */
+enum pba_sky_moment {
+    PBA_SKY_XC = 0, PBA_SKY_YC, PBA_SKY_XMIN, PBA_SKY_XMAX, PBA_SKY_YMIN, PBA_SKY_YMAX,
+    PBA_SKY_SUMW, PBA_SKY_NPTS,
+    PBA_SKY_NM
+};
+enum pba_sky_status {
+    PBA_SKY_OK = 0,
+    PBA_SKY_NO_DATA = 1,      /* edges from the extent, but no entry has two points and a non-zero
+                                 intensity sum */
+    PBA_SKY_ZERO_WEIGHT = 2,  /* intensities sum to zero: the centroid is undefined */
+    PBA_SKY_FLAT = 3,         /* edges from the extent, and an entry has xmin == xmax or
+                                 ymin == ymax (:157-160) */
+    PBA_SKY_NONFINITE = 4,    /* a non-finite position or intensity */
+    PBA_SKY_BAD_EDGES = 5     /* given edges that decrease somewhere */
+};
+int pbagpu_skymap_hist(int n_images, const int *img_seg, const long *seg_off,
+                       const double *d_x, const double *d_y, const double *d_w,
+                       int nx, int ny, double extend_grid, int edges_given,
+                       double *d_edges_x, double *d_edges_y, double *d_hist, unsigned *d_count,
+                       double *d_moments, int *d_status, void *stream);
 
 #ifdef __cplusplus
 }

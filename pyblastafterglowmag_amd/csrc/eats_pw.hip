@@ -62,10 +62,14 @@ struct Lane {
     __device__ double V(int c, int k) const { return lds_pw[c * nt + k]; }
     __device__ double nu(int i) const { return lds_pw[NCOL * nt + i]; }
     // arrival time of step k for a cell with 1 - mu = omu; the steps past i_end_r hold
-    // DBL_MAX in the reference's vector.  Product and sum are rounded separately, as there.
+    // DBL_MAX in the reference's vector.  Product and sum are rounded separately, as there
+    // (hipcc contracts across statements by default, and __dadd_rn / __dmul_rn are plain
+    // operators in its headers: only the pragma keeps the multiply-add unfused).
     __device__ double ttobs(int k, double omu) const {
+#pragma clang fp contract(off)
         if (k >= N) return DBL_MAX;
-        return __dadd_rn(V(C_tt, k), __dmul_rn(V(C_R, k) / kC, omu));
+        const double d = V(C_R, k) / kC * omu;
+        return V(C_tt, k) + d;
     }
     // findClosestIndex over the nt-long arrival-time vector
     __device__ int closest(double t, double omu, int guess) const {

@@ -172,6 +172,15 @@ int h5io_add_dataset(long file, const char *group, const char *name, long n, con
     return rc;
 }
 
+/* [n0][n1] row-major dataset in a group (created on first use) */
+int h5io_add_dataset2d(long file, const char *group, const char *name, int n0, int n1, const double *data) {
+    hid_t g = group_of((hid_t)file, group);
+    if (g < 0) return -1;
+    int rc = write_2d(g, name, n0, n1, data);
+    H5Gclose(g);
+    return rc;
+}
+
 int h5io_add_attr(long file, const char *group, const char *name, double v) {
     hid_t g = group_of((hid_t)file, group);
     if (g < 0) return -1;

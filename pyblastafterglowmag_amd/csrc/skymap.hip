@@ -1,0 +1,248 @@
+// Sky-map binning: intensity-weighted 2-D histogram, flux centroid and data extent of raw
+// sky maps (ProcessRawSkymap._process_file, package/src/PyBlastAfterglowMag/skymap_process.py:465-485).
+//
+// Three launches on the caller's stream:
+//   sky_zero_kernel    clears the histogram (and the optional point counts) with vector stores;
+//   sky_extent_kernel  one workgroup per image: per-entry extent, centroid sums, edges, status;
+//   sky_bin_kernel     a grid over every image's points: bin search in LDS-staged edges and an
+//                      atomic add of the intensity into the image's histogram.
+// The work is HBM / atomic bound and tiny next to the observer stage (one 256 x 256 image of a
+// 64-layer piece-wise jet is 8192 points); it is kept on the device so that sky maps made by
+// pbagpu_eats_skymap_pw never have to leave HBM to be binned.
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <algorithm>
+#include <math.h>
+#include <vector>
+#include "runtime.h"
+#include "timing.h"
+
+namespace pbagpu {
+namespace skybin {
+
+constexpr int BLOCK = 256;
+constexpr int MAX_EDGES = 4096;   // nx + ny edges staged in LDS by the binning kernel (32 KiB)
+
+// Workgroup-wide reductions in LDS (fixed tree: the result does not depend on scheduling).
+template <typename Op>
+__device__ double block_reduce(double v, double *red, Op op) {
+    const int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] = op(red[t], red[t + s]);
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+struct Add { __device__ double operator()(double a, double b) const { return a + b; } };
+// NaN never wins a comparison, so non-finite positions are screened separately.
+struct Min { __device__ double operator()(double a, double b) const { return b < a ? b : a; } };
+struct Max { __device__ double operator()(double a, double b) const { return b > a ? b : a; } };
+
+// np.linspace(lo * scale, hi * scale, num) with num >= 2 (numpy/_core/function_base.py): step = (stop - start)
+// / (num - 1); e_i = i * step + start as two roundings (no fused multiply-add: the pragma keeps
+// hipcc from contracting them), e_last = stop; a zero step (denormal range) takes
+// e_i = (i / div) * delta + start instead.
+__device__ void linspace(double lo, double hi, double scale, int num, double *out) {
+#pragma clang fp contract(off)
+    const double start = lo * scale, stop = hi * scale;
+    const double div = (double)(num - 1);
+    const double delta = stop - start;
+    const double step = delta / div;
+    for (int i = threadIdx.x; i < num; i += BLOCK) {
+        double y;
+        if (step == 0.0) y = ((double)i / div) * delta;
+        else y = (double)i * step;
+        y = y + start;
+        out[i] = (i == num - 1) ? stop : y;
+    }
+}
+
+__global__ void sky_zero_kernel(double *hist, unsigned *count, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        hist[i] = 0.0;
+        if (count) count[i] = 0u;
+    }
+}
+
+// One workgroup per image.
+//  * _get_data_extend (skymap_process.py:107-189): an entry counts when it has at least two
+//    points and its intensities do not sum to zero; the image extent is the min / max of x and
+//    y over the points of those entries; an entry with min == max in x or y is an error there.
+//  * _compute_position_of_the_flux_centroid (:98-105): sum(w x) / sum(w) over every point.
+//  * edges = np.linspace(xmin * extend_grid, xmax * extend_grid, nx) (:480-481) unless given.
+__global__ __launch_bounds__(BLOCK) void sky_extent_kernel(const int *img_seg, const long *seg_off, const double *x,
+                                                           const double *y, const double *w, int nx, int ny,
+                                                           double extend_grid, int edges_given, double *edges_x,
+                                                           double *edges_y, double *mom, int *status) {
+    __shared__ double red[BLOCK];
+    __shared__ int s_bad, s_dec;   // set before different barriers: one flag each
+    const int img = blockIdx.x;
+    const int t = threadIdx.x;
+    if (t == 0) { s_bad = 0; s_dec = 0; }
+    __syncthreads();
+    double sxw = 0.0, syw = 0.0, sw = 0.0;
+    double xmin = DBL_MAX, xmax = -DBL_MAX, ymin = DBL_MAX, ymax = -DBL_MAX;
+    int kept = 0, flat = 0;
+    const long p0 = seg_off[img_seg[img]], p1 = seg_off[img_seg[img + 1]];
+    for (int k = img_seg[img]; k < img_seg[img + 1]; k++) {
+        const long b = seg_off[k], e = seg_off[k + 1];
+        double ex0 = DBL_MAX, ex1 = -DBL_MAX, ey0 = DBL_MAX, ey1 = -DBL_MAX, es = 0.0;
+        int bad = 0;
+        for (long i = b + t; i < e; i += BLOCK) {
+            const double xi = x[i], yi = y[i], wi = w[i];
+            if (!isfinite(xi) || !isfinite(yi) || !isfinite(wi)) bad = 1;
+            ex0 = fmin(ex0, xi); ex1 = fmax(ex1, xi);
+            ey0 = fmin(ey0, yi); ey1 = fmax(ey1, yi);
+            es += wi;
+            sxw += wi * xi; syw += wi * yi; sw += wi;
+        }
+        if (bad) atomicOr(&s_bad, 1);
+        ex0 = block_reduce(ex0, red, Min()); ex1 = block_reduce(ex1, red, Max());
+        ey0 = block_reduce(ey0, red, Min()); ey1 = block_reduce(ey1, red, Max());
+        es = block_reduce(es, red, Add());
+        if (e - b >= 2 && es != 0.0) {
+            kept++;
+            if (ex0 == ex1 || ey0 == ey1) flat = 1;
+            xmin = fmin(xmin, ex0); xmax = fmax(xmax, ex1);
+            ymin = fmin(ymin, ey0); ymax = fmax(ymax, ey1);
+        }
+    }
+    sxw = block_reduce(sxw, red, Add());
+    syw = block_reduce(syw, red, Add());
+    sw = block_reduce(sw, red, Add());
+    // every value below is uniform over the workgroup (reduced, or read after the last barrier)
+    int st = PBA_SKY_OK;
+    if (s_bad) st = PBA_SKY_NONFINITE;
+    else if (sw == 0.0) st = PBA_SKY_ZERO_WEIGHT;   // the centroid is computed first there
+    // the extent's checks run only where the extent makes the edges
+    else if (!edges_given && kept == 0) st = PBA_SKY_NO_DATA;
+    else if (!edges_given && flat) st = PBA_SKY_FLAT;
+    double *ex = edges_x + (size_t)img * nx, *ey = edges_y + (size_t)img * ny;
+    if (st == PBA_SKY_OK && !edges_given) {
+        linspace(xmin, xmax, extend_grid, nx, ex);
+        linspace(ymin, ymax, extend_grid, ny, ey);
+    }
+    if (st == PBA_SKY_OK && edges_given) {
+        int dec = 0;
+        for (int i = t; i + 1 < nx; i += BLOCK) dec |= ex[i] > ex[i + 1] || !isfinite(ex[i]) || !isfinite(ex[i + 1]);
+        for (int i = t; i + 1 < ny; i += BLOCK) dec |= ey[i] > ey[i + 1] || !isfinite(ey[i]) || !isfinite(ey[i + 1]);
+        if (dec) atomicOr(&s_dec, 1);
+        __syncthreads();
+        if (s_dec) st = PBA_SKY_BAD_EDGES;
+    }
+    if (t == 0) {
+        double *m = mom + (size_t)img * PBA_SKY_NM;
+        m[PBA_SKY_XC] = sw != 0.0 ? sxw / sw : 0.0;
+        m[PBA_SKY_YC] = sw != 0.0 ? syw / sw : 0.0;
+        m[PBA_SKY_XMIN] = kept ? xmin : 0.0; m[PBA_SKY_XMAX] = kept ? xmax : 0.0;
+        m[PBA_SKY_YMIN] = kept ? ymin : 0.0; m[PBA_SKY_YMAX] = kept ? ymax : 0.0;
+        m[PBA_SKY_SUMW] = sw;
+        m[PBA_SKY_NPTS] = (double)(p1 - p0);
+        status[img] = st;
+    }
+}
+
+// Number of edges <= v (np.searchsorted(edges, v, side='right')); NaN gives 0.
+__device__ inline int upper_count(const double *e, int n, double v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (e[mid] <= v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// np.histogramdd((x, y), bins=(edges_x, edges_y), weights=w) (numpy/lib/_histograms_impl.py):
+// bin index = searchsorted(edges, v, 'right'), minus one for a value equal to the last edge;
+// indices 0 and len(edges) are outliers.  grid = (chunks, n_images).
+__global__ __launch_bounds__(BLOCK) void sky_bin_kernel(const int *img_seg, const long *seg_off, const double *x,
+                                                        const double *y, const double *w, int nx, int ny,
+                                                        const double *edges_x, const double *edges_y,
+                                                        const int *status, double *hist, unsigned *count) {
+    extern __shared__ double s_e[];
+    const int img = blockIdx.y;
+    if (status[img] != PBA_SKY_OK) return;   // uniform over the workgroup
+    double *sx = s_e, *sy = s_e + nx;
+    for (int i = threadIdx.x; i < nx; i += BLOCK) sx[i] = edges_x[(size_t)img * nx + i];
+    for (int i = threadIdx.x; i < ny; i += BLOCK) sy[i] = edges_y[(size_t)img * ny + i];
+    __syncthreads();
+    const long p0 = seg_off[img_seg[img]], p1 = seg_off[img_seg[img + 1]];
+    const double xl = sx[nx - 1], yl = sy[ny - 1];
+    const size_t nb = (size_t)(nx - 1) * (ny - 1);
+    double *H = hist + (size_t)img * nb;
+    unsigned *Cn = count ? count + (size_t)img * nb : nullptr;
+    for (long i = p0 + (long)blockIdx.x * BLOCK + threadIdx.x; i < p1; i += (long)gridDim.x * BLOCK) {
+        const double xi = x[i], yi = y[i];
+        int ix = upper_count(sx, nx, xi), iy = upper_count(sy, ny, yi);
+        if (xi == xl) ix--;
+        if (yi == yl) iy--;
+        if (ix < 1 || ix > nx - 1 || iy < 1 || iy > ny - 1) continue;
+        const size_t b = (size_t)(ix - 1) * (ny - 1) + (iy - 1);
+        atomicAdd(H + b, w[i]);
+        if (Cn) atomicAdd(Cn + b, 1u);
+    }
+}
+
+}  // namespace skybin
+}  // namespace pbagpu
+
+using namespace pbagpu;
+
+extern "C" int pbagpu_skymap_hist(int n_images, const int *img_seg, const long *seg_off, const double *d_x,
+                                  const double *d_y, const double *d_w, int nx, int ny, double extend_grid,
+                                  int edges_given, double *d_edges_x, double *d_edges_y, double *d_hist,
+                                  unsigned *d_count, double *d_moments, int *d_status, void *stream) {
+    using namespace skybin;
+    hipStream_t s = (hipStream_t)stream;
+    if (n_images < 1 || n_images > 65535 || !img_seg || !seg_off || !d_edges_x || !d_edges_y || !d_hist ||
+        !d_moments || !d_status || nx < 2 || ny < 2 || nx + ny > MAX_EDGES) {
+        set_error_msg("skymap_hist: bad arguments (1 <= n_images <= 65535, nx, ny >= 2, nx + ny <= 4096)");
+        return PBAGPU_E_ARG;
+    }
+    if (img_seg[0] != 0) { set_error_msg("skymap_hist: img_seg[0] must be 0"); return PBAGPU_E_ARG; }
+    for (int i = 0; i < n_images; i++)
+        if (img_seg[i + 1] < img_seg[i]) { set_error_msg("skymap_hist: img_seg decreases"); return PBAGPU_E_ARG; }
+    const int n_segs = img_seg[n_images];
+    if (seg_off[0] != 0) { set_error_msg("skymap_hist: seg_off[0] must be 0"); return PBAGPU_E_ARG; }
+    for (int k = 0; k < n_segs; k++)
+        if (seg_off[k + 1] < seg_off[k]) { set_error_msg("skymap_hist: seg_off decreases"); return PBAGPU_E_ARG; }
+    const long n_pts = seg_off[n_segs];
+    if (n_pts > 0 && (!d_x || !d_y || !d_w)) return PBAGPU_E_ARG;
+    if (!(extend_grid == extend_grid)) { set_error_msg("skymap_hist: extend_grid is NaN"); return PBAGPU_E_ARG; }
+    // points of the largest image: sizes the binning grid
+    long max_pts = 0;
+    for (int i = 0; i < n_images; i++) {
+        const long np_i = seg_off[img_seg[i + 1]] - seg_off[img_seg[i]];
+        if (np_i > max_pts) max_pts = np_i;
+    }
+    auto *dseg = (int *)workspace(WS_SKY_SEG, sizeof(int) * (n_images + 1));
+    auto *doff = (long *)workspace(WS_SKY_OFF, sizeof(long) * (n_segs + 1));
+    if (!dseg || !doff) return PBAGPU_E_HIP;
+    int rc = PBAGPU_OK;
+    PBA_TRY(rc, hipMemcpyAsync(dseg, img_seg, sizeof(int) * (n_images + 1), hipMemcpyHostToDevice, s));
+    PBA_TRY(rc, hipMemcpyAsync(doff, seg_off, sizeof(long) * (n_segs + 1), hipMemcpyHostToDevice, s));
+    if (rc) return rc;
+    const size_t nb = (size_t)n_images * (nx - 1) * (ny - 1);
+    const unsigned zgrid = (unsigned)std::min<size_t>((nb + BLOCK - 1) / BLOCK, 4096);
+    KernelTimer timer(PBAGPU_STAGE_SKY_HIST);
+    timer.start(s);
+    hipLaunchKernelGGL(sky_zero_kernel, dim3(zgrid), dim3(BLOCK), 0, s, d_hist, d_count, nb);
+    hipLaunchKernelGGL(sky_extent_kernel, dim3(n_images), dim3(BLOCK), 0, s, dseg, doff, d_x, d_y, d_w, nx, ny,
+                       extend_grid, edges_given, d_edges_x, d_edges_y, d_moments, d_status);
+    if (max_pts > 0) {
+        const unsigned chunks = (unsigned)std::min<long>((max_pts + BLOCK - 1) / BLOCK, 1024);
+        hipLaunchKernelGGL(sky_bin_kernel, dim3(chunks, n_images), dim3(BLOCK), sizeof(double) * (nx + ny), s, dseg,
+                           doff, d_x, d_y, d_w, nx, ny, d_edges_x, d_edges_y, d_status, d_hist, d_count);
+    }
+    timer.stop(s);
+    PBA_TRY(rc, hipGetLastError());
+    PBA_TRY(rc, hipStreamSynchronize(s));
+    timer.collect();
+    return rc;
+}

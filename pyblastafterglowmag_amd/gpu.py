@@ -16,9 +16,13 @@ _lib = None
 
 EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu_last_error", "pbagpu_timing_enable",
            "pbagpu_timing_read", "pbagpu_timing_units", "pbagpu_dyn_evolve", "pbagpu_ele_evolve", "pbagpu_eats_lightcurve",
-           "pbagpu_eats_lightcurve_pw", "pbagpu_eats_skymap_pw")
-STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW = 0, 1, 2, 3, 4
-N_STAGES = 5
+           "pbagpu_eats_lightcurve_pw", "pbagpu_eats_skymap_pw", "pbagpu_skymap_hist")
+STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW, STAGE_SKY_HIST = 0, 1, 2, 3, 4, 5
+N_STAGES = 6
+# sky-map binning (enum pba_sky_moment / pba_sky_status)
+SKY_MOMENTS = ("xc", "yc", "xmin", "xmax", "ymin", "ymax", "sumw", "npts")
+SKY_STATUS = {0: "ok", 1: "no entry with data", 2: "intensities sum to zero", 3: "flat entry (xmin == xmax or ymin == ymax)",
+              4: "non-finite position or intensity", 5: "given edges decrease"}
 
 
 def lib():
@@ -49,6 +53,8 @@ def lib():
                                                 vp, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.pbagpu_eats_skymap_pw.argtypes = [C.POINTER(abi.RunParams), C.POINTER(abi.EatsPwLayer), C.c_int, vp, C.c_int,
                                             vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp]
+        L.pbagpu_skymap_hist.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_long), vp, vp, vp, C.c_int,
+                                         C.c_int, C.c_double, C.c_int, vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -91,6 +97,54 @@ def _check(rc, what):
     if rc != 0:
         msg = lib().pbagpu_last_error()
         raise RuntimeError(f"{what} failed rc={rc}: {msg.decode() if msg else ''}")
+
+
+def skymap_hist(images, nx, ny, extend_grid=1.0, edges=None, want_count=False, device=0):
+    """Histogram images, centroids and extents of raw sky maps in one pbagpu_skymap_hist call.
+
+    images: one list of entries per image, an entry being host arrays (x, y, intensity) of one
+            shell's principal or counter jet (ProcessRawSkymap._get_skymap's lists).
+    edges:  None (edges from the data extent) or (edges_x [n_images][nx], edges_y [n_images][ny]).
+    Returns a dict of host arrays: hist [n][nx-1][ny-1], count (or None), edges_x, edges_y,
+    moments [n][len(SKY_MOMENTS)], status [n]."""
+    torch = _torch()
+    L = lib()
+    dev = torch.device("cuda", device)
+    n = len(images)
+    img_seg = [0]
+    seg_off = [0]
+    xs, ys, ws = [], [], []
+    for entries in images:
+        for (x, y, w) in entries:
+            x, y, w = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (x, y, w))
+            if not (len(x) == len(y) == len(w)):
+                raise ValueError("skymap_hist: x, y and intensity of an entry differ in length")
+            xs.append(x); ys.append(y); ws.append(w)
+            seg_off.append(seg_off[-1] + len(x))
+        img_seg.append(len(seg_off) - 1)
+    cat = (lambda a: np.concatenate(a) if a else np.zeros(0))
+    d_x, d_y, d_w = (torch.from_numpy(cat(a)).to(dev) for a in (xs, ys, ws))
+    if edges is None:
+        d_ex = torch.empty((n, nx), dtype=torch.float64, device=dev)
+        d_ey = torch.empty((n, ny), dtype=torch.float64, device=dev)
+    else:
+        ex, ey = (np.array(np.broadcast_to(np.asarray(e, dtype=np.float64), (n, m)), order="C")
+                  for e, m in zip(edges, (nx, ny)))
+        d_ex, d_ey = torch.from_numpy(ex).to(dev), torch.from_numpy(ey).to(dev)
+    d_h = torch.empty((n, max(nx - 1, 0), max(ny - 1, 0)), dtype=torch.float64, device=dev)
+    d_c = torch.empty((n, max(nx - 1, 0), max(ny - 1, 0)), dtype=torch.int32, device=dev) if want_count else None
+    d_m = torch.empty((n, len(SKY_MOMENTS)), dtype=torch.float64, device=dev)
+    d_s = torch.empty(n, dtype=torch.int32, device=dev)
+    seg_arr = (C.c_int * len(img_seg))(*img_seg)
+    off_arr = (C.c_long * len(seg_off))(*seg_off)
+    with torch.cuda.device(dev):
+        stream = torch.cuda.current_stream()
+        _check(L.pbagpu_skymap_hist(n, seg_arr, off_arr, _ptr(d_x), _ptr(d_y), _ptr(d_w), nx, ny, float(extend_grid),
+                                    0 if edges is None else 1, _ptr(d_ex), _ptr(d_ey), _ptr(d_h), _ptr(d_c), _ptr(d_m),
+                                    _ptr(d_s), _stream_ptr(stream)), "skymap_hist")
+    return dict(hist=d_h.cpu().numpy(), count=None if d_c is None else d_c.cpu().numpy().view(np.uint32),
+                edges_x=d_ex.cpu().numpy(), edges_y=d_ey.cpu().numpy(), moments=d_m.cpu().numpy(),
+                status=d_s.cpu().numpy())
 
 
 class Run:

@@ -25,6 +25,7 @@ def _load():
         _lib.h5io_create.argtypes = [ctypes.c_char_p]
         _lib.h5io_create.restype = ctypes.c_long
         _lib.h5io_add_dataset.argtypes = [ctypes.c_long, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_long, dp]
+        _lib.h5io_add_dataset2d.argtypes = [ctypes.c_long, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, dp]
         _lib.h5io_add_attr.argtypes = [ctypes.c_long, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double]
         _lib.h5io_close.argtypes = [ctypes.c_long]
         _lib.h5io_attr_count.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
@@ -35,7 +36,7 @@ def _load():
 
 
 class Writer:
-    """Incremental HDF5 writer: 1-D float64 datasets and scalar float64 attributes in groups."""
+    """Incremental HDF5 writer: 1-D / 2-D float64 datasets and scalar float64 attributes in groups."""
 
     def __init__(self, path):
         self.f = _load().h5io_create(path.encode())
@@ -47,6 +48,12 @@ class Writer:
         a = np.ascontiguousarray(np.asarray(data, dtype=np.float64).ravel())
         if _load().h5io_add_dataset(self.f, group.encode(), name.encode(), a.size, _dp(a)) != 0:
             raise IOError(f"{self.path}: dataset {group}/{name}")
+
+    def dataset2d(self, group, name, data):
+        a = np.ascontiguousarray(np.asarray(data, dtype=np.float64))
+        if a.ndim != 2 or _load().h5io_add_dataset2d(self.f, group.encode(), name.encode(), a.shape[0], a.shape[1],
+                                                      _dp(a)) != 0:
+            raise IOError(f"{self.path}: dataset {group}/{name} {a.shape}")
 
     def attr(self, group, name, value):
         if _load().h5io_add_attr(self.f, group.encode(), name.encode(), float(value)) != 0:

@@ -1,0 +1,131 @@
+"""Sky-map binning on the CPU: the oracle held to closed definitions, the reference's raw sky maps
+split into entries, and the host-side parts of the ProcessRawSkymap mirror (FWHM, lateral
+distributions, interpolation) against closed forms."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle_sky as osky  # noqa: E402
+
+from pyblastafterglowmag_amd import skymap_process as sp  # noqa: E402
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def golden_shells(name):
+    """Raw sky maps the reference wrote (raw_skymap_<i>.h5, one shell each)."""
+    z = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    return [[(z[f"sky{i}_xrs"], z[f"sky{i}_yrs"], z[f"sky{i}_intensity"])] for i in range(int(z["sky_n"]))]
+
+
+def test_linspace_restatement_is_numpy_bit_for_bit():
+    """The edge formula the binning kernel evaluates (two roundings, last edge = stop) is np.linspace's;
+    the algebraically equal start + i * delta / div is not, so the check has teeth."""
+    rng = np.random.default_rng(7)
+    differs = 0
+    for _ in range(300):
+        a, b = np.sort(rng.normal(size=2) * 10 ** rng.uniform(-3, 3, size=2))
+        n = int(rng.integers(2, 300))
+        ref = np.linspace(a, b, n)
+        assert np.array_equal(osky.linspace_two_roundings(a, b, n), ref)
+        wrong = np.array([a + i * (b - a) / (n - 1) for i in range(n)])
+        wrong[-1] = b
+        differs += not np.array_equal(wrong, ref)
+    assert differs > 0
+    tiny = 5e-324   # zero step: the denormal branch
+    assert np.array_equal(osky.linspace_two_roundings(0.0, tiny, 5), np.linspace(0.0, tiny, 5))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_histogram_matches_definition(seed):
+    """np.histogramdd (the routine the reference calls) against half-open bins with a closed last
+    bin, summed in input order: bit-exact, including points on edges, on the last edge and outside."""
+    rng = np.random.default_rng(seed)
+    ex = np.linspace(-2.0, 3.0, 11)
+    ey = np.linspace(-1.0, 1.0, 7)
+    x = np.concatenate([rng.uniform(-3, 4, 400), ex, [ex[-1]] * 3, [ex[0] - 1e-12]])
+    y = np.concatenate([rng.uniform(-1.5, 1.5, 400), ey[rng.integers(0, len(ey), len(ex))], [ey[-1]] * 3, [0.0]])
+    w = rng.lognormal(size=len(x))
+    assert np.array_equal(osky.hist_entry(x, y, w, ex, ey), osky.hist_brute(x, y, w, ex, ey))
+    ones = np.ones_like(w)
+    assert np.array_equal(osky.count_entry(x, y, ex, ey), osky.hist_brute(x, y, ones, ex, ey))
+
+
+def test_centroid_and_extent_closed_forms():
+    phi = np.linspace(0, 2 * np.pi, 64, endpoint=False)
+    xs, ys, ds = [3.0 + np.cos(phi)], [-1.0 + 2 * np.sin(phi)], [np.ones(64)]
+    xc, yc = osky.centroid(xs, ys, ds)
+    assert abs(xc - 3.0) < 1e-14 and abs(yc + 1.0) < 1e-14
+    # a single weighted point moves the centroid to the weighted mean
+    xc, yc = osky.centroid([np.array([0.0, 1.0])], [np.array([0.0, 4.0])], [np.array([1.0, 3.0])])
+    assert (xc, yc) == (0.75, 3.0)
+    # empty and one-point entries do not count towards the extent
+    xs = [np.array([0.0, 2.0]), np.array([9.0]), np.array([-5.0, 7.0])]
+    ys = [np.array([0.0, 1.0]), np.array([9.0]), np.array([-1.0, 3.0])]
+    ds = [np.array([1.0, 1.0]), np.array([1.0]), np.array([0.0, 0.0])]
+    assert osky.data_extent(xs, ys, ds) == (0.0, 2.0, 0.0, 1.0)
+    with pytest.raises(osky.SkymapError):
+        osky.data_extent([np.array([1.0, 1.0])], [np.array([0.0, 1.0])], [np.array([1.0, 1.0])])
+    with pytest.raises(osky.SkymapError):
+        osky.data_extent([np.array([1.0])], [np.array([0.0])], [np.array([1.0])])
+    with pytest.raises(osky.SkymapError):
+        osky.centroid([np.array([1.0, 2.0])], [np.array([1.0, 2.0])], [np.zeros(2)])
+
+
+@pytest.mark.parametrize("name", ["tophat_pw_small", "gauss_pw_small"])
+def test_entries_of_reference_raw_maps(name):
+    """The package's split of the reference's raw maps into principal / counter entries."""
+    for shells in golden_shells(name):
+        got = sp.entries_of(shells)
+        xs, ys, ds = osky.get_skymap(shells)
+        assert len(got) == len(xs) == 2
+        for (x, y, w), xo, yo, do in zip(got, xs, ys, ds):
+            assert np.array_equal(x, xo) and np.array_equal(y, yo) and np.array_equal(w, do)
+        # the two jets' entries together hold every point of the shell
+        assert sum(len(e[0]) for e in got) == len(shells[0][0])
+
+
+def test_fwhm_of_gaussian():
+    """Lateral FWHM of a Gaussian: the outermost grid points inside +-sigma sqrt(2 ln 2)."""
+    sigma = 1.7
+    grid = np.linspace(-10, 10, 2001)
+    dist = np.exp(-0.5 * ((grid - 0.4) / sigma) ** 2)
+    x1, x2 = sp.skymap_fwhm(grid, dist, 0.4, 0.5)
+    half = sigma * math.sqrt(2 * math.log(2))
+    h = grid[1] - grid[0]
+    assert abs(x1 - (0.4 - half)) <= h and abs(x2 - (0.4 + half)) <= h
+
+
+def test_lateral_distribution_of_separable_image():
+    """integ: trapezoid over the other axis; for f(x) g(y) that is f(x) times the integral of g."""
+    gx = np.linspace(-1, 1, 41)
+    gy = np.linspace(0, 2, 31)
+    f = 1 + gx ** 2
+    g = np.exp(-gy)
+    img = np.outer(f, g)   # [x][y]
+    ig = np.sum(0.5 * (g[1:] + g[:-1]) * np.diff(gy))
+    assert np.allclose(sp.lateral_distribution(gx, gy, img, "y", "integ"), f * ig, rtol=1e-13)
+    fi = np.sum(0.5 * (f[1:] + f[:-1]) * np.diff(gx))
+    assert np.allclose(sp.lateral_distribution(gx, gy, img, "x", "integ"), g * fi, rtol=1e-13)
+    assert np.allclose(sp.lateral_distribution(gx, gy, img, "y", "max"), f * g.max())
+
+
+def test_interp_image_reproduces_linear_field():
+    """Linear scattered-data interpolation is exact for a linear intensity inside the hull and 0 outside."""
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(-1, 1, size=(2, 300))
+    lin = lambda x, y: 2.0 + 0.5 * x - 0.25 * y  # noqa: E731
+    ent = [(pts[0], pts[1], lin(pts[0], pts[1])), (np.zeros(0), np.zeros(0), np.zeros(0))]
+    ex, ey = np.linspace(-1.2, 1.2, 25), np.linspace(-1.2, 1.2, 17)
+    img = sp.interp_image(ent, ex, ey)
+    cx, cy = 0.5 * (ex[1:] + ex[:-1]), 0.5 * (ey[1:] + ey[:-1])
+    X, Y = np.meshgrid(cx, cy, indexing="ij")
+    inside = img != 0
+    assert inside.sum() > 100
+    assert np.allclose(img[inside], lin(X, Y)[inside], rtol=1e-12)
+    assert not inside[0, 0] and not inside[-1, -1]
