@@ -75,7 +75,8 @@ typedef struct {
     int adi_loss_ele;     /* num_ele_use_adi_loss */
     int ssa;              /* use_ssa: synchrotron self-absorption (computeSSA, numeric_model.h:190-238) */
     int syn;              /* method_synchrotron: 0 = CSYN (SynKernel::cycSynch, kernels.h:414-437),
-                             1 = Dermer09 (SynKernel::synchDermer, kernels.h:451-473) */
+                             1 = Dermer09 (SynKernel::synchDermer, kernels.h:451-473),
+                             2 = Bessel (SynKernel::synchBessel, kernels.h:396-412) */
     int pad_;
 } pba_shock_params;
 
@@ -286,6 +287,12 @@ int pbagpu_eats_skymap_pw(const pba_run_params *params, const pba_eats_pw_layer 
                           const double *d_table, int nt, const double *d_jall, const double *d_aall, int nfreq,
                           int n_images, const double *d_tobs, const double *d_nuobs, int remove_mu,
                           double *d_image, double *d_flux, int *d_status, void *stream);
+
+/* Host evaluation (no GPU) of the routine the electron stage uses on the device for the
+ * Bessel synchrotron kernel: K_{1/3}(x[i]) and K_{4/3}(x[i]) by Temme's method.  No
+ * counterpart in the reference (which calls std::cyl_bessel_k, kernels.h:402-403): it
+ * exists so that the device routine's arithmetic can be checked against libm on the host. */
+int pbagpu_bessel_k_thirds(int n, const double *x, double *k13, double *k43);
 
 /* ------------------------------------------------- sky-map binning */
 /* Histogram image, flux centroid and data extent of raw sky maps: the binning part

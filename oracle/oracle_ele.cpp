@@ -26,6 +26,7 @@
 #include "oracle_common.h"
 #include <cstring>
 #include <cstdio>
+#include <stdexcept>
 
 namespace orc {
 
@@ -38,6 +39,23 @@ double synch_dermer(double B, double nu, double gam) {
     double t2n = 1. + 2.21 * std::pow(x, 2. / 3.) + 0.347 * std::pow(x, 4. / 3.);
     double t2d = 1. + 1.353 * std::pow(x, 2. / 3.) + 0.217 * std::pow(x, 4. / 3.);
     double res = t1n / t1d * t2n / t2d * std::exp(-x);
+    res *= std::sqrt(3.) * std::pow(qe, 3.) * B / (me * c * c);
+    return res;
+}
+
+// SynKernel::synchBessel (kernels.h:396-412): libstdc++'s std::cyl_bessel_k, whose failure for
+// very large arguments ("Argument x too large", a std::runtime_error) the reference turns into 0
+double synch_bessel(double B, double nu, double gam) {
+    double nu_crit = (3. * qe * B * gam * gam) / (4. * pi * me * c);
+    double t = nu / nu_crit;
+    double res;
+    try {
+        double k43 = std::cyl_bessel_k(4. / 3., t);
+        double k13 = std::cyl_bessel_k(1. / 3., t);
+        res = 2 * t * t * (k43 * k13 - 3. / 5 * t * (k43 * k43 - k13 * k13));
+    } catch (const std::runtime_error &) {
+        res = 0;
+    }
     res *= std::sqrt(3.) * std::pow(qe, 3.) * B / (me * c * c);
     return res;
 }
@@ -237,7 +255,9 @@ static void evolve_population(const pba_shock_params &sp, const SSCTab *ssc, int
         // computeRadiationNumericMixed
         for (int i = 0; i < nf; i++)
             for (int j = 0; j < ng; j++)
-                K[(size_t)i * ng + j] = sp.syn == 1 ? synch_dermer(B, nu[i], g.e[j]) : cyc_synch(B, nu[i], g.e[j]);
+                K[(size_t)i * ng + j] = sp.syn == 1   ? synch_dermer(B, nu[i], g.e[j])
+                                        : sp.syn == 2 ? synch_bessel(B, nu[i], g.e[j])
+                                                      : cyc_synch(B, nu[i], g.e[j]);
         std::fill(synj.begin(), synj.end(), 0.);
         for (int i = 0; i < nf - 1; i++) {
             double res = 0.;
@@ -342,3 +362,12 @@ extern "C" int orc_ele_evolve(const pba_run_params *params, int n_runs, const pb
 
 extern "C" double orc_cyc_synch(double B, double nu, double gam) { return orc::cyc_synch(B, nu, gam); }
 extern "C" double orc_ssc_nava(double nu, double gam, double nu_tilde) { return orc::ssc_nava(nu, gam, nu_tilde); }
+extern "C" double orc_synch_bessel(double B, double nu, double gam) { return orc::synch_bessel(B, nu, gam); }
+// libstdc++'s std::cyl_bessel_k as the reference calls it; NaN where it throws
+extern "C" double orc_cyl_bessel_k(double nu, double x) {
+    try {
+        return std::cyl_bessel_k(nu, x);
+    } catch (const std::runtime_error &) {
+        return std::nan("");
+    }
+}

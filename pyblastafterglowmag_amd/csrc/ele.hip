@@ -21,6 +21,7 @@
 // (computeSSCSpectrum, numeric_model.h:247-285) reads the precomputed
 // Blumenthal-Gould / Nava kernel table (SSCKernel::evalSSCkernel, kernels.h:49-117).
 #include "pba_device.h"
+#include "bessel.h"
 #include "timing.h"
 #include "runtime.h"
 #include <vector>
@@ -74,6 +75,14 @@ __device__ __forceinline__ double dermer_synch(double B, double nu, double gam) 
     return t1n / t1d * t2n / t2d * exp(-x) * (sqrt(3.) * (kQe * kQe * kQe) * B / (kMe * kC * kC));
 }
 
+// synchBessel (kernels.h:396-412): t = nu / nu_crit(B, gam); 2 t^2 (K43 K13 - 3/5 t (K43^2 - K13^2))
+// times sqrt(3) qe^3 B / (me c c), with K_{1/3}, K_{4/3} from bessel.h.  Out of line: its series
+// loops would otherwise take registers from the default kernels' hot loop.
+__device__ __noinline__ double bessel_synch(double B, double nu, double gam, BesselThird b) {
+    const double t = nu / ((3. * kQe * B * gam * gam) / (4. * kPi * kMe * kC));
+    return bessel_synch_shape(t, b) * (sqrt(3.) * (kQe * kQe * kQe) * B / (kMe * kC * kC));
+}
+
 // Kernel table K[i][j][k] and its frequency integral KI[j][k] (kernels.h:49-94).
 __global__ void ssc_table_kernel(const double *__restrict__ nu, const double *__restrict__ e, int nf, int ng,
                                  double h_by_mec2, double *__restrict__ K) {
@@ -109,6 +118,7 @@ struct PopDesc {
     double *aall;         // [nt][nf] absorption (use_ssa) or null
     double *fall;         // [nt][ng] or null
     double p, eps_e;
+    BesselThird bes;      // Temme-series constants of the Bessel kernel (syn = 2)
     int shock, ng, nf, max_substeps, adi_loss, ssc, ssa, syn;
 };
 
@@ -344,7 +354,9 @@ __global__ void __launch_bounds__(64) ele_step_kernel(const PopDesc *__restrict_
             const bool emit = i < nf - 1;   // computeSynSpectrum leaves the last frequency at 0
             if (emit || D.ssa) {
                 for (int j = 0; j < ng - 1; j++) {
-                    const double k = D.syn ? dermer_synch(B, nui, D.e[j]) : cyc_synch(B, nui, D.e[j], pref);
+                    const double k = D.syn == 0   ? cyc_synch(B, nui, D.e[j], pref)
+                                     : D.syn == 1 ? dermer_synch(B, nui, D.e[j])
+                                                  : bessel_synch(B, nui, D.e[j], D.bes);
                     if (emit) synj += k * s_f[j] * s_dge[j];
                     if (D.ssa) asum += s_wa[j] * k;
                 }
@@ -629,7 +641,7 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
             return PBAGPU_E_ARG;
         const pba_shock_params &sp = P.shock ? params[P.run].rs : params[P.run].fs;
         if (sp.ngam != ng || sp.nfreq != nf) return PBAGPU_E_ARG;
-        if (sp.syn != 0 && sp.syn != 1) return PBAGPU_E_UNSUPPORTED;
+        if (sp.syn < 0 || sp.syn > 2) return PBAGPU_E_UNSUPPORTED;
         if (sp.ssa && !d_aall) {
             pbagpu::set_error_msg("ele_evolve: a shock with ssa = 1 needs the absorption table d_aall");
             return PBAGPU_E_ARG;
@@ -712,6 +724,7 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
         D.aall = sp.ssa ? d_aall + (size_t)q * nt * nf : nullptr;
         D.ssa = sp.ssa ? 1 : 0;
         D.syn = sp.syn;
+        D.bes = pbagpu::bessel_third_constants();
         D.fall = d_fall ? d_fall + (size_t)q * nt * ng : nullptr;
         D.p = sp.p; D.eps_e = sp.eps_e; D.shock = P.shock; D.ng = ng; D.nf = nf;
         D.max_substeps = sp.max_substeps; D.adi_loss = sp.adi_loss_ele; D.ssc = D.K != nullptr;
@@ -800,4 +813,13 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
     ttab.collect();
     tele.collect();
     return rc;
+}
+
+// Host evaluation of the Bessel kernel's K_{1/3}, K_{4/3} routine (the code the electron stage
+// runs on the device), for checks against libm / scipy.  Touches no GPU.
+extern "C" int pbagpu_bessel_k_thirds(int n, const double *x, double *k13, double *k43) {
+    if (n < 0 || (n > 0 && (!x || !k13 || !k43))) return PBAGPU_E_ARG;
+    const pbagpu::BesselThird b = pbagpu::bessel_third_constants();
+    for (int i = 0; i < n; i++) pbagpu::bessel_k_thirds(x[i], b, k13[i], k43[i]);
+    return PBAGPU_OK;
 }

@@ -16,7 +16,7 @@ _lib = None
 
 EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu_last_error", "pbagpu_timing_enable",
            "pbagpu_timing_read", "pbagpu_timing_units", "pbagpu_dyn_evolve", "pbagpu_ele_evolve", "pbagpu_eats_lightcurve",
-           "pbagpu_eats_lightcurve_pw", "pbagpu_eats_skymap_pw", "pbagpu_skymap_hist")
+           "pbagpu_eats_lightcurve_pw", "pbagpu_eats_skymap_pw", "pbagpu_skymap_hist", "pbagpu_bessel_k_thirds")
 STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW, STAGE_SKY_HIST = 0, 1, 2, 3, 4, 5
 N_STAGES = 6
 # sky-map binning (enum pba_sky_moment / pba_sky_status)
@@ -53,6 +53,7 @@ def lib():
                                                 vp, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.pbagpu_eats_skymap_pw.argtypes = [C.POINTER(abi.RunParams), C.POINTER(abi.EatsPwLayer), C.c_int, vp, C.c_int,
                                             vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp]
+        L.pbagpu_bessel_k_thirds.argtypes = [C.c_int, vp, vp, vp]
         L.pbagpu_skymap_hist.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_long), vp, vp, vp, C.c_int,
                                          C.c_int, C.c_double, C.c_int, vp, vp, vp, vp, vp, vp, vp]
         _lib = L
@@ -97,6 +98,15 @@ def _check(rc, what):
     if rc != 0:
         msg = lib().pbagpu_last_error()
         raise RuntimeError(f"{what} failed rc={rc}: {msg.decode() if msg else ''}")
+
+
+def bessel_k_thirds(x):
+    """K_{1/3}(x), K_{4/3}(x) by the electron stage's device routine, evaluated on the host."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    k13, k43 = np.empty_like(x), np.empty_like(x)
+    _check(lib().pbagpu_bessel_k_thirds(len(x), x.ctypes.data_as(C.c_void_p), k13.ctypes.data_as(C.c_void_p),
+                                        k43.ctypes.data_as(C.c_void_p)), "bessel_k_thirds")
+    return k13, k43
 
 
 def skymap_hist(images, nx, ny, extend_grid=1.0, edges=None, want_count=False, device=0):

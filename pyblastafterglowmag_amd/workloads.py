@@ -136,6 +136,10 @@ SMALL = {
     # the Dermer (2009) synchrotron kernel in place of CSYN
     "tophat_dermer_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
                                 method_synchrotron_fs="Dermer09"),
+    # the Bessel-function synchrotron kernel (K_{4/3}, K_{1/3}) in place of CSYN; a shorter grid,
+    # since the reference's std::cyl_bessel_k takes up to 0.1 ms per call at large arguments
+    "tophat_bessel_small": dict(base="tophat_fs_syn", ntb=300, ngam=48, nfreq=32, n_nu_obs=6, n_t_obs=8,
+                                method_synchrotron_fs="Bessel"),
     # the classical RK4 integrator, one step per output interval, in place of adaptive DOP853
     "tophat_rk4_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
                              integrator="RK4"),

@@ -26,6 +26,10 @@ def _lib():
     L.orc_cyc_synch.argtypes = [C.c_double] * 3
     L.orc_ssc_nava.restype = C.c_double
     L.orc_ssc_nava.argtypes = [C.c_double] * 3
+    L.orc_synch_bessel.restype = C.c_double
+    L.orc_synch_bessel.argtypes = [C.c_double] * 3
+    L.orc_cyl_bessel_k.restype = C.c_double
+    L.orc_cyl_bessel_k.argtypes = [C.c_double] * 2
     return L
 
 
@@ -175,3 +179,45 @@ def test_dynamics_energy_budget_relativistic(tophat_dyn):
     m = G > 10
     assert np.all(np.abs(Et[m] - 1) < 2e-2)
     assert abs(Et[0] - 1) < 1e-12
+
+
+def test_bessel_kernel_routine_matches_libstdcxx_and_scipy():
+    """The device routine for K_{1/3}, K_{4/3} (Temme's series below x = 2, Steed's CF2 above,
+    one evaluation for both orders), run on the host through the library: against libstdc++'s
+    std::cyl_bessel_k, which the reference calls, and against scipy's kv (AMOS)."""
+    from scipy.special import kv
+    from pyblastafterglowmag_amd import gpu
+    L = _lib()
+    x = np.concatenate([np.logspace(-20, np.log10(700.0), 600), [1.999999999, 2.0, 2.000000001]])
+    k13, k43 = gpu.bessel_k_thirds(x)
+    s13 = np.array([L.orc_cyl_bessel_k(1. / 3., v) for v in x])
+    s43 = np.array([L.orc_cyl_bessel_k(4. / 3., v) for v in x])
+    ok = s13 > 1e-290
+    # the same series with the same constants: agreement to a few ulp (K_{4/3}: libstdc++'s mu
+    # for order 4/3 is one ulp below 1/3)
+    assert np.max(np.abs(k13[ok] / s13[ok] - 1)) < 1e-14
+    assert np.max(np.abs(k43[ok] / s43[ok] - 1)) < 1e-14
+    assert np.max(np.abs(k13[ok] / kv(1. / 3., x[ok]) - 1)) < 1e-12
+    assert np.max(np.abs(k43[ok] / kv(4. / 3., x[ok]) - 1)) < 1e-12
+    # K underflows beyond 745; the reference's kernel is 0 there, also where libstdc++ throws
+    k13, k43 = gpu.bessel_k_thirds(np.array([745.0, 1e4, 1e30]))
+    assert not np.any(k13) and not np.any(k43)
+    assert math.isnan(L.orc_cyl_bessel_k(4. / 3., 1e30)) and L.orc_synch_bessel(1.0, 1e30, 1.0) == 0.0
+
+
+@pytest.mark.parametrize("t", [1e-3, 0.05, 0.5, 2.0])
+def test_bessel_kernel_is_the_angle_averaged_synchrotron_function(t):
+    """synchBessel's 2 t^2 (K43 K13 - 3/5 t (K43^2 - K13^2)) is Crusius & Schlickeiser's (1986)
+    closed form of the pitch-angle-averaged synchrotron function at 2t: R(2 t), with R from the
+    defining quadrature (t = nu / nu_c with the same nu_c as CSYN, whose fit follows R(t); the
+    reference notes the factor 2).  Also int_0^inf R(2t) dt = (1/2)(2/3) 8 pi / (9 sqrt 3)."""
+    from scipy import integrate as itg
+    L = _lib()
+    B, gam = 1.0, 1e3
+    nu_c = 3 * QE * B * gam * gam / (4 * math.pi * ME * CL)
+    pref = math.sqrt(3.) * QE ** 3 * B / (ME * CL * CL)
+    assert L.orc_synch_bessel(B, t * nu_c, gam) / pref == pytest.approx(_synch_R(2 * t), rel=1e-6)
+    if t == 2.0:
+        shape = lambda u: L.orc_synch_bessel(B, math.exp(u) * nu_c, gam) / pref * math.exp(u)  # noqa: E731
+        val = itg.quad(shape, -25, 6, limit=400)[0]
+        assert val == pytest.approx(8 * math.pi / (27 * math.sqrt(3)), rel=1e-8)
