@@ -17,7 +17,7 @@ enum WsSlot {
     WS_ELE_GRIDS, WS_ELE_KTAB, WS_ELE_POPS, WS_ELE_WORK, WS_ELE_LIST, WS_ELE_KBF, WS_ELE_MIXED,
     WS_EATS_NU, WS_EATS_LAYERS, WS_EATS_STORE, WS_EATS_COUNTER, WS_EATS_SAMPLES,
     WS_PW_NU, WS_PW_LAYERS, WS_PW_SAMPLES,
-    WS_SKY_SEG, WS_SKY_OFF,
+    WS_SKY_SEG, WS_SKY_OFF, WS_SKY_PART,
     WS_N
 };
 

@@ -1,9 +1,12 @@
 // Sky-map binning: intensity-weighted 2-D histogram, flux centroid and data extent of raw
 // sky maps (ProcessRawSkymap._process_file, package/src/PyBlastAfterglowMag/skymap_process.py:465-485).
 //
-// Three launches on the caller's stream:
+// Four launches on the caller's stream:
 //   sky_zero_kernel    clears the histogram (and the optional point counts) with vector stores;
-//   sky_extent_kernel  one workgroup per image: per-entry extent, centroid sums, edges, status;
+//   sky_partial_kernel one workgroup per work item (<= 4096 points of one entry): min / max of
+//                      x and y and the sums of w, w x, w y;
+//   sky_extent_kernel  one workgroup per image: combines its items in order into the per-entry
+//                      extent and the centroid, then writes the edges and the status;
 //   sky_bin_kernel     a grid over every image's points: bin search in LDS-staged edges and an
 //                      atomic add of the intensity into the image's histogram.
 // The work is HBM / atomic bound and tiny next to the observer stage (one 256 x 256 image of a
@@ -23,25 +26,19 @@ namespace skybin {
 constexpr int BLOCK = 256;
 constexpr int MAX_EDGES = 4096;   // nx + ny edges staged in LDS by the binning kernel (32 KiB)
 
-// Workgroup-wide reductions in LDS (fixed tree: the result does not depend on scheduling).
-template <typename Op>
-__device__ double block_reduce(double v, double *red, Op op) {
-    const int t = threadIdx.x;
-    red[t] = v;
-    __syncthreads();
-    for (int s = BLOCK / 2; s > 0; s >>= 1) {
-        if (t < s) red[t] = op(red[t], red[t + s]);
-        __syncthreads();
-    }
-    const double r = red[0];
-    __syncthreads();
-    return r;
-}
-
+// Reductions: over the 64 lanes of a wave by butterfly shuffles, then over the 4 waves of the
+// workgroup in LDS, in a fixed order (the result does not depend on scheduling).
+constexpr int WAVES = BLOCK / 64;
 struct Add { __device__ double operator()(double a, double b) const { return a + b; } };
 // NaN never wins a comparison, so non-finite positions are screened separately.
 struct Min { __device__ double operator()(double a, double b) const { return b < a ? b : a; } };
 struct Max { __device__ double operator()(double a, double b) const { return b > a ? b : a; } };
+
+template <typename Op>
+__device__ double wave_reduce(double v, Op op) {
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
+    return v;
+}
 
 // np.linspace(lo * scale, hi * scale, num) with num >= 2 (numpy/_core/function_base.py): step = (stop - start)
 // / (num - 1); e_i = i * step + start as two roundings (no fused multiply-add: the pragma keeps
@@ -69,63 +66,98 @@ __global__ void sky_zero_kernel(double *hist, unsigned *count, size_t n) {
     }
 }
 
-// One workgroup per image.
+// Partial statistics of one work item (at most ITEM points of one entry), one workgroup each.
+constexpr int ITEM = 4096;
+enum { P_X0 = 0, P_X1, P_Y0, P_Y1, P_W, P_WX, P_WY, P_BAD, P_N };
+
+__global__ __launch_bounds__(BLOCK) void sky_partial_kernel(const long *item_range, const double *x, const double *y,
+                                                            const double *w, double *part) {
+    __shared__ double red[P_N][WAVES];
+    const long b = item_range[2 * blockIdx.x], e = item_range[2 * blockIdx.x + 1];
+    double v[P_N] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX, 0., 0., 0., 0.};
+    for (long i = b + threadIdx.x; i < e; i += BLOCK) {
+        const double xi = x[i], yi = y[i], wi = w[i];
+        if (!isfinite(xi) || !isfinite(yi) || !isfinite(wi)) v[P_BAD] = 1.;
+        v[P_X0] = fmin(v[P_X0], xi); v[P_X1] = fmax(v[P_X1], xi);
+        v[P_Y0] = fmin(v[P_Y0], yi); v[P_Y1] = fmax(v[P_Y1], yi);
+        v[P_W] += wi; v[P_WX] += wi * xi; v[P_WY] += wi * yi;
+    }
+    v[P_X0] = wave_reduce(v[P_X0], Min()); v[P_X1] = wave_reduce(v[P_X1], Max());
+    v[P_Y0] = wave_reduce(v[P_Y0], Min()); v[P_Y1] = wave_reduce(v[P_Y1], Max());
+    v[P_W] = wave_reduce(v[P_W], Add()); v[P_WX] = wave_reduce(v[P_WX], Add());
+    v[P_WY] = wave_reduce(v[P_WY], Add()); v[P_BAD] = wave_reduce(v[P_BAD], Max());
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < P_N; k++) red[k][wv] = v[k];
+    __syncthreads();
+    if (threadIdx.x < P_N) {
+        const int k = threadIdx.x;
+        double r = red[k][0];
+        for (int q = 1; q < WAVES; q++) {
+            const double o = red[k][q];
+            r = (k == P_X0 || k == P_Y0) ? Min()(r, o) : (k == P_X1 || k == P_Y1 || k == P_BAD) ? Max()(r, o) : r + o;
+        }
+        part[(size_t)blockIdx.x * P_N + k] = r;
+    }
+}
+
+// One workgroup per image: combines its entries' partials in entry and item order.
 //  * _get_data_extend (skymap_process.py:107-189): an entry counts when it has at least two
 //    points and its intensities do not sum to zero; the image extent is the min / max of x and
 //    y over the points of those entries; an entry with min == max in x or y is an error there.
 //  * _compute_position_of_the_flux_centroid (:98-105): sum(w x) / sum(w) over every point.
 //  * edges = np.linspace(xmin * extend_grid, xmax * extend_grid, nx) (:480-481) unless given.
-__global__ __launch_bounds__(BLOCK) void sky_extent_kernel(const int *img_seg, const long *seg_off, const double *x,
-                                                           const double *y, const double *w, int nx, int ny,
-                                                           double extend_grid, int edges_given, double *edges_x,
-                                                           double *edges_y, double *mom, int *status) {
-    __shared__ double red[BLOCK];
-    __shared__ int s_bad, s_dec;   // set before different barriers: one flag each
+__global__ __launch_bounds__(BLOCK) void sky_extent_kernel(const int *img_seg, const long *seg_off, const int *item_off,
+                                                           const double *part, int nx, int ny, double extend_grid,
+                                                           int edges_given, double *edges_x, double *edges_y,
+                                                           double *mom, int *status) {
+    __shared__ double s_r[8];
+    __shared__ int s_st, s_dec;
     const int img = blockIdx.x;
     const int t = threadIdx.x;
-    if (t == 0) { s_bad = 0; s_dec = 0; }
-    __syncthreads();
-    double sxw = 0.0, syw = 0.0, sw = 0.0;
-    double xmin = DBL_MAX, xmax = -DBL_MAX, ymin = DBL_MAX, ymax = -DBL_MAX;
-    int kept = 0, flat = 0;
-    const long p0 = seg_off[img_seg[img]], p1 = seg_off[img_seg[img + 1]];
-    for (int k = img_seg[img]; k < img_seg[img + 1]; k++) {
-        const long b = seg_off[k], e = seg_off[k + 1];
-        double ex0 = DBL_MAX, ex1 = -DBL_MAX, ey0 = DBL_MAX, ey1 = -DBL_MAX, es = 0.0;
-        int bad = 0;
-        for (long i = b + t; i < e; i += BLOCK) {
-            const double xi = x[i], yi = y[i], wi = w[i];
-            if (!isfinite(xi) || !isfinite(yi) || !isfinite(wi)) bad = 1;
-            ex0 = fmin(ex0, xi); ex1 = fmax(ex1, xi);
-            ey0 = fmin(ey0, yi); ey1 = fmax(ey1, yi);
-            es += wi;
-            sxw += wi * xi; syw += wi * yi; sw += wi;
+    if (t == 0) {
+        double sxw = 0., syw = 0., sw = 0.;
+        double xmin = DBL_MAX, xmax = -DBL_MAX, ymin = DBL_MAX, ymax = -DBL_MAX;
+        int kept = 0, flat = 0, bad = 0;
+        for (int k = img_seg[img]; k < img_seg[img + 1]; k++) {
+            double ex0 = DBL_MAX, ex1 = -DBL_MAX, ey0 = DBL_MAX, ey1 = -DBL_MAX, es = 0.;
+            for (int it = item_off[k]; it < item_off[k + 1]; it++) {
+                const double *P = part + (size_t)it * P_N;
+                ex0 = fmin(ex0, P[P_X0]); ex1 = fmax(ex1, P[P_X1]);
+                ey0 = fmin(ey0, P[P_Y0]); ey1 = fmax(ey1, P[P_Y1]);
+                es += P[P_W];
+                sw += P[P_W]; sxw += P[P_WX]; syw += P[P_WY];
+                if (P[P_BAD] != 0.) bad = 1;
+            }
+            if (seg_off[k + 1] - seg_off[k] >= 2 && es != 0.0) {
+                kept++;
+                if (ex0 == ex1 || ey0 == ey1) flat = 1;
+                xmin = fmin(xmin, ex0); xmax = fmax(xmax, ex1);
+                ymin = fmin(ymin, ey0); ymax = fmax(ymax, ey1);
+            }
         }
-        if (bad) atomicOr(&s_bad, 1);
-        ex0 = block_reduce(ex0, red, Min()); ex1 = block_reduce(ex1, red, Max());
-        ey0 = block_reduce(ey0, red, Min()); ey1 = block_reduce(ey1, red, Max());
-        es = block_reduce(es, red, Add());
-        if (e - b >= 2 && es != 0.0) {
-            kept++;
-            if (ex0 == ex1 || ey0 == ey1) flat = 1;
-            xmin = fmin(xmin, ex0); xmax = fmax(xmax, ex1);
-            ymin = fmin(ymin, ey0); ymax = fmax(ymax, ey1);
-        }
+        int st = PBA_SKY_OK;
+        if (bad) st = PBA_SKY_NONFINITE;
+        else if (sw == 0.0) st = PBA_SKY_ZERO_WEIGHT;   // the centroid is computed first there
+        // the extent's checks run only where the extent makes the edges
+        else if (!edges_given && kept == 0) st = PBA_SKY_NO_DATA;
+        else if (!edges_given && flat) st = PBA_SKY_FLAT;
+        double *m = mom + (size_t)img * PBA_SKY_NM;
+        m[PBA_SKY_XC] = sw != 0.0 ? sxw / sw : 0.0;
+        m[PBA_SKY_YC] = sw != 0.0 ? syw / sw : 0.0;
+        m[PBA_SKY_XMIN] = kept ? xmin : 0.0; m[PBA_SKY_XMAX] = kept ? xmax : 0.0;
+        m[PBA_SKY_YMIN] = kept ? ymin : 0.0; m[PBA_SKY_YMAX] = kept ? ymax : 0.0;
+        m[PBA_SKY_SUMW] = sw;
+        m[PBA_SKY_NPTS] = (double)(seg_off[img_seg[img + 1]] - seg_off[img_seg[img]]);
+        s_r[0] = xmin; s_r[1] = xmax; s_r[2] = ymin; s_r[3] = ymax;
+        s_st = st; s_dec = 0;
     }
-    sxw = block_reduce(sxw, red, Add());
-    syw = block_reduce(syw, red, Add());
-    sw = block_reduce(sw, red, Add());
-    // every value below is uniform over the workgroup (reduced, or read after the last barrier)
-    int st = PBA_SKY_OK;
-    if (s_bad) st = PBA_SKY_NONFINITE;
-    else if (sw == 0.0) st = PBA_SKY_ZERO_WEIGHT;   // the centroid is computed first there
-    // the extent's checks run only where the extent makes the edges
-    else if (!edges_given && kept == 0) st = PBA_SKY_NO_DATA;
-    else if (!edges_given && flat) st = PBA_SKY_FLAT;
+    __syncthreads();
+    int st = s_st;   // uniform over the workgroup from here on
     double *ex = edges_x + (size_t)img * nx, *ey = edges_y + (size_t)img * ny;
     if (st == PBA_SKY_OK && !edges_given) {
-        linspace(xmin, xmax, extend_grid, nx, ex);
-        linspace(ymin, ymax, extend_grid, ny, ey);
+        linspace(s_r[0], s_r[1], extend_grid, nx, ex);
+        linspace(s_r[2], s_r[3], extend_grid, ny, ey);
     }
     if (st == PBA_SKY_OK && edges_given) {
         int dec = 0;
@@ -135,16 +167,7 @@ __global__ __launch_bounds__(BLOCK) void sky_extent_kernel(const int *img_seg, c
         __syncthreads();
         if (s_dec) st = PBA_SKY_BAD_EDGES;
     }
-    if (t == 0) {
-        double *m = mom + (size_t)img * PBA_SKY_NM;
-        m[PBA_SKY_XC] = sw != 0.0 ? sxw / sw : 0.0;
-        m[PBA_SKY_YC] = sw != 0.0 ? syw / sw : 0.0;
-        m[PBA_SKY_XMIN] = kept ? xmin : 0.0; m[PBA_SKY_XMAX] = kept ? xmax : 0.0;
-        m[PBA_SKY_YMIN] = kept ? ymin : 0.0; m[PBA_SKY_YMAX] = kept ? ymax : 0.0;
-        m[PBA_SKY_SUMW] = sw;
-        m[PBA_SKY_NPTS] = (double)(p1 - p0);
-        status[img] = st;
-    }
+    if (t == 0) status[img] = st;
 }
 
 // Number of edges <= v (np.searchsorted(edges, v, side='right')); NaN gives 0.
@@ -221,19 +244,41 @@ extern "C" int pbagpu_skymap_hist(int n_images, const int *img_seg, const long *
         const long np_i = seg_off[img_seg[i + 1]] - seg_off[img_seg[i]];
         if (np_i > max_pts) max_pts = np_i;
     }
-    auto *dseg = (int *)workspace(WS_SKY_SEG, sizeof(int) * (n_images + 1));
-    auto *doff = (long *)workspace(WS_SKY_OFF, sizeof(long) * (n_segs + 1));
-    if (!dseg || !doff) return PBAGPU_E_HIP;
+    // work items of at most ITEM points; every entry has at least one (empty entries: an empty one)
+    std::vector<int> item_off(n_segs + 1, 0);
+    std::vector<long> items;
+    for (int k = 0; k < n_segs; k++) {
+        item_off[k] = (int)(items.size() / 2);
+        long b = seg_off[k];
+        do {
+            const long e = std::min<long>(b + ITEM, seg_off[k + 1]);
+            items.push_back(b); items.push_back(e);
+            b = e;
+        } while (b < seg_off[k + 1]);
+    }
+    item_off[n_segs] = (int)(items.size() / 2);
+    const int n_items = item_off[n_segs];
+    auto *dseg = (int *)workspace(WS_SKY_SEG, sizeof(int) * ((size_t)n_images + 1 + n_segs + 1));
+    auto *doff = (long *)workspace(WS_SKY_OFF, sizeof(long) * ((size_t)n_segs + 1 + items.size()));
+    auto *dpart = (double *)workspace(WS_SKY_PART, sizeof(double) * (size_t)n_items * P_N);
+    if (!dseg || !doff || !dpart) return PBAGPU_E_HIP;
+    int *ditem_off = dseg + n_images + 1;
+    long *ditems = doff + n_segs + 1;
     int rc = PBAGPU_OK;
     PBA_TRY(rc, hipMemcpyAsync(dseg, img_seg, sizeof(int) * (n_images + 1), hipMemcpyHostToDevice, s));
+    PBA_TRY(rc, hipMemcpyAsync(ditem_off, item_off.data(), sizeof(int) * (n_segs + 1), hipMemcpyHostToDevice, s));
     PBA_TRY(rc, hipMemcpyAsync(doff, seg_off, sizeof(long) * (n_segs + 1), hipMemcpyHostToDevice, s));
+    if (!items.empty())
+        PBA_TRY(rc, hipMemcpyAsync(ditems, items.data(), sizeof(long) * items.size(), hipMemcpyHostToDevice, s));
     if (rc) return rc;
     const size_t nb = (size_t)n_images * (nx - 1) * (ny - 1);
     const unsigned zgrid = (unsigned)std::min<size_t>((nb + BLOCK - 1) / BLOCK, 4096);
     KernelTimer timer(PBAGPU_STAGE_SKY_HIST);
     timer.start(s);
     hipLaunchKernelGGL(sky_zero_kernel, dim3(zgrid), dim3(BLOCK), 0, s, d_hist, d_count, nb);
-    hipLaunchKernelGGL(sky_extent_kernel, dim3(n_images), dim3(BLOCK), 0, s, dseg, doff, d_x, d_y, d_w, nx, ny,
+    if (n_items > 0)
+        hipLaunchKernelGGL(sky_partial_kernel, dim3(n_items), dim3(BLOCK), 0, s, ditems, d_x, d_y, d_w, dpart);
+    hipLaunchKernelGGL(sky_extent_kernel, dim3(n_images), dim3(BLOCK), 0, s, dseg, doff, ditem_off, dpart, nx, ny,
                        extend_grid, edges_given, d_edges_x, d_edges_y, d_moments, d_status);
     if (max_pts > 0) {
         const unsigned chunks = (unsigned)std::min<long>((max_pts + BLOCK - 1) / BLOCK, 1024);
