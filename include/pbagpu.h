@@ -76,7 +76,8 @@ typedef struct {
     int ssa;              /* use_ssa: synchrotron self-absorption (computeSSA, numeric_model.h:190-238) */
     int syn;              /* method_synchrotron: 0 = CSYN (SynKernel::cycSynch, kernels.h:414-437),
                              1 = Dermer09 (SynKernel::synchDermer, kernels.h:451-473),
-                             2 = Bessel (SynKernel::synchBessel, kernels.h:396-412) */
+                             2 = Bessel (SynKernel::synchBessel, kernels.h:396-412),
+                             3 = GSL (SynKernel::synchGSL, kernels.h:211-342) */
     int pad_;
 } pba_shock_params;
 

@@ -92,7 +92,7 @@ def _yes(v):
 
 
 # method_synchrotron values of the numeric electron path and their code in pba_shock_params.syn
-SYNCHROTRON_KERNELS = {"CSYN": 0, "Dermer09": 1, "Bessel": 2}
+SYNCHROTRON_KERNELS = {"CSYN": 0, "Dermer09": 1, "Bessel": 2, "GSL": 3}
 
 
 def _shock(pars, opts, sh):

@@ -140,6 +140,9 @@ SMALL = {
     # since the reference's std::cyl_bessel_k takes up to 0.1 ms per call at large arguments
     "tophat_bessel_small": dict(base="tophat_fs_syn", ntb=300, ngam=48, nfreq=32, n_nu_obs=6, n_t_obs=8,
                                 method_synchrotron_fs="Bessel"),
+    # GSL's Chebyshev fit of the synchrotron function F(x) in place of CSYN
+    "tophat_gsl_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
+                             method_synchrotron_fs="GSL"),
     # the classical RK4 integrator, one step per output interval, in place of adaptive DOP853
     "tophat_rk4_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
                              integrator="RK4"),

@@ -32,7 +32,7 @@ def _gpu(run):
 
 @pytest.mark.parametrize("name", ["tophat_small", "gauss5_small", "tophat_pw_small", "gauss_pw_small",
                                   "tophat_ssa_small", "gauss5_ssa_small", "tophat_dermer_small", "tophat_bessel_small",
-                                  "tophat_rk4_small"])
+                                  "tophat_gsl_small", "tophat_rk4_small"])
 def test_stages_match_oracle(name):
     g = load_golden(name)
     run = golden_run(g)
@@ -85,12 +85,13 @@ def test_stages_match_oracle(name):
 # reverse-shock emission differs; the largest effect is 1.9e-6 at a pair 1e-6 below the peak.
 LC_TOL = {"tophat_small": 1e-6, "tophat_fs_syn": 1e-6, "gauss5_small": 5e-6, "tophat_pw_small": 1e-6,
           "gauss_pw_small": 5e-6, "tophat64_pw": 1e-6, "tophat_ssa_small": 1e-6, "gauss5_ssa_small": 5e-6,
-          "tophat_dermer_small": 1e-6, "tophat_bessel_small": 1e-6, "tophat_rk4_small": 1e-6}
+          "tophat_dermer_small": 1e-6, "tophat_bessel_small": 1e-6, "tophat_gsl_small": 1e-6,
+          "tophat_rk4_small": 1e-6}
 
 
 @pytest.mark.parametrize("name", ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gauss_pw_small",
                                   "tophat64_pw", "tophat_ssa_small", "gauss5_ssa_small", "tophat_dermer_small",
-                                  "tophat_bessel_small", "tophat_rk4_small"])
+                                  "tophat_bessel_small", "tophat_gsl_small", "tophat_rk4_small"])
 def test_lightcurve_matches_reference(name):
     """Light curves end to end against the reference program; tophat64_pw is the full-size piece-wise
     case (64 layers, 4096 cells, 96 x 128 pairs; the fixture holds every 7th layer)."""

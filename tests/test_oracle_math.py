@@ -30,6 +30,8 @@ def _lib():
     L.orc_synch_bessel.argtypes = [C.c_double] * 3
     L.orc_cyl_bessel_k.restype = C.c_double
     L.orc_cyl_bessel_k.argtypes = [C.c_double] * 2
+    L.orc_synch_gsl.restype = C.c_double
+    L.orc_synch_gsl.argtypes = [C.c_double] * 3
     return L
 
 
@@ -221,3 +223,26 @@ def test_bessel_kernel_is_the_angle_averaged_synchrotron_function(t):
         shape = lambda u: L.orc_synch_bessel(B, math.exp(u) * nu_c, gam) / pref * math.exp(u)  # noqa: E731
         val = itg.quad(shape, -25, 6, limit=400)[0]
         assert val == pytest.approx(8 * math.pi / (27 * math.sqrt(3)), rel=1e-8)
+
+
+@pytest.mark.parametrize("x", [1e-12, 1e-6, 0.01, 0.3, 1.0, 3.9, 4.1, 20.0, 200.0])
+def test_gsl_kernel_is_the_synchrotron_function(x):
+    """synchGSL: the GNU Scientific Library's Chebyshev fit of F(x) = x int_x^inf K_5/3 (single pitch
+    angle) in every branch (small-x expansion, two tables up to x = 4, asymptotic table above),
+    against the defining integral; x = (h nu / me c^2) / (1.5 B / 4.14e13 gam^2) as the reference
+    forms it.  A mistyped coefficient of the tables shows here."""
+    L = _lib()
+    B, gam = 1.0, 1e3
+    en_per_nu = H / ME / CL / CL
+    nu = x * (1.5 * B / 4.14e13 * gam * gam) / en_per_nu
+    pref = math.sqrt(3.) * QE ** 3 * B / MEC2
+    # K_5/3 = -2 K'_2/3 - K_1/3, so F(x) = x (2 K_2/3(x) - int_x^inf K_1/3), an integrand without
+    # the t^-5/3 singularity
+    tail = integrate.quad(lambda t: special.kv(1. / 3., t), x, 1.0, limit=400, epsabs=0, epsrel=1e-13)[0] + \
+        integrate.quad(lambda t: special.kv(1. / 3., t), 1.0, np.inf, limit=400, epsabs=0, epsrel=1e-13)[0]
+    F = x * (2 * special.kv(2. / 3., x) - tail)
+    got = L.orc_synch_gsl(B, nu, gam) / pref
+    assert got == pytest.approx(F, rel=1e-11)
+    if x < 1e-9:   # leading term of the small-x series: 4 pi / (sqrt 3 Gamma(1/3)) (x/2)^(1/3)
+        assert got == pytest.approx(4 * math.pi / (math.sqrt(3) * special.gamma(1. / 3.)) * (x / 2) ** (1. / 3.),
+                                    rel=1e-7)
