@@ -6,7 +6,13 @@ stages on the GPU (dynamics, electrons + emissivity, EATS light curves) and writ
 reference writes for ``do_lc`` / ``save_dynamics`` / ``save_spec`` / ``do_skymap`` (``fname_light_curve``,
 ``fname_dyn``, ``fname_spectrum``, ``raw_skymap_<i>.h5``) in the same layout (see ``output``).  Errors end the process
 with status 1 after a message, as the reference does.
+
+``--process-skymaps`` then does what the reference package's ``PyBlastAfterglow.run(process_skymaps=True)`` does
+after the executable (package/src/PyBlastAfterglowMag/interface.py:1052-1057): ProcessRawSkymap.process_singles over
+the raw maps into ``fname_sky_map`` (default ``skymap.h5``), with the package's default configuration or the JSON
+given by ``--skymap-conf``.
 """
+import json
 import argparse
 import os
 import sys
@@ -22,6 +28,11 @@ def main(argv=None):
     ap.add_argument("loglevel", nargs="?", default="info")
     ap.add_argument("--precision", type=int, default=0,
                     help="0: FP64 everywhere; 1: SSC contraction in BF16 hi/lo on MFMA (FP32 accumulation)")
+    ap.add_argument("--process-skymaps", action="store_true",
+                    help="bin the raw sky maps into fname_sky_map (ProcessRawSkymap.process_singles)")
+    ap.add_argument("--skymap-conf", default=None,
+                    help="JSON ProcessRawSkymap configuration (default: nx=64, ny=32, extend_grid=2, fwhm_fac=0.5, "
+                         "lat_dist_method=integ, no filters)")
     a = ap.parse_args(argv)
     verbose = a.loglevel == "info"
 
@@ -80,6 +91,17 @@ def main(argv=None):
                                          mp_, gp)
         if verbose:
             print(f"[pba] {len(paths)} raw sky maps -> {workdir}raw_skymap_*.h5", flush=True)
+        if a.process_skymaps:
+            from .skymap_process import ProcessRawSkymap, DEFAULT_CONF
+            conf = dict(DEFAULT_CONF, **(json.loads(a.skymap_conf) if a.skymap_conf else {}))
+            out = workdir + go.get("fname_sky_map", "skymap.h5")
+            try:
+                ProcessRawSkymap(conf=conf).process_singles(infpaths=workdir + "raw_skymap_*.h5", outfpath=out)
+            except ValueError as e:
+                print(f"[pba] sky-map processing failed: {e}", file=sys.stderr)
+                return 1
+            if verbose:
+                print(f"[pba] processed sky maps -> {out}", flush=True)
     if save_spec:
         path = workdir + go.get("fname_spectrum", "spec.h5")
         npop = len(pl.pops)

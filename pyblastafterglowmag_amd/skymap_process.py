@@ -23,6 +23,12 @@ import numpy as np
 from . import gpu, h5io
 
 
+# the reference package's default configuration (interface.py:1001-1003)
+DEFAULT_CONF = {"nx": 64, "ny": 32, "extend_grid": 2, "fwhm_fac": 0.5, "lat_dist_method": "integ",
+                "intp_filter": {"type": None, "sigma": 2, "mode": "reflect"},
+                "hist_filter": {"type": None, "sigma": 2, "mode": "reflect"}}
+
+
 def _sorted_files(pattern):
     files = glob.glob(pattern)
     if not files:

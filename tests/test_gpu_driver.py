@@ -68,3 +68,31 @@ def test_driver_raw_skymaps_match_reference(tmp_path):
         for k, ref in zip(["flux", "nshells", "nlayers", "time", "freq", "xmin", "ymin", "xmax", "ymax"],
                           g[f"sky{i}_attrs"]):
             assert h5io.attr(f, k) == pytest.approx(ref, rel=1e-6, abs=0), (i, k)
+
+
+def test_driver_processes_skymaps(tmp_path):
+    """--process-skymaps: the raw maps binned into skymap.h5 (one group per map, the histogram image
+    on the package's default 64 x 32 grid, the centroid) as the reference package does after a run."""
+    import numpy as np
+    from pyblastafterglowmag_amd import pba
+    from pyblastafterglowmag_amd.skymap_process import entries_of
+    name = "tophat_pw_small"
+    main, grb, struct = _case(name)
+    main = dict(main, skymap_times="array 1e5 1e7", skymap_freqs="array 1e9 1e18")
+    grb = dict(grb, do_skymap="yes", save_raw_skymap="yes")
+    wd = str(tmp_path) + "/"
+    inputs.write_run(wd, main, grb, struct)
+    print(f"[gpu] driver sky-map processing on {name}", flush=True)
+    assert pba.main([wd, "parfile.par", "err", "--process-skymaps"]) == 0
+    out = wd + "skymap.h5"
+    assert np.array_equal(h5io.read(out, "times"), [1e5, 1e7, 1e5, 1e7])
+    for i, (t, nu) in enumerate([(1e5, 1e9), (1e7, 1e9), (1e5, 1e18), (1e7, 1e18)]):
+        grp = "time={:.4e} freq={:.4e}".format(t, nu)
+        f = wd + f"raw_skymap_{i}.h5"
+        ents = entries_of([tuple(h5io.read(f, f"shell=0/{v}") for v in ("xrs", "yrs", "intensity"))])
+        x, y, w = (np.concatenate([e[k] for e in ents]) for k in range(3))
+        hist = h5io.read(out, f"{grp}/image_hist")
+        assert hist.size == 63 * 31
+        # every point lies inside the extent-based grid, so the histogram holds the total intensity
+        assert hist.sum() == pytest.approx(w.sum(), rel=1e-12)
+        assert h5io.attr(out, "xc", grp) == pytest.approx(np.sum(w * x) / np.sum(w), rel=1e-12, abs=1e-12 * np.abs(x).max())
