@@ -49,6 +49,39 @@ def test_version_needs_no_device():
     assert lib.pbagpu_version() > 0
 
 
+def test_skymap_hist_rejects_bad_arguments_before_any_device_call():
+    """pbagpu_skymap_hist checks its host arrays first: PBAGPU_E_ARG (-1) with a message, no HIP call."""
+    if not os.path.exists(LIB):
+        pytest.skip("libpbagpu.so not built")
+    lib = C.CDLL(LIB)
+    lib.pbagpu_last_error.restype = C.c_char_p
+    f = lib.pbagpu_skymap_hist
+    f.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_long)] + [C.c_void_p] * 3 + \
+        [C.c_int, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 7
+    dummy = C.c_void_p(8)   # never dereferenced: every case fails validation first
+    seg = (C.c_int * 2)(0, 1)
+    off = (C.c_long * 2)(0, 4)
+
+    def call(n=1, s=seg, o=off, nx=5, ny=5, eg=1.0):
+        return f(n, s, o, dummy, dummy, dummy, nx, ny, eg, 0, dummy, dummy, dummy, None, dummy, dummy, None)
+    assert call(n=0) == -1
+    assert call(nx=1) == -1 and call(ny=1) == -1 and call(nx=3000, ny=1200) == -1
+    assert call(s=(C.c_int * 2)(1, 1)) == -1 and b"img_seg[0]" in lib.pbagpu_last_error()
+    assert call(s=(C.c_int * 2)(0, -1)) == -1
+    assert call(o=(C.c_long * 2)(0, -3)) == -1 and b"seg_off decreases" in lib.pbagpu_last_error()
+    assert call(eg=float("nan")) == -1
+
+
+def test_bessel_host_routine_argument_checks():
+    if not os.path.exists(LIB):
+        pytest.skip("libpbagpu.so not built")
+    lib = C.CDLL(LIB)
+    lib.pbagpu_bessel_k_thirds.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    assert lib.pbagpu_bessel_k_thirds(-1, None, None, None) == -1
+    assert lib.pbagpu_bessel_k_thirds(3, None, None, None) == -1
+    assert lib.pbagpu_bessel_k_thirds(0, None, None, None) == 0
+
+
 _STRUCTS = {
     "pba_shock_params": abi.ShockParams,
     "pba_run_params": abi.RunParams,
