@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
-# algorithmic work per unit (counted from the source; see DESIGN.md for the derivations)
+# algorithmic work per unit, counted from the kernels' source
 FLOPS_PER_RHS = {0: 250.0, 1: 330.0}   # one right-hand side of one blastwave (forward shock / with reverse shock)
 BYTES_PER_SAMPLE = 144.0               # one EATS integrand sample: 7 state columns at 2 steps + 4 emissivities
 FLOPS_PER_CELL = 152.0                 # one piece-wise cell flux, forward-shock term (reverse-shock term not counted)
@@ -186,7 +186,7 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(f"bench.py: --gpus {args.gpus} requested; this build runs on one GPU only "
+        print(f"bench.py: --gpus {args.gpus} requested; the library runs on one GPU only "
               f"(the radiation hot path is measured on a single MI355X).", file=sys.stderr)
         sys.exit(2)
     import torch

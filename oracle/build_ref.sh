@@ -1,7 +1,7 @@
 #!/usr/bin/env bash
 # Build the reference CPU program (pba.out) from its single translation unit
-# under /root/reference/src, into oracle/_ref/ (git-ignored).  Only used as
-# the checker that writes golden fixtures and as the "reference" CPU baseline.
+# under /root/reference/src, into oracle/_ref/ (git-ignored).  Only used to
+# write the golden fixtures and as the "reference" CPU baseline.
 # HDF5 1.10 (C++ API) comes from the image's /opt/conda; only the HDF5 shared
 # objects and their compression/transport dependencies are exposed through a
 # private symlink folder, so that conda's older libstdc++ is not picked up.

@@ -1,8 +1,8 @@
 /*
  * CPU oracle of the per-blastwave radiation hot path — TEST INFRASTRUCTURE ONLY.
  *
- * A plain, scalar, FP64 restatement of the reference algorithm, used as the
- * checker by tests/, by __graft_entry__.smoke() and as the CPU baseline leg of
+ * A plain, scalar, FP64 restatement of the reference algorithm, used to check
+ * the GPU path in tests/ and __graft_entry__.smoke(), and as the CPU baseline leg of
  * bench.py.  Nothing in the product (pyblastafterglowmag_amd/, include/) links
  * or calls it.  Each function cites the reference file:line it restates.
  */
