@@ -129,3 +129,79 @@ def test_interp_image_reproduces_linear_field():
     assert inside.sum() > 100
     assert np.allclose(img[inside], lin(X, Y)[inside], rtol=1e-12)
     assert not inside[0, 0] and not inside[-1, -1]
+
+
+def _oracle_skymap_hist(images, nx, ny, extend_grid=1.0, edges=None, want_count=False, device=0):
+    """The oracle in place of gpu.skymap_hist, with the same result layout (host-logic tests only)."""
+    n = len(images)
+    out = dict(hist=np.zeros((n, nx - 1, ny - 1)), count=None, edges_x=np.zeros((n, nx)), edges_y=np.zeros((n, ny)),
+               moments=np.zeros((n, 8)), status=np.zeros(n, dtype=np.int32))
+    for i, ents in enumerate(images):
+        xs, ys, ds = [e[0] for e in ents], [e[1] for e in ents], [e[2] for e in ents]
+        ext = osky.data_extent(xs, ys, ds)
+        if edges is None:
+            ex, ey = osky.edges(ext[0], ext[1], extend_grid, nx), osky.edges(ext[2], ext[3], extend_grid, ny)
+        else:
+            ex, ey = np.asarray(edges[0])[i], np.asarray(edges[1])[i]
+        out["edges_x"][i], out["edges_y"][i] = ex, ey
+        out["hist"][i] = sum(osky.hist_entry(x, y, d, ex, ey) for x, y, d in zip(xs, ys, ds))
+        xc, yc = osky.centroid(xs, ys, ds)
+        out["moments"][i] = [xc, yc, ext[0], ext[1], ext[2], ext[3], np.concatenate(ds).sum(),
+                             sum(len(d) for d in ds)]
+    return out
+
+
+def _write_raw_maps(dirpath, name):
+    from pyblastafterglowmag_amd import h5io
+    os.makedirs(dirpath, exist_ok=True)
+    for i, sh in enumerate(golden_shells(name)):
+        with h5io.Writer(os.path.join(dirpath, f"raw_skymap_{i}.h5")) as w:
+            for v, a in zip(("xrs", "yrs", "intensity"), sh[0]):
+                w.dataset("shell=0", v, a)
+            for k, v in (("nshells", 1.0), ("time", [1e5, 1e7][i % 2]), ("freq", [1e9, 1e18][i // 2]), ("eps_e", 0.1)):
+                w.attr("/", k, v)
+
+
+def test_process_raw_skymaps_host_logic(tmp_path, monkeypatch):
+    """ProcessRawSkymap's file handling, group names, attributes and common grids, with the oracle
+    standing in for the GPU binning call: files sorted by index, one "time=.. freq=.." group each,
+    the raw attributes copied, nshells = kept shells, and process_multiples' edges spanning both runs."""
+    from pyblastafterglowmag_amd import gpu, h5io
+    monkeypatch.setattr(gpu, "skymap_hist", _oracle_skymap_hist)
+    da, db = str(tmp_path / "a"), str(tmp_path / "b")
+    _write_raw_maps(da, "tophat_pw_small")
+    _write_raw_maps(db, "gauss_pw_small")
+    prep = sp.ProcessRawSkymap(conf=sp.DEFAULT_CONF)
+    out = str(tmp_path / "skymap.h5")
+    prep.process_singles(infpaths=os.path.join(da, "raw_skymap_*.h5"), outfpath=out)
+    assert np.array_equal(h5io.read(out, "times"), [1e5, 1e7, 1e5, 1e7])
+    assert np.array_equal(h5io.read(out, "freqs"), [1e9, 1e9, 1e18, 1e18])
+    groups = sorted(h5io.members(out, "/"))
+    assert groups == sorted(["freqs", "times"] + ["time={:.4e} freq={:.4e}".format(t, f)
+                                                  for f in (1e9, 1e18) for t in (1e5, 1e7)])
+    g = "time=1.0000e+07 freq=1.0000e+18"
+    assert sorted(h5io.members(out, g)) == ["dist_x", "dist_y", "grid_x", "grid_y", "image_hist", "image_intp"]
+    assert h5io.attr(out, "nshells", g) == 1.0 and h5io.attr(out, "eps_e", g) == 0.1
+    for k in ("xc", "yc", "x1", "x2", "y1", "y2", "time", "freq"):
+        h5io.attr(out, k, g)
+    sh = golden_shells("tophat_pw_small")[3]
+    xs, ys, ds = osky.get_skymap(sh)
+    ext = osky.data_extent(xs, ys, ds)
+    ex, ey = osky.edges(ext[0], ext[1], 2, 64), osky.edges(ext[2], ext[3], 2, 32)
+    assert np.array_equal(h5io.read(out, f"{g}/image_hist").reshape(63, 31), osky.hist_image(xs, ys, ds, ex, ey))
+    assert h5io.read(out, f"{g}/image_intp").size == 63 * 31
+    x1, x2 = h5io.attr(out, "x1", g), h5io.attr(out, "x2", g)
+    assert ex[0] <= x1 <= x2 <= ex[-1]
+    # common grids over two runs
+    oa, ob = str(tmp_path / "ma.h5"), str(tmp_path / "mb.h5")
+    prep.process_multiples(infpaths=(os.path.join(da, "raw_skymap_*.h5"), os.path.join(db, "raw_skymap_*.h5")),
+                           outfpaths=(oa, ob))
+    ea = osky.data_extent(*osky.get_skymap(golden_shells("tophat_pw_small")[3]))
+    eb = osky.data_extent(*osky.get_skymap(golden_shells("gauss_pw_small")[3]))
+    gx = np.linspace(min(ea[0], eb[0]) * 2, max(ea[1], eb[1]) * 2, num=64)
+    for path in (oa, ob):
+        assert np.array_equal(h5io.read(path, f"{g}/grid_x"), 0.5 * (gx[1:] + gx[:-1]))
+    with pytest.raises(IndexError):
+        prep.process_multiples(infpaths=(da,), outfpaths=(oa, ob))
+    with pytest.raises(FileNotFoundError):
+        prep.process_singles(infpaths=os.path.join(str(tmp_path), "none_*.h5"), outfpath=out)
