@@ -173,7 +173,6 @@ static double radiation_loss(const Shock &sh, MState &st, double dmdR, double dE
         if (gc <= 1.) gc = 1.001;
         if (gc > gM) gc = gM;
         p_tot = wspn99_ptot(s.p, gm, gc, gM, B, dmdR / mp, 1e6);
-        if (getenv("ORC_RL_TRACE")) fprintf(stderr, "    RL gm=%.6g gc=%.6g gM=%.6g B=%.6g acc=%.6g tcom=%.6g ptot=%.6g betaSh=%.6g\n", gm, gc, gM, B, st.acc, st.tcomov, p_tot, betaSh);
     }
     p_tot *= st.acc;
     p_tot = std::max(p_tot, 0.);
@@ -581,7 +580,6 @@ static int dop853(Run &run, Dop &d, double &time, double *value, double timeN, d
         if ((time + 1.01 * h - timeN) > 0.0) { h = timeN - time; last = true; }
         double err, stf;
         integrate8(run, d, time, value, h, atol, rtol, err, stf);
-        if (getenv("ORC_DYN_TRACE")) fprintf(stderr, "  t=%.17g h=%.17g err=%.17g\n", time, h, err);
         double fac11 = std::pow(err, expo1);
         double fac = fac11 / (1.0 * safe);
         fac = fac < facc2 ? facc2 : (fac > facc1 ? facc1 : fac);
