@@ -5,7 +5,7 @@ parfile + initial-data files written by pyblastafterglowmag_amd.inputs, then sto
 reference's outputs (lc.h5 light curves, dyn.h5 dynamics, spec.h5 comoving spectra) as
 compressed .npz next to this script.  Only data is stored (inputs and outputs).
 
-    python tests/golden/make_golden.py [name ...]
+    python tests/golden/make_golden.py [name ...]      (no name: every fixture, ALL)
 """
 import os
 import shutil
@@ -141,5 +141,10 @@ def main(names):
             shutil.rmtree(wd, ignore_errors=True)
 
 
+# every fixture the tests load
+ALL = ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gauss_pw_small", "tophat64_pw",
+       "tophat_ssa_small", "gauss5_ssa_small", "tophat_dermer_small", "tophat_bessel_small", "tophat_gsl_small",
+       "tophat_rk4_small"]
+
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gauss_pw_small"])
+    main(sys.argv[1:] or ALL)
