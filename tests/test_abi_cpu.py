@@ -160,6 +160,20 @@ def test_unsupported_options_raise(tmp_path):
     abi.run_params_from_parfile(par)
 
 
+def test_synchrotron_kernel_options(tmp_path):
+    """method_synchrotron_fs strings of the numeric electron path map to pba_shock_params.syn
+    (microphysics_base.h:310-335); the analytic-path kernels are rejected."""
+    main, grb, struct = workloads.tophat_fs_syn()
+    for name, code in (("CSYN", 0), ("Dermer09", 1), ("Bessel", 2), ("GSL", 3)):
+        par = inputs.write_parfile(str(tmp_path), main, dict(grb, method_synchrotron_fs=name))
+        P = abi.run_params_from_parfile(par)[0]
+        assert P.fs.syn == code
+    for name in ("Joh06", "WSPN99", "Marg21"):
+        par = inputs.write_parfile(str(tmp_path), main, dict(grb, method_synchrotron_fs=name))
+        with pytest.raises(NotImplementedError):
+            abi.run_params_from_parfile(par)
+
+
 def test_layer_inits_piecewise():
     """Piece-wise blastwaves: E0, M0 per cell (divided by 2i+1, ejecta_id.h:315-322), theta_b0 =
     theta_wing and ncells = n^2 (blastwave_base.h:304-312); adaptive keeps ncells = 1, theta_b0 = theta_c_h."""
