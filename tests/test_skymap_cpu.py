@@ -205,3 +205,20 @@ def test_process_raw_skymaps_host_logic(tmp_path, monkeypatch):
         prep.process_multiples(infpaths=(da,), outfpaths=(oa, ob))
     with pytest.raises(FileNotFoundError):
         prep.process_singles(infpaths=os.path.join(str(tmp_path), "none_*.h5"), outfpath=out)
+
+
+def test_entries_of_several_shells():
+    """Several shells (_get_skymap, :191-328): a shell whose intensities sum to zero is dropped, the
+    principal-jet halves of the kept shells come first, then their counter-jet halves."""
+    rng = np.random.default_rng(4)
+    sh = [tuple(rng.normal(size=6) for _ in range(3)) for _ in range(3)]
+    sh[1] = (sh[1][0], sh[1][1], np.zeros(6))
+    got = sp.entries_of(sh)
+    xs, ys, ds = osky.get_skymap(sh)
+    assert len(got) == 4 and [len(e[0]) for e in got] == [3, 3, 3, 3]
+    assert np.array_equal(got[0][0], sh[0][0][:3]) and np.array_equal(got[1][0], sh[2][0][:3])
+    assert np.array_equal(got[2][0], sh[0][0][3:]) and np.array_equal(got[3][2], sh[2][2][3:])
+    for (x, y, w), xo, yo, do in zip(got, xs, ys, ds):
+        assert np.array_equal(x, xo) and np.array_equal(y, yo) and np.array_equal(w, do)
+    with pytest.raises(ValueError):
+        sp.entries_of([(np.ones(5), np.ones(5), np.ones(5))])
