@@ -101,3 +101,61 @@ def test_raw_skymap_file_layout(name, tmp_path):
         assert sorted(h5io.members(path, "shell=0")) == sorted(abi.IMG_NAMES)
         assert h5io.attr(path, "time") == [1e5, 1e7][i % 2] and h5io.attr(path, "freq") == [1e9, 1e18][i // 2]
         np.testing.assert_array_equal(h5io.read(path, "shell=0/intensity"), g[f"sky{i}_intensity"])
+
+
+def _image(pj, cj):
+    """One raw image [1][PBA_IMG_NV][2 nc]: positions spread on a grid, intensities as given."""
+    nc = len(pj)
+    img = np.zeros((1, abi.PBA_IMG_NV, 2 * nc))
+    for v in range(abi.PBA_IMG_NV):
+        img[0, v] = np.arange(2 * nc) + 10.0 * v
+    img[0, abi.IMG["xrs"]] = np.linspace(-2.0, 3.0, 2 * nc)
+    img[0, abi.IMG["yrs"]] = np.linspace(1.0, -1.0, 2 * nc) ** 3
+    img[0, abi.IMG["intensity"]] = np.concatenate([pj, cj])
+    return img
+
+
+def test_skymap_zero_cell_removal_and_normalisation():
+    """computeEjectaSkyMapPW's normalisation and removeZeroEntries (src/image.h:120-190) on a
+    hand-made image: cells at or below 1e-10 of the peak in both jets are dropped (both jets'
+    entries of a kept cell stay), intensities scale by flux / (dx dy) / max I, bounds span all cells."""
+    pj = np.array([1.0, 0.0, 1e-12, 2.0, 0.0])
+    cj = np.array([0.0, 0.0, 3.0, 0.0, 1e-20])
+    img = _image(pj, cj)
+    flux = np.array([[0.25], [0.5]])   # two layers
+    out = output.skymap_images(img, flux, 2)[0]
+    keep = np.array([0, 2, 3])
+    X, Y = img[0, abi.IMG["xrs"]], img[0, abi.IMG["yrs"]]
+    dfnu = 0.75 / ((X.max() - X.min()) * (Y.max() - Y.min()))
+    for n in abi.IMG_NAMES:
+        raw = img[0, abi.IMG[n]]
+        want = np.concatenate([raw[keep], raw[5 + keep]])
+        if n == "intensity":
+            want = want * (dfnu / 3.0)
+        assert np.array_equal(out["arrays"][n], want), n
+    assert out["fluxs"][0] == 0.75
+    a = out["attrs"]
+    assert a["flux"] == 0.75 and a["nshells"] == 1.0 and a["nlayers"] == 2.0
+    assert (a["xmin"], a["xmax"], a["ymin"], a["ymax"]) == (X.min(), X.max(), Y.min(), Y.max())
+
+
+def test_skymap_one_full_jet_keeps_every_cell():
+    """If one jet has every cell above the threshold nothing is removed (the reference moves the arrays)."""
+    pj = np.array([1.0, 2.0, 3.0])
+    cj = np.array([0.0, 0.0, 0.0])
+    out = output.skymap_images(_image(pj, cj), np.array([[1.0]]), 1)[0]
+    assert len(out["arrays"]["intensity"]) == 6
+
+
+def test_skymap_without_positive_flux_is_not_normalised():
+    """A non-positive (or non-finite) shell flux leaves the intensities as computed and the bounds at
+    their start values (numeric_limits<double>::max() / ::min())."""
+    pj = np.array([1.0, 0.0])
+    cj = np.array([0.5, 0.0])
+    img = _image(pj, cj)
+    for fl in (0.0, -1.0, np.nan):
+        out = output.skymap_images(img, np.array([[fl]]), 1)[0]
+        assert np.array_equal(out["arrays"]["intensity"], np.array([1.0, 0.5]))
+        a = out["attrs"]
+        assert a["xmin"] == np.finfo(float).max and a["xmax"] == np.finfo(float).tiny
+        assert a["flux"] == fl or (np.isnan(fl) and np.isnan(a["flux"]))
