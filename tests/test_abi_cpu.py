@@ -236,3 +236,22 @@ def test_latin_hypercube_members():
         with tempfile.TemporaryDirectory() as d:
             P = abi.run_params_from_parfile(inputs.write_parfile(d, main, grb))[0]
         assert P.n_ism == pytest.approx(main["n_ism"], rel=1e-12) and P.fs.p == pytest.approx(grb["p_fs"], rel=1e-12)
+
+
+def test_batch_populations_and_layer_descriptors():
+    """gpu.populations: a forward-shock population for every blastwave of every run, a reverse-shock
+    one only where the blastwave was evolved with the reverse shock and that run radiates from it
+    (considerReverseShock, src/blastwave/blastwave_radiation.h:961-974); layer descriptors point at
+    their populations and carry each blastwave's i_end_r."""
+    from pyblastafterglowmag_amd import gpu
+    params = (abi.RunParams * 2)()
+    params[0].do_rs_radiation = 1
+    params[1].do_rs_radiation = 0
+    res = (abi.BwResult * 6)()
+    for bw, (fsrs, iend) in enumerate([(1, 10), (0, 11), (1, 12), (1, 20), (1, 21), (0, 22)]):
+        res[bw].fsrs, res[bw].i_end_r = fsrs, iend
+    pops, lays = gpu.populations(params, res, 2, 3)
+    assert [(p.bw, p.run, p.shock) for p in pops] == [(0, 0, 0), (0, 0, 1), (1, 0, 0), (2, 0, 0), (2, 0, 1),
+                                                       (3, 1, 0), (4, 1, 0), (5, 1, 0)]
+    assert lays == [(0, 0, 0, 1, 10), (1, 0, 2, -1, 11), (2, 0, 3, 4, 12),
+                    (3, 1, 5, -1, 20), (4, 1, 6, -1, 21), (5, 1, 7, -1, 22)]
