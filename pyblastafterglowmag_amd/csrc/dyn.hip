@@ -703,6 +703,11 @@ extern "C" int pbagpu_dyn_evolve(const pba_run_params *params, int n_runs, int b
     if (!params || !init || !d_tgrid || !d_table || !res || n_runs < 1 || bw_per_run < 1 || bw_per_run > 64 || nt < 2)
         return PBAGPU_E_ARG;
     hipStream_t s = (hipStream_t)stream;
+    for (int r = 0; r < n_runs; r++)
+        if (params[r].integrator != 0 && params[r].integrator != 1) {
+            pbagpu::set_error_msg("pba_run_params: integrator must be 0 (DOP853E) or 1 (RK4)");
+            return PBAGPU_E_UNSUPPORTED;
+        }
     const size_t nbw = (size_t)n_runs * bw_per_run;
     for (size_t i = 0; i < nbw; i++) {
         // ncells < 1 is an error of the reference too (blastwave.h:175-181)

@@ -255,3 +255,24 @@ def test_batch_populations_and_layer_descriptors():
                                                        (3, 1, 0), (4, 1, 0), (5, 1, 0)]
     assert lays == [(0, 0, 0, 1, 10), (1, 0, 2, -1, 11), (2, 0, 3, 4, 12),
                     (3, 1, 5, -1, 20), (4, 1, 6, -1, 21), (5, 1, 7, -1, 22)]
+
+
+def test_dyn_entry_rejects_unknown_integrator_before_any_device_call():
+    if not os.path.exists(LIB):
+        pytest.skip("libpbagpu.so not built")
+    lib = C.CDLL(LIB)
+    lib.pbagpu_last_error.restype = C.c_char_p
+    f = lib.pbagpu_dyn_evolve
+    f.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.c_int, C.POINTER(abi.LayerInit), C.c_int, C.c_void_p,
+                  C.c_void_p, C.POINTER(abi.BwResult), C.c_void_p]
+    P = (abi.RunParams * 1)()
+    P[0].integrator = 2
+    init = (abi.LayerInit * 1)()
+    init[0].ncells, init[0].theta_b0 = 1.0, 0.1
+    res = (abi.BwResult * 1)()
+    dummy = C.c_void_p(8)   # never dereferenced: the call fails validation first
+    assert f(P, 1, 1, init, 10, dummy, dummy, res, None) == -3
+    assert b"integrator" in lib.pbagpu_last_error()
+    P[0].integrator = 0
+    init[0].ncells = 0.5
+    assert f(P, 1, 1, init, 10, dummy, dummy, res, None) == -1
