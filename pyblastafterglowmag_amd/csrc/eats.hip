@@ -660,7 +660,8 @@ extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, 
                                       const double *d_aall, int nfreq, int n_pairs, const double *d_tobs,
                                       const double *d_nuobs, double *d_out, int *d_status, void *stream) {
     using namespace pbagpu::eats;
-    if (!params || !layers || !d_table || !d_jall || !d_out || !d_status || n_layers < 1 || n_pairs < 1 || nt < 2)
+    if (!params || !layers || !d_table || !d_jall || !d_out || !d_status || n_runs < 1 || n_layers < 1 ||
+        n_pairs < 1 || nt < 2 || nfreq < 2)
         return PBAGPU_E_ARG;
     if (sizeof(double) * ((size_t)NSLOT * nt + 2 * (size_t)nfreq) > 160 * 1024) {
         pbagpu::set_error_msg("eats: time grid too long for the LDS-staged table (nt <= 2000)");
@@ -679,7 +680,12 @@ extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, 
         }
     for (int l = 0; l < n_layers; l++) {
         const pba_eats_layer &Y = layers[l];
-        if (Y.run < 0 || Y.run >= n_runs || Y.i_end_r > nt || Y.i_end_r < 0 || Y.pop_fs < 0) return PBAGPU_E_ARG;
+        if (Y.run < 0 || Y.run >= n_runs || Y.i_end_r > nt || Y.i_end_r < 0 || Y.pop_fs < 0 || Y.pop_rs < -1 ||
+            Y.bw < 0) {
+            pbagpu::set_error_msg("eats_lightcurve: bad layer descriptor (run, bw, pop_fs >= 0, pop_rs >= -1, "
+                                  "0 <= i_end_r <= nt)");
+            return PBAGPU_E_ARG;
+        }
     }
     auto *dnu = (double *)pbagpu::workspace(pbagpu::WS_EATS_NU, sizeof(double) * hnu.size());
     auto *dL = (LayerArgs *)pbagpu::workspace(pbagpu::WS_EATS_LAYERS, sizeof(LayerArgs) * n_layers);

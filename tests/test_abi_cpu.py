@@ -276,3 +276,25 @@ def test_dyn_entry_rejects_unknown_integrator_before_any_device_call():
     P[0].integrator = 0
     init[0].ncells = 0.5
     assert f(P, 1, 1, init, 10, dummy, dummy, res, None) == -1
+
+
+def test_eats_entry_rejects_bad_layer_descriptors_before_any_device_call():
+    if not os.path.exists(LIB):
+        pytest.skip("libpbagpu.so not built")
+    lib = C.CDLL(LIB)
+    lib.pbagpu_last_error.restype = C.c_char_p
+    f = lib.pbagpu_eats_lightcurve
+    f.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsLayer), C.c_int, C.c_void_p, C.c_int,
+                  C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                  C.c_void_p]
+    P = (abi.RunParams * 1)()
+    for sh in (P[0].fs, P[0].rs):
+        sh.nfreq, sh.freq1, sh.freq2 = 8, 1e5, 1e32
+    dummy = C.c_void_p(8)   # never dereferenced: every call fails validation first
+
+    def call(bw=0, pop_fs=0, pop_rs=-1, iend=10, n_runs=1):
+        L = (abi.EatsLayer * 1)(abi.EatsLayer(0.0, 0.1, bw, 0, pop_fs, pop_rs, iend))
+        return f(P, n_runs, L, 1, dummy, 10, dummy, None, 8, 4, dummy, dummy, dummy, dummy, None)
+    assert call(bw=-1) == -1 and b"layer descriptor" in lib.pbagpu_last_error()
+    assert call(pop_rs=-2) == -1 and call(pop_fs=-1) == -1 and call(iend=11) == -1
+    assert call(n_runs=0) == -1 and call(n_runs=-5) == -1
