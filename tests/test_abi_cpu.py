@@ -298,3 +298,20 @@ def test_eats_entry_rejects_bad_layer_descriptors_before_any_device_call():
     assert call(bw=-1) == -1 and b"layer descriptor" in lib.pbagpu_last_error()
     assert call(pop_rs=-2) == -1 and call(pop_fs=-1) == -1 and call(iend=11) == -1
     assert call(n_runs=0) == -1 and call(n_runs=-5) == -1
+
+
+def test_product_path_fails_loudly_without_gpu_or_library(monkeypatch):
+    """No CPU fallback: without a ROCm device the GPU path raises, and without the built library
+    loading it raises with the path it looked for."""
+    import torch
+    from pyblastafterglowmag_amd import gpu
+    if not torch.cuda.is_available():
+        with pytest.raises(RuntimeError, match="ROCm device"):
+            gpu.skymap_hist([[(np.zeros(2), np.zeros(2), np.ones(2))]], 5, 5)
+        import __graft_entry__
+        with pytest.raises(RuntimeError):
+            __graft_entry__.smoke()
+    monkeypatch.setattr(gpu, "_lib", None)
+    monkeypatch.setattr(gpu, "_LIB_PATH", "/nonexistent/libpbagpu.so")
+    with pytest.raises(RuntimeError, match="GPU library missing"):
+        gpu.lib()
