@@ -315,3 +315,13 @@ def test_product_path_fails_loudly_without_gpu_or_library(monkeypatch):
     monkeypatch.setattr(gpu, "_LIB_PATH", "/nonexistent/libpbagpu.so")
     with pytest.raises(RuntimeError, match="GPU library missing"):
         gpu.lib()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one MI355X: --gpus N > 1 exits with a message instead of running."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 2 and "one GPU" in p.stderr
