@@ -7,11 +7,14 @@ light curve at 96 nu_obs x 128 t_obs (theta_obs = 0.363 rad, n_ism = 10^-1.6 cm^
 E_iso,c = 10^54.1 erg, Gamma0 = 300, theta_c = 0.061, theta_w = 0.436).
 
 One step = one batch of independent runs of that workload through the three GPU stages, with
-every input and intermediate resident in device memory.  Batches are double-buffered: while
-the electron and observer stages of batch i run on one stream, the dynamics of batch i+1 run on
-a second stream (the dynamics kernel occupies one wavefront per run, the observer kernel the
-whole chip).  One timed step therefore completes the light curves of one batch;
-value = batch x steps / timed seconds.
+every input and intermediate resident in device memory.  The dynamics of one run is a chain of
+~3e5 dependent ODE steps on one wavefront (tens of seconds of latency at a small share of the
+chip), while the electron and observer stages of a batch fill the whole chip, so batches are
+pipelined: step s launches the dynamics of batch s on its own stream (host thread) and runs the
+electron and observer stages of batch s - depth, whose dynamics are done, on the shared
+radiation stream.  Only light curves whose dynamics, electrons and observer stage all ran inside
+the timed steps are counted: of K timed steps, K - depth batches;
+value = batch x (K - depth) / timed seconds.
 
 The JSON line carries a roofline entry for the kernel with the most device time in the timed
 steps (HIP events recorded by the library on the stream each kernel runs on) and a CPU baseline:
@@ -39,9 +42,11 @@ FLOPS_PER_CELL = 152.0                 # one piece-wise cell flux, forward-shock
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=60, help="independent runs per step")
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--batch", type=int, default=12, help="independent runs per step")
+    ap.add_argument("--depth", type=int, default=3,
+                    help="pipeline depth: batches whose dynamics run while the radiation stages of an older batch run")
     ap.add_argument("--precision", type=int, default=0, help="0: FP64 everywhere")
     ap.add_argument("--workload", default="gauss21_fsrs_ssc",
                     choices=["gauss21_fsrs_ssc", "tophat_fs_syn", "tophat64_pw", "lhs_members"],
@@ -200,54 +205,79 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    if args.workload == "lhs_members":   # batch b holds members b*batch ... (b+1)*batch - 1
-        batches = [[gpu.Run.from_workload(*_workload(workloads, args.workload, b * args.batch + i))
-                    for i in range(args.batch)] for b in range(2)]
+    D = max(args.depth, 1)
+    if args.workload == "lhs_members":     # batch b holds members b*batch ... (b+1)*batch - 1
+        make = lambda b: [gpu.Run.from_workload(*_workload(workloads, args.workload, b * args.batch + i))
+                          for i in range(args.batch)]
     else:
-        batches = [[gpu.Run.from_workload(*_workload(workloads, args.workload, 0))] * args.batch] * 2
-    run = batches[0][0]
-    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
-    pls = [gpu.Pipeline(batches[0], stream=streams[0], precision=args.precision),
-           gpu.Pipeline(batches[1], stream=streams[1], precision=args.precision)]
+        one = gpu.Run.from_workload(*_workload(workloads, args.workload, 0))
+        make = lambda b: [one] * args.batch
+    run = make(0)[0]
+    # D + 1 pipelines in a ring: batch b uses ring[b % (D + 1)]; its dynamics run on that pipeline's own
+    # stream, its electron and observer stages on the shared radiation stream
+    rad_stream = torch.cuda.Stream()
+    ring = [gpu.Pipeline(make(k), stream=torch.cuda.Stream(), precision=args.precision, rad_stream=rad_stream)
+            for k in range(D + 1)]
     gpu.timing_enable(True)
-    t_fill = time.perf_counter()
-    pls[0].dynamics()          # fill the pipeline: dynamics of the first batch
-    print(f"[bench] pipeline filled: dynamics of batch 0 in {time.perf_counter() - t_fill:.1f}s", file=sys.stderr,
-          flush=True)
-
+    threads = {}
     rejected = {}
+    t_start = {}
 
-    def iteration(i):
-        cur, nxt = pls[i % 2], pls[(i + 1) % 2]
-        th = threading.Thread(target=nxt.dynamics)
+    def launch(b):
+        """Dynamics of batch b on its pipeline's stream, in a host thread (the call blocks until done)."""
+        pl = ring[b % (D + 1)]
+        if args.workload == "lhs_members" and b >= D + 1:
+            pl.__init__(make(b), stream=pl.stream, precision=args.precision, rad_stream=rad_stream)
+        t_start[b] = time.perf_counter()
+        th = threading.Thread(target=pl.dynamics)
         th.start()
-        cur.electrons()
-        cur.observer()
-        th.join()
-        print(f"[bench] step {i} done (batch {i} light curves, batch {i + 1} dynamics)", file=sys.stderr, flush=True)
-        dst, est, lst = cur.status()
+        threads[b] = th
+
+    def radiation(b):
+        pl = ring[b % (D + 1)]
+        threads.pop(b).join()
+        pl.electrons()
+        pl.observer()
+        dst, est, lst = pl.status()
         if dst.any() or est.any() or lst.any():
-            bad = rejected_runs(cur, dst, est, lst)
+            bad = rejected_runs(pl, dst, est, lst)
             if args.workload != "lhs_members":
                 print(f"bench.py: non-zero status dyn={sorted(set(dst.tolist()))} ele={sorted(set(est.tolist()))} "
                       f"eats={sorted(set(lst.tolist()))}", file=sys.stderr)
                 sys.exit(3)
             # a member the reference would stop on (its status bits mark where) yields no light curve
-            rejected[i] = len(bad)
-            print(f"[bench] step {i}: {len(bad)} of {cur.n_runs} members flagged (no light curve): {sorted(bad)}",
+            rejected[b] = len(bad)
+            print(f"[bench] batch {b}: {len(bad)} of {pl.n_runs} members flagged (no light curve): {sorted(bad)}",
                   file=sys.stderr, flush=True)
 
-    for i in range(args.warmup):
-        iteration(i)
+    last_b = args.warmup + args.steps - 1 - D   # the last batch whose radiation stages run
+
+    def step(s):
+        if s <= last_b:
+            launch(s)
+        if s >= D:
+            radiation(s - D)
+        print(f"[bench] step {s} done ({'dynamics of batch %d launched, ' % s if s <= last_b else ''}"
+              f"{'light curves of batch %d' % (s - D) if s >= D else 'filling the pipeline'}) "
+              f"t={time.perf_counter() - t_fill:.1f}s", file=sys.stderr, flush=True)
+
+    t_fill = time.perf_counter()
+    for s in range(args.warmup):
+        step(s)
     torch.cuda.synchronize()
     for st in range(gpu.N_STAGES):
         gpu.timing_read(st, reset=True)
         gpu.timing_units(st, reset=True)
     t0 = time.perf_counter()
-    for i in range(args.warmup, args.warmup + args.steps):
-        iteration(i)
+    for s in range(args.warmup, args.warmup + args.steps):
+        step(s)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    for th in threads.values():
+        th.join()
+    # batches whose dynamics started inside the timed steps and whose light curves were done by their end
+    counted = list(range(args.warmup, last_b + 1))
+    pls = [ring[counted[0] % (D + 1)]] if counted else ring[:1]
     stage_ms = {s: gpu.timing_read(s) for s in range(gpu.N_STAGES)}
     if args.stages:
         for s, (ms, n) in stage_ms.items():
@@ -257,8 +287,8 @@ def main():
               file=sys.stderr)
     roof = roofline(stage_ms, pls, gpu)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload, args.cpu_seconds)
-    n_rej = sum(rejected.get(i, 0) for i in range(args.warmup, args.warmup + args.steps))
-    lcs = (args.batch * args.steps - n_rej) / dt
+    n_rej = sum(rejected.get(b, 0) for b in counted)
+    lcs = (args.batch * len(counted) - n_rej) / dt
     line = {
         "metric": "light curves/sec (96nu x128t), plain FP64 vs mixed+surrogate GPU @<=1% err",
         "value": lcs, "unit": "light curves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -266,7 +296,8 @@ def main():
         "dtype": "f64" if args.precision == 0 else "f64+bf16", "data": "synthetic",
         "config": {"workload": args.workload, "runs_per_step": args.batch, "layers": run.n_layers,
                    "ngam": run.params.fs.ngam, "nfreq": run.params.fs.nfreq, "nt": len(run.tgrid),
-                   "pairs": len(run.tobs), "precision": args.precision, "pipeline": "double-buffered",
+                   "pairs": len(run.tobs), "precision": args.precision, "pipeline_depth": D,
+                   "batches_counted": len(counted), "light_curves_counted": args.batch * len(counted) - n_rej,
                    **({"members_flagged": n_rej, "members": "Latin hypercube, seed 20261203, runs b*batch .. "
                        "(b+1)*batch-1 per step"} if args.workload == "lhs_members" else {})},
         "roofline": roof, "cpu_baseline": cpu,

@@ -152,6 +152,16 @@ int pbagpu_version(void);
  * string lets the host reproduce that message. */
 const char *pbagpu_last_error(void);
 
+/* Streams and workspaces.  Every compute entry point runs on the stream it is given
+ * and keeps its device workspace (interval stores, staged parameters, kernel tables)
+ * per (device, stream): the stream is the context.  Calls on different streams may run
+ * at the same time from different host threads (one thread per stream); calls on one
+ * stream must come from one thread at a time, as the reference drives one model from
+ * one thread (src/main.cpp:157-195).  No counterpart in the reference, which owns no
+ * device memory: this frees the workspace of `stream` on the current device, after
+ * waiting for the stream; the next call on it allocates again. */
+int pbagpu_release_stream(void *stream);
+
 /* ---------------------------------------------------- kernel timing */
 /* No counterpart in the reference: instrumentation for benchmarks.  When
  * enabled, every stage records HIP events on the caller's stream immediately

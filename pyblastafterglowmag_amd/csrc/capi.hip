@@ -63,6 +63,8 @@ extern "C" int pbagpu_timing_read(int stage, double *ms_total, long long *launch
 #include "runtime.h"
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <utility>
 
 namespace pbagpu {
 static thread_local char g_err[512] = "";
@@ -74,21 +76,33 @@ void set_error(const char *what, hipError_t e) {
 void set_error_msg(const char *msg) { std::snprintf(g_err, sizeof(g_err), "%s", msg); }
 
 namespace {
-constexpr int kMaxDevices = 16;
 struct Buf { void *p = nullptr; size_t cap = 0; };
-Buf g_ws[kMaxDevices][WS_N];
+struct StreamBufs { Buf b[WS_N]; };
+// (device, stream) -> that stream's buffers; guarded by g_ws_mu.  Buffers are never moved, so
+// a call holds its pointers without the lock while other streams allocate.
+std::mutex g_ws_mu;
+std::map<std::pair<int, hipStream_t>, StreamBufs *> g_ws;
 }  // namespace
 
-void *workspace(int slot, size_t bytes) {
+void *workspace(int slot, size_t bytes, hipStream_t stream) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) { set_error("hipGetDevice", e); return nullptr; }
-    if (dev < 0 || dev >= kMaxDevices || slot < 0 || slot >= WS_N) { set_error_msg("workspace: bad device/slot"); return nullptr; }
-    Buf &b = g_ws[dev][slot];
+    if (slot < 0 || slot >= WS_N) { set_error_msg("workspace: bad slot"); return nullptr; }
+    StreamBufs *sb;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        auto &ref = g_ws[{dev, stream}];
+        if (!ref) ref = new StreamBufs();
+        sb = ref;
+    }
+    Buf &b = sb->b[slot];
     if (bytes == 0) bytes = 256;
     if (b.cap >= bytes) return b.p;
     if (b.p) {
-        e = hipFree(b.p);
+        // earlier work on this stream may still read the old buffer
+        e = hipStreamSynchronize(stream);
+        if (e == hipSuccess) e = hipFree(b.p);
         b.p = nullptr; b.cap = 0;
         if (e != hipSuccess) { set_error("hipFree(workspace)", e); return nullptr; }
     }
@@ -98,6 +112,32 @@ void *workspace(int slot, size_t bytes) {
     b.cap = want;
     return b.p;
 }
+
+int release_stream(hipStream_t stream) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { set_error("hipGetDevice", e); return PBAGPU_E_HIP; }
+    StreamBufs *sb = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        auto it = g_ws.find({dev, stream});
+        if (it == g_ws.end()) return PBAGPU_OK;
+        sb = it->second;
+        g_ws.erase(it);
+    }
+    int rc = PBAGPU_OK;
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) { set_error("hipStreamSynchronize", e); rc = PBAGPU_E_HIP; }
+    for (int k = 0; k < WS_N; k++)
+        if (sb->b[k].p) {
+            e = hipFree(sb->b[k].p);
+            if (e != hipSuccess) { set_error("hipFree(workspace)", e); rc = PBAGPU_E_HIP; }
+        }
+    delete sb;
+    return rc;
+}
 }  // namespace pbagpu
 
 extern "C" const char *pbagpu_last_error(void) { return pbagpu::g_err; }
+
+extern "C" int pbagpu_release_stream(void *stream) { return pbagpu::release_stream((hipStream_t)stream); }

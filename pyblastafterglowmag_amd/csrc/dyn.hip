@@ -33,11 +33,12 @@ constexpr int kMaxStepsPerInterval = 10000000;
 
 struct MState { double acc, tcomov; };
 
-struct Shock { double e0, etop; };
+struct Shock { double e0, etop, nu0_43; };
 
 struct BW {
     double E0, M0, R0, Gamma0, beta0, mom0, theta_b0, theta_core, theta_max, rho_def;
     double ncells;     // cells sharing this blastwave: 1 (adaptive EATS) or the total cell count (piece-wise)
+    double tan_ratio;  // tan(theta_b0 / 2) / tan(theta_core / 2) of the spreading rate
     double Rw0, Wtt0;  // passive components of the reference state vector (enter the step-size estimate)
     int ilayer, nlayers;
     int fsrs, do_rs, do_rs_rad, shutOff, end_evolution, end_spreading;
@@ -60,14 +61,17 @@ __device__ double dthetadr_our(const BW &b, double Gamma, double Gsh, double R, 
     if ((theta < 0.5 * kPi) && (Q0 * mom * b.theta_core < 1)) {
         double fac = 1.;
         if (mom * b.theta_core * Q >= 1.0) fac = Q * (1. - Q0 * mom * b.theta_core) / (Q - Q0);
-        if (b.theta_b0 < b.theta_core) fac *= tan(0.5 * b.theta_b0) / tan(0.5 * b.theta_core);
+        if (b.theta_b0 < b.theta_core) fac *= b.tan_ratio;   // tan(theta_b0/2) / tan(theta_core/2), set once
         double beta = beta_from_gamma(Gsh);
         return vperp / R / Gsh / beta / kC * fac;
     }
     return 0.;
 }
 
-__device__ double wspn99_ptot(double p, double gm, double gc, double gM, double B, double nprime, double nu0) {
+// Integrated WSPN99 synchrotron power (analytic_models.h:889-911, 1051-1078).  Every power x^a
+// is exp(a ln x) with each logarithm taken once (3 logarithms and 8 exponentials instead of 10
+// calls of pow); nu0_43 = nu0^(4/3) is a constant of the run.
+__device__ double wspn99_ptot(double p, double gm, double gc, double gM, double B, double nprime, double nu0_43) {
     double f = (3.0 / (4.0 * kPi)) * (kQe * B) / (kMe * kC);
     double nu_m, nu_c, p_max;
     if (gm < gc) {
@@ -83,29 +87,32 @@ __device__ double wspn99_ptot(double p, double gm, double gc, double gM, double 
         p_max = 2.234 * phipF * kQe * kQe * kQe * nprime * B / (kMe * kC * kC);
     }
     double nu_M = 3.0 / (4.0 * kPi) * gM * gM * kQe * B / (kMe * kC);
+    const double lm = log(nu_m), lc = log(nu_c), lM = log(nu_M);
     double t0, t1, t2;
     if (nu_m <= nu_c) {
-        t0 = (3. / 4.) * pow(1. / nu_m, 1. / 3.) * (pow(nu_m, 4. / 3.) - pow(nu0, 4. / 3.));
-        t1 = -1. / (3. - p) * (2 * pow(1. / nu_m, 1. / 2. - p / 2.)) * (pow(nu_m, 3. / 2. - p / 2.) - pow(nu_c, 3. / 2. - p / 2.));
-        t2 = pow(nu_c / nu_m, -(p - 1.) / 2.) * 2. / (p - 2.) * (nu_c * pow(nu_M, p / 2.) - nu_M * pow(nu_c, p / 2.)) *
-             pow(nu_c * nu_M / nu_c, -p / 2.);
+        t0 = (3. / 4.) * exp(-(1. / 3.) * lm) * (exp((4. / 3.) * lm) - nu0_43);
+        t1 = -1. / (3. - p) * (2 * exp(-(1. / 2. - p / 2.) * lm)) *
+             (exp((3. / 2. - p / 2.) * lm) - exp((3. / 2. - p / 2.) * lc));
+        t2 = exp((-(p - 1.) / 2.) * (lc - lm)) * 2. / (p - 2.) * (nu_c * exp((p / 2.) * lM) - nu_M * exp((p / 2.) * lc)) *
+             exp((-p / 2.) * lM);
     } else {
-        t0 = (3. / 4.) * pow(1. / nu_c, 1. / 3.) * (pow(nu_c, 4. / 3.) - pow(nu0, 4. / 3.));
+        t0 = (3. / 4.) * exp(-(1. / 3.) * lc) * (exp((4. / 3.) * lc) - nu0_43);
         t1 = -2. / sqrt(1. / nu_c) * (sqrt(nu_c) - sqrt(nu_m));
-        t2 = 1. / sqrt(nu_m / nu_c) * 2. / (p - 2.) * (nu_m * pow(nu_M, p / 2.) - nu_M * pow(nu_m, p / 2.)) *
-             pow(nu_m * nu_M / nu_m, -p / 2.);
+        t2 = 1. / sqrt(nu_m / nu_c) * 2. / (p - 2.) * (nu_m * exp((p / 2.) * lM) - nu_M * exp((p / 2.) * lm)) *
+             exp((-p / 2.) * lM);
     }
     return (t0 + t1 + t2) * p_max;
 }
 
-__device__ double radiation_loss(const pba_shock_params &s, const Shock &sh, MState &st, double dmdR, double dEshdR,
-                                 double R, double Gamma, double Gsh, double adi, double m, double Eint, double rho,
-                                 double theta, double ncells) {
+// computeRadiationLoss (microphysics.h:749-795); cth = cos(theta) from the caller
+__device__ __forceinline__ double radiation_loss(const pba_shock_params &s, const Shock &sh, MState &st, double dmdR,
+                                                 double dEshdR, double R, double Gamma, double Gsh, double adi, double m,
+                                                 double Eint, double rho, double cth, double ncells) {
     if (dmdR == 0 || dEshdR == 0 || m == 0 || rho == 0) return 0.;
     double rho2 = rho2t(Gamma, adi, rho);
     double U_p = (Eint == 0 || m == 0) ? 0. : Eint / (m / rho2);
     double betaSh = beta_from_gamma(Gsh), n_prime = rho2 / kMp;
-    double omc = (1.0 - cos(theta) / ncells);
+    double omc = (1.0 - cth / ncells);
     double thickness = m / (2.0 * kPi * omc * rho2 * Gamma * R * R);
     double p_tot = 0.;
     if (!(betaSh < s.beta_min)) {
@@ -122,7 +129,7 @@ __device__ double radiation_loss(const pba_shock_params &s, const Shock &sh, MSt
         double gc = (6. * kPi * kMe * kC) / (kSigmaT * B * B * st.tcomov);
         if (gc <= 1.) gc = 1.001;
         if (gc > gM) gc = gM;
-        p_tot = wspn99_ptot(s.p, gm, gc, gM, B, dmdR / kMp, 1e6);
+        p_tot = wspn99_ptot(s.p, gm, gc, gM, B, dmdR / kMp, sh.nu0_43);
     }
     p_tot *= st.acc;
     p_tot = fmax(p_tot, 0.);
@@ -144,7 +151,9 @@ __device__ void rhs_fs(const pba_run_params &P, const Shock &shfs, BW &b, const 
     double dthetadr = 0.0;
     if ((theta < b.theta_max) && !b.end_spreading && Gamma * beta < fmax(P.mom_when_start_spread, 0. * b.mom0))
         dthetadr = dthetadr_our(b, Gamma, Gsh, R, adi, theta);
-    double dM2dR = 2. * kPi * rho * ((1. - cos(theta)) + (1. / 3.) * sin(theta) * R * dthetadr) * R * R / b.ncells;
+    double sth, cth;
+    sincos(theta, &sth, &cth);
+    double dM2dR = 2. * kPi * rho * ((1. - cth) + (1. / 3.) * sth * R * dthetadr) * R * R / b.ncells;
     double GE = (adi * Gamma * Gamma - adi + 1.0) / Gamma;
     double dGE = (adi * Gamma * Gamma + adi - 1.0) / (Gamma * Gamma);
     double num = (Gamma - 1.) * (GE + 1.) * dM2dR - GE * (adi - 1.) * Eint2 * (dM2dR / M2 - drhodr / rho);
@@ -156,7 +165,7 @@ __device__ void rhs_fs(const pba_run_params &P, const Shock &shfs, BW &b, const 
     double eps_rad = P.fs.epsilon_e_rad;
     if (P.fs.epsilon_e_rad < 0)
         eps_rad = radiation_loss(P.fs, shfs, b.fs_st, dM2dR * b.M0, dEsh2dR * b.M0 * kC * kC, R, Gamma, Gsh, adi,
-                                 M2 * b.M0, Eint2 * b.M0 * kC * kC, rho * b.M0, theta, b.ncells);
+                                 M2 * b.M0, Eint2 * b.M0 * kC * kC, rho * b.M0, cth, b.ncells);
     double dErad2dR = eps_rad * P.fs.eps_e * dEsh2dR;
     double dEint2dR = dEsh2dR + dEad2dR - dErad2dR;
     double mom = mom_from_gamma(Gamma);
@@ -199,7 +208,9 @@ __device__ void rhs_fsrs(const pba_run_params &P, const Shock &shfs, BW &b, cons
     double dthetadr = 0.0;
     if ((theta < b.theta_max) && !b.end_spreading && Gamma * beta < fmax(P.mom_when_start_spread, 0. * b.mom0))
         dthetadr = dthetadr_our(b, Gamma, Gsh, R, adi, theta);
-    double dM2dR = 2. * kPi * rho * ((1. - cos(theta)) + (1. / 3.) * sin(theta) * R * dthetadr) * R * R / b.ncells;
+    double sth, cth;
+    sincos(theta, &sth, &cth);
+    double dM2dR = 2. * kPi * rho * ((1. - cth) + (1. / 3.) * sth * R * dthetadr) * R * R / b.ncells;
     double ddR4 = 0., dM3dR = 0., dlnrho4dR = 0.;
     if (!b.shutOff && Gamma < 0.9999 * G0) {
         double alpha_of = P.tprompt * b.beta0 * kC;
@@ -244,7 +255,7 @@ __device__ void rhs_fsrs(const pba_run_params &P, const Shock &shfs, BW &b, cons
     double eps_rad = P.fs.epsilon_e_rad;
     if (P.fs.epsilon_e_rad < 0)
         eps_rad = radiation_loss(P.fs, shfs, b.fs_st, dM2dR * b.M0, dEsh2dR * b.M0 * kC * kC, R, Gamma, Gsh, adi,
-                                 M2 * b.M0, Eint2 * b.M0 * kC * kC, rho * b.M0, theta, b.ncells);
+                                 M2 * b.M0, Eint2 * b.M0 * kC * kC, rho * b.M0, cth, b.ncells);
     double dErad2dR = eps_rad * P.fs.eps_e * dEsh2dR;
     // the reverse-shock loss is evaluated with rho4 = 0 and returns 0 (blastwave.h:1991-1997)
     double eps_rad_rs = P.rs.epsilon_e_rad < 0 ? 0. : P.rs.epsilon_e_rad;
@@ -345,7 +356,7 @@ __device__ double ctheta_of(double theta, int ilayer, int nlayers) {
     return 0.5 * (xx1 + xx2);
 }
 
-__device__ void post_step(const pba_run_params &P, const Shock &shfs, const Shock &shrs, BW &b, const double *Yc,
+__device__ __forceinline__ void post_step(const pba_run_params &P, const Shock &shfs, const Shock &shrs, BW &b, const double *Yc,
                           int it, double tb, double *T, int nt) {
     if (b.end_evolution) return;
     auto W = [&](int v, double x) { T[(size_t)v * nt + it] = x; };
@@ -462,6 +473,22 @@ constexpr double er1 = 0.1312004499419488073250102996E-01, er6 = -0.122515644637
 constexpr double bhh1 = 0.244094488188976377952755905512E+00, bhh9 = 0.733846688281611857341361741547E+00,
                  bhhC = 0.220588235294117647058823529412E-01;
 
+// Stage vectors live in LDS, KS slots of [NA][bw_per_run] doubles.  Slot of each k_i:
+//   k1 -> 0; k2, k3, k5 -> 1 (each is dead once the next of them is formed); k4, k12 -> 2;
+//   k6 .. k11 -> 3 .. 8.  The RK4 integrator keeps its running increment in slot 3.
+constexpr int KS = 9;
+// Stage s = 2 .. 12 of one DOP853 step: x_s = Y + h * sum_j a_sj k_j over the non-zero a_sj,
+// added in the tableau's order; the stage's derivative goes to slot st_out[s].
+__constant__ int st_n[13] = {0, 0, 1, 2, 2, 3, 3, 4, 5, 6, 7, 8, 9};
+__constant__ int st_out[13] = {0, 0, 1, 1, 2, 1, 3, 4, 5, 6, 7, 8, 2};
+__constant__ int st_slot[13][9] = {
+    {0}, {0}, {0}, {0, 1}, {0, 1}, {0, 1, 2}, {0, 2, 1}, {0, 2, 1, 3}, {0, 2, 1, 3, 4}, {0, 2, 1, 3, 4, 5},
+    {0, 2, 1, 3, 4, 5, 6}, {0, 2, 1, 3, 4, 5, 6, 7}, {0, 2, 1, 3, 4, 5, 6, 7, 8}};
+__constant__ double st_a[13][9] = {
+    {0}, {0}, {a21}, {a31, a32}, {a41, a43}, {a51, a53, a54}, {a61, a64, a65}, {a71, a74, a75, a76},
+    {a81, a84, a85, a86, a87}, {a91, a94, a95, a96, a97, a98}, {aA1, aA4, aA5, aA6, aA7, aA8, aA9},
+    {aB1, aB4, aB5, aB6, aB7, aB8, aB9, aBA}, {aC1, aC4, aC5, aC6, aC7, aC8, aC9, aCA, aCB}};
+
 // Deterministic block-wide sum of one value per lane, added in lane order (all lanes get the same result).
 __device__ __forceinline__ double ordered_sum(double v, double *lds, int n) {
     int lane = threadIdx.x;
@@ -473,14 +500,21 @@ __device__ __forceinline__ double ordered_sum(double v, double *lds, int n) {
     return s;
 }
 
+// Program of one output interval: every pass of the interval loop evaluates the right-hand
+// side exactly once (one call site, so one copy of the RHS in the kernel's code) and then
+// moves the integrator to the state that needs the next evaluation.
+enum Phase { PH_K1I, PH_H2, PH_STAGE, PH_FSAL, PH_RK1, PH_RK2, PH_RK3, PH_RK4 };
+
 __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restrict__ params,
                                                  const pba_layer_init *__restrict__ init, int bw_per_run, int nt,
                                                  const double *__restrict__ tgrid, double *__restrict__ table,
                                                  pba_bw_result *__restrict__ res) {
-    __shared__ double red[64];
+    extern __shared__ double dyn_lds[];   // [KS][NA][bw_per_run] stage vectors, then 64 doubles for sums
+    double *red = dyn_lds + (size_t)KS * NA * bw_per_run;
     const int run = blockIdx.x;
     const int lane = threadIdx.x;
     const bool active = lane < bw_per_run;
+    auto K = [&](int slot, int c) -> double & { return dyn_lds[(slot * NA + c) * bw_per_run + lane]; };
     const pba_run_params P = params[run];
     Shock shfs, shrs;
     {
@@ -488,11 +522,14 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
         shfs.e0 = pow(step, 0.); shfs.etop = pow(step, (double)(P.fs.ngam - 1));
         double stepr = exp(log(P.rs.gam2) / (double)P.rs.ngam);
         shrs.e0 = pow(stepr, 0.); shrs.etop = pow(stepr, (double)(P.rs.ngam - 1));
+        shfs.nu0_43 = shrs.nu0_43 = pow(1e6, 4. / 3.);   // computeRadiationLoss's nu0 = 1e6 Hz
     }
     BW b{};
-    double Y[NA], x[NA], k1[NA], k2[NA], k4[NA], k6[NA], k7[NA], k8[NA], k9[NA], kA[NA], kB[NA];
-    for (int k = 0; k < NA; k++) { Y[k] = 0.; x[k] = 0.; k1[k] = 0.; k2[k] = 0.; k4[k] = 0.; k6[k] = 0.; k7[k] = 0.;
-                                   k8[k] = 0.; k9[k] = 0.; kA[k] = 0.; kB[k] = 0.; }
+    double Y[NA], x[NA];
+    for (int k = 0; k < NA; k++) { Y[k] = 0.; x[k] = 0.; }
+    if (active)
+        for (int sl = 0; sl < KS; sl++)
+            for (int k = 0; k < NA; k++) K(sl, k) = 0.;
     const size_t bw_global = (size_t)run * bw_per_run + lane;
     {   // clear this run's tables, all lanes striding over contiguous memory
         double *Tr = table + (size_t)run * bw_per_run * PBA_NV * nt;
@@ -507,6 +544,7 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
         b.Gamma0 = gamma_from_mom(id.mom0);
         b.R0 = beta_from_mom(id.mom0) * kC * tgrid[0];
         b.theta_b0 = id.theta_b0; b.ncells = id.ncells; b.theta_core = id.theta_core; b.theta_max = P.theta_max;
+        b.tan_ratio = tan(0.5 * b.theta_b0) / tan(0.5 * b.theta_core);
         b.ilayer = id.ilayer; b.nlayers = id.nlayers;
         b.fsrs = P.fsrs; b.do_rs = P.fsrs && P.do_rs; b.do_rs_rad = P.fsrs && P.do_rs_radiation;
         b.rho_def = P.n_ism * kMppme;
@@ -533,7 +571,9 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
             b.Rw0 = .5 * Y[iR];
             b.Wtt0 = Y[itt];
         }
-        post_step(P, shfs, shrs, b, Y, 0, tgrid[0], T, nt);
+        double yc[NA];
+        for (int k = 0; k < NA; k++) yc[k] = Y[k];
+        post_step(P, shfs, shrs, b, yc, 0, tgrid[0], T, nt);
     }
     const double rtol = P.rtol, atol = P.rtol;
     const double nsz = (double)(NEQ_REF * bw_per_run);
@@ -542,136 +582,165 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
     for (int it = 1; it < nt; it++) {
         const double timeN = time + (tgrid[it] - tgrid[it - 1]);
         const double hmax = fabs(timeN - time);
-        if (P.integrator == 1) {
-            // RK4: one classical step over the output interval (rk4_integrator, ode_solvers.h:319-366),
-            // every blastwave on its own (the step has no error control that couples them)
-            const double h = tgrid[it] - tgrid[it - 1];
-            double *kn = k1, *dl = k6;
-            rhs(P, shfs, b, active, Y, kn);
-            for (int k = 0; k < NA; k++) { dl[k] = (1.0 / 6.0) * h * kn[k]; x[k] = Y[k] + 0.5 * h * kn[k]; }
-            rhs(P, shfs, b, active, x, kn);
-            for (int k = 0; k < NA; k++) { dl[k] += (2.0 / 6.0) * h * kn[k]; x[k] = Y[k] + 0.5 * h * kn[k]; }
-            rhs(P, shfs, b, active, x, kn);
-            for (int k = 0; k < NA; k++) { dl[k] += (2.0 / 6.0) * h * kn[k]; x[k] = Y[k] + h * kn[k]; }
-            rhs(P, shfs, b, active, x, kn);
-            for (int k = 0; k < NA; k++) {
-                const double a = dl[k] + (1.0 / 6.0) * h * kn[k];
-                Y[k] += a;
-            }
-            time += h;
-            nrhs += 4;
-            nsteps++;
-        } else {
-        rhs(P, shfs, b, active, Y, k1); nrhs++;
-        // initial step (ode_solvers.h:1596-1628)
-        double h;
-        {
-            double dnf = 0., dny = 0.;
-            if (active) {
-                for (int k = 0; k < NA; k++) {
-                    double sk = atol + rtol * fabs(Y[k]);
-                    dnf += (k1[k] / sk) * (k1[k] / sk);
-                    dny += (Y[k] / sk) * (Y[k] / sk);
-                }
-                double skw = atol + rtol * fabs(b.Rw0);
-                dny += (b.Rw0 / skw) * (b.Rw0 / skw);
-                double skt = atol + rtol * fabs(b.Wtt0);
-                dny += (b.Wtt0 / skt) * (b.Wtt0 / skt);
-            }
-            dnf = ordered_sum(dnf, red, bw_per_run);
-            dny = ordered_sum(dny, red, bw_per_run);
-            double h1 = (dnf <= 1e-10 || dny <= 1e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
-            h1 = fmin(h1, hmax);
-            for (int k = 0; k < NA; k++) x[k] = Y[k] + h1 * k1[k];
-            rhs(P, shfs, b, active, x, k2); nrhs++;
-            double n2 = 0.;
-            if (active)
-                for (int k = 0; k < NA; k++) {
-                    double sk = atol + rtol * x86_int_abs(Y[k]);
-                    double z = (k2[k] - k1[k]) / sk;
-                    n2 += z * z;
-                }
-            n2 = ordered_sum(n2, red, bw_per_run);
-            n2 = sqrt(n2) / h1;
-            double der12 = fmax(n2, sqrt(dnf));
-            double h2 = (der12 <= 1e-15) ? fmax(1e-6, fabs(h1) * 1e-3) : pow(0.01 / der12, 1.0 / 8);
-            h = fmin(fmin(100 * h1, h2), hmax);
-        }
+        const double hrk = tgrid[it] - tgrid[it - 1];
+        int ph = P.integrator == 1 ? PH_RK1 : PH_K1I;
+        int st = 0, outslot = 0, guard = 0;
+        double h = 0., h1 = 0., dnf = 0.;
         bool reject = false, last = false;
-        for (int guard = 0; guard < kMaxStepsPerInterval; guard++) {
-            if ((time + 1.01 * h - timeN) > 0.0) { h = timeN - time; last = true; }
-            // stages (ode_solvers.h:1434-1479); k3 and k5 share k2's storage, kC shares k4's
-            double *k3 = k2, *k5 = k2, *kC = k4;
-            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a21 * k1[k]);
-            rhs(P, shfs, b, active, x, k2);
-            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a31 * k1[k] + a32 * k2[k]);
-            rhs(P, shfs, b, active, x, k3);
-            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a41 * k1[k] + a43 * k3[k]);
-            rhs(P, shfs, b, active, x, k4);
-            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a51 * k1[k] + a53 * k3[k] + a54 * k4[k]);
-            rhs(P, shfs, b, active, x, k5);
-            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a61 * k1[k] + a64 * k4[k] + a65 * k5[k]);
-            rhs(P, shfs, b, active, x, k6);
-            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * (a71 * k1[k] + a74 * k4[k] + a75 * k5[k] + a76 * k6[k]);
-            rhs(P, shfs, b, active, x, k7);
-            for (int k = 0; k < NA; k++)
-                x[k] = Y[k] + h * (a81 * k1[k] + a84 * k4[k] + a85 * k5[k] + a86 * k6[k] + a87 * k7[k]);
-            rhs(P, shfs, b, active, x, k8);
-            for (int k = 0; k < NA; k++)
-                x[k] = Y[k] + h * (a91 * k1[k] + a94 * k4[k] + a95 * k5[k] + a96 * k6[k] + a97 * k7[k] + a98 * k8[k]);
-            rhs(P, shfs, b, active, x, k9);
-            for (int k = 0; k < NA; k++)
-                x[k] = Y[k] + h * (aA1 * k1[k] + aA4 * k4[k] + aA5 * k5[k] + aA6 * k6[k] + aA7 * k7[k] + aA8 * k8[k] +
-                                   aA9 * k9[k]);
-            rhs(P, shfs, b, active, x, kA);
-            for (int k = 0; k < NA; k++)
-                x[k] = Y[k] + h * (aB1 * k1[k] + aB4 * k4[k] + aB5 * k5[k] + aB6 * k6[k] + aB7 * k7[k] + aB8 * k8[k] +
-                                   aB9 * k9[k] + aBA * kA[k]);
-            rhs(P, shfs, b, active, x, kB);
-            for (int k = 0; k < NA; k++)
-                x[k] = Y[k] + h * (aC1 * k1[k] + aC4 * k4[k] + aC5 * k5[k] + aC6 * k6[k] + aC7 * k7[k] + aC8 * k8[k] +
-                                   aC9 * k9[k] + aCA * kA[k] + aCB * kB[k]);
-            rhs(P, shfs, b, active, x, kC);
-            nrhs += 11;
-            double e1s = 0., e2s = 0.;
-            for (int k = 0; k < NA; k++) {
-                double slope = b1 * k1[k] + b6 * k6[k] + b7 * k7[k] + b8 * k8[k] + b9 * k9[k] + bA * kA[k] + bB * kB[k] +
-                               bC * kC[k];
-                double yn = Y[k] + h * slope;
-                double sk = atol + rtol * fmax(fabs(Y[k]), fabs(yn));
-                double e1 = er1 * k1[k] + er6 * k6[k] + er7 * k7[k] + er8 * k8[k] + er9 * k9[k] + erA * kA[k] +
-                            erB * kB[k] + erC * kC[k];
-                double e2 = slope - bhh1 * k1[k] - bhh9 * k9[k] - bhhC * kC[k];
-                if (active) { e1s += (e1 / sk) * (e1 / sk); e2s += (e2 / sk) * (e2 / sk); }
-                x[k] = yn;
+        for (int k = 0; k < NA; k++) x[k] = Y[k];
+        // x_s for stage s, from the stage vectors in LDS
+        auto combo = [&](int s) {
+            if (!active) return;
+            const int n = st_n[s];
+            for (int k = 0; k < NA; k++) x[k] = st_a[s][0] * K(st_slot[s][0], k);
+            for (int j = 1; j < n; j++) {
+                const double a = st_a[s][j];
+                const int sl = st_slot[s][j];
+                for (int k = 0; k < NA; k++) x[k] += a * K(sl, k);
             }
-            e1s = ordered_sum(e1s, red, bw_per_run);
-            e2s = ordered_sum(e2s, red, bw_per_run);
-            double deno = e1s + 0.01 * e2s;
-            if (deno <= 0.0) deno = 1.0;
-            double err = fabs(h) * e1s / sqrt(nsz * deno);
-            double fac11 = pow(err, 1.0 / 8.0);
-            double fac = fac11 / (1.0 * 0.9);
-            fac = fac < 1.0 / 6.0 ? 1.0 / 6.0 : (fac > 1.0 / 0.333 ? 1.0 / 0.333 : fac);
-            double hnew = h / fac;
-            if (err > 1.0) {
-                h /= fmin(1.0 / 0.333, fac11 / 0.9);
-                reject = true;
-                last = false;
-            } else {
-                rhs(P, shfs, b, active, x, k2); nrhs++;   // kD shares k2's storage
-                for (int k = 0; k < NA; k++) { k1[k] = k2[k]; Y[k] = x[k]; }
+            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * x[k];
+        };
+        // start an attempt of one DOP853 step of size h (ode_solvers.h:1434-1479)
+        auto begin_attempt = [&]() {
+            if ((time + 1.01 * h - timeN) > 0.0) { h = timeN - time; last = true; }
+            st = 2;
+            combo(2);
+            outslot = st_out[2];
+            ph = PH_STAGE;
+        };
+        for (;;) {
+            double o[NA];
+            for (int k = 0; k < NA; k++) o[k] = 0.;
+            rhs(P, shfs, b, active, x, o);   // the one right-hand-side evaluation of the loop
+            nrhs++;
+            if (ph == PH_K1I) {
+                // k1 = f(Y), then the initial step (ode_solvers.h:1596-1628)
+                double dny = 0.;
+                dnf = 0.;
+                if (active) {
+                    for (int k = 0; k < NA; k++) {
+                        K(0, k) = o[k];
+                        double sk = atol + rtol * fabs(Y[k]);
+                        dnf += (o[k] / sk) * (o[k] / sk);
+                        dny += (Y[k] / sk) * (Y[k] / sk);
+                    }
+                    double skw = atol + rtol * fabs(b.Rw0);
+                    dny += (b.Rw0 / skw) * (b.Rw0 / skw);
+                    double skt = atol + rtol * fabs(b.Wtt0);
+                    dny += (b.Wtt0 / skt) * (b.Wtt0 / skt);
+                }
+                dnf = ordered_sum(dnf, red, bw_per_run);
+                dny = ordered_sum(dny, red, bw_per_run);
+                h1 = (dnf <= 1e-10 || dny <= 1e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
+                h1 = fmin(h1, hmax);
+                for (int k = 0; k < NA; k++) x[k] = Y[k] + h1 * o[k];
+                ph = PH_H2;
+                continue;
+            }
+            if (ph == PH_H2) {
+                double n2 = 0.;
+                if (active)
+                    for (int k = 0; k < NA; k++) {
+                        double sk = atol + rtol * x86_int_abs(Y[k]);
+                        double z = (o[k] - K(0, k)) / sk;
+                        n2 += z * z;
+                    }
+                n2 = ordered_sum(n2, red, bw_per_run);
+                n2 = sqrt(n2) / h1;
+                double der12 = fmax(n2, sqrt(dnf));
+                double h2 = (der12 <= 1e-15) ? fmax(1e-6, fabs(h1) * 1e-3) : pow(0.01 / der12, 1.0 / 8);
+                h = fmin(fmin(100 * h1, h2), hmax);
+                begin_attempt();
+                continue;
+            }
+            if (ph == PH_STAGE) {
+                if (active)
+                    for (int k = 0; k < NA; k++) K(outslot, k) = o[k];
+                if (st < 12) {
+                    st++;
+                    combo(st);
+                    outslot = st_out[st];
+                    continue;
+                }
+                // all 12 stages done: 8th-order solution and the error estimate
+                double e1s = 0., e2s = 0.;
+                if (active)
+                    for (int k = 0; k < NA; k++) {
+                        const double k1 = K(0, k), k6 = K(3, k), k7 = K(4, k), k8 = K(5, k), k9 = K(6, k),
+                                     kA = K(7, k), kB = K(8, k), kC = K(2, k);
+                        double slope = b1 * k1 + b6 * k6 + b7 * k7 + b8 * k8 + b9 * k9 + bA * kA + bB * kB + bC * kC;
+                        double yn = Y[k] + h * slope;
+                        double sk = atol + rtol * fmax(fabs(Y[k]), fabs(yn));
+                        double e1 = er1 * k1 + er6 * k6 + er7 * k7 + er8 * k8 + er9 * k9 + erA * kA + erB * kB +
+                                    erC * kC;
+                        double e2 = slope - bhh1 * k1 - bhh9 * k9 - bhhC * kC;
+                        e1s += (e1 / sk) * (e1 / sk);
+                        e2s += (e2 / sk) * (e2 / sk);
+                        x[k] = yn;
+                    }
+                e1s = ordered_sum(e1s, red, bw_per_run);
+                e2s = ordered_sum(e2s, red, bw_per_run);
+                double deno = e1s + 0.01 * e2s;
+                if (deno <= 0.0) deno = 1.0;
+                double err = fabs(h) * e1s / sqrt(nsz * deno);
+                double fac11 = pow(err, 1.0 / 8.0);
+                if (err > 1.0) {
+                    h /= fmin(1.0 / 0.333, fac11 / 0.9);
+                    reject = true;
+                    last = false;
+                    if (guard == kMaxStepsPerInterval - 1) { b.status |= PBAGPU_S_NMAX; break; }
+                    guard++;
+                    begin_attempt();
+                    continue;
+                }
+                double fac = fac11 / (1.0 * 0.9);
+                fac = fac < 1.0 / 6.0 ? 1.0 / 6.0 : (fac > 1.0 / 0.333 ? 1.0 / 0.333 : fac);
+                h1 = h / fac;   // the next step size, before the limits applied after acceptance
+                ph = PH_FSAL;   // f(y_new) becomes k1 of the next step
+                continue;
+            }
+            if (ph == PH_FSAL) {
+                if (active)
+                    for (int k = 0; k < NA; k++) { K(0, k) = o[k]; Y[k] = x[k]; }
                 time += h;
                 nsteps++;
                 if (last) break;
+                double hnew = h1;
                 if (fabs(hnew) > hmax) hnew = hmax;
                 if (reject) hnew = fmin(fabs(hnew), fabs(h));
                 reject = false;
+                if (guard == kMaxStepsPerInterval - 1) { b.status |= PBAGPU_S_NMAX; break; }
+                guard++;
                 h = hnew;
+                begin_attempt();
+                continue;
             }
-            if (guard == kMaxStepsPerInterval - 1) b.status |= PBAGPU_S_NMAX;
+            // RK4: one classical step over the output interval (rk4_integrator, ode_solvers.h:319-366),
+            // every blastwave on its own (the step has no error control that couples them);
+            // the running increment is kept in slot 3
+            if (ph == PH_RK1) {
+                if (active)
+                    for (int k = 0; k < NA; k++) { K(3, k) = (1.0 / 6.0) * hrk * o[k]; x[k] = Y[k] + 0.5 * hrk * o[k]; }
+                ph = PH_RK2;
+                continue;
+            }
+            if (ph == PH_RK2 || ph == PH_RK3) {
+                const double c = ph == PH_RK2 ? 0.5 : 1.0;
+                if (active)
+                    for (int k = 0; k < NA; k++) { K(3, k) += (2.0 / 6.0) * hrk * o[k]; x[k] = Y[k] + c * hrk * o[k]; }
+                ph = ph == PH_RK2 ? PH_RK3 : PH_RK4;
+                continue;
+            }
+            // PH_RK4
+            if (active)
+                for (int k = 0; k < NA; k++) {
+                    const double a = K(3, k) + (1.0 / 6.0) * hrk * o[k];
+                    Y[k] += a;
+                }
+            time += hrk;
+            nsteps++;
+            break;
         }
-        }   // DOP853E
         if (active && !b.end_evolution) {
             double g = Y[iGamma], be = beta_from_gamma(g), E2 = Y[iEint2];
             if ((E2 <= 0.) || !isfinite(E2) || (g < 1.) || (be < 0) || (fabs(be) < 1.e-6) || !isfinite(be))
@@ -686,7 +755,11 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
             if (g < 1. || Y[itheta] < b.theta_b0 * 0.99 || Y[itt] <= 0.) b.status |= PBAGPU_S_NONFINITE;
         }
         b.prev_idx_x = it;
-        if (active) post_step(P, shfs, shrs, b, Y, it, tgrid[it], T, nt);
+        if (active) {
+            double yc[NA];   // a copy, so that Y itself stays in registers
+            for (int k = 0; k < NA; k++) yc[k] = Y[k];
+            post_step(P, shfs, shrs, b, yc, it, tgrid[it], T, nt);
+        }
     }
     if (active) {
         pba_bw_result r;
@@ -716,9 +789,9 @@ extern "C" int pbagpu_dyn_evolve(const pba_run_params *params, int n_runs, int b
             return PBAGPU_E_ARG;
         }
     }
-    auto *dP = (pba_run_params *)pbagpu::workspace(pbagpu::WS_DYN_PARAMS, sizeof(pba_run_params) * n_runs);
-    auto *dI = (pba_layer_init *)pbagpu::workspace(pbagpu::WS_DYN_INIT, sizeof(pba_layer_init) * nbw);
-    auto *dR = (pba_bw_result *)pbagpu::workspace(pbagpu::WS_DYN_RES, sizeof(pba_bw_result) * nbw);
+    auto *dP = (pba_run_params *)pbagpu::workspace(pbagpu::WS_DYN_PARAMS, sizeof(pba_run_params) * n_runs, s);
+    auto *dI = (pba_layer_init *)pbagpu::workspace(pbagpu::WS_DYN_INIT, sizeof(pba_layer_init) * nbw, s);
+    auto *dR = (pba_bw_result *)pbagpu::workspace(pbagpu::WS_DYN_RES, sizeof(pba_bw_result) * nbw, s);
     if (!dP || !dI || !dR) return PBAGPU_E_HIP;
     int rc = PBAGPU_OK;
     PBA_TRY(rc, hipMemcpyAsync(dP, params, sizeof(pba_run_params) * n_runs, hipMemcpyHostToDevice, s));
@@ -726,7 +799,10 @@ extern "C" int pbagpu_dyn_evolve(const pba_run_params *params, int n_runs, int b
     pbagpu::KernelTimer timer(PBAGPU_STAGE_DYN);
     if (rc == PBAGPU_OK) {
         timer.start(s);
-        hipLaunchKernelGGL(pbagpu::dyn::dyn_kernel, dim3(n_runs), dim3(64), 0, s, dP, dI, bw_per_run, nt, d_tgrid,
+        const size_t lds = sizeof(double) * ((size_t)pbagpu::dyn::KS * pbagpu::dyn::NA * bw_per_run + 64);
+        (void)hipFuncSetAttribute((const void *)pbagpu::dyn::dyn_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        hipLaunchKernelGGL(pbagpu::dyn::dyn_kernel, dim3(n_runs), dim3(64), lds, s, dP, dI, bw_per_run, nt, d_tgrid,
                            d_table, dR);
         PBA_TRY(rc, hipGetLastError());
         timer.stop(s);

@@ -687,8 +687,8 @@ extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, 
             return PBAGPU_E_ARG;
         }
     }
-    auto *dnu = (double *)pbagpu::workspace(pbagpu::WS_EATS_NU, sizeof(double) * hnu.size());
-    auto *dL = (LayerArgs *)pbagpu::workspace(pbagpu::WS_EATS_LAYERS, sizeof(LayerArgs) * n_layers);
+    auto *dnu = (double *)pbagpu::workspace(pbagpu::WS_EATS_NU, sizeof(double) * hnu.size(), s);
+    auto *dL = (LayerArgs *)pbagpu::workspace(pbagpu::WS_EATS_LAYERS, sizeof(LayerArgs) * n_layers, s);
     if (!dnu || !dL) return PBAGPU_E_HIP;
     std::vector<LayerArgs> hl(n_layers);
     for (int l = 0; l < n_layers; l++) {
@@ -725,15 +725,15 @@ extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, 
         int per_cu = 1;
         PBA_TRY(rc, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, EATS_WG, lds));
         const int n_wg = (int)std::min<long>(n_items, (long)std::max(n_cu, 1) * std::max(per_cu, 1));
-        char *ws = (char *)pbagpu::workspace(pbagpu::WS_EATS_STORE, (size_t)n_wg * EATS_WG * LANE_WS);
-        int *counter = (int *)pbagpu::workspace(pbagpu::WS_EATS_COUNTER, sizeof(int) * n_layers);
+        char *ws = (char *)pbagpu::workspace(pbagpu::WS_EATS_STORE, (size_t)n_wg * EATS_WG * LANE_WS, s);
+        int *counter = (int *)pbagpu::workspace(pbagpu::WS_EATS_COUNTER, sizeof(int) * n_layers, s);
         if (!ws || !counter) rc = PBAGPU_E_HIP;
         if (rc == PBAGPU_OK) PBA_TRY(rc, hipMemsetAsync(counter, 0, sizeof(int) * n_layers, s));
         if (rc == PBAGPU_OK) {
             (void)hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             unsigned long long *d_samples = nullptr;
             if (pbagpu::timing_enabled()) {
-                d_samples = (unsigned long long *)pbagpu::workspace(pbagpu::WS_EATS_SAMPLES, sizeof(unsigned long long));
+                d_samples = (unsigned long long *)pbagpu::workspace(pbagpu::WS_EATS_SAMPLES, sizeof(unsigned long long), s);
                 if (d_samples) PBA_TRY(rc, hipMemsetAsync(d_samples, 0, sizeof(unsigned long long), s));
             }
             timer.start(s);

@@ -286,8 +286,8 @@ static int launch(const pba_run_params *params, int n_runs, const pba_eats_pw_la
         const double ratio = std::pow(10.0, (std::log10(sp.freq2) - std::log10(sp.freq1)) / sp.nfreq);
         for (int i = 0; i < nfreq; i++) { hnu[(size_t)r * nfreq + i] = cur; cur *= ratio; }
     }
-    auto *dnu = (double *)workspace(WS_PW_NU, sizeof(double) * hnu.size());
-    auto *dL = (PwArgs *)workspace(WS_PW_LAYERS, sizeof(PwArgs) * n_layers);
+    auto *dnu = (double *)workspace(WS_PW_NU, sizeof(double) * hnu.size(), s);
+    auto *dL = (PwArgs *)workspace(WS_PW_LAYERS, sizeof(PwArgs) * n_layers, s);
     if (!dnu || !dL) return PBAGPU_E_HIP;
     std::vector<PwArgs> hl(n_layers);
     int off = 0;
@@ -312,7 +312,7 @@ static int launch(const pba_run_params *params, int n_runs, const pba_eats_pw_la
     KernelTimer timer(PBAGPU_STAGE_EATS_PW);
     unsigned long long *d_cells = nullptr;
     if (rc == PBAGPU_OK && timing_enabled()) {
-        d_cells = (unsigned long long *)workspace(WS_PW_SAMPLES, sizeof(unsigned long long));
+        d_cells = (unsigned long long *)workspace(WS_PW_SAMPLES, sizeof(unsigned long long), s);
         if (d_cells) PBA_TRY(rc, hipMemsetAsync(d_cells, 0, sizeof(unsigned long long), s));
     }
     if (rc == PBAGPU_OK) {

@@ -728,10 +728,10 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
     size_t n_ssc = 0;
     std::vector<size_t> ssc_slot(keys.size(), 0);
     for (size_t kk = 0; kk < keys.size(); kk++) if (need_ssc[kk]) ssc_slot[kk] = n_ssc++;
-    auto *dgrids = (double *)pbagpu::workspace(pbagpu::WS_ELE_GRIDS, sizeof(double) * hgrids.size());
-    auto *dK = n_ssc ? (double *)pbagpu::workspace(pbagpu::WS_ELE_KTAB, sizeof(double) * (ktab + (size_t)ng * nf) * n_ssc)
+    auto *dgrids = (double *)pbagpu::workspace(pbagpu::WS_ELE_GRIDS, sizeof(double) * hgrids.size(), s);
+    auto *dK = n_ssc ? (double *)pbagpu::workspace(pbagpu::WS_ELE_KTAB, sizeof(double) * (ktab + (size_t)ng * nf) * n_ssc, s)
                      : nullptr;
-    auto *dpops = (PopDesc *)pbagpu::workspace(pbagpu::WS_ELE_POPS, sizeof(PopDesc) * n_pop);
+    auto *dpops = (PopDesc *)pbagpu::workspace(pbagpu::WS_ELE_POPS, sizeof(PopDesc) * n_pop, s);
     if (!dgrids || (n_ssc && !dK) || !dpops) return PBAGPU_E_HIP;
     int rc = PBAGPU_OK;
     PBA_TRY(rc, hipMemcpyAsync(dgrids, hgrids.data(), sizeof(double) * hgrids.size(), hipMemcpyHostToDevice, s));
@@ -755,7 +755,7 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
     const size_t kbf = (size_t)nf * ng * kp;
     unsigned short *dKb = nullptr;
     if (precision == 1 && n_ssc) {
-        dKb = (unsigned short *)pbagpu::workspace(pbagpu::WS_ELE_KBF, sizeof(unsigned short) * 2 * kbf * n_ssc);
+        dKb = (unsigned short *)pbagpu::workspace(pbagpu::WS_ELE_KBF, sizeof(unsigned short) * 2 * kbf * n_ssc, s);
         if (!dKb) return PBAGPU_E_HIP;
         for (size_t t = 0; t < n_ssc && rc == PBAGPU_OK; t++) {
             const double *K = dK + (ktab + (size_t)ng * nf) * t;
@@ -792,7 +792,7 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
     // per-population state between the per-step launches
     const size_t np = (size_t)n_pop;
     const size_t n_dbl = np * ng * 3 + np * nf * (3 + SSC_JB) + np * 4;
-    double *wsd = (double *)pbagpu::workspace(pbagpu::WS_ELE_WORK, sizeof(double) * n_dbl + sizeof(int) * np * 3);
+    double *wsd = (double *)pbagpu::workspace(pbagpu::WS_ELE_WORK, sizeof(double) * n_dbl + sizeof(int) * np * 3, s);
     // populations of each Compton table, for the batched contraction
     std::vector<int> hlist;
     std::vector<size_t> list_off(n_ssc + 1, 0);
@@ -803,7 +803,7 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
     }
     list_off[n_ssc] = hlist.size();
     int *dlist = hlist.empty() ? nullptr
-                               : (int *)pbagpu::workspace(pbagpu::WS_ELE_LIST, sizeof(int) * hlist.size());
+                               : (int *)pbagpu::workspace(pbagpu::WS_ELE_LIST, sizeof(int) * hlist.size(), s);
     if (!wsd || (!hlist.empty() && !dlist)) return PBAGPU_E_HIP;
     // operands of the mixed-precision contraction (per population of the SSC lists)
     float *d_an = nullptr, *d_wn = nullptr;
@@ -811,7 +811,7 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
     if (precision == 1 && !hlist.empty()) {
         const size_t nl = hlist.size();
         char *mw = (char *)pbagpu::workspace(pbagpu::WS_ELE_MIXED,
-                                             nl * (sizeof(float) * (ng + kp) + sizeof(double) * (SSC_JB + 1)));
+                                             nl * (sizeof(float) * (ng + kp) + sizeof(double) * (SSC_JB + 1)), s);
         if (!mw) return PBAGPU_E_HIP;
         d_amax = (double *)mw;
         d_wmax = d_amax + nl * SSC_JB;

@@ -1,5 +1,5 @@
 // Host-side runtime shared by the stage entry points: error reporting and the library's
-// device workspace (growable buffers per device and slot, so that repeated calls do not
+// device workspace (growable buffers per device, stream and slot, so that repeated calls do not
 // allocate).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -21,10 +21,14 @@ enum WsSlot {
     WS_N
 };
 
-// Device buffer of at least `bytes` for `slot` on the current device.  Growing a slot
-// frees its old buffer (hipFree waits for the device), so a call must not hold a
-// pointer from an earlier call.  Returns nullptr on failure (error recorded).
-void *workspace(int slot, size_t bytes);
+// Device buffer of at least `bytes` for `slot`, owned by (current device, `stream`).  Every
+// stream has its own set of buffers, so calls on different streams (one host thread each) never
+// share or free each other's memory.  Growing a buffer first waits for `stream`, then frees the
+// old buffer, so a call must not hold a pointer from an earlier call on the same stream.
+// Returns nullptr on failure (error recorded).
+void *workspace(int slot, size_t bytes, hipStream_t stream);
+// Free every workspace buffer of (current device, stream), after waiting for the stream.
+int release_stream(hipStream_t stream);
 
 }  // namespace pbagpu
 

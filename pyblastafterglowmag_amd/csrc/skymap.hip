@@ -258,9 +258,9 @@ extern "C" int pbagpu_skymap_hist(int n_images, const int *img_seg, const long *
     }
     item_off[n_segs] = (int)(items.size() / 2);
     const int n_items = item_off[n_segs];
-    auto *dseg = (int *)workspace(WS_SKY_SEG, sizeof(int) * ((size_t)n_images + 1 + n_segs + 1));
-    auto *doff = (long *)workspace(WS_SKY_OFF, sizeof(long) * ((size_t)n_segs + 1 + items.size()));
-    auto *dpart = (double *)workspace(WS_SKY_PART, sizeof(double) * (size_t)n_items * P_N);
+    auto *dseg = (int *)workspace(WS_SKY_SEG, sizeof(int) * ((size_t)n_images + 1 + n_segs + 1), s);
+    auto *doff = (long *)workspace(WS_SKY_OFF, sizeof(long) * ((size_t)n_segs + 1 + items.size()), s);
+    auto *dpart = (double *)workspace(WS_SKY_PART, sizeof(double) * (size_t)n_items * P_N, s);
     if (!dseg || !doff || !dpart) return PBAGPU_E_HIP;
     int *ditem_off = dseg + n_images + 1;
     long *ditems = doff + n_segs + 1;

@@ -16,7 +16,8 @@ _lib = None
 
 EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu_last_error", "pbagpu_timing_enable",
            "pbagpu_timing_read", "pbagpu_timing_units", "pbagpu_dyn_evolve", "pbagpu_ele_evolve", "pbagpu_eats_lightcurve",
-           "pbagpu_eats_lightcurve_pw", "pbagpu_eats_skymap_pw", "pbagpu_skymap_hist", "pbagpu_bessel_k_thirds")
+           "pbagpu_eats_lightcurve_pw", "pbagpu_eats_skymap_pw", "pbagpu_skymap_hist", "pbagpu_bessel_k_thirds",
+           "pbagpu_release_stream")
 STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW, STAGE_SKY_HIST = 0, 1, 2, 3, 4, 5
 N_STAGES = 6
 # sky-map binning (enum pba_sky_moment / pba_sky_status)
@@ -54,10 +55,16 @@ def lib():
         L.pbagpu_eats_skymap_pw.argtypes = [C.POINTER(abi.RunParams), C.POINTER(abi.EatsPwLayer), C.c_int, vp, C.c_int,
                                             vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp]
         L.pbagpu_bessel_k_thirds.argtypes = [C.c_int, vp, vp, vp]
+        L.pbagpu_release_stream.argtypes = [vp]
         L.pbagpu_skymap_hist.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_long), vp, vp, vp, C.c_int,
                                          C.c_int, C.c_double, C.c_int, vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
+
+
+def release_stream(stream):
+    """Free the library's device workspace of a torch stream (pbagpu_release_stream)."""
+    _check(lib().pbagpu_release_stream(_stream_ptr(stream)), "release_stream")
 
 
 def timing_enable(on=True):
@@ -230,7 +237,7 @@ class Pipeline:
     as the reference decides per blastwave whether the reverse shock radiates).
     """
 
-    def __init__(self, runs, stream=None, precision=0):
+    def __init__(self, runs, stream=None, precision=0, rad_stream=None):
         torch = _torch()
         self.torch = torch
         self.L = lib()
@@ -254,6 +261,10 @@ class Pipeline:
         self.precision = precision
         self.stream = stream
         self.sp = _stream_ptr(stream)
+        # electrons and observer stage: on rad_stream if given (a pipeline of batches can keep the
+        # dynamics of several batches in flight on their own streams and share one stream, and
+        # so one set of observer workspaces, for the rest)
+        self.rsp = _stream_ptr(rad_stream) if rad_stream is not None else self.sp
         self.params = (abi.RunParams * self.n_runs)(*[r.params for r in runs])
         self.inits = (abi.LayerInit * (self.n_runs * self.bpr))(*[li for r in runs for li in r.inits])
         nbw = self.n_runs * self.bpr
@@ -300,7 +311,7 @@ class Pipeline:
         _check(self.L.pbagpu_ele_evolve(self.params, self.n_runs, self.parr, npop, _ptr(self.d_table),
                                         self.n_runs * self.bpr, self.nt, _ptr(self.d_jall), _ptr(self.d_aall),
                                         _ptr(self.d_fall),
-                                        _ptr(self.d_est), self.precision, self.sp), "ele_evolve")
+                                        _ptr(self.d_est), self.precision, self.rsp), "ele_evolve")
 
     def observer(self):
         if self.method == "piece-wise":
@@ -308,12 +319,12 @@ class Pipeline:
                                                     _ptr(self.d_table), self.nt, _ptr(self.d_jall), _ptr(self.d_aall),
                                                     self.nf,
                                                     self.npairs, _ptr(self.d_tobs), _ptr(self.d_nuobs),
-                                                    _ptr(self.d_out), _ptr(self.d_lst), self.sp), "eats_lightcurve_pw")
+                                                    _ptr(self.d_out), _ptr(self.d_lst), self.rsp), "eats_lightcurve_pw")
             return
         _check(self.L.pbagpu_eats_lightcurve(self.params, self.n_runs, self.larr, len(self.layers), _ptr(self.d_table),
                                              self.nt, _ptr(self.d_jall), _ptr(self.d_aall), self.nf, self.npairs,
                                              _ptr(self.d_tobs),
-                                             _ptr(self.d_nuobs), _ptr(self.d_out), _ptr(self.d_lst), self.sp),
+                                             _ptr(self.d_nuobs), _ptr(self.d_out), _ptr(self.d_lst), self.rsp),
                "eats_lightcurve")
 
     def skymaps(self, times, freqs, remove_mu=False):
@@ -341,7 +352,7 @@ class Pipeline:
         _check(self.L.pbagpu_eats_skymap_pw(self.params, self.larr, nl, _ptr(self.d_table), self.nt, _ptr(self.d_jall),
                                             _ptr(self.d_aall),
                                             self.nf, n_img, _ptr(d_t), _ptr(d_nu), 1 if remove_mu else 0,
-                                            _ptr(d_img), _ptr(d_flux), _ptr(d_st), self.sp), "eats_skymap_pw")
+                                            _ptr(d_img), _ptr(d_flux), _ptr(d_st), self.rsp), "eats_skymap_pw")
         return d_img.cpu().numpy(), d_flux.cpu().numpy(), d_st.cpu().numpy()
 
     def step(self):
