@@ -30,6 +30,11 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# one hardware queue per stream of the pipeline (depth + 1 dynamics streams and the radiation stream):
+# streams that share a queue run their kernels one after another, which would put the observer
+# stage behind a dynamics kernel; set before the HIP runtime starts
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
@@ -45,8 +50,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--batch", type=int, default=12, help="independent runs per step")
-    ap.add_argument("--depth", type=int, default=3,
+    ap.add_argument("--depth", type=int, default=4,
                     help="pipeline depth: batches whose dynamics run while the radiation stages of an older batch run")
+    ap.add_argument("--dyn-cus", type=int, default=32,
+                    help="compute units reserved for the dynamics streams (the radiation stream gets the others)")
     ap.add_argument("--precision", type=int, default=0, help="0: FP64 everywhere")
     ap.add_argument("--workload", default="gauss21_fsrs_ssc",
                     choices=["gauss21_fsrs_ssc", "tophat_fs_syn", "tophat64_pw", "lhs_members"],
@@ -132,13 +139,16 @@ def cpu_baseline(workload, budget_s):
                        f"scaled to {per_lc:.0f}s per light curve")}
 
 
-def roofline(stage_ms, pls, gpu):
-    """Roofline entry of the stage with the most device time in the timed steps."""
+def roofline(stage_ms, pls, gpu, dom=None):
+    """Roofline entry of one stage (default: the radiation-stream stage with the most device time in
+    the timed steps, which sets the step time; the dynamics overlap it on their own units)."""
     import numpy as np
     from pyblastafterglowmag_amd import abi
     names = {gpu.STAGE_DYN: "dyn_kernel", gpu.STAGE_ELE: "ele (per-step kernels)", gpu.STAGE_EATS: "eats_kernel",
              gpu.STAGE_SSC_TABLE: "ssc_table", gpu.STAGE_EATS_PW: "eats_pw_kernel"}
-    dom = max(stage_ms, key=lambda s: stage_ms[s][0])
+    if dom is None:
+        crit = [gpu.STAGE_ELE, gpu.STAGE_EATS, gpu.STAGE_EATS_PW]
+        dom = max(crit, key=lambda s: stage_ms[s][0])
     ms, launches = stage_ms[dom]
     avg_s = ms / max(launches, 1) / 1e3
     roof = {"kernel": names[dom], "avg_launch_ms": ms / max(launches, 1), "bound": None, "achieved": None,
@@ -215,8 +225,13 @@ def main():
     run = make(0)[0]
     # D + 1 pipelines in a ring: batch b uses ring[b % (D + 1)]; its dynamics run on that pipeline's own
     # stream, its electron and observer stages on the shared radiation stream
-    rad_stream = torch.cuda.Stream()
-    ring = [gpu.Pipeline(make(k), stream=torch.cuda.Stream(), precision=args.precision, rad_stream=rad_stream)
+    # the dynamics run on compute units [0, dyn_cus) and the electron and observer stages on the rest, so
+    # that the persistent observer kernel, which holds its units until it ends, never keeps a batch's
+    # dynamics from starting
+    n_cu = gpu.device_cu_count()
+    dyn_cus = min(max(args.dyn_cus, 1), n_cu - 1)
+    rad_stream = gpu.CuStream(dyn_cus, n_cu - dyn_cus)
+    ring = [gpu.Pipeline(make(k), stream=gpu.CuStream(0, dyn_cus), precision=args.precision, rad_stream=rad_stream)
             for k in range(D + 1)]
     gpu.timing_enable(True)
     threads = {}
@@ -286,6 +301,7 @@ def main():
               f"{gpu.timing_units(gpu.STAGE_EATS):.3e}; piece-wise cells {gpu.timing_units(gpu.STAGE_EATS_PW):.3e}",
               file=sys.stderr)
     roof = roofline(stage_ms, pls, gpu)
+    roof_dyn = roofline(stage_ms, pls, gpu, dom=gpu.STAGE_DYN) if stage_ms[gpu.STAGE_DYN][1] else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload, args.cpu_seconds)
     n_rej = sum(rejected.get(b, 0) for b in counted)
     lcs = (args.batch * len(counted) - n_rej) / dt
@@ -297,10 +313,11 @@ def main():
         "config": {"workload": args.workload, "runs_per_step": args.batch, "layers": run.n_layers,
                    "ngam": run.params.fs.ngam, "nfreq": run.params.fs.nfreq, "nt": len(run.tgrid),
                    "pairs": len(run.tobs), "precision": args.precision, "pipeline_depth": D,
+                   "dynamics_cus": dyn_cus, "radiation_cus": n_cu - dyn_cus,
                    "batches_counted": len(counted), "light_curves_counted": args.batch * len(counted) - n_rej,
                    **({"members_flagged": n_rej, "members": "Latin hypercube, seed 20261203, runs b*batch .. "
                        "(b+1)*batch-1 per step"} if args.workload == "lhs_members" else {})},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "roofline_dynamics": roof_dyn, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

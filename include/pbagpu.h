@@ -77,7 +77,8 @@ typedef struct {
     int syn;              /* method_synchrotron: 0 = CSYN (SynKernel::cycSynch, kernels.h:414-437),
                              1 = Dermer09 (SynKernel::synchDermer, kernels.h:451-473),
                              2 = Bessel (SynKernel::synchBessel, kernels.h:396-412),
-                             3 = GSL (SynKernel::synchGSL, kernels.h:211-342) */
+                             3 = GSL (SynKernel::synchGSL, kernels.h:211-342; F(x) from the
+                                 integral representation of K_5/3, pbagpu_synchrotron_f) */
     int pad_;
 } pba_shock_params;
 
@@ -161,6 +162,15 @@ const char *pbagpu_last_error(void);
  * device memory: this frees the workspace of `stream` on the current device, after
  * waiting for the stream; the next call on it allocates again. */
 int pbagpu_release_stream(void *stream);
+/* A stream whose kernels run only on compute units [cu_first, cu_first + cu_count) of the
+ * current device (cu_count <= 0: a plain non-blocking stream on every unit).  No
+ * counterpart in the reference.  The dynamics of a run is a long chain of dependent ODE
+ * steps on one wavefront, while the persistent observer kernel holds every unit it is
+ * given until it ends; giving the dynamics a few units of their own lets batches of
+ * runs be pipelined.  The persistent observer grid is sized to its stream's units. */
+int pbagpu_stream_create(int cu_first, int cu_count, void **stream);
+/* pbagpu_release_stream, then destroy a stream made by pbagpu_stream_create. */
+int pbagpu_stream_destroy(void *stream);
 
 /* ---------------------------------------------------- kernel timing */
 /* No counterpart in the reference: instrumentation for benchmarks.  When
@@ -304,6 +314,12 @@ int pbagpu_eats_skymap_pw(const pba_run_params *params, const pba_eats_pw_layer 
  * counterpart in the reference (which calls std::cyl_bessel_k, kernels.h:402-403): it
  * exists so that the device routine's arithmetic can be checked against libm on the host. */
 int pbagpu_bessel_k_thirds(int n, const double *x, double *k13, double *k43);
+/* Host evaluation (no GPU) of the device routine behind method_synchrotron = GSL: the
+ * synchrotron function F(x[i]) = x int_x^inf K_5/3(t) dt.  The reference takes F from GSL's
+ * Chebyshev fits (SynKernel::synchGSL, kernels.h:211-342); this library integrates the
+ * integral representation of K_5/3 instead.  No counterpart in the reference: it exists so
+ * that the routine can be checked against the defining integral on the host. */
+int pbagpu_synchrotron_f(int n, const double *x, double *f);
 
 /* ------------------------------------------------- sky-map binning */
 /* Histogram image, flux centroid and data extent of raw sky maps: the binning part

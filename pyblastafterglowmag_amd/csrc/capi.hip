@@ -141,3 +141,52 @@ int release_stream(hipStream_t stream) {
 extern "C" const char *pbagpu_last_error(void) { return pbagpu::g_err; }
 
 extern "C" int pbagpu_release_stream(void *stream) { return pbagpu::release_stream((hipStream_t)stream); }
+
+// ------------------------------------------------------------ CU-partitioned streams
+namespace pbagpu {
+int stream_cu_count(hipStream_t stream) {
+    int dev = 0, n_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 1;
+    uint32_t mask[32] = {0};
+    if (hipExtStreamGetCUMask(stream, 32, mask) != hipSuccess) return n_cu;
+    int c = 0;
+    for (int i = 0; i < 32 && i * 32 < n_cu; i++) c += __builtin_popcount(mask[i]);
+    return (c > 0 && c <= n_cu) ? c : n_cu;
+}
+}  // namespace pbagpu
+
+extern "C" int pbagpu_stream_create(int cu_first, int cu_count, void **stream) {
+    if (!stream) return PBAGPU_E_ARG;
+    *stream = nullptr;
+    int dev = 0, n_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) { pbagpu::set_error("hipDeviceGetAttribute", e); return PBAGPU_E_HIP; }
+    hipStream_t s = nullptr;
+    if (cu_count <= 0) {
+        e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e != hipSuccess) { pbagpu::set_error("hipStreamCreateWithFlags", e); return PBAGPU_E_HIP; }
+        *stream = (void *)s;
+        return PBAGPU_OK;
+    }
+    if (cu_first < 0 || cu_first + cu_count > n_cu || n_cu > 32 * 32) {
+        pbagpu::set_error_msg("pbagpu_stream_create: CU range outside the device");
+        return PBAGPU_E_ARG;
+    }
+    uint32_t mask[32] = {0};
+    for (int c = cu_first; c < cu_first + cu_count; c++) mask[c / 32] |= 1u << (c % 32);
+    e = hipExtStreamCreateWithCUMask(&s, (uint32_t)((n_cu + 31) / 32), mask);
+    if (e != hipSuccess) { pbagpu::set_error("hipExtStreamCreateWithCUMask", e); return PBAGPU_E_HIP; }
+    *stream = (void *)s;
+    return PBAGPU_OK;
+}
+
+extern "C" int pbagpu_stream_destroy(void *stream) {
+    if (!stream) return PBAGPU_E_ARG;
+    int rc = pbagpu::release_stream((hipStream_t)stream);
+    hipError_t e = hipStreamDestroy((hipStream_t)stream);
+    if (e != hipSuccess) { pbagpu::set_error("hipStreamDestroy", e); return PBAGPU_E_HIP; }
+    return rc;
+}

@@ -714,9 +714,7 @@ extern "C" int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs, 
     pbagpu::KernelTimer timer(PBAGPU_STAGE_EATS);
     if (rc == PBAGPU_OK) {
         const size_t lds = sizeof(double) * ((size_t)NSLOT * nt + 2 * (size_t)nfreq);
-        int dev = 0, n_cu = 0;
-        PBA_TRY(rc, hipGetDevice(&dev));
-        PBA_TRY(rc, hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+        const int n_cu = pbagpu::stream_cu_count(s);   // the stream's CU mask bounds the persistent grid
         const long n_items = ((long)n_pairs + EATS_WG - 1) / EATS_WG * n_layers;   // workgroup-sized pieces
         bool ssa = false;   // any absorption table: the instantiation with the absorption code
         for (const LayerArgs &A : hl) ssa = ssa || A.afs || A.ars;

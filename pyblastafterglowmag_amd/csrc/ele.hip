@@ -83,59 +83,37 @@ __device__ __noinline__ double bessel_synch(double B, double nu, double gam, Bes
     return bessel_synch_shape(t, b) * (sqrt(3.) * (kQe * kQe * kQe) * B / (kMe * kC * kC));
 }
 
-// synchGSL (kernels.h:211-342): the synchrotron function F(x) = x int_x^inf K_5/3 by the GNU
-// Scientific Library's Chebyshev fits (gsl_sf_synchrotron_1: three coefficient tables, a
-// small-x expansion and a large-x asymptotic form), x = (h nu / me c^2) / (1.5 B / 4.14e13 gam^2),
-// times sqrt(3) qe^3 B / (me c^2).  The tables are GSL's published ones, as the reference has them.
-__constant__ double kGslSyn1[13] = {
-    30.364682982501076273, 17.079395277408394574, 4.560132133545072889, 0.549281246730419979,
-    0.372976075069301172e-01, 0.161362430201041242e-02, 0.481916772120371e-04, 0.10512425288938e-05,
-    0.174638504670e-07, 0.22815486544e-09, 0.240443082e-11, 0.2086588e-13, 0.15167e-15};
-__constant__ double kGslSyn1a[23] = {
-    2.1329305161355000985, 0.741352864954200240e-01, 0.86968099909964198e-02, 0.11703826248775692e-02,
-    0.1645105798619192e-03, 0.240201021420640e-04, 0.35827756389389e-05, 0.5447747626984e-06,
-    0.838802856196e-07, 0.13069882684e-07, 0.2053099071e-08, 0.325187537e-09, 0.517914041e-10,
-    0.83002988e-11, 0.13352728e-11, 0.2159150e-12, 0.349967e-13, 0.56994e-14, 0.9291e-15,
-    0.152e-15, 0.249e-16, 0.41e-17, 0.7e-18};
-__constant__ double kGslSyn2[12] = {
-    0.4490721623532660844, 0.898353677994187218e-01, 0.81044573772151290e-02, 0.4261716991089162e-03,
-    0.147609631270746e-04, 0.3628633615300e-06, 0.66634807498e-08, 0.949077166e-10, 0.1079125e-11,
-    0.10022e-13, 0.77e-16, 0.5e-18};
-
-// Clenshaw summation of sum_j c_j T_j(t) - c_0 / 2 on [-1, 1], without fused multiply-adds
-__device__ double chebyshev(const double *cf, int order, double t) {
-#pragma clang fp contract(off)
-    const double y = (2.0 * t - (-1.0) - 1.0) / (1.0 - (-1.0)), y2 = 2.0 * y;
-    double d = 0.0, dd = 0.0;
-    for (int j = order; j > 0; j--) {
-        const double keep = d;
-        d = y2 * d - dd + cf[j];
-        dd = keep;
+// method_synchrotron = GSL (SynKernel::synchGSL, kernels.h:211-342): the single-pitch-angle
+// synchrotron function F(x) = x int_x^inf K_5/3(t) dt at x = (h nu / me c^2) / (1.5 B / 4.14e13
+// gam^2), times sqrt(3) qe^3 B / (me c^2), as the reference forms them.  F itself is evaluated
+// here from the integral representation K_nu(t) = int_0^inf exp(-t cosh u) cosh(nu u) du
+// (DLMF 10.32.9): integrating over t >= x gives
+//     F(x) = x int_0^inf cosh(5u/3) / cosh(u) exp(-x cosh u) du,
+// a positive integrand analytic in the strip |Im u| < pi/2, so the trapezoidal rule in u converges
+// geometrically.  Step h = min(0.14, 0.75 / sqrt(x)) (the integrand's width shrinks as 1/sqrt(x) for
+// large x); the sum stops once x (cosh u - 1) > 46 + 2u/3 (terms below e^-46 of the first).
+// Below x = 1e-25 the leading term 4 pi / (sqrt(3) Gamma(1/3)) (x/2)^(1/3) is used (next term
+// relative ~x^(2/3)); above x = 745, e^-x underflows and F = 0.
+PBA_HD double synchrotron_F(double x) {
+    if (!(x > 0.)) return 0.;
+    if (x < 1e-25) return 4.0 * kPi / (sqrt(3.0) * 2.6789385347077476337) * cbrt(0.5 * x);  // Gamma(1/3)
+    if (x > 745.) return 0.;
+    const double h = fmin(0.14, 0.75 / sqrt(x));
+    double s = 0.5 * exp(-x);
+    for (int k = 1; k < 4096; k++) {
+        const double u = k * h;
+        const double eu = exp(u), e53 = exp((5.0 / 3.0) * u);
+        const double ch = 0.5 * (eu + 1.0 / eu);
+        s += 0.5 * (e53 + 1.0 / e53) / ch * exp(-x * ch);
+        if (x * (ch - 1.0) > 46.0 + (2.0 / 3.0) * u) break;
     }
-    return y * d - dd + 0.5 * cf[0];
+    return x * h * s;
 }
 
 __device__ __noinline__ double gsl_synch(double B, double nu, double gam) {
-#pragma clang fp contract(off)
     const double ec = 1.5 * B / 4.14e13;
     const double x = nu * (kH / kMe / kC / kC) / (ec * gam * gam);
-    const double small = 2 * 1.4142135623730951 * 1.4901161193847656e-08;
-    const double large = -8.0 * (-7.0839641853226408e02) / 7.0;
-    double r;
-    if (x < small) {
-        const double z = pow(x, 1.0 / 3.0);
-        r = 2.14952824153447863671 * z * (1 - 8.43812762813205e-01 * z * z);
-    } else if (x <= 4.0) {
-        const double px = pow(x, 1.0 / 3.0), px11 = pow(px, 11.);
-        const double t = x * x / 8.0 - 1.0;
-        r = px * chebyshev(kGslSyn1, 12, t) - px11 * chebyshev(kGslSyn2, 11, t) - (kPi / 1.7320508075688772) * x;
-    } else if (x < large) {
-        const double t = (12.0 - x) / (x + 4.0);
-        r = sqrt(x) * chebyshev(kGslSyn1a, 22, t) * exp(0.2257913526447274323630976 - x);
-    } else {
-        r = 0.0;
-    }
-    return r * (sqrt(3.) * pow(kQe, 3.) * B / kMec2);
+    return synchrotron_F(x) * (sqrt(3.) * pow(kQe, 3.) * B / kMec2);
 }
 
 // Kernel table K[i][j][k] and its frequency integral KI[j][k] (kernels.h:49-94).
@@ -877,5 +855,11 @@ extern "C" int pbagpu_bessel_k_thirds(int n, const double *x, double *k13, doubl
     if (n < 0 || (n > 0 && (!x || !k13 || !k43))) return PBAGPU_E_ARG;
     const pbagpu::BesselThird b = pbagpu::bessel_third_constants();
     for (int i = 0; i < n; i++) pbagpu::bessel_k_thirds(x[i], b, k13[i], k43[i]);
+    return PBAGPU_OK;
+}
+
+extern "C" int pbagpu_synchrotron_f(int n, const double *x, double *f) {
+    if (n < 0 || (n > 0 && (!x || !f))) return PBAGPU_E_ARG;
+    for (int i = 0; i < n; i++) f[i] = pbagpu::ele::synchrotron_F(x[i]);
     return PBAGPU_OK;
 }

@@ -29,6 +29,8 @@ enum WsSlot {
 void *workspace(int slot, size_t bytes, hipStream_t stream);
 // Free every workspace buffer of (current device, stream), after waiting for the stream.
 int release_stream(hipStream_t stream);
+// Compute units a stream may use (its CU mask; all of the device for a plain stream).
+int stream_cu_count(hipStream_t stream);
 
 }  // namespace pbagpu
 

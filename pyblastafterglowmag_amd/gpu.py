@@ -17,7 +17,8 @@ _lib = None
 EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu_last_error", "pbagpu_timing_enable",
            "pbagpu_timing_read", "pbagpu_timing_units", "pbagpu_dyn_evolve", "pbagpu_ele_evolve", "pbagpu_eats_lightcurve",
            "pbagpu_eats_lightcurve_pw", "pbagpu_eats_skymap_pw", "pbagpu_skymap_hist", "pbagpu_bessel_k_thirds",
-           "pbagpu_release_stream")
+           "pbagpu_release_stream", "pbagpu_stream_create", "pbagpu_stream_destroy",
+           "pbagpu_synchrotron_f")
 STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW, STAGE_SKY_HIST = 0, 1, 2, 3, 4, 5
 N_STAGES = 6
 # sky-map binning (enum pba_sky_moment / pba_sky_status)
@@ -56,6 +57,9 @@ def lib():
                                             vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp]
         L.pbagpu_bessel_k_thirds.argtypes = [C.c_int, vp, vp, vp]
         L.pbagpu_release_stream.argtypes = [vp]
+        L.pbagpu_synchrotron_f.argtypes = [C.c_int, vp, vp]
+        L.pbagpu_stream_create.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.pbagpu_stream_destroy.argtypes = [vp]
         L.pbagpu_skymap_hist.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_long), vp, vp, vp, C.c_int,
                                          C.c_int, C.c_double, C.c_int, vp, vp, vp, vp, vp, vp, vp]
         _lib = L
@@ -65,6 +69,30 @@ def lib():
 def release_stream(stream):
     """Free the library's device workspace of a torch stream (pbagpu_release_stream)."""
     _check(lib().pbagpu_release_stream(_stream_ptr(stream)), "release_stream")
+
+
+class CuStream:
+    """A HIP stream restricted to compute units [cu_first, cu_first + cu_count) (pbagpu_stream_create);
+    cu_count <= 0 gives a plain stream.  Usable wherever the Pipeline takes a stream."""
+
+    def __init__(self, cu_first=0, cu_count=0):
+        _torch()
+        p = C.c_void_p()
+        _check(lib().pbagpu_stream_create(int(cu_first), int(cu_count), C.byref(p)), "stream_create")
+        self.cuda_stream = p.value
+
+    def synchronize(self):
+        import torch
+        torch.cuda.ExternalStream(self.cuda_stream).synchronize()
+
+    def close(self):
+        if self.cuda_stream:
+            _check(lib().pbagpu_stream_destroy(C.c_void_p(self.cuda_stream)), "stream_destroy")
+            self.cuda_stream = None
+
+
+def device_cu_count(device=0):
+    return _torch().cuda.get_device_properties(device).multi_processor_count
 
 
 def timing_enable(on=True):
@@ -114,6 +142,15 @@ def bessel_k_thirds(x):
     _check(lib().pbagpu_bessel_k_thirds(len(x), x.ctypes.data_as(C.c_void_p), k13.ctypes.data_as(C.c_void_p),
                                         k43.ctypes.data_as(C.c_void_p)), "bessel_k_thirds")
     return k13, k43
+
+
+def synchrotron_f(x):
+    """F(x) = x int_x^inf K_5/3 by the electron stage's device routine (GSL kernel option), on the host."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    f = np.empty_like(x)
+    _check(lib().pbagpu_synchrotron_f(len(x), x.ctypes.data_as(C.c_void_p), f.ctypes.data_as(C.c_void_p)),
+           "synchrotron_f")
+    return f
 
 
 def skymap_hist(images, nx, ny, extend_grid=1.0, edges=None, want_count=False, device=0):
@@ -360,13 +397,23 @@ class Pipeline:
         self.electrons()
         self.observer()
 
+    def _on_rad_stream(self):
+        """torch context on the stream of the electron and observer stages.  Copies to the host
+        must not go through the null stream: it waits for every blocking stream, and so for the
+        dynamics kernels of other batches that run on CU-masked streams meanwhile."""
+        if self.rsp is None or not self.rsp.value:
+            return self.torch.cuda.stream(self.torch.cuda.current_stream())
+        return self.torch.cuda.stream(self.torch.cuda.ExternalStream(self.rsp.value))
+
     def status(self):
         npop = len(self.pops)
-        return (np.array([r.status for r in self.res]), self.d_est[:npop].cpu().numpy(),
-                self.d_lst[:len(self.layers)].cpu().numpy())
+        with self._on_rad_stream():
+            return (np.array([r.status for r in self.res]), self.d_est[:npop].cpu().numpy(),
+                    self.d_lst[:len(self.layers)].cpu().numpy())
 
     def flux(self):
-        return self.d_out.cpu().numpy().reshape(self.n_runs, self.bpr, self.npairs)
+        with self._on_rad_stream():
+            return self.d_out.cpu().numpy().reshape(self.n_runs, self.bpr, self.npairs)
 
 
 def lightcurves(runs, stream=None, want_intermediates=False, precision=0):

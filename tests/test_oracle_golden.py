@@ -54,7 +54,8 @@ def _oracle_all(g):
     ("gauss5_ssa_small", 1e-4, 1e-5, 1e-8),
     ("tophat_dermer_small", 0.0, 0.0, 0.0),
     ("tophat_bessel_small", 0.0, 0.0, 0.0),
-    ("tophat_gsl_small", 0.0, 0.0, 0.0),
+    # F(x) of the GSL kernel option is computed from its integral, not from GSL's fits
+    ("tophat_gsl_small", 0.0, 1e-11, 1e-11),
     ("tophat_rk4_small", 0.0, 0.0, 0.0),
 ])
 def test_oracle_matches_reference(name, tol_dyn, tol_j, tol_lc):

@@ -225,24 +225,56 @@ def test_bessel_kernel_is_the_angle_averaged_synchrotron_function(t):
         assert val == pytest.approx(8 * math.pi / (27 * math.sqrt(3)), rel=1e-8)
 
 
+def _F_exact(x):
+    """F(x) = x int_x^inf K_5/3 written through K_5/3 = -2 K'_2/3 - K_1/3 as x (2 K_2/3(x) - int_x^inf
+    K_1/3), an integrand without the t^-5/3 singularity (scipy)."""
+    tail = integrate.quad(lambda t: special.kv(1. / 3., t), x, 1.0, limit=400, epsabs=0, epsrel=1e-13)[0] + \
+        integrate.quad(lambda t: special.kv(1. / 3., t), 1.0, np.inf, limit=400, epsabs=0, epsrel=1e-13)[0] \
+        if x < 1.0 else integrate.quad(lambda t: special.kv(1. / 3., t), x, np.inf, limit=400, epsabs=0,
+                                       epsrel=1e-13)[0]
+    return x * (2 * special.kv(2. / 3., x) - tail)
+
+
 @pytest.mark.parametrize("x", [1e-12, 1e-6, 0.01, 0.3, 1.0, 3.9, 4.1, 20.0, 200.0])
 def test_gsl_kernel_is_the_synchrotron_function(x):
-    """synchGSL: the GNU Scientific Library's Chebyshev fit of F(x) = x int_x^inf K_5/3 (single pitch
-    angle) in every branch (small-x expansion, two tables up to x = 4, asymptotic table above),
-    against the defining integral; x = (h nu / me c^2) / (1.5 B / 4.14e13 gam^2) as the reference
-    forms it.  A mistyped coefficient of the tables shows here."""
+    """method_synchrotron = GSL (SynKernel::synchGSL): sqrt(3) qe^3 B / (me c^2) F(x) with F(x) =
+    x int_x^inf K_5/3 (single pitch angle) and x = (h nu / me c^2) / (1.5 B / 4.14e13 gam^2) as the
+    reference forms it.  The oracle integrates the representation of K_5/3; here F is the defining
+    integral by scipy.  A wrong step, stopping rule or prefactor shows here."""
     L = _lib()
     B, gam = 1.0, 1e3
     en_per_nu = H / ME / CL / CL
     nu = x * (1.5 * B / 4.14e13 * gam * gam) / en_per_nu
     pref = math.sqrt(3.) * QE ** 3 * B / MEC2
-    # K_5/3 = -2 K'_2/3 - K_1/3, so F(x) = x (2 K_2/3(x) - int_x^inf K_1/3), an integrand without
-    # the t^-5/3 singularity
-    tail = integrate.quad(lambda t: special.kv(1. / 3., t), x, 1.0, limit=400, epsabs=0, epsrel=1e-13)[0] + \
-        integrate.quad(lambda t: special.kv(1. / 3., t), 1.0, np.inf, limit=400, epsabs=0, epsrel=1e-13)[0]
-    F = x * (2 * special.kv(2. / 3., x) - tail)
+    F = _F_exact(x)
     got = L.orc_synch_gsl(B, nu, gam) / pref
     assert got == pytest.approx(F, rel=1e-11)
     if x < 1e-9:   # leading term of the small-x series: 4 pi / (sqrt 3 Gamma(1/3)) (x/2)^(1/3)
         assert got == pytest.approx(4 * math.pi / (math.sqrt(3) * special.gamma(1. / 3.)) * (x / 2) ** (1. / 3.),
                                     rel=1e-7)
+
+
+def test_synchrotron_function_device_routine():
+    """The electron stage's device routine for F(x) (run on the host through pbagpu_synchrotron_f)
+    against the defining integral over 1e-3 <= x <= 500, against the asymptotic forms at the
+    ends, and against the oracle's evaluation."""
+    from pyblastafterglowmag_amd import gpu
+    xs = np.array([1e-3, 0.03, 0.2, 0.7, 1.5, 3.0, 8.0, 25.0, 90.0, 500.0])
+    f = gpu.synchrotron_f(xs)
+    for x, v in zip(xs, f):
+        assert v == pytest.approx(_F_exact(x), rel=1e-11), x
+    # F(x) -> sqrt(pi x / 2) e^-x (1 + 55/(72 x)) for large x; 4 pi/(sqrt 3 Gamma(1/3)) (x/2)^(1/3) small
+    big = gpu.synchrotron_f(np.array([700.0]))[0]
+    assert big == pytest.approx(math.sqrt(math.pi * 350.0) * math.exp(-700.0) * (1 + 55 / (72 * 700.0)), rel=1e-5)
+    tiny = gpu.synchrotron_f(np.array([1e-20, 1e-30]))
+    lead = 4 * math.pi / (math.sqrt(3) * special.gamma(1. / 3.)) * (np.array([1e-20, 1e-30]) / 2) ** (1. / 3.)
+    np.testing.assert_allclose(tiny, lead, rtol=1e-12)
+    assert list(gpu.synchrotron_f(np.array([0.0, -1.0, 800.0]))) == [0.0, 0.0, 0.0]
+    L = _lib()
+    B, gam = 1.0, 1e3
+    pref = math.sqrt(3.) * QE ** 3 * B / MEC2
+    for x in np.logspace(-24, 2.8, 60):
+        nu = x * (1.5 * B / 4.14e13 * gam * gam) / (H / ME / CL / CL)
+        o = L.orc_synch_gsl(B, nu, gam) / pref
+        xd = nu * (H / ME / CL / CL) / (1.5 * B / 4.14e13 * gam * gam)
+        assert gpu.synchrotron_f(np.array([xd]))[0] == pytest.approx(o, rel=1e-13), x
