@@ -185,7 +185,8 @@ enum {
     PBAGPU_STAGE_SSC_TABLE = 3,/* Compton kernel table build */
     PBAGPU_STAGE_EATS_PW = 4,  /* eats_pw_kernel (piece-wise observer stage) */
     PBAGPU_STAGE_SKY_HIST = 5, /* sky_bin_kernel (sky-map histogram) */
-    PBAGPU_N_STAGES = 6
+    PBAGPU_STAGE_EATS_VEC = 6, /* eats_vec_kernel (frequency-vectorised observer stage) */
+    PBAGPU_N_STAGES = 7
 };
 int pbagpu_timing_enable(int on);
 int pbagpu_timing_read(int stage, double *ms_total, long long *launches, int reset);
@@ -257,6 +258,26 @@ int pbagpu_eats_lightcurve(const pba_run_params *params, int n_runs,
                            const double *d_table, int nt, const double *d_jall, const double *d_aall, int nfreq,
                            int n_pairs, const double *d_tobs, const double *d_nuobs,
                            double *d_out, int *d_status, void *stream);
+
+/* The same light curves on a tensor grid of n_t observer times x n_nu frequencies
+ * (pairs t-major, as cast_times_freqs, src/utilitites/utils.h:1419-1448), by a quadrature
+ * shared across frequencies: one wavefront per (layer, t_obs) integrates every nu_obs with
+ * one adaptive Gauss-Kronrod 7-15 tree in phi and in 1 - cos(theta), refined until
+ * |K15 - G7| <= rtol |I| on every interval share for every frequency.  The integrand, its
+ * domain and its arithmetic (FP64) are those of pbagpu_eats_lightcurve; the CADRE trees of
+ * the reference (one per pair, rtol_theta / rtol_phi) are replaced, so the result agrees
+ * with it to the quadrature tolerance, not bit for bit.  The surface geometry of a sample
+ * (mu, the arrival-time search, the state interpolation) is computed once for all
+ * frequencies instead of once per pair.  No counterpart in the reference; it replaces the
+ * same call site as pbagpu_eats_lightcurve (EjectaBase::evalEjectaLightCurves,
+ * src/ejecta/ejecta_base.h:1164-1192).  n_nu <= 128; synchrotron self-absorption is not
+ * implemented here (PBAGPU_E_UNSUPPORTED).
+ *   d_tobs [n_t], d_nuobs [n_nu] device; d_out [n_layers][n_t * n_nu] device, written, mJy */
+int pbagpu_eats_lightcurve_vec(const pba_run_params *params, int n_runs,
+                               const pba_eats_layer *layers, int n_layers,
+                               const double *d_table, int nt, const double *d_jall, int nfreq,
+                               int n_t, const double *d_tobs, int n_nu, const double *d_nuobs,
+                               double rtol, double *d_out, int *d_status, void *stream);
 
 /* Piece-wise EATS: the layer's light curve as the sum over its 2i+1 phi cells
  * at the layer centre angle, primary jet and (counter_jet) counter-jet.

@@ -18,6 +18,7 @@ enum WsSlot {
     WS_EATS_NU, WS_EATS_LAYERS, WS_EATS_STORE, WS_EATS_COUNTER, WS_EATS_SAMPLES,
     WS_PW_NU, WS_PW_LAYERS, WS_PW_SAMPLES,
     WS_SKY_SEG, WS_SKY_OFF, WS_SKY_PART,
+    WS_VEC_NU, WS_VEC_LAYERS, WS_VEC_SAMPLES,
     WS_N
 };
 

@@ -18,9 +18,9 @@ EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu
            "pbagpu_timing_read", "pbagpu_timing_units", "pbagpu_dyn_evolve", "pbagpu_ele_evolve", "pbagpu_eats_lightcurve",
            "pbagpu_eats_lightcurve_pw", "pbagpu_eats_skymap_pw", "pbagpu_skymap_hist", "pbagpu_bessel_k_thirds",
            "pbagpu_release_stream", "pbagpu_stream_create", "pbagpu_stream_destroy",
-           "pbagpu_synchrotron_f")
-STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW, STAGE_SKY_HIST = 0, 1, 2, 3, 4, 5
-N_STAGES = 6
+           "pbagpu_synchrotron_f", "pbagpu_eats_lightcurve_vec")
+STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW, STAGE_SKY_HIST, STAGE_EATS_VEC = 0, 1, 2, 3, 4, 5, 6
+N_STAGES = 7
 # sky-map binning (enum pba_sky_moment / pba_sky_status)
 SKY_MOMENTS = ("xc", "yc", "xmin", "xmax", "ymin", "ymax", "sumw", "npts")
 SKY_STATUS = {0: "ok", 1: "no entry with data", 2: "intensities sum to zero", 3: "flat entry (xmin == xmax or ymin == ymax)",
@@ -58,6 +58,9 @@ def lib():
         L.pbagpu_bessel_k_thirds.argtypes = [C.c_int, vp, vp, vp]
         L.pbagpu_release_stream.argtypes = [vp]
         L.pbagpu_synchrotron_f.argtypes = [C.c_int, vp, vp]
+        L.pbagpu_eats_lightcurve_vec.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsLayer), C.c_int,
+                                                 vp, C.c_int, vp, C.c_int, C.c_int, vp, C.c_int, vp, C.c_double, vp, vp,
+                                                 vp]
         L.pbagpu_stream_create.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.pbagpu_stream_destroy.argtypes = [vp]
         L.pbagpu_skymap_hist.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_long), vp, vp, vp, C.c_int,
@@ -363,6 +366,32 @@ class Pipeline:
                                              _ptr(self.d_tobs),
                                              _ptr(self.d_nuobs), _ptr(self.d_out), _ptr(self.d_lst), self.rsp),
                "eats_lightcurve")
+
+    def tensor_grid(self):
+        """(unique t_obs, unique nu_obs) of the batch's pairs, which are t-major over their product."""
+        r0 = self.runs[0]
+        n_nu = int(np.count_nonzero(r0.tobs == r0.tobs[0]))
+        ts, fs = r0.tobs[::n_nu], r0.nuobs[:n_nu]
+        if len(ts) * n_nu != len(r0.tobs) or not (np.array_equal(np.repeat(ts, n_nu), r0.tobs)
+                                                    and np.array_equal(np.tile(fs, len(ts)), r0.nuobs)):
+            raise ValueError("observer pairs are not a t-major tensor grid")
+        return ts, fs
+
+    def observer_vec(self, rtol=1e-4):
+        """Adaptive-method light curves by the frequency-vectorised observer kernel
+        (pbagpu_eats_lightcurve_vec): one Gauss-Kronrod tree per (layer, t_obs) shared by all nu_obs,
+        refined to rtol on every frequency.  Writes the same d_out layout as observer()."""
+        if self.method == "piece-wise":
+            raise NotImplementedError("the frequency-vectorised observer stage is for the adaptive method")
+        if not hasattr(self, "d_t_u"):
+            ts, fs = self.tensor_grid()
+            self.d_t_u = self.torch.from_numpy(np.ascontiguousarray(ts)).to(self.dev)
+            self.d_nu_u = self.torch.from_numpy(np.ascontiguousarray(fs)).to(self.dev)
+        _check(self.L.pbagpu_eats_lightcurve_vec(self.params, self.n_runs, self.larr, len(self.layers),
+                                                 _ptr(self.d_table), self.nt, _ptr(self.d_jall), self.nf,
+                                                 len(self.d_t_u), _ptr(self.d_t_u), len(self.d_nu_u), _ptr(self.d_nu_u),
+                                                 float(rtol), _ptr(self.d_out), _ptr(self.d_lst), self.rsp),
+               "eats_lightcurve_vec")
 
     def skymaps(self, times, freqs, remove_mu=False):
         """Raw piece-wise sky maps of a one-run batch at every (freq, time), frequency-major as
