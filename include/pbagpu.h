@@ -342,6 +342,83 @@ int pbagpu_bessel_k_thirds(int n, const double *x, double *k13, double *k43);
  * that the routine can be checked against the defining integral on the host. */
 int pbagpu_synchrotron_f(int n, const double *x, double *f);
 
+/* ------------------------------------------- learned SSC emissivity */
+/* No counterpart in the reference: a multilayer perceptron standing in for the SSC
+ * contraction computeSSCSpectrum (src/microphysics/numeric_model.h:247-285), trained on
+ * pairs that the plain contraction itself produces, with the plain contraction as the
+ * fallback and the judge on a sampled subset of cells every step.
+ *
+ * pba_mlp: dims[0] -> dims[1] -> ... -> dims[n_layers]; SiLU after every layer but the
+ * last; weights w[l] [dims[l]][dims[l+1]] and biases b[l] [dims[l+1]] in FP32 on the
+ * device (the master copy); products in BF16 on MFMA with FP32 accumulation.  The
+ * gradient (gw, gb) and Adam moment (mw, vw, mb, vb) buffers, of the same shapes, are
+ * needed by pbagpu_mlp_train_step only. */
+#define PBA_MLP_MAXL 8
+typedef struct {
+    int n_layers;
+    int dims[PBA_MLP_MAXL + 1];
+    float *w[PBA_MLP_MAXL], *b[PBA_MLP_MAXL];
+    float *gw[PBA_MLP_MAXL], *gb[PBA_MLP_MAXL];
+    float *mw[PBA_MLP_MAXL], *vw[PBA_MLP_MAXL], *mb[PBA_MLP_MAXL], *vb[PBA_MLP_MAXL];
+} pba_mlp;
+
+/* Inference: d_y [batch][dims[n_layers]] = net(d_x [batch][dims[0]]), FP32 device arrays. */
+int pbagpu_mlp_forward(const pba_mlp *net, int batch, const float *d_x, float *d_y, void *stream);
+/* One training step on a minibatch: forward, mean square error against d_t
+ * [batch][dims[n_layers]] (its value to *d_loss, device, if not NULL), backward (gradients
+ * to gw, gb) and an Adam update (beta1 0.9, beta2 0.999, eps 1e-8; `step` >= 1 counts the
+ * updates for the bias correction; lr = 0 leaves the weights and moments' effect out). */
+int pbagpu_mlp_train_step(const pba_mlp *net, int batch, const float *d_x, const float *d_t, float lr,
+                          int step, float *d_loss, void *stream);
+/* The product the network is built from, for tests: d_c [M][N] = A B with BF16-rounded
+ * operands and FP32 accumulation; d_a holds A [M][K] (trans_a = 0) or A^T [K][M]; d_b holds
+ * B [K][N] (trans_b = 0) or B^T [N][K]; FP32 device arrays. */
+int pbagpu_gemm_bf16(int M, int N, int K, const float *d_a, int trans_a, const float *d_b, int trans_b,
+                     float *d_c, void *stream);
+
+/* Surrogate of the SSC emissivity of one shock's grids (ngam electron bins, nfreq
+ * frequencies).  Its input, for a cell with electron spectrum f [ngam] and Compton photon
+ * weights w [nfreq] (w_k = n_k / nu_k (nu_{k+1} - nu_k), the weights the plain contraction
+ * takes), is x = [ (log10 max(f_j / max f, 1e-30) + 30) / 30 ] (ngam values),
+ * [ (log10 max(w_k / max w, 1e-30) + 30) / 30 ] (nfreq), log10(B) / 10 (1 value): dims[0] =
+ * ngam + nfreq + 1.  Its output y [nfreq] gives j_ssc[i] = max f max w 10^(y_offset[i] +
+ * y_scale y[i]) (the contraction is bilinear in f and w, so the two maxima scale it exactly).
+ * Every step, population q is also computed by the plain contraction when
+ * (131 q + 37 it) % check_every == 0; where the surrogate is off by more than check_tol
+ * (relative, over frequencies above 1e-6 of the spectrum's peak), that population takes the
+ * plain contraction for the rest of the call. */
+typedef struct {
+    pba_mlp net;
+    const float *y_offset;   /* [nfreq] device */
+    float y_scale;
+    float check_tol;
+    int check_every;         /* 100: one population in a hundred per step */
+    int pad_;
+} pba_ssc_surrogate;
+typedef struct {
+    long long inferred;      /* population-steps whose SSC emissivity the network gave */
+    long long checked;       /* population-steps also computed by the plain contraction */
+    long long failed;        /* checks above check_tol */
+    long long fallback;      /* population-steps computed by the plain contraction after a failed check */
+    double max_err;          /* largest relative error seen by a check */
+} pba_surrogate_stats;
+
+/* Training pairs: the surrogate's input features of n cells (f [n][ngam], w [n][nfreq],
+ * B [n], device) into d_x [n][ngam + nfreq + 1] and the scale max f max w into d_scale [n]. */
+int pbagpu_ssc_features(const pba_shock_params *shock, int n, const double *d_f, const double *d_w,
+                        const double *d_B, float *d_x, double *d_scale, void *stream);
+/* The plain SSC emissivity of n cells (the contraction of computeSSCSpectrum on the shock's
+ * grids and Compton table, FP64): d_j [n][nfreq]. */
+int pbagpu_ssc_emissivity(const pba_shock_params *shock, int n, const double *d_f, const double *d_w,
+                          double *d_j, void *stream);
+/* pbagpu_ele_evolve with the SSC emissivity from the surrogate (and the sampled check);
+ * d_aall must be NULL (no absorption); stats (host, may be NULL) receives the counts. */
+int pbagpu_ele_evolve_surrogate(const pba_run_params *params, int n_runs,
+                                const pba_population *pops, int n_pop,
+                                const double *d_table, int bw_total, int nt,
+                                double *d_jall, double *d_fall, int *d_status,
+                                const pba_ssc_surrogate *sur, pba_surrogate_stats *stats, void *stream);
+
 /* ------------------------------------------------- sky-map binning */
 /* Histogram image, flux centroid and data extent of raw sky maps: the binning part
  * of ProcessRawSkymap._process_file (package/src/PyBlastAfterglowMag/skymap_process.py:465-485),

@@ -403,3 +403,33 @@ extern "C" double orc_cyl_bessel_k(double nu, double x) {
     }
 }
 extern "C" double orc_synch_gsl(double B, double nu, double gam) { return orc::synch_gsl(B, nu, gam); }
+
+// The SSC contraction of computeSSCSpectrum (numeric_model.h:247-285) for given cells: electron
+// spectrum f [n][ng] and Compton photon weights w [n][nf] (w_k = n_k / nu_k (nu_{k+1} - nu_k), the
+// factor the contraction applies to the photon density), as evolve_population forms sscj.
+extern "C" int orc_ssc_emissivity(const pba_shock_params *sp, int n, const double *f, const double *w, double *out) {
+    if (!sp || n < 0) return PBAGPU_E_ARG;
+    const int ng = sp->ngam, nf = sp->nfreq;
+    orc::EleGrid g;
+    g.build(sp->gam2, sp->ngam);
+    const std::vector<double> nu = orc::freq_grid(*sp);
+    orc::SSCTab tab;
+    tab.build(nu, g.e);
+#pragma omp parallel for schedule(dynamic)
+    for (int c = 0; c < n; c++) {
+        const double *fc = f + (size_t)c * ng, *wc = w + (size_t)c * nf;
+        std::vector<double> scat(ng, 0.);
+        for (int i = 0; i < nf; i++) {
+            for (int j = 0; j < ng - 1; j++) {
+                double s = 0.;
+                const double *Kij = &tab.K[((size_t)i * ng + j) * nf];
+                for (int k = 0; k < nf - 1; k++) s += 3. / 4. * orc::sigmaT * orc::c / (g.e[j] * g.e[j]) * wc[k] * Kij[k];
+                scat[j] = s;
+            }
+            double acc_i = 0.;
+            for (int j = 0; j < ng - 1; j++) acc_i += fc[j] * orc::h * nu[i] * scat[j] * (g.e[j + 1] - g.e[j]);
+            out[(size_t)c * nf + i] = acc_i;
+        }
+    }
+    return PBAGPU_OK;
+}

@@ -139,3 +139,17 @@ def eats_skymap_pw(params, layers, table, jall, tobs, nuobs, remove_mu=False, aa
     if rc != 0:
         raise RuntimeError(f"orc_eats_skymap_pw rc={rc}")
     return image, flux, st
+
+
+def ssc_emissivity(shock, f, w):
+    """The SSC contraction (computeSSCSpectrum) of cells f [n][ngam], w [n][nfreq] on the host."""
+    L = lib()
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    out = np.zeros((f.shape[0], shock.nfreq))
+    L.orc_ssc_emissivity.argtypes = [C.POINTER(abi.ShockParams), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    rc = L.orc_ssc_emissivity(C.byref(shock), f.shape[0], f.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p),
+                              out.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise RuntimeError(f"orc_ssc_emissivity rc={rc}")
+    return out

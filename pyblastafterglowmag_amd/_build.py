@@ -51,7 +51,7 @@ def build_h5io():
     return out
 
 
-GPU_SOURCES = ["capi.hip", "dyn.hip", "ele.hip", "eats.hip", "eats_vec.hip", "eats_pw.hip", "skymap.hip"]
+GPU_SOURCES = ["capi.hip", "dyn.hip", "ele.hip", "eats.hip", "eats_vec.hip", "eats_pw.hip", "skymap.hip", "mlp.hip"]
 
 
 def build_gpu():

@@ -71,6 +71,26 @@ class Population(C.Structure):
     _fields_ = [("bw", C.c_int), ("run", C.c_int), ("shock", C.c_int), ("pad_", C.c_int)]
 
 
+PBA_MLP_MAXL = 8
+_FP = C.POINTER(C.c_float)
+
+
+class Mlp(C.Structure):
+    """pba_mlp: layer sizes and device pointers (FP32) of weights, biases, gradients and Adam moments."""
+    _fields_ = [("n_layers", C.c_int), ("dims", C.c_int * (PBA_MLP_MAXL + 1))] + \
+        [(n, C.c_void_p * PBA_MLP_MAXL) for n in ("w", "b", "gw", "gb", "mw", "vw", "mb", "vb")]
+
+
+class SscSurrogate(C.Structure):
+    _fields_ = [("net", Mlp), ("y_offset", C.c_void_p), ("y_scale", C.c_float), ("check_tol", C.c_float),
+                ("check_every", C.c_int), ("pad_", C.c_int)]
+
+
+class SurrogateStats(C.Structure):
+    _fields_ = [("inferred", C.c_longlong), ("checked", C.c_longlong), ("failed", C.c_longlong),
+                ("fallback", C.c_longlong), ("max_err", C.c_double)]
+
+
 class EatsLayer(C.Structure):
     _fields_ = [("theta_c_l", C.c_double), ("theta_c_h", C.c_double), ("bw", C.c_int), ("run", C.c_int),
                 ("pop_fs", C.c_int), ("pop_rs", C.c_int), ("i_end_r", C.c_int)]

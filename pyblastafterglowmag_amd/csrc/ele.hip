@@ -24,6 +24,7 @@
 #include "bessel.h"
 #include "timing.h"
 #include "runtime.h"
+#include "mlp_internal.h"
 #include <vector>
 
 namespace pbagpu {
@@ -426,14 +427,24 @@ __global__ void __launch_bounds__(64) ele_step_kernel(const PopDesc *__restrict_
 // workgroup sums its block of Lorentz-factor bins with four independent k-chains and writes a
 // partial sum, which ele_emit_kernel adds up in j-block order.
 constexpr int SSC_JB = 8;
+// Population q0 is checked against the plain contraction at step it (surrogate mode).
+__device__ __forceinline__ bool sur_sampled(int q0, int it, int every) {
+    return every > 0 && ((131LL * q0 + 37LL * it) % every) == 0;
+}
+
+// sel_fb: null (every population of the list) or, in surrogate mode, the per-population fallback
+// flags: then only populations sampled for the check at step sel_it or fallen back are computed.
 __global__ void __launch_bounds__(64) ssc_step_kernel(const int *__restrict__ plist, int npk, int n_pop,
                                                       const double *__restrict__ K, const double *__restrict__ e,
-                                                      const double *__restrict__ nu, int ng, int nf, PopState S) {
+                                                      const double *__restrict__ nu, int ng, int nf, PopState S,
+                                                      const int *__restrict__ sel_fb, int sel_it, int sel_every) {
     extern __shared__ double s_wl[];   // [nf][64] photon weights of the 64 populations
     const int lane = threadIdx.x;
     const int idx = blockIdx.x * 64 + lane;
     const int q0 = idx < npk ? plist[idx] : -1;
-    const bool act = q0 >= 0 && S.flag[3 * q0 + 1] != 0;
+    bool act = q0 >= 0 && S.flag[3 * q0 + 1] != 0;
+    if (sel_fb && act) act = sel_fb[q0] != 0 || sur_sampled(q0, sel_it, sel_every);
+    if (__ballot(act) == 0ull) return;   // the whole (single-wavefront) workgroup leaves together
     for (int k = 0; k < nf; k++) s_wl[k * 64 + lane] = act ? S.wt[(size_t)k * n_pop + q0] : 0.;
     __syncthreads();
     const double c0 = 3. / 4. * kSigmaT * kC;
@@ -593,6 +604,128 @@ __global__ void __launch_bounds__(64) ssc_mfma_kernel(const int *__restrict__ pl
     }
 }
 
+// ------------------------------------------------------------ learned SSC emissivity
+// Input features of one cell (one wavefront): f_j = fj(j), w_k = wk(k) for k < nf - 1 (the weights
+// the contraction takes; the last one is unused and given as 0), magnetic field B.
+template <class FA, class WA>
+__device__ void sur_features_row(FA fj, WA wk, int ng, int nf, double B, float *__restrict__ x,
+                                 double *__restrict__ scale) {
+    const int lane = threadIdx.x & 63;
+    double fm = 0., wm = 0.;
+    for (int j = lane; j < ng; j += 64) fm = fmax(fm, fj(j));
+    for (int k = lane; k < nf - 1; k += 64) wm = fmax(wm, wk(k));
+    for (int d = 32; d >= 1; d >>= 1) { fm = fmax(fm, __shfl_xor(fm, d, 64)); wm = fmax(wm, __shfl_xor(wm, d, 64)); }
+    for (int j = lane; j < ng; j += 64) {
+        const double v = fm > 0. ? fj(j) / fm : 0.;
+        x[j] = (float)((log10(fmax(v, 1e-30)) + 30.) / 30.);
+    }
+    for (int k = lane; k < nf; k += 64) {
+        const double v = (k < nf - 1 && wm > 0.) ? wk(k) / wm : 0.;
+        x[ng + k] = (float)((log10(fmax(v, 1e-30)) + 30.) / 30.);
+    }
+    if (lane == 0) {
+        x[ng + nf] = B > 0. ? (float)(log10(B) / 10.) : 0.f;
+        *scale = fm * wm;
+    }
+}
+
+__global__ void __launch_bounds__(64) ssc_features_kernel(const double *__restrict__ f, const double *__restrict__ w,
+                                                          const double *__restrict__ B, int ng, int nf,
+                                                          float *__restrict__ x, double *__restrict__ scale) {
+    const size_t c = blockIdx.x;
+    const int dx = ng + nf + 1;
+    sur_features_row([&](int j) { return f[c * ng + j]; }, [&](int k) { return w[c * nf + k]; }, ng, nf, B[c],
+                     x + c * dx, scale + c);
+}
+
+// features of the populations of one Compton table at step it, from the electron stage's state
+__global__ void __launch_bounds__(64) sur_feat_step_kernel(const PopDesc *__restrict__ pops,
+                                                           const int *__restrict__ plist, int npk, int n_pop, int it,
+                                                           int nt, int ng, int nf, PopState S, float *__restrict__ x,
+                                                           double *__restrict__ scale) {
+    const int q = blockIdx.x;
+    const int q0 = plist[q];
+    const PopDesc &D = pops[q0];
+    const double B = D.T[(size_t)(D.shock ? PV_B3 : PV_B) * nt + it];
+    sur_features_row([&](int j) { return S.ft[(size_t)j * n_pop + q0]; },
+                     [&](int k) { return S.wt[(size_t)k * n_pop + q0]; }, ng, nf, B,
+                     x + (size_t)q * (ng + nf + 1), scale + q);
+}
+
+// Surrogate statistics on the device: inferred, checked, failed, fallback, max error (bits)
+struct SurStats { unsigned long long n[5]; };
+
+// SSC emissivity of the populations of one Compton table at step it: the network's value, or the
+// plain contraction's (already in S.jssc) for populations that failed a check; the sampled
+// populations are compared with the plain contraction first.
+__global__ void __launch_bounds__(64) sur_apply_kernel(const int *__restrict__ plist, int npk, int n_pop, int it,
+                                                       int nf, const float *__restrict__ y,
+                                                       const float *__restrict__ y_off, float y_scale,
+                                                       const double *__restrict__ scale, PopState S,
+                                                       int *__restrict__ fb, int every, double tol,
+                                                       SurStats *__restrict__ st) {
+    const int q = blockIdx.x, lane = threadIdx.x;
+    const int q0 = plist[q];
+    if (!S.flag[3 * q0 + 1]) return;
+    double *J = S.jssc + (size_t)q0 * nf * SSC_JB;
+    if (fb[q0]) {   // computed by the plain contraction this step
+        if (lane == 0) atomicAdd(&st->n[3], 1ull);
+        return;
+    }
+    const bool chk = sur_sampled(q0, it, every);
+    const double sc = scale[q];
+    double err = 0., peak = 0.;
+    if (chk) {
+        for (int i = lane; i < nf; i += 64) {
+            double ex = 0.;
+            for (int jb = 0; jb < SSC_JB; jb++) ex += J[(size_t)i * SSC_JB + jb];
+            peak = fmax(peak, fabs(ex));
+        }
+        for (int d = 32; d >= 1; d >>= 1) peak = fmax(peak, __shfl_xor(peak, d, 64));
+        for (int i = lane; i < nf; i += 64) {
+            double ex = 0.;
+            for (int jb = 0; jb < SSC_JB; jb++) ex += J[(size_t)i * SSC_JB + jb];
+            const double sv = sc > 0. ? sc * exp10((double)y_off[i] + (double)y_scale * (double)y[(size_t)q * nf + i]) : 0.;
+            if (fabs(ex) > 1e-6 * peak) err = fmax(err, fabs(sv - ex) / fabs(ex));
+        }
+        for (int d = 32; d >= 1; d >>= 1) err = fmax(err, __shfl_xor(err, d, 64));
+        if (lane == 0) {
+            atomicAdd(&st->n[1], 1ull);
+            atomicMax(&st->n[4], (unsigned long long)__double_as_longlong(err));
+        }
+        if (err > tol) {   // keep the plain value and use the plain contraction from now on
+            if (lane == 0) { fb[q0] = 1; atomicAdd(&st->n[2], 1ull); }
+            return;
+        }
+    }
+    if (lane == 0) atomicAdd(&st->n[0], 1ull);
+    for (int i = lane; i < nf; i += 64) {
+        const double sv = sc > 0. ? sc * exp10((double)y_off[i] + (double)y_scale * (double)y[(size_t)q * nf + i]) : 0.;
+        J[(size_t)i * SSC_JB] = sv;
+        for (int jb = 1; jb < SSC_JB; jb++) J[(size_t)i * SSC_JB + jb] = 0.;
+    }
+}
+
+__global__ void transpose_kernel(const double *__restrict__ a, int rows, int cols, double *__restrict__ at) {
+    const size_t n = (size_t)rows * cols;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = i / cols, c = i % cols;
+        at[c * rows + r] = a[i];
+    }
+}
+
+__global__ void sum_jb_kernel(const double *__restrict__ jssc, size_t n, double *__restrict__ out) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double v = 0.;
+        for (int jb = 0; jb < SSC_JB; jb++) v += jssc[i * SSC_JB + jb];
+        out[i] = v;
+    }
+}
+
+__global__ void fill_int_kernel(int *__restrict__ a, int n, int v, int stride_val) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = stride_val ? i : v;
+}
+
 __global__ void ele_status_kernel(const int *__restrict__ flag, int n_pop, int *__restrict__ status) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n_pop) status[q] = flag[3 * q + 2];
@@ -651,13 +784,15 @@ static void build_grids(const pba_shock_params &s, GridSet &G) {
 }  // namespace ele
 }  // namespace pbagpu
 
-extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const pba_population *pops, int n_pop,
-                                 const double *d_table, int bw_total, int nt, double *d_jall, double *d_aall,
-                                 double *d_fall, int *d_status, int precision, void *stream) {
+// precision 0: FP64; 1: SSC contraction in BF16 hi/lo on MFMA; 2: SSC emissivity from the surrogate
+static int ele_evolve_impl(const pba_run_params *params, int n_runs, const pba_population *pops, int n_pop,
+                           const double *d_table, int bw_total, int nt, double *d_jall, double *d_aall,
+                           double *d_fall, int *d_status, int precision, const pba_ssc_surrogate *sur,
+                           pba_surrogate_stats *stats, void *stream) {
     using namespace pbagpu::ele;
     if (!params || !pops || !d_table || !d_jall || !d_status || n_pop < 1 || nt < 1 || n_runs < 1) return PBAGPU_E_ARG;
     if (pops[0].run < 0 || pops[0].run >= n_runs) return PBAGPU_E_ARG;
-    if (precision != 0 && precision != 1) return PBAGPU_E_UNSUPPORTED;
+    if (precision < 0 || precision > 2 || (precision == 2) != (sur != nullptr)) return PBAGPU_E_UNSUPPORTED;
     hipStream_t s = (hipStream_t)stream;
     const pba_shock_params &s0 = pops[0].shock ? params[pops[0].run].rs : params[pops[0].run].fs;
     const int ng = s0.ngam, nf = s0.nfreq;
@@ -796,6 +931,36 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
         d_an = (float *)(d_wmax + nl);
         d_wn = d_an + nl * ng;
     }
+    // surrogate: features, outputs and scales of the listed populations, fallback flags, counters
+    const int dx = ng + nf + 1;
+    float *d_sx = nullptr, *d_sy = nullptr;
+    double *d_sscale = nullptr;
+    int *d_fb = nullptr;
+    SurStats *d_st = nullptr;
+    if (precision == 2) {
+        if (d_aall) {
+            pbagpu::set_error_msg("ele_evolve_surrogate: synchrotron self-absorption is not implemented with the surrogate");
+            return PBAGPU_E_UNSUPPORTED;
+        }
+        const pba_mlp &net = sur->net;
+        if (pbagpu::mlp::check_network(&net) != PBAGPU_OK || net.dims[0] != dx || net.dims[net.n_layers] != nf ||
+            !sur->y_offset || sur->check_every < 1 || !(sur->check_tol > 0.f)) {
+            pbagpu::set_error_msg("ele_evolve_surrogate: network dims must be [ngam + nfreq + 1 ... nfreq]; y_offset, "
+                                  "check_every >= 1 and check_tol > 0 are needed");
+            return PBAGPU_E_ARG;
+        }
+        const size_t nl = std::max<size_t>(hlist.size(), 1);
+        char *sw = (char *)pbagpu::workspace(pbagpu::WS_SUR, nl * (sizeof(float) * (dx + nf) + sizeof(double)) +
+                                                                 sizeof(int) * np + sizeof(SurStats) + 1024, s);
+        if (!sw) return PBAGPU_E_HIP;
+        d_st = (SurStats *)sw;
+        d_sscale = (double *)(sw + 256);
+        d_sx = (float *)(d_sscale + nl);
+        d_sy = d_sx + nl * dx;
+        d_fb = (int *)(d_sy + nl * nf);
+        PBA_TRY(rc, hipMemsetAsync(d_st, 0, sizeof(SurStats), s));
+        PBA_TRY(rc, hipMemsetAsync(d_fb, 0, sizeof(int) * np, s));
+    }
     PopState S;
     S.f = wsd; S.ft = S.f + np * ng; S.tf = S.ft + np * ng; S.wt = S.tf + np * nf;
     S.jsyn = S.wt + np * nf; S.jssc = S.jsyn + np * nf; S.scal = S.jssc + np * nf * SSC_JB;
@@ -828,9 +993,23 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
                                        dim3(64), 0, s, dlist + lo, npk, nu, ng, nf, kp, dKb + 2 * kbf * t,
                                        dKb + 2 * kbf * t + kbf, d_an + lo * ng, d_amax + lo * SSC_JB, d_wn + lo * kp,
                                        d_wmax + lo, S);
+                } else if (precision == 2) {
+                    const size_t lo = list_off[t];
+                    hipLaunchKernelGGL(sur_feat_step_kernel, dim3((unsigned)npk), dim3(64), 0, s, dpops, dlist + lo, npk,
+                                       n_pop, it, nt, ng, nf, S, d_sx, d_sscale);
+                    int rm = pbagpu::mlp::forward_into(sur->net, npk, d_sx, d_sy, pbagpu::WS_MLP_ELE, s);
+                    if (rm != PBAGPU_OK) { rc = rm; break; }
+                    // the plain contraction for the sampled and the fallen-back populations
+                    hipLaunchKernelGGL(ssc_step_kernel, dim3((unsigned)((npk + 63) / 64), (unsigned)nf, (unsigned)SSC_JB),
+                                       dim3(64), ssc_lds, s, dlist + lo, npk, n_pop, K, e, nu, ng, nf, S,
+                                       (const int *)d_fb, it, sur->check_every);
+                    hipLaunchKernelGGL(sur_apply_kernel, dim3((unsigned)npk), dim3(64), 0, s, dlist + lo, npk, n_pop, it,
+                                       nf, d_sy, sur->y_offset, sur->y_scale, d_sscale, S, d_fb, sur->check_every,
+                                       (double)sur->check_tol, d_st);
                 } else {
                     hipLaunchKernelGGL(ssc_step_kernel, dim3((unsigned)((npk + 63) / 64), (unsigned)nf, (unsigned)SSC_JB),
-                                       dim3(64), ssc_lds, s, dlist + list_off[t], npk, n_pop, K, e, nu, ng, nf, S);
+                                       dim3(64), ssc_lds, s, dlist + list_off[t], npk, n_pop, K, e, nu, ng, nf, S,
+                                       (const int *)nullptr, 0, 0);
                 }
             }
             hipLaunchKernelGGL(ele_emit_kernel, dim3(n_pop), dim3(64), 0, s, dpops, it, nt, S);
@@ -843,9 +1022,112 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
                                d_status);
         PBA_TRY(rc, hipGetLastError());
     }
+    SurStats hst{};
+    if (d_st && rc == PBAGPU_OK) PBA_TRY(rc, hipMemcpyAsync(&hst, d_st, sizeof(SurStats), hipMemcpyDeviceToHost, s));
     PBA_TRY(rc, hipStreamSynchronize(s));
     ttab.collect();
     tele.collect();
+    if (stats) {
+        stats->inferred = (long long)hst.n[0]; stats->checked = (long long)hst.n[1];
+        stats->failed = (long long)hst.n[2]; stats->fallback = (long long)hst.n[3];
+        stats->max_err = __builtin_bit_cast(double, hst.n[4]);
+    }
+    return rc;
+}
+
+extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const pba_population *pops, int n_pop,
+                                 const double *d_table, int bw_total, int nt, double *d_jall, double *d_aall,
+                                 double *d_fall, int *d_status, int precision, void *stream) {
+    if (precision != 0 && precision != 1) return PBAGPU_E_UNSUPPORTED;
+    return ele_evolve_impl(params, n_runs, pops, n_pop, d_table, bw_total, nt, d_jall, d_aall, d_fall, d_status,
+                           precision, nullptr, nullptr, stream);
+}
+
+extern "C" int pbagpu_ele_evolve_surrogate(const pba_run_params *params, int n_runs, const pba_population *pops,
+                                           int n_pop, const double *d_table, int bw_total, int nt, double *d_jall,
+                                           double *d_fall, int *d_status, const pba_ssc_surrogate *sur,
+                                           pba_surrogate_stats *stats, void *stream) {
+    if (!sur) return PBAGPU_E_ARG;
+    return ele_evolve_impl(params, n_runs, pops, n_pop, d_table, bw_total, nt, d_jall, nullptr, d_fall, d_status, 2,
+                           sur, stats, stream);
+}
+
+// grids and Compton table of one shock, for the batched entry points below
+static int shock_tables(const pba_shock_params &sh, hipStream_t s, const double **e, const double **nu,
+                        const double **K) {
+    using namespace pbagpu::ele;
+    const int ng = sh.ngam, nf = sh.nfreq;
+    GridSet G;
+    build_grids(sh, G);
+    const size_t ktab = (size_t)nf * ng * nf;
+    double *d = (double *)pbagpu::workspace(pbagpu::WS_SSCX, sizeof(double) * (ng + nf + ktab + (size_t)ng * nf), s);
+    if (!d) return PBAGPU_E_HIP;
+    int rc = PBAGPU_OK;
+    PBA_TRY(rc, hipMemcpyAsync(d, G.e.data(), sizeof(double) * ng, hipMemcpyHostToDevice, s));
+    PBA_TRY(rc, hipMemcpyAsync(d + ng, G.nu.data(), sizeof(double) * nf, hipMemcpyHostToDevice, s));
+    double *dK = d + ng + nf;
+    const double h_by_mec2 = pbagpu::kH / pbagpu::kMe / pbagpu::kC / pbagpu::kC;
+    hipLaunchKernelGGL(ssc_table_kernel, dim3((unsigned)((ktab + 255) / 256)), dim3(256), 0, s, d + ng, d, nf, ng,
+                       h_by_mec2, dK);
+    PBA_TRY(rc, hipGetLastError());
+    *e = d; *nu = d + ng; *K = dK;
+    return rc;
+}
+
+extern "C" int pbagpu_ssc_features(const pba_shock_params *shock, int n, const double *d_f, const double *d_w,
+                                   const double *d_B, float *d_x, double *d_scale, void *stream) {
+    using namespace pbagpu::ele;
+    if (!shock || n < 0 || (n > 0 && (!d_f || !d_w || !d_B || !d_x || !d_scale))) return PBAGPU_E_ARG;
+    if (shock->ngam < 2 || shock->nfreq < 2) return PBAGPU_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    int rc = PBAGPU_OK;
+    if (n > 0) {
+        hipLaunchKernelGGL(ssc_features_kernel, dim3((unsigned)n), dim3(64), 0, s, d_f, d_w, d_B, shock->ngam,
+                           shock->nfreq, d_x, d_scale);
+        PBA_TRY(rc, hipGetLastError());
+    }
+    PBA_TRY(rc, hipStreamSynchronize(s));
+    return rc;
+}
+
+extern "C" int pbagpu_ssc_emissivity(const pba_shock_params *shock, int n, const double *d_f, const double *d_w,
+                                     double *d_j, void *stream) {
+    using namespace pbagpu::ele;
+    if (!shock || n < 0 || (n > 0 && (!d_f || !d_w || !d_j))) return PBAGPU_E_ARG;
+    const int ng = shock->ngam, nf = shock->nfreq;
+    if (ng < 2 || ng > MAXG || nf < 2 || nf > MAXF) return PBAGPU_E_ARG;
+    if (n == 0) return PBAGPU_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const double *e, *nu, *K;
+    int rc = shock_tables(*shock, s, &e, &nu, &K);
+    if (rc != PBAGPU_OK) return rc;
+    // cells go through the contraction in chunks, laid out as the electron stage's state
+    const int CH = std::min(n, 16384);
+    const size_t per = (size_t)CH * (ng + nf + (size_t)nf * SSC_JB);
+    char *w = (char *)pbagpu::workspace(pbagpu::WS_SUR, sizeof(double) * per + sizeof(int) * 4 * (size_t)CH + 512, s);
+    if (!w) return PBAGPU_E_HIP;
+    PopState S{};
+    S.ft = (double *)w;
+    S.wt = S.ft + (size_t)CH * ng;
+    S.jssc = S.wt + (size_t)CH * nf;
+    S.flag = (int *)(S.jssc + (size_t)CH * nf * SSC_JB);
+    int *plist = S.flag + 3 * (size_t)CH;
+    hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)((3 * CH + 255) / 256)), dim3(256), 0, s, S.flag, 3 * CH, 1, 0);
+    hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)((CH + 255) / 256)), dim3(256), 0, s, plist, CH, 0, 1);
+    const size_t ssc_lds = sizeof(double) * (size_t)nf * 64;
+    if (ssc_lds > 64 * 1024)
+        (void)hipFuncSetAttribute((const void *)ssc_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)ssc_lds);
+    for (int c0 = 0; c0 < n && rc == PBAGPU_OK; c0 += CH) {
+        const int m = std::min(CH, n - c0);
+        hipLaunchKernelGGL(transpose_kernel, dim3(1024), dim3(256), 0, s, d_f + (size_t)c0 * ng, m, ng, S.ft);
+        hipLaunchKernelGGL(transpose_kernel, dim3(1024), dim3(256), 0, s, d_w + (size_t)c0 * nf, m, nf, S.wt);
+        hipLaunchKernelGGL(ssc_step_kernel, dim3((unsigned)((m + 63) / 64), (unsigned)nf, (unsigned)SSC_JB), dim3(64),
+                           ssc_lds, s, plist, m, m, K, e, nu, ng, nf, S, (const int *)nullptr, 0, 0);
+        hipLaunchKernelGGL(sum_jb_kernel, dim3(1024), dim3(256), 0, s, S.jssc, (size_t)m * nf, d_j + (size_t)c0 * nf);
+        PBA_TRY(rc, hipGetLastError());
+    }
+    PBA_TRY(rc, hipStreamSynchronize(s));
     return rc;
 }
 

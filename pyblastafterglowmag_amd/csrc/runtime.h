@@ -19,6 +19,7 @@ enum WsSlot {
     WS_PW_NU, WS_PW_LAYERS, WS_PW_SAMPLES,
     WS_SKY_SEG, WS_SKY_OFF, WS_SKY_PART,
     WS_VEC_NU, WS_VEC_LAYERS, WS_VEC_SAMPLES,
+    WS_MLP, WS_MLP_ELE, WS_SUR, WS_SSCX,
     WS_N
 };
 

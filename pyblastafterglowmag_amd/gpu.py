@@ -18,7 +18,8 @@ EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu
            "pbagpu_timing_read", "pbagpu_timing_units", "pbagpu_dyn_evolve", "pbagpu_ele_evolve", "pbagpu_eats_lightcurve",
            "pbagpu_eats_lightcurve_pw", "pbagpu_eats_skymap_pw", "pbagpu_skymap_hist", "pbagpu_bessel_k_thirds",
            "pbagpu_release_stream", "pbagpu_stream_create", "pbagpu_stream_destroy",
-           "pbagpu_synchrotron_f", "pbagpu_eats_lightcurve_vec")
+           "pbagpu_synchrotron_f", "pbagpu_eats_lightcurve_vec", "pbagpu_mlp_forward", "pbagpu_mlp_train_step",
+           "pbagpu_gemm_bf16", "pbagpu_ssc_features", "pbagpu_ssc_emissivity", "pbagpu_ele_evolve_surrogate")
 STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW, STAGE_SKY_HIST, STAGE_EATS_VEC = 0, 1, 2, 3, 4, 5, 6
 N_STAGES = 7
 # sky-map binning (enum pba_sky_moment / pba_sky_status)
@@ -58,6 +59,14 @@ def lib():
         L.pbagpu_bessel_k_thirds.argtypes = [C.c_int, vp, vp, vp]
         L.pbagpu_release_stream.argtypes = [vp]
         L.pbagpu_synchrotron_f.argtypes = [C.c_int, vp, vp]
+        L.pbagpu_mlp_forward.argtypes = [C.POINTER(abi.Mlp), C.c_int, vp, vp, vp]
+        L.pbagpu_mlp_train_step.argtypes = [C.POINTER(abi.Mlp), C.c_int, vp, vp, C.c_float, C.c_int, vp, vp]
+        L.pbagpu_gemm_bf16.argtypes = [C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp]
+        L.pbagpu_ssc_features.argtypes = [C.POINTER(abi.ShockParams), C.c_int, vp, vp, vp, vp, vp, vp]
+        L.pbagpu_ssc_emissivity.argtypes = [C.POINTER(abi.ShockParams), C.c_int, vp, vp, vp, vp]
+        L.pbagpu_ele_evolve_surrogate.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.Population),
+                                                  C.c_int, vp, C.c_int, C.c_int, vp, vp, vp,
+                                                  C.POINTER(abi.SscSurrogate), C.POINTER(abi.SurrogateStats), vp]
         L.pbagpu_eats_lightcurve_vec.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.EatsLayer), C.c_int,
                                                  vp, C.c_int, vp, C.c_int, C.c_int, vp, C.c_int, vp, C.c_double, vp, vp,
                                                  vp]
@@ -352,6 +361,21 @@ class Pipeline:
                                         self.n_runs * self.bpr, self.nt, _ptr(self.d_jall), _ptr(self.d_aall),
                                         _ptr(self.d_fall),
                                         _ptr(self.d_est), self.precision, self.rsp), "ele_evolve")
+
+    def electrons_surrogate(self, sur, want_f=False):
+        """Electron stage with the SSC emissivity from a trained surrogate (surrogate.SscSurrogate),
+        checked against the plain contraction on a sampled subset every step.  Returns the counts
+        (abi.SurrogateStats)."""
+        npop = len(self.pops)
+        self.d_fall = self.torch.empty((npop, self.nt, self.runs[0].params.fs.ngam), dtype=self.torch.float64,
+                                       device=self.dev) if want_f else None
+        st = abi.SurrogateStats()
+        spec = sur.spec()
+        _check(self.L.pbagpu_ele_evolve_surrogate(self.params, self.n_runs, self.parr, npop, _ptr(self.d_table),
+                                                  self.n_runs * self.bpr, self.nt, _ptr(self.d_jall), _ptr(self.d_fall),
+                                                  _ptr(self.d_est), C.byref(spec), C.byref(st), self.rsp),
+               "ele_evolve_surrogate")
+        return st
 
     def observer(self):
         if self.method == "piece-wise":
