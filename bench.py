@@ -6,15 +6,21 @@ shocks, 200 electron Lorentz-factor bins x 96 comoving frequencies, 1000 burster
 light curve at 96 nu_obs x 128 t_obs (theta_obs = 0.363 rad, n_ism = 10^-1.6 cm^-3,
 E_iso,c = 10^54.1 erg, Gamma0 = 300, theta_c = 0.061, theta_w = 0.436).
 
-One step = one batch of independent runs of that workload through the three GPU stages, with
-every input and intermediate resident in device memory.  The dynamics of one run is a chain of
-~3e5 dependent ODE steps on one wavefront (tens of seconds of latency at a small share of the
-chip), while the electron and observer stages of a batch fill the whole chip, so batches are
-pipelined: step s launches the dynamics of batch s on its own stream (host thread) and runs the
-electron and observer stages of batch s - depth, whose dynamics are done, on the shared
-radiation stream.  Only light curves whose dynamics, electrons and observer stage all ran inside
-the timed steps are counted: of K timed steps, K - depth batches;
-value = batch x (K - depth) / timed seconds.
+Two legs run one after the other, each a pipeline of batches of independent runs of that workload
+with every input and intermediate resident in device memory:
+  mixed (the value): SSC contraction in BF16 hi/lo on MFMA with FP32 accumulation and the
+        frequency-vectorised observer stage (one adaptive Gauss-Kronrod tree per (layer, t_obs)
+        shared by all 96 frequencies, rtol 1e-2); the dynamics and the Chang-Cooper update FP64;
+  fp64: every stage FP64, the observer stage as the reference's per-pair CADRE quadrature.
+The dynamics of one run is a chain of ~3e5 dependent ODE steps on one wavefront (tens of seconds
+of latency on a small share of the chip), while the electron and observer stages of a batch fill
+the chip, so step s launches the dynamics of batch s on its own stream (compute units
+[0, dyn_cus)) and runs the electron and observer stages of batch s - depth, whose dynamics are
+done, on the radiation stream (the other units).  Only light curves whose dynamics, electrons and
+observer stage all ran inside the timed steps are counted: of K timed steps, K - depth batches;
+value = batch x (K - depth) / timed seconds.  After the mixed leg, the FP64 electron and observer
+stages are run on 4 of its runs, and the largest relative difference of their layer-summed light
+curves is reported (the mixed path's budget is 1 %).
 
 The JSON line carries a roofline entry for the kernel with the most device time in the timed
 steps (HIP events recorded by the library on the stream each kernel runs on) and a CPU baseline:
@@ -42,19 +48,30 @@ PEAK_HBM_GBS = 8000.0
 FLOPS_PER_RHS = {0: 250.0, 1: 330.0}   # one right-hand side of one blastwave (forward shock / with reverse shock)
 BYTES_PER_SAMPLE = 144.0               # one EATS integrand sample: 7 state columns at 2 steps + 4 emissivities
 FLOPS_PER_CELL = 152.0                 # one piece-wise cell flux, forward-shock term (reverse-shock term not counted)
+# one node of the vectorised observer: 7 state columns at the 2 bracketing steps; per frequency at
+# that node, 4 emissivities of the bilinear lookup
+BYTES_PER_NODE = 112.0
+BYTES_PER_NU = 32.0
 
 
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--batch", type=int, default=12, help="independent runs per step")
+    ap.add_argument("--steps", type=int, default=8, help="timed steps of the mixed leg")
+    ap.add_argument("--warmup", type=int, default=4, help="warm-up steps of the mixed leg (>= depth fills it)")
+    ap.add_argument("--batch", type=int, default=128, help="independent runs per step, mixed leg")
     ap.add_argument("--depth", type=int, default=4,
                     help="pipeline depth: batches whose dynamics run while the radiation stages of an older batch run")
-    ap.add_argument("--dyn-cus", type=int, default=32,
+    ap.add_argument("--dyn-cus", type=int, default=128,
                     help="compute units reserved for the dynamics streams (the radiation stream gets the others)")
-    ap.add_argument("--precision", type=int, default=0, help="0: FP64 everywhere")
+    ap.add_argument("--vec-rtol", type=float, default=1e-2, help="quadrature tolerance of the vectorised observer")
+    ap.add_argument("--mode", default="both", choices=["both", "mixed", "fp64"])
+    ap.add_argument("--fp64-batch", type=int, default=12)
+    ap.add_argument("--fp64-depth", type=int, default=2)
+    ap.add_argument("--fp64-steps", type=int, default=4)
+    ap.add_argument("--fp64-warmup", type=int, default=2)
+    ap.add_argument("--fp64-dyn-cus", type=int, default=32)
+    ap.add_argument("--precision", type=int, default=1, help="SSC contraction of the mixed leg: 1 BF16 hi/lo, 0 FP64")
     ap.add_argument("--workload", default="gauss21_fsrs_ssc",
                     choices=["gauss21_fsrs_ssc", "tophat_fs_syn", "tophat64_pw", "lhs_members"],
                     help="lhs_members: every run a different member of the 1024-member Latin-hypercube batch")
@@ -145,9 +162,10 @@ def roofline(stage_ms, pls, gpu, dom=None):
     import numpy as np
     from pyblastafterglowmag_amd import abi
     names = {gpu.STAGE_DYN: "dyn_kernel", gpu.STAGE_ELE: "ele (per-step kernels)", gpu.STAGE_EATS: "eats_kernel",
-             gpu.STAGE_SSC_TABLE: "ssc_table", gpu.STAGE_EATS_PW: "eats_pw_kernel"}
+             gpu.STAGE_SSC_TABLE: "ssc_table", gpu.STAGE_EATS_PW: "eats_pw_kernel",
+             gpu.STAGE_EATS_VEC: "eats_vec_kernel"}
     if dom is None:
-        crit = [gpu.STAGE_ELE, gpu.STAGE_EATS, gpu.STAGE_EATS_PW]
+        crit = [gpu.STAGE_ELE, gpu.STAGE_EATS, gpu.STAGE_EATS_PW, gpu.STAGE_EATS_VEC]
         dom = max(crit, key=lambda s: stage_ms[s][0])
     ms, launches = stage_ms[dom]
     avg_s = ms / max(launches, 1) / 1e3
@@ -158,6 +176,15 @@ def roofline(stage_ms, pls, gpu, dom=None):
         ach = samples * BYTES_PER_SAMPLE / avg_s / 1e9
         roof.update(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
                     samples_per_launch=samples, bytes_per_sample=BYTES_PER_SAMPLE)
+    elif dom == gpu.STAGE_EATS_VEC:
+        nodes = gpu.timing_units(gpu.STAGE_EATS_VEC) / max(launches, 1)
+        r0 = pls[0].runs[0]
+        nnu = int(np.count_nonzero(r0.tobs == r0.tobs[0]))   # the pairs are t-major
+        byts = nodes * (BYTES_PER_NODE + nnu * BYTES_PER_NU)
+        ach = byts / avg_s / 1e9
+        roof.update(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
+                    nodes_per_launch=nodes, bytes_per_node=BYTES_PER_NODE, bytes_per_node_frequency=BYTES_PER_NU,
+                    frequencies=nnu)
     elif dom == gpu.STAGE_EATS_PW:
         pl = pls[0]
         cells = gpu.timing_units(gpu.STAGE_EATS_PW) / max(launches, 1)
@@ -197,6 +224,106 @@ def roofline(stage_ms, pls, gpu, dom=None):
     return roof
 
 
+def run_leg(name, args, gpu, torch, make, batch, depth, warmup, steps, dyn_cus, precision, vec):
+    """One pipelined leg; returns its measurements and the ring of pipelines (for the roofline)."""
+    D = max(depth, 1)
+    n_cu = gpu.device_cu_count()
+    dyn_cus = min(max(dyn_cus, 1), n_cu - 1)
+    rad_stream = gpu.CuStream(dyn_cus, n_cu - dyn_cus)
+    dyn_stream = [gpu.CuStream(0, dyn_cus) for _ in range(D + 1)]
+    ring = [gpu.Pipeline(make(k, batch), stream=dyn_stream[k], precision=precision, rad_stream=rad_stream)
+            for k in range(D + 1)]
+    threads, rejected = {}, {}
+    last_b = warmup + steps - 1 - D   # the last batch whose radiation stages run
+
+    def launch(b):
+        pl = ring[b % (D + 1)]
+        if args.workload == "lhs_members" and b >= D + 1:
+            pl.__init__(make(b, batch), stream=pl.stream, precision=precision, rad_stream=rad_stream)
+        th = threading.Thread(target=pl.dynamics)
+        th.start()
+        threads[b] = th
+
+    def radiation(b):
+        pl = ring[b % (D + 1)]
+        threads.pop(b).join()
+        pl.electrons()
+        if vec and pl.method != "piece-wise":
+            pl.observer_vec(args.vec_rtol)
+        else:
+            pl.observer()
+        dst, est, lst = pl.status()
+        if dst.any() or est.any() or lst.any():
+            bad = rejected_runs(pl, dst, est, lst)
+            if args.workload != "lhs_members":
+                print(f"bench.py: non-zero status dyn={sorted(set(dst.tolist()))} ele={sorted(set(est.tolist()))} "
+                      f"eats={sorted(set(lst.tolist()))}", file=sys.stderr)
+                sys.exit(3)
+            # a member the reference would stop on (its status bits mark where) yields no light curve
+            rejected[b] = len(bad)
+
+    t_fill = time.perf_counter()
+
+    def step(s):
+        if s <= last_b:
+            launch(s)
+        if s >= D:
+            radiation(s - D)
+        print(f"[bench:{name}] step {s} done ({'dynamics of batch %d launched, ' % s if s <= last_b else ''}"
+              f"{'light curves of batch %d' % (s - D) if s >= D else 'filling the pipeline'}) "
+              f"t={time.perf_counter() - t_fill:.1f}s", file=sys.stderr, flush=True)
+
+    for s in range(warmup):
+        step(s)
+    torch.cuda.synchronize()
+    for st in range(gpu.N_STAGES):
+        gpu.timing_read(st, reset=True)
+        gpu.timing_units(st, reset=True)
+    t0 = time.perf_counter()
+    for s in range(warmup, warmup + steps):
+        step(s)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    for th in threads.values():
+        th.join()
+    stage_ms = {s: gpu.timing_read(s) for s in range(gpu.N_STAGES)}
+    counted = list(range(warmup, last_b + 1))
+    n_rej = sum(rejected.get(b, 0) for b in counted)
+    lcs = batch * len(counted) - n_rej
+    pls = [ring[counted[0] % (D + 1)]] if counted else ring[:1]
+    out = {"value": lcs / dt, "ms_per_step": dt / steps * 1e3, "steps": steps, "warmup": warmup, "runs_per_step": batch,
+           "pipeline_depth": D, "dynamics_cus": dyn_cus, "radiation_cus": n_cu - dyn_cus,
+           "batches_counted": len(counted), "light_curves_counted": lcs,
+           "roofline": roofline(stage_ms, pls, gpu),
+           "roofline_dynamics": roofline(stage_ms, pls, gpu, dom=gpu.STAGE_DYN) if stage_ms[gpu.STAGE_DYN][1] else None}
+    if args.workload == "lhs_members":
+        out["members_flagged"] = n_rej
+    if args.stages:
+        for st_, (ms, n) in stage_ms.items():
+            print(f"[stage:{name}] {st_}: {ms:.1f} ms over {n} launches", file=sys.stderr)
+    return out, ring, [rad_stream] + dyn_stream
+
+
+def mixed_error(pl, n_check=4):
+    """Largest relative difference of the layer-summed light curves of the first n_check runs of a
+    pipeline between its (mixed) result and the FP64 electron and observer stages on the same tables."""
+    import numpy as np
+    mix = pl.flux()[:n_check].sum(axis=1)
+    prec = pl.precision
+    pl.precision = 0
+    pl.electrons()
+    nl = n_check * pl.bpr
+    pl.observer(first_layers=nl)
+    pl.precision = prec
+    ref = pl.flux()[:n_check].sum(axis=1)
+    worst = 0.0
+    for a, b in zip(mix, ref):
+        m = np.abs(b) > 1e-12 * np.abs(b).max()
+        if m.any():
+            worst = max(worst, float(np.max(np.abs(a[m] - b[m]) / np.abs(b[m]))))
+    return worst
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -215,109 +342,51 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    D = max(args.depth, 1)
     if args.workload == "lhs_members":     # batch b holds members b*batch ... (b+1)*batch - 1
-        make = lambda b: [gpu.Run.from_workload(*_workload(workloads, args.workload, b * args.batch + i))
-                          for i in range(args.batch)]
+        make = lambda b, n: [gpu.Run.from_workload(*_workload(workloads, args.workload, b * n + i)) for i in range(n)]
     else:
         one = gpu.Run.from_workload(*_workload(workloads, args.workload, 0))
-        make = lambda b: [one] * args.batch
-    run = make(0)[0]
-    # D + 1 pipelines in a ring: batch b uses ring[b % (D + 1)]; its dynamics run on that pipeline's own
-    # stream, its electron and observer stages on the shared radiation stream
-    # the dynamics run on compute units [0, dyn_cus) and the electron and observer stages on the rest, so
-    # that the persistent observer kernel, which holds its units until it ends, never keeps a batch's
-    # dynamics from starting
-    n_cu = gpu.device_cu_count()
-    dyn_cus = min(max(args.dyn_cus, 1), n_cu - 1)
-    rad_stream = gpu.CuStream(dyn_cus, n_cu - dyn_cus)
-    ring = [gpu.Pipeline(make(k), stream=gpu.CuStream(0, dyn_cus), precision=args.precision, rad_stream=rad_stream)
-            for k in range(D + 1)]
+        make = lambda b, n: [one] * n
+    run = make(0, 1)[0]
     gpu.timing_enable(True)
-    threads = {}
-    rejected = {}
-    t_start = {}
-
-    def launch(b):
-        """Dynamics of batch b on its pipeline's stream, in a host thread (the call blocks until done)."""
-        pl = ring[b % (D + 1)]
-        if args.workload == "lhs_members" and b >= D + 1:
-            pl.__init__(make(b), stream=pl.stream, precision=args.precision, rad_stream=rad_stream)
-        t_start[b] = time.perf_counter()
-        th = threading.Thread(target=pl.dynamics)
-        th.start()
-        threads[b] = th
-
-    def radiation(b):
-        pl = ring[b % (D + 1)]
-        threads.pop(b).join()
-        pl.electrons()
-        pl.observer()
-        dst, est, lst = pl.status()
-        if dst.any() or est.any() or lst.any():
-            bad = rejected_runs(pl, dst, est, lst)
-            if args.workload != "lhs_members":
-                print(f"bench.py: non-zero status dyn={sorted(set(dst.tolist()))} ele={sorted(set(est.tolist()))} "
-                      f"eats={sorted(set(lst.tolist()))}", file=sys.stderr)
-                sys.exit(3)
-            # a member the reference would stop on (its status bits mark where) yields no light curve
-            rejected[b] = len(bad)
-            print(f"[bench] batch {b}: {len(bad)} of {pl.n_runs} members flagged (no light curve): {sorted(bad)}",
-                  file=sys.stderr, flush=True)
-
-    last_b = args.warmup + args.steps - 1 - D   # the last batch whose radiation stages run
-
-    def step(s):
-        if s <= last_b:
-            launch(s)
-        if s >= D:
-            radiation(s - D)
-        print(f"[bench] step {s} done ({'dynamics of batch %d launched, ' % s if s <= last_b else ''}"
-              f"{'light curves of batch %d' % (s - D) if s >= D else 'filling the pipeline'}) "
-              f"t={time.perf_counter() - t_fill:.1f}s", file=sys.stderr, flush=True)
-
-    t_fill = time.perf_counter()
-    for s in range(args.warmup):
-        step(s)
-    torch.cuda.synchronize()
-    for st in range(gpu.N_STAGES):
-        gpu.timing_read(st, reset=True)
-        gpu.timing_units(st, reset=True)
-    t0 = time.perf_counter()
-    for s in range(args.warmup, args.warmup + args.steps):
-        step(s)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    for th in threads.values():
-        th.join()
-    # batches whose dynamics started inside the timed steps and whose light curves were done by their end
-    counted = list(range(args.warmup, last_b + 1))
-    pls = [ring[counted[0] % (D + 1)]] if counted else ring[:1]
-    stage_ms = {s: gpu.timing_read(s) for s in range(gpu.N_STAGES)}
-    if args.stages:
-        for s, (ms, n) in stage_ms.items():
-            print(f"[stage] {s}: {ms:.1f} ms over {n} launches", file=sys.stderr)
-        print(f"[stage] wall per step {dt / args.steps * 1e3:.1f} ms; eats samples "
-              f"{gpu.timing_units(gpu.STAGE_EATS):.3e}; piece-wise cells {gpu.timing_units(gpu.STAGE_EATS_PW):.3e}",
-              file=sys.stderr)
-    roof = roofline(stage_ms, pls, gpu)
-    roof_dyn = roofline(stage_ms, pls, gpu, dom=gpu.STAGE_DYN) if stage_ms[gpu.STAGE_DYN][1] else None
+    legs = {}
+    if args.mode in ("both", "mixed"):
+        legs["mixed"], ring, streams = run_leg("mixed", args, gpu, torch, make, args.batch, args.depth, args.warmup,
+                                               args.steps, args.dyn_cus, args.precision, vec=True)
+        t0 = time.perf_counter()
+        legs["mixed"]["max_rel_err_vs_fp64"] = mixed_error(ring[0], n_check=min(4, args.batch))
+        print(f"[bench:mixed] light-curve check against the FP64 stages on 4 runs: "
+              f"{legs['mixed']['max_rel_err_vs_fp64']:.3e} ({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+        del ring
+        for s_ in streams:
+            s_.close()
+    if args.mode in ("both", "fp64"):
+        steps = args.steps if args.mode == "fp64" else args.fp64_steps
+        warm = args.warmup if args.mode == "fp64" else args.fp64_warmup
+        legs["fp64"], ring, streams = run_leg("fp64", args, gpu, torch, make, args.fp64_batch, args.fp64_depth, warm,
+                                              steps, args.fp64_dyn_cus, 0, vec=False)
+        del ring
+        for s_ in streams:
+            s_.close()
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.workload, args.cpu_seconds)
-    n_rej = sum(rejected.get(b, 0) for b in counted)
-    lcs = (args.batch * len(counted) - n_rej) / dt
+    head = legs["mixed"] if "mixed" in legs else legs["fp64"]
     line = {
         "metric": "light curves/sec (96nu x128t), plain FP64 vs mixed+surrogate GPU @<=1% err",
-        "value": lcs, "unit": "light curves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == 0 else "f64+bf16", "data": "synthetic",
-        "config": {"workload": args.workload, "runs_per_step": args.batch, "layers": run.n_layers,
-                   "ngam": run.params.fs.ngam, "nfreq": run.params.fs.nfreq, "nt": len(run.tgrid),
-                   "pairs": len(run.tobs), "precision": args.precision, "pipeline_depth": D,
-                   "dynamics_cus": dyn_cus, "radiation_cus": n_cu - dyn_cus,
-                   "batches_counted": len(counted), "light_curves_counted": args.batch * len(counted) - n_rej,
-                   **({"members_flagged": n_rej, "members": "Latin hypercube, seed 20261203, runs b*batch .. "
-                       "(b+1)*batch-1 per step"} if args.workload == "lhs_members" else {})},
-        "roofline": roof, "roofline_dynamics": roof_dyn, "cpu_baseline": cpu,
+        "value": head["value"], "unit": "light curves/s", "n_gpus": 1, "steps": head["steps"],
+        "warmup": head["warmup"], "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64+bf16" if "mixed" in legs else "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "layers": run.n_layers, "ngam": run.params.fs.ngam,
+                   "nfreq": run.params.fs.nfreq, "nt": len(run.tgrid), "pairs": len(run.tobs),
+                   "legs": {k: {kk: vv for kk, vv in v.items() if not kk.startswith("roofline")}
+                            for k, v in legs.items()},
+                   "value_leg": "mixed" if "mixed" in legs else "fp64",
+                   "mixed": "SSC contraction BF16 hi/lo on MFMA (FP32 accumulation), frequency-vectorised observer "
+                            f"stage rtol {args.vec_rtol:g}; dynamics and Chang-Cooper FP64",
+                   **({"members": "Latin hypercube, seed 20261203, runs b*batch .. (b+1)*batch-1 per step"}
+                      if args.workload == "lhs_members" else {})},
+        "roofline": head["roofline"], "roofline_dynamics": head["roofline_dynamics"],
+        "roofline_fp64_leg": legs["fp64"]["roofline"] if ("fp64" in legs and "mixed" in legs) else None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

@@ -331,6 +331,15 @@ class Pipeline:
         self.d_out = torch.empty((nbw, self.npairs), dtype=torch.float64, device=self.dev)
         self.d_lst = torch.zeros(nbw, dtype=torch.int32, device=self.dev)
         self.pops = self.layers = None
+        # the tensor grid of the frequency-vectorised observer stage, made here: a host-to-device copy
+        # later would go through the null stream and wait for other batches' dynamics kernels
+        if self.method != "piece-wise":
+            try:
+                ts, fs = self.tensor_grid()
+                self.d_t_u = torch.from_numpy(np.ascontiguousarray(ts)).to(self.dev)
+                self.d_nu_u = torch.from_numpy(np.ascontiguousarray(fs)).to(self.dev)
+            except ValueError:
+                pass
         torch.cuda.synchronize()
 
     def dynamics(self):
@@ -377,15 +386,18 @@ class Pipeline:
                "ele_evolve_surrogate")
         return st
 
-    def observer(self):
+    def observer(self, first_layers=None):
+        """Observer stage (the reference's per-pair quadrature) of every layer, or of the first
+        `first_layers` layers (the runs of a batch are laid out one after the other)."""
+        nl = len(self.layers) if first_layers is None else min(int(first_layers), len(self.layers))
         if self.method == "piece-wise":
-            _check(self.L.pbagpu_eats_lightcurve_pw(self.params, self.n_runs, self.larr, len(self.layers),
+            _check(self.L.pbagpu_eats_lightcurve_pw(self.params, self.n_runs, self.larr, nl,
                                                     _ptr(self.d_table), self.nt, _ptr(self.d_jall), _ptr(self.d_aall),
                                                     self.nf,
                                                     self.npairs, _ptr(self.d_tobs), _ptr(self.d_nuobs),
                                                     _ptr(self.d_out), _ptr(self.d_lst), self.rsp), "eats_lightcurve_pw")
             return
-        _check(self.L.pbagpu_eats_lightcurve(self.params, self.n_runs, self.larr, len(self.layers), _ptr(self.d_table),
+        _check(self.L.pbagpu_eats_lightcurve(self.params, self.n_runs, self.larr, nl, _ptr(self.d_table),
                                              self.nt, _ptr(self.d_jall), _ptr(self.d_aall), self.nf, self.npairs,
                                              _ptr(self.d_tobs),
                                              _ptr(self.d_nuobs), _ptr(self.d_out), _ptr(self.d_lst), self.rsp),

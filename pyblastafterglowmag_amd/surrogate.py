@@ -14,7 +14,7 @@ w_k = n_k / nu_k (nu_{k+1} - nu_k), where n is the photon density of the previou
 density is proportional to the previous step's total emissivity divided by nu, so
 w_k is proportional to j_{k, it-1} / nu_k^2 (nu_{k+1} - nu_k); the constant cancels in the
 features, which take f / max f and w / max w.  The target is the plain contraction of (f, w),
-divided by max f max w (exact, as the contraction is bilinear), in log10 with a floor 12 decades
+divided by max f max w (exact, as the contraction is bilinear), in log10 with a floor 6 decades
 below the cell's peak, standardised per frequency.
 """
 import ctypes as C
@@ -24,7 +24,7 @@ import numpy as np
 
 from . import abi, gpu
 
-FLOOR_DECADES = 12.0
+FLOOR_DECADES = 6.0
 
 
 def _torch():
