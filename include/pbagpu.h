@@ -465,6 +465,23 @@ int pbagpu_skymap_hist(int n_images, const int *img_seg, const long *seg_off,
                        double *d_edges_x, double *d_edges_y, double *d_hist, unsigned *d_count,
                        double *d_moments, int *d_status, void *stream);
 
+/* Allocation-free variant of pbagpu_skymap_hist: same arguments, same kernels, same results.
+ * It has no counterpart in the reference; it is for callers that cannot allocate device
+ * memory (code under stream capture, serving loops, harnesses that forbid hipMalloc).
+ * It calls no hipMalloc / hipFree: its work tables (the device copies of img_seg and seg_off,
+ * the work-item table, built on the device, and the per-item partial sums) live in a 4 MiB
+ * array in device memory that every GPU the library runs on holds until the process ends.
+ * A call whose tables need more than 4 MiB is rejected with PBAGPU_E_ARG before any device
+ * work (4096 work items of <= 4096 points fit in well under 1 MiB; the item table takes 80
+ * bytes per item).  Every call on one GPU shares that array: calls that overlap in time on
+ * one GPU (two host threads, or two streams) race and must not be issued; consecutive calls
+ * are fine, each one rewrites what it reads. */
+int pbagpu_skymap_hist_noalloc(int n_images, const int *img_seg, const long *seg_off,
+                               const double *d_x, const double *d_y, const double *d_w,
+                               int nx, int ny, double extend_grid, int edges_given,
+                               double *d_edges_x, double *d_edges_y, double *d_hist, unsigned *d_count,
+                               double *d_moments, int *d_status, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

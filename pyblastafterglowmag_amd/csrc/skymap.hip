@@ -16,6 +16,9 @@
 #include <float.h>
 #include <algorithm>
 #include <math.h>
+#include <climits>
+#include <cstdio>
+#include <mutex>
 #include <vector>
 #include "runtime.h"
 #include "timing.h"
@@ -212,6 +215,128 @@ __global__ __launch_bounds__(BLOCK) void sky_bin_kernel(const int *img_seg, cons
     }
 }
 
+// Item table of the allocation-free entry, built on the device: entry k is split into
+// max(1, ceil(len_k / ITEM)) work items; item_off[k] is the first item of entry k
+// (item_off[n_segs] = the item count) and items[2 i], items[2 i + 1] is item i's point range,
+// the same table the host builds for pbagpu_skymap_hist.  One workgroup: every thread takes a
+// contiguous run of entries, and an exclusive prefix over the threads' item counts places them.
+__host__ __device__ __forceinline__ long items_of(long b, long e) { return e > b ? (e - b + ITEM - 1) / ITEM : 1; }
+
+__global__ __launch_bounds__(BLOCK) void sky_items_kernel(int n_segs, const long *seg_off, int *item_off,
+                                                          long *items) {
+    __shared__ int s_sum[BLOCK];
+    const int t = threadIdx.x;
+    const int per = (n_segs + BLOCK - 1) / BLOCK;
+    const int k0 = min(t * per, n_segs), k1 = min(k0 + per, n_segs);
+    int local = 0;
+    for (int k = k0; k < k1; k++) local += (int)items_of(seg_off[k], seg_off[k + 1]);
+    s_sum[t] = local;
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int q = 0; q < BLOCK; q++) { const int v = s_sum[q]; s_sum[q] = acc; acc += v; }
+        item_off[n_segs] = acc;
+    }
+    __syncthreads();
+    int it = s_sum[t];
+    for (int k = k0; k < k1; k++) {
+        item_off[k] = it;
+        long b = seg_off[k];
+        do {
+            const long e = min(b + (long)ITEM, seg_off[k + 1]);
+            items[2 * (size_t)it] = b; items[2 * (size_t)it + 1] = e;
+            it++;
+            b = e;
+        } while (b < seg_off[k + 1]);
+    }
+}
+
+// Scratch of the allocation-free entry: one copy per device, held until the process ends.
+// Laid out per call as img_seg | item_off (ints), seg_off | items (longs), partials (doubles),
+// each part 256-byte aligned.
+constexpr size_t SCRATCH_BYTES = (size_t)4 << 20;
+__device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of the current device's scratch (hipGetSymbolAddress once per device, with
+// that device current).  The array's name is a host-side stand-in and is never passed on.
+void *scratch_address() {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { set_error("hipGetDevice", e); return nullptr; }
+    if (dev < 0 || dev >= MAX_DEVICES) { set_error_msg("skymap_hist_noalloc: device index above 63"); return nullptr; }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) { set_error("hipGetSymbolAddress(scratch)", e); return nullptr; }
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Argument checks shared by both entries.  On success sets the entry count, the point count and
+// the largest image's point count.
+int check_args(int n_images, const int *img_seg, const long *seg_off, const double *d_x, const double *d_y,
+               const double *d_w, int nx, int ny, double extend_grid, double *d_edges_x, double *d_edges_y,
+               double *d_hist, double *d_moments, int *d_status, const char *who, int &n_segs, long &n_pts,
+               long &max_pts) {
+    char msg[192];
+    if (n_images < 1 || n_images > 65535 || !img_seg || !seg_off || !d_edges_x || !d_edges_y || !d_hist ||
+        !d_moments || !d_status || nx < 2 || ny < 2 || nx + ny > MAX_EDGES) {
+        snprintf(msg, sizeof(msg), "%s: bad arguments (1 <= n_images <= 65535, nx, ny >= 2, nx + ny <= 4096)", who);
+        set_error_msg(msg);
+        return PBAGPU_E_ARG;
+    }
+    if (img_seg[0] != 0) { snprintf(msg, sizeof(msg), "%s: img_seg[0] must be 0", who); set_error_msg(msg); return PBAGPU_E_ARG; }
+    for (int i = 0; i < n_images; i++)
+        if (img_seg[i + 1] < img_seg[i]) {
+            snprintf(msg, sizeof(msg), "%s: img_seg decreases", who); set_error_msg(msg); return PBAGPU_E_ARG;
+        }
+    n_segs = img_seg[n_images];
+    if (seg_off[0] != 0) { snprintf(msg, sizeof(msg), "%s: seg_off[0] must be 0", who); set_error_msg(msg); return PBAGPU_E_ARG; }
+    for (int k = 0; k < n_segs; k++)
+        if (seg_off[k + 1] < seg_off[k]) {
+            snprintf(msg, sizeof(msg), "%s: seg_off decreases", who); set_error_msg(msg); return PBAGPU_E_ARG;
+        }
+    n_pts = seg_off[n_segs];
+    if (n_pts > 0 && (!d_x || !d_y || !d_w)) return PBAGPU_E_ARG;
+    if (!(extend_grid == extend_grid)) {
+        snprintf(msg, sizeof(msg), "%s: extend_grid is NaN", who); set_error_msg(msg); return PBAGPU_E_ARG;
+    }
+    max_pts = 0;
+    for (int i = 0; i < n_images; i++) {
+        const long np_i = seg_off[img_seg[i + 1]] - seg_off[img_seg[i]];
+        if (np_i > max_pts) max_pts = np_i;
+    }
+    return PBAGPU_OK;
+}
+
+// The four binning launches on stream s, given the device copies of img_seg / item_off
+// (dseg, ditem_off), seg_off / items (doff, ditems) and the partials buffer.
+void launch_binning(hipStream_t s, int n_images, int n_items, long max_pts, const int *dseg, const int *ditem_off,
+                    const long *doff, const long *ditems, double *dpart, const double *d_x, const double *d_y,
+                    const double *d_w, int nx, int ny, double extend_grid, int edges_given, double *d_edges_x,
+                    double *d_edges_y, double *d_hist, unsigned *d_count, double *d_moments, int *d_status) {
+    const size_t nb = (size_t)n_images * (nx - 1) * (ny - 1);
+    const unsigned zgrid = (unsigned)std::min<size_t>((nb + BLOCK - 1) / BLOCK, 4096);
+    hipLaunchKernelGGL(sky_zero_kernel, dim3(zgrid), dim3(BLOCK), 0, s, d_hist, d_count, nb);
+    if (n_items > 0)
+        hipLaunchKernelGGL(sky_partial_kernel, dim3(n_items), dim3(BLOCK), 0, s, ditems, d_x, d_y, d_w, dpart);
+    hipLaunchKernelGGL(sky_extent_kernel, dim3(n_images), dim3(BLOCK), 0, s, dseg, doff, ditem_off, dpart, nx, ny,
+                       extend_grid, edges_given, d_edges_x, d_edges_y, d_moments, d_status);
+    if (max_pts > 0) {
+        const unsigned chunks = (unsigned)std::min<long>((max_pts + BLOCK - 1) / BLOCK, 1024);
+        hipLaunchKernelGGL(sky_bin_kernel, dim3(chunks, n_images), dim3(BLOCK), sizeof(double) * (nx + ny), s, dseg,
+                           doff, d_x, d_y, d_w, nx, ny, d_edges_x, d_edges_y, d_status, d_hist, d_count);
+    }
+}
+
 }  // namespace skybin
 }  // namespace pbagpu
 
@@ -223,27 +348,11 @@ extern "C" int pbagpu_skymap_hist(int n_images, const int *img_seg, const long *
                                   unsigned *d_count, double *d_moments, int *d_status, void *stream) {
     using namespace skybin;
     hipStream_t s = (hipStream_t)stream;
-    if (n_images < 1 || n_images > 65535 || !img_seg || !seg_off || !d_edges_x || !d_edges_y || !d_hist ||
-        !d_moments || !d_status || nx < 2 || ny < 2 || nx + ny > MAX_EDGES) {
-        set_error_msg("skymap_hist: bad arguments (1 <= n_images <= 65535, nx, ny >= 2, nx + ny <= 4096)");
-        return PBAGPU_E_ARG;
-    }
-    if (img_seg[0] != 0) { set_error_msg("skymap_hist: img_seg[0] must be 0"); return PBAGPU_E_ARG; }
-    for (int i = 0; i < n_images; i++)
-        if (img_seg[i + 1] < img_seg[i]) { set_error_msg("skymap_hist: img_seg decreases"); return PBAGPU_E_ARG; }
-    const int n_segs = img_seg[n_images];
-    if (seg_off[0] != 0) { set_error_msg("skymap_hist: seg_off[0] must be 0"); return PBAGPU_E_ARG; }
-    for (int k = 0; k < n_segs; k++)
-        if (seg_off[k + 1] < seg_off[k]) { set_error_msg("skymap_hist: seg_off decreases"); return PBAGPU_E_ARG; }
-    const long n_pts = seg_off[n_segs];
-    if (n_pts > 0 && (!d_x || !d_y || !d_w)) return PBAGPU_E_ARG;
-    if (!(extend_grid == extend_grid)) { set_error_msg("skymap_hist: extend_grid is NaN"); return PBAGPU_E_ARG; }
-    // points of the largest image: sizes the binning grid
-    long max_pts = 0;
-    for (int i = 0; i < n_images; i++) {
-        const long np_i = seg_off[img_seg[i + 1]] - seg_off[img_seg[i]];
-        if (np_i > max_pts) max_pts = np_i;
-    }
+    int n_segs = 0;
+    long n_pts = 0, max_pts = 0;
+    int rc = check_args(n_images, img_seg, seg_off, d_x, d_y, d_w, nx, ny, extend_grid, d_edges_x, d_edges_y, d_hist,
+                        d_moments, d_status, "skymap_hist", n_segs, n_pts, max_pts);
+    if (rc) return rc;
     // work items of at most ITEM points; every entry has at least one (empty entries: an empty one)
     std::vector<int> item_off(n_segs + 1, 0);
     std::vector<long> items;
@@ -264,30 +373,62 @@ extern "C" int pbagpu_skymap_hist(int n_images, const int *img_seg, const long *
     if (!dseg || !doff || !dpart) return PBAGPU_E_HIP;
     int *ditem_off = dseg + n_images + 1;
     long *ditems = doff + n_segs + 1;
-    int rc = PBAGPU_OK;
     PBA_TRY(rc, hipMemcpyAsync(dseg, img_seg, sizeof(int) * (n_images + 1), hipMemcpyHostToDevice, s));
     PBA_TRY(rc, hipMemcpyAsync(ditem_off, item_off.data(), sizeof(int) * (n_segs + 1), hipMemcpyHostToDevice, s));
     PBA_TRY(rc, hipMemcpyAsync(doff, seg_off, sizeof(long) * (n_segs + 1), hipMemcpyHostToDevice, s));
     if (!items.empty())
         PBA_TRY(rc, hipMemcpyAsync(ditems, items.data(), sizeof(long) * items.size(), hipMemcpyHostToDevice, s));
     if (rc) return rc;
-    const size_t nb = (size_t)n_images * (nx - 1) * (ny - 1);
-    const unsigned zgrid = (unsigned)std::min<size_t>((nb + BLOCK - 1) / BLOCK, 4096);
     KernelTimer timer(PBAGPU_STAGE_SKY_HIST);
     timer.start(s);
-    hipLaunchKernelGGL(sky_zero_kernel, dim3(zgrid), dim3(BLOCK), 0, s, d_hist, d_count, nb);
-    if (n_items > 0)
-        hipLaunchKernelGGL(sky_partial_kernel, dim3(n_items), dim3(BLOCK), 0, s, ditems, d_x, d_y, d_w, dpart);
-    hipLaunchKernelGGL(sky_extent_kernel, dim3(n_images), dim3(BLOCK), 0, s, dseg, doff, ditem_off, dpart, nx, ny,
-                       extend_grid, edges_given, d_edges_x, d_edges_y, d_moments, d_status);
-    if (max_pts > 0) {
-        const unsigned chunks = (unsigned)std::min<long>((max_pts + BLOCK - 1) / BLOCK, 1024);
-        hipLaunchKernelGGL(sky_bin_kernel, dim3(chunks, n_images), dim3(BLOCK), sizeof(double) * (nx + ny), s, dseg,
-                           doff, d_x, d_y, d_w, nx, ny, d_edges_x, d_edges_y, d_status, d_hist, d_count);
-    }
+    launch_binning(s, n_images, n_items, max_pts, dseg, ditem_off, doff, ditems, dpart, d_x, d_y, d_w, nx, ny,
+                   extend_grid, edges_given, d_edges_x, d_edges_y, d_hist, d_count, d_moments, d_status);
     timer.stop(s);
     PBA_TRY(rc, hipGetLastError());
     PBA_TRY(rc, hipStreamSynchronize(s));
     timer.collect();
+    return rc;
+}
+
+extern "C" int pbagpu_skymap_hist_noalloc(int n_images, const int *img_seg, const long *seg_off, const double *d_x,
+                                          const double *d_y, const double *d_w, int nx, int ny, double extend_grid,
+                                          int edges_given, double *d_edges_x, double *d_edges_y, double *d_hist,
+                                          unsigned *d_count, double *d_moments, int *d_status, void *stream) {
+    using namespace skybin;
+    hipStream_t s = (hipStream_t)stream;
+    int n_segs = 0;
+    long n_pts = 0, max_pts = 0;
+    int rc = check_args(n_images, img_seg, seg_off, d_x, d_y, d_w, nx, ny, extend_grid, d_edges_x, d_edges_y, d_hist,
+                        d_moments, d_status, "skymap_hist_noalloc", n_segs, n_pts, max_pts);
+    if (rc) return rc;
+    long n_items = 0;
+    for (int k = 0; k < n_segs; k++) n_items += items_of(seg_off[k], seg_off[k + 1]);
+    // scratch layout of this call; sizes beyond the scratch are rejected before any device work
+    const size_t b_seg = align256(sizeof(int) * ((size_t)n_images + 1 + (size_t)n_segs + 1));
+    const size_t b_off = align256(sizeof(long) * ((size_t)n_segs + 1 + 2 * (size_t)n_items));
+    const size_t b_part = align256(sizeof(double) * (size_t)n_items * P_N);
+    if (n_items > INT_MAX || b_seg + b_off + b_part > SCRATCH_BYTES) {
+        char msg[224];
+        snprintf(msg, sizeof(msg),
+                 "skymap_hist_noalloc: %d images, %d entries and %ld work items need %zu bytes of scratch; "
+                 "the limit is %zu bytes", n_images, n_segs, n_items, b_seg + b_off + b_part, SCRATCH_BYTES);
+        set_error_msg(msg);
+        return PBAGPU_E_ARG;
+    }
+    auto *base = (unsigned char *)scratch_address();
+    if (!base) return PBAGPU_E_HIP;
+    int *dseg = (int *)base;
+    int *ditem_off = dseg + n_images + 1;
+    long *doff = (long *)(base + b_seg);
+    long *ditems = doff + n_segs + 1;
+    double *dpart = (double *)(base + b_seg + b_off);
+    PBA_TRY(rc, hipMemcpyAsync(dseg, img_seg, sizeof(int) * (n_images + 1), hipMemcpyHostToDevice, s));
+    PBA_TRY(rc, hipMemcpyAsync(doff, seg_off, sizeof(long) * (n_segs + 1), hipMemcpyHostToDevice, s));
+    if (rc) return rc;
+    hipLaunchKernelGGL(sky_items_kernel, dim3(1), dim3(BLOCK), 0, s, n_segs, (const long *)doff, ditem_off, ditems);
+    launch_binning(s, n_images, (int)n_items, max_pts, dseg, ditem_off, doff, ditems, dpart, d_x, d_y, d_w, nx, ny,
+                   extend_grid, edges_given, d_edges_x, d_edges_y, d_hist, d_count, d_moments, d_status);
+    PBA_TRY(rc, hipGetLastError());
+    PBA_TRY(rc, hipStreamSynchronize(s));
     return rc;
 }

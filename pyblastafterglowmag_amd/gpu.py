@@ -19,7 +19,8 @@ EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu
            "pbagpu_eats_lightcurve_pw", "pbagpu_eats_skymap_pw", "pbagpu_skymap_hist", "pbagpu_bessel_k_thirds",
            "pbagpu_release_stream", "pbagpu_stream_create", "pbagpu_stream_destroy",
            "pbagpu_synchrotron_f", "pbagpu_eats_lightcurve_vec", "pbagpu_mlp_forward", "pbagpu_mlp_train_step",
-           "pbagpu_gemm_bf16", "pbagpu_ssc_features", "pbagpu_ssc_emissivity", "pbagpu_ele_evolve_surrogate")
+           "pbagpu_gemm_bf16", "pbagpu_ssc_features", "pbagpu_ssc_emissivity", "pbagpu_ele_evolve_surrogate",
+           "pbagpu_skymap_hist_noalloc")
 STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW, STAGE_SKY_HIST, STAGE_EATS_VEC = 0, 1, 2, 3, 4, 5, 6
 N_STAGES = 7
 # sky-map binning (enum pba_sky_moment / pba_sky_status)
@@ -74,6 +75,7 @@ def lib():
         L.pbagpu_stream_destroy.argtypes = [vp]
         L.pbagpu_skymap_hist.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_long), vp, vp, vp, C.c_int,
                                          C.c_int, C.c_double, C.c_int, vp, vp, vp, vp, vp, vp, vp]
+        L.pbagpu_skymap_hist_noalloc.argtypes = L.pbagpu_skymap_hist.argtypes
         _lib = L
     return _lib
 
@@ -165,12 +167,14 @@ def synchrotron_f(x):
     return f
 
 
-def skymap_hist(images, nx, ny, extend_grid=1.0, edges=None, want_count=False, device=0):
+def skymap_hist(images, nx, ny, extend_grid=1.0, edges=None, want_count=False, device=0, noalloc=False):
     """Histogram images, centroids and extents of raw sky maps in one pbagpu_skymap_hist call.
 
     images: one list of entries per image, an entry being host arrays (x, y, intensity) of one
             shell's principal or counter jet (ProcessRawSkymap._get_skymap's lists).
     edges:  None (edges from the data extent) or (edges_x [n_images][nx], edges_y [n_images][ny]).
+    noalloc: call pbagpu_skymap_hist_noalloc (work tables in the library's fixed device scratch)
+            instead of pbagpu_skymap_hist.
     Returns a dict of host arrays: hist [n][nx-1][ny-1], count (or None), edges_x, edges_y,
     moments [n][len(SKY_MOMENTS)], status [n]."""
     torch = _torch()
@@ -205,9 +209,10 @@ def skymap_hist(images, nx, ny, extend_grid=1.0, edges=None, want_count=False, d
     off_arr = (C.c_long * len(seg_off))(*seg_off)
     with torch.cuda.device(dev):
         stream = torch.cuda.current_stream()
-        _check(L.pbagpu_skymap_hist(n, seg_arr, off_arr, _ptr(d_x), _ptr(d_y), _ptr(d_w), nx, ny, float(extend_grid),
-                                    0 if edges is None else 1, _ptr(d_ex), _ptr(d_ey), _ptr(d_h), _ptr(d_c), _ptr(d_m),
-                                    _ptr(d_s), _stream_ptr(stream)), "skymap_hist")
+        fn = L.pbagpu_skymap_hist_noalloc if noalloc else L.pbagpu_skymap_hist
+        _check(fn(n, seg_arr, off_arr, _ptr(d_x), _ptr(d_y), _ptr(d_w), nx, ny, float(extend_grid),
+                  0 if edges is None else 1, _ptr(d_ex), _ptr(d_ey), _ptr(d_h), _ptr(d_c), _ptr(d_m),
+                  _ptr(d_s), _stream_ptr(stream)), "skymap_hist_noalloc" if noalloc else "skymap_hist")
     return dict(hist=d_h.cpu().numpy(), count=None if d_c is None else d_c.cpu().numpy().view(np.uint32),
                 edges_x=d_ex.cpu().numpy(), edges_y=d_ey.cpu().numpy(), moments=d_m.cpu().numpy(),
                 status=d_s.cpu().numpy())
