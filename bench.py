@@ -68,10 +68,17 @@ def parse():
     ap.add_argument("--mode", default="both", choices=["both", "mixed", "fp64"])
     ap.add_argument("--fp64-batch", type=int, default=12)
     ap.add_argument("--fp64-depth", type=int, default=2)
-    ap.add_argument("--fp64-steps", type=int, default=4)
+    ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--fp64-warmup", type=int, default=2)
     ap.add_argument("--fp64-dyn-cus", type=int, default=32)
     ap.add_argument("--precision", type=int, default=1, help="SSC contraction of the mixed leg: 1 BF16 hi/lo, 0 FP64")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="mixed leg with the learned SSC emissivity (trained at set-up, untimed, on the FP64 cells of "
+                         "Latin-hypercube members other than the workload) in place of the BF16 contraction")
+    ap.add_argument("--train-members", default="3,5,6,7", help="Latin-hypercube members whose cells train the surrogate")
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--check-tol", type=float, default=0.2,
+                    help="surrogate: relative error of a sampled check above which a population falls back")
     ap.add_argument("--workload", default="gauss21_fsrs_ssc",
                     choices=["gauss21_fsrs_ssc", "tophat_fs_syn", "tophat64_pw", "lhs_members"],
                     help="lhs_members: every run a different member of the 1024-member Latin-hypercube batch")
@@ -224,7 +231,33 @@ def roofline(stage_ms, pls, gpu, dom=None):
     return roof
 
 
-def run_leg(name, args, gpu, torch, make, batch, depth, warmup, steps, dyn_cus, precision, vec):
+def train_surrogate(args, gpu, torch, workloads):
+    """Untimed set-up of the surrogate leg: the FP64 electron stage of Latin-hypercube members
+    (parameters other than the workload's, on its grids) gives the training cells; the network is
+    trained by the library's BF16 forward / backward kernels and Adam.  Returns (SscSurrogate, info)."""
+    from pyblastafterglowmag_amd import surrogate
+    t0 = time.perf_counter()
+    members = [int(x) for x in args.train_members.split(",") if x]
+    pl = gpu.Pipeline([gpu.Run.from_workload(*workloads.lhs_member(i, ragged_nt=False)) for i in members])
+    pl.dynamics()
+    pl.electrons(want_f=True)
+    f, w, B, sh = surrogate.cells_from_pipeline(pl)
+    x, sc, ylog, _ = surrogate.targets(sh, f, w, B)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sur, hist = surrogate.train(x, ylog, steps=args.train_steps, batch=4096)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    sur.check_tol = args.check_tol
+    info = {"train_members": members, "train_cells": int(x.shape[0]), "train_steps": args.train_steps,
+            "pairs_seconds": t1 - t0, "train_seconds": t2 - t1, "loss_first": hist[0][1], "loss_last": hist[-1][1],
+            "check_tol": args.check_tol, "check_every": sur.check_every}
+    print(f"[bench:surrogate] {info}", file=sys.stderr, flush=True)
+    del pl
+    return sur, info
+
+
+def run_leg(name, args, gpu, torch, make, batch, depth, warmup, steps, dyn_cus, precision, vec, sur=None):
     """One pipelined leg; returns its measurements and the ring of pipelines (for the roofline)."""
     D = max(depth, 1)
     n_cu = gpu.device_cu_count()
@@ -244,10 +277,18 @@ def run_leg(name, args, gpu, torch, make, batch, depth, warmup, steps, dyn_cus, 
         th.start()
         threads[b] = th
 
+    sur_stats = {"inferred": 0, "checked": 0, "failed": 0, "fallback": 0, "max_err": 0.0}
+
     def radiation(b):
         pl = ring[b % (D + 1)]
         threads.pop(b).join()
-        pl.electrons()
+        if sur is not None:
+            st = pl.electrons_surrogate(sur)
+            for k in ("inferred", "checked", "failed", "fallback"):
+                sur_stats[k] += getattr(st, k)
+            sur_stats["max_err"] = max(sur_stats["max_err"], st.max_err)
+        else:
+            pl.electrons()
         if vec and pl.method != "piece-wise":
             pl.observer_vec(args.vec_rtol)
         else:
@@ -298,6 +339,8 @@ def run_leg(name, args, gpu, torch, make, batch, depth, warmup, steps, dyn_cus, 
            "roofline_dynamics": roofline(stage_ms, pls, gpu, dom=gpu.STAGE_DYN) if stage_ms[gpu.STAGE_DYN][1] else None}
     if args.workload == "lhs_members":
         out["members_flagged"] = n_rej
+    if sur is not None:
+        out["surrogate_population_steps"] = sur_stats
     if args.stages:
         for st_, (ms, n) in stage_ms.items():
             print(f"[stage:{name}] {st_}: {ms:.1f} ms over {n} launches", file=sys.stderr)
@@ -350,9 +393,14 @@ def main():
     run = make(0, 1)[0]
     gpu.timing_enable(True)
     legs = {}
+    sur, sur_info = (None, None)
+    if args.surrogate and args.mode in ("both", "mixed"):
+        sur, sur_info = train_surrogate(args, gpu, torch, workloads)
     if args.mode in ("both", "mixed"):
         legs["mixed"], ring, streams = run_leg("mixed", args, gpu, torch, make, args.batch, args.depth, args.warmup,
-                                               args.steps, args.dyn_cus, args.precision, vec=True)
+                                               args.steps, args.dyn_cus, args.precision, vec=True, sur=sur)
+        if sur_info:
+            legs["mixed"]["surrogate"] = sur_info
         t0 = time.perf_counter()
         legs["mixed"]["max_rel_err_vs_fp64"] = mixed_error(ring[0], n_check=min(4, args.batch))
         print(f"[bench:mixed] light-curve check against the FP64 stages on 4 runs: "
@@ -380,8 +428,10 @@ def main():
                    "legs": {k: {kk: vv for kk, vv in v.items() if not kk.startswith("roofline")}
                             for k, v in legs.items()},
                    "value_leg": "mixed" if "mixed" in legs else "fp64",
-                   "mixed": "SSC contraction BF16 hi/lo on MFMA (FP32 accumulation), frequency-vectorised observer "
-                            f"stage rtol {args.vec_rtol:g}; dynamics and Chang-Cooper FP64",
+                   "mixed": ("SSC emissivity from the learned surrogate (BF16 MFMA network, FP32 accumulation; sampled "
+                             "checks against the plain contraction with per-population fallback)" if sur is not None
+                             else "SSC contraction BF16 hi/lo on MFMA (FP32 accumulation)")
+                            + f", frequency-vectorised observer stage rtol {args.vec_rtol:g}; dynamics and Chang-Cooper FP64",
                    **({"members": "Latin hypercube, seed 20261203, runs b*batch .. (b+1)*batch-1 per step"}
                       if args.workload == "lhs_members" else {})},
         "roofline": head["roofline"], "roofline_dynamics": head["roofline_dynamics"],

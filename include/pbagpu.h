@@ -90,6 +90,10 @@ typedef struct {
                                      point, 10^log10(tb0), can differ from it by an ulp) */
     double tprompt, rs_shutoff_rho, min_gamma0_for_rs, rs_gamma0_frac_no_exceed;
     double mom_when_start_spread, theta_max;
+    double ism_A0, ism_s, ism_r_ej, ism_r_ism; /* density profile when n_ism <= 0 (parfile A0, s, r_ej,
+                                     r_ism; RhoISM::evaluateRhoDrhoDrDefault,
+                                     src/blastwave/blastwave_components.h:846-893): n = A0 r_ej^-s
+                                     inside r_ej, A0 R^-s up to r_ism, A0 r_ism^-s beyond */
     double rtol_theta, rtol_phi;  /* EATS tolerances */
     int nmax_theta, nmax_phi;
     int fsrs;                     /* bw_type = fsrs */
@@ -115,6 +119,10 @@ typedef struct {
     double theta_b0;  /* initial half-opening angle: theta_c_h (adaptive) or theta_wing (piece-wise) */
     double ncells;    /* 1 (adaptive) or the total cell count n^2 of the jet (piece-wise); >= 1 */
     int ilayer, nlayers;
+    int ngam_fs, ngam_rs; /* this blastwave's Chang-Cooper grid size for each shock, or 0 for the
+                             run's (pba_shock_params ngam).  The reference shares one grid per run
+                             (CommonTables, src/microphysics/common.h:100-110); a run of the
+                             piece-wise workload with per-layer electron grids sets them per layer. */
 } pba_layer_init;
 
 /* ------------------------------------------------ per-step table layout */
@@ -208,6 +216,18 @@ int pbagpu_dyn_evolve(const pba_run_params *params, int n_runs, int bw_per_run,
                       const pba_layer_init *init, int nt, const double *d_tgrid,
                       double *d_table, pba_bw_result *res, void *stream);
 
+/* The same for runs whose time grids differ in length (and values), in one launch: run r has
+ * its own parfile grid of nt_run[r] steps (MakeLogspaceVec(tb0, tb1, ntb) of that run).
+ *   nt_run   [n_runs] host, 2 <= nt_run[r] <= nt_max
+ *   d_tgrid  [n_runs][nt_max] device: row r holds run r's grid in its first nt_run[r] entries
+ *   d_table  [n_runs*bw_per_run][PBA_NV][nt_max] device, written: run r's steps in columns
+ *            [0, nt_run[r]), zeros after them (no step flags there, so the electron stage
+ *            skips them; every blastwave's i_end_r <= nt_run[r] bounds the observer stage)
+ * The electron and observer stages then take nt = nt_max. */
+int pbagpu_dyn_evolve_ragged(const pba_run_params *params, int n_runs, int bw_per_run,
+                             const pba_layer_init *init, int nt_max, const int *nt_run,
+                             const double *d_tgrid, double *d_table, pba_bw_result *res, void *stream);
+
 /* ------------------------------------------------------------- stage 2 */
 /* One electron population: shock (0 forward, 1 reverse) of blastwave `bw`
  * (index into the dynamics table), with the microphysics of run `run`. */
@@ -215,7 +235,8 @@ typedef struct {
     int bw;
     int run;
     int shock;
-    int pad_;
+    int ngam;   /* Chang-Cooper grid size of this population, or 0 for its shock's ngam (the
+                   blastwave's pba_layer_init ngam_fs / ngam_rs) */
 } pba_population;
 
 /* Electron kinetics + emissivity of n_pop populations over all nt steps.
@@ -227,14 +248,48 @@ typedef struct {
  *            population's shock has ssa = 1
  *   d_fall   [n_pop][nt][ngam]  device or NULL: electron spectrum per step (ele.f_all)
  *   d_status [n_pop] device int, written
- * The grid sizes ngam/nfreq come from the shock parameters and must be equal
- * within one call.  precision: 0 = FP64 everywhere; 1 = SSC contraction in
- * BF16 hi/lo split on MFMA with FP32 accumulation. */
+ * nfreq comes from the shock parameters and must be equal within one call; ngam
+ * (the population's, or its shock's) may differ between populations: every population
+ * keeps its own grid, Compton table and state, and d_fall rows are laid out with the
+ * largest ngam of the call as their stride ([n_pop][nt][ngam_max], a population's
+ * spectrum in the first ngam entries of a row).  precision: 0 = FP64 everywhere; 1 = SSC
+ * contraction in BF16 hi/lo split on MFMA with FP32 accumulation. */
 int pbagpu_ele_evolve(const pba_run_params *params, int n_runs,
                       const pba_population *pops, int n_pop,
                       const double *d_table, int bw_total, int nt,
                       double *d_jall, double *d_aall, double *d_fall, int *d_status,
                       int precision, void *stream);
+
+/* The comoving spectra that the reference keeps per step with save_spec = yes
+ * (ElectronAndRadiation::storeDenseOutput, src/microphysics/microphysics.h:602-633, and
+ * Electrons::save_to_all, src/microphysics/base_state.h:200-211) and writes into spec.h5
+ * (Ejecta::computeSaveEjectaSpectrum, src/ejecta/ejecta.h:354-509).  Every pointer is a
+ * device array or NULL (not written); a population's rows stay 0 at steps where it does not
+ * radiate, as in the reference.
+ *   syn_j, syn_i   [n_pop][nt][nfreq]  syn.j_all = syn.j accel_frac, syn.i_all (its intensity
+ *                                     with the absorption depth a_all dr', computeIntensity's
+ *                                     iAPPROX rule)
+ *   ssc_j          [n_pop][nt][nfreq]  ssc.j_all = ssc.j accel_frac (ssc.i_all equals it: the
+ *                                     SSC absorption is 0 without pair production)
+ *   total_f        [n_pop][nt][nfreq]  total_rad.f_all, the photon density of the step
+ *   total_i        [n_pop][nt][nfreq]  total_rad.i_all
+ *   gam_dot_syn, gam_dot_adi, gam_dot_ssc  [n_pop][nt][ngam_max]  the cooling rates of the last
+ *                                     Chang-Cooper step (0 in the last bin and before the first)
+ * syn.f_all and ssc.f_all are never set by the reference (0 in its files); syn.a_all and
+ * total_rad.a_all equal d_aall. */
+typedef struct {
+    double *syn_j, *syn_i;
+    double *ssc_j;
+    double *total_f, *total_i;
+    double *gam_dot_syn, *gam_dot_adi, *gam_dot_ssc;
+} pba_spectra;
+
+/* pbagpu_ele_evolve plus the spectra above (the reference's save_spec = yes path). */
+int pbagpu_ele_evolve_spectra(const pba_run_params *params, int n_runs,
+                              const pba_population *pops, int n_pop,
+                              const double *d_table, int bw_total, int nt,
+                              double *d_jall, double *d_aall, double *d_fall, int *d_status,
+                              int precision, const pba_spectra *spec, void *stream);
 
 /* ------------------------------------------------------------- stage 3 */
 /* Observer light curve of every layer at n_pairs (t_obs, nu_obs) pairs.

@@ -42,13 +42,14 @@ struct Shock {
 
 struct BW {
     double E0, M0, R0, Gamma0, beta0, mom0, theta_a, theta_b0, ncells, ctheta0, theta_w, theta_core;
-    double theta_max, Rd, rho_def;
+    double theta_max, Rd, rho_def, drhodr0;
     int ilayer, nlayers;
     bool fsrs, do_rs, do_rs_rad, shutOff, end_evolution, end_spreading;
     int prev_idx_x;
     int i_end_r;
     MState fs_st, rs_st;
     int status;
+    Shock fs_sh, rs_sh;   // this blastwave's electron grids (the run's, or pba_layer_init ngam_fs / ngam_rs)
 };
 
 struct Run {
@@ -99,6 +100,24 @@ static double dthetadr_our(const BW &b, double Gamma, double GammaShk, double R,
 }
 
 // EQS::dmdr (usingdthdR) (blastwave_components.h:59-61)
+// RhoISM::evaluateRhoDrhoDrDefault (blastwave_components.h:846-893): the ambient density and its
+// radial derivative at R; uniform for n_ism > 0, else the step-function (wind) profile of A0, s,
+// r_ej, r_ism.  The reference exits for rho > 1 (:886-890).
+static void ism_density(const pba_run_params &P, double R, double &rho, double &drhodr, int &status) {
+    if (P.n_ism > 0.0) {
+        rho = P.n_ism * mppme;
+        drhodr = 0.0;
+    } else {
+        if (R < P.ism_r_ej) { rho = P.ism_A0 * std::pow(P.ism_r_ej, -P.ism_s) * mppme; drhodr = 0.0; }
+        if ((P.ism_r_ej <= R) && (R < P.ism_r_ism)) {
+            rho = P.ism_A0 * std::pow(R, -P.ism_s) * mppme;
+            drhodr = rho * (-P.ism_s / R);
+        }
+        if (P.ism_r_ism <= R) { rho = P.ism_A0 * std::pow(P.ism_r_ism, -P.ism_s) * mppme; drhodr = 0.0; }
+    }
+    if (rho < 0 || rho > 1.) status |= PBAGPU_S_NONFINITE;
+}
+
 static double dmdr(double R, double dthetadR, double theta, double rho) {
     return 2. * pi * rho * ((1. - std::cos(theta)) + (1. / 3.) * std::sin(theta) * R * dthetadR) * R * R;
 }
@@ -255,8 +274,10 @@ static void rhs_fs(Run &run, BW &b, const double *Y, double *out) {
     const pba_run_params &P = *run.P;
     double R = Y[iR], Gamma = Y[iGamma], Eint2 = Y[iEint2], theta = Y[itheta], M2 = Y[iM2];
     if (Gamma < 1 || M2 < 0 || theta < 0) b.status |= PBAGPU_S_NONFINITE;
-    double rho = b.rho_def / b.M0;
-    double drhodr = 0.0 / b.M0;
+    double rho = 0., drhodr = 0.;
+    ism_density(P, R, rho, drhodr, b.status);
+    rho /= b.M0;
+    drhodr /= b.M0;
     double beta = BetaFromGamma(Gamma);
     double adi = adiNava(Gamma);
     double Gsh = GammaSh(Gamma, adi);
@@ -280,7 +301,7 @@ static void rhs_fs(Run &run, BW &b, const double *Y, double *out) {
     if (P.adiab_loss) dEad2dR = -(adi - 1.0) * Eint2 * dlnV2dR;
     double eps_rad = P.fs.epsilon_e_rad;
     if (P.fs.epsilon_e_rad < 0)
-        eps_rad = radiation_loss(run.fs, b.fs_st, dM2dR * b.M0, dEsh2dR * b.M0 * c * c, R, Gamma, Gsh, adi,
+        eps_rad = radiation_loss(b.fs_sh, b.fs_st, dM2dR * b.M0, dEsh2dR * b.M0 * c * c, R, Gamma, Gsh, adi,
                                  M2 * b.M0, Eint2 * b.M0 * c * c, rho * b.M0, theta, b.ncells);
     double dErad2dR = eps_rad * P.fs.eps_e * dEsh2dR;
     double dEint2dR = dEsh2dR + dEad2dR - dErad2dR;
@@ -317,8 +338,10 @@ static void rhs_fsrs(Run &run, BW &b, const double *Y, double *out) {
     double adi3 = adiNava((double)Gamma43);
     double GammaRsh = GammaSh((double)Gamma43, adi3);
     double dRshRshdt = (double)((long double)BetaFromGamma(GammaRsh) * c);
-    double rho = b.rho_def / b.M0;
-    double drhodr = 0.0 / b.M0;
+    double rho = 0., drhodr = 0.;
+    ism_density(P, R, rho, drhodr, b.status);
+    rho /= b.M0;
+    drhodr /= b.M0;
     double dRdt = beta * c;
     double adi = adiNava(Gamma);
     double Gsh = GammaSh(Gamma, adi);
@@ -381,7 +404,7 @@ static void rhs_fsrs(Run &run, BW &b, const double *Y, double *out) {
     }
     double eps_rad = P.fs.epsilon_e_rad;
     if (P.fs.epsilon_e_rad < 0)
-        eps_rad = radiation_loss(run.fs, b.fs_st, dM2dR * b.M0, dEsh2dR * b.M0 * c * c, R, Gamma, Gsh, adi,
+        eps_rad = radiation_loss(b.fs_sh, b.fs_st, dM2dR * b.M0, dEsh2dR * b.M0 * c * c, R, Gamma, Gsh, adi,
                                  M2 * b.M0, Eint2 * b.M0 * c * c, rho * b.M0, theta, b.ncells);
     double dErad2dR = eps_rad * P.fs.eps_e * dEsh2dR;
     // the reverse-shock loss is called with rho4 = 0 and therefore returns 0 (blastwave.h:1991-1997)
@@ -618,7 +641,10 @@ static void bw_init(Run &run, BW &b, const pba_layer_init &id, double t0_grid, d
     b.fsrs = P.fsrs != 0; b.do_rs = b.fsrs && P.do_rs != 0; b.do_rs_rad = b.fsrs && P.do_rs_radiation != 0;
     b.shutOff = false; b.end_evolution = false; b.end_spreading = false;
     b.prev_idx_x = 0; b.status = 0;
-    b.rho_def = P.n_ism * mppme;
+    // density at the start radius (evaluateRhoDrhoDrDefault(R0), blastwave.h:92-101): the RHS
+    // re-evaluates it at every R, the stored m_rho_ keeps this value
+    b.rho_def = 0.; b.drhodr0 = 0.;
+    ism_density(P, b.R0, b.rho_def, b.drhodr0, b.status);
     // setParamsRS: too slow layers evolve without the reverse shock (blastwave_base.h:437-445)
     if (b.fsrs) {
         double min_g0 = b.do_rs ? P.min_gamma0_for_rs : 0.;
@@ -676,7 +702,7 @@ static void post_step(Run &run, BW &b, const double *Ycode, int it, double tb, d
     // addOtherVars (blastwave.h:525-640)
     V(PV_ctheta) = ctheta_of(V(PV_theta), b.ilayer, b.nlayers);
     V(PV_rho) = b.rho_def;
-    V(PV_drhodr) = 0.;
+    V(PV_drhodr) = b.drhodr0;
     V(PV_adi) = adiNava(V(PV_Gamma));
     V(PV_rho2) = rho2t(V(PV_Gamma), V(PV_adi), V(PV_rho));
     V(PV_GammaFsh) = GammaSh(V(PV_Gamma), V(PV_adi));
@@ -721,7 +747,7 @@ static void post_step(Run &run, BW &b, const double *Ycode, int it, double tb, d
         bool rs = b.fsrs && b.do_rs_rad && (it > 0) && (T[(size_t)PV_thickness_rs * nt + it - 1] > 0) &&
                   (V(PV_GammaRsh) > 0) && (V(PV_U_p3) > 0);
         if (rs) {
-            Lim L = ele_limits(run.rs, b.rs_st, V(PV_U_p3), V(PV_GammaRsh), V(PV_tt), V(PV_tcomov), V(PV_rho3) / mp);
+            Lim L = ele_limits(b.rs_sh, b.rs_st, V(PV_U_p3), V(PV_GammaRsh), V(PV_tt), V(PV_tcomov), V(PV_rho3) / mp);
             if (L.ok) {
                 V(PV_B3) = L.B; V(PV_gm_rs) = L.gm; V(PV_gM_rs) = L.gM; V(PV_gc_rs) = L.gc;
                 V(PV_Theta_rs) = L.Theta; V(PV_z_cool_rs) = L.z_cool; V(PV_nprime_rs) = L.nprime;
@@ -729,7 +755,7 @@ static void post_step(Run &run, BW &b, const double *Ycode, int it, double tb, d
             }
             flags |= 2;
         }
-        Lim L = ele_limits(run.fs, b.fs_st, V(PV_U_p), V(PV_GammaFsh), V(PV_tt), V(PV_tcomov), V(PV_rho2) / mp);
+        Lim L = ele_limits(b.fs_sh, b.fs_st, V(PV_U_p), V(PV_GammaFsh), V(PV_tt), V(PV_tcomov), V(PV_rho2) / mp);
         if (L.ok) {
             V(PV_B) = L.B; V(PV_gm) = L.gm; V(PV_gM) = L.gM; V(PV_gc) = L.gc; V(PV_Theta) = L.Theta;
             V(PV_z_cool) = L.z_cool; V(PV_nprime) = L.nprime; V(PV_accel_frac) = L.acc;
@@ -754,6 +780,21 @@ static void evolve_run(const pba_run_params *P, const pba_layer_init *init, int 
     size_t neq = (size_t)nbw * NEQ;
     std::vector<double> Y(neq, 0.);
     for (int i = 0; i < nbw; i++) {
+        // a blastwave with its own electron-grid size: the grid's first and last points for the
+        // gamma_max clamp (computeGammaMax, microphysics_base.h:824-837)
+        BW &bw = run.bws[i];
+        bw.fs_sh = run.fs;
+        bw.rs_sh = run.rs;
+        if (init[i].ngam_fs > 0) {
+            EleGrid g;
+            g.build(P->fs.gam2, init[i].ngam_fs);
+            bw.fs_sh = Shock{&P->fs, g.e[0], g.e[init[i].ngam_fs - 1]};
+        }
+        if (init[i].ngam_rs > 0 && P->fsrs) {
+            EleGrid g;
+            g.build(P->rs.gam2, init[i].ngam_rs);
+            bw.rs_sh = Shock{&P->rs, g.e[0], g.e[init[i].ngam_rs - 1]};
+        }
         bw_init(run, run.bws[i], init[i], tgrid[0], &Y[(size_t)i * NEQ]);
         run.bws[i].i_end_r = nt;
         std::fill(table + (size_t)i * PBA_NV * nt, table + (size_t)(i + 1) * PBA_NV * nt, 0.);

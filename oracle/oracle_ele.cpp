@@ -27,7 +27,9 @@
 #include "oracle_common.h"
 #include <cstring>
 #include <cstdio>
+#include <map>
 #include <stdexcept>
+#include <tuple>
 
 namespace orc {
 
@@ -170,7 +172,7 @@ static void power_law(const EleGrid &g, double p, double gm, double gc, double g
 }
 
 static void evolve_population(const pba_shock_params &sp, const SSCTab *ssc, int shock, const double *T, int nt,
-                              double *jall, double *aall, double *fall, int *status) {
+                              double *jall, double *aall, double *fall, int fstride, int *status) {
     const int ng = sp.ngam, nf = sp.nfreq;
     EleGrid g;
     g.build(sp.gam2, ng);
@@ -347,7 +349,7 @@ static void evolve_population(const pba_shock_params &sp, const SSCTab *ssc, int
             if (aall && sp.ssa) aall[(size_t)it * nf + i] = syna[i] / n_ele * acc * (V(vrho, it) / mp);
         }
         if (fall)
-            for (int i = 0; i < ng; i++) fall[(size_t)it * ng + i] = f[i];
+            for (int i = 0; i < ng; i++) fall[(size_t)it * fstride + i] = f[i];
     }
 }
 
@@ -358,35 +360,42 @@ extern "C" int orc_ele_evolve(const pba_run_params *params, int n_runs, const pb
                               int *status) {
     if (!params || !pops || !table || !jall || !status || n_pop < 1) return PBAGPU_E_ARG;
     const pba_shock_params &s0 = pops[0].shock ? params[pops[0].run].rs : params[pops[0].run].fs;
-    const int ng = s0.ngam, nf = s0.nfreq;
-    // one SSC table per (run, shock) that needs it
-    std::vector<orc::SSCTab> tabs((size_t)n_runs * 2);
-    std::vector<char> built((size_t)n_runs * 2, 0);
+    const int nf = s0.nfreq;
+    // every population's shock parameters with its own grid size (pba_population ngam, 0: the
+    // shock's); f_all rows have the largest grid size as their stride
+    std::vector<pba_shock_params> sps(n_pop);
+    int NG = 0;
     for (int q = 0; q < n_pop; q++) {
         const pba_population &P = pops[q];
-        if (P.run < 0 || P.run >= n_runs || P.bw < 0 || P.bw >= bw_total) return PBAGPU_E_ARG;
-        const pba_shock_params &sp = P.shock ? params[P.run].rs : params[P.run].fs;
-        if (sp.ngam != ng || sp.nfreq != nf) return PBAGPU_E_ARG;
-        size_t key = (size_t)P.run * 2 + (P.shock ? 1 : 0);
-        if (sp.ssc && !built[key]) {
+        if (P.run < 0 || P.run >= n_runs || P.bw < 0 || P.bw >= bw_total || P.ngam < 0) return PBAGPU_E_ARG;
+        sps[q] = P.shock ? params[P.run].rs : params[P.run].fs;
+        if (P.ngam > 0) sps[q].ngam = P.ngam;
+        if (sps[q].nfreq != nf || sps[q].ngam < 2) return PBAGPU_E_ARG;
+        NG = std::max(NG, sps[q].ngam);
+    }
+    // one SSC table per (run, shock, grid size) that needs it
+    std::map<std::tuple<int, int, int>, orc::SSCTab> tabs;
+    for (int q = 0; q < n_pop; q++) {
+        const pba_shock_params &sp = sps[q];
+        auto key = std::make_tuple(pops[q].run, pops[q].shock, sp.ngam);
+        if (sp.ssc && !tabs.count(key)) {
             orc::EleGrid g;
             g.build(sp.gam2, sp.ngam);
             tabs[key].build(orc::freq_grid(sp), g.e);
-            built[key] = 1;
         }
     }
 #pragma omp parallel for schedule(dynamic)
     for (int q = 0; q < n_pop; q++) {
         const pba_population &P = pops[q];
-        const pba_shock_params &sp = P.shock ? params[P.run].rs : params[P.run].fs;
-        size_t key = (size_t)P.run * 2 + (P.shock ? 1 : 0);
+        const pba_shock_params &sp = sps[q];
         status[q] = 0;
         std::fill(jall + (size_t)q * nt * nf, jall + (size_t)(q + 1) * nt * nf, 0.);
-        if (fall) std::fill(fall + (size_t)q * nt * ng, fall + (size_t)(q + 1) * nt * ng, 0.);
+        if (fall) std::fill(fall + (size_t)q * nt * NG, fall + (size_t)(q + 1) * nt * NG, 0.);
         if (aall) std::fill(aall + (size_t)q * nt * nf, aall + (size_t)(q + 1) * nt * nf, 0.);
-        orc::evolve_population(sp, sp.ssc ? &tabs[key] : nullptr, P.shock, table + (size_t)P.bw * PBA_NV * nt, nt,
+        const orc::SSCTab *tab = sp.ssc ? &tabs.at(std::make_tuple(P.run, P.shock, sp.ngam)) : nullptr;
+        orc::evolve_population(sp, tab, P.shock, table + (size_t)P.bw * PBA_NV * nt, nt,
                                jall + (size_t)q * nt * nf, aall ? aall + (size_t)q * nt * nf : nullptr,
-                               fall ? fall + (size_t)q * nt * ng : nullptr, &status[q]);
+                               fall ? fall + (size_t)q * nt * NG : nullptr, NG, &status[q]);
     }
     return PBAGPU_OK;
 }

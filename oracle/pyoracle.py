@@ -71,7 +71,9 @@ def ele_evolve(params, pops, table, n_runs=1, want_f=False, want_a=False):
     npop = len(pops)
     p0 = params[pops[0].run]
     sh = p0.fs if pops[0].shock == 0 else p0.rs
-    nf, ng = sh.nfreq, sh.ngam
+    nf = sh.nfreq
+    # f_all rows: the largest electron grid of the populations (its own ngam, or its shock's)
+    ng = max(P.ngam if P.ngam > 0 else (params[P.run].rs.ngam if P.shock else params[P.run].fs.ngam) for P in pops)
     tab = np.ascontiguousarray(table)
     jall = np.zeros((npop, nt, nf))
     fall = np.zeros((npop, nt, ng)) if want_f else None

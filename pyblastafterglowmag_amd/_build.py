@@ -67,6 +67,18 @@ def build_gpu():
     return out
 
 
+def build_cabi_driver():
+    """tests/cabi/pba_gpu_run.cpp: a C++ program that calls the C ABI with no Python in the process
+    (the GPU test test_gpu_cabi.py runs it)."""
+    src = os.path.join(ROOT, "tests", "cabi", "pba_gpu_run.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(LIB, "pba_gpu_run")
+    _run([HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), src,
+          "-L" + LIB, "-Wl,-rpath,$ORIGIN", "-lpbagpu", "-o", out])
+    return out
+
+
 ORACLE = os.path.join(ROOT, "oracle")
 
 
@@ -91,6 +103,7 @@ def build_reference():
 def build_all(reference=True):
     build_h5io()
     build_gpu()
+    build_cabi_driver()
     build_oracle()
     if reference:
         build_reference()

@@ -43,7 +43,8 @@ class RunParams(C.Structure):
                 ("rtol", C.c_double), ("tb0", C.c_double),
                 ("tprompt", C.c_double), ("rs_shutoff_rho", C.c_double), ("min_gamma0_for_rs", C.c_double),
                 ("rs_gamma0_frac_no_exceed", C.c_double), ("mom_when_start_spread", C.c_double),
-                ("theta_max", C.c_double), ("rtol_theta", C.c_double), ("rtol_phi", C.c_double),
+                ("theta_max", C.c_double), ("ism_A0", C.c_double), ("ism_s", C.c_double), ("ism_r_ej", C.c_double),
+                ("ism_r_ism", C.c_double), ("rtol_theta", C.c_double), ("rtol_phi", C.c_double),
                 ("nmax_theta", C.c_int), ("nmax_phi", C.c_int), ("fsrs", C.c_int), ("do_rs", C.c_int),
                 ("do_rs_radiation", C.c_int), ("adiab_loss", C.c_int), ("adiab_loss_rs", C.c_int),
                 ("exponential_rho4", C.c_int), ("init_deltaR4", C.c_int), ("counter_jet", C.c_int),
@@ -54,7 +55,8 @@ class LayerInit(C.Structure):
     _fields_ = [("E0", C.c_double), ("M0", C.c_double), ("mom0", C.c_double), ("theta_c_l", C.c_double),
                 ("theta_c_h", C.c_double), ("theta_c", C.c_double), ("ctheta0", C.c_double),
                 ("theta_wing", C.c_double), ("theta_core", C.c_double), ("theta_b0", C.c_double),
-                ("ncells", C.c_double), ("ilayer", C.c_int), ("nlayers", C.c_int)]
+                ("ncells", C.c_double), ("ilayer", C.c_int), ("nlayers", C.c_int), ("ngam_fs", C.c_int),
+                ("ngam_rs", C.c_int)]
 
 
 class BwResult(C.Structure):
@@ -68,7 +70,7 @@ class EatsPwLayer(C.Structure):
 
 
 class Population(C.Structure):
-    _fields_ = [("bw", C.c_int), ("run", C.c_int), ("shock", C.c_int), ("pad_", C.c_int)]
+    _fields_ = [("bw", C.c_int), ("run", C.c_int), ("shock", C.c_int), ("ngam", C.c_int)]
 
 
 PBA_MLP_MAXL = 8
@@ -84,6 +86,16 @@ class Mlp(C.Structure):
 class SscSurrogate(C.Structure):
     _fields_ = [("net", Mlp), ("y_offset", C.c_void_p), ("y_scale", C.c_float), ("check_tol", C.c_float),
                 ("check_every", C.c_int), ("pad_", C.c_int)]
+
+
+class Spectra(C.Structure):
+    """pba_spectra: device pointers of the dense-output spectra (0 = not written)."""
+    _fields_ = [(n, C.c_void_p) for n in ("syn_j", "syn_i", "ssc_j", "total_f", "total_i", "gam_dot_syn", "gam_dot_adi",
+                                          "gam_dot_ssc")]
+
+
+SPECTRA_NU = ("syn_j", "syn_i", "ssc_j", "total_f", "total_i")
+SPECTRA_GAM = ("gam_dot_syn", "gam_dot_adi", "gam_dot_ssc")
 
 
 class SurrogateStats(C.Structure):
@@ -161,12 +173,15 @@ def run_params(mp_, mo, gp, go):
         have = mo.get(key, go.get(key, allowed[0]))
         if have not in allowed:
             raise NotImplementedError(f"{key}={have} (only {' / '.join(allowed)} is implemented)")
-    if mp_.get("n_ism", -1) <= 0:
-        raise NotImplementedError("only a uniform ISM (n_ism > 0) is implemented")
+    if mp_.get("n_ism", -1) <= 0 and (mp_.get("s", -1) < 0 or mp_.get("A0", -1) < 0):
+        # RhoISM::evaluateRhoDrhoDrDefault exits there (blastwave_components.h:857-860)
+        raise ValueError("if n_ism < 0; positive A0 and s are expected (wind)")
     if int(mp_.get("iout", 1)) != 1:
         raise NotImplementedError("iout != 1")
     P = RunParams()
     P.n_ism = mp_["n_ism"]
+    P.ism_A0, P.ism_s = mp_.get("A0", -1.), mp_.get("s", -1.)
+    P.ism_r_ej, P.ism_r_ism = mp_.get("r_ej", -1.), mp_.get("r_ism", -1.)
     P.d_l = mp_["d_l"]
     P.z = mp_["z"]
     P.theta_obs = mp_["theta_obs"]

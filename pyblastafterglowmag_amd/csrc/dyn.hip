@@ -36,7 +36,7 @@ struct MState { double acc, tcomov; };
 struct Shock { double e0, etop, nu0_43; };
 
 struct BW {
-    double E0, M0, R0, Gamma0, beta0, mom0, theta_b0, theta_core, theta_max, rho_def;
+    double E0, M0, R0, Gamma0, beta0, mom0, theta_b0, theta_core, theta_max, rho_def, drhodr0;
     double ncells;     // cells sharing this blastwave: 1 (adaptive EATS) or the total cell count (piece-wise)
     double tan_ratio;  // tan(theta_b0 / 2) / tan(theta_core / 2) of the spreading rate
     double Rw0, Wtt0;  // passive components of the reference state vector (enter the step-size estimate)
@@ -139,11 +139,32 @@ __device__ __forceinline__ double radiation_loss(const pba_shock_params &s, cons
     return eps_rad;
 }
 
+// Ambient density and its radial derivative at radius R (RhoISM::evaluateRhoDrhoDrDefault,
+// src/blastwave/blastwave_components.h:846-893): uniform for n_ism > 0, else constant inside r_ej,
+// A0 R^-s between r_ej and r_ism, constant beyond.  rho > 1 g/cm^3 is where the reference exits.
+__device__ __forceinline__ void ism_density(const pba_run_params &P, double R, double &rho, double &drhodr,
+                                            int &status) {
+    if (P.n_ism > 0.0) {
+        rho = P.n_ism * kMppme;
+        drhodr = 0.0;
+    } else {
+        if (R < P.ism_r_ej) { rho = P.ism_A0 * pow(P.ism_r_ej, -P.ism_s) * kMppme; drhodr = 0.0; }
+        if ((P.ism_r_ej <= R) && (R < P.ism_r_ism)) {
+            rho = P.ism_A0 * pow(R, -P.ism_s) * kMppme;
+            drhodr = rho * (-P.ism_s / R);
+        }
+        if (P.ism_r_ism <= R) { rho = P.ism_A0 * pow(P.ism_r_ism, -P.ism_s) * kMppme; drhodr = 0.0; }
+    }
+    if (rho < 0 || rho > 1.) status |= PBAGPU_S_NONFINITE;
+}
+
 __device__ void rhs_fs(const pba_run_params &P, const Shock &shfs, BW &b, const double *Y, double *o) {
     double R = Y[iR], Gamma = Y[iGamma], Eint2 = Y[iEint2], theta = Y[itheta], M2 = Y[iM2];
     if (Gamma < 1 || M2 < 0 || theta < 0) b.status |= PBAGPU_S_NONFINITE;
-    double rho = b.rho_def / b.M0;
-    double drhodr = 0.0 / b.M0;
+    double rho, drhodr;
+    ism_density(P, R, rho, drhodr, b.status);
+    rho /= b.M0;
+    drhodr /= b.M0;
     double beta = beta_from_gamma(Gamma);
     double adi = adi_nava(Gamma);
     double Gsh = gamma_shock(Gamma, adi);
@@ -200,8 +221,10 @@ __device__ void rhs_fsrs(const pba_run_params &P, const Shock &shfs, BW &b, cons
     if (G43 < 1.) G43 = 1.;
     double adi3 = adi_nava(G43);
     double GRsh = gamma_shock(G43, adi3);
-    double rho = b.rho_def / b.M0;
-    double drhodr = 0.0 / b.M0;
+    double rho, drhodr;
+    ism_density(P, R, rho, drhodr, b.status);
+    rho /= b.M0;
+    drhodr /= b.M0;
     double dRdt = beta * kC;
     double adi = adi_nava(Gamma);
     double Gsh = gamma_shock(Gamma, adi);
@@ -376,8 +399,10 @@ __device__ __forceinline__ void post_step(const pba_run_params &P, const Shock &
     double R = Yc[iR], theta = Yc[itheta];
     if (R < 1. || G < 1.) b.status |= PBAGPU_S_NONFINITE;
     W(PV_ctheta, ctheta_of(theta, b.ilayer, b.nlayers));
+    // addOtherVars stores the density set at the blastwave's start (m_rho_, blastwave.h:92-101,
+    // 539-541), and the downstream quantities are formed from it
     double rho = b.rho_def;
-    W(PV_rho, rho); W(PV_drhodr, 0.);
+    W(PV_rho, rho); W(PV_drhodr, b.drhodr0);
     double adi = adi_nava(G);
     double rho2 = rho2t(G, adi, rho);
     double Gsh = gamma_shock(G, adi);
@@ -505,15 +530,20 @@ __device__ __forceinline__ double ordered_sum(double v, double *lds, int n) {
 // moves the integrator to the state that needs the next evaluation.
 enum Phase { PH_K1I, PH_H2, PH_STAGE, PH_FSAL, PH_RK1, PH_RK2, PH_RK3, PH_RK4 };
 
+// nt is the table's step stride; run r integrates nt_run[r] steps (all nt when nt_run is null) on
+// the time grid tgrid + r * tg_stride (tg_stride 0: one grid for every run).
 __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restrict__ params,
                                                  const pba_layer_init *__restrict__ init, int bw_per_run, int nt,
-                                                 const double *__restrict__ tgrid, double *__restrict__ table,
+                                                 const int *__restrict__ nt_run, const double *__restrict__ tgrid_all,
+                                                 int tg_stride, double *__restrict__ table,
                                                  pba_bw_result *__restrict__ res) {
     extern __shared__ double dyn_lds[];   // [KS][NA][bw_per_run] stage vectors, then 64 doubles for sums
     double *red = dyn_lds + (size_t)KS * NA * bw_per_run;
     const int run = blockIdx.x;
     const int lane = threadIdx.x;
     const bool active = lane < bw_per_run;
+    const int ntr = nt_run ? nt_run[run] : nt;
+    const double *__restrict__ tgrid = tgrid_all + (size_t)run * tg_stride;
     auto K = [&](int slot, int c) -> double & { return dyn_lds[(slot * NA + c) * bw_per_run + lane]; };
     const pba_run_params P = params[run];
     Shock shfs, shrs;
@@ -540,6 +570,16 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
     double *T = active ? table + bw_global * PBA_NV * nt : nullptr;
     if (active) {
         const pba_layer_init id = init[bw_global];
+        // this blastwave's own electron grids (the grid top clamps gamma_max, computeGammaMax,
+        // microphysics_base.h:824-837, which enters the radiative-loss term)
+        if (id.ngam_fs > 0) {
+            const double step = exp(log(P.fs.gam2) / (double)id.ngam_fs);
+            shfs.e0 = pow(step, 0.); shfs.etop = pow(step, (double)(id.ngam_fs - 1));
+        }
+        if (id.ngam_rs > 0) {
+            const double step = exp(log(P.rs.gam2) / (double)id.ngam_rs);
+            shrs.e0 = pow(step, 0.); shrs.etop = pow(step, (double)(id.ngam_rs - 1));
+        }
         b.E0 = id.E0; b.M0 = id.M0; b.mom0 = id.mom0;
         b.Gamma0 = gamma_from_mom(id.mom0);
         b.R0 = beta_from_mom(id.mom0) * kC * tgrid[0];
@@ -547,9 +587,9 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
         b.tan_ratio = tan(0.5 * b.theta_b0) / tan(0.5 * b.theta_core);
         b.ilayer = id.ilayer; b.nlayers = id.nlayers;
         b.fsrs = P.fsrs; b.do_rs = P.fsrs && P.do_rs; b.do_rs_rad = P.fsrs && P.do_rs_radiation;
-        b.rho_def = P.n_ism * kMppme;
+        ism_density(P, b.R0, b.rho_def, b.drhodr0, b.status);   // at R0 (blastwave.h:92)
         b.fs_st.acc = 1.; b.fs_st.tcomov = -1.; b.rs_st.acc = 1.; b.rs_st.tcomov = -1.;
-        b.i_end_r = nt;
+        b.i_end_r = ntr;
         if (b.fsrs) {
             double min_g0 = b.do_rs ? P.min_gamma0_for_rs : 0.;
             if (b.Gamma0 < min_g0) { b.do_rs = 0; b.do_rs_rad = 0; b.shutOff = 1; b.fsrs = 0; }
@@ -579,7 +619,7 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
     const double nsz = (double)(NEQ_REF * bw_per_run);
     double time = P.tb0;
     int nsteps = 0, nrhs = 0;
-    for (int it = 1; it < nt; it++) {
+    for (int it = 1; it < ntr; it++) {
         const double timeN = time + (tgrid[it] - tgrid[it - 1]);
         const double hmax = fabs(timeN - time);
         const double hrk = tgrid[it] - tgrid[it - 1];
@@ -771,8 +811,9 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
 }  // namespace dyn
 }  // namespace pbagpu
 
-extern "C" int pbagpu_dyn_evolve(const pba_run_params *params, int n_runs, int bw_per_run, const pba_layer_init *init,
-                                 int nt, const double *d_tgrid, double *d_table, pba_bw_result *res, void *stream) {
+static int dyn_evolve_impl(const pba_run_params *params, int n_runs, int bw_per_run, const pba_layer_init *init,
+                           int nt, const int *nt_run, const double *d_tgrid, int tg_stride, double *d_table,
+                           pba_bw_result *res, void *stream) {
     if (!params || !init || !d_tgrid || !d_table || !res || n_runs < 1 || bw_per_run < 1 || bw_per_run > 64 || nt < 2)
         return PBAGPU_E_ARG;
     hipStream_t s = (hipStream_t)stream;
@@ -781,6 +822,12 @@ extern "C" int pbagpu_dyn_evolve(const pba_run_params *params, int n_runs, int b
             pbagpu::set_error_msg("pba_run_params: integrator must be 0 (DOP853E) or 1 (RK4)");
             return PBAGPU_E_UNSUPPORTED;
         }
+    if (nt_run)
+        for (int r = 0; r < n_runs; r++)
+            if (nt_run[r] < 2 || nt_run[r] > nt) {
+                pbagpu::set_error_msg("dyn_evolve_ragged: every run needs 2 <= nt_run <= nt_max steps");
+                return PBAGPU_E_ARG;
+            }
     const size_t nbw = (size_t)n_runs * bw_per_run;
     for (size_t i = 0; i < nbw; i++) {
         // ncells < 1 is an error of the reference too (blastwave.h:175-181)
@@ -788,22 +835,28 @@ extern "C" int pbagpu_dyn_evolve(const pba_run_params *params, int n_runs, int b
             pbagpu::set_error_msg("pba_layer_init: ncells must be >= 1 and theta_b0 > 0");
             return PBAGPU_E_ARG;
         }
+        if (init[i].ngam_fs < 0 || init[i].ngam_fs == 1 || init[i].ngam_rs < 0 || init[i].ngam_rs == 1) {
+            pbagpu::set_error_msg("pba_layer_init: ngam_fs / ngam_rs must be 0 (the run's) or >= 2");
+            return PBAGPU_E_ARG;
+        }
     }
     auto *dP = (pba_run_params *)pbagpu::workspace(pbagpu::WS_DYN_PARAMS, sizeof(pba_run_params) * n_runs, s);
     auto *dI = (pba_layer_init *)pbagpu::workspace(pbagpu::WS_DYN_INIT, sizeof(pba_layer_init) * nbw, s);
     auto *dR = (pba_bw_result *)pbagpu::workspace(pbagpu::WS_DYN_RES, sizeof(pba_bw_result) * nbw, s);
-    if (!dP || !dI || !dR) return PBAGPU_E_HIP;
+    int *dN = nt_run ? (int *)pbagpu::workspace(pbagpu::WS_DYN_NT, sizeof(int) * n_runs, s) : nullptr;
+    if (!dP || !dI || !dR || (nt_run && !dN)) return PBAGPU_E_HIP;
     int rc = PBAGPU_OK;
     PBA_TRY(rc, hipMemcpyAsync(dP, params, sizeof(pba_run_params) * n_runs, hipMemcpyHostToDevice, s));
     PBA_TRY(rc, hipMemcpyAsync(dI, init, sizeof(pba_layer_init) * nbw, hipMemcpyHostToDevice, s));
+    if (nt_run) PBA_TRY(rc, hipMemcpyAsync(dN, nt_run, sizeof(int) * n_runs, hipMemcpyHostToDevice, s));
     pbagpu::KernelTimer timer(PBAGPU_STAGE_DYN);
     if (rc == PBAGPU_OK) {
         timer.start(s);
         const size_t lds = sizeof(double) * ((size_t)pbagpu::dyn::KS * pbagpu::dyn::NA * bw_per_run + 64);
         (void)hipFuncSetAttribute((const void *)pbagpu::dyn::dyn_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds);
-        hipLaunchKernelGGL(pbagpu::dyn::dyn_kernel, dim3(n_runs), dim3(64), lds, s, dP, dI, bw_per_run, nt, d_tgrid,
-                           d_table, dR);
+        hipLaunchKernelGGL(pbagpu::dyn::dyn_kernel, dim3(n_runs), dim3(64), lds, s, dP, dI, bw_per_run, nt,
+                           (const int *)dN, d_tgrid, tg_stride, d_table, dR);
         PBA_TRY(rc, hipGetLastError());
         timer.stop(s);
     }
@@ -811,4 +864,16 @@ extern "C" int pbagpu_dyn_evolve(const pba_run_params *params, int n_runs, int b
     PBA_TRY(rc, hipStreamSynchronize(s));
     timer.collect();
     return rc;
+}
+
+extern "C" int pbagpu_dyn_evolve(const pba_run_params *params, int n_runs, int bw_per_run, const pba_layer_init *init,
+                                 int nt, const double *d_tgrid, double *d_table, pba_bw_result *res, void *stream) {
+    return dyn_evolve_impl(params, n_runs, bw_per_run, init, nt, nullptr, d_tgrid, 0, d_table, res, stream);
+}
+
+extern "C" int pbagpu_dyn_evolve_ragged(const pba_run_params *params, int n_runs, int bw_per_run,
+                                        const pba_layer_init *init, int nt_max, const int *nt_run,
+                                        const double *d_tgrid, double *d_table, pba_bw_result *res, void *stream) {
+    if (!nt_run) return PBAGPU_E_ARG;
+    return dyn_evolve_impl(params, n_runs, bw_per_run, init, nt_max, nt_run, d_tgrid, nt_max, d_table, res, stream);
 }

@@ -151,6 +151,8 @@ struct PopDesc {
     double *jall;         // [nt][nf]
     double *aall;         // [nt][nf] absorption (use_ssa) or null
     double *fall;         // [nt][ng] or null
+    // spectra of the dense output ([nt][nf], gam_dot [nt][ng]) or null (pba_spectra)
+    double *sp_synj, *sp_syni, *sp_sscj, *sp_totf, *sp_toti, *sp_gds, *sp_gda, *sp_gdc;
     double p, eps_e;
     BesselThird bes;      // Temme-series constants of the Bessel kernel (syn = 2)
     int shock, ng, nf, max_substeps, adi_loss, ssc, ssa, syn;
@@ -177,8 +179,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Per-population state between the per-step launches (device workspace).
 struct PopState {
-    double *f;     // [n_pop][ng]   electron spectrum
-    double *ft;    // [ng][n_pop]   the same, transposed for the Compton contraction
+    double *f;     // [n_pop][ngs]  electron spectrum (a population's first ng entries)
+    double *ft;    // [ngs][n_pop]  the same, transposed for the Compton contraction
     double *tf;    // [n_pop][nf]   photon density of the previous step (total_rad.f)
     double *wt;    // [nf][n_pop]   Compton photon weights of this step
     double *jsyn;  // [n_pop][nf]
@@ -187,6 +189,10 @@ struct PopState {
     double *dfde;  // [n_pop][ng]   d(f/gamma^2)/dgamma of the last computed step (computeSSA's
                    //               ele.dfde: bins where f = 0 keep their earlier value)
     int *flag;     // [n_pop][3]    initialised, active at this step, status bits
+    double *gdot;  // [n_pop][3][ngs] cooling rates (syn, adi, ssc) of the last Chang-Cooper step, or
+                   //               null (only kept for the spectra of the dense output)
+    int ngs;       // row stride of the per-population electron arrays (f, dfde, gdot) and of the
+                   // f_all / gam_dot outputs: the largest grid of the call
 };
 
 // Step `it` of every population, part 1: injection / Chang-Cooper update and the
@@ -222,7 +228,7 @@ __global__ void __launch_bounds__(64) ele_step_kernel(const PopDesc *__restrict_
     double f[MPL], src[MPL], bb[MPL], cp[MPL];
     for (int r = 0; r < MPL; r++) {
         int j = j0 + r;
-        f[r] = (r < m && j < ng) ? S.f[(size_t)q0 * ng + j] : 0.;
+        f[r] = (r < m && j < ng) ? S.f[(size_t)q0 * S.ngs + j] : 0.;
         src[r] = 0.; bb[r] = 1.; cp[r] = 0.;
     }
     for (int j = lane; j < ng; j += 64) s_dge[j] = (j < ng - 1) ? D.e[j + 1] - D.e[j] : 0.;
@@ -297,7 +303,12 @@ __global__ void __launch_bounds__(64) ele_step_kernel(const PopDesc *__restrict_
                             for (int k = 0; k < nf - 1; k++) res += s_w[k] * KIr[k];
                             gssc = csc * res / ge / ge;
                         }
-                        hv = cf * B * B * ge * ge + dlnVdt * (ge * ge - 1.) / (3. * ge) + gssc;
+                        const double gsyn = cf * B * B * ge * ge, gadi = dlnVdt * (ge * ge - 1.) / (3. * ge);
+                        hv = gsyn + gadi + gssc;
+                        if (S.gdot && r >= 0) {
+                            double *G = S.gdot + (size_t)q0 * 3 * S.ngs;
+                            G[j] = gsyn; G[S.ngs + j] = gadi; G[2 * S.ngs + j] = gssc;
+                        }
                     }
                     heat[r + 1] = hv;
                 }
@@ -352,7 +363,7 @@ __global__ void __launch_bounds__(64) ele_step_kernel(const PopDesc *__restrict_
             if (j < ng) {
                 s_f[j] = f[r];
                 if (j < ng - 1) ne_part += f[r] * s_dge[j];
-                S.f[(size_t)q0 * ng + j] = f[r];
+                S.f[(size_t)q0 * S.ngs + j] = f[r];
                 S.ft[(size_t)j * n_pop + q0] = f[r];
             }
         }
@@ -369,7 +380,7 @@ __global__ void __launch_bounds__(64) ele_step_kernel(const PopDesc *__restrict_
             // second-order stencils at the ends (not divided by a spacing); bins with f = 0 keep
             // the value of the last step that set them
             auto tmp = [&](int j) { return s_f[j] / D.e[j] / D.e[j]; };
-            double *dfde = S.dfde + (size_t)q0 * ng;
+            double *dfde = S.dfde + (size_t)q0 * S.ngs;
             for (int j = lane; j < ng; j += 64) {
                 const double tj = tmp(j);
                 if (tj > 0) {
@@ -409,8 +420,17 @@ __global__ void __launch_bounds__(64) ele_step_kernel(const PopDesc *__restrict_
         if (D.fall)
             for (int r = 0; r < m; r++) {
                 int j = j0 + r;
-                if (j < ng) D.fall[(size_t)it * ng + j] = f[r];
+                if (j < ng) D.fall[(size_t)it * S.ngs + j] = f[r];
             }
+        // cooling rates of the last Chang-Cooper step (Electrons::save_to_all, base_state.h:200-211)
+        if (D.sp_gds && S.gdot) {
+            const double *G = S.gdot + (size_t)q0 * 3 * S.ngs;
+            for (int j = lane; j < ng; j += 64) {
+                D.sp_gds[(size_t)it * S.ngs + j] = G[j];
+                D.sp_gda[(size_t)it * S.ngs + j] = G[S.ngs + j];
+                D.sp_gdc[(size_t)it * S.ngs + j] = G[2 * S.ngs + j];
+            }
+        }
         for (int d = 32; d >= 1; d >>= 1) st |= __shfl_xor(st, d, 64);
         if (lane == 0) {
             FL[0] = 1; FL[1] = 1; FL[2] |= st;
@@ -687,6 +707,8 @@ __global__ void __launch_bounds__(64) sur_apply_kernel(const int *__restrict__ p
             for (int jb = 0; jb < SSC_JB; jb++) ex += J[(size_t)i * SSC_JB + jb];
             const double sv = sc > 0. ? sc * exp10((double)y_off[i] + (double)y_scale * (double)y[(size_t)q * nf + i]) : 0.;
             if (fabs(ex) > 1e-6 * peak) err = fmax(err, fabs(sv - ex) / fabs(ex));
+            // an all-zero plain emissivity is met only by an all-zero prediction
+            if (peak == 0. && sv != 0.) err = INFINITY;
         }
         for (int d = 32; d >= 1; d >>= 1) err = fmax(err, __shfl_xor(err, d, 64));
         if (lane == 0) {
@@ -731,6 +753,15 @@ __global__ void ele_status_kernel(const int *__restrict__ flag, int n_pop, int *
     if (q < n_pop) status[q] = flag[3 * q + 2];
 }
 
+// computeIntensity with the iAPPROX rule (src/microphysics/microphysics.h:11-58): Dermer (2009)'s
+// u(tau) for a depth above 1e-3, the emissivity itself below
+__device__ __forceinline__ double intensity_approx(double em, double dtau) {
+    if (dtau == 0) return em;
+    const double e = exp(-dtau);
+    const double u = 1. / 2. + e / dtau - (1 - e) / (dtau * dtau);
+    return dtau > 1.e-3 ? em * (3. * u / dtau) : em;
+}
+
 // Step `it`, part 3: total emissivity per electron, photon density for the next step
 // (computeRadiationNumericMixed, src/microphysics/microphysics.h:492-599).
 __global__ void __launch_bounds__(64) ele_emit_kernel(const PopDesc *__restrict__ pops, int it, int nt, PopState S) {
@@ -749,10 +780,22 @@ __global__ void __launch_bounds__(64) ele_emit_kernel(const PopDesc *__restrict_
         double sscj = 0.;
         if (D.ssc)
             for (int jb = 0; jb < SSC_JB; jb++) sscj += S.jssc[((size_t)q0 * nf + i) * SSC_JB + jb];
-        S.tf[(size_t)q0 * nf + i] = synj * fac / (kH * nui) + (D.ssc ? sscj * fac / (kH * nui) : 0.);
+        const double tf = synj * fac / (kH * nui) + (D.ssc ? sscj * fac / (kH * nui) : 0.);
+        S.tf[(size_t)q0 * nf + i] = tf;
         double jt = (synj + sscj) * acc / n_ele;
         if (!isfinite(jt) || jt < 0.) st |= PBAGPU_S_NONFINITE;
         D.jall[(size_t)it * nf + i] = jt;
+        if (D.sp_synj) {
+            // storeDenseOutput (microphysics.h:602-633) and total_rad's intensity (:580-583):
+            // the depth is the normalised absorption times the comoving shell thickness
+            const size_t o = (size_t)it * nf + i;
+            const double dtau = D.aall ? D.aall[o] * SC[0] : 0.;
+            D.sp_synj[o] = synj * acc;
+            D.sp_syni[o] = intensity_approx(synj * acc, dtau);
+            D.sp_sscj[o] = sscj * acc;
+            D.sp_totf[o] = tf;
+            D.sp_toti[o] = intensity_approx((synj + sscj) * acc, dtau);
+        }
     }
     for (int d = 32; d >= 1; d >>= 1) st |= __shfl_xor(st, d, 64);
     if (lane == 0 && st) S.flag[3 * q0 + 2] |= st;
@@ -788,92 +831,129 @@ static void build_grids(const pba_shock_params &s, GridSet &G) {
 static int ele_evolve_impl(const pba_run_params *params, int n_runs, const pba_population *pops, int n_pop,
                            const double *d_table, int bw_total, int nt, double *d_jall, double *d_aall,
                            double *d_fall, int *d_status, int precision, const pba_ssc_surrogate *sur,
-                           pba_surrogate_stats *stats, void *stream) {
+                           pba_surrogate_stats *stats, const pba_spectra *spec, void *stream) {
     using namespace pbagpu::ele;
     if (!params || !pops || !d_table || !d_jall || !d_status || n_pop < 1 || nt < 1 || n_runs < 1) return PBAGPU_E_ARG;
     if (pops[0].run < 0 || pops[0].run >= n_runs) return PBAGPU_E_ARG;
     if (precision < 0 || precision > 2 || (precision == 2) != (sur != nullptr)) return PBAGPU_E_UNSUPPORTED;
     hipStream_t s = (hipStream_t)stream;
+    // spectra: the three cooling-rate rows go together (one kept state per population), and so do
+    // the five emission rows
+    const bool want_gdot = spec && spec->gam_dot_syn && spec->gam_dot_adi && spec->gam_dot_ssc;
+    const bool want_emit = spec && spec->syn_j && spec->syn_i && spec->ssc_j && spec->total_f && spec->total_i;
+    if (spec && ((!want_gdot && (spec->gam_dot_syn || spec->gam_dot_adi || spec->gam_dot_ssc)) ||
+                 (!want_emit && (spec->syn_j || spec->syn_i || spec->ssc_j || spec->total_f || spec->total_i)))) {
+        pbagpu::set_error_msg("ele_evolve_spectra: give all three gam_dot arrays or none, and all five spectra or none");
+        return PBAGPU_E_ARG;
+    }
     const pba_shock_params &s0 = pops[0].shock ? params[pops[0].run].rs : params[pops[0].run].fs;
-    const int ng = s0.ngam, nf = s0.nfreq;
-    if (ng < 2 || ng > MAXG || nf < 2 || nf > MAXF) return PBAGPU_E_ARG;
+    const int nf = s0.nfreq;
+    if (nf < 2 || nf > MAXF) return PBAGPU_E_ARG;
+    // every population's own electron grid size (its override or its shock's); NG = the largest
+    auto pop_ng = [&](int q) {
+        const pba_population &P = pops[q];
+        const pba_shock_params &sp = P.shock ? params[P.run].rs : params[P.run].fs;
+        return P.ngam > 0 ? P.ngam : sp.ngam;
+    };
     const double h_by_mec2 = pbagpu::kH / pbagpu::kMe / pbagpu::kC / pbagpu::kC;
     // one grid set (and Compton table if any population of that grid needs it) per distinct
-    // grid (gam2, freq1, freq2); runs of a batch usually share one
-    struct Key { double gam2, freq1, freq2; };
+    // grid (gam2, freq1, freq2, ngam); runs of a batch usually share one
+    struct Key { double gam2, freq1, freq2; int ng; };
     std::vector<Key> keys;
     std::vector<int> key_of(n_pop);
     std::vector<char> need_ssc;
+    int NG = 0, ng_min = MAXG + 1;
     for (int q = 0; q < n_pop; q++) {
         const pba_population &P = pops[q];
         if (P.run < 0 || P.run >= n_runs || P.bw < 0 || P.bw >= bw_total || (P.shock != 0 && P.shock != 1))
             return PBAGPU_E_ARG;
         const pba_shock_params &sp = P.shock ? params[P.run].rs : params[P.run].fs;
-        if (sp.ngam != ng || sp.nfreq != nf) return PBAGPU_E_ARG;
+        const int ngq = pop_ng(q);
+        if (ngq < 2 || ngq > MAXG || P.ngam < 0 || sp.nfreq != nf) {
+            pbagpu::set_error_msg("ele_evolve: a population's ngam must be 2..512 and nfreq equal in one call");
+            return PBAGPU_E_ARG;
+        }
+        NG = std::max(NG, ngq);
+        ng_min = std::min(ng_min, ngq);
         if (sp.syn < 0 || sp.syn > 3) return PBAGPU_E_UNSUPPORTED;
         if (sp.ssa && !d_aall) {
             pbagpu::set_error_msg("ele_evolve: a shock with ssa = 1 needs the absorption table d_aall");
             return PBAGPU_E_ARG;
         }
         size_t pos = 0;
-        while (pos < keys.size() &&
-               !(keys[pos].gam2 == sp.gam2 && keys[pos].freq1 == sp.freq1 && keys[pos].freq2 == sp.freq2))
+        while (pos < keys.size() && !(keys[pos].gam2 == sp.gam2 && keys[pos].freq1 == sp.freq1 &&
+                                      keys[pos].freq2 == sp.freq2 && keys[pos].ng == ngq))
             pos++;
-        if (pos == keys.size()) { keys.push_back({sp.gam2, sp.freq1, sp.freq2}); need_ssc.push_back(0); }
+        if (pos == keys.size()) { keys.push_back({sp.gam2, sp.freq1, sp.freq2, ngq}); need_ssc.push_back(0); }
         if (sp.ssc) need_ssc[pos] = 1;
         key_of[q] = (int)pos;
     }
-    const size_t ktab = (size_t)nf * ng * nf;
-    const size_t per_key = (size_t)ng + (ng - 1) + (ng + 1) + ng + nf;
-    std::vector<double> hgrids(per_key * keys.size());
+    // grids of key kk at dgrids + goff[kk]: e [ng], half_e [ng-1], delta_grid [ng+1],
+    // delta_grid_bar [ng], nu [nf]
+    std::vector<size_t> goff(keys.size() + 1, 0);
+    for (size_t kk = 0; kk < keys.size(); kk++) goff[kk + 1] = goff[kk] + 4 * (size_t)keys[kk].ng + nf;
+    auto nu_of = [&](const double *base, int ngk) { return base + 4 * (size_t)ngk; };
+    std::vector<double> hgrids(goff[keys.size()]);
     for (size_t kk = 0; kk < keys.size(); kk++) {
+        const int ngk = keys[kk].ng;
         pba_shock_params sp = s0;
-        sp.gam2 = keys[kk].gam2; sp.freq1 = keys[kk].freq1; sp.freq2 = keys[kk].freq2;
+        sp.gam2 = keys[kk].gam2; sp.freq1 = keys[kk].freq1; sp.freq2 = keys[kk].freq2; sp.ngam = ngk;
         GridSet G;
         build_grids(sp, G);
-        double *o = &hgrids[per_key * kk];
-        std::copy(G.e.begin(), G.e.end(), o); o += ng;
-        std::copy(G.half_e.begin(), G.half_e.end(), o); o += ng - 1;
-        std::copy(G.dg.begin(), G.dg.end(), o); o += ng + 1;
-        std::copy(G.dgbar.begin(), G.dgbar.end(), o); o += ng;
+        double *o = &hgrids[goff[kk]];
+        std::copy(G.e.begin(), G.e.end(), o); o += ngk;
+        std::copy(G.half_e.begin(), G.half_e.end(), o); o += ngk - 1;
+        std::copy(G.dg.begin(), G.dg.end(), o); o += ngk + 1;
+        std::copy(G.dgbar.begin(), G.dgbar.end(), o); o += ngk;
         std::copy(G.nu.begin(), G.nu.end(), o);
     }
+    // Compton tables: slot t holds K [nf][ng][nf] then its integral [ng][nf], at dK + koff[t]
     size_t n_ssc = 0;
-    std::vector<size_t> ssc_slot(keys.size(), 0);
-    for (size_t kk = 0; kk < keys.size(); kk++) if (need_ssc[kk]) ssc_slot[kk] = n_ssc++;
+    std::vector<size_t> ssc_slot(keys.size(), 0), slot_key;
+    for (size_t kk = 0; kk < keys.size(); kk++)
+        if (need_ssc[kk]) { ssc_slot[kk] = n_ssc++; slot_key.push_back(kk); }
+    std::vector<size_t> koff(n_ssc + 1, 0);
+    for (size_t t = 0; t < n_ssc; t++) {
+        const size_t ngk = keys[slot_key[t]].ng;
+        koff[t + 1] = koff[t] + (size_t)nf * ngk * nf + ngk * nf;
+    }
     auto *dgrids = (double *)pbagpu::workspace(pbagpu::WS_ELE_GRIDS, sizeof(double) * hgrids.size(), s);
-    auto *dK = n_ssc ? (double *)pbagpu::workspace(pbagpu::WS_ELE_KTAB, sizeof(double) * (ktab + (size_t)ng * nf) * n_ssc, s)
-                     : nullptr;
+    auto *dK = n_ssc ? (double *)pbagpu::workspace(pbagpu::WS_ELE_KTAB, sizeof(double) * koff[n_ssc], s) : nullptr;
     auto *dpops = (PopDesc *)pbagpu::workspace(pbagpu::WS_ELE_POPS, sizeof(PopDesc) * n_pop, s);
     if (!dgrids || (n_ssc && !dK) || !dpops) return PBAGPU_E_HIP;
     int rc = PBAGPU_OK;
     PBA_TRY(rc, hipMemcpyAsync(dgrids, hgrids.data(), sizeof(double) * hgrids.size(), hipMemcpyHostToDevice, s));
     pbagpu::KernelTimer ttab(PBAGPU_STAGE_SSC_TABLE), tele(PBAGPU_STAGE_ELE);
     if (n_ssc) ttab.start(s);
-    for (size_t kk = 0; kk < keys.size() && rc == PBAGPU_OK; kk++) {
-        if (!need_ssc[kk]) continue;
-        double *base = dgrids + per_key * kk;
-        double *K = dK + (ktab + (size_t)ng * nf) * ssc_slot[kk];
+    for (size_t t = 0; t < n_ssc && rc == PBAGPU_OK; t++) {
+        const size_t kk = slot_key[t];
+        const int ngk = keys[kk].ng;
+        const size_t ktab = (size_t)nf * ngk * nf;
+        const double *base = dgrids + goff[kk];
+        double *K = dK + koff[t];
         double *KI = K + ktab;
-        const double *e = base, *nu = base + ng + (ng - 1) + (ng + 1) + ng;
-        hipLaunchKernelGGL(ssc_table_kernel, dim3((unsigned)((ktab + 255) / 256)), dim3(256), 0, s, nu, e, nf, ng,
+        const double *e = base, *nu = nu_of(base, ngk);
+        hipLaunchKernelGGL(ssc_table_kernel, dim3((unsigned)((ktab + 255) / 256)), dim3(256), 0, s, nu, e, nf, ngk,
                            h_by_mec2, K);
         PBA_TRY(rc, hipGetLastError());
-        hipLaunchKernelGGL(ssc_integral_kernel, dim3((unsigned)((ng * nf + 255) / 256)), dim3(256), 0, s, nu, K, nf, ng,
-                           KI);
+        hipLaunchKernelGGL(ssc_integral_kernel, dim3((unsigned)((ngk * nf + 255) / 256)), dim3(256), 0, s, nu, K, nf,
+                           ngk, KI);
         PBA_TRY(rc, hipGetLastError());
     }
-    // BF16 pair of every Compton table for the mixed-precision contraction
+    // BF16 pair of every Compton table for the mixed-precision contraction: slot t's hi part at
+    // dKb + kboff[t], its lo part nf * ng * kp entries after it
     const int kp = (nf + 15) / 16 * 16;
-    const size_t kbf = (size_t)nf * ng * kp;
+    std::vector<size_t> kboff(n_ssc + 1, 0);
+    for (size_t t = 0; t < n_ssc; t++) kboff[t + 1] = kboff[t] + 2 * (size_t)nf * keys[slot_key[t]].ng * kp;
     unsigned short *dKb = nullptr;
     if (precision == 1 && n_ssc) {
-        dKb = (unsigned short *)pbagpu::workspace(pbagpu::WS_ELE_KBF, sizeof(unsigned short) * 2 * kbf * n_ssc, s);
+        dKb = (unsigned short *)pbagpu::workspace(pbagpu::WS_ELE_KBF, sizeof(unsigned short) * kboff[n_ssc], s);
         if (!dKb) return PBAGPU_E_HIP;
         for (size_t t = 0; t < n_ssc && rc == PBAGPU_OK; t++) {
-            const double *K = dK + (ktab + (size_t)ng * nf) * t;
-            hipLaunchKernelGGL(ssc_table_bf16_kernel, dim3((unsigned)((kbf + 255) / 256)), dim3(256), 0, s, K, nf, ng, kp,
-                               dKb + 2 * kbf * t, dKb + 2 * kbf * t + kbf);
+            const int ngk = keys[slot_key[t]].ng;
+            const size_t kbf = (size_t)nf * ngk * kp;
+            hipLaunchKernelGGL(ssc_table_bf16_kernel, dim3((unsigned)((kbf + 255) / 256)), dim3(256), 0, s,
+                               (const double *)(dK + koff[t]), nf, ngk, kp, dKb + kboff[t], dKb + kboff[t] + kbf);
             PBA_TRY(rc, hipGetLastError());
         }
     }
@@ -882,29 +962,45 @@ static int ele_evolve_impl(const pba_run_params *params, int n_runs, const pba_p
     for (int q = 0; q < n_pop; q++) {
         const pba_population &P = pops[q];
         const pba_shock_params &sp = P.shock ? params[P.run].rs : params[P.run].fs;
-        int kk = key_of[q];
-        double *base = dgrids + per_key * kk;
+        const int kk = key_of[q];
+        const int ng = keys[kk].ng;
+        double *base = dgrids + goff[kk];
         PopDesc &D = hp[q];
         D.T = d_table + (size_t)P.bw * PBA_NV * nt;
         D.e = base; D.half_e = base + ng; D.dg = base + ng + (ng - 1); D.dgbar = D.dg + (ng + 1); D.nu = D.dgbar + ng;
-        D.K = (sp.ssc && need_ssc[kk]) ? dK + (ktab + (size_t)ng * nf) * ssc_slot[kk] : nullptr;
-        D.KI = D.K ? D.K + ktab : nullptr;
+        D.K = (sp.ssc && need_ssc[kk]) ? dK + koff[ssc_slot[kk]] : nullptr;
+        D.KI = D.K ? D.K + (size_t)nf * ng * nf : nullptr;
         D.jall = d_jall + (size_t)q * nt * nf;
         D.aall = sp.ssa ? d_aall + (size_t)q * nt * nf : nullptr;
         D.ssa = sp.ssa ? 1 : 0;
         D.syn = sp.syn;
         D.bes = pbagpu::bessel_third_constants();
-        D.fall = d_fall ? d_fall + (size_t)q * nt * ng : nullptr;
+        D.fall = d_fall ? d_fall + (size_t)q * nt * NG : nullptr;
+        auto rows = [&](double *base, int n) { return base ? base + (size_t)q * nt * n : nullptr; };
+        D.sp_synj = want_emit ? rows(spec->syn_j, nf) : nullptr;
+        D.sp_syni = want_emit ? rows(spec->syn_i, nf) : nullptr;
+        D.sp_sscj = want_emit ? rows(spec->ssc_j, nf) : nullptr;
+        D.sp_totf = want_emit ? rows(spec->total_f, nf) : nullptr;
+        D.sp_toti = want_emit ? rows(spec->total_i, nf) : nullptr;
+        D.sp_gds = want_gdot ? rows(spec->gam_dot_syn, NG) : nullptr;
+        D.sp_gda = want_gdot ? rows(spec->gam_dot_adi, NG) : nullptr;
+        D.sp_gdc = want_gdot ? rows(spec->gam_dot_ssc, NG) : nullptr;
         D.p = sp.p; D.eps_e = sp.eps_e; D.shock = P.shock; D.ng = ng; D.nf = nf;
         D.max_substeps = sp.max_substeps; D.adi_loss = sp.adi_loss_ele; D.ssc = D.K != nullptr;
     }
     PBA_TRY(rc, hipMemcpyAsync(dpops, hp.data(), sizeof(PopDesc) * n_pop, hipMemcpyHostToDevice, s));
     PBA_TRY(rc, hipMemsetAsync(d_jall, 0, sizeof(double) * (size_t)n_pop * nt * nf, s));
     if (d_aall) PBA_TRY(rc, hipMemsetAsync(d_aall, 0, sizeof(double) * (size_t)n_pop * nt * nf, s));
-    if (d_fall) PBA_TRY(rc, hipMemsetAsync(d_fall, 0, sizeof(double) * (size_t)n_pop * nt * ng, s));
+    if (d_fall) PBA_TRY(rc, hipMemsetAsync(d_fall, 0, sizeof(double) * (size_t)n_pop * nt * NG, s));
+    if (spec) {
+        for (double *a : {spec->syn_j, spec->syn_i, spec->ssc_j, spec->total_f, spec->total_i})
+            if (a) PBA_TRY(rc, hipMemsetAsync(a, 0, sizeof(double) * (size_t)n_pop * nt * nf, s));
+        for (double *a : {spec->gam_dot_syn, spec->gam_dot_adi, spec->gam_dot_ssc})
+            if (a) PBA_TRY(rc, hipMemsetAsync(a, 0, sizeof(double) * (size_t)n_pop * nt * NG, s));
+    }
     // per-population state between the per-step launches
     const size_t np = (size_t)n_pop;
-    const size_t n_dbl = np * ng * 3 + np * nf * (3 + SSC_JB) + np * 4;
+    const size_t n_dbl = np * NG * 3 + np * nf * (3 + SSC_JB) + np * 4 + (want_gdot ? np * 3 * NG : 0);
     double *wsd = (double *)pbagpu::workspace(pbagpu::WS_ELE_WORK, sizeof(double) * n_dbl + sizeof(int) * np * 3, s);
     // populations of each Compton table, for the batched contraction
     std::vector<int> hlist;
@@ -924,15 +1020,15 @@ static int ele_evolve_impl(const pba_run_params *params, int n_runs, const pba_p
     if (precision == 1 && !hlist.empty()) {
         const size_t nl = hlist.size();
         char *mw = (char *)pbagpu::workspace(pbagpu::WS_ELE_MIXED,
-                                             nl * (sizeof(float) * (ng + kp) + sizeof(double) * (SSC_JB + 1)), s);
+                                             nl * (sizeof(float) * (NG + kp) + sizeof(double) * (SSC_JB + 1)), s);
         if (!mw) return PBAGPU_E_HIP;
         d_amax = (double *)mw;
         d_wmax = d_amax + nl * SSC_JB;
         d_an = (float *)(d_wmax + nl);
-        d_wn = d_an + nl * ng;
+        d_wn = d_an + nl * NG;
     }
     // surrogate: features, outputs and scales of the listed populations, fallback flags, counters
-    const int dx = ng + nf + 1;
+    const int dx = NG + nf + 1;
     float *d_sx = nullptr, *d_sy = nullptr;
     double *d_sscale = nullptr;
     int *d_fb = nullptr;
@@ -941,6 +1037,10 @@ static int ele_evolve_impl(const pba_run_params *params, int n_runs, const pba_p
         if (d_aall) {
             pbagpu::set_error_msg("ele_evolve_surrogate: synchrotron self-absorption is not implemented with the surrogate");
             return PBAGPU_E_UNSUPPORTED;
+        }
+        if (ng_min != NG) {
+            pbagpu::set_error_msg("ele_evolve_surrogate: the populations of one call need one electron-grid size");
+            return PBAGPU_E_ARG;
         }
         const pba_mlp &net = sur->net;
         if (pbagpu::mlp::check_network(&net) != PBAGPU_OK || net.dims[0] != dx || net.dims[net.n_layers] != nf ||
@@ -962,10 +1062,12 @@ static int ele_evolve_impl(const pba_run_params *params, int n_runs, const pba_p
         PBA_TRY(rc, hipMemsetAsync(d_fb, 0, sizeof(int) * np, s));
     }
     PopState S;
-    S.f = wsd; S.ft = S.f + np * ng; S.tf = S.ft + np * ng; S.wt = S.tf + np * nf;
+    S.ngs = NG;
+    S.f = wsd; S.ft = S.f + np * NG; S.tf = S.ft + np * NG; S.wt = S.tf + np * nf;
     S.jsyn = S.wt + np * nf; S.jssc = S.jsyn + np * nf; S.scal = S.jssc + np * nf * SSC_JB;
     S.dfde = S.scal + np * 4;
-    S.flag = (int *)(S.dfde + np * ng);
+    S.gdot = want_gdot ? S.dfde + np * NG : nullptr;
+    S.flag = (int *)(S.dfde + np * NG + (want_gdot ? np * 3 * NG : 0));
     PBA_TRY(rc, hipMemsetAsync(wsd, 0, sizeof(double) * n_dbl + sizeof(int) * np * 3, s));
     if (dlist) PBA_TRY(rc, hipMemcpyAsync(dlist, hlist.data(), sizeof(int) * hlist.size(), hipMemcpyHostToDevice, s));
     const size_t ssc_lds = sizeof(double) * (size_t)nf * 64;
@@ -981,17 +1083,19 @@ static int ele_evolve_impl(const pba_run_params *params, int n_runs, const pba_p
                 if (npk == 0) continue;
                 size_t kk = 0;
                 while (!(need_ssc[kk] && ssc_slot[kk] == t)) kk++;
-                const double *base = dgrids + per_key * kk;
-                const double *e = base, *nu = base + ng + (ng - 1) + (ng + 1) + ng;
-                const double *K = dK + (ktab + (size_t)ng * nf) * t;
+                const int ng = keys[kk].ng;
+                const double *base = dgrids + goff[kk];
+                const double *e = base, *nu = nu_of(base, ng);
+                const double *K = dK + koff[t];
                 if (precision == 1) {
                     const size_t lo = list_off[t];
                     hipLaunchKernelGGL(ssc_prep_kernel, dim3((unsigned)npk), dim3(64), 0, s, dlist + lo, npk, n_pop, e, ng,
-                                       nf, kp, S, d_an + lo * ng, d_amax + lo * SSC_JB, d_wn + lo * kp, d_wmax + lo);
+                                       nf, kp, S, d_an + lo * NG, d_amax + lo * SSC_JB, d_wn + lo * kp, d_wmax + lo);
                     hipLaunchKernelGGL(ssc_mfma_kernel, dim3((unsigned)((nf + 15) / 16), (unsigned)((npk + 15) / 16),
                                                              (unsigned)SSC_JB),
-                                       dim3(64), 0, s, dlist + lo, npk, nu, ng, nf, kp, dKb + 2 * kbf * t,
-                                       dKb + 2 * kbf * t + kbf, d_an + lo * ng, d_amax + lo * SSC_JB, d_wn + lo * kp,
+                                       dim3(64), 0, s, dlist + lo, npk, nu, ng, nf, kp, dKb + kboff[t],
+                                       dKb + kboff[t] + (size_t)nf * ng * kp, d_an + lo * NG, d_amax + lo * SSC_JB,
+                                       d_wn + lo * kp,
                                        d_wmax + lo, S);
                 } else if (precision == 2) {
                     const size_t lo = list_off[t];
@@ -1040,7 +1144,17 @@ extern "C" int pbagpu_ele_evolve(const pba_run_params *params, int n_runs, const
                                  double *d_fall, int *d_status, int precision, void *stream) {
     if (precision != 0 && precision != 1) return PBAGPU_E_UNSUPPORTED;
     return ele_evolve_impl(params, n_runs, pops, n_pop, d_table, bw_total, nt, d_jall, d_aall, d_fall, d_status,
-                           precision, nullptr, nullptr, stream);
+                           precision, nullptr, nullptr, nullptr, stream);
+}
+
+extern "C" int pbagpu_ele_evolve_spectra(const pba_run_params *params, int n_runs, const pba_population *pops,
+                                         int n_pop, const double *d_table, int bw_total, int nt, double *d_jall,
+                                         double *d_aall, double *d_fall, int *d_status, int precision,
+                                         const pba_spectra *spec, void *stream) {
+    if (precision != 0 && precision != 1) return PBAGPU_E_UNSUPPORTED;
+    if (!spec) return PBAGPU_E_ARG;
+    return ele_evolve_impl(params, n_runs, pops, n_pop, d_table, bw_total, nt, d_jall, d_aall, d_fall, d_status,
+                           precision, nullptr, nullptr, spec, stream);
 }
 
 extern "C" int pbagpu_ele_evolve_surrogate(const pba_run_params *params, int n_runs, const pba_population *pops,
@@ -1049,7 +1163,7 @@ extern "C" int pbagpu_ele_evolve_surrogate(const pba_run_params *params, int n_r
                                            pba_surrogate_stats *stats, void *stream) {
     if (!sur) return PBAGPU_E_ARG;
     return ele_evolve_impl(params, n_runs, pops, n_pop, d_table, bw_total, nt, d_jall, nullptr, d_fall, d_status, 2,
-                           sur, stats, stream);
+                           sur, stats, nullptr, stream);
 }
 
 // grids and Compton table of one shock, for the batched entry points below

@@ -20,7 +20,7 @@ EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu
            "pbagpu_release_stream", "pbagpu_stream_create", "pbagpu_stream_destroy",
            "pbagpu_synchrotron_f", "pbagpu_eats_lightcurve_vec", "pbagpu_mlp_forward", "pbagpu_mlp_train_step",
            "pbagpu_gemm_bf16", "pbagpu_ssc_features", "pbagpu_ssc_emissivity", "pbagpu_ele_evolve_surrogate",
-           "pbagpu_skymap_hist_noalloc")
+           "pbagpu_skymap_hist_noalloc", "pbagpu_ele_evolve_spectra", "pbagpu_dyn_evolve_ragged")
 STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW, STAGE_SKY_HIST, STAGE_EATS_VEC = 0, 1, 2, 3, 4, 5, 6
 N_STAGES = 7
 # sky-map binning (enum pba_sky_moment / pba_sky_status)
@@ -44,8 +44,11 @@ def lib():
         L.pbagpu_set_device.argtypes = [C.c_int]
         L.pbagpu_dyn_evolve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.c_int, C.POINTER(abi.LayerInit), C.c_int,
                                         vp, vp, C.POINTER(abi.BwResult), vp]
+        L.pbagpu_dyn_evolve_ragged.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.c_int, C.POINTER(abi.LayerInit),
+                                               C.c_int, C.POINTER(C.c_int), vp, vp, C.POINTER(abi.BwResult), vp]
         L.pbagpu_ele_evolve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.Population), C.c_int, vp,
                                         C.c_int, C.c_int, vp, vp, vp, vp, C.c_int, vp]
+        L.pbagpu_ele_evolve_spectra.argtypes = L.pbagpu_ele_evolve.argtypes[:-1] + [C.POINTER(abi.Spectra), vp]
         L.pbagpu_last_error.restype = C.c_char_p
         L.pbagpu_last_error.argtypes = []
         L.pbagpu_timing_enable.argtypes = [C.c_int]
@@ -255,29 +258,41 @@ class Run:
         go["use_1d_id"] = "yes"
         P = abi.run_params(mp_, mo, gp, go)
         method = go.get("method_eats", "adaptive")
+        struct = dict(struct)
+        ngs = struct.pop("layer_ngam", None)
         idd = inputs.jet_structure(kind=method, **struct)
         tobs, nuobs = abi.obs_pairs(mo)
-        return cls(P, abi.layer_inits(idd, method), abi.tgrid(mp_), tobs, nuobs, method)
+        inits = abi.layer_inits(idd, method)
+        if ngs is not None:   # a Chang-Cooper grid of its own for every layer (both shocks)
+            if len(ngs) != len(inits):
+                raise ValueError(f"layer_ngam has {len(ngs)} entries for {len(inits)} layers")
+            for li, n in zip(inits, ngs):
+                li.ngam_fs = li.ngam_rs = int(n)
+        return cls(P, inits, abi.tgrid(mp_), tobs, nuobs, method)
 
     @property
     def n_layers(self):
         return len(self.inits)
 
 
-def populations(params_list, results, n_runs, bw_per_run):
+def populations(params_list, results, n_runs, bw_per_run, inits=None):
     """Electron populations and EATS layer descriptors of a batch of runs (forward shock of every
-    blastwave; reverse shock where the blastwave kept it and RS radiation is on)."""
+    blastwave; reverse shock where the blastwave kept it and RS radiation is on).  With the
+    blastwaves' initial data (inits [n_runs * bw_per_run]) every population carries its
+    blastwave's electron-grid size (pba_layer_init ngam_fs / ngam_rs; 0: the run's)."""
     pops, lays = [], []
     for r in range(n_runs):
         P = params_list[r]
         for i in range(bw_per_run):
             bw = r * bw_per_run + i
+            ng_fs = inits[bw].ngam_fs if inits is not None else 0
+            ng_rs = inits[bw].ngam_rs if inits is not None else 0
             pfs = len(pops)
-            pops.append(abi.Population(bw, r, 0, 0))
+            pops.append(abi.Population(bw, r, 0, ng_fs))
             prs = -1
             if results[bw].fsrs and P.do_rs_radiation:
                 prs = len(pops)
-                pops.append(abi.Population(bw, r, 1, 0))
+                pops.append(abi.Population(bw, r, 1, ng_rs))
             lays.append((bw, r, pfs, prs, results[bw].i_end_r))
     return pops, lays
 
@@ -300,15 +315,18 @@ class Pipeline:
         self.n_runs = len(runs)
         r0 = runs[0]
         self.bpr = r0.n_layers
-        self.nt = len(r0.tgrid)
+        # runs may have time grids of their own (lengths and values): the tables then have
+        # nt = the longest grid's steps, and run r fills its first nt_run[r] columns
+        self.nt = max(len(r.tgrid) for r in runs)
+        self.nt_run = [len(r.tgrid) for r in runs]
+        self.ragged = any(len(r.tgrid) != len(r0.tgrid) or not np.array_equal(r.tgrid, r0.tgrid) for r in runs)
         for r in runs:
-            if r.n_layers != self.bpr or len(r.tgrid) != self.nt or len(r.tobs) != len(r0.tobs):
-                raise ValueError("runs of one batch need equal layer count, time grid and pair count")
+            if r.n_layers != self.bpr or len(r.tobs) != len(r0.tobs):
+                raise ValueError("runs of one batch need equal layer count and pair count")
             if r.method != r0.method:
                 raise ValueError("runs of one batch share the EATS method")
-            if not np.array_equal(r.tgrid, r0.tgrid) or not np.array_equal(r.tobs, r0.tobs) \
-                    or not np.array_equal(r.nuobs, r0.nuobs):
-                raise ValueError("runs of one batch share the time grid and the observer pairs")
+            if not np.array_equal(r.tobs, r0.tobs) or not np.array_equal(r.nuobs, r0.nuobs):
+                raise ValueError("runs of one batch share the observer pairs")
         self.method = r0.method
         self.nf = r0.params.fs.nfreq
         self.npairs = len(r0.tobs)
@@ -322,7 +340,14 @@ class Pipeline:
         self.params = (abi.RunParams * self.n_runs)(*[r.params for r in runs])
         self.inits = (abi.LayerInit * (self.n_runs * self.bpr))(*[li for r in runs for li in r.inits])
         nbw = self.n_runs * self.bpr
-        self.d_tg = torch.from_numpy(r0.tgrid).to(self.dev)
+        if self.ragged:
+            tg = np.zeros((self.n_runs, self.nt))
+            for k, r in enumerate(runs):
+                tg[k, :len(r.tgrid)] = r.tgrid
+                tg[k, len(r.tgrid):] = r.tgrid[-1]
+            self.d_tg = torch.from_numpy(tg).to(self.dev)
+        else:
+            self.d_tg = torch.from_numpy(r0.tgrid).to(self.dev)
         self.d_table = torch.empty((nbw, abi.PBA_NV, self.nt), dtype=torch.float64, device=self.dev)
         self.res = (abi.BwResult * nbw)()
         # at most two populations per blastwave
@@ -348,9 +373,18 @@ class Pipeline:
         torch.cuda.synchronize()
 
     def dynamics(self):
-        _check(self.L.pbagpu_dyn_evolve(self.params, self.n_runs, self.bpr, self.inits, self.nt, _ptr(self.d_tg),
-                                        _ptr(self.d_table), self.res, self.sp), "dyn_evolve")
-        pops, lays = populations(self.params, self.res, self.n_runs, self.bpr)
+        if self.ragged:
+            ntr = (C.c_int * self.n_runs)(*self.nt_run)
+            _check(self.L.pbagpu_dyn_evolve_ragged(self.params, self.n_runs, self.bpr, self.inits, self.nt, ntr,
+                                                   _ptr(self.d_tg), _ptr(self.d_table), self.res, self.sp),
+                   "dyn_evolve_ragged")
+        else:
+            _check(self.L.pbagpu_dyn_evolve(self.params, self.n_runs, self.bpr, self.inits, self.nt, _ptr(self.d_tg),
+                                            _ptr(self.d_table), self.res, self.sp), "dyn_evolve")
+        self._set_populations()
+
+    def _set_populations(self):
+        pops, lays = populations(self.params, self.res, self.n_runs, self.bpr, self.inits)
         self.pops = pops
         self.parr = (abi.Population * len(pops))(*pops)
         if self.method == "piece-wise":
@@ -367,21 +401,46 @@ class Pipeline:
                 self.larr[i] = abi.EatsLayer(li.theta_c_l, li.theta_c_h, bw, r, pfs, prs, iend)
         self.layers = lays
 
-    def electrons(self, want_f=False):
+    def use_tables(self, table, results):
+        """Take the dynamics tables [n_bw][PBA_NV][nt] (host array) and per-blastwave results (the
+        oracle's, say) in place of running the dynamics stage, so that the electron and observer
+        stages run on given tables."""
+        self.d_table.copy_(self.torch.from_numpy(np.ascontiguousarray(table)))
+        for i, r in enumerate(results):
+            self.res[i] = r
+        self._set_populations()
+
+    def ngam_max(self):
+        """Largest electron-grid size of the batch's populations (the row stride of f_all)."""
+        return max(P.ngam if P.ngam > 0 else (self.params[P.run].rs.ngam if P.shock else self.params[P.run].fs.ngam)
+                   for P in self.pops)
+
+    def electrons(self, want_f=False, spectra=False):
+        """Electron stage of every population.  spectra=True also keeps the dense-output spectra of
+        the reference's save_spec = yes (pbagpu_ele_evolve_spectra) in self.d_spec: a dict of
+        [n_pop][nt][nfreq] (syn_j, syn_i, ssc_j, total_f, total_i) and [n_pop][nt][ngam]
+        (gam_dot_syn, gam_dot_adi, gam_dot_ssc) device tensors."""
         npop = len(self.pops)
-        self.d_fall = self.torch.empty((npop, self.nt, self.runs[0].params.fs.ngam), dtype=self.torch.float64,
+        ng = self.ngam_max()
+        self.d_fall = self.torch.empty((npop, self.nt, ng), dtype=self.torch.float64,
                                        device=self.dev) if want_f else None
-        _check(self.L.pbagpu_ele_evolve(self.params, self.n_runs, self.parr, npop, _ptr(self.d_table),
-                                        self.n_runs * self.bpr, self.nt, _ptr(self.d_jall), _ptr(self.d_aall),
-                                        _ptr(self.d_fall),
-                                        _ptr(self.d_est), self.precision, self.rsp), "ele_evolve")
+        args = [self.params, self.n_runs, self.parr, npop, _ptr(self.d_table), self.n_runs * self.bpr, self.nt,
+                _ptr(self.d_jall), _ptr(self.d_aall), _ptr(self.d_fall), _ptr(self.d_est), self.precision]
+        if not spectra:
+            self.d_spec = None
+            _check(self.L.pbagpu_ele_evolve(*args, self.rsp), "ele_evolve")
+            return
+        mk = lambda n: self.torch.empty((npop, self.nt, n), dtype=self.torch.float64, device=self.dev)
+        self.d_spec = {k: mk(self.nf) for k in abi.SPECTRA_NU} | {k: mk(ng) for k in abi.SPECTRA_GAM}
+        sp = abi.Spectra(**{k: v.data_ptr() for k, v in self.d_spec.items()})
+        _check(self.L.pbagpu_ele_evolve_spectra(*args, C.byref(sp), self.rsp), "ele_evolve_spectra")
 
     def electrons_surrogate(self, sur, want_f=False):
         """Electron stage with the SSC emissivity from a trained surrogate (surrogate.SscSurrogate),
         checked against the plain contraction on a sampled subset every step.  Returns the counts
         (abi.SurrogateStats)."""
         npop = len(self.pops)
-        self.d_fall = self.torch.empty((npop, self.nt, self.runs[0].params.fs.ngam), dtype=self.torch.float64,
+        self.d_fall = self.torch.empty((npop, self.nt, self.ngam_max()), dtype=self.torch.float64,
                                        device=self.dev) if want_f else None
         st = abi.SurrogateStats()
         spec = sur.spec()
@@ -486,12 +545,11 @@ class Pipeline:
             return self.d_out.cpu().numpy().reshape(self.n_runs, self.bpr, self.npairs)
 
 
-def lightcurves(runs, stream=None, want_intermediates=False, precision=0):
-    """Run the three GPU stages for a list of Runs with equal grid sizes.
-
-    Returns (flux [n_runs, n_layers, n_pairs] numpy array, extras dict)."""
-    pl = Pipeline(runs, stream=stream, precision=precision)
-    pl.dynamics()
+def radiate(pl, want_intermediates=False, precision=None):
+    """Electron and observer stages of a Pipeline whose dynamics ran (or whose tables were given);
+    returns (flux [n_runs, n_layers, n_pairs], extras) as lightcurves() does."""
+    if precision is not None:
+        pl.precision = precision
     pl.electrons(want_f=want_intermediates)
     pl.observer()
     dst, est, lst = pl.status()
@@ -504,3 +562,12 @@ def lightcurves(runs, stream=None, want_intermediates=False, precision=0):
                       fall=pl.d_fall.cpu().numpy(), pops=pl.pops, layers=pl.layers,
                       results=[(r.status, r.i_end_r, r.fsrs, r.n_steps, r.n_rhs) for r in pl.res])
     return flux, extras
+
+
+def lightcurves(runs, stream=None, want_intermediates=False, precision=0):
+    """Run the three GPU stages for a list of Runs with equal grid sizes.
+
+    Returns (flux [n_runs, n_layers, n_pairs] numpy array, extras dict)."""
+    pl = Pipeline(runs, stream=stream, precision=precision)
+    pl.dynamics()
+    return radiate(pl, want_intermediates)

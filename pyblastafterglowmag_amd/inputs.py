@@ -233,7 +233,10 @@ def write_run(workdir, main, grb, struct):
     from . import h5io
     os.makedirs(workdir, exist_ok=True)
     kind = "piece-wise" if grb.get("method_eats", "adaptive") == "piece-wise" else "adaptive"
-    idd = jet_structure(kind=kind, **struct)
+    if struct.get("layer_ngam") is not None:
+        raise ValueError("per-layer electron grids (layer_ngam) have no parfile form: the reference shares one "
+                         "grid per run")
+    idd = jet_structure(kind=kind, **{k: v for k, v in struct.items() if k != "layer_ngam"})
     fname = grb.get("fname_ejecta_id", "id_a.h5" if kind == "adaptive" else "id_pw.h5")
     grb = dict(grb, fname_ejecta_id=fname, use_1d_id="yes", method_eats=kind)
     h5io.write_id(os.path.join(workdir, fname), idd["r"], idd["mom"], idd["theta"], idd["ctheta"],

@@ -89,17 +89,46 @@ def write_dynamics(path, run, table, fsrs, mp_, gp):
             w.attr("/", k, v)
 
 
-def write_spectra(path, run, pops, jall, fall, ele_grid, nu_grid, mp_, gp):
-    """jall: [n_pop, nt, nf] total comoving emissivity per electron; fall: [n_pop, nt, ng]."""
+def write_spectra(path, run, pops, jall, fall, ele_grid, nu_grid, mp_, gp, spectra=None, aall=None):
+    """spec.h5 as Ejecta::computeSaveEjectaSpectrum writes it (src/ejecta/ejecta.h:354-509).
+
+    jall: [n_pop, nt, nf] total comoving emissivity per electron; fall: [n_pop, nt, ng] (n_ele);
+    spectra: None or the dict of pbagpu_ele_evolve_spectra's arrays (syn_j, syn_i, ssc_j, total_f,
+    total_i [n_pop, nt, nf]; gam_dot_syn, gam_dot_adi, gam_dot_ssc [n_pop, nt, ng]);
+    aall: [n_pop, nt, nf] absorption (use_ssa = yes) or None.
+    Per population (forward or reverse shock of a layer) the reference writes n_ele and gam_dot_*,
+    syn_f / syn_j / syn_i (+ syn_a with SSA), ssc_f / ssc_j / ssc_i with SSC and total_f / total_j /
+    total_i (+ total_a with SSA); syn_f and ssc_f are never filled there (zeros), ssc_i equals ssc_j
+    (no SSC absorption without pair production) and syn_a, total_a are the absorption table."""
     nl = run.n_layers
     nt = len(run.tgrid)
     with h5io.Writer(path) as w:
         for q, P in enumerate(pops):
             g = f"shell=0 layer={P.bw}"
             tag = "fs" if P.shock == 0 else "rs"
+            sh = run.params.fs if P.shock == 0 else run.params.rs
             if fall is not None:
                 w.dataset(g, f"n_ele_{tag}", fall[q])
+            if spectra is None:
+                w.dataset(g, f"total_j_{tag}", jall[q])
+                continue
+            for k in ("gam_dot_syn", "gam_dot_adi", "gam_dot_ssc"):
+                w.dataset(g, f"{k}_{tag}", spectra[k][q])
+            zeros = np.zeros_like(jall[q])
+            w.dataset(g, f"syn_f_{tag}", zeros)
+            w.dataset(g, f"syn_j_{tag}", spectra["syn_j"][q])
+            w.dataset(g, f"syn_i_{tag}", spectra["syn_i"][q])
+            if sh.ssa:
+                w.dataset(g, f"syn_a_{tag}", aall[q])
+            if sh.ssc:
+                w.dataset(g, f"ssc_f_{tag}", zeros)
+                w.dataset(g, f"ssc_j_{tag}", spectra["ssc_j"][q])
+                w.dataset(g, f"ssc_i_{tag}", spectra["ssc_j"][q])
+            w.dataset(g, f"total_f_{tag}", spectra["total_f"][q])
             w.dataset(g, f"total_j_{tag}", jall[q])
+            w.dataset(g, f"total_i_{tag}", spectra["total_i"][q])
+            if sh.ssa:
+                w.dataset(g, f"total_a_{tag}", aall[q])
         w.dataset("/", "times_freqs", np.repeat(run.tgrid, len(nu_grid)))
         w.dataset("/", "freqs", np.tile(nu_grid, nt))
         w.dataset("/", "times_gams", np.repeat(run.tgrid, len(ele_grid)))

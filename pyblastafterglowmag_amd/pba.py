@@ -55,7 +55,7 @@ def main(argv=None):
     if dst.any():
         print(f"[pba] blastwave evolution failed: status {dst.tolist()}", file=sys.stderr)
         return 1
-    pl.electrons(want_f=save_spec)
+    pl.electrons(want_f=save_spec, spectra=save_spec)
     pl.observer()
     _, est, lst = pl.status()
     if est.any() or lst.any():
@@ -106,8 +106,10 @@ def main(argv=None):
         path = workdir + go.get("fname_spectrum", "spec.h5")
         npop = len(pl.pops)
         sh = run.params.fs
+        spectra = {k: v.cpu().numpy() for k, v in pl.d_spec.items()}
+        aall = pl.d_aall[:npop].cpu().numpy() if pl.d_aall is not None else None
         output.write_spectra(path, run, pl.pops, pl.d_jall[:npop].cpu().numpy(), pl.d_fall.cpu().numpy(),
-                             output.ele_grid(sh), output.freq_grid(sh), mp_, gp)
+                             output.ele_grid(sh), output.freq_grid(sh), mp_, gp, spectra=spectra, aall=aall)
         if verbose:
             print(f"[pba] spectra -> {path}", flush=True)
     return 0

@@ -65,16 +65,27 @@ def latin_hypercube(n, d, seed=LHS_SEED):
     return (strata + rng.random((n, d))) / n
 
 
-def lhs_member(i, n=1024, seed=LHS_SEED, theta_w_over_theta_c=4.0):
+LHS_NTB = (500, 1500)
+
+
+def lhs_ntb(i, seed=LHS_SEED):
+    """Time-grid length of member i: uniform on LHS_NTB (inclusive), from a generator seeded by
+    (seed, i), so a member's grid does not depend on the batch it is in."""
+    return int(np.random.default_rng([seed, i]).integers(LHS_NTB[0], LHS_NTB[1] + 1))
+
+
+def lhs_member(i, n=1024, seed=LHS_SEED, theta_w_over_theta_c=4.0, ragged_nt=True):
     """Member i of the n-member Latin-hypercube batch: the 21-layer Gaussian jet of
     gauss21_fsrs_ssc (forward + reverse shock, synchrotron + SSC, same grids) with the eight
     parameters of LHS_RANGES drawn from the hypercube.  The jet's wing is theta_w =
     theta_w_over_theta_c x theta_c (capped at the 0.436 rad of gauss21_fsrs_ssc), so that every
-    member keeps its 21 layers (the initial data drop layers with beta0 <= 1e-5)."""
+    member keeps its 21 layers (the initial data drop layers with beta0 <= 1e-5).  With ragged_nt
+    the member's burster-frame grid has lhs_ntb(i) steps over the same span (else 1000)."""
     u = latin_hypercube(n, len(LHS_RANGES), seed)[i]
     v = {name: float(lo + (hi - lo) * x) for (name, lo, hi), x in zip(LHS_RANGES, u)}
     main, grb, struct = gauss21_fsrs_ssc()
-    main = dict(main, n_ism=10 ** v["log10_n_ism"], theta_obs=v["theta_obs"])
+    main = dict(main, n_ism=10 ** v["log10_n_ism"], theta_obs=v["theta_obs"],
+                ntb=lhs_ntb(i, seed) if ragged_nt else main["ntb"])
     micro = {}
     for sh in ("fs", "rs"):
         micro.update({f"eps_e_{sh}": 10 ** v["log10_eps_e"], f"eps_b_{sh}": 10 ** v["log10_eps_b"], f"p_{sh}": v["p"]})
@@ -84,7 +95,33 @@ def lhs_member(i, n=1024, seed=LHS_SEED, theta_w_over_theta_c=4.0):
     return main, grb, struct
 
 
-WORKLOADS = {"tophat_fs_syn": tophat_fs_syn, "gauss21_fsrs_ssc": gauss21_fsrs_ssc, "tophat64_pw": tophat64_pw}
+# Per-layer electron grids of the piece-wise Gaussian workload: a seeded uniform draw per layer
+C3_NGAM = (128, 256, 384, 512)
+
+
+def layer_ngam(n_layers, choices=C3_NGAM, seed=LHS_SEED):
+    """Chang-Cooper grid size of every layer, uniform over `choices` (a generator seeded by
+    (seed, n_layers))."""
+    rng = np.random.default_rng([seed, n_layers])
+    return tuple(int(x) for x in rng.choice(np.asarray(choices), n_layers))
+
+
+def gauss64_pw_ng():
+    """The 21-layer Gaussian jet's physics (forward + reverse shock, synchrotron + SSC, 96
+    comoving frequencies, 1000 steps, 96 nu x 128 t_obs) on a 64-layer piece-wise grid (2i+1 phi
+    cells per layer, 4096 cells) with a Chang-Cooper grid of its own for every layer, drawn from
+    128, 256, 384, 512 bins (layer_ngam).  The wing is cut at 0.2 rad, as in gauss_pw_small: the
+    reference stops on the slower wing layers of the full 0.436 rad jet (negative cell intensities,
+    microphysics.h:835-841).  The reference shares one grid per run, so it computes this workload
+    one grid size at a time."""
+    main, grb, struct = gauss21_fsrs_ssc()
+    grb = dict(grb, method_eats="piece-wise")
+    struct = dict(struct, n_layers_pw=64, theta_w=0.2, layer_ngam=layer_ngam(64))
+    return main, grb, struct
+
+
+WORKLOADS = {"tophat_fs_syn": tophat_fs_syn, "gauss21_fsrs_ssc": gauss21_fsrs_ssc, "tophat64_pw": tophat64_pw,
+             "gauss64_pw_ng": gauss64_pw_ng}
 
 
 def scaled(name, ntb=None, ngam=None, nfreq=None, n_nu_obs=None, n_t_obs=None, n_layers=None, **over):
@@ -105,9 +142,10 @@ def scaled(name, ntb=None, ngam=None, nfreq=None, n_nu_obs=None, n_t_obs=None, n
     if n_layers and struct["struct"] == "gaussian":
         struct["n_layers_a"] = n_layers
     for k, v in over.items():
-        if k in struct or k == "n_layers_pw":
+        if k in struct or k in ("n_layers_pw", "layer_ngam"):
             struct[k] = v
-        elif k in ("tb0", "tb1", "ntb", "n_ism", "d_l", "z", "theta_obs", "lc_freqs", "lc_times", "rtol", "integrator"):
+        elif k in ("tb0", "tb1", "ntb", "n_ism", "A0", "s", "r_ej", "r_ism", "d_l", "z", "theta_obs", "lc_freqs",
+                   "lc_times", "rtol", "integrator"):
             main[k] = v
         else:
             grb[k] = v
@@ -126,6 +164,19 @@ SMALL = {
     # emissivity times the reverse-shock mass) is large enough for that step to show.
     "gauss_pw_small": dict(base="gauss21_fsrs_ssc", ntb=1000, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
                            method_eats="piece-wise", n_layers_pw=6, theta_w=0.2, rs_shutOff_criterion_rho=1e-30),
+    # the jet of gauss_pw_small with a grid size of its own per layer (the GPU path's per-layer
+    # grids), and the reference's uniform runs at the other two sizes (one run per size pins it)
+    "gauss_pw_ng_small": dict(base="gauss21_fsrs_ssc", ntb=1000, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
+                              method_eats="piece-wise", n_layers_pw=6, theta_w=0.2, rs_shutOff_criterion_rho=1e-30,
+                              layer_ngam=(32, 64, 48, 48, 32, 64)),
+    "gauss_pw_ng32": dict(base="gauss21_fsrs_ssc", ntb=1000, ngam=32, nfreq=48, n_nu_obs=6, n_t_obs=8,
+                          method_eats="piece-wise", n_layers_pw=6, theta_w=0.2, rs_shutOff_criterion_rho=1e-30),
+    "gauss_pw_ng48": dict(base="gauss21_fsrs_ssc", ntb=1000, ngam=48, nfreq=48, n_nu_obs=6, n_t_obs=8,
+                          method_eats="piece-wise", n_layers_pw=6, theta_w=0.2, rs_shutOff_criterion_rho=1e-30),
+    # the same jet at the default rs_shutOff_criterion_rho = 1e-50: the reverse shock of a layer ends
+    # at the step where M3 first rounds to >= 1
+    "gauss_pw_default": dict(base="gauss21_fsrs_ssc", ntb=1000, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
+                             method_eats="piece-wise", n_layers_pw=6, theta_w=0.2),
     "tophat_pw_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
                             method_eats="piece-wise", n_layers_pw=4),
     # synchrotron self-absorption on (the radio points of the light curve are self-absorbed early on)
@@ -143,6 +194,10 @@ SMALL = {
     # GSL's Chebyshev fit of the synchrotron function F(x) in place of CSYN
     "tophat_gsl_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
                              method_synchrotron_fs="GSL"),
+    # a wind-like ambient medium (n_ism < 0: density A0 R^-2 between r_ej and r_ism, constant inside and
+    # outside, RhoISM::evaluateRhoDrhoDrDefault) in place of the uniform ISM
+    "tophat_wind_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
+                              n_ism=-1., A0=3e35, s=2., r_ej=1e16, r_ism=3e18),
     # the classical RK4 integrator, one step per output interval, in place of adaptive DOP853
     "tophat_rk4_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
                              integrator="RK4"),

@@ -47,3 +47,39 @@ def rel_err(a, b, floor_frac=0.0):
     if not np.any(m):
         return 0.0
     return float(np.max(np.abs(a[m] - b[m]) / np.abs(b[m])))
+
+
+def rs_end_steps(gamma43):
+    """Per layer: the last step at which the reverse shock is on (Gamma43 != 0), or -1."""
+    out = []
+    for row in np.atleast_2d(gamma43):
+        nz = np.nonzero(row != 0)[0]
+        out.append(int(nz[-1]) if len(nz) else -1)
+    return out
+
+
+# Session caches of the GPU tests: a golden fixture's Pipeline after its dynamics stage (the
+# reverse-shock runs take tens of seconds of one-wavefront latency each) and the oracle's
+# dynamics of the same run; tests rerun the electron and observer stages on them as they need.
+_GPU_PIPELINES = {}
+_ORACLE_DYN = {}
+
+
+def gpu_pipeline(name):
+    """(run, Pipeline) of golden fixture `name` with its GPU dynamics done (computed once)."""
+    if name not in _GPU_PIPELINES:
+        from pyblastafterglowmag_amd import gpu
+        run = golden_run(load_golden(name))
+        pl = gpu.Pipeline([run])
+        pl.dynamics()
+        _GPU_PIPELINES[name] = (run, pl)
+    return _GPU_PIPELINES[name]
+
+
+def oracle_dynamics(name):
+    """The oracle's (table, results) of golden fixture `name` (computed once)."""
+    if name not in _ORACLE_DYN:
+        import pyoracle
+        run = golden_run(load_golden(name))
+        _ORACLE_DYN[name] = pyoracle.dyn_evolve(run.params, run.inits, run.tgrid)
+    return _ORACLE_DYN[name]

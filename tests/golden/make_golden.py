@@ -33,11 +33,17 @@ def build_case(name):
     return workloads.WORKLOADS[name]()
 
 
+# spec.h5 datasets beside n_ele / total_j (src/ejecta/ejecta.h:379-440), stored for SPEC_FULL
+SPEC_DATASETS = ["gam_dot_syn", "gam_dot_adi", "gam_dot_ssc", "syn_f", "syn_j", "syn_i", "syn_a", "ssc_f", "ssc_j",
+                 "ssc_i", "total_f", "total_i", "total_a"]
+SPEC_FULL = {"tophat_small": 3, "gauss5_small": 20, "tophat_ssa_small": 3}   # name: step stride
+
 SKY_ATTRS = ["flux", "nshells", "nlayers", "time", "freq", "xmin", "ymin", "xmax", "ymax"]
 
 
-# full-size cases: light curves of every `stride`-th layer, dynamics of layer 0, no spectra or sky maps
-LARGE = {"tophat64_pw": 7}
+# light curves of every `stride`-th layer, dynamics of layer 0, no spectra or sky maps: the full-size
+# case and the uniform-grid references of the per-layer-grid jet
+LARGE = {"tophat64_pw": 7, "gauss_pw_ng32": 1, "gauss_pw_ng48": 1}
 
 
 def run_reference(name, wd, spec_every=1):
@@ -110,6 +116,18 @@ def run_reference(name, wd, spec_every=1):
             out[f"jall_{tag}"] = np.stack([J[i] for i in ok])
             out[f"fall_{tag}"] = np.stack(F)
     out["spec_every"] = np.array(spec_every)
+    # every other dataset of spec.h5 (per shock, every SPEC_FULL[name]-th step), for the fixtures
+    # that pin the spectra writer
+    if name in SPEC_FULL:
+        full_every = SPEC_FULL[name]
+        out["spec_full_every"] = np.array(full_every)
+        for tag in ("fs", "rs"):
+            ok = [il for il in range(nl) if h5io.has(spec, f"shell=0 layer={il}/total_j_{tag}")]
+            for ds in SPEC_DATASETS:
+                rows = [h5io.read(spec, f"shell=0 layer={il}/{ds}_{tag}").reshape(nt, -1)[::full_every]
+                        for il in ok if h5io.has(spec, f"shell=0 layer={il}/{ds}_{tag}")]
+                if rows:
+                    out[f"spec_{ds}_{tag}"] = np.stack(rows)
     if sky:
         n = 0
         while os.path.exists(os.path.join(wd, f"raw_skymap_{n}.h5")):
@@ -142,9 +160,10 @@ def main(names):
 
 
 # every fixture the tests load
-ALL = ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gauss_pw_small", "tophat64_pw",
+ALL = ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gauss_pw_small", "gauss_pw_default",
+       "tophat64_pw", "gauss_pw_ng32", "gauss_pw_ng48",
        "tophat_ssa_small", "gauss5_ssa_small", "tophat_dermer_small", "tophat_bessel_small", "tophat_gsl_small",
-       "tophat_rk4_small"]
+       "tophat_rk4_small", "tophat_wind_small"]
 
 if __name__ == "__main__":
     main(sys.argv[1:] or ALL)

@@ -43,6 +43,13 @@ def test_driver_outputs_match_reference(name, tmp_path):
     nt, nf = int(main.get("ntb", 1000)), P.fs.nfreq
     j = h5io.read(spec, "shell=0 layer=0/total_j_fs").reshape(nt, nf)[::every]
     assert rel_err(j, g["jall_fs"][0], 1e-12) <= 1e-6
+    # every dataset the reference writes into a layer group, and the values of those the fixture holds
+    assert sorted(h5io.members(spec, "shell=0 layer=0")) == sorted(str(x) for x in g["spec_layer_members"])
+    for ds in ("syn_j", "syn_i", "total_f", "total_i", "gam_dot_syn", "gam_dot_adi"):
+        key = f"spec_{ds}_fs"
+        if key in g:
+            v = h5io.read(spec, f"shell=0 layer=0/{ds}_fs").reshape(nt, -1)[::int(g["spec_full_every"])]
+            assert rel_err(v, g[key][0], 1e-12) <= 1e-6, (name, ds)
 
 
 def test_driver_raw_skymaps_match_reference(tmp_path):

@@ -15,7 +15,7 @@ pairs above 1e-8 of the layer's peak flux):
 import numpy as np
 import pytest
 
-from conftest import golden_run, load_golden
+from conftest import golden_run, gpu_pipeline, load_golden
 
 pytestmark = pytest.mark.gpu
 
@@ -29,10 +29,10 @@ def _rel(a, b, floor=1e-8):
     return out
 
 
-def _pipeline(run):
-    from pyblastafterglowmag_amd import gpu
-    pl = gpu.Pipeline([run])
-    pl.dynamics()
+def _pipeline(name):
+    """The fixture's cached Pipeline (dynamics done) with its FP64 electron stage run."""
+    run, pl = gpu_pipeline(name)
+    pl.precision = 0
     pl.electrons()
     return pl
 
@@ -42,7 +42,7 @@ def test_vec_observer_matches_plain_and_reference(name):
     g = load_golden(name)
     run = golden_run(g)
     print(f"[gpu] {name}: vectorised observer", flush=True)
-    pl = _pipeline(run)
+    pl = _pipeline(name)
     pl.observer()
     plain = pl.flux()[0].copy()
     st = pl.status()

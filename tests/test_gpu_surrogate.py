@@ -12,7 +12,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import golden_run, load_golden
+from conftest import golden_run, gpu_pipeline, load_golden
 import oracle_mlp as om
 import pyoracle
 
@@ -137,10 +137,8 @@ def test_surrogate_on_a_run_and_fallback():
     the FP64 stage bit for bit."""
     import torch
     from pyblastafterglowmag_amd import gpu, surrogate
-    g = load_golden("gauss5_small")
-    run = golden_run(g)
-    pl = gpu.Pipeline([run])
-    pl.dynamics()
+    run, pl = gpu_pipeline("gauss5_small")
+    pl.precision = 0
     pl.electrons(want_f=True)
     j64 = pl.d_jall[:len(pl.pops)].clone()
     f, w, B, sh = surrogate.cells_from_pipeline(pl)
@@ -166,5 +164,7 @@ def test_surrogate_on_a_run_and_fallback():
                                      check_every=1)
     st2 = pl.electrons_surrogate(sur_bad)
     print(f"  fallback run: inferred {st2.inferred} failed {st2.failed} fallback {st2.fallback}", flush=True)
-    assert st2.inferred == 0 and st2.failed > 0
+    # the only network values kept are those of checked steps whose plain emissivity is all zero
+    # (no seed photons yet), where the prediction is zero too (the network's scale is 0 there)
+    assert st2.failed > 0 and st2.inferred + st2.failed == st2.checked
     assert torch.equal(pl.d_jall[:len(pl.pops)], j64)

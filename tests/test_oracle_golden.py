@@ -8,7 +8,7 @@ shared adaptive step first differs by an ulp, and stays within the tolerances be
 import numpy as np
 import pytest
 
-from conftest import golden_run, load_golden, rel_err
+from conftest import golden_run, load_golden, rel_err, rs_end_steps
 import pyoracle
 from pyblastafterglowmag_amd import abi
 
@@ -50,6 +50,7 @@ def _oracle_all(g):
     ("gauss5_small", 1e-4, 1e-5, 1e-8),
     ("tophat_pw_small", 0.0, 0.0, 0.0),
     ("gauss_pw_small", 1e-4, 1e-5, 1e-8),
+    ("gauss_pw_default", 1e-4, 1e-5, 1e-8),
     ("tophat_ssa_small", 0.0, 0.0, 0.0),
     ("gauss5_ssa_small", 1e-4, 1e-5, 1e-8),
     ("tophat_dermer_small", 0.0, 0.0, 0.0),
@@ -57,6 +58,7 @@ def _oracle_all(g):
     # F(x) of the GSL kernel option is computed from its integral, not from GSL's fits
     ("tophat_gsl_small", 0.0, 1e-11, 1e-11),
     ("tophat_rk4_small", 0.0, 0.0, 0.0),
+    ("tophat_wind_small", 0.0, 0.0, 0.0),
 ])
 def test_oracle_matches_reference(name, tol_dyn, tol_j, tol_lc):
     g = load_golden(name)
@@ -140,3 +142,72 @@ def test_oracle_full_piecewise_tophat():
     lc, st2 = pyoracle.eats_lightcurve_pw(run.params, lays, table, jall, run.tobs, run.nuobs)
     assert not st2.any()
     np.testing.assert_array_equal(lc, g["lc"])
+
+
+def test_reverse_shock_end_is_decided_by_the_last_ulp():
+    """At the default rs_shutOff_criterion_rho (1e-50) a layer's reverse shock ends at the first step
+    where the swept-up fraction M3/M0, which tends to 1 from below, rounds to >= 1
+    (isToStopReverseShock, src/blastwave/blastwave.h:453-471).  The oracle reproduces the reference
+    there step for step.  Moving the initial mass and momentum of every layer by one ulp moves that
+    step for some layer, and that layer's light curve then changes by more than 1 %: at this setting
+    the reference's own result depends on the last bit of its arithmetic, so an implementation with
+    other rounding (fused multiply-adds, another libm, no x87 long double) cannot be held to a
+    tolerance below that change on the layers whose end step differs."""
+    g = load_golden("gauss_pw_default")
+    names = [str(n) for n in g["dyn_names"]]
+    ref_end = rs_end_steps(g["dyn"][:, names.index("Gamma43")])
+    run, table, res, pops, jall, fall, lc = _oracle_all(g)
+    assert rs_end_steps(table[:, abi.PV["Gamma43"]]) == ref_end
+    assert max(rel_err(lc[il], g["lc"][il], 1e-30) for il in range(run.n_layers)) <= 1e-12
+    # the same inputs, initial mass and momentum one ulp larger
+    run2 = golden_run(g)
+    for li in run2.inits:
+        li.M0 *= 1 + 2.0 ** -52
+        li.mom0 *= 1 + 2.0 ** -52
+    table2, res2 = pyoracle.dyn_evolve(run2.params, run2.inits, run2.tgrid)
+    pops2, lays2 = _populations(run2, res2)
+    jall2, _, st2 = pyoracle.ele_evolve(run2.params, pops2, table2)
+    lc2, st3 = pyoracle.eats_lightcurve_pw(run2.params, lays2, table2, jall2, run2.tobs, run2.nuobs)
+    assert not st2.any() and not st3.any()
+    end2 = rs_end_steps(table2[:, abi.PV["Gamma43"]])
+    moved = [il for il in range(run.n_layers) if end2[il] != ref_end[il]]
+    assert moved, (end2, ref_end)
+    assert max(rel_err(lc2[il], lc[il], 1e-30) for il in moved) > 1e-2
+    for il in set(range(run.n_layers)) - set(moved):
+        assert rel_err(lc2[il], lc[il], 1e-30) <= 1e-8, il
+
+
+def _oracle_pw_run(run):
+    """Every stage of a piece-wise run on the oracle, populations carrying the layers' grid sizes."""
+    from pyblastafterglowmag_amd import gpu
+    table, res = pyoracle.dyn_evolve(run.params, run.inits, run.tgrid)
+    pops, lays = gpu.populations((abi.RunParams * 1)(run.params), res, 1, run.n_layers, run.inits)
+    parr = (abi.Population * len(pops))(*pops)
+    jall, _, st = pyoracle.ele_evolve(run.params, parr, table)
+    larr = (abi.EatsPwLayer * len(lays))()
+    for i, (bw, r, pfs, prs, iend) in enumerate(lays):
+        li = run.inits[bw]
+        larr[i] = abi.EatsPwLayer(li.ctheta0, li.ncells, abi.cells_in_layer(li.ilayer), bw, r, pfs,
+                                  1 if prs >= 0 else 0, iend)
+    lc, st2 = pyoracle.eats_lightcurve_pw(run.params, larr, table, jall, run.tobs, run.nuobs)
+    assert not st.any() and not st2.any() and not any(r.status for r in res)
+    return lc
+
+
+def test_per_layer_electron_grids_match_uniform_reference_runs():
+    """Per-layer Chang-Cooper grids (pba_layer_init ngam_fs / ngam_rs, pba_population ngam), which
+    the reference cannot express in one run (it shares one grid per run, common.h:100-110), pinned
+    one grid size at a time: every layer of the 6-layer Gaussian piece-wise jet with grid sizes
+    (32, 64, 48, 48, 32, 64) against the same layer of the reference's run with that size for every
+    layer (1e-9: the layers share one adaptive ODE step, whose sequence depends on all layers'
+    grids); and the same jet with every layer set to 48 bins against the reference's 48-bin run
+    (1e-14: the same arithmetic)."""
+    from pyblastafterglowmag_amd import gpu, workloads
+    refs = {32: load_golden("gauss_pw_ng32"), 48: load_golden("gauss_pw_ng48"), 64: load_golden("gauss_pw_small")}
+    main, grb, struct = workloads.small("gauss_pw_ng_small")
+    lc = _oracle_pw_run(gpu.Run.from_workload(main, grb, dict(struct, layer_ngam=(48,) * 6)))
+    for il in range(6):
+        assert rel_err(lc[il], refs[48]["lc"][il], 1e-30) <= 1e-14, il
+    lc = _oracle_pw_run(gpu.Run.from_workload(main, grb, struct))
+    for il, n in enumerate(struct["layer_ngam"]):
+        assert rel_err(lc[il], refs[n]["lc"][il], 1e-30) <= 1e-9, (il, n)

@@ -64,21 +64,27 @@ def test_dynamics_file_layout(name, tmp_path):
     np.testing.assert_array_equal(counts, g["dyn_root_counts"])
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", CASES + ["tophat_ssa_small", "gauss5_ssa_small"])
 def test_spectrum_file_grids(name, tmp_path):
-    """Grid datasets of the spectrum file equal the reference's (times_freqs/freqs/times_gams/gams),
-    and the datasets written per layer are among the reference's names."""
+    """The spectrum file's root (grids times_freqs/freqs/times_gams/gams and the layer groups) and the
+    datasets of layer 0 are exactly the reference's names, for both shocks and with or without SSC
+    and SSA."""
     g = load_golden(name)
     run = golden_run(g)
     P, mp_, mo, gp, go = _parse(g, tmp_path)
     nt, nf, ng = len(run.tgrid), P.fs.nfreq, P.fs.ngam
-    pops = [abi.Population(i, 0, 0, 0) for i in range(run.n_layers)]
+    pops = []
+    for i in range(run.n_layers):
+        pops.append(abi.Population(i, 0, 0, 0))
+        if i in [int(x) for x in g.get("spec_layers_rs", [])]:
+            pops.append(abi.Population(i, 0, 1, 0))
     path = str(tmp_path / "spec.h5")
-    output.write_spectra(path, run, pops, np.zeros((len(pops), nt, nf)), np.zeros((len(pops), nt, ng)),
-                         output.ele_grid(P.fs), output.freq_grid(P.fs), mp_, gp)
-    assert set(h5io.members(path, "/")) <= set(str(x) for x in g["spec_root_members"])
-    assert set(h5io.members(path, "shell=0 layer=0")) <= set(str(x) for x in g["spec_layer_members"])
-    assert {"times_freqs", "freqs", "times_gams", "gams"} <= set(h5io.members(path, "/"))
+    zn = np.zeros((len(pops), nt, nf))
+    spectra = {k: zn for k in abi.SPECTRA_NU} | {k: np.zeros((len(pops), nt, ng)) for k in abi.SPECTRA_GAM}
+    output.write_spectra(path, run, pops, zn, np.zeros((len(pops), nt, ng)), output.ele_grid(P.fs),
+                         output.freq_grid(P.fs), mp_, gp, spectra=spectra, aall=zn)
+    assert sorted(h5io.members(path, "/")) == sorted(str(x) for x in g["spec_root_members"])
+    assert sorted(h5io.members(path, "shell=0 layer=0")) == sorted(str(x) for x in g["spec_layer_members"])
 
 
 @pytest.mark.parametrize("name", ["tophat_pw_small", "gauss_pw_small"])
