@@ -84,6 +84,8 @@ def parse():
                     help="lhs_members: every run a different member of the 1024-member Latin-hypercube batch")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget of the electron and observer parts of the CPU-baseline sample")
+    ap.add_argument("--ele-graph", type=int, default=0, choices=[0, 1],
+                    help="capture the electron stage's per-step launches in one graph (PBAGPU_OPT_ELE_GRAPH)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stages", action="store_true", help="also print per-stage device times to stderr")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -392,6 +394,7 @@ def main():
         make = lambda b, n: [one] * n
     run = make(0, 1)[0]
     gpu.timing_enable(True)
+    gpu.set_option(gpu.OPT_ELE_GRAPH, args.ele_graph)
     legs = {}
     sur, sur_info = (None, None)
     if args.surrogate and args.mode in ("both", "mixed"):
@@ -428,6 +431,7 @@ def main():
                    "legs": {k: {kk: vv for kk, vv in v.items() if not kk.startswith("roofline")}
                             for k, v in legs.items()},
                    "value_leg": "mixed" if "mixed" in legs else "fp64",
+                   "electron_stage_graph": bool(args.ele_graph),
                    "mixed": ("SSC emissivity from the learned surrogate (BF16 MFMA network, FP32 accumulation; sampled "
                              "checks against the plain contraction with per-population fallback)" if sur is not None
                              else "SSC contraction BF16 hi/lo on MFMA (FP32 accumulation)")

@@ -204,6 +204,21 @@ int pbagpu_timing_read(int stage, double *ms_total, long long *launches, int res
  * per-blastwave results and the step flags on the host). */
 int pbagpu_timing_units(int stage, double *units, int reset);
 
+/* Library options (process-wide; 0 by default).  No counterpart in the reference.
+ *   PBAGPU_OPT_ELE_GRAPH  1: the electron stage captures its per-step launches (three or more
+ *                         per output step) into one HIP graph and launches it once per call,
+ *                         instead of launching them one by one (FP64 and BF16 contractions;
+ *                         the surrogate mode always launches directly), on a created stream
+ *                         (calls on the null stream, which cannot be captured, stay direct).
+ *                         Results are identical.  Returns PBAGPU_E_ARG for an unknown option or
+ *                         a value other than 0 and 1. */
+enum {
+    PBAGPU_OPT_ELE_GRAPH = 0,
+    PBAGPU_N_OPTIONS = 1
+};
+int pbagpu_set_option(int which, long long value);
+int pbagpu_get_option(int which, long long *value);
+
 /* ------------------------------------------------------------- stage 1 */
 /* Evolve n_runs independent models of bw_per_run blastwaves each.
  *   params   [n_runs]            host array

@@ -190,3 +190,33 @@ extern "C" int pbagpu_stream_destroy(void *stream) {
     if (e != hipSuccess) { pbagpu::set_error("hipStreamDestroy", e); return PBAGPU_E_HIP; }
     return rc;
 }
+
+// ------------------------------------------------------------------- options
+#include <atomic>
+
+namespace pbagpu {
+static std::atomic<long long> g_options[PBAGPU_N_OPTIONS];
+long long option(int which) { return (which >= 0 && which < PBAGPU_N_OPTIONS) ? g_options[which].load() : 0; }
+}  // namespace pbagpu
+
+extern "C" int pbagpu_set_option(int which, long long value) {
+    if (which < 0 || which >= PBAGPU_N_OPTIONS) {
+        pbagpu::set_error_msg("pbagpu_set_option: unknown option");
+        return PBAGPU_E_ARG;
+    }
+    if (which == PBAGPU_OPT_ELE_GRAPH && value != 0 && value != 1) {
+        pbagpu::set_error_msg("pbagpu_set_option: PBAGPU_OPT_ELE_GRAPH takes 0 or 1");
+        return PBAGPU_E_ARG;
+    }
+    pbagpu::g_options[which].store(value);
+    return PBAGPU_OK;
+}
+
+extern "C" int pbagpu_get_option(int which, long long *value) {
+    if (which < 0 || which >= PBAGPU_N_OPTIONS || !value) {
+        pbagpu::set_error_msg("pbagpu_get_option: unknown option or null value");
+        return PBAGPU_E_ARG;
+    }
+    *value = pbagpu::g_options[which].load();
+    return PBAGPU_OK;
+}

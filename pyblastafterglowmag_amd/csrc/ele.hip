@@ -504,15 +504,18 @@ __global__ void __launch_bounds__(64) ssc_step_kernel(const int *__restrict__ pl
 
 // ---------------------------------------------------------------------------------------------
 // Mixed-precision SSC contraction (precision = 1): the same sum as ssc_step_kernel, on MFMA in
-// BF16 with FP32 accumulation.  Written as a GEMM over the flattened (j, k) index:
-//   C[i][p] = sum_{j in block} sum_k K[i][j][k] * (an_j^p * wn_k^p)
-// with an_j = a_j / max_{j in block} a_j, a_j = f_j (gamma_{j+1}-gamma_j) (3/4 sigma_T c)/gamma_j^2,
-// and wn_k = w_k / max_k w_k, so every operand lies in [0, 1] whatever the decades the spectra
-// span; the block and photon scales multiply the FP32 result back in FP64.  Both operands are
-// split into a BF16 pair x = hi + lo and the product is hi*hi + hi*lo + lo*hi (the lo*lo term is
-// below FP32 resolution), which keeps ~16 significant bits per operand.
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+// BF16 with FP32 accumulation, taken in two levels:
+//   T[i][j][p] = sum_k K[i][j][k] wn_k^p                 (a GEMM over k, on v_mfma_f32_16x16x32_bf16)
+//   C[i][p]    = sum_chunks amax_c^p sum_{j in chunk c} an_j^p T[i][j][p]   (FP32 within a chunk,
+//                                                        FP64 across the chunks)
+// with a_j = f_j (gamma_{j+1}-gamma_j) (3/4 sigma_T c)/gamma_j^2 normalised per chunk of 16
+// Lorentz-factor bins (an_j = a_j / amax_c, amax_c = max of a_j over the chunk) and w_k normalised
+// by its maximum (wn_k = w_k / wmax), so every operand lies in [0, 1] whatever the decades the
+// spectra span.  The GEMM operands K and wn are split into BF16 pairs x = hi + lo (the table once,
+// the photon weights once per step) and the product is hi*hi + hi*lo + lo*hi (the lo*lo term is
+// below FP32 resolution), ~16 significant bits per operand; an_j enters in FP32.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 
 __device__ __forceinline__ unsigned short bf16_rn(float x) {
     unsigned u = __float_as_uint(x);
@@ -521,7 +524,7 @@ __device__ __forceinline__ unsigned short bf16_rn(float x) {
 }
 __device__ __forceinline__ float bf16_to_f(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
 
-// K (FP64 [nf][ng][nf]) -> BF16 pair [nf][ng][kp], k padded with zeros to kp (multiple of 16)
+// K (FP64 [nf][ng][nf]) -> BF16 pair [nf][ng][kp], k padded with zeros to kp (a multiple of 32)
 __global__ void ssc_table_bf16_kernel(const double *__restrict__ K, int nf, int ng, int kp,
                                       unsigned short *__restrict__ Khi, unsigned short *__restrict__ Klo) {
     size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -535,93 +538,115 @@ __global__ void ssc_table_bf16_kernel(const double *__restrict__ K, int nf, int 
     Klo[idx] = bf16_rn(v - bf16_to_f(h));
 }
 
+constexpr int SSC_CH = 16;                 // Lorentz-factor bins per normalisation chunk (one MFMA row block)
+constexpr int SSC_MAXKS = MAXF / 32;       // k-steps of 32 at the largest frequency grid
+
 // Per population of the list: normalised operands and their scales for this step.
-//   an [npk][ng]  (FP32), amax [npk][SSC_JB] (FP64), wn [npk][kp] (FP32), wmax [npk] (FP64)
+//   an  [npk][nch * 16] FP32 (0 past the last contributing bin), amax [npk][nch] FP64,
+//   wnh, wnl [npk][kp] BF16 pair of the normalised photon weights, wmax [npk] FP64
 __global__ void __launch_bounds__(64) ssc_prep_kernel(const int *__restrict__ plist, int npk, int n_pop,
-                                                      const double *__restrict__ e, int ng, int nf, int kp,
+                                                      const double *__restrict__ e, int ng, int nf, int kp, int nch,
                                                       PopState S, float *__restrict__ an, double *__restrict__ amax,
-                                                      float *__restrict__ wn, double *__restrict__ wmax) {
+                                                      unsigned short *__restrict__ wnh, unsigned short *__restrict__ wnl,
+                                                      double *__restrict__ wmax) {
     const int q = blockIdx.x, lane = threadIdx.x;
     const int q0 = plist[q];
     const bool act = S.flag[3 * q0 + 1] != 0;
     const double c0 = 3. / 4. * kSigmaT * kC;
-    const int jn = ng - 1, js = (jn + SSC_JB - 1) / SSC_JB;
-    for (int jb = 0; jb < SSC_JB; jb++) {
-        const int j0 = jb * js, j1 = min(j0 + js, jn);
-        double m = 0.;
-        for (int j = j0 + lane; j < j1; j += 64) {
-            double ej = e[j];
-            double aj = act ? S.ft[(size_t)j * n_pop + q0] * (e[j + 1] - e[j]) * (c0 / (ej * ej)) : 0.;
-            m = fmax(m, fabs(aj));
+    const int jn = ng - 1;
+    for (int c = 0; c < nch; c++) {
+        const int j = c * SSC_CH + (lane & (SSC_CH - 1));
+        double aj = 0.;
+        if (act && j < jn && lane < SSC_CH) {
+            const double ej = e[j];
+            aj = S.ft[(size_t)j * n_pop + q0] * (e[j + 1] - e[j]) * (c0 / (ej * ej));
         }
-        for (int d = 32; d >= 1; d >>= 1) m = fmax(m, __shfl_xor(m, d, 64));
-        for (int j = j0 + lane; j < j1; j += 64) {
-            double ej = e[j];
-            double aj = act ? S.ft[(size_t)j * n_pop + q0] * (e[j + 1] - e[j]) * (c0 / (ej * ej)) : 0.;
-            an[(size_t)q * ng + j] = m > 0. ? (float)(aj / m) : 0.f;
-        }
-        if (lane == 0) amax[(size_t)q * SSC_JB + jb] = m;
+        double m = fabs(aj);
+        for (int d = 8; d >= 1; d >>= 1) m = fmax(m, __shfl_xor(m, d, 64));   // over the chunk's 16 lanes
+        if (lane < SSC_CH) an[(size_t)q * nch * SSC_CH + j] = m > 0. ? (float)(aj / m) : 0.f;
+        if (lane == 0) amax[(size_t)q * nch + c] = m;
     }
     double w = 0.;
     for (int k = lane; k < nf - 1; k += 64) w = fmax(w, fabs(act ? S.wt[(size_t)k * n_pop + q0] : 0.));
     for (int d = 32; d >= 1; d >>= 1) w = fmax(w, __shfl_xor(w, d, 64));
     for (int k = lane; k < kp; k += 64) {
-        double wk = (act && k < nf - 1) ? S.wt[(size_t)k * n_pop + q0] : 0.;
-        wn[(size_t)q * kp + k] = w > 0. ? (float)(wk / w) : 0.f;
+        const double wk = (act && k < nf - 1) ? S.wt[(size_t)k * n_pop + q0] : 0.;
+        const float v = w > 0. ? (float)(wk / w) : 0.f;
+        const unsigned short h = bf16_rn(v);
+        wnh[(size_t)q * kp + k] = h;
+        wnl[(size_t)q * kp + k] = bf16_rn(v - bf16_to_f(h));
     }
     if (lane == 0) wmax[q] = w;
 }
 
-// One wavefront = one 16 (frequencies i) x 16 (populations) tile of one j-block.
-// grid = (ceil(nf/16), ceil(npk/16), SSC_JB)
-__global__ void __launch_bounds__(64) ssc_mfma_kernel(const int *__restrict__ plist, int npk, const double *__restrict__ nu,
-                                                      int ng, int nf, int kp, const unsigned short *__restrict__ Khi,
-                                                      const unsigned short *__restrict__ Klo,
-                                                      const float *__restrict__ an, const double *__restrict__ amax,
-                                                      const float *__restrict__ wn, const double *__restrict__ wmax,
-                                                      PopState S) {
-    const int lane = threadIdx.x;
-    const int i0 = blockIdx.x * 16, p0 = blockIdx.y * 16, jb = blockIdx.z;
-    const int jn = ng - 1, js = (jn + SSC_JB - 1) / SSC_JB;
-    const int j0 = jb * js, j1 = min(j0 + js, jn);
-    // operand coordinates of this lane (bf16 16x16x16: A row = lane%16, k = 4*(lane/16)+r;
-    // B column = lane%16, k = 4*(lane/16)+r; D row = 4*(lane/16)+r, column = lane%16)
-    const int ia = i0 + (lane & 15);
-    const int pb = p0 + (lane & 15);
-    const int kq = 4 * (lane >> 4);
-    const bool arow = ia < nf, bcol = pb < npk;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int j = j0; j < j1; j++) {
-        const float anj = bcol ? an[(size_t)pb * ng + j] : 0.f;
-        for (int kc = 0; kc < kp; kc += 16) {
-            bf16x4 ahi, alo, bhi, blo;
-            if (arow) {
-                const size_t o = ((size_t)ia * ng + j) * kp + kc + kq;
-                const unsigned short *ph = Khi + o, *pl = Klo + o;
-                for (int r = 0; r < 4; r++) { ahi[r] = (short)ph[r]; alo[r] = (short)pl[r]; }
-            } else {
-                for (int r = 0; r < 4; r++) { ahi[r] = 0; alo[r] = 0; }
-            }
-            for (int r = 0; r < 4; r++) {
-                float v = bcol ? anj * wn[(size_t)pb * kp + kc + kq + r] : 0.f;
-                unsigned short h = bf16_rn(v);
-                bhi[r] = (short)h;
-                blo[r] = (short)bf16_rn(v - bf16_to_f(h));
-            }
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, bhi, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, blo, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bhi, acc, 0, 0, 0);
+__device__ __forceinline__ bf16x8 load_bf16x8(const unsigned short *p) {
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(p));
+}
+
+// grid = (nf, ceil(npk / 64)), 4 wavefronts; wavefront w takes the 16 populations
+// blockIdx.y * 64 + 16 w + (lane mod 16) at frequency i = blockIdx.x, and runs over every chunk of
+// 16 Lorentz-factor bins.  v_mfma_f32_16x16x32_bf16: lane l holds row l mod 16 of A (bin j) and
+// column l mod 16 of B (population), eight consecutive k from 8 (l / 16); its four results are rows
+// 4 (l / 16) + r of column l mod 16.  The photon weights of the lane's population stay in
+// registers for all chunks; the table rows of a chunk are 16-byte loads shared through L1 by the
+// four wavefronts.
+__global__ void __launch_bounds__(256) ssc_mfma_kernel(const int *__restrict__ plist, int npk, const double *__restrict__ nu,
+                                                       int ng, int nf, int kp, int nch,
+                                                       const unsigned short *__restrict__ Khi,
+                                                       const unsigned short *__restrict__ Klo,
+                                                       const float *__restrict__ an, const double *__restrict__ amax,
+                                                       const unsigned short *__restrict__ wnh,
+                                                       const unsigned short *__restrict__ wnl,
+                                                       const double *__restrict__ wmax, PopState S) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = blockIdx.x;
+    const int col = lane & 15, g = lane >> 4;
+    const int pc = blockIdx.y * 64 + wave * 16 + col;
+    const bool pv = pc < npk;
+    const int nks = kp / 32;
+    bf16x8 bh[SSC_MAXKS], bl[SSC_MAXKS];
+    const bf16x8 zero8 = __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u));
+#pragma unroll
+    for (int ks = 0; ks < SSC_MAXKS; ks++) {
+        bh[ks] = zero8; bl[ks] = zero8;
+        if (ks < nks && pv) {
+            const size_t o = (size_t)pc * kp + ks * 32 + 8 * g;
+            bh[ks] = load_bf16x8(wnh + o);
+            bl[ks] = load_bf16x8(wnl + o);
         }
     }
-    const int pc = p0 + (lane & 15);
-    if (pc >= npk) return;
+    const int jn = ng - 1;
+    double total = 0.;
+    for (int c = 0; c < nch; c++) {
+        const int j = c * SSC_CH + col;          // this lane's A row
+        const bool jv = j < jn;
+        const size_t rowo = ((size_t)i * ng + (jv ? j : 0)) * kp + 8 * g;
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < SSC_MAXKS; ks++) {   // compile-time bound keeps bh, bl in registers
+            if (ks >= nks) break;
+            const bf16x8 ah = jv ? load_bf16x8(Khi + rowo + ks * 32) : zero8;
+            const bf16x8 al = jv ? load_bf16x8(Klo + rowo + ks * 32) : zero8;
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[ks], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[ks], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[ks], acc, 0, 0, 0);
+        }
+        // acc[r] = T[i][c * 16 + 4 g + r][pc]: weight by an and sum over the chunk's 16 bins
+        float part = 0.f;
+        if (pv) {
+            const float4 a4 = *reinterpret_cast<const float4 *>(an + (size_t)pc * nch * SSC_CH + c * SSC_CH + 4 * g);
+            part = a4.x * acc[0] + a4.y * acc[1] + a4.z * acc[2] + a4.w * acc[3];
+        }
+        part += __shfl_xor(part, 16, 64);
+        part += __shfl_xor(part, 32, 64);
+        if (pv) total += amax[(size_t)pc * nch + c] * (double)part;
+    }
+    if (!pv || g != 0) return;
     const int q0 = plist[pc];
     if (S.flag[3 * q0 + 1] == 0) return;
-    const double scale = amax[(size_t)pc * SSC_JB + jb] * wmax[pc];
-    for (int r = 0; r < 4; r++) {
-        const int i = i0 + 4 * (lane >> 4) + r;
-        if (i < nf) S.jssc[((size_t)q0 * nf + i) * SSC_JB + jb] = kH * nu[i] * (scale * (double)acc[r]);
-    }
+    double *J = S.jssc + ((size_t)q0 * nf + i) * SSC_JB;
+    J[0] = kH * nu[i] * (wmax[pc] * total);
+    for (int jb = 1; jb < SSC_JB; jb++) J[jb] = 0.;
 }
 
 // ------------------------------------------------------------ learned SSC emissivity
@@ -942,7 +967,7 @@ static int ele_evolve_impl(const pba_run_params *params, int n_runs, const pba_p
     }
     // BF16 pair of every Compton table for the mixed-precision contraction: slot t's hi part at
     // dKb + kboff[t], its lo part nf * ng * kp entries after it
-    const int kp = (nf + 15) / 16 * 16;
+    const int kp = (nf + 31) / 32 * 32;
     std::vector<size_t> kboff(n_ssc + 1, 0);
     for (size_t t = 0; t < n_ssc; t++) kboff[t + 1] = kboff[t] + 2 * (size_t)nf * keys[slot_key[t]].ng * kp;
     unsigned short *dKb = nullptr;
@@ -1015,17 +1040,22 @@ static int ele_evolve_impl(const pba_run_params *params, int n_runs, const pba_p
                                : (int *)pbagpu::workspace(pbagpu::WS_ELE_LIST, sizeof(int) * hlist.size(), s);
     if (!wsd || (!hlist.empty() && !dlist)) return PBAGPU_E_HIP;
     // operands of the mixed-precision contraction (per population of the SSC lists)
-    float *d_an = nullptr, *d_wn = nullptr;
+    float *d_an = nullptr;
+    unsigned short *d_wnh = nullptr, *d_wnl = nullptr;
     double *d_amax = nullptr, *d_wmax = nullptr;
+    const int NCH = (NG - 1 + SSC_CH - 1) / SSC_CH;   // normalisation chunks of the largest grid
     if (precision == 1 && !hlist.empty()) {
         const size_t nl = hlist.size();
         char *mw = (char *)pbagpu::workspace(pbagpu::WS_ELE_MIXED,
-                                             nl * (sizeof(float) * (NG + kp) + sizeof(double) * (SSC_JB + 1)), s);
+                                             nl * (sizeof(float) * NCH * SSC_CH + sizeof(double) * (NCH + 1) +
+                                                   2 * sizeof(unsigned short) * kp) + 256, s);
         if (!mw) return PBAGPU_E_HIP;
-        d_amax = (double *)mw;
-        d_wmax = d_amax + nl * SSC_JB;
-        d_an = (float *)(d_wmax + nl);
-        d_wn = d_an + nl * NG;
+        // 16-byte vector loads read an, wnh and wnl: they come first, each a multiple of 64 bytes long
+        d_an = (float *)mw;
+        d_wnh = (unsigned short *)(d_an + nl * NCH * SSC_CH);
+        d_wnl = d_wnh + nl * kp;
+        d_amax = (double *)(d_wnl + nl * kp);
+        d_wmax = d_amax + nl * NCH;
     }
     // surrogate: features, outputs and scales of the listed populations, fallback flags, counters
     const int dx = NG + nf + 1;
@@ -1074,8 +1104,17 @@ static int ele_evolve_impl(const pba_run_params *params, int n_runs, const pba_p
     if (ssc_lds > 64 * 1024)
         (void)hipFuncSetAttribute((const void *)ssc_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)ssc_lds);
+    // The per-step launches (3 or more per step) can be captured into one graph and launched once:
+    // the library option PBAGPU_OPT_ELE_GRAPH, for the FP64 and BF16 contractions (the surrogate's
+    // network grows its own workspace during the first steps, which a capture cannot record), on a
+    // created stream (the null stream cannot be captured: its calls stay eager).
+    const bool use_graph = pbagpu::option(PBAGPU_OPT_ELE_GRAPH) != 0 && precision != 2 && s != nullptr &&
+                           s != hipStreamPerThread;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
     if (rc == PBAGPU_OK) {
         tele.start(s);
+        if (use_graph) PBA_TRY(rc, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (int it = 0; it < nt && rc == PBAGPU_OK; it++) {
             hipLaunchKernelGGL(ele_step_kernel, dim3(n_pop), dim3(64), 0, s, dpops, n_pop, it, nt, S);
             for (size_t t = 0; t < n_ssc; t++) {
@@ -1089,14 +1128,14 @@ static int ele_evolve_impl(const pba_run_params *params, int n_runs, const pba_p
                 const double *K = dK + koff[t];
                 if (precision == 1) {
                     const size_t lo = list_off[t];
+                    const int nch = (ng - 1 + SSC_CH - 1) / SSC_CH;   // chunks of this table's grid (<= NCH)
                     hipLaunchKernelGGL(ssc_prep_kernel, dim3((unsigned)npk), dim3(64), 0, s, dlist + lo, npk, n_pop, e, ng,
-                                       nf, kp, S, d_an + lo * NG, d_amax + lo * SSC_JB, d_wn + lo * kp, d_wmax + lo);
-                    hipLaunchKernelGGL(ssc_mfma_kernel, dim3((unsigned)((nf + 15) / 16), (unsigned)((npk + 15) / 16),
-                                                             (unsigned)SSC_JB),
-                                       dim3(64), 0, s, dlist + lo, npk, nu, ng, nf, kp, dKb + kboff[t],
-                                       dKb + kboff[t] + (size_t)nf * ng * kp, d_an + lo * NG, d_amax + lo * SSC_JB,
-                                       d_wn + lo * kp,
-                                       d_wmax + lo, S);
+                                       nf, kp, nch, S, d_an + lo * NCH * SSC_CH, d_amax + lo * NCH, d_wnh + lo * kp,
+                                       d_wnl + lo * kp, d_wmax + lo);
+                    hipLaunchKernelGGL(ssc_mfma_kernel, dim3((unsigned)nf, (unsigned)((npk + 63) / 64)), dim3(256), 0,
+                                       s, dlist + lo, npk, nu, ng, nf, kp, nch, dKb + kboff[t],
+                                       dKb + kboff[t] + (size_t)nf * ng * kp, d_an + lo * NCH * SSC_CH,
+                                       d_amax + lo * NCH, d_wnh + lo * kp, d_wnl + lo * kp, d_wmax + lo, S);
                 } else if (precision == 2) {
                     const size_t lo = list_off[t];
                     hipLaunchKernelGGL(sur_feat_step_kernel, dim3((unsigned)npk), dim3(64), 0, s, dpops, dlist + lo, npk,
@@ -1119,6 +1158,13 @@ static int ele_evolve_impl(const pba_run_params *params, int n_runs, const pba_p
             hipLaunchKernelGGL(ele_emit_kernel, dim3(n_pop), dim3(64), 0, s, dpops, it, nt, S);
             PBA_TRY(rc, hipGetLastError());
         }
+        if (use_graph) {
+            // end the capture whatever happened inside it, then launch the step sequence once
+            hipError_t e = hipStreamEndCapture(s, &graph);
+            if (rc == PBAGPU_OK && e != hipSuccess) { pbagpu::set_error("hipStreamEndCapture", e); rc = PBAGPU_E_HIP; }
+            if (rc == PBAGPU_OK) PBA_TRY(rc, hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+            if (rc == PBAGPU_OK) PBA_TRY(rc, hipGraphLaunch(graph_exec, s));
+        }
         tele.stop(s);
         // status of every population
         if (rc == PBAGPU_OK)
@@ -1129,6 +1175,8 @@ static int ele_evolve_impl(const pba_run_params *params, int n_runs, const pba_p
     SurStats hst{};
     if (d_st && rc == PBAGPU_OK) PBA_TRY(rc, hipMemcpyAsync(&hst, d_st, sizeof(SurStats), hipMemcpyDeviceToHost, s));
     PBA_TRY(rc, hipStreamSynchronize(s));
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
     ttab.collect();
     tele.collect();
     if (stats) {

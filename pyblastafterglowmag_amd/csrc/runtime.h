@@ -33,6 +33,8 @@ void *workspace(int slot, size_t bytes, hipStream_t stream);
 int release_stream(hipStream_t stream);
 // Compute units a stream may use (its CU mask; all of the device for a plain stream).
 int stream_cu_count(hipStream_t stream);
+// Value of a library option (pbagpu_set_option).
+long long option(int which);
 
 }  // namespace pbagpu
 

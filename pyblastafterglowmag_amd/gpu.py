@@ -20,7 +20,9 @@ EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu
            "pbagpu_release_stream", "pbagpu_stream_create", "pbagpu_stream_destroy",
            "pbagpu_synchrotron_f", "pbagpu_eats_lightcurve_vec", "pbagpu_mlp_forward", "pbagpu_mlp_train_step",
            "pbagpu_gemm_bf16", "pbagpu_ssc_features", "pbagpu_ssc_emissivity", "pbagpu_ele_evolve_surrogate",
-           "pbagpu_skymap_hist_noalloc", "pbagpu_ele_evolve_spectra", "pbagpu_dyn_evolve_ragged")
+           "pbagpu_skymap_hist_noalloc", "pbagpu_ele_evolve_spectra", "pbagpu_dyn_evolve_ragged",
+           "pbagpu_set_option", "pbagpu_get_option")
+OPT_ELE_GRAPH = 0
 STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW, STAGE_SKY_HIST, STAGE_EATS_VEC = 0, 1, 2, 3, 4, 5, 6
 N_STAGES = 7
 # sky-map binning (enum pba_sky_moment / pba_sky_status)
@@ -49,6 +51,8 @@ def lib():
         L.pbagpu_ele_evolve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.Population), C.c_int, vp,
                                         C.c_int, C.c_int, vp, vp, vp, vp, C.c_int, vp]
         L.pbagpu_ele_evolve_spectra.argtypes = L.pbagpu_ele_evolve.argtypes[:-1] + [C.POINTER(abi.Spectra), vp]
+        L.pbagpu_set_option.argtypes = [C.c_int, C.c_longlong]
+        L.pbagpu_get_option.argtypes = [C.c_int, C.POINTER(C.c_longlong)]
         L.pbagpu_last_error.restype = C.c_char_p
         L.pbagpu_last_error.argtypes = []
         L.pbagpu_timing_enable.argtypes = [C.c_int]
@@ -81,6 +85,17 @@ def lib():
         L.pbagpu_skymap_hist_noalloc.argtypes = L.pbagpu_skymap_hist.argtypes
         _lib = L
     return _lib
+
+
+def set_option(which, value):
+    """pbagpu_set_option (OPT_ELE_GRAPH: capture the electron stage's per-step launches in a graph)."""
+    _check(lib().pbagpu_set_option(int(which), int(value)), "set_option")
+
+
+def get_option(which):
+    v = C.c_longlong(0)
+    _check(lib().pbagpu_get_option(int(which), C.byref(v)), "get_option")
+    return v.value
 
 
 def release_stream(stream):

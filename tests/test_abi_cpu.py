@@ -49,6 +49,22 @@ def test_version_needs_no_device():
     assert lib.pbagpu_version() > 0
 
 
+
+def test_library_options_need_no_device():
+    """pbagpu_set_option / pbagpu_get_option: PBAGPU_OPT_ELE_GRAPH takes 0 or 1 (default 0); an
+    unknown option or value is PBAGPU_E_ARG and leaves the setting alone."""
+    if not os.path.exists(LIB):
+        pytest.skip("libpbagpu.so not built")
+    lib = C.CDLL(LIB)
+    lib.pbagpu_set_option.argtypes = [C.c_int, C.c_longlong]
+    lib.pbagpu_get_option.argtypes = [C.c_int, C.POINTER(C.c_longlong)]
+    v = C.c_longlong(-5)
+    assert lib.pbagpu_get_option(0, C.byref(v)) == 0 and v.value == 0
+    assert lib.pbagpu_set_option(0, 2) == -1 and lib.pbagpu_set_option(0, -1) == -1
+    assert lib.pbagpu_set_option(1, 0) == -1 and lib.pbagpu_get_option(1, C.byref(v)) == -1
+    assert lib.pbagpu_set_option(0, 1) == 0 and lib.pbagpu_get_option(0, C.byref(v)) == 0 and v.value == 1
+    assert lib.pbagpu_set_option(0, 0) == 0 and lib.pbagpu_get_option(0, C.byref(v)) == 0 and v.value == 0
+
 def test_skymap_hist_rejects_bad_arguments_before_any_device_call():
     """pbagpu_skymap_hist checks its host arrays first: PBAGPU_E_ARG (-1) with a message, no HIP call."""
     if not os.path.exists(LIB):

@@ -1,0 +1,71 @@
+"""Electron stage with its per-step launches captured in one graph (library option
+PBAGPU_OPT_ELE_GRAPH) against the same stage launched eagerly: the kernels and their arguments
+are the same, so every output is bit-identical.  Three graph calls in a row, each a fresh capture
+and launch, with eager library and torch work on the same device between them."""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import gpu_pipeline
+
+pytestmark = pytest.mark.gpu
+
+
+def _outputs(pl):
+    out = {"jall": pl.d_jall, "aall": pl.d_aall, "fall": pl.d_fall, "est": pl.d_est}
+    return {k: v.detach().cpu().numpy().copy() for k, v in out.items() if v is not None}
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_electron_graph_matches_eager(precision):
+    import torch
+    from pyblastafterglowmag_amd import gpu
+    run, pl0 = gpu_pipeline("gauss5_small")
+    # the same dynamics, with the electron stage on a created stream (the null stream stays eager)
+    stream = torch.cuda.Stream()
+    pl = gpu.Pipeline([run], precision=precision, stream=stream)
+    pl.dynamics()
+    print(f"[gpu] electron stage graph vs eager, precision {precision}, {len(pl.pops)} populations, "
+          f"{pl.nt} steps", flush=True)
+    assert gpu.get_option(gpu.OPT_ELE_GRAPH) == 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pl.electrons(want_f=True)
+    torch.cuda.synchronize()
+    t_eager = time.perf_counter() - t0
+    ref = _outputs(pl)
+    # the null-stream pipeline with the option on: launched directly, same results
+    pl0.precision = precision
+    try:
+        gpu.set_option(gpu.OPT_ELE_GRAPH, 1)
+        pl0.electrons(want_f=True)
+    finally:
+        gpu.set_option(gpu.OPT_ELE_GRAPH, 0)
+    got0 = _outputs(pl0)
+    for k, v in ref.items():
+        assert np.array_equal(got0[k], v, equal_nan=True), (precision, "null stream", k)
+    assert not ref["est"].any(), ref["est"]
+    try:
+        for rep in range(3):
+            gpu.set_option(gpu.OPT_ELE_GRAPH, 1)
+            assert gpu.get_option(gpu.OPT_ELE_GRAPH) == 1
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            pl.electrons(want_f=True)
+            torch.cuda.synchronize()
+            t_graph = time.perf_counter() - t0
+            got = _outputs(pl)
+            for k, v in ref.items():
+                assert np.array_equal(got[k], v, equal_nan=True), (precision, rep, k)
+            print(f"[gpu]   call {rep}: bit-identical; {t_graph * 1e3:.1f} ms (eager {t_eager * 1e3:.1f} ms)",
+                  flush=True)
+            # eager work between the graph calls: the library's eager stage and a torch reduction
+            gpu.set_option(gpu.OPT_ELE_GRAPH, 0)
+            pl.electrons()
+            s = torch.ones(1 << 20, dtype=torch.float64, device=pl.dev).sum().item()
+            assert s == float(1 << 20)
+            assert np.array_equal(pl.d_jall.cpu().numpy(), ref["jall"])
+    finally:
+        gpu.set_option(gpu.OPT_ELE_GRAPH, 0)
+
