@@ -158,7 +158,15 @@ def cpu_baseline(workload, budget_s):
         stride //= 2
         print(f"[cpu] observer {k_pair} pairs {t_eats:.1f}s", file=sys.stderr, flush=True)
     per_lc = t_dyn + t_ele / max(k_pop, 1) * n_pop + t_eats / max(k_pair, 1) * n_pair
+    # One light curve of this workload's parfile, one thread each, on an 8-core Xeon at 2.7 GHz: the
+    # reference program (oracle/_ref/pba.out, built from its sources) 1494.1 s, the oracle 1137.8 s
+    # (dynamics 31.3 s, electrons 152.8 s, observer 953.6 s).  The reference is this much slower than
+    # its restatement on the same host.
+    ref_over_port = 1494.1 / 1137.8
     return {"value": 1.0 / per_lc, "unit": "light curves/s", "cores": threads, "kind": "port",
+            "calibration": {"reference_s": 1494.1, "port_s": 1137.8, "reference_over_port": round(ref_over_port, 3),
+                            "host": "8-core Xeon 2.7 GHz, one thread each, the gauss21_fsrs_ssc parfile",
+                            "implied_reference_value": 1.0 / per_lc / ref_over_port},
             "sample": (f"oracle (CPU restatement, OpenMP over populations and layers, {threads} threads) on one "
                        f"{workload} run: full dynamics {t_dyn:.1f}s, electrons of {k_pop}/{n_pop} populations "
                        f"{t_ele:.1f}s, observer quadrature of all layers at {k_pair}/{n_pair} pairs {t_eats:.1f}s; "

@@ -210,11 +210,17 @@ int pbagpu_timing_units(int stage, double *units, int reset);
  *                         instead of launching them one by one (FP64 and BF16 contractions;
  *                         the surrogate mode always launches directly), on a created stream
  *                         (calls on the null stream, which cannot be captured, stay direct).
- *                         Results are identical.  Returns PBAGPU_E_ARG for an unknown option or
- *                         a value other than 0 and 1. */
+ *                         Results are identical.
+ *   PBAGPU_OPT_DYN_RUNS_PER_WAVE  runs the dynamics stage packs into one 64-lane wavefront: 0 (the
+ *                         default) only as many as the runs outnumber the SIMDs of the stream's
+ *                         compute units (at most floor(64 / bw_per_run)); n >= 1 at most n.  A
+ *                         run's results do not depend on the packing.
+ * Returns PBAGPU_E_ARG for an unknown option or a value out of its range (graph 0 or 1, runs per
+ * wavefront 0 .. 64). */
 enum {
     PBAGPU_OPT_ELE_GRAPH = 0,
-    PBAGPU_N_OPTIONS = 1
+    PBAGPU_OPT_DYN_RUNS_PER_WAVE = 1,
+    PBAGPU_N_OPTIONS = 2
 };
 int pbagpu_set_option(int which, long long value);
 int pbagpu_get_option(int which, long long *value);

@@ -208,6 +208,10 @@ extern "C" int pbagpu_set_option(int which, long long value) {
         pbagpu::set_error_msg("pbagpu_set_option: PBAGPU_OPT_ELE_GRAPH takes 0 or 1");
         return PBAGPU_E_ARG;
     }
+    if (which == PBAGPU_OPT_DYN_RUNS_PER_WAVE && (value < 0 || value > 64)) {
+        pbagpu::set_error_msg("pbagpu_set_option: PBAGPU_OPT_DYN_RUNS_PER_WAVE takes 0 .. 64");
+        return PBAGPU_E_ARG;
+    }
     pbagpu::g_options[which].store(value);
     return PBAGPU_OK;
 }

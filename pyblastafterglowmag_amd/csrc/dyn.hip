@@ -71,7 +71,7 @@ __device__ double dthetadr_our(const BW &b, double Gamma, double Gsh, double R, 
 // Integrated WSPN99 synchrotron power (analytic_models.h:889-911, 1051-1078).  Every power x^a
 // is exp(a ln x) with each logarithm taken once (3 logarithms and 8 exponentials instead of 10
 // calls of pow); nu0_43 = nu0^(4/3) is a constant of the run.
-__device__ double wspn99_ptot(double p, double gm, double gc, double gM, double B, double nprime, double nu0_43) {
+__device__ __forceinline__ double wspn99_ptot(double p, double gm, double gc, double gM, double B, double nprime, double nu0_43) {
     double f = (3.0 / (4.0 * kPi)) * (kQe * B) / (kMe * kC);
     double nu_m, nu_c, p_max;
     if (gm < gc) {
@@ -514,38 +514,71 @@ __constant__ double st_a[13][9] = {
     {a81, a84, a85, a86, a87}, {a91, a94, a95, a96, a97, a98}, {aA1, aA4, aA5, aA6, aA7, aA8, aA9},
     {aB1, aB4, aB5, aB6, aB7, aB8, aB9, aBA}, {aC1, aC4, aC5, aC6, aC7, aC8, aC9, aCA, aCB}};
 
-// Deterministic block-wide sum of one value per lane, added in lane order (all lanes get the same result).
-__device__ __forceinline__ double ordered_sum(double v, double *lds, int n) {
-    int lane = threadIdx.x;
+// Sums of two values per lane over every run's lanes [base, base + n), added in lane order (each
+// lane gets its run's sums).  Called by the whole wavefront.
+__device__ __forceinline__ void run_sums(double &v1, double &v2, double *red, int base, int n) {
+    const int lane = threadIdx.x;
     __syncthreads();
-    lds[lane] = v;
+    red[lane] = v1;
+    red[64 + lane] = v2;
     __syncthreads();
-    double s = 0.;
-    for (int i = 0; i < n; i++) s += lds[i];
-    return s;
+    double s1 = 0., s2 = 0.;
+    for (int i = 0; i < n; i++) { s1 += red[base + i]; s2 += red[64 + base + i]; }
+    v1 = s1;
+    v2 = s2;
 }
 
-// Program of one output interval: every pass of the interval loop evaluates the right-hand
-// side exactly once (one call site, so one copy of the RHS in the kernel's code) and then
-// moves the integrator to the state that needs the next evaluation.
+// Program of one output interval: every pass of the loop evaluates the right-hand side exactly
+// once (one call site, so one copy of the RHS in the kernel's code) and then moves the integrator
+// to the state that needs the next evaluation.
 enum Phase { PH_K1I, PH_H2, PH_STAGE, PH_FSAL, PH_RK1, PH_RK2, PH_RK3, PH_RK4 };
 
+// One wavefront integrates rpw runs: lane l owns blastwave l mod bw_per_run of run
+// blockIdx.x * rpw + l / bw_per_run (rpw * bw_per_run <= 64).  Each run keeps its own output
+// interval, step size and integrator phase in its lanes, so the runs of a wavefront need not step
+// together; every pass of the loop evaluates one right-hand side on every lane whose run is still
+// integrating, and the loop ends when the last of them has done its grid.  The run-wide norms of
+// the step control are sums over the run's lanes in lane order, as with one run per wavefront, so
+// a run's arithmetic does not depend on its neighbours.
+// KG = false: one run per wavefront (rpw = 1), stage vectors in LDS as [KS][NA][bw_per_run].
+// KG = true: stage vectors in the global work array kg, [wavefront][KS][NA][64] (a wavefront of
+// three 21-blastwave runs would need 82 KB of LDS, one wavefront per CU).
 // nt is the table's step stride; run r integrates nt_run[r] steps (all nt when nt_run is null) on
 // the time grid tgrid + r * tg_stride (tg_stride 0: one grid for every run).
+template <bool KG>
 __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restrict__ params,
-                                                 const pba_layer_init *__restrict__ init, int bw_per_run, int nt,
-                                                 const int *__restrict__ nt_run, const double *__restrict__ tgrid_all,
-                                                 int tg_stride, double *__restrict__ table,
-                                                 pba_bw_result *__restrict__ res) {
-    extern __shared__ double dyn_lds[];   // [KS][NA][bw_per_run] stage vectors, then 64 doubles for sums
-    double *red = dyn_lds + (size_t)KS * NA * bw_per_run;
-    const int run = blockIdx.x;
+                                                 const pba_layer_init *__restrict__ init, int n_runs, int bw_per_run,
+                                                 int rpw, int nt, const int *__restrict__ nt_run,
+                                                 const double *__restrict__ tgrid_all, int tg_stride,
+                                                 double *__restrict__ table, pba_bw_result *__restrict__ res,
+                                                 double *__restrict__ kg) {
+    // LDS: (KG = false: [KS][NA][bw_per_run] stage vectors,) the state Y as [NA][ys], 128 doubles for
+    // sums, then one BW per lane slot (ys slots)
+    extern __shared__ double dyn_lds[];
+    const int ys = KG ? 64 : bw_per_run;
+    double *ylds = KG ? dyn_lds : dyn_lds + (size_t)KS * NA * bw_per_run;
+    double *red = ylds + (size_t)NA * ys;
+    BW *bws = reinterpret_cast<BW *>(red + 128);
     const int lane = threadIdx.x;
-    const bool active = lane < bw_per_run;
-    const int ntr = nt_run ? nt_run[run] : nt;
-    const double *__restrict__ tgrid = tgrid_all + (size_t)run * tg_stride;
-    auto K = [&](int slot, int c) -> double & { return dyn_lds[(slot * NA + c) * bw_per_run + lane]; };
-    const pba_run_params P = params[run];
+    const int g = lane / bw_per_run, li = lane - g * bw_per_run;
+    const int run0 = blockIdx.x * rpw;
+    const int run = run0 + g;
+    const bool active = g < rpw && run < n_runs;           // the lane holds a blastwave
+    const int runc = KG ? (active ? run : run0) : run0;     // one run per wavefront: uniform
+    const int base = active ? g * bw_per_run : 0;          // first lane of the lane's run
+    double *kw = KG ? kg + (size_t)blockIdx.x * KS * NA * 64 : nullptr;
+    // the ODE state of the lane's blastwave (in LDS: registers are the loop's scarcest resource)
+    auto Yv = [&](int c) -> double & { return ylds[c * ys + lane]; };
+    auto K = [&](int slot, int c) -> double & {
+        if constexpr (KG) return kw[(slot * NA + c) * 64 + lane];
+        else return dyn_lds[(slot * NA + c) * bw_per_run + lane];
+    };
+    const int ntr = nt_run ? nt_run[runc] : nt;
+    const double *__restrict__ tgrid = tgrid_all + (size_t)runc * tg_stride;
+    // one run per wavefront: a uniform copy (scalar registers); packed: read where used (a
+    // lane-varying copy of the 400-byte struct would live in scratch)
+    const pba_run_params Pu = params[run0];
+    const pba_run_params &P = KG ? params[runc] : Pu;
     Shock shfs, shrs;
     {
         double step = exp(log(P.fs.gam2) / (double)P.fs.ngam);
@@ -554,16 +587,21 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
         shrs.e0 = pow(stepr, 0.); shrs.etop = pow(stepr, (double)(P.rs.ngam - 1));
         shfs.nu0_43 = shrs.nu0_43 = pow(1e6, 4. / 3.);   // computeRadiationLoss's nu0 = 1e6 Hz
     }
-    BW b{};
-    double Y[NA], x[NA];
-    for (int k = 0; k < NA; k++) { Y[k] = 0.; x[k] = 0.; }
-    if (active)
+    // the blastwave's constants and flags (in LDS; padding lanes alias slot 0 and never touch it)
+    BW &b = bws[active ? lane : 0];
+    if (active) b = BW{};
+    double x[NA];
+    for (int k = 0; k < NA; k++) x[k] = 0.;
+    if (active) {
+        for (int k = 0; k < NA; k++) Yv(k) = 0.;
         for (int sl = 0; sl < KS; sl++)
             for (int k = 0; k < NA; k++) K(sl, k) = 0.;
-    const size_t bw_global = (size_t)run * bw_per_run + lane;
-    {   // clear this run's tables, all lanes striding over contiguous memory
-        double *Tr = table + (size_t)run * bw_per_run * PBA_NV * nt;
-        const size_t n = (size_t)bw_per_run * PBA_NV * nt;
+    }
+    const size_t bw_global = (size_t)run * bw_per_run + li;
+    {   // clear the tables of this wavefront's runs, all lanes striding over contiguous memory
+        const int nrb = min(rpw, n_runs - run0);
+        double *Tr = table + (size_t)run0 * bw_per_run * PBA_NV * nt;
+        const size_t n = (size_t)nrb * bw_per_run * PBA_NV * nt;
         for (size_t i = lane; i < n; i += 64) Tr[i] = 0.;
         __syncthreads();
     }
@@ -599,206 +637,208 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
             b.beta0 = beta_from_gamma(b.Gamma0);
             double M20 = (2.0 / 3.0) * kPi * (1.0 - cos(b.theta_b0)) * b.rho_def * pow(b.R0, 3) / b.ncells;
             if (b.M0 <= 0.) b.M0 = b.E0 / (b.Gamma0 * kC * kC);
-            Y[iR] = b.R0;
+            Yv(iR) = b.R0;
             double b0m = beta_from_mom(b.mom0);
-            Y[itt] = b.R0 / (kC * b0m) * sqrt(1. + b.R0 * b.R0 * 0. * 0.) - b.R0 / kC;
-            Y[itcomov] = b.R0 / (b.beta0 * b.Gamma0 * kC);
-            Y[iEint2] = (b.Gamma0 - 1.) * M20 / b.M0 + 0. / b.M0 / kC / kC;
-            Y[iGamma] = b.Gamma0;
-            Y[itheta] = b.theta_b0;
-            Y[iM2] = M20 / b.M0;
-            if (b.fsrs && P.init_deltaR4) Y[ideltaR4] = kC * P.tprompt * b.beta0;
-            b.Rw0 = .5 * Y[iR];
-            b.Wtt0 = Y[itt];
+            Yv(itt) = b.R0 / (kC * b0m) * sqrt(1. + b.R0 * b.R0 * 0. * 0.) - b.R0 / kC;
+            Yv(itcomov) = b.R0 / (b.beta0 * b.Gamma0 * kC);
+            Yv(iEint2) = (b.Gamma0 - 1.) * M20 / b.M0 + 0. / b.M0 / kC / kC;
+            Yv(iGamma) = b.Gamma0;
+            Yv(itheta) = b.theta_b0;
+            Yv(iM2) = M20 / b.M0;
+            if (b.fsrs && P.init_deltaR4) Yv(ideltaR4) = kC * P.tprompt * b.beta0;
+            b.Rw0 = .5 * Yv(iR);
+            b.Wtt0 = Yv(itt);
         }
         double yc[NA];
-        for (int k = 0; k < NA; k++) yc[k] = Y[k];
+        for (int k = 0; k < NA; k++) yc[k] = Yv(k);
         post_step(P, shfs, shrs, b, yc, 0, tgrid[0], T, nt);
     }
     const double rtol = P.rtol, atol = P.rtol;
     const double nsz = (double)(NEQ_REF * bw_per_run);
     double time = P.tb0;
     int nsteps = 0, nrhs = 0;
-    for (int it = 1; it < ntr; it++) {
-        const double timeN = time + (tgrid[it] - tgrid[it - 1]);
-        const double hmax = fabs(timeN - time);
-        const double hrk = tgrid[it] - tgrid[it - 1];
-        int ph = P.integrator == 1 ? PH_RK1 : PH_K1I;
-        int st = 0, outslot = 0, guard = 0;
-        double h = 0., h1 = 0., dnf = 0.;
-        bool reject = false, last = false;
-        for (int k = 0; k < NA; k++) x[k] = Y[k];
-        // x_s for stage s, from the stage vectors in LDS
-        auto combo = [&](int s) {
-            if (!active) return;
-            const int n = st_n[s];
-            for (int k = 0; k < NA; k++) x[k] = st_a[s][0] * K(st_slot[s][0], k);
-            for (int j = 1; j < n; j++) {
-                const double a = st_a[s][j];
-                const int sl = st_slot[s][j];
-                for (int k = 0; k < NA; k++) x[k] += a * K(sl, k);
-            }
-            for (int k = 0; k < NA; k++) x[k] = Y[k] + h * x[k];
-        };
-        // start an attempt of one DOP853 step of size h (ode_solvers.h:1434-1479)
-        auto begin_attempt = [&]() {
-            if ((time + 1.01 * h - timeN) > 0.0) { h = timeN - time; last = true; }
-            st = 2;
-            combo(2);
-            outslot = st_out[2];
-            ph = PH_STAGE;
-        };
-        for (;;) {
-            double o[NA];
-            for (int k = 0; k < NA; k++) o[k] = 0.;
-            rhs(P, shfs, b, active, x, o);   // the one right-hand-side evaluation of the loop
-            nrhs++;
+    // the run's state inside its current output interval it
+    int it = 1;
+    bool live = active && ntr > 1;
+    double timeN = 0., hmax = 0., hrk = 0.;
+    int ph = PH_K1I, st = 0, outslot = 0, guard = 0;
+    double h = 0., h1 = 0., dnf = 0.;
+    bool reject = false, last = false;
+    // x_s for stage s, from the stage vectors
+    auto combo = [&](int s) {
+        const int n = st_n[s];
+        for (int k = 0; k < NA; k++) x[k] = st_a[s][0] * K(st_slot[s][0], k);
+        for (int j = 1; j < n; j++) {
+            const double a = st_a[s][j];
+            const int sl = st_slot[s][j];
+            for (int k = 0; k < NA; k++) x[k] += a * K(sl, k);
+        }
+        for (int k = 0; k < NA; k++) x[k] = Yv(k) + h * x[k];
+    };
+    // start an attempt of one DOP853 step of size h (ode_solvers.h:1434-1479)
+    auto begin_attempt = [&]() {
+        if ((time + 1.01 * h - timeN) > 0.0) { h = timeN - time; last = true; }
+        st = 2;
+        combo(2);
+        outslot = st_out[2];
+        ph = PH_STAGE;
+    };
+    auto start_interval = [&]() {
+        timeN = time + (tgrid[it] - tgrid[it - 1]);
+        hmax = fabs(timeN - time);
+        hrk = tgrid[it] - tgrid[it - 1];
+        ph = P.integrator == 1 ? PH_RK1 : PH_K1I;
+        st = 0; outslot = 0; guard = 0;
+        h = 0.; h1 = 0.; dnf = 0.;
+        reject = false; last = false;
+        for (int k = 0; k < NA; k++) x[k] = Yv(k);
+    };
+    if (live) start_interval();
+    while (__ballot(live) != 0ull) {
+        double o[NA];
+        for (int k = 0; k < NA; k++) o[k] = 0.;
+        rhs(P, shfs, b, live, x, o);   // the one right-hand-side evaluation of the loop
+        if (live) nrhs++;
+        // per-lane terms of the run-wide norms this pass needs (initial step, error estimate)
+        double c1 = 0., c2 = 0.;
+        bool need = false;
+        if (live) {
             if (ph == PH_K1I) {
                 // k1 = f(Y), then the initial step (ode_solvers.h:1596-1628)
-                double dny = 0.;
-                dnf = 0.;
-                if (active) {
-                    for (int k = 0; k < NA; k++) {
-                        K(0, k) = o[k];
-                        double sk = atol + rtol * fabs(Y[k]);
-                        dnf += (o[k] / sk) * (o[k] / sk);
-                        dny += (Y[k] / sk) * (Y[k] / sk);
-                    }
-                    double skw = atol + rtol * fabs(b.Rw0);
-                    dny += (b.Rw0 / skw) * (b.Rw0 / skw);
-                    double skt = atol + rtol * fabs(b.Wtt0);
-                    dny += (b.Wtt0 / skt) * (b.Wtt0 / skt);
+                need = true;
+                for (int k = 0; k < NA; k++) {
+                    K(0, k) = o[k];
+                    double sk = atol + rtol * fabs(Yv(k));
+                    c1 += (o[k] / sk) * (o[k] / sk);
+                    c2 += (Yv(k) / sk) * (Yv(k) / sk);
                 }
-                dnf = ordered_sum(dnf, red, bw_per_run);
-                dny = ordered_sum(dny, red, bw_per_run);
-                h1 = (dnf <= 1e-10 || dny <= 1e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
-                h1 = fmin(h1, hmax);
-                for (int k = 0; k < NA; k++) x[k] = Y[k] + h1 * o[k];
-                ph = PH_H2;
-                continue;
-            }
-            if (ph == PH_H2) {
-                double n2 = 0.;
-                if (active)
-                    for (int k = 0; k < NA; k++) {
-                        double sk = atol + rtol * x86_int_abs(Y[k]);
-                        double z = (o[k] - K(0, k)) / sk;
-                        n2 += z * z;
-                    }
-                n2 = ordered_sum(n2, red, bw_per_run);
-                n2 = sqrt(n2) / h1;
-                double der12 = fmax(n2, sqrt(dnf));
-                double h2 = (der12 <= 1e-15) ? fmax(1e-6, fabs(h1) * 1e-3) : pow(0.01 / der12, 1.0 / 8);
-                h = fmin(fmin(100 * h1, h2), hmax);
-                begin_attempt();
-                continue;
-            }
-            if (ph == PH_STAGE) {
-                if (active)
-                    for (int k = 0; k < NA; k++) K(outslot, k) = o[k];
-                if (st < 12) {
-                    st++;
-                    combo(st);
-                    outslot = st_out[st];
-                    continue;
+                double skw = atol + rtol * fabs(b.Rw0);
+                c2 += (b.Rw0 / skw) * (b.Rw0 / skw);
+                double skt = atol + rtol * fabs(b.Wtt0);
+                c2 += (b.Wtt0 / skt) * (b.Wtt0 / skt);
+            } else if (ph == PH_H2) {
+                need = true;
+                for (int k = 0; k < NA; k++) {
+                    double sk = atol + rtol * x86_int_abs(Yv(k));
+                    double z = (o[k] - K(0, k)) / sk;
+                    c1 += z * z;
                 }
-                // all 12 stages done: 8th-order solution and the error estimate
-                double e1s = 0., e2s = 0.;
-                if (active)
+            } else if (ph == PH_STAGE) {
+                for (int k = 0; k < NA; k++) K(outslot, k) = o[k];
+                if (st == 12) {
+                    // all 12 stages done: 8th-order solution and the error estimate
+                    need = true;
                     for (int k = 0; k < NA; k++) {
                         const double k1 = K(0, k), k6 = K(3, k), k7 = K(4, k), k8 = K(5, k), k9 = K(6, k),
                                      kA = K(7, k), kB = K(8, k), kC = K(2, k);
                         double slope = b1 * k1 + b6 * k6 + b7 * k7 + b8 * k8 + b9 * k9 + bA * kA + bB * kB + bC * kC;
-                        double yn = Y[k] + h * slope;
-                        double sk = atol + rtol * fmax(fabs(Y[k]), fabs(yn));
+                        double yn = Yv(k) + h * slope;
+                        double sk = atol + rtol * fmax(fabs(Yv(k)), fabs(yn));
                         double e1 = er1 * k1 + er6 * k6 + er7 * k7 + er8 * k8 + er9 * k9 + erA * kA + erB * kB +
                                     erC * kC;
                         double e2 = slope - bhh1 * k1 - bhh9 * k9 - bhhC * kC;
-                        e1s += (e1 / sk) * (e1 / sk);
-                        e2s += (e2 / sk) * (e2 / sk);
+                        c1 += (e1 / sk) * (e1 / sk);
+                        c2 += (e2 / sk) * (e2 / sk);
                         x[k] = yn;
                     }
-                e1s = ordered_sum(e1s, red, bw_per_run);
-                e2s = ordered_sum(e2s, red, bw_per_run);
-                double deno = e1s + 0.01 * e2s;
-                if (deno <= 0.0) deno = 1.0;
-                double err = fabs(h) * e1s / sqrt(nsz * deno);
-                double fac11 = pow(err, 1.0 / 8.0);
-                if (err > 1.0) {
-                    h /= fmin(1.0 / 0.333, fac11 / 0.9);
-                    reject = true;
-                    last = false;
-                    if (guard == kMaxStepsPerInterval - 1) { b.status |= PBAGPU_S_NMAX; break; }
-                    guard++;
-                    begin_attempt();
-                    continue;
                 }
-                double fac = fac11 / (1.0 * 0.9);
-                fac = fac < 1.0 / 6.0 ? 1.0 / 6.0 : (fac > 1.0 / 0.333 ? 1.0 / 0.333 : fac);
-                h1 = h / fac;   // the next step size, before the limits applied after acceptance
-                ph = PH_FSAL;   // f(y_new) becomes k1 of the next step
-                continue;
             }
-            if (ph == PH_FSAL) {
-                if (active)
-                    for (int k = 0; k < NA; k++) { K(0, k) = o[k]; Y[k] = x[k]; }
+        }
+        if (__ballot(need) != 0ull) run_sums(c1, c2, red, base, bw_per_run);
+        bool iv_done = false;
+        if (live) {
+            if (ph == PH_K1I) {
+                dnf = c1;
+                const double dny = c2;
+                h1 = (dnf <= 1e-10 || dny <= 1e-10) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
+                h1 = fmin(h1, hmax);
+                for (int k = 0; k < NA; k++) x[k] = Yv(k) + h1 * o[k];
+                ph = PH_H2;
+            } else if (ph == PH_H2) {
+                const double n2 = sqrt(c1) / h1;
+                double der12 = fmax(n2, sqrt(dnf));
+                double h2 = (der12 <= 1e-15) ? fmax(1e-6, fabs(h1) * 1e-3) : pow(0.01 / der12, 1.0 / 8);
+                h = fmin(fmin(100 * h1, h2), hmax);
+                begin_attempt();
+            } else if (ph == PH_STAGE) {
+                if (st < 12) {
+                    st++;
+                    combo(st);
+                    outslot = st_out[st];
+                } else {
+                    const double e1s = c1, e2s = c2;
+                    double deno = e1s + 0.01 * e2s;
+                    if (deno <= 0.0) deno = 1.0;
+                    double err = fabs(h) * e1s / sqrt(nsz * deno);
+                    double fac11 = pow(err, 1.0 / 8.0);
+                    if (err > 1.0) {
+                        h /= fmin(1.0 / 0.333, fac11 / 0.9);
+                        reject = true;
+                        last = false;
+                        if (guard == kMaxStepsPerInterval - 1) { b.status |= PBAGPU_S_NMAX; iv_done = true; }
+                        else { guard++; begin_attempt(); }
+                    } else {
+                        double fac = fac11 / (1.0 * 0.9);
+                        fac = fac < 1.0 / 6.0 ? 1.0 / 6.0 : (fac > 1.0 / 0.333 ? 1.0 / 0.333 : fac);
+                        h1 = h / fac;   // the next step size, before the limits applied after acceptance
+                        ph = PH_FSAL;   // f(y_new) becomes k1 of the next step
+                    }
+                }
+            } else if (ph == PH_FSAL) {
+                for (int k = 0; k < NA; k++) { K(0, k) = o[k]; Yv(k) = x[k]; }
                 time += h;
                 nsteps++;
-                if (last) break;
-                double hnew = h1;
-                if (fabs(hnew) > hmax) hnew = hmax;
-                if (reject) hnew = fmin(fabs(hnew), fabs(h));
-                reject = false;
-                if (guard == kMaxStepsPerInterval - 1) { b.status |= PBAGPU_S_NMAX; break; }
-                guard++;
-                h = hnew;
-                begin_attempt();
-                continue;
-            }
-            // RK4: one classical step over the output interval (rk4_integrator, ode_solvers.h:319-366),
-            // every blastwave on its own (the step has no error control that couples them);
-            // the running increment is kept in slot 3
-            if (ph == PH_RK1) {
-                if (active)
-                    for (int k = 0; k < NA; k++) { K(3, k) = (1.0 / 6.0) * hrk * o[k]; x[k] = Y[k] + 0.5 * hrk * o[k]; }
+                if (last) iv_done = true;
+                else {
+                    double hnew = h1;
+                    if (fabs(hnew) > hmax) hnew = hmax;
+                    if (reject) hnew = fmin(fabs(hnew), fabs(h));
+                    reject = false;
+                    if (guard == kMaxStepsPerInterval - 1) { b.status |= PBAGPU_S_NMAX; iv_done = true; }
+                    else { guard++; h = hnew; begin_attempt(); }
+                }
+            } else if (ph == PH_RK1) {
+                // RK4: one classical step over the output interval (rk4_integrator, ode_solvers.h:319-366),
+                // every blastwave on its own (the step has no error control that couples them);
+                // the running increment is kept in slot 3
+                for (int k = 0; k < NA; k++) { K(3, k) = (1.0 / 6.0) * hrk * o[k]; x[k] = Yv(k) + 0.5 * hrk * o[k]; }
                 ph = PH_RK2;
-                continue;
-            }
-            if (ph == PH_RK2 || ph == PH_RK3) {
+            } else if (ph == PH_RK2 || ph == PH_RK3) {
                 const double c = ph == PH_RK2 ? 0.5 : 1.0;
-                if (active)
-                    for (int k = 0; k < NA; k++) { K(3, k) += (2.0 / 6.0) * hrk * o[k]; x[k] = Y[k] + c * hrk * o[k]; }
+                for (int k = 0; k < NA; k++) { K(3, k) += (2.0 / 6.0) * hrk * o[k]; x[k] = Yv(k) + c * hrk * o[k]; }
                 ph = ph == PH_RK2 ? PH_RK3 : PH_RK4;
-                continue;
-            }
-            // PH_RK4
-            if (active)
+            } else {   // PH_RK4
                 for (int k = 0; k < NA; k++) {
                     const double a = K(3, k) + (1.0 / 6.0) * hrk * o[k];
-                    Y[k] += a;
+                    Yv(k) += a;
                 }
-            time += hrk;
-            nsteps++;
-            break;
-        }
-        if (active && !b.end_evolution) {
-            double g = Y[iGamma], be = beta_from_gamma(g), E2 = Y[iEint2];
-            if ((E2 <= 0.) || !isfinite(E2) || (g < 1.) || (be < 0) || (fabs(be) < 1.e-6) || !isfinite(be))
-                b.status |= PBAGPU_S_TERMINATED;
-            if (!b.end_spreading && Y[itheta] > 0.99 * b.theta_max) b.end_spreading = 1;
-            if (!(b.shutOff || !b.do_rs)) {
-                double alpha_of = P.tprompt * b.beta0 * kC;
-                double rho4f = b.M0 / (2.0 * alpha_of * kPi * (1.0 - cos(b.theta_b0))) / (Y[iR] * Y[iR]);
-                double rho4 = P.exponential_rho4 ? rho4f * exp(-Y[ideltaR4] / alpha_of) : rho4f;
-                if ((Y[iGamma] < b.Gamma0) && (rho4 < P.rs_shutoff_rho || Y[iM3] >= 1.)) b.shutOff = 1;
+                time += hrk;
+                nsteps++;
+                iv_done = true;
             }
-            if (g < 1. || Y[itheta] < b.theta_b0 * 0.99 || Y[itt] <= 0.) b.status |= PBAGPU_S_NONFINITE;
         }
-        b.prev_idx_x = it;
-        if (active) {
+        if (iv_done) {
+            // end of output interval it (advanceTimeSubStep's checks, then insertSolution)
+            if (!b.end_evolution) {
+                double gg = Yv(iGamma), be = beta_from_gamma(gg), E2 = Yv(iEint2);
+                if ((E2 <= 0.) || !isfinite(E2) || (gg < 1.) || (be < 0) || (fabs(be) < 1.e-6) || !isfinite(be))
+                    b.status |= PBAGPU_S_TERMINATED;
+                if (!b.end_spreading && Yv(itheta) > 0.99 * b.theta_max) b.end_spreading = 1;
+                if (!(b.shutOff || !b.do_rs)) {
+                    double alpha_of = P.tprompt * b.beta0 * kC;
+                    double rho4f = b.M0 / (2.0 * alpha_of * kPi * (1.0 - cos(b.theta_b0))) / (Yv(iR) * Yv(iR));
+                    double rho4 = P.exponential_rho4 ? rho4f * exp(-Yv(ideltaR4) / alpha_of) : rho4f;
+                    if ((Yv(iGamma) < b.Gamma0) && (rho4 < P.rs_shutoff_rho || Yv(iM3) >= 1.)) b.shutOff = 1;
+                }
+                if (gg < 1. || Yv(itheta) < b.theta_b0 * 0.99 || Yv(itt) <= 0.) b.status |= PBAGPU_S_NONFINITE;
+            }
+            b.prev_idx_x = it;
             double yc[NA];   // a copy, so that Y itself stays in registers
-            for (int k = 0; k < NA; k++) yc[k] = Y[k];
+            for (int k = 0; k < NA; k++) yc[k] = Yv(k);
             post_step(P, shfs, shrs, b, yc, it, tgrid[it], T, nt);
+            it++;
+            if (it < ntr) start_interval();
+            else live = false;
         }
     }
     if (active) {
@@ -849,14 +889,42 @@ static int dyn_evolve_impl(const pba_run_params *params, int n_runs, int bw_per_
     PBA_TRY(rc, hipMemcpyAsync(dP, params, sizeof(pba_run_params) * n_runs, hipMemcpyHostToDevice, s));
     PBA_TRY(rc, hipMemcpyAsync(dI, init, sizeof(pba_layer_init) * nbw, hipMemcpyHostToDevice, s));
     if (nt_run) PBA_TRY(rc, hipMemcpyAsync(dN, nt_run, sizeof(int) * n_runs, hipMemcpyHostToDevice, s));
+    // Runs per wavefront.  A wavefront's pass time grows with the runs it carries (on a full chip,
+    // three 21-blastwave runs per wavefront take 2.7 times one run's time), so runs are packed only
+    // as far as they outnumber the SIMDs the stream may use (one wavefront per SIMD: the kernel
+    // needs all 512 registers of a lane); the option overrides this with a fixed count.
+    const long long opt_rpw = pbagpu::option(PBAGPU_OPT_DYN_RUNS_PER_WAVE);
+    const int rpw_max = 64 / bw_per_run;
+    int rpw;
+    if (opt_rpw > 0) rpw = (int)std::min<long long>(opt_rpw, rpw_max);
+    else {
+        const int simds = 4 * std::max(1, pbagpu::stream_cu_count(s));
+        rpw = std::min(rpw_max, std::max(1, (n_runs + simds - 1) / simds));
+    }
+    if (rpw > n_runs) rpw = n_runs;
+    const int nblk = (n_runs + rpw - 1) / rpw;
+    using pbagpu::dyn::KS;
+    using pbagpu::dyn::NA;
+    double *dK = nullptr;
+    if (rpw > 1) {
+        dK = (double *)pbagpu::workspace(pbagpu::WS_DYN_K, sizeof(double) * (size_t)nblk * KS * NA * 64, s);
+        if (!dK) return PBAGPU_E_HIP;
+    }
     pbagpu::KernelTimer timer(PBAGPU_STAGE_DYN);
     if (rc == PBAGPU_OK) {
         timer.start(s);
-        const size_t lds = sizeof(double) * ((size_t)pbagpu::dyn::KS * pbagpu::dyn::NA * bw_per_run + 64);
-        (void)hipFuncSetAttribute((const void *)pbagpu::dyn::dyn_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds);
-        hipLaunchKernelGGL(pbagpu::dyn::dyn_kernel, dim3(n_runs), dim3(64), lds, s, dP, dI, bw_per_run, nt,
-                           (const int *)dN, d_tgrid, tg_stride, d_table, dR);
+        if (rpw > 1) {
+            hipLaunchKernelGGL(pbagpu::dyn::dyn_kernel<true>, dim3(nblk), dim3(64),
+                               sizeof(double) * (NA * 64 + 128) + sizeof(pbagpu::dyn::BW) * 64, s, dP, dI,
+                               n_runs, bw_per_run, rpw, nt, (const int *)dN, d_tgrid, tg_stride, d_table, dR, dK);
+        } else {
+            const size_t lds = sizeof(double) * ((size_t)(KS + 1) * NA * bw_per_run + 128) +
+                               sizeof(pbagpu::dyn::BW) * bw_per_run;
+            (void)hipFuncSetAttribute((const void *)pbagpu::dyn::dyn_kernel<false>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL(pbagpu::dyn::dyn_kernel<false>, dim3(nblk), dim3(64), lds, s, dP, dI, n_runs,
+                               bw_per_run, 1, nt, (const int *)dN, d_tgrid, tg_stride, d_table, dR, (double *)nullptr);
+        }
         PBA_TRY(rc, hipGetLastError());
         timer.stop(s);
     }

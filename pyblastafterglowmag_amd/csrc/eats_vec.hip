@@ -80,6 +80,9 @@ struct Ctx {
     __device__ double th(int i) const { return vlds[2 * nt + i]; }
     __device__ double nufs(int i) const { return vlds[3 * nt + i]; }
     __device__ double nurs(int i) const { return vlds[3 * nt + nf + i]; }
+    // 1 / (nu_{i+1} - nu_i) of the two grids
+    __device__ double idfs(int i) const { return vlds[3 * nt + 2 * nf + i]; }
+    __device__ double idrs(int i) const { return vlds[3 * nt + 3 * nf + i]; }
     __device__ double col(int v, int i) const { return T[(size_t)v * nt + i]; }
     __device__ double mu_at(int i) const { return (tb(i) - tz) / R(i) * kC; }
     __device__ int mu_index(double mu) const {
@@ -95,18 +98,21 @@ struct Ctx {
 
 // One node of the inner integral: the surface geometry at 1 - cos(theta) = x (EATS::integrand and
 // fluxDensAdaptiveWithComov up to the emissivity lookup).  A node with valid = 0 contributes 0.
+// Everything here depends on the node only, so the per-frequency work of a node is a grid lookup,
+// four table loads and a bilinear blend.
 struct Node {
     int valid, ia;
     double Dnu;        // Gamma (1 - beta mu) with the interpolated beta: nu' = (1 + z) nu Dnu
-    double D;          // Doppler factor with beta from the interpolated Gamma: em_lab = em' / D^2
-    double r, y1, y2;  // radius at the emission time and at the bracketing steps
+    double lDnu;       // ln Dnu: ln nu' = ln((1 + z) nu) + ln Dnu estimates the frequency-grid index
+    double iD2;        // 1 / D^2, D the Doppler factor with beta from the interpolated Gamma
+    double wy2, wy1;   // radial weights (r2 - r) / (r2 - r1) and (r - r1) / (r2 - r1) of the bracketing steps
     double fac_fs;     // ne / (2 pi (1 - cos theta_bw)) / Gamma / (1 - mu beta), times the absorption
     double fac_rs;     // the same with the reverse shock's ne, or 0
 };
 
 __device__ Node node_geom(const Ctx &C, double x) {
     Node n;
-    n.valid = 0; n.ia = 0; n.Dnu = 1.; n.D = 1.; n.r = 0.; n.y1 = 0.; n.y2 = 0.; n.fac_fs = 0.; n.fac_rs = 0.;
+    n.valid = 0; n.ia = 0; n.Dnu = 1.; n.lDnu = 0.; n.iD2 = 1.; n.wy2 = 0.; n.wy1 = 0.; n.fac_fs = 0.; n.fac_rs = 0.;
     const int N = C.i_end_r;
     if (C.R(0) == 0.0 && C.R(N - 1) == 0.0) return n;
     const double cos_theta = 1.0 - x;
@@ -135,8 +141,13 @@ __device__ Node node_geom(const Ctx &C, double x) {
     // no absorption table: optical depth 0 unless mu equals the shock velocity
     const double att = (mu == bg) ? -expm1(-1.0e100) / 1.0e100 : 1.0;
     const double geo = 1.0 / (2. * kPi * omc) / Gamma / (1.0 - mu * bg) * att;
-    n.valid = 1; n.ia = ia; n.Dnu = Gamma * (1.0 - beta * mu); n.D = Gamma * (1.0 - bg * mu);
-    n.r = r; n.y1 = Ra; n.y2 = Rb;
+    n.valid = 1; n.ia = ia; n.Dnu = Gamma * (1.0 - beta * mu);
+    n.lDnu = log(n.Dnu);
+    const double D = Gamma * (1.0 - bg * mu);
+    n.iD2 = 1.0 / (D * D);
+    const double idy = 1.0 / (Rb - Ra);
+    n.wy2 = (Rb - r) * idy;
+    n.wy1 = (r - Ra) * idy;
     n.fac_fs = ne * geo;
     if (C.has_rs && C.col(PV_thickness_rs, ia) > 0 && C.col(PV_thickness_rs, ib) > 0)
         n.fac_rs = li(PV_M3) / kMp * geo;
@@ -145,21 +156,28 @@ __device__ Node node_geom(const Ctx &C, double x) {
 
 // luminosity of one shock at comoving frequency freq (computeFluxDensity with useNe / smooth tau);
 // frequencies off the comoving grid set `st` as where the reference stops
+// lfreq = ln((1 + z) nu_obs) + ln Dnu, an estimate of ln freq for the grid index
 template <bool RS>
-__device__ __forceinline__ double shock_lum(const Ctx &C, const double *jall, const Node &n, double freq, double fac,
-                                            int &st) {
+__device__ __forceinline__ double shock_lum(const Ctx &C, const double *jall, const Node &n, double freq, double lfreq,
+                                            double fac, int &st) {
     const int nf = C.nf;
     auto nu = [&](int i) { return RS ? C.nurs(i) : C.nufs(i); };
     if (freq >= nu(nf - 1) || freq <= nu(0)) { st |= PBAGPU_S_NU_OUT_OF_GRID; return 0.; }
     // find_index on the log-spaced grid: the index from the logarithm, then moved to the bracketing
     // grid points (the grid is built by repeated multiplication, so it is checked against its values)
-    int ia_nu = (int)((log(freq) - C.lnu0[RS]) * C.ilr[RS]);
+    int ia_nu = (int)((lfreq - C.lnu0[RS]) * C.ilr[RS]);
     ia_nu = ia_nu < 0 ? 0 : (ia_nu > nf - 2 ? nf - 2 : ia_nu);
     while (ia_nu > 0 && nu(ia_nu) > freq) ia_nu--;
     while (ia_nu < nf - 2 && nu(ia_nu + 1) <= freq) ia_nu++;
     const int ib_nu = ia_nu + 1;
-    const double em = table_bilinear(jall, nf, n.ia, n.ia + 1, ia_nu, ib_nu, nu(ia_nu), nu(ib_nu), n.y1, n.y2, freq, n.r);
-    const double em_lab = em / (n.D * n.D);
+    // bilinear interpolation of the [step][frequency] table inside the cell (ia, ia + 1) x (ia_nu, ib_nu)
+    // (Interp2d::InterpolateBilinear, interpolators.h:724-808), with the node's radial weights
+    const double *r1 = jall + (size_t)n.ia * nf, *r2 = r1 + nf;
+    const double q11 = r1[ia_nu], q21 = r1[ib_nu], q12 = r2[ia_nu], q22 = r2[ib_nu];
+    const double idx = RS ? C.idrs(ia_nu) : C.idfs(ia_nu);
+    const double em = ((nu(ib_nu) - freq) * idx) * (q11 * n.wy2 + q12 * n.wy1) +
+                      ((freq - nu(ia_nu)) * idx) * (q21 * n.wy2 + q22 * n.wy1);
+    const double em_lab = em * n.iD2;
     if (em_lab < 0 || !isfinite(em_lab)) st |= PBAGPU_S_NONFINITE;
     const double lum = em_lab * fac;   // fac holds 1 / (1 - mu beta) with beta from Gamma
     if (!(lum >= 0) || !isfinite(lum)) st |= PBAGPU_S_NONFINITE;
@@ -171,6 +189,7 @@ struct Wave {
     const double *jfs, *jrs;
     int lane;
     double nuz[NU_PER_LANE];   // (1 + z) nu_obs of this lane's frequencies (0: no frequency)
+    double lnuz[NU_PER_LANE];  // their logarithms
     double rtol;
     int st;
     unsigned long long samples;
@@ -186,16 +205,16 @@ __device__ void gk_inner(Wave &W, double a, double b, double K[NU_PER_LANE], dou
     for (int k = 0; k < NG; k++) {
         if (!bcast_i(mine.valid, k)) continue;
         Node n;
-        n.valid = 1; n.ia = bcast_i(mine.ia, k); n.Dnu = bcast(mine.Dnu, k); n.D = bcast(mine.D, k);
-        n.r = bcast(mine.r, k);
-        n.y1 = bcast(mine.y1, k); n.y2 = bcast(mine.y2, k); n.fac_fs = bcast(mine.fac_fs, k);
+        n.valid = 1; n.ia = bcast_i(mine.ia, k); n.Dnu = bcast(mine.Dnu, k); n.lDnu = bcast(mine.lDnu, k);
+        n.iD2 = bcast(mine.iD2, k);
+        n.wy2 = bcast(mine.wy2, k); n.wy1 = bcast(mine.wy1, k); n.fac_fs = bcast(mine.fac_fs, k);
         n.fac_rs = bcast(mine.fac_rs, k);
         const double wk = gk_kw(k), wg = gk_gw(k);
         for (int j = 0; j < NU_PER_LANE; j++) {
             if (W.nuz[j] == 0.) continue;
-            const double freq = W.nuz[j] * n.Dnu;
-            double f = shock_lum<false>(W.C, W.jfs, n, freq, n.fac_fs, W.st);
-            if (n.fac_rs != 0.) f += shock_lum<true>(W.C, W.jrs, n, freq, n.fac_rs, W.st);
+            const double freq = W.nuz[j] * n.Dnu, lfreq = W.lnuz[j] + n.lDnu;
+            double f = shock_lum<false>(W.C, W.jfs, n, freq, lfreq, n.fac_fs, W.st);
+            if (n.fac_rs != 0.) f += shock_lum<true>(W.C, W.jrs, n, freq, lfreq, n.fac_rs, W.st);
             K[j] += wk * f;
             G[j] += wg * f;
         }
@@ -329,7 +348,7 @@ __device__ void outer(Wave &W, double th0c, double th1c, double T[NU_PER_LANE]) 
 
 // grid: (ceil(n_t / WPB), n_layers); wavefront w of a workgroup integrates t_obs index
 // blockIdx.x * WPB + w of layer blockIdx.y at every frequency
-__global__ void __launch_bounds__(WAVE * WPB) eats_vec_kernel(const VLayer *__restrict__ layers, int n_t,
+__global__ void __launch_bounds__(WAVE * WPB, 2) eats_vec_kernel(const VLayer *__restrict__ layers, int n_t,
                                                               const double *__restrict__ tobs, int n_nu,
                                                               const double *__restrict__ nuobs, double rtol,
                                                               double *__restrict__ out, int *__restrict__ status,
@@ -344,6 +363,8 @@ __global__ void __launch_bounds__(WAVE * WPB) eats_vec_kernel(const VLayer *__re
     for (int i = threadIdx.x; i < nf; i += blockDim.x) {
         vlds[3 * nt + i] = L.nufs[i];
         vlds[3 * nt + nf + i] = L.nurs[i];
+        vlds[3 * nt + 2 * nf + i] = i < nf - 1 ? 1.0 / (L.nufs[i + 1] - L.nufs[i]) : 0.;
+        vlds[3 * nt + 3 * nf + i] = i < nf - 1 ? 1.0 / (L.nurs[i + 1] - L.nurs[i]) : 0.;
     }
     __syncthreads();
     const int it = blockIdx.x * WPB + threadIdx.x / WAVE;
@@ -371,6 +392,7 @@ __global__ void __launch_bounds__(WAVE * WPB) eats_vec_kernel(const VLayer *__re
     for (int j = 0; j < NU_PER_LANE; j++) {
         const int q = W.lane + WAVE * j;
         W.nuz[j] = q < n_nu ? (1.0 + L.z) * nuobs[q] : 0.;
+        W.lnuz[j] = q < n_nu ? log(W.nuz[j]) : 0.;
     }
     double flux[NU_PER_LANE] = {0., 0.};
     if (L.i_end_r == 1) W.st |= PBAGPU_S_NONFINITE;   // one usable step: no surface
@@ -415,7 +437,7 @@ extern "C" int pbagpu_eats_lightcurve_vec(const pba_run_params *params, int n_ru
         pbagpu::set_error_msg("eats_lightcurve_vec: at most 128 observer frequencies per call");
         return PBAGPU_E_ARG;
     }
-    const size_t lds = sizeof(double) * (3 * (size_t)nt + 2 * (size_t)nfreq);
+    const size_t lds = sizeof(double) * (3 * (size_t)nt + 4 * (size_t)nfreq);
     if (lds > 160 * 1024) {
         pbagpu::set_error_msg("eats_lightcurve_vec: time grid too long for the LDS-staged columns");
         return PBAGPU_E_ARG;

@@ -13,7 +13,7 @@ void set_error(const char *what, hipError_t e);
 void set_error_msg(const char *msg);
 
 enum WsSlot {
-    WS_DYN_PARAMS = 0, WS_DYN_INIT, WS_DYN_RES, WS_DYN_NT,
+    WS_DYN_PARAMS = 0, WS_DYN_INIT, WS_DYN_RES, WS_DYN_NT, WS_DYN_K,
     WS_ELE_GRIDS, WS_ELE_KTAB, WS_ELE_POPS, WS_ELE_WORK, WS_ELE_LIST, WS_ELE_KBF, WS_ELE_MIXED,
     WS_EATS_NU, WS_EATS_LAYERS, WS_EATS_STORE, WS_EATS_COUNTER, WS_EATS_SAMPLES,
     WS_PW_NU, WS_PW_LAYERS, WS_PW_SAMPLES,

@@ -23,6 +23,7 @@ EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu
            "pbagpu_skymap_hist_noalloc", "pbagpu_ele_evolve_spectra", "pbagpu_dyn_evolve_ragged",
            "pbagpu_set_option", "pbagpu_get_option")
 OPT_ELE_GRAPH = 0
+OPT_DYN_RUNS_PER_WAVE = 1
 STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW, STAGE_SKY_HIST, STAGE_EATS_VEC = 0, 1, 2, 3, 4, 5, 6
 N_STAGES = 7
 # sky-map binning (enum pba_sky_moment / pba_sky_status)
@@ -88,7 +89,8 @@ def lib():
 
 
 def set_option(which, value):
-    """pbagpu_set_option (OPT_ELE_GRAPH: capture the electron stage's per-step launches in a graph)."""
+    """pbagpu_set_option (OPT_ELE_GRAPH: capture the electron stage's per-step launches in a graph;
+    OPT_DYN_RUNS_PER_WAVE: runs per dynamics wavefront, 0 = as many as fit)."""
     _check(lib().pbagpu_set_option(int(which), int(value)), "set_option")
 
 

@@ -49,6 +49,22 @@ def rel_err(a, b, floor_frac=0.0):
     return float(np.max(np.abs(a[m] - b[m]) / np.abs(b[m])))
 
 
+def dyn_excess(mine, ref, gamma_ref, rtol=1e-4, ulps=64):
+    """Largest |mine - ref| / ((rtol + ulps * eps / (Gamma - 1)) |ref|) over the elements (<= 1
+    passes).  In the late, nearly Newtonian phase every quantity formed from Gamma - 1 (beta, the
+    internal energy, the swept-up mass through the energy equation) carries the error of Gamma
+    divided by Gamma - 1: at Gamma - 1 = 2e-12 a 12-ulp difference of Gamma is a 1e-3 relative
+    difference of beta.  The bound is rtol plus a difference of `ulps` ulp of Gamma carried through
+    that division; zeros must match exactly."""
+    mine, ref, g = np.asarray(mine), np.asarray(ref), np.asarray(gamma_ref)
+    tol = rtol + ulps * np.finfo(float).eps / np.maximum(g - 1.0, 1e-300)
+    diff = np.abs(mine - ref)
+    den = tol * np.abs(ref)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.where(den > 0, diff / np.where(den > 0, den, 1.0), np.where(diff > 0, np.inf, 0.0))
+    return float(np.max(r)) if r.size else 0.0
+
+
 def rs_end_steps(gamma43):
     """Per layer: the last step at which the reverse shock is on (Gamma43 != 0), or -1."""
     out = []

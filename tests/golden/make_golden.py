@@ -6,6 +6,9 @@ reference's outputs (lc.h5 light curves, dyn.h5 dynamics, spec.h5 comoving spect
 compressed .npz next to this script.  Only data is stored (inputs and outputs).
 
     python tests/golden/make_golden.py [name ...]      (no name: every fixture, ALL)
+
+gauss21_fsrs_ssc (the full 21-layer jet, 200 x 96 grids, 12,288 pairs) takes about 25 minutes of
+one core.
 """
 import os
 import shutil
@@ -43,7 +46,10 @@ SKY_ATTRS = ["flux", "nshells", "nlayers", "time", "freq", "xmin", "ymin", "xmax
 
 # light curves of every `stride`-th layer, dynamics of layer 0, no spectra or sky maps: the full-size
 # case and the uniform-grid references of the per-layer-grid jet
-LARGE = {"tophat64_pw": 7, "gauss_pw_ng32": 1, "gauss_pw_ng48": 1}
+LARGE = {"tophat64_pw": 7, "gauss_pw_ng32": 1, "gauss_pw_ng48": 1, "gauss21_fsrs_ssc": 4}
+# large cases that keep the dynamics of every stored light-curve layer (not only layer 0): the
+# reverse shock's end per layer decides which layers a test holds to the tight bound
+LARGE_DYN_ALL = {"gauss21_fsrs_ssc"}
 
 
 def run_reference(name, wd, spec_every=1):
@@ -81,20 +87,22 @@ def run_reference(name, wd, spec_every=1):
     dyn = os.path.join(wd, "dyn.h5")
     names = DYN_VARS + (DYN_VARS_RS if P.fsrs else [])
     nt = int(mp_["ntb"])
-    n_dyn = 1 if large else nl
+    dyn_layers = (lc_layers if name in LARGE_DYN_ALL else [0]) if large else list(range(nl))
+    n_dyn = len(dyn_layers)
     D = np.zeros((n_dyn, len(names), nt))
-    for il in range(n_dyn):
+    for i, il in enumerate(dyn_layers):
         for iv, v in enumerate(names):
             ds = f"shell=0 layer={il}/{v}"
             if h5io.has(dyn, ds):
-                D[il, iv] = h5io.read(dyn, ds)
+                D[i, iv] = h5io.read(dyn, ds)
     out["dyn"] = D
+    out["dyn_layers"] = np.array(dyn_layers)
     out["dyn_names"] = np.array(names)
     out["dyn_layer_members"] = np.array(h5io.members(dyn, "shell=0 layer=0"))
     names_da = h5io.attr_names(dyn, "shell=0 layer=0")
     out["dyn_layer_attr_names"] = np.array(names_da)
     out["dyn_layer_attrs"] = np.array([[h5io.attr(dyn, k, f"shell=0 layer={il}") for k in names_da]
-                                       for il in range(n_dyn)])
+                                       for il in dyn_layers])
     out["dyn_root_attrs"] = np.array(h5io.attr_names(dyn, "/"))
     out["dyn_root_counts"] = np.array([h5io.attr(dyn, k) for k in ("nshells", "nlayers", "ntimes", "ncells")])
     if large:
@@ -163,7 +171,7 @@ def main(names):
 ALL = ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gauss_pw_small", "gauss_pw_default",
        "tophat64_pw", "gauss_pw_ng32", "gauss_pw_ng48",
        "tophat_ssa_small", "gauss5_ssa_small", "tophat_dermer_small", "tophat_bessel_small", "tophat_gsl_small",
-       "tophat_rk4_small", "tophat_wind_small"]
+       "tophat_rk4_small", "tophat_wind_small", "gauss21_fsrs_ssc"]
 
 if __name__ == "__main__":
     main(sys.argv[1:] or ALL)

@@ -51,8 +51,9 @@ def test_version_needs_no_device():
 
 
 def test_library_options_need_no_device():
-    """pbagpu_set_option / pbagpu_get_option: PBAGPU_OPT_ELE_GRAPH takes 0 or 1 (default 0); an
-    unknown option or value is PBAGPU_E_ARG and leaves the setting alone."""
+    """pbagpu_set_option / pbagpu_get_option: PBAGPU_OPT_ELE_GRAPH takes 0 or 1 and
+    PBAGPU_OPT_DYN_RUNS_PER_WAVE 0 .. 64 (both default 0); an unknown option or value is
+    PBAGPU_E_ARG and leaves the setting alone."""
     if not os.path.exists(LIB):
         pytest.skip("libpbagpu.so not built")
     lib = C.CDLL(LIB)
@@ -61,7 +62,11 @@ def test_library_options_need_no_device():
     v = C.c_longlong(-5)
     assert lib.pbagpu_get_option(0, C.byref(v)) == 0 and v.value == 0
     assert lib.pbagpu_set_option(0, 2) == -1 and lib.pbagpu_set_option(0, -1) == -1
-    assert lib.pbagpu_set_option(1, 0) == -1 and lib.pbagpu_get_option(1, C.byref(v)) == -1
+    assert lib.pbagpu_set_option(2, 0) == -1 and lib.pbagpu_get_option(2, C.byref(v)) == -1
+    assert lib.pbagpu_get_option(1, C.byref(v)) == 0 and v.value == 0
+    assert lib.pbagpu_set_option(1, 65) == -1 and lib.pbagpu_set_option(1, -1) == -1
+    assert lib.pbagpu_set_option(1, 3) == 0 and lib.pbagpu_get_option(1, C.byref(v)) == 0 and v.value == 3
+    assert lib.pbagpu_set_option(1, 0) == 0
     assert lib.pbagpu_set_option(0, 1) == 0 and lib.pbagpu_get_option(0, C.byref(v)) == 0 and v.value == 1
     assert lib.pbagpu_set_option(0, 0) == 0 and lib.pbagpu_get_option(0, C.byref(v)) == 0 and v.value == 0
 
