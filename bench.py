@@ -59,10 +59,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8, help="timed steps of the mixed leg")
     ap.add_argument("--warmup", type=int, default=4, help="warm-up steps of the mixed leg (>= depth fills it)")
-    ap.add_argument("--batch", type=int, default=128, help="independent runs per step, mixed leg")
+    ap.add_argument("--batch", type=int, default=184, help="independent runs per step, mixed leg")
     ap.add_argument("--depth", type=int, default=4,
                     help="pipeline depth: batches whose dynamics run while the radiation stages of an older batch run")
-    ap.add_argument("--dyn-cus", type=int, default=128,
+    ap.add_argument("--dyn-cus", type=int, default=184,
                     help="compute units reserved for the dynamics streams (the radiation stream gets the others)")
     ap.add_argument("--vec-rtol", type=float, default=1e-2, help="quadrature tolerance of the vectorised observer")
     ap.add_argument("--mode", default="both", choices=["both", "mixed", "fp64"])

@@ -387,7 +387,9 @@ class Pipeline:
                 self.d_nu_u = torch.from_numpy(np.ascontiguousarray(fs)).to(self.dev)
             except ValueError:
                 pass
-        torch.cuda.synchronize()
+        # the allocations and copies above ran on torch's current stream: wait for that stream only
+        # (a device-wide wait would also wait for other batches' dynamics on their own streams)
+        torch.cuda.current_stream().synchronize()
 
     def dynamics(self):
         if self.ragged:

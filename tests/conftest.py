@@ -77,25 +77,91 @@ def rs_end_steps(gamma43):
 # Session caches of the GPU tests: a golden fixture's Pipeline after its dynamics stage (the
 # reverse-shock runs take tens of seconds of one-wavefront latency each) and the oracle's
 # dynamics of the same run; tests rerun the electron and observer stages on them as they need.
+# At the start of a session with GPU tests both are computed ahead, in the order the selected
+# tests use them: GPU dynamics of up to 4 fixtures at a time, each on its own stream (the library
+# runs calls on different streams from different host threads side by side; 4 is the process's
+# hardware queue count), and the oracle's dynamics on 6 host threads.
 _GPU_PIPELINES = {}
 _ORACLE_DYN = {}
+_GPU_FUT = {}
+_ORACLE_FUT = {}
+_POOLS = []
+
+# tests that take a fixture's Pipeline from gpu_pipeline (by their `name` parameter) or its
+# oracle dynamics from oracle_dynamics, and the fixed fixtures of tests without that parameter
+_PIPELINE_TESTS = {"test_stages_match_oracle", "test_lightcurve_matches_reference", "test_mixed_precision_ssc_lightcurve",
+                   "test_skymaps_match_oracle_and_reference", "test_vec_observer_matches_plain_and_reference",
+                   "test_cpp_caller_matches_reference_and_python"}
+_PIPELINE_FIXED = {"test_full_size_c2_matches_reference_and_oracle": "gauss21_fsrs_ssc",
+                   "test_electron_graph_matches_eager": "gauss5_small",
+                   "test_surrogate_on_a_run_and_fallback": "gauss5_small"}
+_ORACLE_TESTS = {"test_stages_match_oracle", "test_spectra_match_reference"}
+
+
+def _make_pipeline(name):
+    import torch
+    from pyblastafterglowmag_amd import gpu
+    run = golden_run(load_golden(name))
+    pl = gpu.Pipeline([run], stream=torch.cuda.Stream())
+    pl.dynamics()
+    return run, pl
+
+
+def _make_oracle(name):
+    import pyoracle
+    run = golden_run(load_golden(name))
+    return pyoracle.dyn_evolve(run.params, run.inits, run.tgrid)
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _prefetch_dynamics(request):
+    items = [it for it in request.session.items if it.get_closest_marker("gpu")]
+    pipes, orcs = [], []
+    for it in items:
+        fn = it.originalname if hasattr(it, "originalname") else it.name
+        name = getattr(it, "callspec", None) and it.callspec.params.get("name")
+        if fn in _PIPELINE_TESTS and name and name not in pipes:
+            pipes.append(name)
+        if fn in _PIPELINE_FIXED and _PIPELINE_FIXED[fn] not in pipes:
+            pipes.append(_PIPELINE_FIXED[fn])
+        if fn in _ORACLE_TESTS and name and name not in orcs:
+            orcs.append(name)
+    started = False
+    if items:
+        try:
+            import torch
+            started = torch.cuda.is_available()
+        except ImportError:
+            started = False
+    if started:
+        import concurrent.futures as cf
+        from pyblastafterglowmag_amd import gpu
+        gpu.lib()   # load and declare the library once, before the worker threads call it
+        gpool, opool = cf.ThreadPoolExecutor(max_workers=4), cf.ThreadPoolExecutor(max_workers=6)
+        _POOLS.extend([gpool, opool])
+        for n in pipes:
+            _GPU_FUT[n] = gpool.submit(_make_pipeline, n)
+        for n in orcs:
+            _ORACLE_FUT[n] = opool.submit(_make_oracle, n)
+        print(f"[conftest] dynamics computed ahead: GPU {pipes}, oracle {orcs}", flush=True)
+    yield
+    for p in _POOLS:
+        p.shutdown(wait=True, cancel_futures=True)
+    _POOLS.clear()
 
 
 def gpu_pipeline(name):
-    """(run, Pipeline) of golden fixture `name` with its GPU dynamics done (computed once)."""
+    """(run, Pipeline) of golden fixture `name` with its GPU dynamics done (computed once, on its
+    own stream)."""
     if name not in _GPU_PIPELINES:
-        from pyblastafterglowmag_amd import gpu
-        run = golden_run(load_golden(name))
-        pl = gpu.Pipeline([run])
-        pl.dynamics()
-        _GPU_PIPELINES[name] = (run, pl)
+        fut = _GPU_FUT.pop(name, None)
+        _GPU_PIPELINES[name] = fut.result() if fut is not None else _make_pipeline(name)
     return _GPU_PIPELINES[name]
 
 
 def oracle_dynamics(name):
     """The oracle's (table, results) of golden fixture `name` (computed once)."""
     if name not in _ORACLE_DYN:
-        import pyoracle
-        run = golden_run(load_golden(name))
-        _ORACLE_DYN[name] = pyoracle.dyn_evolve(run.params, run.inits, run.tgrid)
+        fut = _ORACLE_FUT.pop(name, None)
+        _ORACLE_DYN[name] = fut.result() if fut is not None else _make_oracle(name)
     return _ORACLE_DYN[name]

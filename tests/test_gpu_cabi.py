@@ -11,7 +11,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from conftest import golden_run, load_golden, rel_err
+from conftest import golden_run, gpu_pipeline, load_golden, rel_err
 from pyblastafterglowmag_amd import abi
 
 pytestmark = pytest.mark.gpu
@@ -57,5 +57,6 @@ def test_cpp_caller_matches_reference_and_python(name):
         e = rel_err(out[il], g["lc"][k], 1e-30)
         print(f"  layer {il}: vs reference {e:.3e}", flush=True)
         assert e <= 1e-6, (name, il, e)
-    flux, _ = gpu.lightcurves([run])
+    _, pl = gpu_pipeline(name)   # the same run through the Python host path
+    flux, _ = gpu.radiate(pl, precision=0)
     np.testing.assert_array_equal(out, flux[0])

@@ -21,11 +21,10 @@ def _outputs(pl):
 def test_electron_graph_matches_eager(precision):
     import torch
     from pyblastafterglowmag_amd import gpu
-    run, pl0 = gpu_pipeline("gauss5_small")
-    # the same dynamics, with the electron stage on a created stream (the null stream stays eager)
-    stream = torch.cuda.Stream()
-    pl = gpu.Pipeline([run], precision=precision, stream=stream)
-    pl.dynamics()
+    # the fixture's pipeline runs on a created stream, where the option captures
+    run, pl = gpu_pipeline("gauss5_small")
+    assert pl.sp is not None and pl.sp.value
+    pl.precision = precision
     print(f"[gpu] electron stage graph vs eager, precision {precision}, {len(pl.pops)} populations, "
           f"{pl.nt} steps", flush=True)
     assert gpu.get_option(gpu.OPT_ELE_GRAPH) == 0
@@ -35,8 +34,10 @@ def test_electron_graph_matches_eager(precision):
     torch.cuda.synchronize()
     t_eager = time.perf_counter() - t0
     ref = _outputs(pl)
-    # the null-stream pipeline with the option on: launched directly, same results
-    pl0.precision = precision
+    assert not ref["est"].any(), ref["est"]
+    # the same tables on the null stream with the option on: launched directly, same results
+    pl0 = gpu.Pipeline([run], precision=precision)
+    pl0.use_tables(pl.d_table.cpu().numpy(), pl.res)
     try:
         gpu.set_option(gpu.OPT_ELE_GRAPH, 1)
         pl0.electrons(want_f=True)
@@ -45,7 +46,6 @@ def test_electron_graph_matches_eager(precision):
     got0 = _outputs(pl0)
     for k, v in ref.items():
         assert np.array_equal(got0[k], v, equal_nan=True), (precision, "null stream", k)
-    assert not ref["est"].any(), ref["est"]
     try:
         for rep in range(3):
             gpu.set_option(gpu.OPT_ELE_GRAPH, 1)
@@ -68,4 +68,3 @@ def test_electron_graph_matches_eager(precision):
             assert np.array_equal(pl.d_jall.cpu().numpy(), ref["jall"])
     finally:
         gpu.set_option(gpu.OPT_ELE_GRAPH, 0)
-

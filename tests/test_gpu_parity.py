@@ -8,6 +8,8 @@ the same bound the oracle keeps against the reference program.  Electron and obs
 stages are compared on identical inputs (the GPU's own tables): 1e-9 and 1e-10.  Light
 curves end to end against the reference program: 1e-6 relative (the FP64-path target).
 """
+import concurrent.futures as cf
+
 import numpy as np
 import pytest
 
@@ -278,11 +280,13 @@ def test_heterogeneous_batch_matches_oracle():
     runs = [_lhs_small(0), _lhs_small(1)]
     nl = runs[0].n_layers
     print(f"[gpu] heterogeneous batch: 2 runs x {nl} layers", flush=True)
-    flux, ex = gpu.lightcurves(runs, want_intermediates=True)
-    assert not ex["dyn_status"].any() and not ex["ele_status"].any() and not ex["eats_status"].any()
     params = (abi.RunParams * 2)(*[r.params for r in runs])
     inits = (abi.LayerInit * (2 * nl))(*[li for r in runs for li in r.inits])
-    table_o, res_o = pyoracle.dyn_evolve(params, inits, runs[0].tgrid, n_runs=2)
+    with cf.ThreadPoolExecutor(max_workers=1) as pool:   # the oracle's dynamics beside the GPU's
+        fut = pool.submit(pyoracle.dyn_evolve, params, inits, runs[0].tgrid, n_runs=2)
+        flux, ex = gpu.lightcurves(runs, want_intermediates=True)
+        table_o, res_o = fut.result()
+    assert not ex["dyn_status"].any() and not ex["ele_status"].any() and not ex["eats_status"].any()
     for bw in range(2 * nl):
         assert ex["results"][bw][1] == res_o[bw].i_end_r and ex["results"][bw][2] == res_o[bw].fsrs, bw
         for n in DYN_CHECK:
@@ -319,13 +323,16 @@ def test_ragged_time_grids_match_oracle():
     assert len(set(nts)) == len(nts), nts
     nl = runs[0].n_layers
     print(f"[gpu] ragged batch: {len(runs)} runs x {nl} layers, steps {nts}", flush=True)
-    flux, ex = gpu.lightcurves(runs, want_intermediates=True)
+    with cf.ThreadPoolExecutor(max_workers=len(runs)) as pool:   # the oracle's dynamics beside the GPU's
+        futs = [pool.submit(pyoracle.dyn_evolve, run.params, run.inits, run.tgrid) for run in runs]
+        flux, ex = gpu.lightcurves(runs, want_intermediates=True)
+        oracle = [f.result() for f in futs]
     assert not ex["dyn_status"].any() and not ex["ele_status"].any() and not ex["eats_status"].any()
     ntm = max(nts)
     assert ex["table"].shape[-1] == ntm
     got = flux.reshape(len(runs) * nl, -1)
     for r, run in enumerate(runs):
-        table_o, res_o = pyoracle.dyn_evolve(run.params, run.inits, run.tgrid)
+        table_o, res_o = oracle[r]
         for il in range(nl):
             bw = r * nl + il
             assert ex["results"][bw][1] == res_o[il].i_end_r and ex["results"][bw][2] == res_o[il].fsrs, bw
@@ -359,31 +366,53 @@ def test_ragged_time_grids_match_oracle():
 
 
 def test_packed_dynamics_bit_identical():
-    """Dynamics with several runs per wavefront (library option PBAGPU_OPT_DYN_RUNS_PER_WAVE)
-    against one run per wavefront: three Latin-hypercube members with their own parameters and
-    time-grid lengths in one ragged launch, packed 3 and 2 per wavefront (the last wavefront then
-    holds one run).  Every table entry and per-blastwave result is bit-identical: a run's arithmetic
-    and its run-wide sums do not depend on the lanes beside it."""
+    """Dynamics with several runs per wavefront (library option PBAGPU_OPT_DYN_RUNS_PER_WAVE), one
+    pbagpu_dyn_evolve_ragged launch per group, against each fixture's own one-run launch: (a) seven
+    one-layer top-hat fixtures with their own parameters, microphysics options, ambient media and
+    integrators (DOP853 and RK4 in one wavefront), 3 per wavefront, so the last wavefront holds one
+    run; (b) the two 5-layer reverse-shock fixtures in one wavefront.  Every table entry and
+    per-blastwave result is bit-identical: a run's arithmetic and its run-wide sums do not depend on
+    the lanes beside it."""
+    import ctypes as C
+    import torch
     from pyblastafterglowmag_amd import gpu
-    runs = [_lhs_small(i, ragged_nt=True) for i in (1, 2, 3)]
-    nl = runs[0].n_layers
-    assert 3 * nl <= 64
-    print(f"[gpu] packed dynamics: 3 runs x {nl} layers, steps {[len(r.tgrid) for r in runs]}", flush=True)
-    out = {}
-    try:
-        for rpw in (1, 3, 2):
+    groups = [(["tophat_small", "tophat_ssa_small", "tophat_dermer_small", "tophat_bessel_small", "tophat_gsl_small",
+                "tophat_rk4_small", "tophat_wind_small"], 3),
+              (["gauss5_small", "gauss5_ssa_small"], 2)]
+    L = gpu.lib()
+    for names, rpw in groups:
+        single = [gpu_pipeline(n) for n in names]
+        runs = [r for r, _ in single]
+        n, bpr = len(runs), runs[0].n_layers
+        nts = [len(r.tgrid) for r in runs]
+        ntm = max(nts)
+        tg = np.zeros((n, ntm))
+        for k, r in enumerate(runs):
+            tg[k, :nts[k]] = r.tgrid
+            tg[k, nts[k]:] = r.tgrid[-1]
+        d_tg = torch.from_numpy(tg).cuda()
+        params = (abi.RunParams * n)(*[r.params for r in runs])
+        inits = (abi.LayerInit * (n * bpr))(*[li for r in runs for li in r.inits])
+        table = torch.empty((n * bpr, abi.PBA_NV, ntm), dtype=torch.float64, device="cuda")
+        res = (abi.BwResult * (n * bpr))()
+        stream = torch.cuda.Stream()
+        print(f"[gpu] packed dynamics: {n} runs x {bpr} layers, {rpw} per wavefront", flush=True)
+        try:
             gpu.set_option(gpu.OPT_DYN_RUNS_PER_WAVE, rpw)
-            pl = gpu.Pipeline(runs)
-            pl.dynamics()
-            out[rpw] = (pl.d_table.cpu().numpy(),
-                        [(r.status, r.i_end_r, r.fsrs, r.n_steps, r.n_rhs) for r in pl.res])
-            print(f"[gpu]   {rpw} run(s) per wavefront done", flush=True)
-    finally:
-        gpu.set_option(gpu.OPT_DYN_RUNS_PER_WAVE, 0)
-    assert not any(r[0] for r in out[1][1])
-    for rpw in (3, 2):
-        assert np.array_equal(out[rpw][0], out[1][0], equal_nan=True), rpw
-        assert out[rpw][1] == out[1][1], rpw
+            rc = L.pbagpu_dyn_evolve_ragged(params, n, bpr, inits, ntm, (C.c_int * n)(*nts), C.c_void_p(d_tg.data_ptr()),
+                                            C.c_void_p(table.data_ptr()), res, C.c_void_p(stream.cuda_stream))
+        finally:
+            gpu.set_option(gpu.OPT_DYN_RUNS_PER_WAVE, 0)
+        assert rc == 0, L.pbagpu_last_error()
+        got = table.cpu().numpy()
+        for k, (run, pl) in enumerate(single):
+            rows = slice(k * bpr, (k + 1) * bpr)
+            assert np.array_equal(got[rows, :, :nts[k]], pl.d_table.cpu().numpy(), equal_nan=True), names[k]
+            assert not got[rows, :, nts[k]:].any(), names[k]
+            for i in range(bpr):
+                a, b = res[k * bpr + i], pl.res[i]
+                assert (a.status, a.i_end_r, a.fsrs, a.n_steps, a.n_rhs) == \
+                    (b.status, b.i_end_r, b.fsrs, b.n_steps, b.n_rhs), (names[k], i)
 
 
 def test_per_layer_electron_grids_match_oracle_and_reference():
