@@ -98,8 +98,8 @@ LC_TOL = {"tophat_small": 1e-6, "tophat_fs_syn": 1e-6, "gauss5_small": 1e-6, "to
           "gauss_pw_small": 1e-6, "gauss_pw_default": 1e-6, "tophat64_pw": 1e-6, "tophat_ssa_small": 1e-6,
           "gauss5_ssa_small": 1e-6, "tophat_dermer_small": 1e-6, "tophat_bessel_small": 1e-6,
           "tophat_gsl_small": 1e-6, "tophat_rk4_small": 1e-6, "tophat_wind_small": 1e-6,
-          # full-size jet: the reference evaluates Gamma43 and dM3/dR in x87 extended precision, so
-          # the reverse-shock layers' dynamics of any FP64 restatement differ from it by ~1e-6 (the
+          # full-size jet: once the shared adaptive step first differs by an ulp (the CPU oracle's
+          # too), the reverse-shock layers' dynamics part from the reference's at ~1e-6 (the
           # oracle's layer 4: 6.6e-7 in U_p), and the light curves' high-frequency tails, 13 decades
           # below a layer's peak, amplify that: the oracle's own light curve of layer 4 is 6.3e-6 off
           # (2.2e-6 above 1e-10 of the peak); measured on the GPU: 1.5e-5
