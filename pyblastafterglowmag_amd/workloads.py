@@ -179,6 +179,11 @@ SMALL = {
                              method_eats="piece-wise", n_layers_pw=6, theta_w=0.2),
     "tophat_pw_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
                             method_eats="piece-wise", n_layers_pw=4),
+    # the 64-layer piece-wise Gaussian jet (4096 cells, both shocks, SSC) of gauss64_pw_ng at the
+    # reduced microphysics grids, one grid for every layer, with the reverse shock ending on the
+    # density criterion as in gauss_pw_small
+    "gauss64_pw_small": dict(base="gauss21_fsrs_ssc", ntb=1000, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
+                             method_eats="piece-wise", n_layers_pw=64, theta_w=0.2, rs_shutOff_criterion_rho=1e-30),
     # synchrotron self-absorption on (the radio points of the light curve are self-absorbed early on)
     "tophat_ssa_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
                              use_ssa_fs="yes"),

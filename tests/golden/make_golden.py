@@ -46,17 +46,19 @@ SKY_ATTRS = ["flux", "nshells", "nlayers", "time", "freq", "xmin", "ymin", "xmax
 
 # light curves of every `stride`-th layer, dynamics of layer 0, no spectra or sky maps: the full-size
 # case and the uniform-grid references of the per-layer-grid jet
-LARGE = {"tophat64_pw": 7, "gauss_pw_ng32": 1, "gauss_pw_ng48": 1, "gauss21_fsrs_ssc": 4}
+LARGE = {"tophat64_pw": 7, "gauss_pw_ng32": 1, "gauss_pw_ng48": 1, "gauss21_fsrs_ssc": 4, "gauss64_pw_small": 8}
 # large cases that keep the dynamics of every stored light-curve layer (not only layer 0): the
 # reverse shock's end per layer decides which layers a test holds to the tight bound
-LARGE_DYN_ALL = {"gauss21_fsrs_ssc"}
+LARGE_DYN_ALL = {"gauss21_fsrs_ssc", "gauss64_pw_small"}
+# large piece-wise cases that keep their raw sky maps
+LARGE_SKY = {"gauss64_pw_small"}
 
 
 def run_reference(name, wd, spec_every=1):
     main, grb, struct = build_case(name)
     large = name in LARGE
     grb = dict(grb, save_dynamics="yes", save_spec="no" if large else "yes")
-    sky = grb.get("method_eats") == "piece-wise" and not large
+    sky = grb.get("method_eats") == "piece-wise" and (not large or name in LARGE_SKY)
     if sky:   # raw sky maps at 2 frequencies x 2 observer times
         main = dict(main, skymap_times="array 1e5 1e7", skymap_freqs="array 1e9 1e18")
         grb = dict(grb, do_skymap="yes", save_raw_skymap="yes")
@@ -107,6 +109,8 @@ def run_reference(name, wd, spec_every=1):
     out["dyn_root_counts"] = np.array([h5io.attr(dyn, k) for k in ("nshells", "nlayers", "ntimes", "ncells")])
     if large:
         out["spec_every"] = np.array(spec_every)
+        if sky:
+            _store_sky(out, wd)
         return out
     spec = os.path.join(wd, "spec.h5")
     for tag in ("fs", "rs"):
@@ -137,20 +141,25 @@ def run_reference(name, wd, spec_every=1):
                 if rows:
                     out[f"spec_{ds}_{tag}"] = np.stack(rows)
     if sky:
-        n = 0
-        while os.path.exists(os.path.join(wd, f"raw_skymap_{n}.h5")):
-            f = os.path.join(wd, f"raw_skymap_{n}.h5")
-            for v in h5io.members(f, "shell=0"):
-                out[f"sky{n}_{v}"] = h5io.read(f, f"shell=0/{v}")
-            out[f"sky{n}_fluxs"] = h5io.read(f, "fluxs")
-            out[f"sky{n}_attrs"] = np.array([h5io.attr(f, k) for k in SKY_ATTRS])
-            n += 1
-        out["sky_n"] = np.array(n)
-        out["sky_root_members"] = np.array(h5io.members(os.path.join(wd, "raw_skymap_0.h5"), "/"))
-        out["sky_root_attrs"] = np.array(h5io.attr_names(os.path.join(wd, "raw_skymap_0.h5"), "/"))
+        _store_sky(out, wd)
     out["spec_root_members"] = np.array(h5io.members(spec, "/"))
     out["spec_layer_members"] = np.array(h5io.members(spec, "shell=0 layer=0"))
     return out
+
+
+def _store_sky(out, wd):
+    """The raw sky maps raw_skymap_<n>.h5 of a run, every dataset and attribute."""
+    n = 0
+    while os.path.exists(os.path.join(wd, f"raw_skymap_{n}.h5")):
+        f = os.path.join(wd, f"raw_skymap_{n}.h5")
+        for v in h5io.members(f, "shell=0"):
+            out[f"sky{n}_{v}"] = h5io.read(f, f"shell=0/{v}")
+        out[f"sky{n}_fluxs"] = h5io.read(f, "fluxs")
+        out[f"sky{n}_attrs"] = np.array([h5io.attr(f, k) for k in SKY_ATTRS])
+        n += 1
+    out["sky_n"] = np.array(n)
+    out["sky_root_members"] = np.array(h5io.members(os.path.join(wd, "raw_skymap_0.h5"), "/"))
+    out["sky_root_attrs"] = np.array(h5io.attr_names(os.path.join(wd, "raw_skymap_0.h5"), "/"))
 
 
 def main(names):
@@ -171,7 +180,7 @@ def main(names):
 ALL = ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gauss_pw_small", "gauss_pw_default",
        "tophat64_pw", "gauss_pw_ng32", "gauss_pw_ng48",
        "tophat_ssa_small", "gauss5_ssa_small", "tophat_dermer_small", "tophat_bessel_small", "tophat_gsl_small",
-       "tophat_rk4_small", "tophat_wind_small", "gauss21_fsrs_ssc"]
+       "tophat_rk4_small", "tophat_wind_small", "gauss21_fsrs_ssc", "gauss64_pw_small"]
 
 if __name__ == "__main__":
     main(sys.argv[1:] or ALL)

@@ -107,7 +107,7 @@ LC_TOL = {"tophat_small": 1e-6, "tophat_fs_syn": 1e-6, "gauss5_small": 1e-6, "to
 MOVED_TOL = 1.0
 # raw sky-map cells (every cell of every layer at 2 t_obs x 2 nu_obs) against the reference's files;
 # in the reverse-shock run a cell's intensity carries the shared ODE step's ulp-level differences
-SKY_TOL = {"tophat_pw_small": 1e-6, "gauss_pw_small": 5e-6}
+SKY_TOL = {"tophat_pw_small": 1e-6, "gauss_pw_small": 5e-6, "gauss64_pw_small": 5e-6}
 
 
 @pytest.mark.parametrize("name", ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gauss_pw_small",
@@ -225,11 +225,12 @@ def test_mixed_precision_ssc_lightcurve(name):
     print(f"[gpu] {name}: max relative light-curve difference mixed vs FP64 = {worst:.3e}", flush=True)
 
 
-@pytest.mark.parametrize("name", ["tophat_pw_small", "gauss_pw_small"])
+@pytest.mark.parametrize("name", ["tophat_pw_small", "gauss_pw_small", "gauss64_pw_small"])
 def test_skymaps_match_oracle_and_reference(name):
     """Raw piece-wise sky maps (pbagpu_eats_skymap_pw): every cell's entry against the oracle on the
     GPU's own tables (1e-10), and the normalised, cleaned images against the reference program's
-    raw_skymap_<i>.h5 (the light-curve tolerance of the fixture)."""
+    raw_skymap_<i>.h5 (SKY_TOL).  gauss64_pw_small is the 64-layer, 4096-cell jet with both shocks
+    and SSC: 4 images of 4096 cells each."""
     from pyblastafterglowmag_amd import gpu, output
     g = load_golden(name)
     run = golden_run(g)

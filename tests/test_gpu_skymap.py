@@ -59,7 +59,7 @@ def check_image(res, i, ents, ex=None, ey=None, nx=None, ny=None, eg=None):
     return hist
 
 
-@pytest.mark.parametrize("name", ["tophat_pw_small", "gauss_pw_small"])
+@pytest.mark.parametrize("name", ["tophat_pw_small", "gauss_pw_small", "gauss64_pw_small"])
 @pytest.mark.parametrize("nx,ny,eg", [(257, 257, 1.0), (64, 32, 2.0)])
 def test_reference_raw_maps(name, nx, ny, eg):
     """All raw maps of one run in one call: 256 x 256 bins (the large piece-wise workload's maps)

@@ -93,7 +93,7 @@ def test_oracle_full_tophat_bit_exact():
 SKY_ATTRS = ["flux", "nshells", "nlayers", "time", "freq", "xmin", "ymin", "xmax", "ymax"]
 
 
-@pytest.mark.parametrize("name,tol", [("tophat_pw_small", 0.0), ("gauss_pw_small", 1e-12)])
+@pytest.mark.parametrize("name,tol", [("tophat_pw_small", 0.0), ("gauss_pw_small", 1e-12), ("gauss64_pw_small", 1e-11)])
 def test_oracle_skymaps_match_reference(name, tol):
     """Raw piece-wise sky maps: the oracle's per-cell arrays, normalised and cleaned by the host
     code the GPU path uses (output.skymap_images), against the reference's raw_skymap_<i>.h5."""
