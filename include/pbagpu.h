@@ -422,7 +422,7 @@ int pbagpu_synchrotron_f(int n, const double *x, double *f);
 /* No counterpart in the reference: a multilayer perceptron standing in for the SSC
  * contraction computeSSCSpectrum (src/microphysics/numeric_model.h:247-285), trained on
  * pairs that the plain contraction itself produces, with the plain contraction as the
- * fallback and the judge on a sampled subset of cells every step.
+ * fallback and the check on a sampled subset of cells every step.
  *
  * pba_mlp: dims[0] -> dims[1] -> ... -> dims[n_layers]; SiLU after every layer but the
  * last; weights w[l] [dims[l]][dims[l+1]] and biases b[l] [dims[l+1]] in FP32 on the

@@ -1,7 +1,7 @@
 // Multilayer perceptron for the learned SSC emissivity: BF16 operands on MFMA with FP32
 // accumulation, SiLU hidden layers, linear output; forward pass, mean-square-error backward pass
-// and Adam update, all on the GPU.  The network has no counterpart in the reference; its judge is
-// the plain SSC contraction (computeSSCSpectrum, src/microphysics/numeric_model.h:247-285).
+// and Adam update, all on the GPU.  The network has no counterpart in the reference; it is checked
+// against the plain SSC contraction (computeSSCSpectrum, src/microphysics/numeric_model.h:247-285).
 //
 // Every product is one kernel, gemm_bf16: C[M][N] (FP32) = sum_k A(m, k) B(k, n) with A and B read
 // through strides, so that the forward product X W, the input gradient dZ W^T and the weight
