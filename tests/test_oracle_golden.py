@@ -144,13 +144,16 @@ def test_oracle_full_piecewise_tophat():
     np.testing.assert_array_equal(lc, g["lc"])
 
 
-def test_oracle_full_gaussian_jet_dynamics():
-    """The 21-layer Gaussian jet at full size (both shocks, 1000 steps): the oracle's dynamics of the
-    fixture's layers (every 4th) against the reference's dyn.h5.  Layers whose reverse shock ends at
-    the reference's step (the end step is decided at the last ulp) agree to 1e-4, the reverse-shock
-    bound of the other fixtures, plus 64 ulp of Gamma carried into Gamma - 1 (conftest.dyn_excess:
-    the outer layers reach Gamma - 1 ~ 2e-12, where the measured 12-ulp difference of Gamma is a
-    4e-4 relative difference of beta)."""
+def test_oracle_full_gaussian_jet():
+    """The 21-layer Gaussian jet at full size (both shocks, SSC, 200 gamma x 96 nu, 1000 steps):
+    the oracle against the reference's dyn.h5 and lc.h5 for the fixture's layers (every 4th).
+    Dynamics: layers whose reverse shock ends at the reference's step (the end step is decided at
+    the last ulp) agree to 1e-4, the reverse-shock bound of the other fixtures, plus 64 ulp of Gamma
+    carried into Gamma - 1 (conftest.dyn_excess: the outer layers reach Gamma - 1 ~ 2e-12, where
+    the measured 12-ulp difference of Gamma is a 4e-4 relative difference of beta).  Light curves
+    through the oracle's own electron and observer stages, at every 16th of the 12,288 pairs: 1e-5
+    (over all pairs the largest difference is 6.3e-6, layer 4's reverse-shock term in the
+    high-frequency tail 13 decades below its peak; the other layers are within 3e-7)."""
     g = load_golden("gauss21_fsrs_ssc")
     run = golden_run(g)
     table, res = pyoracle.dyn_evolve(run.params, run.inits, run.tgrid)
@@ -168,6 +171,24 @@ def test_oracle_full_gaussian_jet_dynamics():
         for iv, n in enumerate(names):
             assert dyn_excess(table[il, abi.PV[n]], g["dyn"][k, iv], gam) <= 1.0, (il, n)
     assert matched >= len(layers) // 2, (mine_end, ref_end)
+    # light curves of the recorded layers: the oracle's electron and observer stages on its table
+    pops, lays = [], []
+    for il in layers:
+        pfs, prs = len(pops), -1
+        pops.append(abi.Population(il, 0, 0, 0))
+        if res[il].fsrs and run.params.do_rs_radiation:
+            prs = len(pops)
+            pops.append(abi.Population(il, 0, 1, 0))
+        li = run.inits[il]
+        lays.append(abi.EatsLayer(li.theta_c_l, li.theta_c_h, il, 0, pfs, prs, res[il].i_end_r))
+    jall, _, st = pyoracle.ele_evolve(run.params, (abi.Population * len(pops))(*pops), table)
+    assert not st.any()
+    idx = np.arange(0, len(run.tobs), 16)
+    lc, st2 = pyoracle.eats_lightcurve(run.params, (abi.EatsLayer * len(lays))(*lays), table, jall, run.tobs[idx],
+                                       run.nuobs[idx])
+    assert not st2.any()
+    for k, il in enumerate(layers):
+        assert rel_err(lc[k], g["lc"][k][idx], 1e-30) <= 1e-5, il
 
 
 def test_reverse_shock_end_is_decided_by_the_last_ulp():
