@@ -113,8 +113,9 @@ INTEGRATORS = ("DOP853E", "RK4")   # pba_run_params.integrator 0 / 1
 SUPPORTED_OPTS = dict(
     method_eats=EATS_METHODS, method_quad="CADRE", method_comp_mode="comovSpec", method_spread="our",
     method_limit_spread="MomValAndFrac", method_dgdr="our", method_eos="Nava13", method_dmdr="usingdthdr",
-    integrator=INTEGRATORS, do_average_solution="no", lc_use_freq_to_time="no", ebl_tbl_fpath="none",
+    integrator=INTEGRATORS, do_average_solution="no", lc_use_freq_to_time="no",
 )
+# ebl_tbl_fpath: "none" or an EBL optical-depth table (pyblastafterglowmag_amd.ebl), any path
 
 
 def _yes(v):

@@ -9,7 +9,7 @@ import os
 
 import numpy as np
 
-from . import abi, h5io, inputs
+from . import abi, ebl, h5io, inputs
 
 _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libpbagpu.so")
 _lib = None
@@ -245,13 +245,16 @@ class Run:
     parfile (src/main.cpp:93-154, src/ejecta/ejecta_base.h:141-345, src/ejecta/ejecta.h:35-80).
     """
 
-    def __init__(self, params, inits, tgrid, tobs, nuobs, method="adaptive"):
+    def __init__(self, params, inits, tgrid, tobs, nuobs, method="adaptive", ebl_table=None):
         self.method = method
         self.params = params
         self.inits = inits
         self.tgrid = np.ascontiguousarray(tgrid, dtype=np.float64)
         self.tobs = np.ascontiguousarray(tobs, dtype=np.float64)
         self.nuobs = np.ascontiguousarray(nuobs, dtype=np.float64)
+        # EBL attenuation exp(-tau(nu_obs, z)) per observer pair (ebl_tbl_fpath), or None
+        self.ebl_table = ebl_table
+        self.attenuation = None if ebl_table is None else ebl_table.attenuation(self.nuobs, params.z)
 
     @classmethod
     def from_workdir(cls, workdir, parfile="parfile.par"):
@@ -264,7 +267,9 @@ class Run:
                    theta_core=h5io.attr(idf, "theta_core"))
         tobs, nuobs = abi.obs_pairs(mo)
         method = go.get("method_eats", "adaptive")
-        return cls(P, abi.layer_inits(idd, method), abi.tgrid(mp_), tobs, nuobs, method)
+        tab = ebl.resolve_table_path(go.get("ebl_tbl_fpath", "none"), workdir)
+        return cls(P, abi.layer_inits(idd, method), abi.tgrid(mp_), tobs, nuobs, method,
+                   ebl_table=None if tab is None else ebl.EblTable.load(tab))
 
     @classmethod
     def from_workload(cls, main, grb, struct):
@@ -285,11 +290,22 @@ class Run:
                 raise ValueError(f"layer_ngam has {len(ngs)} entries for {len(inits)} layers")
             for li, n in zip(inits, ngs):
                 li.ngam_fs = li.ngam_rs = int(n)
-        return cls(P, inits, abi.tgrid(mp_), tobs, nuobs, method)
+        tab = ebl.resolve_table_path(go.get("ebl_tbl_fpath", "none"))
+        return cls(P, inits, abi.tgrid(mp_), tobs, nuobs, method,
+                   ebl_table=None if tab is None else ebl.EblTable.load(tab))
 
     @property
     def n_layers(self):
         return len(self.inits)
+
+
+def apply_attenuation(flux, runs):
+    """Layer light curves flux [n_runs][n_layers][n_pairs] times each run's EBL attenuation
+    (Run.attenuation, per pair; runs without a table are left as they are)."""
+    if all(r.attenuation is None for r in runs):
+        return flux
+    att = np.stack([r.attenuation if r.attenuation is not None else np.ones(flux.shape[-1]) for r in runs])
+    return flux * att[:, None, :]
 
 
 def populations(params_list, results, n_runs, bw_per_run, inits=None):
@@ -560,8 +576,11 @@ class Pipeline:
                     self.d_lst[:len(self.layers)].cpu().numpy())
 
     def flux(self):
+        """Light curves [n_runs][n_layers][n_pairs] of the last observer stage, with each run's EBL
+        attenuation applied (EATS::evalFluxDensA, src/eats.h:849-853)."""
         with self._on_rad_stream():
-            return self.d_out.cpu().numpy().reshape(self.n_runs, self.bpr, self.npairs)
+            out = self.d_out.cpu().numpy().reshape(self.n_runs, self.bpr, self.npairs)
+        return apply_attenuation(out, self.runs)
 
 
 def radiate(pl, want_intermediates=False, precision=None):

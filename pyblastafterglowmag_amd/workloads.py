@@ -206,6 +206,11 @@ SMALL = {
     # the classical RK4 integrator, one step per output interval, in place of adaptive DOP853
     "tophat_rk4_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
                              integrator="RK4"),
+    # the source at z = 0.7 with SSC on (TeV photons), 12 observer frequencies from 1e9 to 1e28 Hz, 3
+    # of them inside the frequency range of an EBL optical-depth table (the reference's
+    # ebl_tbl_fpath, set by the caller)
+    "tophat_ebl_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_t_obs=8, z=0.7,
+                             lc_freqs="array logspace 1e9 1e28 12", method_ssc_fs="numeric"),
 }
 
 

@@ -35,7 +35,10 @@ def golden_run(g):
                theta_wing=float(g["id_theta_wing"]), theta_core=float(g["id_theta_core"]))
     tobs, nuobs = abi.obs_pairs(mo)
     method = go.get("method_eats", "adaptive")
-    return gpu.Run(P, abi.layer_inits(idd, method), abi.tgrid(mp_), tobs, nuobs, method)
+    # an EBL case carries the optical-depth table it was run with
+    from pyblastafterglowmag_amd import ebl
+    tab = ebl.EblTable(g["ebl_freq"], g["ebl_z"], g["ebl_tau"]) if "ebl_tau" in g else None
+    return gpu.Run(P, abi.layer_inits(idd, method), abi.tgrid(mp_), tobs, nuobs, method, ebl_table=tab)
 
 
 def rel_err(a, b, floor_frac=0.0):

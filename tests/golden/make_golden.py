@@ -24,6 +24,10 @@ sys.path.insert(0, ROOT)
 from pyblastafterglowmag_amd import abi, h5io, inputs, workloads  # noqa: E402
 
 PBA = os.path.join(ROOT, "oracle", "_ref", "pba.out")
+# the reference's EBL optical-depth table (Franceschini & Rodighiero 2018), for the cases that
+# attenuate their light curves; the fixture keeps the table's arrays beside the outputs
+EBL_TABLE = "/root/reference/data/EBL/Franceschini18/table.h5"
+EBL_CASES = {"tophat_ebl_small"}
 DYN_VARS = ["R", "Gamma", "beta", "theta", "M2", "Eint2", "tt", "tcomov", "rho2", "GammaFsh", "thickness", "U_p",
             "gamma_min", "gamma_max", "gamma_c", "B", "accel_frac", "Rsh"]
 DYN_VARS_RS = ["M3", "Eint3", "deltaR4", "Gamma43", "rho3", "GammaRsh", "thichness_rs", "U_p3", "gamma_min_rs",
@@ -58,6 +62,8 @@ def run_reference(name, wd, spec_every=1):
     main, grb, struct = build_case(name)
     large = name in LARGE
     grb = dict(grb, save_dynamics="yes", save_spec="no" if large else "yes")
+    if name in EBL_CASES:
+        grb["ebl_tbl_fpath"] = EBL_TABLE
     sky = grb.get("method_eats") == "piece-wise" and (not large or name in LARGE_SKY)
     if sky:   # raw sky maps at 2 frequencies x 2 observer times
         main = dict(main, skymap_times="array 1e5 1e7", skymap_freqs="array 1e9 1e18")
@@ -69,6 +75,9 @@ def run_reference(name, wd, spec_every=1):
     out = {}
     with open(os.path.join(wd, "parfile.par")) as f:
         out["parfile"] = np.frombuffer(f.read().encode(), dtype=np.uint8)
+    if name in EBL_CASES:
+        for k in ("freq", "z", "tau"):
+            out["ebl_" + k] = h5io.read(EBL_TABLE, k)
     for k in ("mom", "ctheta", "ek", "mass"):
         out["id_" + k] = h5io.read(idf, k)
     out["id_theta_wing"] = np.array(h5io.attr(idf, "theta_wing"))
@@ -180,7 +189,7 @@ def main(names):
 ALL = ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gauss_pw_small", "gauss_pw_default",
        "tophat64_pw", "gauss_pw_ng32", "gauss_pw_ng48",
        "tophat_ssa_small", "gauss5_ssa_small", "tophat_dermer_small", "tophat_bessel_small", "tophat_gsl_small",
-       "tophat_rk4_small", "tophat_wind_small", "gauss21_fsrs_ssc", "gauss64_pw_small"]
+       "tophat_rk4_small", "tophat_wind_small", "gauss21_fsrs_ssc", "gauss64_pw_small", "tophat_ebl_small"]
 
 if __name__ == "__main__":
     main(sys.argv[1:] or ALL)
