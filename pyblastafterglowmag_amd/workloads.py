@@ -95,6 +95,18 @@ def lhs_member(i, n=1024, seed=LHS_SEED, theta_w_over_theta_c=4.0, ragged_nt=Tru
     return main, grb, struct
 
 
+def lhs_small(i, ragged_nt=True):
+    """Member i of the Latin-hypercube workload with the microphysics grids reduced (64 Lorentz-factor
+    bins, 48 comoving frequencies) and 6 x 8 observer pairs: the members the batch tests run."""
+    main, grb, struct = lhs_member(i, ragged_nt=ragged_nt)
+    main = dict(main, lc_freqs="array logspace 1e8 1e29 6", lc_times="array logspace 3e3 1e10 8")
+    grb = dict(grb)
+    for sh in ("fs", "rs"):
+        grb[f"ngam_{sh}"] = 64
+        grb[f"nfreq_{sh}"] = 48
+    return main, grb, struct
+
+
 # Per-layer electron grids of the piece-wise Gaussian workload: a seeded uniform draw per layer
 C3_NGAM = (128, 256, 384, 512)
 

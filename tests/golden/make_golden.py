@@ -37,6 +37,8 @@ DYN_VARS_RS = ["M3", "Eint3", "deltaR4", "Gamma43", "rho3", "GammaRsh", "thichne
 def build_case(name):
     if name in workloads.SMALL:
         return workloads.small(name)
+    if name.startswith("lhs_small_"):   # a member of the Latin-hypercube batch at reduced grids
+        return workloads.lhs_small(int(name.rsplit("_", 1)[1]))
     return workloads.WORKLOADS[name]()
 
 
@@ -50,10 +52,13 @@ SKY_ATTRS = ["flux", "nshells", "nlayers", "time", "freq", "xmin", "ymin", "xmax
 
 # light curves of every `stride`-th layer, dynamics of layer 0, no spectra or sky maps: the full-size
 # case and the uniform-grid references of the per-layer-grid jet
-LARGE = {"tophat64_pw": 7, "gauss_pw_ng32": 1, "gauss_pw_ng48": 1, "gauss21_fsrs_ssc": 4, "gauss64_pw_small": 8}
+LARGE = {"tophat64_pw": 7, "gauss_pw_ng32": 1, "gauss_pw_ng48": 1, "gauss21_fsrs_ssc": 4, "gauss64_pw_small": 8,
+         "lhs_small_0": 4, "lhs_small_42": 4}
 # large cases that keep the dynamics of every stored light-curve layer (not only layer 0): the
 # reverse shock's end per layer decides which layers a test holds to the tight bound
-LARGE_DYN_ALL = {"gauss21_fsrs_ssc", "gauss64_pw_small"}
+LARGE_DYN_ALL = {"gauss21_fsrs_ssc", "gauss64_pw_small", "lhs_small_0", "lhs_small_42"}
+# dynamics kept at every n-th step only (dyn_every in the fixture)
+DYN_EVERY = {"lhs_small_0": 10, "lhs_small_42": 10}
 # large piece-wise cases that keep their raw sky maps
 LARGE_SKY = {"gauss64_pw_small"}
 
@@ -106,7 +111,9 @@ def run_reference(name, wd, spec_every=1):
             ds = f"shell=0 layer={il}/{v}"
             if h5io.has(dyn, ds):
                 D[i, iv] = h5io.read(dyn, ds)
-    out["dyn"] = D
+    every = DYN_EVERY.get(name, 1)
+    out["dyn"] = np.ascontiguousarray(D[:, :, ::every])
+    out["dyn_every"] = np.array(every)
     out["dyn_layers"] = np.array(dyn_layers)
     out["dyn_names"] = np.array(names)
     out["dyn_layer_members"] = np.array(h5io.members(dyn, "shell=0 layer=0"))
@@ -189,7 +196,8 @@ def main(names):
 ALL = ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gauss_pw_small", "gauss_pw_default",
        "tophat64_pw", "gauss_pw_ng32", "gauss_pw_ng48",
        "tophat_ssa_small", "gauss5_ssa_small", "tophat_dermer_small", "tophat_bessel_small", "tophat_gsl_small",
-       "tophat_rk4_small", "tophat_wind_small", "gauss21_fsrs_ssc", "gauss64_pw_small", "tophat_ebl_small"]
+       "tophat_rk4_small", "tophat_wind_small", "gauss21_fsrs_ssc", "gauss64_pw_small", "tophat_ebl_small",
+       "lhs_small_0", "lhs_small_42"]
 
 if __name__ == "__main__":
     main(sys.argv[1:] or ALL)

@@ -264,12 +264,7 @@ def test_skymaps_match_oracle_and_reference(name):
 
 def _lhs_small(i, ragged_nt=False):
     from pyblastafterglowmag_amd import gpu, workloads
-    main, grb, struct = workloads.lhs_member(i, ragged_nt=ragged_nt)
-    main = dict(main, lc_freqs="array logspace 1e8 1e29 6", lc_times="array logspace 3e3 1e10 8")
-    for sh in ("fs", "rs"):
-        grb[f"ngam_{sh}"] = 64
-        grb[f"nfreq_{sh}"] = 48
-    return gpu.Run.from_workload(main, grb, struct)
+    return gpu.Run.from_workload(*workloads.lhs_small(i, ragged_nt=ragged_nt))
 
 
 def test_heterogeneous_batch_matches_oracle():
