@@ -113,7 +113,7 @@ INTEGRATORS = ("DOP853E", "RK4")   # pba_run_params.integrator 0 / 1
 SUPPORTED_OPTS = dict(
     method_eats=EATS_METHODS, method_quad="CADRE", method_comp_mode="comovSpec", method_spread="our",
     method_limit_spread="MomValAndFrac", method_dgdr="our", method_eos="Nava13", method_dmdr="usingdthdr",
-    integrator=INTEGRATORS, do_average_solution="no", lc_use_freq_to_time="no",
+    integrator=INTEGRATORS, do_average_solution="no",
 )
 # ebl_tbl_fpath: "none" or an EBL optical-depth table (pyblastafterglowmag_amd.ebl), any path
 
@@ -270,7 +270,15 @@ def tgrid(mp_):
 
 
 def obs_pairs(mo):
-    """(t_obs, nu_obs) pairs, time-major (cast_times_freqs, src/utilitites/utils.h:1419-1448)."""
+    """(t_obs, nu_obs) pairs (cast_times_freqs, src/utilitites/utils.h:1419-1448): every time with
+    every frequency, time-major; with lc_use_freq_to_time = yes the i-th time with the i-th
+    frequency (the two lists must have one length)."""
     ts = inputs.vec_from_string(mo["lc_times"])
     fs = inputs.vec_from_string(mo["lc_freqs"])
+    one_to_one = _yes(mo.get("lc_use_freq_to_time", "no"))
+    if one_to_one:
+        if len(ts) != len(fs):
+            raise ValueError("lc_use_freq_to_time = yes: lc_times and lc_freqs need one length "
+                             f"({len(ts)} != {len(fs)})")
+        return ts.copy(), fs.copy()
     return np.repeat(ts, len(fs)), np.tile(fs, len(ts))

@@ -157,7 +157,7 @@ def scaled(name, ntb=None, ngam=None, nfreq=None, n_nu_obs=None, n_t_obs=None, n
         if k in struct or k in ("n_layers_pw", "layer_ngam"):
             struct[k] = v
         elif k in ("tb0", "tb1", "ntb", "n_ism", "A0", "s", "r_ej", "r_ism", "d_l", "z", "theta_obs", "lc_freqs",
-                   "lc_times", "rtol", "integrator"):
+                   "lc_times", "lc_use_freq_to_time", "rtol", "integrator"):
             main[k] = v
         else:
             grb[k] = v
@@ -223,6 +223,10 @@ SMALL = {
     # ebl_tbl_fpath, set by the caller)
     "tophat_ebl_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, n_t_obs=8, z=0.7,
                              lc_freqs="array logspace 1e9 1e28 12", method_ssc_fs="numeric"),
+    # one frequency per observer time (lc_use_freq_to_time = yes): 16 pairs, the i-th time with the
+    # i-th frequency, a frequency sweep that rises with time
+    "tophat_f2t_small": dict(base="tophat_fs_syn", ntb=300, ngam=64, nfreq=48, lc_use_freq_to_time="yes",
+                             lc_times="array logspace 3e3 1e10 16", lc_freqs="array logspace 1e8 1e20 16"),
 }
 
 

@@ -197,7 +197,7 @@ ALL = ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gaus
        "tophat64_pw", "gauss_pw_ng32", "gauss_pw_ng48",
        "tophat_ssa_small", "gauss5_ssa_small", "tophat_dermer_small", "tophat_bessel_small", "tophat_gsl_small",
        "tophat_rk4_small", "tophat_wind_small", "gauss21_fsrs_ssc", "gauss64_pw_small", "tophat_ebl_small",
-       "lhs_small_0", "lhs_small_42"]
+       "lhs_small_0", "lhs_small_42", "tophat_f2t_small"]
 
 if __name__ == "__main__":
     main(sys.argv[1:] or ALL)

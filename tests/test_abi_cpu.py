@@ -346,3 +346,17 @@ def test_bench_refuses_more_than_one_gpu():
     p = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
                        timeout=120)
     assert p.returncode == 2 and "one GPU" in p.stderr
+
+
+def test_observer_pairs_grid_and_one_to_one():
+    """cast_times_freqs (src/utilitites/utils.h:1419-1448): every time with every frequency, time-major;
+    with lc_use_freq_to_time = yes the i-th time with the i-th frequency, lists of one length."""
+    mo = {"lc_times": "array 1 2 3", "lc_freqs": "array 10 20"}
+    t, f = abi.obs_pairs(mo)
+    np.testing.assert_array_equal(t, [1, 1, 2, 2, 3, 3])
+    np.testing.assert_array_equal(f, [10, 20, 10, 20, 10, 20])
+    t, f = abi.obs_pairs(dict(mo, lc_freqs="array 10 20 30", lc_use_freq_to_time="yes"))
+    np.testing.assert_array_equal(t, [1, 2, 3])
+    np.testing.assert_array_equal(f, [10, 20, 30])
+    with pytest.raises(ValueError):
+        abi.obs_pairs(dict(mo, lc_use_freq_to_time="yes"))

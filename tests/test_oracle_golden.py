@@ -59,6 +59,8 @@ def _oracle_all(g):
     ("tophat_gsl_small", 0.0, 1e-11, 1e-11),
     ("tophat_rk4_small", 0.0, 0.0, 0.0),
     ("tophat_wind_small", 0.0, 0.0, 0.0),
+    # lc_use_freq_to_time = yes: 16 one-to-one (t_obs, nu_obs) pairs
+    ("tophat_f2t_small", 0.0, 0.0, 0.0),
 ])
 def test_oracle_matches_reference(name, tol_dyn, tol_j, tol_lc):
     g = load_golden(name)
