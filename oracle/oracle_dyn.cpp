@@ -8,6 +8,8 @@
  *   - initial conditions      BlastWave::setInitConditions   (blastwave.h:73-241)
  *   - forward-shock RHS       BlastWave::rhs_fs              (blastwave.h:1515-1768)
  *   - forward+reverse RHS     BlastWave::rhs_fsrs            (blastwave.h:1771-2116)
+ *   - kilonova ejecta RHS     BlastWave::setStandardISM + rhs_fs_dense (blastwave.h:769-781,
+ *                             1282-1295, 2120-2497), bw_type = fs_dense, entry orc_dyn_evolve_ejecta
  *   - radiative loss          ElectronAndRadiation::computeRadiationLoss / computePtot
  *                             (microphysics.h:204-220, 749-795; analytic_models.h:889-911, 1051-1078)
  *   - embedded DOP853         dop853_integrator::integrate / _integrate8 / _determine_initial_step
@@ -45,6 +47,11 @@ struct BW {
     double theta_max, Rd, rho_def, drhodr0;
     int ilayer, nlayers;
     bool fsrs, do_rs, do_rs_rad, shutOff, end_evolution, end_spreading;
+    // ejecta blastwave (bw_type = fs_dense): evolved by rhs_fs_dense in a standard ISM; `spread`
+    // is method_spread != None.  The upstream state that the RHS leaves in the blastwave's density
+    // object (RhoISM m_rho_, m_drhodr_, m_GammaRel) is what addOtherVars stores afterwards.
+    bool dense = false, spread = true, rhs_seen = false;
+    double rho_rhs = 0., drhodr_rhs = 0., grel_rhs = -1.;
     int prev_idx_x;
     int i_end_r;
     MState fs_st, rs_st;
@@ -436,14 +443,98 @@ static void rhs_fsrs(Run &run, BW &b, const double *Y, double *out) {
     out[ideltaR4] = dRdt * ddeltaR4dR;
 }
 
-// RHS_comb over all blastwaves of one run (model_evolve.h:1587-1689)
+// EQS::evalElapsedTime (blastwave_components.h:165-175): dt_obs/dR, the spreading form or the
+// radial one
+static double elapsed_dttdr_radial(double R, double mom, double dthetadr, bool spread) {
+    double xx = 1. + mom * mom * mom / (std::sqrt(1. + mom * mom));
+    double beta = BetaFromMom(mom);
+    if (spread) return 1. / (c * beta) * std::sqrt(1. + R * R * dthetadr * dthetadr) - (1. / c);
+    return 1. / (c * xx * (1. + beta));
+}
+
+// Ejecta blastwave in a standard ISM: BlastWave::setStandardISM / set_standard_ism
+// (blastwave.h:769-781, 1282-1295), then BlastWave::rhs_fs_dense (blastwave.h:2120-2497) with
+// method_dgdr = our, method_dmdr = usingdthdr, no energy injection, nuclear heating or
+// luminosity loss (dEinjdt = dEnuc = dElum = 0: no magnetar, do_nucinj = no, no collisions).
+// The upstream is at rest (GammaRho = 1, dGammaRho/dR = 0, sound speed 0) and GammaRel = Gamma,
+// dGammaRel/dGamma = 1; the zero terms are kept so that every sum rounds as the reference's.
+// dtheta/dR = 0 and Rsh is not evolved (its derivative is never written).
+static void rhs_fs_dense(Run &run, BW &b, const double *Y, double *out) {
+    const pba_run_params &P = *run.P;
+    double Gamma = Y[iGamma];
+    double R = Y[iR], Eint2 = Y[iEint2], theta = Y[itheta], M2 = Y[iM2];
+    if (Gamma < 1.) { Gamma = 1. + 1.e-5; b.status |= PBAGPU_S_NONFINITE; }
+    double beta = BetaFromGamma(Y[iGamma]);
+    if (!std::isfinite(R) || !std::isfinite(Gamma) || M2 < 0. || !std::isfinite(M2) || !std::isfinite(Eint2) ||
+        Eint2 < 0)
+        b.status |= PBAGPU_S_NONFINITE;
+    // set_standard_ism: evaluateRhoDrhoDrDefault at (R, ctheta); ctheta only matters for profiles
+    // that depend on angle, which the default one does not.  GammaRel is the blastwave's Gamma,
+    // clamped as in the RHS (setStandardISM, blastwave.h:771-775).
+    double rho_c = 0., drhodr_c = 0.;
+    ism_density(P, R, rho_c, drhodr_c, b.status);
+    const double GammaRho = 1., GammaRel = Gamma, dGammaRelDGamma = 1., dGammaRhodR = 0., cs_cbm = 0.;
+    b.rho_rhs = rho_c; b.drhodr_rhs = drhodr_c; b.grel_rhs = GammaRel; b.rhs_seen = true;
+    double rho = rho_c / b.M0;
+    double drhodr = drhodr_c / b.M0;
+    double dRdt = beta * c;
+    double Gamma_ = GammaRel;
+    double adi = adiNava(Gamma_);
+    double dthetadr = 0.0;
+    double dM2dR = dmdr(R, dthetadr, theta, rho) / b.ncells;
+    if (dM2dR < 0.) b.status |= PBAGPU_S_NONFINITE;
+    // energy sources, all zero here
+    const double xi_inj = 1.;
+    double dEinjdt = 0. / (b.M0 * c * c) / b.ncells;
+    double dEinjdR = dEinjdt / dRdt;
+    double Doppler = Gamma / (1. - beta * std::cos(0.));
+    double dEingdR_abs = dEinjdR;
+    double dEingdR_abs_dop = dEingdR_abs / Doppler / Doppler;
+    double dEnucdR = 0. / (b.M0 * c * c) / dRdt;
+    double dElumdR = 0. / (b.M0 * c * c) / dRdt;
+    // dGamma/dR, method "our"
+    double GammaEff = (adi * Gamma_ * Gamma_ - adi + 1.0) / Gamma_;
+    double dGammaEffdGamma = (adi * Gamma_ * Gamma_ + adi - 1.0) / (Gamma_ * Gamma_);
+    double num1 = (Gamma - GammaRho + GammaEff * (GammaRel - 1.)) * dM2dR;
+    double num2 = -GammaEff * (adi - 1.) * Eint2 * (dM2dR / M2 - drhodr / rho - dGammaRhodR / GammaRho);
+    double num3 = -(1. + GammaEff / Gamma * xi_inj) * dEingdR_abs;
+    double denum1 = (1. + M2);
+    double denum2 = Eint2 * dGammaEffdGamma;
+    double denom3 = GammaEff * (adi - 1.) * Eint2 * dGammaRelDGamma / GammaRel;
+    double dGammadR = -1. * (num1 + num2 + num3) / (denum1 + denum2 + denom3);
+    double dlnV2dR = dM2dR / M2 - drhodr / rho - (1. / GammaRel) * dGammaRelDGamma * dGammadR + dGammaRhodR / GammaRho;
+    double dEad2dR = 0.0;
+    if (P.adiab_loss) dEad2dR = -(adi - 1.0) * Eint2 * dlnV2dR;
+    double dEsh2dR = (GammaRel - 1.0) * dM2dR;
+    if (cs_cbm > BetaFromGamma(GammaRel)) dEsh2dR *= 1e-10;   // subsonic upstream: no shock
+    double dErad2dR = P.fs.epsilon_e_rad * dEsh2dR;
+    double dEint2dR = dEsh2dR + dEad2dR - dErad2dR + dEnucdR - dElumdR + dEingdR_abs_dop;
+    double dtcomov_dR = 1.0 / beta / Gamma / c;
+    double dttdr = elapsed_dttdr_radial(R, Gamma * beta, dthetadr, b.spread);
+    if (!std::isfinite(dRdt) || !std::isfinite(dGammadR) || dM2dR < 0. || !std::isfinite(dlnV2dR))
+        b.status |= PBAGPU_S_NONFINITE;
+    out[iR] = dRdt;
+    out[itt] = dRdt * dttdr;
+    out[itcomov] = dRdt * dtcomov_dR;
+    out[iGamma] = dRdt * dGammadR;
+    out[iEint2] = dRdt * dEint2dR;
+    out[itheta] = dRdt * dthetadr;
+    out[iErad2] = dRdt * dErad2dR;
+    out[iEsh2] = dRdt * dEsh2dR;
+    out[iEad2] = dRdt * dEad2dR;
+    out[iM2] = dRdt * dM2dR;
+}
+
+// RHS_comb over all blastwaves of one run (model_evolve.h:1587-1689): jet layers by bw_type,
+// ejecta blastwaves always through setStandardISM + rhs_fs_dense (:1631-1651)
 static void rhs_all(Run &run, double x, const double *Y, double *out) {
     (void)x;
     run.n_rhs++;
     for (size_t i = 0; i < run.bws.size(); i++) {
         BW &b = run.bws[i];
         if (b.end_evolution) continue;
-        if (b.fsrs) rhs_fsrs(run, b, Y + i * NEQ, out + i * NEQ);
+        if (b.dense) rhs_fs_dense(run, b, Y + i * NEQ, out + i * NEQ);
+        else if (b.fsrs) rhs_fsrs(run, b, Y + i * NEQ, out + i * NEQ);
         else rhs_fs(run, b, Y + i * NEQ, out + i * NEQ);
     }
 }
@@ -638,7 +729,8 @@ static void bw_init(Run &run, BW &b, const pba_layer_init &id, double t0_grid, d
     b.ctheta0 = id.ctheta0; b.theta_w = id.theta_wing; b.theta_core = id.theta_core;
     b.theta_max = P.theta_max;
     b.ilayer = id.ilayer; b.nlayers = id.nlayers;
-    b.fsrs = P.fsrs != 0; b.do_rs = b.fsrs && P.do_rs != 0; b.do_rs_rad = b.fsrs && P.do_rs_radiation != 0;
+    b.fsrs = !b.dense && P.fsrs != 0;
+    b.do_rs = b.fsrs && P.do_rs != 0; b.do_rs_rad = b.fsrs && P.do_rs_radiation != 0;
     b.shutOff = false; b.end_evolution = false; b.end_spreading = false;
     b.prev_idx_x = 0; b.status = 0;
     // density at the start radius (evaluateRhoDrhoDrDefault(R0), blastwave.h:92-101): the RHS
@@ -657,11 +749,16 @@ static void bw_init(Run &run, BW &b, const pba_layer_init &id, double t0_grid, d
     if (b.M0 <= 0.) b.M0 = b.E0 / (b.Gamma0 * c * c);
     b.Rd = std::pow(3. / (4. * pi) * 1. / (c * c * mp) * b.E0 / (rho / mp * b.Gamma0 * b.Gamma0), 1. / 3.);
     Y[iR] = b.R0;
-    // EQS::init_elapsed_time with spreading on (blastwave_components.h:85-108)
-    {
+    // EQS::init_elapsed_time (blastwave_components.h:141-163): the jet spreads ("our"); an
+    // ejecta blastwave takes the radial form unless method_spread is set
+    if (b.spread) {
         double beta0m = BetaFromMom(b.mom0);
         double dThetadr0 = 0.0;
         Y[itt] = b.R0 / (c * beta0m) * (std::sqrt(1. + b.R0 * b.R0 * dThetadr0 * dThetadr0)) - b.R0 / c;
+    } else {
+        double xx = 1. + b.mom0 * b.mom0 * b.mom0 / (std::sqrt(1. + b.mom0 * b.mom0));
+        double beta0m = BetaFromMom(b.mom0);
+        Y[itt] = b.R0 / (c * xx * (1.0 + beta0m));
     }
     Y[itcomov] = b.R0 / (b.beta0 * b.Gamma0 * c);
     Y[iEint2] = (b.Gamma0 - 1.) * M20 / b.M0;
@@ -703,6 +800,13 @@ static void post_step(Run &run, BW &b, const double *Ycode, int it, double tb, d
     V(PV_ctheta) = ctheta_of(V(PV_theta), b.ilayer, b.nlayers);
     V(PV_rho) = b.rho_def;
     V(PV_drhodr) = b.drhodr0;
+    if (b.dense) {
+        // the density object as the last RHS call left it (set_standard_ism); before the first
+        // call, setInitConditions' values and GammaRel unset (-1), stored as Gamma (:544-560)
+        if (b.rhs_seen) { V(PV_rho) = b.rho_rhs; V(PV_drhodr) = b.drhodr_rhs; }
+        double grel = b.rhs_seen ? b.grel_rhs : -1.;
+        V(PV_GammaRel) = grel < 1. ? V(PV_Gamma) : grel;
+    }
     V(PV_adi) = adiNava(V(PV_Gamma));
     V(PV_rho2) = rho2t(V(PV_Gamma), V(PV_adi), V(PV_rho));
     V(PV_GammaFsh) = GammaSh(V(PV_Gamma), V(PV_adi));
@@ -741,6 +845,13 @@ static void post_step(Run &run, BW &b, const double *Ycode, int it, double tb, d
         if (b.i_end_r > it - 1) b.i_end_r = it - 1;
         valid = false;
     }
+    // ejecta: a subsonic upstream (sound speed 0 in a standard ISM) has no shock (:918-928); the
+    // velocity test below uses GammaRel, which equals Gamma here
+    if (valid && b.dense && (0. >= V(PV_beta))) {
+        V(PV_B) = 0.;
+        if (b.i_end_r > it - 1) b.i_end_r = it - 1;
+        valid = false;
+    }
     if (valid && ((V(PV_U_p) <= 0) || (V(PV_rho2) <= 0))) valid = false;
     if (valid && ((V(PV_R) < 1) || (V(PV_beta) < P.fs.beta_min))) valid = false;
     if (valid) {
@@ -765,8 +876,10 @@ static void post_step(Run &run, BW &b, const double *Ycode, int it, double tb, d
     V(PV_mflags) = flags;
 }
 
+// ejecta: 0 for jet layers; 1 for ejecta blastwaves (bw_type = fs_dense), 2 for ejecta blastwaves
+// with method_spread set (the radial or the spreading form of dt_obs/dR)
 static void evolve_run(const pba_run_params *P, const pba_layer_init *init, int nbw, int nt, const double *tgrid,
-                       double *table, pba_bw_result *res) {
+                       double *table, pba_bw_result *res, int ejecta = 0) {
     Run run;
     run.P = P;
     EleGrid gfs, grs;
@@ -795,6 +908,8 @@ static void evolve_run(const pba_run_params *P, const pba_layer_init *init, int 
             g.build(P->rs.gam2, init[i].ngam_rs);
             bw.rs_sh = Shock{&P->rs, g.e[0], g.e[init[i].ngam_rs - 1]};
         }
+        bw.dense = ejecta != 0;
+        bw.spread = ejecta != 1;
         bw_init(run, run.bws[i], init[i], tgrid[0], &Y[(size_t)i * NEQ]);
         run.bws[i].i_end_r = nt;
         std::fill(table + (size_t)i * PBA_NV * nt, table + (size_t)(i + 1) * PBA_NV * nt, 0.);
@@ -888,5 +1003,22 @@ extern "C" int orc_dyn_evolve(const pba_run_params *params, int n_runs, int bw_p
     for (int r = 0; r < n_runs; r++)
         orc::evolve_run(&params[r], init + (size_t)r * bw_per_run, bw_per_run, nt, tgrid,
                         table + (size_t)r * bw_per_run * PBA_NV * nt, res + (size_t)r * bw_per_run);
+    return PBAGPU_OK;
+}
+
+// The ejecta of a kilonova run: every blastwave evolved as BlastWave::rhs_fs_dense in a standard
+// ISM (model_evolve.h:1627-1651), ordered as the reference's state vector, layer-major
+// (for each layer, its velocity shells; model_evolve.h:225-234).  spread: method_spread != None.
+extern "C" int orc_dyn_evolve_ejecta(const pba_run_params *params, int n_runs, int bw_per_run,
+                                     const pba_layer_init *init, int spread, int nt, const double *tgrid,
+                                     double *table, pba_bw_result *res) {
+    if (!params || !init || !tgrid || !table || !res || n_runs < 1 || bw_per_run < 1 || nt < 2) return PBAGPU_E_ARG;
+    for (int r = 0; r < n_runs; r++)
+        if (params[r].fsrs) return PBAGPU_E_ARG;   // ejecta blastwaves have no reverse shock
+#pragma omp parallel for schedule(dynamic)
+    for (int r = 0; r < n_runs; r++)
+        orc::evolve_run(&params[r], init + (size_t)r * bw_per_run, bw_per_run, nt, tgrid,
+                        table + (size_t)r * bw_per_run * PBA_NV * nt, res + (size_t)r * bw_per_run,
+                        spread ? 2 : 1);
     return PBAGPU_OK;
 }

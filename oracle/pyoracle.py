@@ -20,6 +20,8 @@ def lib():
         ip = C.POINTER(C.c_int)
         _lib.orc_dyn_evolve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.c_int, C.POINTER(abi.LayerInit),
                                         C.c_int, dp, dp, C.POINTER(abi.BwResult)]
+        _lib.orc_dyn_evolve_ejecta.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.c_int, C.POINTER(abi.LayerInit),
+                                               C.c_int, C.c_int, dp, dp, C.POINTER(abi.BwResult)]
         if hasattr(_lib, "orc_ele_evolve"):
             _lib.orc_ele_evolve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.Population), C.c_int,
                                             dp, C.c_int, C.c_int, dp, dp, dp, ip]
@@ -56,6 +58,23 @@ def dyn_evolve(params, inits, tgrid, n_runs=1):
     rc = lib().orc_dyn_evolve(params, n_runs, bpr, inits, nt, _dp(tg), _dp(table), res)
     if rc != 0:
         raise RuntimeError(f"orc_dyn_evolve rc={rc}")
+    return table, res
+
+
+def dyn_evolve_ejecta(params, inits, tgrid, spread=False, n_runs=1):
+    """Ejecta blastwaves of a kilonova run (rhs_fs_dense in a standard ISM); inits in the
+    reference's state-vector order, layer-major."""
+    if isinstance(params, abi.RunParams):
+        params = (abi.RunParams * 1)(params)
+    nbw_total = len(inits)
+    bpr = nbw_total // n_runs
+    nt = len(tgrid)
+    tg = np.ascontiguousarray(tgrid, dtype=np.float64)
+    table = np.zeros((nbw_total, abi.PBA_NV, nt))
+    res = (abi.BwResult * nbw_total)()
+    rc = lib().orc_dyn_evolve_ejecta(params, n_runs, bpr, inits, 1 if spread else 0, nt, _dp(tg), _dp(table), res)
+    if rc != 0:
+        raise RuntimeError(f"orc_dyn_evolve_ejecta rc={rc}")
     return table, res
 
 

@@ -165,11 +165,21 @@ def run_params_from_parfile(path):
     """(RunParams, main_pars, main_opts, grb_pars, grb_opts) from a parfile with a GRB section."""
     mp_, mo = inputs.read_parfile(path, inputs.SEC_MAIN)
     gp, go = inputs.read_parfile(path, inputs.SEC_GRB)
+    kp, ko = inputs.read_parfile(path, inputs.SEC_KN)
+    if ko.get("run_bws", "no") == "yes":
+        # the ejecta's right-hand side (rhs_fs_dense) is not in the device's dynamics stage
+        raise NotImplementedError(f"{path}: kilonova ejecta (kN section, run_bws = yes) are not evolved by this "
+                                  "path; only the jet (GRB section) is")
+    if go.get("run_bws", "no") != "yes":
+        raise ValueError(f"{path}: no jet to evolve (GRB section missing or run_bws = no)")
     return run_params(mp_, mo, gp, go), mp_, mo, gp, go
 
 
-def run_params(mp_, mo, gp, go):
-    for key, want in SUPPORTED_OPTS.items():
+def run_params(mp_, mo, gp, go, opts=None):
+    """RunParams of the main section and one afterglow section; `opts` is the table of options the
+    path implements (SUPPORTED_OPTS for a jet, EJECTA_OPTS for kilonova ejecta)."""
+    ejecta = opts is EJECTA_OPTS
+    for key, want in (opts or SUPPORTED_OPTS).items():
         allowed = want if isinstance(want, tuple) else (want,)
         have = mo.get(key, go.get(key, allowed[0]))
         if have not in allowed:
@@ -201,7 +211,7 @@ def run_params(mp_, mo, gp, go):
     P.nmax_theta = int(gp.get("nmax_theta", 1000))
     P.nmax_phi = int(gp.get("nmax_phi", 1000))
     bw_type = go.get("bw_type", "fs")
-    if bw_type not in ("fs", "fsrs"):
+    if bw_type not in (("fs_dense",) if ejecta else ("fs", "fsrs")):
         raise NotImplementedError(f"bw_type={bw_type}")
     P.fsrs = 1 if bw_type == "fsrs" else 0
     P.do_rs = _yes(go.get("do_rs", "no")) if P.fsrs else 0
@@ -253,6 +263,54 @@ def layer_inits(idd, method="adaptive"):
         L.ilayer = i
         L.nlayers = n
     return arr
+
+
+# Options of the kilonova ejecta (parfile kN section) that the ejecta path implements: every
+# ejecta blastwave goes through setStandardISM + rhs_fs_dense (src/model_evolve.h:1627-1651),
+# bw_type = fs_dense stores the upstream's state (src/blastwave/blastwave.h:544-560).  method_spread
+# only selects the form of dt_obs/dR there (dtheta/dR = 0 in rhs_fs_dense).
+EJECTA_OPTS = dict(
+    method_eats=EATS_METHODS, method_quad="CADRE", method_comp_mode="comovSpec", method_spread=("None", "our"),
+    method_dgdr="our", method_eos="Nava13", method_dmdr="usingdthdr", method_collision="none",
+    bw_type="fs_dense", do_nucinj="no", do_eninj_inside_rhs="no", use_1d_id="no", load_r0="no",
+    integrator=INTEGRATORS, do_average_solution="no",
+)
+
+
+def run_params_ejecta(mp_, mo, kp, ko):
+    """(RunParams, spread) of the ejecta of a kilonova run: parfile main and kN sections."""
+    P = run_params(mp_, mo, kp, ko, opts=EJECTA_OPTS)
+    return P, ko.get("method_spread", "None") != "None"
+
+
+def ejecta_inits(id2d, method="adaptive"):
+    """LayerInit array of the blastwaves of a 2-D [shells x layers] initial-data dict (r, mom,
+    theta, ctheta, ek, mass), in the reference's state-vector order: layer-major, the velocity
+    shells of layer 0 first (src/model_evolve.h:225-234).  Returns (inits, order) with order the
+    (shell, layer) of each entry.
+
+    Per shell the loader builds the layers' angular grids from theta_wing, the last layer's theta
+    of the last shell, as for a jet (EjectaID2::_load_id_file, src/ejecta/ejecta_id.h:221-300).
+    A blastwave with ek, mass or r not above 0, or beta0 <= 1e-6, is not evolved
+    (setEjectaBwPars, src/ejecta/ejecta_base.h:255-265): its E0 = M0 = 0 here."""
+    ek = np.atleast_2d(np.asarray(id2d["ek"], dtype=np.float64))
+    ns, nl = ek.shape
+    mass = np.atleast_2d(np.asarray(id2d["mass"], dtype=np.float64))
+    mom = np.atleast_2d(np.asarray(id2d["mom"], dtype=np.float64))
+    theta = np.atleast_2d(np.asarray(id2d["theta"], dtype=np.float64))
+    tw = float(theta[ns - 1, nl - 1])
+    out = (LayerInit * (ns * nl))()
+    order = []
+    for il in range(nl):
+        for ish in range(ns):
+            row = layer_inits(dict(ek=ek[ish], mass=mass[ish], mom=mom[ish], theta_core=tw, theta_wing=tw), method)
+            L = out[len(order)]
+            C.memmove(C.addressof(L), C.addressof(row[il]), C.sizeof(LayerInit))
+            beta0 = mom[ish, il] / math.sqrt(mom[ish, il] * mom[ish, il] + 1.0)
+            if not (ek[ish, il] > 0 and mass[ish, il] > 0 and beta0 > 1e-6):
+                L.E0 = L.M0 = 0.0
+            order.append((ish, il))
+    return out, order
 
 
 def cells_in_layer(i):

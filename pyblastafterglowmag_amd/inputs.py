@@ -87,13 +87,17 @@ def _section(lines, sep1, default, new):
     return d
 
 
-def write_parfile(workdir, main=None, grb=None, fname="parfile.par"):
-    """Write a parfile with the main section and (optionally) the GRB section."""
+def write_parfile(workdir, main=None, grb=None, fname="parfile.par", kn=None):
+    """Write a parfile with the main section and (optionally) the GRB and kN sections.  The kN
+    section takes the GRB section's key set (the reference package writes both from one
+    default table, package/src/PyBlastAfterglowMag/parfile_tools.py:361-426)."""
     os.makedirs(workdir, exist_ok=True)
     lines = []
     _section(lines, SEC_MAIN, MAIN_DEFAULTS, main)
     if grb is not None:
         _section(lines, SEC_GRB, GRB_DEFAULTS, grb)
+    if kn is not None:
+        _section(lines, SEC_KN, GRB_DEFAULTS, kn)
     with open(os.path.join(workdir, fname), "w") as f:
         for line in lines:
             f.write(line + "\n")
@@ -242,3 +246,40 @@ def write_run(workdir, main, grb, struct):
     h5io.write_id(os.path.join(workdir, fname), idd["r"], idd["mom"], idd["theta"], idd["ctheta"],
                   idd["ek"], idd["mass"], idd["theta_core"], idd["theta_wing"])
     return write_parfile(workdir, main, grb)
+
+
+def ejecta_structure(n_shells, n_layers, mom_min, mom_max, ek_min, ek_max, seed, theta_wing=np.pi / 2.0):
+    """Initial data of kilonova-like ejecta: [n_shells x n_layers] arrays of r, mom, theta, ctheta,
+    ek, mass, in the layout EjectaID2::_load_id_file reads with use_1d_id = no
+    (src/ejecta/ejecta_id.h:221-273).
+
+    Shells have momenta beta*Gamma log-spaced over [mom_min, mom_max]; shell i carries a kinetic
+    energy E_i drawn log-uniform over [ek_min, ek_max] (numpy default_rng(seed)).  The angular
+    layers are those of an adaptive jet of half-opening theta_wing (layer j spans the cone up to
+    (j + 1) theta_wing / n_layers): as for the jet's layers, a layer holds the energy of its
+    whole cone, E_i (1 - cos theta_h) / 2, and mass E / ((Gamma - 1) c^2).  theta is each layer's
+    upper edge (the loader takes theta_wing from the last one), ctheta its centre; r = 0 (the
+    loader places every shell at beta c t0, load_r0 = no)."""
+    mom = np.logspace(np.log10(mom_min), np.log10(mom_max), n_shells)
+    e_shell = 10.0 ** np.random.default_rng(seed).uniform(np.log10(ek_min), np.log10(ek_max), n_shells)
+    th_h = (np.arange(n_layers) + 1.0) * theta_wing / n_layers
+    th_c = th_h - 0.5 * theta_wing / n_layers
+    gam = np.sqrt(mom * mom + 1.0)
+    ek = e_shell[:, None] * ((1.0 - np.cos(th_h)) / 2.0)[None, :]
+    mass = ek / ((gam[:, None] - 1.0) * C_LIGHT * C_LIGHT)
+    return dict(r=np.zeros_like(ek), mom=np.repeat(mom[:, None], n_layers, axis=1),
+                theta=np.tile(th_h, (n_shells, 1)), ctheta=np.tile(th_c, (n_shells, 1)), ek=ek, mass=mass)
+
+
+def write_kn_run(workdir, main, kn, struct):
+    """Write parfile.par (main + kN sections, no jet) and the ejecta's 2-D initial-data file for a
+    kilonova run; returns the parfile path.  struct: keyword arguments of ejecta_structure."""
+    from . import h5io
+    os.makedirs(workdir, exist_ok=True)
+    idd = ejecta_structure(**struct)
+    fname = kn.get("fname_ejecta_id", "id_kn.h5")
+    kn = dict(kn, fname_ejecta_id=fname, use_1d_id="no")
+    tw = float(idd["theta"][-1, -1])
+    h5io.write_id(os.path.join(workdir, fname), idd["r"], idd["mom"], idd["theta"], idd["ctheta"], idd["ek"],
+                  idd["mass"], tw, tw, one_d=False)
+    return write_parfile(workdir, main, None, kn=kn)

@@ -5,7 +5,9 @@
 * ``gauss21_fsrs_ssc``   : 21-layer Gaussian jet, forward + reverse shock, synchrotron + SSC,
                            theta_obs = 0.363 rad, same grids.
 * ``tophat64_pw``        : the top-hat jet on a 64-layer piece-wise grid (4096 cells).
-Small variants (fewer steps / bins / pairs) are derived with ``scaled``.
+* ``kn_c5``              : kilonova-like ejecta, 32 velocity shells x 30 polar layers (960
+                           blastwaves), synchrotron + SSC, 128 gamma bins, 128 comoving freqs.
+Small variants (fewer steps / bins / pairs) are derived with ``scaled`` (``kn_scaled`` for ejecta).
 """
 import copy
 
@@ -234,3 +236,67 @@ def small(name):
     cfg = dict(SMALL[name])
     base = cfg.pop("base")
     return scaled(base, **cfg)
+
+
+# Kilonova-like ejecta: velocity shells with beta*Gamma log-spaced over KN_MOM and per-shell kinetic
+# energies drawn log-uniform over KN_EK (ejecta_structure, seeded)
+KN_MOM = (0.1, 3.0)
+KN_EK = (1e48, 1e51)
+KN_SEED = 20261203
+
+
+def kn_c5():
+    """Kilonova-like ejecta without a jet: 32 velocity shells (beta*Gamma 0.1..3) x 30 polar layers
+    (adaptive, up to pi/2) = 960 blastwaves in one ODE system, every one evolved by the
+    reference's ejecta right-hand side (bw_type = fs_dense) in a uniform ISM of 1e-2 cm^-3;
+    eps_e = 0.1, eps_b = 1e-2, p = 2.5, synchrotron + SSC on 128 Lorentz-factor bins and 128
+    comoving frequencies, 1000 steps, 96 nu x 128 t_obs from 1e4 s to 1e10 s.  Returns (main,
+    kn, struct) for inputs.write_kn_run."""
+    main = dict(tb0=1e3, tb1=1e14, ntb=1000, n_ism=1e-2, d_l=1.27e26, z=0.0099, theta_obs=0.0,
+                lc_freqs="array logspace 1e8 1e29 96", lc_times="array logspace 1e4 1e10 128")
+    kn = dict(eps_e_fs=0.1, eps_b_fs=1e-2, p_fs=2.5, epsilon_e_rad_fs=0., ngam_fs=128, gam1_fs=1., gam2_fs=1e8,
+              freq1_fs=1e5, freq2_fs=1e32, nfreq_fs=128, method_ssc_fs="numeric", method_ele_fs="numeric",
+              bw_type="fs_dense", do_rs="no", method_eats="adaptive", method_spread="None",
+              fname_ejecta_id="id_kn.h5", fname_dyn="dyn_kn.h5", fname_light_curve="lc_kn.h5")
+    struct = dict(n_shells=32, n_layers=30, mom_min=KN_MOM[0], mom_max=KN_MOM[1], ek_min=KN_EK[0], ek_max=KN_EK[1],
+                  seed=KN_SEED)
+    return main, kn, struct
+
+
+KN_WORKLOADS = {"kn_c5": kn_c5}
+
+
+def kn_scaled(name, n_shells=None, n_layers=None, ntb=None, ngam=None, nfreq=None, n_nu_obs=None, n_t_obs=None,
+              **kn_over):
+    """A smaller variant of a kilonova workload (fewer blastwaves, steps, bins or pairs; physics
+    unchanged).  Other keywords override kN-section keys."""
+    main, kn, struct = (copy.deepcopy(x) for x in KN_WORKLOADS[name]())
+    if n_shells:
+        struct["n_shells"] = n_shells
+    if n_layers:
+        struct["n_layers"] = n_layers
+    if ntb:
+        main["ntb"] = ntb
+    if ngam:
+        kn["ngam_fs"] = ngam
+    if nfreq:
+        kn["nfreq_fs"] = nfreq
+    if n_nu_obs:
+        main["lc_freqs"] = f"array logspace 1e8 1e29 {n_nu_obs}"
+    if n_t_obs:
+        main["lc_times"] = f"array logspace 1e4 1e10 {n_t_obs}"
+    kn.update(kn_over)
+    return main, kn, struct
+
+
+KN_SMALL = {
+    # 4 velocity shells x 3 layers (12 blastwaves) at reduced grids, small enough for the CPU oracle
+    "kn_small": dict(n_shells=4, n_layers=3, ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8),
+    # the same ejecta with the observer integral over piece-wise cells (2i+1 per layer)
+    "kn_pw_small": dict(n_shells=4, n_layers=3, ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
+                        method_eats="piece-wise"),
+}
+
+
+def kn_small(name):
+    return kn_scaled("kn_c5", **KN_SMALL[name])

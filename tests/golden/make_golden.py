@@ -163,6 +163,65 @@ def run_reference(name, wd, spec_every=1):
     return out
 
 
+# kilonova ejecta (parfile kN section, no jet): every (shell, layer) blastwave's dynamics and light
+# curve; KN_DYN_ONLY runs the dynamics alone (no electrons or light curves) and keeps the
+# blastwaves of KN_DYN_KEEP, every KN_DYN_EVERY-th step
+KN_CASES = {"kn_small", "kn_pw_small", "kn_c5_dyn"}
+KN_DYN_ONLY = {"kn_c5_dyn"}
+KN_DYN_KEEP = {"kn_c5_dyn": ((0, 10, 21, 31), (0, 14, 29))}
+KN_DYN_EVERY = {"kn_c5_dyn": 10}
+KN_DYN_VARS = DYN_VARS + ["GammaRel", "rho", "ctheta", "Erad2", "Esh2", "Ead2"]
+KN_DYN_VARS_ONLY = ["R", "Gamma", "beta", "theta", "M2", "Eint2", "tt", "tcomov", "rho2", "GammaFsh", "thickness",
+                    "U_p", "GammaRel", "rho", "Esh2", "Ead2"]
+
+
+def build_kn_case(name):
+    if name == "kn_c5_dyn":
+        main, kn, struct = workloads.kn_c5()
+        return main, dict(kn, do_mphys_in_situ="no", do_mphys_in_ppr="no", do_lc="no"), struct
+    return workloads.kn_small(name)
+
+
+def run_reference_kn(name, wd):
+    main, kn, struct = build_kn_case(name)
+    kn = dict(kn, save_dynamics="yes", save_spec="no")
+    inputs.write_kn_run(wd, main, kn, struct)
+    subprocess.check_call([PBA, wd + "/", "parfile.par", "err"], stdout=subprocess.DEVNULL)
+    out = {}
+    with open(os.path.join(wd, "parfile.par")) as f:
+        out["parfile"] = np.frombuffer(f.read().encode(), dtype=np.uint8)
+    idf = os.path.join(wd, kn["fname_ejecta_id"])
+    dyn = os.path.join(wd, kn["fname_dyn"])
+    ns, nl = int(h5io.attr(dyn, "nshells")), int(h5io.attr(dyn, "nlayers"))
+    for k in ("r", "mom", "theta", "ctheta", "ek", "mass"):
+        out["id_" + k] = h5io.read(idf, k).reshape(ns, nl)
+    shells, layers = KN_DYN_KEEP.get(name, (tuple(range(ns)), tuple(range(nl))))
+    names = KN_DYN_VARS_ONLY if name in KN_DYN_ONLY else KN_DYN_VARS
+    every = KN_DYN_EVERY.get(name, 1)
+    nt = int(main["ntb"])
+    D = np.zeros((len(shells), len(layers), len(names), nt))
+    for i, ish in enumerate(shells):
+        for j, il in enumerate(layers):
+            for iv, v in enumerate(names):
+                D[i, j, iv] = h5io.read(dyn, f"shell={ish} layer={il}/{v}")
+    out["dyn"] = np.ascontiguousarray(D[..., ::every])
+    out["dyn_every"] = np.array(every)
+    out["dyn_shells"] = np.array(shells)
+    out["dyn_layers"] = np.array(layers)
+    out["dyn_names"] = np.array(names)
+    out["dyn_layer_members"] = np.array(h5io.members(dyn, "shell=0 layer=0"))
+    out["dyn_root_counts"] = np.array([h5io.attr(dyn, k) for k in ("nshells", "nlayers", "ntimes", "ncells")])
+    if name in KN_DYN_ONLY:
+        return out
+    lc = os.path.join(wd, kn["fname_light_curve"])
+    out["times"] = h5io.read(lc, "times")
+    out["freqs"] = h5io.read(lc, "freqs")
+    out["lc"] = np.stack([np.stack([h5io.read(lc, f"shell={ish} layer={il}/fluxdens") for il in range(nl)])
+                          for ish in range(ns)])
+    out["lc_root_members"] = np.array(h5io.members(lc, "/"))
+    return out
+
+
 def _store_sky(out, wd):
     """The raw sky maps raw_skymap_<n>.h5 of a run, every dataset and attribute."""
     n = 0
@@ -185,7 +244,10 @@ def main(names):
         wd = tempfile.mkdtemp(prefix=f"golden_{name}_")
         try:
             every = {"tophat_small": 1, "gauss5_small": 5}.get(name, 10)
-            out = run_reference(name, wd, spec_every=every)
+            if name in KN_CASES:
+                out = run_reference_kn(name, wd)
+            else:
+                out = run_reference(name, wd, spec_every=every)
             np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
             print(name, {k: v.shape for k, v in out.items()})
         finally:
@@ -197,7 +259,7 @@ ALL = ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gaus
        "tophat64_pw", "gauss_pw_ng32", "gauss_pw_ng48",
        "tophat_ssa_small", "gauss5_ssa_small", "tophat_dermer_small", "tophat_bessel_small", "tophat_gsl_small",
        "tophat_rk4_small", "tophat_wind_small", "gauss21_fsrs_ssc", "gauss64_pw_small", "tophat_ebl_small",
-       "lhs_small_0", "lhs_small_42", "tophat_f2t_small"]
+       "lhs_small_0", "lhs_small_42", "tophat_f2t_small", "kn_small", "kn_pw_small", "kn_c5_dyn"]
 
 if __name__ == "__main__":
     main(sys.argv[1:] or ALL)

@@ -1,0 +1,189 @@
+"""Kilonova ejecta (the parfile's kN section, no jet) on the CPU oracle against the reference
+program (tests/golden/kn_*.npz written by tests/golden/make_golden.py from oracle/_ref/pba.out).
+
+Every ejecta blastwave is evolved by the reference's ejecta right-hand side (setStandardISM +
+rhs_fs_dense, src/model_evolve.h:1627-1651) in one ODE system with a shared adaptive step; the
+electrons and the observer integral are those of the jet's layers.  The oracle reproduces the
+reference bit for bit: every stored dynamical quantity of every (shell, layer) blastwave, and
+the light curve of each.  The host-side pieces (initial-data writer, the state-vector order,
+option checks) are tested against what the reference's loader reads.
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, rel_err
+import pyoracle
+from pyblastafterglowmag_amd import abi, h5io, inputs, workloads
+
+
+def kn_golden_run(g):
+    """(params, spread, inits, order, tgrid, tobs, nuobs, method) rebuilt from a kilonova fixture's
+    stored parfile and 2-D initial data."""
+    text = bytes(g["parfile"]).decode()
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "parfile.par")
+        with open(p, "w") as f:
+            f.write(text)
+        mp_, mo = inputs.read_parfile(p, inputs.SEC_MAIN)
+        kp, ko = inputs.read_parfile(p, inputs.SEC_KN)
+        assert inputs.read_parfile(p, inputs.SEC_GRB) == ({}, {})   # no jet in the run
+    P, spread = abi.run_params_ejecta(mp_, mo, kp, ko)
+    idd = {k: g["id_" + k] for k in ("r", "mom", "theta", "ctheta", "ek", "mass")}
+    method = ko.get("method_eats", "adaptive")
+    inits, order = abi.ejecta_inits(idd, method)
+    tobs, nuobs = abi.obs_pairs(mo) if "lc_times" in mo else (None, None)
+    return P, spread, inits, order, abi.tgrid(mp_), tobs, nuobs, method
+
+
+def _observer_layers(inits, res, method):
+    n = len(inits)
+    if method == "piece-wise":
+        return (abi.EatsPwLayer * n)(*[abi.EatsPwLayer(inits[k].ctheta0, inits[k].ncells,
+                                                      abi.cells_in_layer(inits[k].ilayer), k, 0, k, 0,
+                                                      res[k].i_end_r) for k in range(n)])
+    return (abi.EatsLayer * n)(*[abi.EatsLayer(inits[k].theta_c_l, inits[k].theta_c_h, k, 0, k, -1, res[k].i_end_r)
+                                 for k in range(n)])
+
+
+@pytest.mark.parametrize("name", ["kn_small", "kn_pw_small"])
+def test_oracle_ejecta_matches_reference(name):
+    """4 velocity shells x 3 layers (12 blastwaves, 300 steps, SSC on): dynamics of every
+    blastwave and the light curve of every (shell, layer) equal the reference's bit for bit."""
+    g = load_golden(name)
+    P, spread, inits, order, tgrid, tobs, nuobs, method = kn_golden_run(g)
+    assert not spread and P.fs.ssc == 1 and method == ("piece-wise" if "pw" in name else "adaptive")
+    np.testing.assert_array_equal(tobs, g["times"])
+    np.testing.assert_array_equal(nuobs, g["freqs"])
+    table, res = pyoracle.dyn_evolve_ejecta(P, inits, tgrid, spread=spread)
+    assert all(r.status == 0 for r in res)
+    names = [str(n) for n in g["dyn_names"]]
+    for k, (ish, il) in enumerate(order):
+        for iv, n in enumerate(names):
+            np.testing.assert_array_equal(table[k, abi.PV[n]], g["dyn"][ish, il, iv], err_msg=f"{name} {ish} {il} {n}")
+    pops = (abi.Population * len(order))(*[abi.Population(k, 0, 0, 0) for k in range(len(order))])
+    jall, _, st, aall = pyoracle.ele_evolve(P, pops, table, want_a=True)
+    assert not st.any()
+    eats = pyoracle.eats_lightcurve_pw if method == "piece-wise" else pyoracle.eats_lightcurve
+    lc, st2 = eats(P, _observer_layers(inits, res, method), table, jall, tobs, nuobs, aall=aall)
+    assert not st2.any()
+    for k, (ish, il) in enumerate(order):
+        assert rel_err(lc[k], g["lc"][ish, il], 1e-30) == 0.0, (name, ish, il)
+    # the summed light curve is dominated by the fast shells and is not zero
+    assert np.all(g["lc"].sum(axis=(0, 1)) > 0)
+
+
+def test_oracle_full_size_ejecta_dynamics_match_reference():
+    """The full C5 ejecta: 32 shells x 30 layers = 960 blastwaves in one ODE system, 1000 output
+    steps (about 11,000 shared adaptive steps).  The reference ran the dynamics alone; the
+    oracle's dynamics of 12 blastwaves spread over the shells and layers (every 10th step, the
+    fixture's subset) equal its output bit for bit.  About 16 s of one core."""
+    g = load_golden("kn_c5_dyn")
+    P, spread, inits, order, tgrid, *_ = kn_golden_run(g)
+    assert len(inits) == 960 and len(tgrid) == 1000 and P.fs.ngam == 128 and P.fs.nfreq == 128
+    table, res = pyoracle.dyn_evolve_ejecta(P, inits, tgrid, spread=spread)
+    assert all(r.status == 0 for r in res) and res[0].n_steps > 10000
+    names = [str(n) for n in g["dyn_names"]]
+    every = int(g["dyn_every"])
+    for i, ish in enumerate(g["dyn_shells"]):
+        for j, il in enumerate(g["dyn_layers"]):
+            k = order.index((int(ish), int(il)))
+            for iv, n in enumerate(names):
+                np.testing.assert_array_equal(table[k, abi.PV[n], ::every], g["dyn"][i, j, iv],
+                                              err_msg=f"shell {ish} layer {il} {n}")
+
+
+def test_ejecta_state_order_is_layer_major():
+    """The reference's state vector holds, for each layer, its velocity shells in order
+    (src/model_evolve.h:225-234); the shared step's error norm sums in that order."""
+    idd = inputs.ejecta_structure(3, 2, 0.1, 3.0, 1e48, 1e51, seed=1)
+    inits, order = abi.ejecta_inits(idd)
+    assert order == [(0, 0), (1, 0), (2, 0), (0, 1), (1, 1), (2, 1)]
+    for k, (ish, il) in enumerate(order):
+        assert inits[k].E0 == idd["ek"][ish, il] and inits[k].M0 == idd["mass"][ish, il]
+        assert inits[k].mom0 == idd["mom"][ish, il] and inits[k].ilayer == il and inits[k].nlayers == 2
+        assert inits[k].ncells == 1.0
+    # piece-wise: energies and masses per cell, every blastwave starts at theta_wing
+    pw, _ = abi.ejecta_inits(idd, "piece-wise")
+    assert pw[3].E0 == idd["ek"][0, 1] / 3 and pw[3].theta_b0 == pytest.approx(np.pi / 2) and pw[3].ncells == 4.0
+    # a blastwave without mass, energy or velocity is not evolved (its E0 = M0 = 0)
+    idd["ek"][2, 1] = 0.0
+    idd["mom"][1, 0] = 1e-7
+    z, _ = abi.ejecta_inits(idd)
+    assert z[5].E0 == z[5].M0 == 0.0 and z[1].E0 == z[1].M0 == 0.0 and z[0].E0 > 0
+
+
+def test_ejecta_structure_layout():
+    """The C5 ejecta: 32 shells with beta*Gamma log-spaced over [0.1, 3], each with one energy drawn
+    log-uniform over [1e48, 1e51]; layer j holds the energy of the cone up to theta_h, E (1 - cos
+    theta_h) / 2, so the cone to pi/2 holds half of it; mass E / ((Gamma - 1) c^2)."""
+    main, kn, struct = workloads.kn_c5()
+    d = inputs.ejecta_structure(**struct)
+    assert d["ek"].shape == (32, 30)
+    mom = d["mom"][:, 0]
+    assert mom[0] == pytest.approx(0.1) and mom[-1] == pytest.approx(3.0)
+    np.testing.assert_allclose(np.diff(np.log(mom)), np.log(30.0) / 31)
+    e_shell = 2.0 * d["ek"][:, -1]
+    assert np.all((e_shell >= 1e48) & (e_shell <= 1e51))
+    assert len(np.unique(np.round(np.log10(e_shell), 6))) == 32   # a draw per shell
+    gam = np.sqrt(mom * mom + 1)
+    np.testing.assert_allclose(d["mass"] * (gam[:, None] - 1) * inputs.C_LIGHT ** 2, d["ek"], rtol=1e-14)
+    np.testing.assert_allclose(d["theta"][0], np.arange(1, 31) * (np.pi / 2) / 30, rtol=1e-15)
+    assert np.all(np.diff(d["ek"], axis=1) > 0)
+    # the same seed gives the same ejecta
+    np.testing.assert_array_equal(inputs.ejecta_structure(**struct)["ek"], d["ek"])
+
+
+def test_kn_run_files_read_back(tmp_path):
+    """write_kn_run: a parfile with the main and kN sections only, and the 2-D initial data in
+    the use_1d_id = no layout the reference's loader reads (datasets [nshells][nlayers])."""
+    main, kn, struct = workloads.kn_small("kn_small")
+    wd = str(tmp_path) + "/"
+    inputs.write_kn_run(wd, main, kn, struct)
+    kp, ko = inputs.read_parfile(wd + "parfile.par", inputs.SEC_KN)
+    assert ko["bw_type"] == "fs_dense" and ko["use_1d_id"] == "no" and ko["method_spread"] == "None"
+    assert inputs.read_parfile(wd + "parfile.par", inputs.SEC_GRB) == ({}, {})
+    d = inputs.ejecta_structure(**struct)
+    for k in ("mom", "ek", "mass", "theta", "ctheta", "r"):
+        np.testing.assert_array_equal(h5io.read(wd + ko["fname_ejecta_id"], k).reshape(4, 3), d[k])
+    # the fixture was written by this writer (with the dynamics file on)
+    g = load_golden("kn_small")
+    np.testing.assert_array_equal(g["id_ek"], d["ek"])
+    inputs.write_kn_run(wd, main, dict(kn, save_dynamics="yes", save_spec="no"), struct)
+    assert bytes(g["parfile"]).decode() == open(wd + "parfile.par").read()
+
+
+@pytest.mark.parametrize("key,value", [("bw_type", "fs"), ("method_dgdr", "peer"), ("method_collision", "simple"),
+                                       ("do_nucinj", "yes"), ("use_1d_id", "yes"), ("method_eos", "Peer12")])
+def test_ejecta_options_outside_the_path_are_refused(tmp_path, key, value):
+    main, kn, struct = workloads.kn_small("kn_small")
+    wd = str(tmp_path) + "/"
+    inputs.write_kn_run(wd, main, kn, struct)
+    mp_, mo = inputs.read_parfile(wd + "parfile.par", inputs.SEC_MAIN)
+    kp, ko = inputs.read_parfile(wd + "parfile.par", inputs.SEC_KN)
+    abi.run_params_ejecta(mp_, mo, kp, ko)
+    with pytest.raises(NotImplementedError):
+        abi.run_params_ejecta(mp_, mo, kp, dict(ko, **{key: value}))
+
+
+def test_ejecta_entry_refuses_a_reverse_shock():
+    g = load_golden("kn_small")
+    P, spread, inits, order, tgrid, *_ = kn_golden_run(g)
+    P.fsrs = 1
+    with pytest.raises(RuntimeError):
+        pyoracle.dyn_evolve_ejecta(P, inits, tgrid)
+
+
+def test_jet_path_refuses_a_kilonova_parfile(tmp_path):
+    """The device pipeline evolves jets only: a parfile whose kN section runs ejecta is refused
+    with a clear error, not read as a jet."""
+    main, kn, struct = workloads.kn_small("kn_small")
+    wd = str(tmp_path) + "/"
+    inputs.write_kn_run(wd, main, kn, struct)
+    with pytest.raises(NotImplementedError, match="kilonova"):
+        abi.run_params_from_parfile(wd + "parfile.par")
+    inputs.write_kn_run(wd, main, dict(kn, run_bws="no"), struct)
+    with pytest.raises(ValueError, match="no jet"):
+        abi.run_params_from_parfile(wd + "parfile.par")
