@@ -267,10 +267,12 @@ KN_WORKLOADS = {"kn_c5": kn_c5}
 
 
 def kn_scaled(name, n_shells=None, n_layers=None, ntb=None, ngam=None, nfreq=None, n_nu_obs=None, n_t_obs=None,
-              **kn_over):
+              main=None, **kn_over):
     """A smaller variant of a kilonova workload (fewer blastwaves, steps, bins or pairs; physics
-    unchanged).  Other keywords override kN-section keys."""
+    unchanged).  `main` overrides main-section keys, other keywords kN-section keys."""
+    main_over = main or {}
     main, kn, struct = (copy.deepcopy(x) for x in KN_WORKLOADS[name]())
+    main.update(main_over)
     if n_shells:
         struct["n_shells"] = n_shells
     if n_layers:
@@ -295,6 +297,14 @@ KN_SMALL = {
     # the same ejecta with the observer integral over piece-wise cells (2i+1 per layer)
     "kn_pw_small": dict(n_shells=4, n_layers=3, ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
                         method_eats="piece-wise"),
+    # method_spread set: the ejecta RHS keeps dtheta/dR = 0 but takes dt_obs/dR and the initial
+    # observer time in their spreading form (EQS::evalElapsedTime, init_elapsed_time)
+    "kn_spread_small": dict(n_shells=4, n_layers=3, ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
+                            method_spread="our"),
+    # a wind-like ambient medium (n_ism < 0: A0 R^-2 between r_ej and r_ism): the stored density is
+    # the one the last RHS call evaluated (set_standard_ism), not the start radius's
+    "kn_wind_small": dict(n_shells=4, n_layers=3, ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
+                          main=dict(n_ism=-1., A0=3e35, s=2., r_ej=1e16, r_ism=3e18)),
 }
 
 

@@ -166,11 +166,11 @@ def run_reference(name, wd, spec_every=1):
 # kilonova ejecta (parfile kN section, no jet): every (shell, layer) blastwave's dynamics and light
 # curve; KN_DYN_ONLY runs the dynamics alone (no electrons or light curves) and keeps the
 # blastwaves of KN_DYN_KEEP, every KN_DYN_EVERY-th step
-KN_CASES = {"kn_small", "kn_pw_small", "kn_c5_dyn"}
+KN_CASES = {"kn_small", "kn_pw_small", "kn_spread_small", "kn_wind_small", "kn_c5_dyn"}
 KN_DYN_ONLY = {"kn_c5_dyn"}
 KN_DYN_KEEP = {"kn_c5_dyn": ((0, 10, 21, 31), (0, 14, 29))}
 KN_DYN_EVERY = {"kn_c5_dyn": 10}
-KN_DYN_VARS = DYN_VARS + ["GammaRel", "rho", "ctheta", "Erad2", "Esh2", "Ead2"]
+KN_DYN_VARS = DYN_VARS + ["GammaRel", "rho", "drhodr", "ctheta", "Erad2", "Esh2", "Ead2"]
 KN_DYN_VARS_ONLY = ["R", "Gamma", "beta", "theta", "M2", "Eint2", "tt", "tcomov", "rho2", "GammaFsh", "thickness",
                     "U_p", "GammaRel", "rho", "Esh2", "Ead2"]
 
@@ -259,7 +259,8 @@ ALL = ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gaus
        "tophat64_pw", "gauss_pw_ng32", "gauss_pw_ng48",
        "tophat_ssa_small", "gauss5_ssa_small", "tophat_dermer_small", "tophat_bessel_small", "tophat_gsl_small",
        "tophat_rk4_small", "tophat_wind_small", "gauss21_fsrs_ssc", "gauss64_pw_small", "tophat_ebl_small",
-       "lhs_small_0", "lhs_small_42", "tophat_f2t_small", "kn_small", "kn_pw_small", "kn_c5_dyn"]
+       "lhs_small_0", "lhs_small_42", "tophat_f2t_small", "kn_small", "kn_pw_small", "kn_spread_small",
+       "kn_wind_small", "kn_c5_dyn"]
 
 if __name__ == "__main__":
     main(sys.argv[1:] or ALL)
