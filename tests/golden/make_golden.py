@@ -166,10 +166,10 @@ def run_reference(name, wd, spec_every=1):
 # kilonova ejecta (parfile kN section, no jet): every (shell, layer) blastwave's dynamics and light
 # curve; KN_DYN_ONLY runs the dynamics alone (no electrons or light curves) and keeps the
 # blastwaves of KN_DYN_KEEP, every KN_DYN_EVERY-th step
-KN_CASES = {"kn_small", "kn_pw_small", "kn_spread_small", "kn_wind_small", "kn_c5_dyn"}
+KN_CASES = {"kn_small", "kn_pw_small", "kn_spread_small", "kn_wind_small", "kn_c5_dyn", "kn_c5_lc"}
 KN_DYN_ONLY = {"kn_c5_dyn"}
-KN_DYN_KEEP = {"kn_c5_dyn": ((0, 10, 21, 31), (0, 14, 29))}
-KN_DYN_EVERY = {"kn_c5_dyn": 10}
+KN_DYN_KEEP = {"kn_c5_dyn": ((0, 10, 21, 31), (0, 14, 29)), "kn_c5_lc": ((0, 10, 21, 31), (0, 14, 29))}
+KN_DYN_EVERY = {"kn_c5_dyn": 10, "kn_c5_lc": 10}
 KN_DYN_VARS = DYN_VARS + ["GammaRel", "rho", "drhodr", "ctheta", "Erad2", "Esh2", "Ead2"]
 KN_DYN_VARS_ONLY = ["R", "Gamma", "beta", "theta", "M2", "Eint2", "tt", "tcomov", "rho2", "GammaFsh", "thickness",
                     "U_p", "GammaRel", "rho", "Esh2", "Ead2"]
@@ -179,6 +179,8 @@ def build_kn_case(name):
     if name == "kn_c5_dyn":
         main, kn, struct = workloads.kn_c5()
         return main, dict(kn, do_mphys_in_situ="no", do_mphys_in_ppr="no", do_lc="no"), struct
+    if name == "kn_c5_lc":   # the C5 ejecta at reduced microphysics grids, without SSC: 6 minutes
+        return workloads.kn_scaled("kn_c5", ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8, method_ssc_fs="none")
     return workloads.kn_small(name)
 
 
@@ -216,8 +218,8 @@ def run_reference_kn(name, wd):
     lc = os.path.join(wd, kn["fname_light_curve"])
     out["times"] = h5io.read(lc, "times")
     out["freqs"] = h5io.read(lc, "freqs")
-    out["lc"] = np.stack([np.stack([h5io.read(lc, f"shell={ish} layer={il}/fluxdens") for il in range(nl)])
-                          for ish in range(ns)])
+    out["lc"] = np.stack([np.stack([h5io.read(lc, f"shell={ish} layer={il}/fluxdens") for il in layers])
+                          for ish in shells])
     out["lc_root_members"] = np.array(h5io.members(lc, "/"))
     return out
 
@@ -260,7 +262,7 @@ ALL = ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gaus
        "tophat_ssa_small", "gauss5_ssa_small", "tophat_dermer_small", "tophat_bessel_small", "tophat_gsl_small",
        "tophat_rk4_small", "tophat_wind_small", "gauss21_fsrs_ssc", "gauss64_pw_small", "tophat_ebl_small",
        "lhs_small_0", "lhs_small_42", "tophat_f2t_small", "kn_small", "kn_pw_small", "kn_spread_small",
-       "kn_wind_small", "kn_c5_dyn"]
+       "kn_wind_small", "kn_c5_dyn", "kn_c5_lc"]
 
 if __name__ == "__main__":
     main(sys.argv[1:] or ALL)

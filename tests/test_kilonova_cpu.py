@@ -102,6 +102,39 @@ def test_oracle_full_size_ejecta_dynamics_match_reference():
                                               err_msg=f"shell {ish} layer {il} {n}")
 
 
+def test_oracle_full_geometry_ejecta_light_curves_match_reference():
+    """The C5 ejecta (960 blastwaves, 1000 steps) at reduced microphysics grids (64 gamma x 48 nu,
+    no SSC, 6 x 8 pairs): the reference computed every blastwave's electrons and light curve.  The
+    oracle evolves the dynamics of all 960 (they share one adaptive step) and the electrons and
+    light curves of the 12 blastwaves the fixture keeps (the ejecta RHS has no electron term, so
+    the others do not act on them): dynamics, limits and light curves equal the reference's bit
+    for bit.  About 20 s of one core."""
+    g = load_golden("kn_c5_lc")
+    P, spread, inits, order, tgrid, tobs, nuobs, method = kn_golden_run(g)
+    assert len(inits) == 960 and P.fs.ssc == 0 and P.fs.ngam == 64
+    table, res = pyoracle.dyn_evolve_ejecta(P, inits, tgrid, spread=spread)
+    assert all(r.status == 0 for r in res)
+    names = [str(n) for n in g["dyn_names"]]
+    every = int(g["dyn_every"])
+    keep = [(int(ish), int(il)) for ish in g["dyn_shells"] for il in g["dyn_layers"]]
+    ks = [order.index(sl) for sl in keep]
+    for (ish, il), k in zip(keep, ks):
+        i, j = list(g["dyn_shells"]).index(ish), list(g["dyn_layers"]).index(il)
+        for iv, n in enumerate(names):
+            np.testing.assert_array_equal(table[k, abi.PV[n], ::every], g["dyn"][i, j, iv],
+                                          err_msg=f"shell {ish} layer {il} {n}")
+    pops = (abi.Population * len(ks))(*[abi.Population(k, 0, 0, 0) for k in ks])
+    jall, _, st = pyoracle.ele_evolve(P, pops, table)
+    assert not st.any()
+    lays = (abi.EatsLayer * len(ks))(*[abi.EatsLayer(inits[k].theta_c_l, inits[k].theta_c_h, k, 0, q, -1,
+                                                     res[k].i_end_r) for q, k in enumerate(ks)])
+    lc, st2 = pyoracle.eats_lightcurve(P, lays, table, jall, tobs, nuobs)
+    assert not st2.any()
+    for q, (ish, il) in enumerate(keep):
+        i, j = list(g["dyn_shells"]).index(ish), list(g["dyn_layers"]).index(il)
+        assert rel_err(lc[q], g["lc"][i, j], 1e-30) == 0.0, (ish, il)
+
+
 def test_ejecta_state_order_is_layer_major():
     """The reference's state vector holds, for each layer, its velocity shells in order
     (src/model_evolve.h:225-234); the shared step's error norm sums in that order."""
