@@ -82,38 +82,44 @@ def test_oracle_ejecta_matches_reference(name):
         assert len(np.unique(rho)) > 10 and np.any(table[k, abi.PV["drhodr"]] < 0)
 
 
-def test_oracle_full_size_ejecta_dynamics_match_reference():
-    """The full C5 ejecta: 32 shells x 30 layers = 960 blastwaves in one ODE system, 1000 output
-    steps (about 11,000 shared adaptive steps).  The reference ran the dynamics alone; the
-    oracle's dynamics of 12 blastwaves spread over the shells and layers (every 10th step, the
-    fixture's subset) equal its output bit for bit.  About 16 s of one core."""
-    g = load_golden("kn_c5_dyn")
-    P, spread, inits, order, tgrid, *_ = kn_golden_run(g)
-    assert len(inits) == 960 and len(tgrid) == 1000 and P.fs.ngam == 128 and P.fs.nfreq == 128
-    table, res = pyoracle.dyn_evolve_ejecta(P, inits, tgrid, spread=spread)
-    assert all(r.status == 0 for r in res) and res[0].n_steps > 10000
-    names = [str(n) for n in g["dyn_names"]]
-    every = int(g["dyn_every"])
-    for i, ish in enumerate(g["dyn_shells"]):
-        for j, il in enumerate(g["dyn_layers"]):
-            k = order.index((int(ish), int(il)))
-            for iv, n in enumerate(names):
-                np.testing.assert_array_equal(table[k, abi.PV[n], ::every], g["dyn"][i, j, iv],
-                                              err_msg=f"shell {ish} layer {il} {n}")
+def _same_dynamics_inputs(P, Q):
+    """Run parameters equal but for the microphysics grids and SSC, which the ejecta RHS never
+    reads (rhs_fs_dense has no electron or radiative-loss integral term)."""
+    Q2 = abi.RunParams.from_buffer_copy(Q)
+    for sh in (Q2.fs, Q2.rs):
+        sh.ngam, sh.nfreq, sh.ssc = P.fs.ngam, P.fs.nfreq, P.fs.ssc
+    return bytes(Q2) == bytes(P)
 
 
-def test_oracle_full_geometry_ejecta_light_curves_match_reference():
-    """The C5 ejecta (960 blastwaves, 1000 steps) at reduced microphysics grids (64 gamma x 48 nu,
-    no SSC, 6 x 8 pairs): the reference computed every blastwave's electrons and light curve.  The
-    oracle evolves the dynamics of all 960 (they share one adaptive step) and the electrons and
-    light curves of the 12 blastwaves the fixture keeps (the ejecta RHS has no electron term, so
-    the others do not act on them): dynamics, limits and light curves equal the reference's bit
-    for bit.  About 20 s of one core."""
+def test_oracle_full_size_ejecta_match_reference():
+    """The C5 ejecta at full size: 32 shells x 30 layers = 960 blastwaves in one ODE system, 1000
+    output steps (about 11,000 shared adaptive steps), against two reference runs:
+      - kn_c5_dyn, the C5 parfile itself (128 gamma x 128 nu, SSC), dynamics only;
+      - kn_c5_lc, the same ejecta at reduced microphysics grids (64 gamma x 48 nu, no SSC, 6 x 8
+        pairs) with every blastwave's electrons and light curve.
+    The two share every input of the dynamics.  The oracle evolves all 960 blastwaves once; the
+    dynamics of 12 blastwaves spread over the shells and layers (every 10th step, the fixtures'
+    subset) equal both runs' bit for bit, and the electron limits and light curves of those 12
+    equal kn_c5_lc's (the ejecta RHS has no electron term, so the other blastwaves' electrons do
+    not act on them).  About 16 s of one core."""
     g = load_golden("kn_c5_lc")
     P, spread, inits, order, tgrid, tobs, nuobs, method = kn_golden_run(g)
-    assert len(inits) == 960 and P.fs.ssc == 0 and P.fs.ngam == 64
+    assert len(inits) == 960 and len(tgrid) == 1000 and P.fs.ssc == 0 and P.fs.ngam == 64
+    gd = load_golden("kn_c5_dyn")
+    Pd, spread_d, inits_d, order_d, tgrid_d, *_ = kn_golden_run(gd)
+    assert Pd.fs.ngam == 128 and Pd.fs.nfreq == 128 and Pd.fs.ssc == 1 and spread_d == spread
+    assert _same_dynamics_inputs(P, Pd) and np.array_equal(tgrid_d, tgrid) and order_d == order
+    assert bytes(inits_d) == bytes(inits)
     table, res = pyoracle.dyn_evolve_ejecta(P, inits, tgrid, spread=spread)
-    assert all(r.status == 0 for r in res)
+    assert all(r.status == 0 for r in res) and res[0].n_steps > 10000
+    names_d = [str(n) for n in gd["dyn_names"]]
+    every_d = int(gd["dyn_every"])
+    for i, ish in enumerate(gd["dyn_shells"]):
+        for j, il in enumerate(gd["dyn_layers"]):
+            k = order.index((int(ish), int(il)))
+            for iv, n in enumerate(names_d):
+                np.testing.assert_array_equal(table[k, abi.PV[n], ::every_d], gd["dyn"][i, j, iv],
+                                              err_msg=f"C5 dynamics: shell {ish} layer {il} {n}")
     names = [str(n) for n in g["dyn_names"]]
     every = int(g["dyn_every"])
     keep = [(int(ish), int(il)) for ish in g["dyn_shells"] for il in g["dyn_layers"]]
