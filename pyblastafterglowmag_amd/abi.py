@@ -291,21 +291,23 @@ def ejecta_inits(id2d, method="adaptive"):
 
     Per shell the loader builds the layers' angular grids from theta_wing, the last layer's theta
     of the last shell, as for a jet (EjectaID2::_load_id_file, src/ejecta/ejecta_id.h:221-300).
-    A blastwave with ek, mass or r not above 0, or beta0 <= 1e-6, is not evolved
-    (setEjectaBwPars, src/ejecta/ejecta_base.h:255-265): its E0 = M0 = 0 here."""
+    A blastwave with ek or mass not above 0, or beta0 <= 1e-6 (its start radius beta0 c t0 then
+    also fails the r > 0 test), is not evolved (setEjectaBwPars, src/ejecta/ejecta_base.h:255-265):
+    its E0 = M0 = 0 here."""
     ek = np.atleast_2d(np.asarray(id2d["ek"], dtype=np.float64))
     ns, nl = ek.shape
     mass = np.atleast_2d(np.asarray(id2d["mass"], dtype=np.float64))
     mom = np.atleast_2d(np.asarray(id2d["mom"], dtype=np.float64))
     theta = np.atleast_2d(np.asarray(id2d["theta"], dtype=np.float64))
     tw = float(theta[ns - 1, nl - 1])
+    rows = [layer_inits(dict(ek=ek[ish], mass=mass[ish], mom=mom[ish], theta_core=tw, theta_wing=tw), method)
+            for ish in range(ns)]
     out = (LayerInit * (ns * nl))()
     order = []
     for il in range(nl):
         for ish in range(ns):
-            row = layer_inits(dict(ek=ek[ish], mass=mass[ish], mom=mom[ish], theta_core=tw, theta_wing=tw), method)
             L = out[len(order)]
-            C.memmove(C.addressof(L), C.addressof(row[il]), C.sizeof(LayerInit))
+            C.memmove(C.addressof(L), C.addressof(rows[ish][il]), C.sizeof(LayerInit))
             beta0 = mom[ish, il] / math.sqrt(mom[ish, il] * mom[ish, il] + 1.0)
             if not (ek[ish, il] > 0 and mass[ish, il] > 0 and beta0 > 1e-6):
                 L.E0 = L.M0 = 0.0
