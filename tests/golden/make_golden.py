@@ -8,7 +8,10 @@ compressed .npz next to this script.  Only data is stored (inputs and outputs).
     python tests/golden/make_golden.py [name ...]      (no name: every fixture, ALL)
 
 gauss21_fsrs_ssc (the full 21-layer jet, 200 x 96 grids, 12,288 pairs) takes about 25 minutes of
-one core.
+one core.  The kilonova cases (kn_*: ejecta only, written by inputs.write_kn_run) store every
+(shell, layer) blastwave's dynamics and light curve, or a subset for the full-size ejecta:
+kn_c5_dyn (the C5 parfile, dynamics only) takes about 1 minute, kn_c5_lc (the C5 ejecta at
+reduced microphysics grids with light curves) about 6 minutes.
 """
 import os
 import shutil
