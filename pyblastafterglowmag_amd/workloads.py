@@ -305,6 +305,10 @@ KN_SMALL = {
     # the one the last RHS call evaluated (set_standard_ism), not the start radius's
     "kn_wind_small": dict(n_shells=4, n_layers=3, ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
                           main=dict(n_ism=-1., A0=3e35, s=2., r_ej=1e16, r_ism=3e18)),
+    # the classical RK4 integrator: the stored GammaRel is the Gamma of the last RHS call, the
+    # state of RK4's last stage rather than the step's end
+    "kn_rk4_small": dict(n_shells=4, n_layers=3, ntb=300, ngam=64, nfreq=48, n_nu_obs=6, n_t_obs=8,
+                         main=dict(integrator="RK4")),
 }
 
 

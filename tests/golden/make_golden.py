@@ -169,7 +169,8 @@ def run_reference(name, wd, spec_every=1):
 # kilonova ejecta (parfile kN section, no jet): every (shell, layer) blastwave's dynamics and light
 # curve; KN_DYN_ONLY runs the dynamics alone (no electrons or light curves) and keeps the
 # blastwaves of KN_DYN_KEEP, every KN_DYN_EVERY-th step
-KN_CASES = {"kn_small", "kn_pw_small", "kn_spread_small", "kn_wind_small", "kn_c5_dyn", "kn_c5_lc"}
+KN_CASES = {"kn_small", "kn_pw_small", "kn_spread_small", "kn_wind_small", "kn_rk4_small", "kn_c5_dyn",
+            "kn_c5_lc"}
 KN_DYN_ONLY = {"kn_c5_dyn"}
 KN_DYN_KEEP = {"kn_c5_dyn": ((0, 10, 21, 31), (0, 14, 29)), "kn_c5_lc": ((0, 10, 21, 31), (0, 14, 29))}
 KN_DYN_EVERY = {"kn_c5_dyn": 10, "kn_c5_lc": 10}
@@ -265,7 +266,7 @@ ALL = ["tophat_small", "gauss5_small", "tophat_fs_syn", "tophat_pw_small", "gaus
        "tophat_ssa_small", "gauss5_ssa_small", "tophat_dermer_small", "tophat_bessel_small", "tophat_gsl_small",
        "tophat_rk4_small", "tophat_wind_small", "gauss21_fsrs_ssc", "gauss64_pw_small", "tophat_ebl_small",
        "lhs_small_0", "lhs_small_42", "tophat_f2t_small", "kn_small", "kn_pw_small", "kn_spread_small",
-       "kn_wind_small", "kn_c5_dyn", "kn_c5_lc"]
+       "kn_wind_small", "kn_rk4_small", "kn_c5_dyn", "kn_c5_lc"]
 
 if __name__ == "__main__":
     main(sys.argv[1:] or ALL)

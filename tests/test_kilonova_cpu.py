@@ -48,16 +48,18 @@ def _observer_layers(inits, res, method):
                                  for k in range(n)])
 
 
-@pytest.mark.parametrize("name", ["kn_small", "kn_pw_small", "kn_spread_small", "kn_wind_small"])
+@pytest.mark.parametrize("name", ["kn_small", "kn_pw_small", "kn_spread_small", "kn_wind_small", "kn_rk4_small"])
 def test_oracle_ejecta_matches_reference(name):
     """4 velocity shells x 3 layers (12 blastwaves, 300 steps, SSC on): dynamics of every
     blastwave and the light curve of every (shell, layer) equal the reference's bit for bit.
     Cases: adaptive and piece-wise observer integrals; method_spread set (the spreading form of
-    dt_obs/dR); a wind-like medium (the stored density is the last RHS call's)."""
+    dt_obs/dR); a wind-like medium (the stored density is the last RHS call's); the RK4 integrator
+    (the stored GammaRel is the Gamma of RK4's last stage, not the step's end)."""
     g = load_golden(name)
     P, spread, inits, order, tgrid, tobs, nuobs, method = kn_golden_run(g)
     assert spread == (name == "kn_spread_small") and P.fs.ssc == 1
     assert method == ("piece-wise" if "pw" in name else "adaptive") and (P.n_ism < 0) == (name == "kn_wind_small")
+    assert P.integrator == (1 if name == "kn_rk4_small" else 0)
     np.testing.assert_array_equal(tobs, g["times"])
     np.testing.assert_array_equal(nuobs, g["freqs"])
     table, res = pyoracle.dyn_evolve_ejecta(P, inits, tgrid, spread=spread)
@@ -76,6 +78,9 @@ def test_oracle_ejecta_matches_reference(name):
         assert rel_err(lc[k], g["lc"][ish, il], 1e-30) == 0.0, (name, ish, il)
     # the summed light curve is dominated by the fast shells and is not zero
     assert np.all(g["lc"].sum(axis=(0, 1)) > 0)
+    if name == "kn_rk4_small":
+        k = order.index((3, 2))
+        assert np.sum(table[k, abi.PV["GammaRel"]] != table[k, abi.PV["Gamma"]]) > 290
     if name == "kn_wind_small":   # the stored density follows the blastwave through the profile
         k = order.index((3, 2))
         rho = table[k, abi.PV["rho"]]
