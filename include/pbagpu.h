@@ -249,6 +249,22 @@ int pbagpu_dyn_evolve_ragged(const pba_run_params *params, int n_runs, int bw_pe
                              const pba_layer_init *init, int nt_max, const int *nt_run,
                              const double *d_tgrid, double *d_table, pba_bw_result *res, void *stream);
 
+/* Kilonova ejecta: every blastwave of every run evolved as the reference evolves ejecta,
+ * setStandardISM + BlastWave::rhs_fs_dense (src/model_evolve.h:1627-1651,
+ * src/blastwave/blastwave.h:769-781, 2120-2497), with method_dgdr = our and no energy injection,
+ * nuclear heating or shell collisions; replaces the same RHS callback and step loop as
+ * pbagpu_dyn_evolve.  The blastwaves of a run are in the reference's state-vector order,
+ * layer-major (for each layer, its velocity shells; src/model_evolve.h:225-234); init carries
+ * the per-blastwave initial data of the 2-D [shells][layers] file (EjectaID2, use_1d_id = no).
+ * spread: method_spread set (the spreading form of dt_obs/dR and of the initial observer time;
+ * dtheta/dR stays 0).  Besides the jet's columns the table holds GammaRel, and rho / drhodr are
+ * those of the last RHS evaluation, as the reference stores them.  Runs of at most 64
+ * blastwaves (one wavefront per run; PBAGPU_E_UNSUPPORTED beyond); params[r].fsrs must be 0.
+ * The electron and observer stages take the resulting tables as a jet's. */
+int pbagpu_dyn_evolve_ejecta(const pba_run_params *params, int n_runs, int bw_per_run,
+                             const pba_layer_init *init, int spread, int nt, const double *d_tgrid,
+                             double *d_table, pba_bw_result *res, void *stream);
+
 /* ------------------------------------------------------------- stage 2 */
 /* One electron population: shock (0 forward, 1 reverse) of blastwave `bw`
  * (index into the dynamics table), with the microphysics of run `run`. */

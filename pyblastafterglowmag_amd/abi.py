@@ -167,9 +167,10 @@ def run_params_from_parfile(path):
     gp, go = inputs.read_parfile(path, inputs.SEC_GRB)
     kp, ko = inputs.read_parfile(path, inputs.SEC_KN)
     if ko.get("run_bws", "no") == "yes":
-        # the ejecta's right-hand side (rhs_fs_dense) is not in the device's dynamics stage
-        raise NotImplementedError(f"{path}: kilonova ejecta (kN section, run_bws = yes) are not evolved by this "
-                                  "path; only the jet (GRB section) is")
+        # the drop-in driver reads and writes the jet's files; ejecta go through run_params_ejecta,
+        # ejecta_inits and pbagpu_dyn_evolve_ejecta (gpu.Pipeline.dynamics_ejecta)
+        raise NotImplementedError(f"{path}: kilonova ejecta (kN section, run_bws = yes) are not read by the jet's "
+                                  "parfile path; use run_params_ejecta / ejecta_inits and Pipeline.dynamics_ejecta")
     if go.get("run_bws", "no") != "yes":
         raise ValueError(f"{path}: no jet to evolve (GRB section missing or run_bws = no)")
     return run_params(mp_, mo, gp, go), mp_, mo, gp, go

@@ -309,10 +309,76 @@ __device__ void rhs_fsrs(const pba_run_params &P, const Shock &shfs, BW &b, cons
     o[ideltaR4] = dRdt * ddR4;
 }
 
-__device__ __forceinline__ void rhs(const pba_run_params &P, const Shock &shfs, BW &b, bool active, const double *Y,
-                                    double *o) {
+// Kind of the blastwaves of a launch: the jet's (rhs_fs / rhs_fsrs), or kilonova ejecta evolved by
+// rhs_fs_dense with dt_obs/dR in its radial or (method_spread set) spreading form.
+enum Kind { KIND_JET = 0, KIND_EJECTA = 1, KIND_EJECTA_SPREAD = 2 };
+
+// The upstream state that an ejecta blastwave's last RHS call left in its density object
+// (set_standard_ism, src/blastwave/blastwave.h:1282-1295); addOtherVars stores it (:539-560).
+struct Upstream { double rho, drhodr, grel; int seen, pad_; };
+
+// Ejecta blastwave in a standard ISM: setStandardISM + BlastWave::rhs_fs_dense
+// (src/blastwave/blastwave.h:769-781, 2120-2497; src/model_evolve.h:1627-1651) with method_dgdr =
+// our, method_dmdr = usingdthdr and no energy injection, nuclear heating or luminosity loss: the
+// upstream is at rest (GammaRho = 1), GammaRel = Gamma, dtheta/dR = 0, and Rsh is not evolved.
+__device__ void rhs_fs_dense(const pba_run_params &P, BW &b, Upstream &up, bool spread, const double *Y, double *o) {
+    double Gamma = Y[iGamma];
+    const double R = Y[iR], Eint2 = Y[iEint2], theta = Y[itheta], M2 = Y[iM2];
+    if (Gamma < 1.) { Gamma = 1. + 1.e-5; b.status |= PBAGPU_S_NONFINITE; }
+    const double beta = beta_from_gamma(Y[iGamma]);
+    if (!isfinite(R) || !isfinite(Gamma) || M2 < 0. || !isfinite(M2) || !isfinite(Eint2) || Eint2 < 0)
+        b.status |= PBAGPU_S_NONFINITE;
+    double rho_c, drhodr_c;
+    ism_density(P, R, rho_c, drhodr_c, b.status);
+    const double GRel = Gamma;   // GammaRel, clamped with Gamma (setStandardISM)
+    up.rho = rho_c; up.drhodr = drhodr_c; up.grel = GRel; up.seen = 1;
+    const double rho = rho_c / b.M0, drhodr = drhodr_c / b.M0;
+    const double dRdt = beta * kC;
+    const double adi = adi_nava(GRel);
+    const double dthetadr = 0.0;
+    const double dM2dR = 2. * kPi * rho * ((1. - cos(theta)) + (1. / 3.) * sin(theta) * R * dthetadr) * R * R / b.ncells;
+    // energy injection, nuclear heating and luminosity terms: all zero here
+    const double Dop = Gamma / (1. - beta * cos(0.));
+    const double dEinjdR = 0. / (b.M0 * kC * kC) / b.ncells / dRdt;
+    const double dEinj_dop = dEinjdR / Dop / Dop;
+    const double dEnucdR = 0. / (b.M0 * kC * kC) / dRdt, dElumdR = 0. / (b.M0 * kC * kC) / dRdt;
+    const double GE = (adi * GRel * GRel - adi + 1.0) / GRel;
+    const double dGE = (adi * GRel * GRel + adi - 1.0) / (GRel * GRel);
+    const double num1 = (Gamma - 1. + GE * (GRel - 1.)) * dM2dR;
+    const double num2 = -GE * (adi - 1.) * Eint2 * (dM2dR / M2 - drhodr / rho - 0. / 1.);
+    const double num3 = -(1. + GE / Gamma * 1.) * dEinjdR;
+    const double den = (1. + M2) + Eint2 * dGE + GE * (adi - 1.) * Eint2 * 1. / GRel;
+    const double dGdR = -1. * (num1 + num2 + num3) / den;
+    const double dlnV2dR = dM2dR / M2 - drhodr / rho - (1. / GRel) * 1. * dGdR + 0. / 1.;
+    const double dEad2dR = P.adiab_loss ? -(adi - 1.0) * Eint2 * dlnV2dR : 0.0;
+    const double dEsh2dR = (GRel - 1.0) * dM2dR;   // a subsonic upstream would scale it by 1e-10; c_s = 0 here
+    const double dErad2dR = P.fs.epsilon_e_rad * dEsh2dR;
+    const double dEint2dR = dEsh2dR + dEad2dR - dErad2dR + dEnucdR - dElumdR + dEinj_dop;
+    // EQS::evalElapsedTime (src/blastwave/blastwave_components.h:165-175)
+    const double mom = Gamma * beta;
+    const double bm = beta_from_mom(mom);
+    double dttdr;
+    if (spread) dttdr = 1. / (kC * bm) * sqrt(1. + R * R * dthetadr * dthetadr) - (1. / kC);
+    else dttdr = 1. / (kC * (1. + mom * mom * mom / sqrt(1. + mom * mom)) * (1. + bm));
+    if (!isfinite(dRdt) || !isfinite(dGdR) || dM2dR < 0. || !isfinite(dlnV2dR)) b.status |= PBAGPU_S_NONFINITE;
+    o[iR] = dRdt;
+    o[itt] = dRdt * dttdr;
+    o[itcomov] = dRdt * (1.0 / beta / Gamma / kC);
+    o[iGamma] = dRdt * dGdR;
+    o[iEint2] = dRdt * dEint2dR;
+    o[itheta] = dRdt * dthetadr;
+    o[iErad2] = dRdt * dErad2dR;
+    o[iEsh2] = dRdt * dEsh2dR;
+    o[iEad2] = dRdt * dEad2dR;
+    o[iM2] = dRdt * dM2dR;
+}
+
+template <int KIND>
+__device__ __forceinline__ void rhs(const pba_run_params &P, const Shock &shfs, BW &b, Upstream *up, bool active,
+                                    const double *Y, double *o) {
     if (!active || b.end_evolution) return;  // derivative entries of inactive blastwaves keep their (zero) values
-    if (b.fsrs) rhs_fsrs(P, shfs, b, Y, o);
+    if constexpr (KIND != KIND_JET) rhs_fs_dense(P, b, *up, KIND == KIND_EJECTA_SPREAD, Y, o);
+    else if (b.fsrs) rhs_fsrs(P, shfs, b, Y, o);
     else rhs_fs(P, shfs, b, Y, o);
 }
 
@@ -379,8 +445,9 @@ __device__ double ctheta_of(double theta, int ilayer, int nlayers) {
     return 0.5 * (xx1 + xx2);
 }
 
-__device__ __forceinline__ void post_step(const pba_run_params &P, const Shock &shfs, const Shock &shrs, BW &b, const double *Yc,
-                          int it, double tb, double *T, int nt) {
+template <int KIND>
+__device__ __forceinline__ void post_step(const pba_run_params &P, const Shock &shfs, const Shock &shrs, BW &b,
+                                          const Upstream *up, const double *Yc, int it, double tb, double *T, int nt) {
     if (b.end_evolution) return;
     auto W = [&](int v, double x) { T[(size_t)v * nt + it] = x; };
     auto Rd = [&](int v, int t) { return T[(size_t)v * nt + t]; };
@@ -400,9 +467,16 @@ __device__ __forceinline__ void post_step(const pba_run_params &P, const Shock &
     if (R < 1. || G < 1.) b.status |= PBAGPU_S_NONFINITE;
     W(PV_ctheta, ctheta_of(theta, b.ilayer, b.nlayers));
     // addOtherVars stores the density set at the blastwave's start (m_rho_, blastwave.h:92-101,
-    // 539-541), and the downstream quantities are formed from it
-    double rho = b.rho_def;
-    W(PV_rho, rho); W(PV_drhodr, b.drhodr0);
+    // 539-541), and the downstream quantities are formed from it; an ejecta blastwave's density
+    // object holds what its last RHS call evaluated, and GammaRel (unset, -1, before the first
+    // call: then Gamma is stored, :544-560)
+    double rho = b.rho_def, drhodr = b.drhodr0;
+    if constexpr (KIND != KIND_JET) {
+        if (up->seen) { rho = up->rho; drhodr = up->drhodr; }
+        const double grel = up->seen ? up->grel : -1.;
+        W(PV_GammaRel, grel < 1. ? G : grel);
+    }
+    W(PV_rho, rho); W(PV_drhodr, drhodr);
     double adi = adi_nava(G);
     double rho2 = rho2t(G, adi, rho);
     double Gsh = gamma_shock(G, adi);
@@ -432,6 +506,13 @@ __device__ __forceinline__ void post_step(const pba_run_params &P, const Shock &
     if (valid && beta < P.fs.beta_min) {
         if (b.i_end_r > it - 1) b.i_end_r = it - 1;
         valid = false;
+    }
+    if constexpr (KIND != KIND_JET) {   // subsonic upstream (sound speed 0 here): no shock (blastwave_radiation.h:918-928)
+        if (valid && 0. >= beta) {
+            W(PV_B, 0.);
+            if (b.i_end_r > it - 1) b.i_end_r = it - 1;
+            valid = false;
+        }
     }
     if (valid && (U_p <= 0 || rho2 <= 0)) valid = false;
     if (valid && R < 1) valid = false;
@@ -545,7 +626,7 @@ enum Phase { PH_K1I, PH_H2, PH_STAGE, PH_FSAL, PH_RK1, PH_RK2, PH_RK3, PH_RK4 };
 // three 21-blastwave runs would need 82 KB of LDS, one wavefront per CU).
 // nt is the table's step stride; run r integrates nt_run[r] steps (all nt when nt_run is null) on
 // the time grid tgrid + r * tg_stride (tg_stride 0: one grid for every run).
-template <bool KG>
+template <bool KG, int KIND = KIND_JET>
 __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restrict__ params,
                                                  const pba_layer_init *__restrict__ init, int n_runs, int bw_per_run,
                                                  int rpw, int nt, const int *__restrict__ nt_run,
@@ -553,7 +634,7 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
                                                  double *__restrict__ table, pba_bw_result *__restrict__ res,
                                                  double *__restrict__ kg) {
     // LDS: (KG = false: [KS][NA][bw_per_run] stage vectors,) the state Y as [NA][ys], 128 doubles for
-    // sums, then one BW per lane slot (ys slots)
+    // sums, then one BW per lane slot (ys slots), then (ejecta) one Upstream per lane slot
     extern __shared__ double dyn_lds[];
     const int ys = KG ? 64 : bw_per_run;
     double *ylds = KG ? dyn_lds : dyn_lds + (size_t)KS * NA * bw_per_run;
@@ -590,6 +671,11 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
     // the blastwave's constants and flags (in LDS; padding lanes alias slot 0 and never touch it)
     BW &b = bws[active ? lane : 0];
     if (active) b = BW{};
+    Upstream *up = nullptr;
+    if constexpr (KIND != KIND_JET) {
+        up = reinterpret_cast<Upstream *>(bws + ys) + (active ? lane : 0);
+        if (active) *up = Upstream{0., 0., -1., 0, 0};
+    }
     double x[NA];
     for (int k = 0; k < NA; k++) x[k] = 0.;
     if (active) {
@@ -639,7 +725,14 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
             if (b.M0 <= 0.) b.M0 = b.E0 / (b.Gamma0 * kC * kC);
             Yv(iR) = b.R0;
             double b0m = beta_from_mom(b.mom0);
-            Yv(itt) = b.R0 / (kC * b0m) * sqrt(1. + b.R0 * b.R0 * 0. * 0.) - b.R0 / kC;
+            // EQS::init_elapsed_time (blastwave_components.h:141-163): the spreading form for the jet
+            // and for ejecta with method_spread set, else the radial one
+            if constexpr (KIND == KIND_EJECTA) {
+                const double xx = 1. + b.mom0 * b.mom0 * b.mom0 / (sqrt(1. + b.mom0 * b.mom0));
+                Yv(itt) = b.R0 / (kC * xx * (1.0 + b0m));
+            } else {
+                Yv(itt) = b.R0 / (kC * b0m) * sqrt(1. + b.R0 * b.R0 * 0. * 0.) - b.R0 / kC;
+            }
             Yv(itcomov) = b.R0 / (b.beta0 * b.Gamma0 * kC);
             Yv(iEint2) = (b.Gamma0 - 1.) * M20 / b.M0 + 0. / b.M0 / kC / kC;
             Yv(iGamma) = b.Gamma0;
@@ -651,7 +744,7 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
         }
         double yc[NA];
         for (int k = 0; k < NA; k++) yc[k] = Yv(k);
-        post_step(P, shfs, shrs, b, yc, 0, tgrid[0], T, nt);
+        post_step<KIND>(P, shfs, shrs, b, up, yc, 0, tgrid[0], T, nt);
     }
     const double rtol = P.rtol, atol = P.rtol;
     const double nsz = (double)(NEQ_REF * bw_per_run);
@@ -697,7 +790,7 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
     while (__ballot(live) != 0ull) {
         double o[NA];
         for (int k = 0; k < NA; k++) o[k] = 0.;
-        rhs(P, shfs, b, live, x, o);   // the one right-hand-side evaluation of the loop
+        rhs<KIND>(P, shfs, b, up, live, x, o);   // the one right-hand-side evaluation of the loop
         if (live) nrhs++;
         // per-lane terms of the run-wide norms this pass needs (initial step, error estimate)
         double c1 = 0., c2 = 0.;
@@ -835,7 +928,7 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
             b.prev_idx_x = it;
             double yc[NA];   // a copy, so that Y itself stays in registers
             for (int k = 0; k < NA; k++) yc[k] = Yv(k);
-            post_step(P, shfs, shrs, b, yc, it, tgrid[it], T, nt);
+            post_step<KIND>(P, shfs, shrs, b, up, yc, it, tgrid[it], T, nt);
             it++;
             if (it < ntr) start_interval();
             else live = false;
@@ -851,12 +944,27 @@ __global__ void __launch_bounds__(64) dyn_kernel(const pba_run_params *__restric
 }  // namespace dyn
 }  // namespace pbagpu
 
+// kind: pbagpu::dyn::Kind of every blastwave of the launch
 static int dyn_evolve_impl(const pba_run_params *params, int n_runs, int bw_per_run, const pba_layer_init *init,
                            int nt, const int *nt_run, const double *d_tgrid, int tg_stride, double *d_table,
-                           pba_bw_result *res, void *stream) {
-    if (!params || !init || !d_tgrid || !d_table || !res || n_runs < 1 || bw_per_run < 1 || bw_per_run > 64 || nt < 2)
+                           pba_bw_result *res, void *stream, int kind = pbagpu::dyn::KIND_JET) {
+    if (!params || !init || !d_tgrid || !d_table || !res || n_runs < 1 || bw_per_run < 1 || nt < 2)
         return PBAGPU_E_ARG;
+    if (bw_per_run > 64) {
+        // one wavefront per run: the blastwaves of a run share one adaptive step through LDS
+        pbagpu::set_error_msg(kind == pbagpu::dyn::KIND_JET
+                                  ? "dyn_evolve: at most 64 blastwaves per run"
+                                  : "dyn_evolve_ejecta: at most 64 blastwaves per run (one wavefront per run); "
+                                    "larger ejecta need a kernel that spreads one run over several wavefronts");
+        return PBAGPU_E_UNSUPPORTED;
+    }
     hipStream_t s = (hipStream_t)stream;
+    if (kind != pbagpu::dyn::KIND_JET)
+        for (int r = 0; r < n_runs; r++)
+            if (params[r].fsrs) {
+                pbagpu::set_error_msg("dyn_evolve_ejecta: ejecta blastwaves have no reverse shock (fsrs must be 0)");
+                return PBAGPU_E_ARG;
+            }
     for (int r = 0; r < n_runs; r++)
         if (params[r].integrator != 0 && params[r].integrator != 1) {
             pbagpu::set_error_msg("pba_run_params: integrator must be 0 (DOP853E) or 1 (RK4)");
@@ -896,7 +1004,8 @@ static int dyn_evolve_impl(const pba_run_params *params, int n_runs, int bw_per_
     const long long opt_rpw = pbagpu::option(PBAGPU_OPT_DYN_RUNS_PER_WAVE);
     const int rpw_max = 64 / bw_per_run;
     int rpw;
-    if (opt_rpw > 0) rpw = (int)std::min<long long>(opt_rpw, rpw_max);
+    if (kind != pbagpu::dyn::KIND_JET) rpw = 1;   // ejecta: one run per wavefront
+    else if (opt_rpw > 0) rpw = (int)std::min<long long>(opt_rpw, rpw_max);
     else {
         const int simds = 4 * std::max(1, pbagpu::stream_cu_count(s));
         rpw = std::min(rpw_max, std::max(1, (n_runs + simds - 1) / simds));
@@ -917,13 +1026,23 @@ static int dyn_evolve_impl(const pba_run_params *params, int n_runs, int bw_per_
             hipLaunchKernelGGL(pbagpu::dyn::dyn_kernel<true>, dim3(nblk), dim3(64),
                                sizeof(double) * (NA * 64 + 128) + sizeof(pbagpu::dyn::BW) * 64, s, dP, dI,
                                n_runs, bw_per_run, rpw, nt, (const int *)dN, d_tgrid, tg_stride, d_table, dR, dK);
-        } else {
+        } else if (kind == pbagpu::dyn::KIND_JET) {
             const size_t lds = sizeof(double) * ((size_t)(KS + 1) * NA * bw_per_run + 128) +
                                sizeof(pbagpu::dyn::BW) * bw_per_run;
             (void)hipFuncSetAttribute((const void *)pbagpu::dyn::dyn_kernel<false>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             hipLaunchKernelGGL(pbagpu::dyn::dyn_kernel<false>, dim3(nblk), dim3(64), lds, s, dP, dI, n_runs,
                                bw_per_run, 1, nt, (const int *)dN, d_tgrid, tg_stride, d_table, dR, (double *)nullptr);
+        } else {
+            // the jet's LDS layout plus one Upstream record per blastwave
+            const size_t lds = sizeof(double) * ((size_t)(KS + 1) * NA * bw_per_run + 128) +
+                               (sizeof(pbagpu::dyn::BW) + sizeof(pbagpu::dyn::Upstream)) * bw_per_run;
+            auto kern = kind == pbagpu::dyn::KIND_EJECTA_SPREAD
+                            ? pbagpu::dyn::dyn_kernel<false, pbagpu::dyn::KIND_EJECTA_SPREAD>
+                            : pbagpu::dyn::dyn_kernel<false, pbagpu::dyn::KIND_EJECTA>;
+            (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL(kern, dim3(nblk), dim3(64), lds, s, dP, dI, n_runs, bw_per_run, 1, nt,
+                               (const int *)dN, d_tgrid, tg_stride, d_table, dR, (double *)nullptr);
         }
         PBA_TRY(rc, hipGetLastError());
         timer.stop(s);
@@ -944,4 +1063,11 @@ extern "C" int pbagpu_dyn_evolve_ragged(const pba_run_params *params, int n_runs
                                         const double *d_tgrid, double *d_table, pba_bw_result *res, void *stream) {
     if (!nt_run) return PBAGPU_E_ARG;
     return dyn_evolve_impl(params, n_runs, bw_per_run, init, nt_max, nt_run, d_tgrid, nt_max, d_table, res, stream);
+}
+
+extern "C" int pbagpu_dyn_evolve_ejecta(const pba_run_params *params, int n_runs, int bw_per_run,
+                                        const pba_layer_init *init, int spread, int nt, const double *d_tgrid,
+                                        double *d_table, pba_bw_result *res, void *stream) {
+    return dyn_evolve_impl(params, n_runs, bw_per_run, init, nt, nullptr, d_tgrid, 0, d_table, res, stream,
+                           spread ? pbagpu::dyn::KIND_EJECTA_SPREAD : pbagpu::dyn::KIND_EJECTA);
 }

@@ -21,7 +21,7 @@ EXPORTS = ("pbagpu_device_count", "pbagpu_set_device", "pbagpu_version", "pbagpu
            "pbagpu_synchrotron_f", "pbagpu_eats_lightcurve_vec", "pbagpu_mlp_forward", "pbagpu_mlp_train_step",
            "pbagpu_gemm_bf16", "pbagpu_ssc_features", "pbagpu_ssc_emissivity", "pbagpu_ele_evolve_surrogate",
            "pbagpu_skymap_hist_noalloc", "pbagpu_ele_evolve_spectra", "pbagpu_dyn_evolve_ragged",
-           "pbagpu_set_option", "pbagpu_get_option")
+           "pbagpu_set_option", "pbagpu_get_option", "pbagpu_dyn_evolve_ejecta")
 OPT_ELE_GRAPH = 0
 OPT_DYN_RUNS_PER_WAVE = 1
 STAGE_DYN, STAGE_ELE, STAGE_EATS, STAGE_SSC_TABLE, STAGE_EATS_PW, STAGE_SKY_HIST, STAGE_EATS_VEC = 0, 1, 2, 3, 4, 5, 6
@@ -49,6 +49,8 @@ def lib():
                                         vp, vp, C.POINTER(abi.BwResult), vp]
         L.pbagpu_dyn_evolve_ragged.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.c_int, C.POINTER(abi.LayerInit),
                                                C.c_int, C.POINTER(C.c_int), vp, vp, C.POINTER(abi.BwResult), vp]
+        L.pbagpu_dyn_evolve_ejecta.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.c_int, C.POINTER(abi.LayerInit),
+                                               C.c_int, C.c_int, vp, vp, C.POINTER(abi.BwResult), vp]
         L.pbagpu_ele_evolve.argtypes = [C.POINTER(abi.RunParams), C.c_int, C.POINTER(abi.Population), C.c_int, vp,
                                         C.c_int, C.c_int, vp, vp, vp, vp, C.c_int, vp]
         L.pbagpu_ele_evolve_spectra.argtypes = L.pbagpu_ele_evolve.argtypes[:-1] + [C.POINTER(abi.Spectra), vp]
@@ -416,6 +418,17 @@ class Pipeline:
         else:
             _check(self.L.pbagpu_dyn_evolve(self.params, self.n_runs, self.bpr, self.inits, self.nt, _ptr(self.d_tg),
                                             _ptr(self.d_table), self.res, self.sp), "dyn_evolve")
+        self._set_populations()
+
+    def dynamics_ejecta(self, spread=False):
+        """Dynamics of kilonova ejecta (pbagpu_dyn_evolve_ejecta): the runs' blastwaves are ejecta
+        blastwaves in the reference's layer-major order (abi.ejecta_inits); spread is method_spread
+        != None (abi.run_params_ejecta).  Runs of one time grid, at most 64 blastwaves each."""
+        if self.ragged:
+            raise ValueError("ejecta runs of one batch share one time grid")
+        _check(self.L.pbagpu_dyn_evolve_ejecta(self.params, self.n_runs, self.bpr, self.inits, 1 if spread else 0,
+                                               self.nt, _ptr(self.d_tg), _ptr(self.d_table), self.res, self.sp),
+               "dyn_evolve_ejecta")
         self._set_populations()
 
     def _set_populations(self):

@@ -1,19 +1,19 @@
-"""The device's electron and observer stages on kilonova ejecta blastwaves, against the reference
-program's per-(shell, layer) light curves (tests/golden/kn_*.npz).
+"""Kilonova ejecta on the device, against the CPU oracle and the reference program's
+per-(shell, layer) light curves (tests/golden/kn_*.npz).
 
-The device's dynamics stage has no ejecta right-hand side (rhs_fs_dense) yet, so the dynamics
-tables come from the CPU oracle, which reproduces the reference's ejecta dynamics bit for bit
-(tests/test_kilonova_cpu.py).  From there the device computes the electron spectra, the
-emissivities and the observer integral of every blastwave through the C ABI
-(pbagpu_ele_evolve, pbagpu_eats_lightcurve / _pw), as for a jet's layers.
+* The electron and observer stages on the oracle's ejecta dynamics tables (bit-exact with the
+  reference's, tests/test_kilonova_cpu.py): pbagpu_ele_evolve, pbagpu_eats_lightcurve / _pw as
+  for a jet's layers.
+* The whole path on the device: pbagpu_dyn_evolve_ejecta (rhs_fs_dense for every blastwave of a
+  run, one wavefront per run), then the same two stages.
 """
 import numpy as np
 import pytest
 
-from conftest import load_golden, rel_err
+from conftest import dyn_excess, load_golden, rel_err
 from test_kilonova_cpu import kn_golden_run
 import pyoracle
-from pyblastafterglowmag_amd import abi
+from pyblastafterglowmag_amd import abi, inputs, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -46,3 +46,61 @@ def test_device_radiation_of_ejecta_matches_reference(name):
         worst = max(worst, e)
         assert e <= 1e-6, (name, ish, il, e)
     print(f"[gpu]   light curves vs the reference: {worst:.2e}", flush=True)
+
+
+# dynamics compared with the oracle's (the jet tests' set plus the ejecta's upstream columns)
+DYN_CHECK = ["R", "Gamma", "beta", "M2", "Eint2", "tt", "tcomov", "rho2", "GammaFsh", "thickness", "U_p",
+             "gamma_min", "gamma_max", "B", "accel_frac", "rho", "drhodr", "GammaRel"]
+# light curves: the jets' end-to-end bound (measured on one MI355X: 7e-15 .. 6.8e-11); dynamics: the
+# jets' conditioning rule at rtol 1e-4 (measured: at most 0.61 % of it)
+LC_TOL = 1e-6
+DYN_RTOL = 1e-4
+
+
+@pytest.mark.parametrize("name", ["kn_small", "kn_pw_small", "kn_spread_small", "kn_wind_small", "kn_rk4_small"])
+def test_device_ejecta_path_matches_oracle_and_reference(name):
+    """The whole ejecta path on the device: pbagpu_dyn_evolve_ejecta, then the electron and
+    observer stages.  Dynamics against the oracle's (conftest.dyn_excess: rtol plus 64 ulp of
+    Gamma carried into Gamma - 1; gamma_min to the 2e-3 resolution of the reference's bisection),
+    equal status bits and last usable steps; light curves against the reference's."""
+    from pyblastafterglowmag_amd import gpu
+    g = load_golden(name)
+    P, spread, inits, order, tgrid, tobs, nuobs, method = kn_golden_run(g)
+    table_o, res_o = pyoracle.dyn_evolve_ejecta(P, inits, tgrid, spread=spread)
+    print(f"[gpu] {name}: ejecta dynamics, electrons and observer stage of {len(order)} blastwaves "
+          f"({method}, spread={spread}, integrator={P.integrator})", flush=True)
+    run = gpu.Run(P, inits, tgrid, tobs, nuobs, method)
+    pl = gpu.Pipeline([run])
+    pl.dynamics_ejecta(spread)
+    flux, ex = gpu.radiate(pl, want_intermediates=True, precision=0)
+    assert not np.asarray(ex["ele_status"]).any() and not np.asarray(ex["eats_status"]).any()
+    worst_dyn = 0.0
+    for k in range(len(order)):
+        st, iend = ex["results"][k][0], ex["results"][k][1]
+        assert st == res_o[k].status == 0 and iend == res_o[k].i_end_r, (name, k, st, iend)
+        gam = table_o[k, abi.PV["Gamma"]]
+        for n in DYN_CHECK:
+            e = dyn_excess(ex["table"][k, abi.PV[n]], table_o[k, abi.PV[n]], gam,
+                           rtol=2e-3 if n == "gamma_min" else DYN_RTOL)
+            worst_dyn = max(worst_dyn, e)
+            assert e <= 1.0, (name, k, n, e)
+    worst = 0.0
+    for k, (ish, il) in enumerate(order):
+        e = rel_err(flux[0, k], g["lc"][ish, il], 1e-30)
+        worst = max(worst, e)
+        assert e <= LC_TOL, (name, ish, il, e)
+    print(f"[gpu]   dynamics vs oracle: {worst_dyn:.3e} of the bound; light curves vs the reference: {worst:.3e}",
+          flush=True)
+
+
+def test_device_ejecta_refuses_runs_beyond_one_wavefront():
+    """A run of more than 64 ejecta blastwaves (C5's 960) is refused before any launch."""
+    from pyblastafterglowmag_amd import gpu
+    main, kn, struct = workloads.kn_small("kn_small")
+    d = inputs.ejecta_structure(**dict(struct, n_shells=13, n_layers=5))
+    inits, order = abi.ejecta_inits(d)
+    P = kn_golden_run(load_golden("kn_small"))[0]
+    run = gpu.Run(P, inits, abi.inputs.logspace_vec(3, 14, 20), np.array([1e5]), np.array([1e9]))
+    pl = gpu.Pipeline([run])
+    with pytest.raises(RuntimeError, match="64 blastwaves"):
+        pl.dynamics_ejecta(False)
