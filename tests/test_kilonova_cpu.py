@@ -14,7 +14,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from conftest import load_golden, rel_err
+from conftest import dyn_excess, load_golden, rel_err
 import pyoracle
 from pyblastafterglowmag_amd import abi, h5io, inputs, workloads
 
@@ -238,3 +238,36 @@ def test_jet_path_refuses_a_kilonova_parfile(tmp_path):
     inputs.write_kn_run(wd, main, dict(kn, run_bws="no"), struct)
     with pytest.raises(ValueError, match="no jet"):
         abi.run_params_from_parfile(wd + "parfile.par")
+
+
+def test_c5_split_by_layer_stays_within_the_light_curve_bound():
+    """C5's 960 ejecta blastwaves evolved as 30 runs of 32 (one layer's shells each, one shared
+    step per run), as a device launch of at most 64 blastwaves per run does, against the
+    reference's single shared step (kn_c5_lc): splitting moves the step sequence, so the
+    dynamics part at the ODE tolerance (beta: 6.8e-7) and the light curves of the 12 kept
+    blastwaves by at most 1.8e-12, far inside the 1e-6 light-curve bound."""
+    g = load_golden("kn_c5_lc")
+    P, spread, inits, order, tgrid, tobs, nuobs, method = kn_golden_run(g)
+    params = (abi.RunParams * 30)(*[P] * 30)
+    table, res = pyoracle.dyn_evolve_ejecta(params, inits, tgrid, spread=spread, n_runs=30)
+    assert all(r.status == 0 for r in res)
+    assert len({res[32 * i].n_steps for i in range(30)}) > 1   # every run takes its own steps
+    names = [str(n) for n in g["dyn_names"]]
+    keep = [(int(ish), int(il)) for ish in g["dyn_shells"] for il in g["dyn_layers"]]
+    ks = [order.index(sl) for sl in keep]
+    for (ish, il), k in zip(keep, ks):
+        i, j = list(g["dyn_shells"]).index(ish), list(g["dyn_layers"]).index(il)
+        gam = g["dyn"][i, j, names.index("Gamma")]
+        for n in ("R", "Gamma", "beta", "M2", "Eint2", "U_p", "B"):
+            e = dyn_excess(table[k, abi.PV[n], ::10], g["dyn"][i, j, names.index(n)], gam, rtol=1e-5)
+            assert e <= 1.0, (ish, il, n, e)
+    pops = (abi.Population * len(ks))(*[abi.Population(k, k // 32, 0, 0) for k in ks])
+    jall, _, st = pyoracle.ele_evolve(params, pops, table, n_runs=30)
+    assert not st.any()
+    lays = (abi.EatsLayer * len(ks))(*[abi.EatsLayer(inits[k].theta_c_l, inits[k].theta_c_h, k, k // 32, q, -1,
+                                                     res[k].i_end_r) for q, k in enumerate(ks)])
+    lc, st2 = pyoracle.eats_lightcurve(params, lays, table, jall, tobs, nuobs, n_runs=30)
+    assert not st2.any()
+    for q, (ish, il) in enumerate(keep):
+        i, j = list(g["dyn_shells"]).index(ish), list(g["dyn_layers"]).index(il)
+        assert rel_err(lc[q], g["lc"][i, j], 1e-30) <= 1e-10, (ish, il)

@@ -104,3 +104,31 @@ def test_device_ejecta_refuses_runs_beyond_one_wavefront():
     pl = gpu.Pipeline([run])
     with pytest.raises(RuntimeError, match="64 blastwaves"):
         pl.dynamics_ejecta(False)
+
+
+def test_device_c5_ejecta_as_one_run_per_layer():
+    """C5's 960 ejecta blastwaves (kn_c5_lc: 1000 steps, reduced microphysics grids, 6 x 8 pairs) on
+    the device as a batch of 30 runs of 32 blastwaves, each run one layer's velocity shells:
+    dynamics, electrons and observer stage.  The light curves of the 12 blastwaves the fixture
+    keeps against the reference's single-step run, 1e-6; the split alone moves them by at most
+    1.8e-12 (tests/test_kilonova_cpu.py)."""
+    from pyblastafterglowmag_amd import gpu
+    g = load_golden("kn_c5_lc")
+    P, spread, inits, order, tgrid, tobs, nuobs, method = kn_golden_run(g)
+    assert len(inits) == 960
+    print("[gpu] kn_c5_lc: 960 ejecta blastwaves as 30 runs of 32", flush=True)
+    runs = [gpu.Run(P, (abi.LayerInit * 32)(*inits[32 * i:32 * (i + 1)]), tgrid, tobs, nuobs, method)
+            for i in range(30)]
+    pl = gpu.Pipeline(runs)
+    pl.dynamics_ejecta(spread)
+    flux, ex = gpu.radiate(pl, precision=0)
+    assert not np.asarray(ex["dyn_status"]).any() and not np.asarray(ex["ele_status"]).any()
+    assert not np.asarray(ex["eats_status"]).any()
+    worst = 0.0
+    for i, ish in enumerate(g["dyn_shells"]):
+        for j, il in enumerate(g["dyn_layers"]):
+            r, b = divmod(order.index((int(ish), int(il))), 32)
+            e = rel_err(flux[r, b], g["lc"][i, j], 1e-30)
+            worst = max(worst, e)
+            assert e <= LC_TOL, (int(ish), int(il), e)
+    print(f"[gpu]   light curves of 12 blastwaves vs the reference: {worst:.3e}", flush=True)
