@@ -95,6 +95,22 @@ long h5io_dataset_len(const char *path, const char *dset) {
     return (long)n;
 }
 
+/* Rank of dataset `dset` and its first `maxdims` extents (dims), or < 0 on error. */
+int h5io_dataset_dims(const char *path, const char *dset, long *dims, int maxdims) {
+    quiet();
+    hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    hid_t ds = H5Dopen2(f, dset, H5P_DEFAULT);
+    if (ds < 0) { H5Fclose(f); return -2; }
+    hid_t sp = H5Dget_space(ds);
+    hsize_t d[8];
+    int rank = H5Sget_simple_extent_dims(sp, d, NULL);
+    if (rank > 8) rank = -3;
+    for (int i = 0; i < rank && i < maxdims; i++) dims[i] = (long)d[i];
+    H5Sclose(sp); H5Dclose(ds); H5Fclose(f);
+    return rank;
+}
+
 int h5io_read_dataset(const char *path, const char *dset, double *out, long n) {
     quiet();
     hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);

@@ -19,6 +19,7 @@ def _load():
         _lib.h5io_dataset_len.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
         _lib.h5io_dataset_len.restype = ctypes.c_long
         _lib.h5io_read_dataset.argtypes = [ctypes.c_char_p, ctypes.c_char_p, dp, ctypes.c_long]
+        _lib.h5io_dataset_dims.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long), ctypes.c_int]
         _lib.h5io_read_attr.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, dp]
         _lib.h5io_write_lc.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_long,
                                        dp, dp, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), dp]
@@ -111,6 +112,15 @@ def write_id(path, r, mom, theta, ctheta, ek, mass, theta_core, theta_wing, one_
                                float(theta_core), float(theta_wing))
     if rc != 0:
         raise IOError(f"h5io_write_id({path}) failed rc={rc}")
+
+
+def shape(path, dset):
+    """Extents of a dataset (a tuple, as numpy gives it)."""
+    dims = (ctypes.c_long * 8)()
+    rank = _load().h5io_dataset_dims(path.encode(), dset.encode(), dims, 8)
+    if rank < 0:
+        raise KeyError(f"{dset} not found in {path} (rc={rank})")
+    return tuple(int(dims[i]) for i in range(rank))
 
 
 def has(path, dset):

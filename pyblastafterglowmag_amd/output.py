@@ -7,7 +7,8 @@
   ``times_freqs/freqs/times_gams/gams`` grids; the GPU path does not keep the separate
   synchrotron/SSC parts, intensities or cooling rates, so those datasets are not written.
 
-Group names are ``"shell=0 layer=<i>"``; file attributes are nshells, nlayers (and ntimes,
+Group names are ``"shell=0 layer=<i>"`` for a jet's layers and ``"shell=<i> layer=<j>"`` for kilonova
+ejecta (``write_lightcurves_ejecta``, ``write_dynamics_ejecta``); file attributes are nshells, nlayers (and ntimes,
 ncells for dynamics/spectra; ncells is the piece-wise cell count n^2 for either EATS method, as
 EjectaID2 sets it, ejecta_id.h:180-181) plus every numeric parameter of the main and GRB parfile
 sections.
@@ -41,22 +42,38 @@ def file_attrs(mp_, gp, extra):
     return out
 
 
-def layer_attrs(run, il, dyn=False):
-    """Per-layer attributes (BlastWave Pars: E0, M0, Gamma0, ..., blastwave.h:73-241)."""
-    li = run.inits[il]
-    P = run.params
+def _rho_at(P, R):
+    """Ambient density at R (RhoISM::evaluateRhoDrhoDrDefault, src/blastwave/blastwave_components.h:846-893)."""
+    mppme = M_PROTON + 9.1094e-28
+    if P.n_ism > 0:
+        return P.n_ism * mppme
+    if R < P.ism_r_ej:
+        return P.ism_A0 * P.ism_r_ej ** (-P.ism_s) * mppme
+    if R < P.ism_r_ism:
+        return P.ism_A0 * R ** (-P.ism_s) * mppme
+    return P.ism_A0 * P.ism_r_ism ** (-P.ism_s) * mppme
+
+
+def bw_attrs(P, li, tb0, il, dyn=False):
+    """Attributes of one blastwave's group (BlastWave Pars: E0, M0, Gamma0, ..., set in
+    blastwave.h:73-241; Rd from the density at the start radius); il is its layer."""
     g0 = math.sqrt(li.mom0 * li.mom0 + 1.0)
     beta0 = li.mom0 / g0
-    tb0 = float(run.tgrid[0])
     M0 = li.M0 if li.M0 > 0 else li.E0 / (g0 * C_LIGHT * C_LIGHT)
-    rho = P.n_ism * (M_PROTON + 9.1094e-28)
+    r0 = beta0 * C_LIGHT * tb0
+    rho = _rho_at(P, r0)
     rd = (3. / (4. * math.pi) * 1. / (C_LIGHT * C_LIGHT * M_PROTON) * li.E0 / (rho / M_PROTON * g0 * g0)) ** (1. / 3.)
-    out = {"E0": li.E0, "M0": M0, "Gamma0": g0, "beta0": beta0, "mom0": li.mom0, "R0": beta0 * C_LIGHT * tb0,
+    out = {"E0": li.E0, "M0": M0, "Gamma0": g0, "beta0": beta0, "mom0": li.mom0, "R0": r0,
            "Eint0": 0.0, "Rd": rd, "t0": tb0, "theta_b0": li.theta_b0, "theta_a": 0.0}
     if dyn:
         out.update({"theta_c": li.theta_c, "theta_c_l": li.theta_c_l, "theta_c_h": li.theta_c_h,
                     "cil": float(abi.cells_in_layer(il))})
     return out
+
+
+def layer_attrs(run, il, dyn=False):
+    """Per-layer attributes of a jet's layer il."""
+    return bw_attrs(run.params, run.inits[il], float(run.tgrid[0]), il, dyn)
 
 
 def write_lightcurves(path, run, flux, mp_, gp):
@@ -87,6 +104,65 @@ def write_dynamics(path, run, table, fsrs, mp_, gp):
         extra = {"nshells": 1.0, "nlayers": float(nl), "ntimes": float(nt), "ncells": float(abi.total_cells(nl))}
         for k, v in file_attrs(mp_, gp, extra).items():
             w.attr("/", k, v)
+
+
+# BW::VARS of the ejecta's blastwave type (iFS_DENSE, src/blastwave/blastwave_pars.h:274-282): the
+# forward-shock set and the upstream's state
+DYN_DENSE = DYN_FS + ["GammaRho", "dGammaRhodr", "dGammaReldGamma", "PCBM", "dPCBMdrho", "MCBM", "CSCBM"]
+# their values in a standard ISM at rest (set_standard_ism, blastwave.h:1282-1295); the swept-up
+# mass of the upstream is unset (-1, RhoISM, blastwave_components.h:905) until the first RHS call
+_DENSE_UPSTREAM = {"GammaRho": 1.0, "dGammaRhodr": 0.0, "dGammaReldGamma": 1.0, "PCBM": 0.0, "dPCBMdrho": 0.0,
+                   "MCBM": 0.0, "CSCBM": 0.0}
+
+
+def _ejecta_groups(order):
+    """(k, group name) of every ejecta blastwave in the reference's file order (shells outer,
+    layers inner, src/ejecta/ejecta.h:179-181, 538-541); k indexes the blastwave in `order`."""
+    return [(k, f"shell={ish} layer={il}") for k, (ish, il) in sorted(enumerate(order), key=lambda x: x[1])]
+
+
+def write_lightcurves_ejecta(path, P, inits, order, tgrid, tobs, nuobs, flux, mp_, kp):
+    """Light curves of kilonova ejecta (Ejecta::saveDenseLightCurves, src/ejecta/ejecta.h:511-573):
+    one group per (shell, layer) blastwave; flux [n_bw, n_pairs] in the order of `order`."""
+    ns, nl = 1 + max(o[0] for o in order), 1 + max(o[1] for o in order)
+    tb0 = float(tgrid[0])
+    with h5io.Writer(path) as w:
+        for k, g in _ejecta_groups(order):
+            w.dataset(g, "fluxdens", flux[k])
+            for key, v in bw_attrs(P, inits[k], tb0, order[k][1]).items():
+                w.attr(g, key, v)
+        w.dataset("/", "times", tobs)
+        w.dataset("/", "freqs", nuobs)
+        for key, v in file_attrs(mp_, kp, {"nshells": float(ns), "nlayers": float(nl)}).items():
+            w.attr("/", key, v)
+
+
+def write_dynamics_ejecta(path, P, inits, order, tgrid, table, mp_, kp):
+    """Dynamics of kilonova ejecta (Ejecta::saveEjectaBWsDynamics, src/ejecta/ejecta.h:165-222):
+    one group per (shell, layer) blastwave with the ejecta type's datasets; table [n_bw, PBA_NV, nt]
+    in the order of `order`.  The upstream columns are written where the blastwave was evolved."""
+    ns, nl = 1 + max(o[0] for o in order), 1 + max(o[1] for o in order)
+    nt = table.shape[2]
+    tb0 = float(tgrid[0])
+    with h5io.Writer(path) as w:
+        for k, g in _ejecta_groups(order):
+            T = table[k]
+            live = T[abi.PV["R"]] > 0
+            for name in DYN_DENSE:
+                if name in abi.PV:
+                    data = T[abi.PV[name]]
+                elif name in _DENSE_UPSTREAM:
+                    data = np.where(live, _DENSE_UPSTREAM[name], 0.0)
+                    if name == "MCBM" and live[0]:
+                        data[0] = -1.0
+                else:
+                    data = np.zeros(nt)
+                w.dataset(g, name, data)
+            for key, v in bw_attrs(P, inits[k], tb0, order[k][1], dyn=True).items():
+                w.attr(g, key, v)
+        extra = {"nshells": float(ns), "nlayers": float(nl), "ntimes": float(nt), "ncells": float(abi.total_cells(nl))}
+        for key, v in file_attrs(mp_, kp, extra).items():
+            w.attr("/", key, v)
 
 
 def write_spectra(path, run, pops, jall, fall, ele_grid, nu_grid, mp_, gp, spectra=None, aall=None):

@@ -1,7 +1,7 @@
 """Drop-in driver: ``python -m pyblastafterglowmag_amd.pba <workdir> <parfile> [loglevel] [--precision N]``.
 
 Mirrors the reference CLI ``pba.out <workdir> <parfile> <loglevel>`` (src/main.cpp:208-384) for the
-GRB afterglow part of a parfile: it reads the same parfile and initial-data file, runs the three
+GRB afterglow part of a parfile, or for its kilonova ejecta (kN section, without a jet: ``run_ejecta``): it reads the same parfile and initial-data file, runs the three
 stages on the GPU (dynamics, electrons + emissivity, EATS light curves) and writes the files the
 reference writes for ``do_lc`` / ``save_dynamics`` / ``save_spec`` / ``do_skymap`` (``fname_light_curve``,
 ``fname_dyn``, ``fname_spectrum``, ``raw_skymap_<i>.h5``) in the same layout (see ``output``).  Errors end the process
@@ -36,8 +36,17 @@ def main(argv=None):
     a = ap.parse_args(argv)
     verbose = a.loglevel == "info"
 
-    from . import abi, gpu, output
+    from . import abi, gpu, inputs, output
     workdir = a.workdir if a.workdir.endswith("/") else a.workdir + "/"
+    kp, ko = inputs.read_parfile(os.path.join(workdir, a.parfile), inputs.SEC_KN)
+    if ko.get("run_bws", "no") == "yes":
+        _, go_jet = inputs.read_parfile(os.path.join(workdir, a.parfile), inputs.SEC_GRB)
+        if go_jet.get("run_bws", "no") == "yes":
+            # the reference evolves ejecta behind a jet through prepareDensProfileFromJet, which
+            # exits as not implemented (src/blastwave/blastwave.h:1296-1300)
+            print("[pba] a jet and kilonova ejecta in one run are not supported", file=sys.stderr)
+            return 1
+        return run_ejecta(workdir, a.parfile, kp, ko, a.precision, verbose)
     try:
         P, mp_, mo, gp, go = abi.run_params_from_parfile(os.path.join(workdir, a.parfile))
     except (NotImplementedError, KeyError, ValueError) as e:
@@ -112,6 +121,81 @@ def main(argv=None):
                              output.ele_grid(sh), output.freq_grid(sh), mp_, gp, spectra=spectra, aall=aall)
         if verbose:
             print(f"[pba] spectra -> {path}", flush=True)
+    return 0
+
+
+# datasets of the ejecta's 2-D initial-data file ([nshells][nlayers]; EjectaID2::_load_id_file fills a
+# missing one with zeros, src/ejecta/ejecta_id.h:235-245)
+_ID_FIELDS = ("r", "mom", "theta", "ctheta", "ek", "mass")
+
+
+def ejecta_runs(P, inits, order, tgrid, tobs, nuobs, method, ebl_table=None):
+    """The ejecta's blastwaves as device runs: one run when they fit one wavefront (64), else one
+    run per layer (its velocity shells).  The reference steps all of them together; one step per
+    layer moves the step sequence, by 1.8e-12 in C5's light curves."""
+    from . import abi, gpu
+    n = len(inits)
+    ns = 1 + max(o[0] for o in order)
+    per = n if n <= 64 else ns
+    if per > 64:
+        raise NotImplementedError(f"{ns} velocity shells per layer: a run of more than 64 ejecta blastwaves")
+    return [gpu.Run(P, (abi.LayerInit * per)(*inits[i:i + per]), tgrid, tobs, nuobs, method, ebl_table=ebl_table)
+            for i in range(0, n, per)]
+
+
+def run_ejecta(workdir, parfile, kp, ko, precision=0, verbose=False):
+    """The kilonova ejecta of a parfile (kN section, no jet): the reference's ejecta branch of
+    main.cpp (setPars and processEvolved, src/main.cpp:338-347, 378) on the device; writes fname_light_curve (do_lc) and
+    fname_dyn (save_dynamics) as Ejecta::saveDenseLightCurves / saveEjectaBWsDynamics do."""
+    from . import abi, ebl, gpu, h5io, inputs, output
+    mp_, mo = inputs.read_parfile(os.path.join(workdir, parfile), inputs.SEC_MAIN)
+    try:
+        P, spread = abi.run_params_ejecta(mp_, mo, kp, ko)
+        for key in ("do_skymap", "save_spec"):
+            if ko.get(key, "no") == "yes":
+                raise NotImplementedError(f"{key} = yes for kilonova ejecta")
+        idf = os.path.join(workdir, ko["fname_ejecta_id"])
+        shape = h5io.shape(idf, "mass")
+        if len(shape) != 2:
+            raise ValueError(f"{idf}: use_1d_id = no expects [nshells][nlayers] datasets")
+        idd = {k: (h5io.read(idf, k).reshape(shape) if h5io.has(idf, k) else np.zeros(shape)) for k in _ID_FIELDS}
+        method = ko.get("method_eats", "adaptive")
+        inits, order = abi.ejecta_inits(idd, method)
+        tgrid = abi.tgrid(mp_)
+        tobs, nuobs = abi.obs_pairs(mo)
+        tab = ebl.resolve_table_path(ko.get("ebl_tbl_fpath", "none"), workdir)
+        runs = ejecta_runs(P, inits, order, tgrid, tobs, nuobs, method,
+                           None if tab is None else ebl.EblTable.load(tab))
+    except (NotImplementedError, KeyError, ValueError, FileNotFoundError) as e:
+        print(f"[pba] unsupported or invalid kilonova parfile: {e}", file=sys.stderr)
+        return 1
+    t0 = time.time()
+    pl = gpu.Pipeline(runs, precision=precision)
+    pl.dynamics_ejecta(spread)
+    dst = np.array([r.status for r in pl.res])
+    if dst.any():
+        print(f"[pba] ejecta evolution failed: status {dst.tolist()}", file=sys.stderr)
+        return 1
+    pl.electrons()
+    pl.observer()
+    _, est, lst = pl.status()
+    if est.any() or lst.any():
+        print(f"[pba] microphysics/EATS failed: electrons {est.tolist()} light curve {lst.tolist()}", file=sys.stderr)
+        return 1
+    if verbose:
+        print(f"[pba] {len(inits)} ejecta blastwaves in {len(runs)} run(s) evolved and integrated in "
+              f"{time.time() - t0:.2f}s", flush=True)
+    if ko.get("do_lc", "yes") == "yes":
+        path = workdir + ko.get("fname_light_curve", "lc.h5")
+        flux = pl.flux().reshape(len(inits), len(tobs))
+        output.write_lightcurves_ejecta(path, P, inits, order, tgrid, tobs, nuobs, flux, mp_, kp)
+        if verbose:
+            print(f"[pba] light curves -> {path}", flush=True)
+    if ko.get("save_dynamics", "no") == "yes":
+        path = workdir + ko.get("fname_dyn", "dyn.h5")
+        output.write_dynamics_ejecta(path, P, inits, order, tgrid, pl.d_table.cpu().numpy(), mp_, kp)
+        if verbose:
+            print(f"[pba] dynamics -> {path}", flush=True)
     return 0
 
 

@@ -174,7 +174,8 @@ KN_CASES = {"kn_small", "kn_pw_small", "kn_spread_small", "kn_wind_small", "kn_r
 KN_DYN_ONLY = {"kn_c5_dyn"}
 KN_DYN_KEEP = {"kn_c5_dyn": ((0, 10, 21, 31), (0, 14, 29)), "kn_c5_lc": ((0, 10, 21, 31), (0, 14, 29))}
 KN_DYN_EVERY = {"kn_c5_dyn": 10, "kn_c5_lc": 10}
-KN_DYN_VARS = DYN_VARS + ["GammaRel", "rho", "drhodr", "ctheta", "Erad2", "Esh2", "Ead2"]
+KN_DYN_VARS = DYN_VARS + ["GammaRel", "rho", "drhodr", "ctheta", "Erad2", "Esh2", "Ead2", "GammaRho", "dGammaRhodr",
+                           "dGammaReldGamma", "PCBM", "dPCBMdrho", "MCBM", "CSCBM"]
 KN_DYN_VARS_ONLY = ["R", "Gamma", "beta", "theta", "M2", "Eint2", "tt", "tcomov", "rho2", "GammaFsh", "thickness",
                     "U_p", "GammaRel", "rho", "Esh2", "Ead2"]
 
@@ -217,6 +218,11 @@ def run_reference_kn(name, wd):
     out["dyn_names"] = np.array(names)
     out["dyn_layer_members"] = np.array(h5io.members(dyn, "shell=0 layer=0"))
     out["dyn_root_counts"] = np.array([h5io.attr(dyn, k) for k in ("nshells", "nlayers", "ntimes", "ncells")])
+    out["dyn_root_attrs"] = np.array(h5io.attr_names(dyn, "/"))
+    names_da = h5io.attr_names(dyn, "shell=0 layer=0")
+    out["dyn_layer_attr_names"] = np.array(names_da)
+    out["dyn_layer_attrs"] = np.array([[[h5io.attr(dyn, k, f"shell={ish} layer={il}") for k in names_da]
+                                        for il in layers] for ish in shells])
     if name in KN_DYN_ONLY:
         return out
     lc = os.path.join(wd, kn["fname_light_curve"])
@@ -225,6 +231,11 @@ def run_reference_kn(name, wd):
     out["lc"] = np.stack([np.stack([h5io.read(lc, f"shell={ish} layer={il}/fluxdens") for il in layers])
                           for ish in shells])
     out["lc_root_members"] = np.array(h5io.members(lc, "/"))
+    out["lc_root_attrs"] = np.array(h5io.attr_names(lc, "/"))
+    names_la = h5io.attr_names(lc, "shell=0 layer=0")
+    out["lc_layer_attr_names"] = np.array(names_la)
+    out["lc_layer_attrs"] = np.array([[[h5io.attr(lc, k, f"shell={ish} layer={il}") for k in names_la]
+                                       for il in layers] for ish in shells])
     return out
 
 

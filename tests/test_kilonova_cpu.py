@@ -9,6 +9,7 @@ the light curve of each.  The host-side pieces (initial-data writer, the state-v
 option checks) are tested against what the reference's loader reads.
 """
 import os
+import types
 import tempfile
 
 import numpy as np
@@ -16,7 +17,7 @@ import pytest
 
 from conftest import dyn_excess, load_golden, rel_err
 import pyoracle
-from pyblastafterglowmag_amd import abi, h5io, inputs, workloads
+from pyblastafterglowmag_amd import abi, h5io, inputs, output, workloads
 
 
 def kn_golden_run(g):
@@ -67,7 +68,9 @@ def test_oracle_ejecta_matches_reference(name):
     names = [str(n) for n in g["dyn_names"]]
     for k, (ish, il) in enumerate(order):
         for iv, n in enumerate(names):
-            np.testing.assert_array_equal(table[k, abi.PV[n]], g["dyn"][ish, il, iv], err_msg=f"{name} {ish} {il} {n}")
+            if n in abi.PV:   # the upstream's constant columns are the writer's (test below)
+                np.testing.assert_array_equal(table[k, abi.PV[n]], g["dyn"][ish, il, iv],
+                                              err_msg=f"{name} {ish} {il} {n}")
     pops = (abi.Population * len(order))(*[abi.Population(k, 0, 0, 0) for k in range(len(order))])
     jall, _, st, aall = pyoracle.ele_evolve(P, pops, table, want_a=True)
     assert not st.any()
@@ -271,3 +274,139 @@ def test_c5_split_by_layer_stays_within_the_light_curve_bound():
     for q, (ish, il) in enumerate(keep):
         i, j = list(g["dyn_shells"]).index(ish), list(g["dyn_layers"]).index(il)
         assert rel_err(lc[q], g["lc"][i, j], 1e-30) <= 1e-10, (ish, il)
+
+
+def _sections(g, tmp_path):
+    p = tmp_path / "parfile.par"
+    p.write_bytes(bytes(g["parfile"]))
+    mp_, mo = inputs.read_parfile(str(p), inputs.SEC_MAIN)
+    kp, ko = inputs.read_parfile(str(p), inputs.SEC_KN)
+    return mp_, mo, kp, ko
+
+
+@pytest.mark.parametrize("name", ["kn_small", "kn_pw_small"])
+def test_ejecta_output_files_follow_the_reference(name, tmp_path):
+    """lc_kn.h5 and dyn_kn.h5 as the reference writes them for ejecta: one group per (shell, layer)
+    blastwave, the ejecta type's datasets (the upstream's columns included), every attribute
+    name and value, and the files' attributes; checked against the reference's files of the same
+    run (the light curves written are the reference's own; the dynamics a synthetic table)."""
+    g = load_golden(name)
+    P, spread, inits, order, tgrid, tobs, nuobs, method = kn_golden_run(g)
+    mp_, mo, kp, ko = _sections(g, tmp_path)
+    path = str(tmp_path / "lc_kn.h5")
+    flux = np.stack([g["lc"][ish, il] for ish, il in order])
+    output.write_lightcurves_ejecta(path, P, inits, order, tgrid, tobs, nuobs, flux, mp_, kp)
+    assert sorted(h5io.members(path, "/")) == sorted(str(x) for x in g["lc_root_members"])
+    assert sorted(h5io.attr_names(path, "/")) == sorted(str(x) for x in g["lc_root_attrs"])
+    names = [str(x) for x in g["lc_layer_attr_names"]]
+    for ish, il in order:
+        grp = f"shell={ish} layer={il}"
+        assert sorted(h5io.attr_names(path, grp)) == sorted(names)
+        for key, ref in zip(names, g["lc_layer_attrs"][ish, il]):
+            assert h5io.attr(path, key, grp) == pytest.approx(ref, rel=1e-12, abs=0), (grp, key)
+        np.testing.assert_array_equal(h5io.read(path, grp + "/fluxdens"), g["lc"][ish, il])
+    np.testing.assert_array_equal(h5io.read(path, "times"), g["times"])
+    assert h5io.attr(path, "nshells") == 4 and h5io.attr(path, "n_ism") == P.n_ism
+
+    path = str(tmp_path / "dyn_kn.h5")
+    nt = len(tgrid)
+    table = np.zeros((len(order), abi.PBA_NV, nt))
+    table[:, abi.PV["R"]] = 1e15
+    table[:, abi.PV["Gamma"]] = 2.0
+    output.write_dynamics_ejecta(path, P, inits, order, tgrid, table, mp_, kp)
+    assert sorted(h5io.attr_names(path, "/")) == sorted(str(x) for x in g["dyn_root_attrs"])
+    assert [h5io.attr(path, k) for k in ("nshells", "nlayers", "ntimes", "ncells")] == list(g["dyn_root_counts"])
+    names = [str(x) for x in g["dyn_layer_attr_names"]]
+    dyn_names = [str(n) for n in g["dyn_names"]]
+    for ish, il in order:
+        grp = f"shell={ish} layer={il}"
+        assert sorted(h5io.members(path, grp)) == sorted(str(x) for x in g["dyn_layer_members"])
+        for key, ref in zip(names, g["dyn_layer_attrs"][ish, il]):
+            assert h5io.attr(path, key, grp) == pytest.approx(ref, rel=1e-12, abs=0), (grp, key)
+        np.testing.assert_array_equal(h5io.read(path, grp + "/Gamma"), np.full(nt, 2.0))
+        for n in output._DENSE_UPSTREAM:
+            np.testing.assert_array_equal(h5io.read(path, grp + "/" + n), g["dyn"][ish, il, dyn_names.index(n)],
+                                          err_msg=f"{grp} {n}")
+
+
+class _StandInPipeline:
+    """The device pipeline's interface with recognisable results: blastwave k of the batch gets
+    Gamma = 2 + k and light curve 1000 k + pair index (the driver's placement is what is tested)."""
+    made = []
+
+    def __init__(self, runs, precision=0, **kw):
+        self.runs = runs
+        n = sum(r.n_layers for r in runs)
+        self.n, nt, self.npairs = n, len(runs[0].tgrid), len(runs[0].tobs)
+        self.res = [abi.BwResult() for _ in range(n)]
+        t = np.zeros((n, abi.PBA_NV, nt))
+        t[:, abi.PV["R"]] = 1e15
+        for k in range(n):
+            t[k, abi.PV["Gamma"]] = 2.0 + k
+        self._table = t
+        self.d_table = types.SimpleNamespace(cpu=lambda: types.SimpleNamespace(numpy=lambda: self._table))
+        self.spread = None
+        _StandInPipeline.made.append(self)
+
+    def dynamics_ejecta(self, spread):
+        self.spread = spread
+
+    def electrons(self, **kw):
+        pass
+
+    def observer(self, **kw):
+        pass
+
+    def status(self):
+        return np.zeros(self.n, int), np.zeros(self.n, int), np.zeros(self.n, int)
+
+    def flux(self):
+        k = np.arange(self.n)[:, None]
+        return (1000.0 * k + np.arange(self.npairs)[None, :]).reshape(len(self.runs), -1, self.npairs)
+
+
+@pytest.mark.parametrize("shells,layers,n_runs", [(4, 3, 1), (13, 5, 5)])
+def test_driver_places_every_ejecta_blastwave(tmp_path, monkeypatch, shells, layers, n_runs):
+    """pba on a kilonova parfile: one device run when the blastwaves fit one wavefront, else one
+    run per layer (its shells); every blastwave's light curve and dynamics land in its own
+    (shell, layer) group."""
+    from pyblastafterglowmag_amd import gpu, pba
+    _StandInPipeline.made.clear()
+    monkeypatch.setattr(gpu, "Pipeline", _StandInPipeline)
+    main, kn, struct = workloads.kn_small("kn_small")
+    struct = dict(struct, n_shells=shells, n_layers=layers)
+    wd = str(tmp_path) + "/"
+    inputs.write_kn_run(wd, main, dict(kn, save_dynamics="yes", method_spread="our"), struct)
+    assert pba.main([wd, "parfile.par", "err"]) == 0
+    pl = _StandInPipeline.made[-1]
+    assert len(pl.runs) == n_runs and pl.spread is True
+    if n_runs > 1:   # one layer per run, its shells in order
+        for r, run in enumerate(pl.runs):
+            assert {run.inits[b].ilayer for b in range(run.n_layers)} == {r}
+    d = inputs.ejecta_structure(**struct)
+    _, order = abi.ejecta_inits(d)
+    for k, (ish, il) in enumerate(order):
+        grp = f"shell={ish} layer={il}"
+        np.testing.assert_array_equal(h5io.read(wd + kn["fname_light_curve"], grp + "/fluxdens"),
+                                      1000.0 * k + np.arange(pl.npairs))
+        assert h5io.read(wd + kn["fname_dyn"], grp + "/Gamma")[0] == 2.0 + k
+        assert h5io.attr(wd + kn["fname_dyn"], "E0", grp) == d["ek"][ish, il]
+    assert h5io.attr(wd + kn["fname_light_curve"], "nshells") == shells
+
+
+def test_driver_refuses_what_the_ejecta_path_does_not_run(tmp_path, monkeypatch, capsys):
+    from pyblastafterglowmag_amd import gpu, pba
+    monkeypatch.setattr(gpu, "Pipeline", _StandInPipeline)
+    main, kn, struct = workloads.kn_small("kn_small")
+    wd = str(tmp_path) + "/"
+    # more than 64 velocity shells per layer: no run fits one wavefront
+    inputs.write_kn_run(wd, main, kn, dict(struct, n_shells=65, n_layers=2))
+    assert pba.main([wd, "parfile.par", "err"]) == 1
+    assert "more than 64" in capsys.readouterr().err
+    # sky maps of ejecta
+    inputs.write_kn_run(wd, main, dict(kn, do_skymap="yes"), struct)
+    assert pba.main([wd, "parfile.par", "err"]) == 1
+    # a jet and ejecta in one parfile
+    inputs.write_parfile(wd, main, dict(run_bws="yes"), kn=dict(kn, run_bws="yes"))
+    assert pba.main([wd, "parfile.par", "err"]) == 1
+    assert "jet and kilonova ejecta" in capsys.readouterr().err

@@ -132,3 +132,26 @@ def test_device_c5_ejecta_as_one_run_per_layer():
             worst = max(worst, e)
             assert e <= LC_TOL, (int(ish), int(il), e)
     print(f"[gpu]   light curves of 12 blastwaves vs the reference: {worst:.3e}", flush=True)
+
+
+def test_driver_on_a_kilonova_parfile(tmp_path):
+    """pba <workdir> <parfile> on the kn_small parfile (ejecta only): the files it writes against
+    the reference's for the same parfile: light curves 1e-6, dynamics by the conditioning rule."""
+    from pyblastafterglowmag_amd import h5io, pba
+    g = load_golden("kn_small")
+    P, spread, inits, order, tgrid, tobs, nuobs, method = kn_golden_run(g)
+    main, kn, struct = workloads.kn_small("kn_small")
+    wd = str(tmp_path) + "/"
+    inputs.write_kn_run(wd, main, dict(kn, save_dynamics="yes", save_spec="no"), struct)
+    assert open(wd + "parfile.par").read() == bytes(g["parfile"]).decode()
+    print("[gpu] driver on the kn_small parfile", flush=True)
+    assert pba.main([wd, "parfile.par", "err"]) == 0
+    names = [str(n) for n in g["dyn_names"]]
+    for ish, il in order:
+        grp = f"shell={ish} layer={il}"
+        e = rel_err(h5io.read(wd + kn["fname_light_curve"], grp + "/fluxdens"), g["lc"][ish, il], 1e-30)
+        assert e <= LC_TOL, (grp, e)
+        gam = g["dyn"][ish, il, names.index("Gamma")]
+        for n in ("R", "Gamma", "beta", "M2", "Eint2", "U_p", "B", "GammaRel", "GammaRho", "MCBM"):
+            mine = h5io.read(wd + kn["fname_dyn"], grp + "/" + n)
+            assert dyn_excess(mine, g["dyn"][ish, il, names.index(n)], gam, rtol=DYN_RTOL) <= 1.0, (grp, n)
